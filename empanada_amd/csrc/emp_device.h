@@ -1,0 +1,84 @@
+// Device-side constants and integer arithmetic shared by the kernels.
+//
+// Keys are (h1, h2): h1 = H(R=5, P1 = 2^29-3), h2 = H(R=7, P2 = 2^29-33), the two 32-bit halves of
+// the reference's 64-bit key (assembly.cuh:29-34,50-52). Every reduction below is exact: with
+// P = 2^29 - c, 2^29 = c (mod P), so folding the bits above 29 as hi*c + lo preserves the residue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emp {
+
+constexpr uint32_t P1 = 536870909u;  // 2^29 - 3
+constexpr uint32_t P2 = 536870879u;  // 2^29 - 33
+constexpr uint32_t C1 = 3u;
+constexpr uint32_t C2 = 33u;
+constexpr uint32_t R1 = 5u;
+constexpr uint32_t R2 = 7u;
+constexpr uint32_t INV1 = 107374182u;  // 5^(P1-2) mod P1 (overlap.cu:121)
+constexpr uint32_t INV2 = 76695840u;   // 7^(P2-2) mod P2 (overlap.cu:122)
+constexpr uint32_t MASK29 = (1u << 29) - 1;
+constexpr int MAX_RD_LEN = 255;
+
+// Column store: reads are grouped in chunks of 2^CHUNK_SHIFT; a chunk holds, per column j,
+// CHUNK_WORDS 32-bit words of 16 two-bit codes (read k at bits 2*(k%16)+1:2*(k%16)).
+constexpr int CHUNK_SHIFT = 20;
+constexpr uint32_t CHUNK_READS = 1u << CHUNK_SHIFT;
+constexpr uint32_t CHUNK_WORDS = CHUNK_READS / 16;
+
+// One (fingerprint, oriented read id) entry, 12 bytes.
+struct Entry {
+    uint32_t h1, h2, rid;
+};
+
+// x mod P1 for x < 2^32.
+__host__ __device__ __forceinline__ uint32_t red1(uint32_t x) {
+    uint32_t y = (x >> 29) * C1 + (x & MASK29);  // < 2^29 + 21 < 2*P1
+    return y >= P1 ? y - P1 : y;
+}
+
+// x mod P2 for x < 2^32.
+__host__ __device__ __forceinline__ uint32_t red2(uint32_t x) {
+    uint32_t y = (x >> 29) * C2 + (x & MASK29);  // < 2^29 + 231 < 2*P2
+    return y >= P2 ? y - P2 : y;
+}
+
+// x mod (2^29 - c) for any 64-bit x, c <= 33.
+__host__ __device__ __forceinline__ uint32_t red64(uint64_t x, uint32_t c, uint32_t p) {
+    uint64_t y = (x >> 29) * c + (x & MASK29);  // < 2^41 + 2^29
+    y = (y >> 29) * c + (y & MASK29);           // < 2^29 + 2^18
+    uint32_t z = (uint32_t)y;
+    return z >= p ? z - p : z;
+}
+
+// ENCODING (assembly.cuh:34) of a 2-bit code (c>>1)&3: A0 C1 T2 G3 -> 1..4. The complement
+// (preprocess.cu:18: A<->T, C<->G) of a code is code ^ 2.
+__host__ __device__ __forceinline__ uint32_t enc_of_code(uint32_t code) { return code + 1; }
+
+// Horner step h*R + e (mod P); h < P, e <= 4 keeps the sum below 2^32.
+__host__ __device__ __forceinline__ uint32_t step1(uint32_t h, uint32_t e) { return red1(h * R1 + e); }
+__host__ __device__ __forceinline__ uint32_t step2(uint32_t h, uint32_t e) { return red2(h * R2 + e); }
+
+// NextSfx (overlap.cu:57-75): drop the first base of a length-l suffix. m1 = R1^(l-1) mod P1,
+// m2 = R2^(l-1) mod P2. e*m < 4*2^29, so the 32-bit product is exact as in the reference.
+__host__ __device__ __forceinline__ void next_sfx(uint32_t &h1, uint32_t &h2, uint32_t e, uint32_t m1, uint32_t m2) {
+    uint32_t t1 = red1(e * m1), t2 = red2(e * m2);
+    uint32_t v1 = h1 + P1 - t1, v2 = h2 + P2 - t2;  // in (0, 2P)
+    h1 = v1 >= P1 ? v1 - P1 : v1;
+    h2 = v2 >= P2 ? v2 - P2 : v2;
+}
+
+// PrevPfx (overlap.cu:77-91): ((INV * ((h - e) mod 2^64)) mod 2^64) mod P, keeping the
+// reference's uint64_t wrap when h < e.
+__host__ __device__ __forceinline__ void prev_pfx(uint32_t &h1, uint32_t &h2, uint32_t e) {
+    uint64_t x1 = (uint64_t)INV1 * ((uint64_t)h1 - (uint64_t)e);
+    uint64_t x2 = (uint64_t)INV2 * ((uint64_t)h2 - (uint64_t)e);
+    h1 = red64(x1, C1, P1);
+    h2 = red64(x2, C2, P2);
+}
+
+// Sort rank of a read id inside a group of equal keys: (rid & 1, rid) as one 32-bit value.
+__host__ __device__ __forceinline__ uint32_t rid_rank(uint32_t rid) { return (rid >> 1) | (rid << 31); }
+
+}  // namespace emp

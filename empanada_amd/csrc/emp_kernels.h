@@ -1,0 +1,95 @@
+// Kernel argument blocks and launchers (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emp_device.h"
+
+namespace emp {
+
+// Bucket capacity (entries per bucket per list) and matched-record capacity of the LDS join.
+constexpr int BUCKET_CAP = 2048;
+constexpr int MATCH_CAP = 512;
+// Target mean bucket load: the partition fan-out is chosen so that lists average this many
+// entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
+constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
+
+// Round description for regeneration: keys become those of length l-1 for round t.
+struct RoundArgs {
+    int32_t active;    // 0: no regeneration (entries are only counted)
+    uint32_t col_t;    // column t
+    uint32_t col_back; // column L-1-t
+    uint32_t m1, m2;   // R1^(l-1) mod P1, R2^(l-1) mod P2, l = L - t
+};
+
+struct MapArgs {
+    const uint8_t *reads;    // n x L row-major, 4-byte aligned
+    uint32_t n;              // reads in this batch
+    int32_t L, Lpad;         // Lpad: LDS row stride (multiple of 4, odd number of dwords)
+    uint64_t k0;             // global index of the first read (multiple of 16)
+    uint32_t *const *chunks; // device table of column chunks
+    Entry *entries;          // linear entries, index 2k and 2k+1
+    uint32_t *err;           // set to 1 when a byte is not ACGTacgt
+};
+
+struct ScatterArgs {
+    const Entry *in;
+    uint64_t n;
+    Entry *out;
+    uint32_t *out_cnt;
+    uint32_t cap;
+    int32_t shift;
+};
+
+struct DedupArgs {
+    const Entry *in;
+    const uint32_t *cnt;
+    const uint32_t *const *chunks;
+    RoundArgs r0;
+    Entry *s_out, *p_out;
+    uint32_t *s_cnt, *p_cnt;
+    uint32_t cap;
+    int32_t out_shift;
+    uint32_t *big, *n_big;
+    uint32_t big_cap;
+    unsigned long long *n_kept;
+};
+
+struct JoinArgs {
+    const Entry *s_in, *p_in;
+    const uint32_t *s_cnt, *p_cnt;
+    const uint32_t *const *chunks;
+    RoundArgs rn;  // regeneration for the next round
+    Entry *s_out, *p_out;
+    uint32_t *s_out_cnt, *p_out_cnt;
+    uint32_t cap;
+    int32_t out_shift;
+    uint32_t *pool_from, *pool_to;
+    unsigned long long *pool_cursor;
+    unsigned long long pool_cap;
+    unsigned long long *edge_off;
+    uint32_t *edge_cnt;
+    uint32_t *big, *n_big;
+    uint32_t big_cap;
+    unsigned long long *surv_s, *surv_p;
+};
+
+struct EdgeCopyArgs {
+    const uint32_t *cnt;
+    const unsigned long long *src_off;
+    const unsigned long long *dst_off;
+    uint32_t nb;
+    const uint32_t *pool_from, *pool_to;
+    uint32_t *out_from, *out_to;
+};
+
+hipError_t launch_map(const MapArgs &a, hipStream_t s);
+hipError_t launch_scatter(const ScatterArgs &a, hipStream_t s);
+hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s);
+hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s);
+hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s);
+hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,
+                       uint64_t *out, hipStream_t s);
+
+}  // namespace emp

@@ -1,0 +1,516 @@
+// HIP kernels of the fingerprint -> partition -> suffix/prefix join path, for gfx950 (wave64).
+//
+// Data in HBM:
+//   reads    row-major ASCII kept reads (input of the map kernel only)
+//   columns  2-bit codes per read position, chunked by 2^20 reads (emp_device.h)
+//   entries  12-byte (h1, h2, rid) records, first linear (map output), then bucketed:
+//            bucket b of a list holds the entries whose key has top bits b (h1 >> (29 - bits)),
+//            in a fixed-capacity slot array [bucket][CAP] plus a count per bucket. Order inside
+//            a bucket is arbitrary: every order the output depends on is re-established in LDS.
+//
+// Kernels:
+//   k_map      transposition + forward/reverse-complement Rabin-Karp (fingerprint.cu:75-153)
+//   k_scatter  linear entries -> buckets
+//   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222), then
+//              regenerate for round 0 and scatter into the suffix and prefix lists
+//   k_join     per bucket of round t: LDS hash join of suffixes against prefixes, edges of the
+//              matched key groups in (key, rid&1, rid) x (rid&1, rid) order, removal of used
+//              entries, regeneration of the survivors for round t+1 and scatter into its buckets
+//              (overlap.cu:258-290, 361-456, 557-710)
+//   k_edge_*   bucket-ordered compaction of the edges of a round
+#include "emp_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace emp {
+
+// ---------------------------------------------------------------------------------------------
+// Column gather: 2-bit code of read k at column `col`.
+__device__ __forceinline__ uint32_t col_code(const uint32_t *const *__restrict__ chunks, uint32_t col, uint32_t k) {
+    const uint32_t *ch = chunks[k >> CHUNK_SHIFT];
+    uint32_t w = ch[(size_t)col * CHUNK_WORDS + ((k >> 4) & (CHUNK_WORDS - 1))];
+    return (w >> (2 * (k & 15))) & 3u;
+}
+
+// Base code of a live entry for the round described by `r`. Suffix list: even ids read column
+// t, odd ids the complement of column L-1-t; prefix list: the other way round (overlap.cu:266-267,
+// 684-690: rid/2 picks the read, odd ids are complemented).
+__device__ __forceinline__ uint32_t entry_code(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, uint32_t rid,
+                                               bool sfx) {
+    uint32_t k = rid >> 1;
+    bool odd = rid & 1;
+    uint32_t col = (sfx != odd) ? r.col_t : r.col_back;
+    uint32_t c = col_code(chunks, col, k);
+    return odd ? (c ^ 2u) : c;
+}
+
+__device__ __forceinline__ void regen(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, Entry &e, bool sfx) {
+    uint32_t code = entry_code(chunks, r, e.rid, sfx);
+    uint32_t ev = enc_of_code(code);
+    if (sfx)
+        next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
+    else
+        prev_pfx(e.h1, e.h2, ev);
+}
+
+__device__ __forceinline__ void put_entry(Entry *__restrict__ out, uint32_t *__restrict__ cnt, uint32_t cap, int shift,
+                                          const Entry &e) {
+    uint32_t b = e.h1 >> shift;
+    uint32_t slot = atomicAdd(cnt + b, 1u);
+    if (slot < cap) out[(size_t)b * cap + slot] = e;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Map: one work-group of 256 threads per 256 reads.
+__global__ void __launch_bounds__(256) k_map(MapArgs a) {
+    extern __shared__ __align__(16) uint8_t tile[];
+    const uint32_t first = blockIdx.x * 256u;
+    const uint32_t nt = min(256u, a.n - first);
+    const int L = a.L, Lp = a.Lpad;
+    const uint8_t *src = a.reads + (size_t)first * L;
+    const uint32_t nbytes = nt * (uint32_t)L;
+    const uint32_t ndw = nbytes >> 2;
+    uint32_t bad = 0;
+    for (uint32_t d = threadIdx.x; d < ndw; d += 256) {
+        uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(src) + d);
+        uint32_t g = d * 4;
+        uint32_t r = g / (uint32_t)L, p = g - r * (uint32_t)L;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint8_t c = (uint8_t)(w >> (8 * i));
+            uint8_t lc = c | 0x20;
+            bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
+            tile[r * Lp + p] = c;
+            if (++p == (uint32_t)L) { p = 0; ++r; }
+        }
+    }
+    for (uint32_t g = ndw * 4 + threadIdx.x; g < nbytes; g += 256) {
+        uint8_t c = src[g];
+        uint8_t lc = c | 0x20;
+        bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
+        uint32_t r = g / (uint32_t)L;
+        tile[r * Lp + (g - r * (uint32_t)L)] = c;
+    }
+    if (bad) atomicOr(a.err, 1u);
+    __syncthreads();
+
+    const uint32_t t = threadIdx.x;
+    if (t < nt) {
+        const uint8_t *row = tile + t * Lp;
+        uint32_t f1 = 0, f2 = 0, r1 = 0, r2 = 0;
+        for (int p = 0; p < L; ++p) {
+            uint32_t code = (row[p] >> 1) & 3u;
+            uint32_t e = enc_of_code(code);
+            f1 = step1(f1, e);
+            f2 = step2(f2, e);
+            uint32_t ce = enc_of_code(((row[L - 1 - p] >> 1) & 3u) ^ 2u);
+            r1 = step1(r1, ce);
+            r2 = step2(r2, ce);
+        }
+        uint64_t k = a.k0 + first + t;
+        Entry ef = {f1, f2, (uint32_t)(2 * k)};
+        Entry er = {r1, r2, (uint32_t)(2 * k + 1)};
+        a.entries[2 * k] = ef;
+        a.entries[2 * k + 1] = er;
+    }
+
+    // Columns: word w of the tile covers reads 16w..16w+15; task id = w * L + j keeps a wave on
+    // consecutive columns of the same 16 rows (conflict-free LDS byte reads).
+    const uint32_t nw = (nt + 15) / 16;
+    const uint32_t ntask = nw * (uint32_t)L;
+    const uint64_t w0 = (a.k0 + first) >> 4;
+    for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
+        uint32_t w = task / (uint32_t)L, j = task - w * (uint32_t)L;
+        uint32_t word = 0;
+        uint32_t rmax = min(16u, nt - 16 * w);
+        for (uint32_t i = 0; i < rmax; ++i) word |= ((uint32_t)(tile[(16 * w + i) * Lp + j] >> 1) & 3u) << (2 * i);
+        uint64_t W = w0 + w;
+        uint32_t *ch = a.chunks[W >> (CHUNK_SHIFT - 4)];
+        ch[(size_t)j * CHUNK_WORDS + (W & (CHUNK_WORDS - 1))] = word;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
+    uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+    if (i >= a.n) return;
+    Entry e = a.in[i];
+    put_entry(a.out, a.out_cnt, a.cap, a.shift, e);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block helpers (256 threads = 4 waves of 64).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* >= 4 */, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    for (int i = 0; i < wv; ++i) base += sh[i];
+    total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return base + x - v;
+}
+
+// Bitonic sort of n2 (power of two) records in LDS by (key, rank), ascending; `pay` (may be
+// null) is permuted along.
+__device__ void bitonic_lds(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n2) {
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += 256) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t ka = key[i], kb = key[ixj];
+                    uint32_t ra = rank[i], rb = rank[ixj];
+                    bool a_gt = (ka > kb) || (ka == kb && ra > rb);
+                    bool up = (i & k) == 0;
+                    if (a_gt == up) {
+                        key[i] = kb; key[ixj] = ka;
+                        rank[i] = rb; rank[ixj] = ra;
+                        if (pay) { uint32_t t = pay[i]; pay[i] = pay[ixj]; pay[ixj] = t; }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dedup: per bucket keep the entry with the smallest id of each key (stable sort by key of an
+// id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306).
+template <int CAP>
+__global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
+    constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    __shared__ uint32_t h1s[CAP], h2s[CAP], rids[CAP];
+    __shared__ uint32_t head[CAP];
+    __shared__ uint16_t nxt[CAP];
+    __shared__ uint32_t s_kept;
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.cnt[b];
+    if (n > (uint32_t)CAP) {
+        if (threadIdx.x == 0) { uint32_t i = atomicAdd(a.n_big, 1u); if (i < a.big_cap) a.big[i] = b; }
+        return;
+    }
+    if (n == 0) return;
+    const Entry *in = a.in + (size_t)b * CAP;
+    for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
+    if (threadIdx.x == 0) s_kept = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        Entry e = in[i];
+        h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += 256) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
+    __syncthreads();
+    uint32_t kept = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        uint32_t h1 = h1s[i], h2 = h2s[i], rid = rids[i];
+        bool keep = true;
+        for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY && keep; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j])) {
+            if (h1s[j] == h1 && h2s[j] == h2 && rids[j] < rid) keep = false;
+        }
+        if (keep) {
+            ++kept;
+            if (a.r0.active) {
+                Entry s = {h1, h2, rid}, p = {h1, h2, rid};
+                regen(a.chunks, a.r0, s, true);
+                regen(a.chunks, a.r0, p, false);
+                put_entry(a.s_out, a.s_cnt, a.cap, a.out_shift, s);
+                put_entry(a.p_out, a.p_cnt, a.cap, a.out_shift, p);
+            }
+        }
+    }
+    if (kept) atomicAdd(&s_kept, kept);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_kept) atomicAdd(a.n_kept, (unsigned long long)s_kept);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Join of round t for one bucket.
+template <int CAP, int MCAP>
+__global__ void __launch_bounds__(256) k_join(JoinArgs a) {
+    constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    constexpr int PER = CAP / 256;
+    __shared__ uint32_t pk1[CAP], pk2[CAP], prid[CAP];
+    __shared__ uint32_t head[CAP];
+    __shared__ uint16_t nxt[CAP];
+    __shared__ uint8_t used[CAP];
+    __shared__ uint64_t msk[MCAP], mpk[MCAP];
+    __shared__ uint32_t msr[MCAP], mpr[MCAP];
+    __shared__ uint32_t msc[MCAP];
+    __shared__ uint32_t sh[8];
+    __shared__ uint32_t s_ms, s_mp, s_ovf;
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_surv_s, s_surv_p;
+
+    const uint32_t b = blockIdx.x;
+    const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+    if (ns > (uint32_t)CAP || np > (uint32_t)CAP) {
+        if (threadIdx.x == 0) {
+            uint32_t i = atomicAdd(a.n_big, 1u);
+            if (i < a.big_cap) a.big[i] = b;
+            a.edge_cnt[b] = 0;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) { s_ms = 0; s_mp = 0; s_ovf = 0; s_surv_s = 0; s_surv_p = 0; }
+    const Entry *sin = a.s_in + (size_t)b * CAP;
+    const Entry *pin = a.p_in + (size_t)b * CAP;
+    for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
+    for (uint32_t i = threadIdx.x; i < np; i += 256) {
+        Entry e = pin[i];
+        pk1[i] = e.h1; pk2[i] = e.h2; prid[i] = e.rid; used[i] = 0;
+    }
+    Entry se[PER];
+    uint32_t sc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        uint32_t i = threadIdx.x + 256 * q;
+        if (i < ns) se[q] = sin[i];
+        sc[q] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += 256) nxt[i] = (uint16_t)atomicExch(head + (pk2[i] & (CAP - 1)), i);
+    __syncthreads();
+
+    // Probe: count valid partners (different read) of every suffix; flag the prefixes used.
+    if (np) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            uint32_t i = threadIdx.x + 256 * q;
+            if (i < ns) {
+                const uint32_t h1 = se[q].h1, h2 = se[q].h2, rd = se[q].rid >> 1;
+                uint32_t c = 0;
+                for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j])) {
+                    if (pk2[j] == h2 && pk1[j] == h1 && (prid[j] >> 1) != rd) {
+                        ++c;
+                        used[j] = 1;
+                    }
+                }
+                sc[q] = c;
+                if (c) {
+                    uint32_t m = atomicAdd(&s_ms, 1u);
+                    if (m < (uint32_t)MCAP) {
+                        msk[m] = ((uint64_t)h1 << 32) | h2;
+                        msr[m] = rid_rank(se[q].rid);
+                        msc[m] = c;
+                    } else {
+                        s_ovf = 1;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += 256) {
+        if (used[i]) {
+            uint32_t m = atomicAdd(&s_mp, 1u);
+            if (m < (uint32_t)MCAP) {
+                mpk[m] = ((uint64_t)pk1[i] << 32) | pk2[i];
+                mpr[m] = rid_rank(prid[i]);
+            } else {
+                s_ovf = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (s_ovf) {
+        if (threadIdx.x == 0) {
+            uint32_t i = atomicAdd(a.n_big, 1u);
+            if (i < a.big_cap) a.big[i] = b;
+            a.edge_cnt[b] = 0;
+        }
+        return;
+    }
+    const uint32_t nms = s_ms, nmp = s_mp;
+
+    // Edges of the bucket: sort matched suffixes and prefixes by (key, rank), then emit the
+    // cross product of each key group skipping same-read pairs (overlap.cu:557-601).
+    uint32_t nedge = 0;
+    if (nms) {
+        uint32_t n2 = 1;
+        while (n2 < nms) n2 <<= 1;
+        for (uint32_t i = nms + threadIdx.x; i < n2; i += 256) { msk[i] = ~0ull; msr[i] = ~0u; msc[i] = 0; }
+        uint32_t m2 = 1;
+        while (m2 < nmp) m2 <<= 1;
+        for (uint32_t i = nmp + threadIdx.x; i < m2; i += 256) { mpk[i] = ~0ull; mpr[i] = ~0u; }
+        __syncthreads();
+        bitonic_lds(msk, msr, msc, n2);
+        bitonic_lds(mpk, mpr, nullptr, m2);
+        // exclusive scan of per-suffix edge counts (nms <= MCAP = 2 * 256)
+        static_assert(MCAP <= 512, "scan covers two records per thread");
+        uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
+        uint32_t c0 = i0 < nms ? msc[i0] : 0, c1 = i1 < nms ? msc[i1] : 0;
+        uint32_t tot;
+        uint32_t off = block_excl_scan(c0 + c1, sh, tot);
+        nedge = tot;
+        __syncthreads();
+        if (i0 < nms) msc[i0] = off;
+        if (i1 < nms) msc[i1] = off + c0;
+        if (threadIdx.x == 0) {
+            unsigned long long base = atomicAdd(a.pool_cursor, (unsigned long long)tot);
+            s_base = base;
+            a.edge_off[b] = base;
+            a.edge_cnt[b] = tot;
+        }
+        __syncthreads();
+        const unsigned long long base = s_base;
+        if (base + tot <= a.pool_cap) {
+            for (uint32_t i = threadIdx.x; i < nms; i += 256) {
+                const uint64_t k = msk[i];
+                const uint32_t srid = (msr[i] << 1) | (msr[i] >> 31);
+                // lower bound of k among the matched prefixes
+                uint32_t lo = 0, hi = nmp;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (mpk[mid] < k) lo = mid + 1; else hi = mid;
+                }
+                unsigned long long pos = base + msc[i];
+                for (uint32_t j = lo; j < nmp && mpk[j] == k; ++j) {
+                    const uint32_t prd = (mpr[j] << 1) | (mpr[j] >> 31);
+                    if ((prd >> 1) != (srid >> 1)) {
+                        a.pool_from[pos] = srid;
+                        a.pool_to[pos] = prd;
+                        ++pos;
+                    }
+                }
+            }
+        }
+    } else if (threadIdx.x == 0) {
+        a.edge_cnt[b] = 0;
+        a.edge_off[b] = 0;
+    }
+
+    // Survivors: suffixes with no edge, prefixes never a target (overlap.cu:424-429,694-710).
+    uint32_t ks = 0, kp = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        uint32_t i = threadIdx.x + 256 * q;
+        if (i < ns && sc[q] == 0) {
+            ++ks;
+            if (a.rn.active) {
+                Entry e = se[q];
+                regen(a.chunks, a.rn, e, true);
+                put_entry(a.s_out, a.s_out_cnt, a.cap, a.out_shift, e);
+            }
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < np; i += 256) {
+        if (!used[i]) {
+            ++kp;
+            if (a.rn.active) {
+                Entry e = {pk1[i], pk2[i], prid[i]};
+                regen(a.chunks, a.rn, e, false);
+                put_entry(a.p_out, a.p_out_cnt, a.cap, a.out_shift, e);
+            }
+        }
+    }
+    if (ks) atomicAdd(&s_surv_s, ks);
+    if (kp) atomicAdd(&s_surv_p, kp);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_surv_s) atomicAdd(a.surv_s, (unsigned long long)s_surv_s);
+        if (s_surv_p) atomicAdd(a.surv_p, (unsigned long long)s_surv_p);
+        (void)nedge;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Edge compaction: exclusive scan of per-bucket edge counts in bucket (= key) order, then copy.
+__global__ void __launch_bounds__(1024) k_edge_scan(const uint32_t *__restrict__ cnt, uint32_t nb,
+                                                    unsigned long long *__restrict__ dst_off) {
+    __shared__ unsigned long long part[1024];
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = min(nb, threadIdx.x * per), hi = min(nb, lo + per);
+    unsigned long long s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += cnt[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - s;
+    for (uint32_t i = lo; i < hi; ++i) {
+        dst_off[i] = run;
+        run += cnt[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
+    const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= a.nb) return;
+    const uint32_t c = a.cnt[b];
+    const unsigned long long src = a.src_off[b], dst = a.dst_off[b];
+    for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
+        a.out_from[dst + i] = a.pool_from[src + i];
+        a.out_to[dst + i] = a.pool_to[src + i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stage kernel: rolling transforms on a plain key array.
+__global__ void __launch_bounds__(256) k_roll(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ bases,
+                                              uint64_t n, int kind, uint32_t m1, uint32_t m2, uint64_t *__restrict__ out) {
+    uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    uint32_t h1 = (uint32_t)(k >> 32), h2 = (uint32_t)k;
+    uint32_t e = enc_of_code((bases[i] >> 1) & 3u);
+    if (kind == 0)
+        next_sfx(h1, h2, e, m1, m2);
+    else
+        prev_pfx(h1, h2, e);
+    out[i] = ((uint64_t)h1 << 32) | h2;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers.
+hipError_t launch_map(const MapArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    uint32_t grid = (a.n + 255) / 256;
+    size_t lds = (size_t)256 * a.Lpad;
+    hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const ScatterArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_dedup<BUCKET_CAP>, dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s) {
+    hipLaunchKernelGGL((k_join<BUCKET_CAP, MATCH_CAP>), dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(1024), 0, s, a.cnt, a.nb, dst_off);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_edge_copy, dim3((a.nb + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,
+                       uint64_t *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_roll, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, keys, bases, n, kind, m1, m2, out);
+    return hipGetLastError();
+}
+
+}  // namespace emp

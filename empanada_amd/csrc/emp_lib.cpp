@@ -1,0 +1,901 @@
+// Host side of libempanada_hip: contexts, staging, the round loop and the C ABI.
+//
+// Reference map: main() (main.cu:9-77) drives read_block / transpose / fingerprint / queue pushes
+// per FASTQ block, then init_match + build_graph. Here a context replaces the globals and
+// singletons (Config, Kmp/Vmp, M, d_buf: config.h:112-115, mempool.cuh:89-91, overlap.cu:41-43),
+// the host CQueues become HBM-resident bucketed lists, and every Thrust stage is a kernel of
+// emp_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/empanada.h"
+#include "../../include/empanada_stages.h"
+#include "emp_fastq.h"
+#include "emp_kernels.h"
+
+using namespace emp;
+
+namespace {
+
+constexpr uint32_t STAGE_READS = 1u << 20;  // reads per host->device batch (multiple of 16)
+constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large-bucket path
+
+struct DevCounters {
+    unsigned long long pool_cursor;
+    unsigned long long surv_s, surv_p;
+    unsigned long long kept;
+    uint32_t n_big;
+    uint32_t err;
+};
+
+double now_ms() {
+    timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec * 1e3 + t.tv_nsec * 1e-6;
+}
+
+uint32_t modpow(uint64_t a, uint64_t e, uint32_t p) {
+    uint64_t r = 1;
+    a %= p;
+    while (e) {
+        if (e & 1) r = r * a % p;
+        a = a * a % p;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+// Partition fan-out: smallest bits with ceil(X / 2^bits) <= BUCKET_TARGET.
+int bucket_bits(uint64_t x) {
+    int b = 0;
+    while (b < 24 && ((x + (1ull << b) - 1) >> b) > BUCKET_TARGET) ++b;
+    return b;
+}
+
+struct KStat {
+    std::string name;
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0;
+};
+
+struct Pending {
+    int k;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+}  // namespace
+
+struct emp_ctx {
+    emp_opts opts{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    uint64_t bytes = 0;
+    std::vector<std::pair<void *, size_t>> allocs;
+
+    // reads / columns / entries
+    int L = 0;
+    uint64_t n = 0;  // reads mapped (submitted to the device)
+    std::vector<uint32_t *> chunks;
+    uint32_t **d_chunk_tab = nullptr;
+    size_t chunk_tab_cap = 0;
+    Entry *d_entries = nullptr;
+    uint64_t entries_cap = 0;
+    uint32_t *d_err = nullptr;
+
+    // host staging (pinned, double-buffered)
+    char *h_stage[2] = {nullptr, nullptr};
+    uint8_t *d_batch[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_busy[2] = {false, false};
+    uint32_t stage_n = 0;
+    int stage_cur = 0;
+    size_t stage_bytes = 0;
+
+    // overlap phase
+    Entry *d_list[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t *d_cnt[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t nb_alloc = 0;
+    uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
+    uint32_t *h_from = nullptr, *h_to = nullptr;
+    uint64_t pool_cap = 0;
+    unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr;
+    uint32_t *d_edge_cnt = nullptr;
+    uint32_t *d_big = nullptr;
+    DevCounters *d_ctr = nullptr, *h_ctr = nullptr;
+
+    // statistics
+    std::vector<emp_round_stats> rounds;
+    emp_summary summary{};
+
+    // kernel timing
+    std::vector<KStat> kstats;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+};
+
+namespace {
+
+int fail(emp_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                                   \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return fail((c), EMP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+int dalloc(emp_ctx *c, void **p, size_t sz) {
+    if (sz == 0) sz = 256;
+    if (c->opts.hbm_budget && c->bytes + sz > c->opts.hbm_budget)
+        return fail(c, EMP_ERR_NOMEM, "HBM budget of %llu bytes exceeded (need %llu more)",
+                    (unsigned long long)c->opts.hbm_budget, (unsigned long long)sz);
+    hipError_t e = hipMalloc(p, sz);
+    if (e != hipSuccess) return fail(c, EMP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+    c->bytes += sz;
+    c->allocs.push_back({*p, sz});
+    return EMP_OK;
+}
+
+void dfree(emp_ctx *c, void *p) {
+    if (!p) return;
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        if (c->allocs[i].first == p) {
+            c->bytes -= c->allocs[i].second;
+            c->allocs.erase(c->allocs.begin() + (long)i);
+            break;
+        }
+    (void)hipFree(p);
+}
+
+int kstat_id(emp_ctx *c, const char *name) {
+    for (size_t i = 0; i < c->kstats.size(); ++i)
+        if (c->kstats[i].name == name) return (int)i;
+    c->kstats.push_back(KStat{name});
+    return (int)c->kstats.size() - 1;
+}
+
+hipEvent_t get_event(emp_ctx *c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Brackets a launch with events when profiling. `bytes` is the launch's algorithmic traffic.
+struct Timed {
+    emp_ctx *c;
+    int k = -1;
+    hipEvent_t a = nullptr;
+    double bytes;
+    Timed(emp_ctx *c_, const char *name, double bytes_) : c(c_), bytes(bytes_) {
+        if (c->opts.profile) {
+            k = kstat_id(c, name);
+            a = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (k >= 0) {
+            hipEvent_t b = get_event(c);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back(Pending{k, a, b, bytes});
+        }
+    }
+};
+
+// Call after a stream synchronisation: folds finished launch timings into the statistics.
+void collect_timings(emp_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        KStat &k = c->kstats[p.k];
+        k.launches++;
+        k.ms += ms;
+        k.bytes += p.bytes;
+        c->free_events.push_back(p.a);
+        c->free_events.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+int sync(emp_ctx *c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_timings(c);
+    return EMP_OK;
+}
+
+int ensure_chunks(emp_ctx *c, uint64_t n_total) {
+    size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
+    if (need <= c->chunks.size()) return EMP_OK;
+    while (c->chunks.size() < need) {
+        void *p = nullptr;
+        int rc = dalloc(c, &p, (size_t)c->L * CHUNK_WORDS * sizeof(uint32_t));
+        if (rc) return rc;
+        c->chunks.push_back((uint32_t *)p);
+    }
+    if (c->chunks.size() > c->chunk_tab_cap) {
+        size_t cap = std::max<size_t>(64, c->chunk_tab_cap * 2);
+        while (cap < c->chunks.size()) cap *= 2;
+        if (c->d_chunk_tab) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dfree(c, c->d_chunk_tab);
+        }
+        void *p = nullptr;
+        int rc = dalloc(c, &p, cap * sizeof(uint32_t *));
+        if (rc) return rc;
+        c->d_chunk_tab = (uint32_t **)p;
+        c->chunk_tab_cap = cap;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_chunk_tab, c->chunks.data(), c->chunks.size() * sizeof(uint32_t *),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMP_OK;
+}
+
+int ensure_entries(emp_ctx *c, uint64_t n_entries) {
+    if (n_entries <= c->entries_cap) return EMP_OK;
+    uint64_t cap = std::max<uint64_t>(c->entries_cap * 2, 1u << 16);
+    while (cap < n_entries) cap *= 2;
+    void *p = nullptr;
+    int rc = dalloc(c, &p, cap * sizeof(Entry));
+    if (rc) return rc;
+    if (c->d_entries) {
+        HIPCHK(c, hipMemcpyAsync(p, c->d_entries, 2 * c->n * sizeof(Entry), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c, c->d_entries);
+    }
+    c->d_entries = (Entry *)p;
+    c->entries_cap = cap;
+    return EMP_OK;
+}
+
+void free_reads_storage(emp_ctx *c) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto p : c->chunks) dfree(c, p);
+    c->chunks.clear();
+    for (int i = 0; i < 2; ++i) {
+        dfree(c, c->d_batch[i]);
+        c->d_batch[i] = nullptr;
+        if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+        c->h_stage[i] = nullptr;
+    }
+    c->stage_bytes = 0;
+}
+
+int set_len(emp_ctx *c, int L) {
+    if (L <= 0 || L > MAX_RD_LEN) return fail(c, EMP_ERR_READLEN, "read length %d outside [1, 255]", L);
+    if (c->L == L) return EMP_OK;
+    if (c->n == 0 && c->stage_n == 0) {
+        // first read of a run: (re)size the per-length storage
+        if (c->L != 0) free_reads_storage(c);
+        c->L = L;
+        return EMP_OK;
+    }
+    return fail(c, EMP_ERR_READLEN, "All reads must be of equal length (%d vs %d)", c->L, L);
+}
+
+// Runs the map kernel over n reads at device address d (4-byte aligned), appending at c->n.
+int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
+    if (nreads == 0) return EMP_OK;
+    if (c->n % 16) return fail(c, EMP_ERR_ARG, "a batch follows one whose size is not a multiple of 16");
+    if (2 * (c->n + nreads) >= (1ull << 32)) return fail(c, EMP_ERR_TOO_MANY, "more than 2^31 - 1 reads");
+    int rc = ensure_chunks(c, c->n + nreads);
+    if (rc) return rc;
+    rc = ensure_entries(c, 2 * (c->n + nreads));
+    if (rc) return rc;
+    int Lpad = (c->L + 3) & ~3;
+    if (((Lpad / 4) & 1) == 0) Lpad += 4;
+    uint64_t done = 0;
+    while (done < nreads) {
+        uint32_t m = (uint32_t)std::min<uint64_t>(nreads - done, 1u << 30);
+        MapArgs a;
+        a.reads = d + done * (uint64_t)c->L;
+        a.n = m;
+        a.L = c->L;
+        a.Lpad = Lpad;
+        a.k0 = c->n + done;
+        a.chunks = c->d_chunk_tab;
+        a.entries = c->d_entries;
+        a.err = c->d_err;
+        {
+            Timed t(c, "k_map", (double)m * c->L * 1.25 + 24.0 * m);
+            HIPCHK(c, launch_map(a, c->stream));
+        }
+        done += m;
+    }
+    c->n += nreads;
+    return EMP_OK;
+}
+
+int stage_submit(emp_ctx *c) {
+    if (c->stage_n == 0) return EMP_OK;
+    int s = c->stage_cur;
+    size_t bytes = (size_t)c->stage_n * c->L;
+    if (!c->d_batch[s]) {
+        void *p = nullptr;
+        int rc = dalloc(c, &p, c->stage_bytes);
+        if (rc) return rc;
+        c->d_batch[s] = (uint8_t *)p;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_batch[s], c->h_stage[s], bytes, hipMemcpyHostToDevice, c->stream));
+    int rc = map_batch(c, c->d_batch[s], c->stage_n);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->stage_ev[s], c->stream));
+    c->stage_busy[s] = true;
+    c->stage_cur ^= 1;
+    c->stage_n = 0;
+    int t = c->stage_cur;
+    if (c->stage_busy[t]) {
+        HIPCHK(c, hipEventSynchronize(c->stage_ev[t]));
+        c->stage_busy[t] = false;
+    }
+    return EMP_OK;
+}
+
+int stage_read(emp_ctx *c, const char *seq, size_t len) {
+    int rc = set_len(c, (int)std::min<size_t>(len, 1u << 20));
+    if (rc) return rc;
+    if (!c->h_stage[0]) {
+        c->stage_bytes = (size_t)STAGE_READS * c->L;
+        for (int i = 0; i < 2; ++i) {
+            HIPCHK(c, hipHostMalloc((void **)&c->h_stage[i], c->stage_bytes, hipHostMallocDefault));
+            if (!c->stage_ev[i]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+        }
+    }
+    memcpy(c->h_stage[c->stage_cur] + (size_t)c->stage_n * c->L, seq, len);
+    if (++c->stage_n == STAGE_READS) return stage_submit(c);
+    return EMP_OK;
+}
+
+int read_cb(void *user, const char *seq, size_t len) { return stage_read((emp_ctx *)user, seq, len); }
+
+int ensure_lists(emp_ctx *c, uint32_t nb) {
+    if (nb <= c->nb_alloc) return EMP_OK;
+    for (int i = 0; i < 4; ++i) {
+        dfree(c, c->d_list[i]);
+        dfree(c, c->d_cnt[i]);
+        c->d_list[i] = nullptr;
+        c->d_cnt[i] = nullptr;
+    }
+    dfree(c, c->d_edge_off);
+    dfree(c, c->d_dst_off);
+    dfree(c, c->d_edge_cnt);
+    c->nb_alloc = 0;
+    for (int i = 0; i < 4; ++i) {
+        void *p = nullptr;
+        int rc = dalloc(c, &p, (size_t)nb * BUCKET_CAP * sizeof(Entry));
+        if (rc) return rc;
+        c->d_list[i] = (Entry *)p;
+        rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t));
+        if (rc) return rc;
+        c->d_cnt[i] = (uint32_t *)p;
+    }
+    void *p = nullptr;
+    int rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long));
+    if (rc) return rc;
+    c->d_edge_off = (unsigned long long *)p;
+    rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long));
+    if (rc) return rc;
+    c->d_dst_off = (unsigned long long *)p;
+    rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t));
+    if (rc) return rc;
+    c->d_edge_cnt = (uint32_t *)p;
+    c->nb_alloc = nb;
+    return EMP_OK;
+}
+
+int ensure_pool(emp_ctx *c, uint64_t cap) {
+    if (cap <= c->pool_cap) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->d_pool_from);
+    dfree(c, c->d_pool_to);
+    dfree(c, c->d_out_from);
+    dfree(c, c->d_out_to);
+    if (c->h_from) (void)hipHostFree(c->h_from);
+    if (c->h_to) (void)hipHostFree(c->h_to);
+    c->h_from = c->h_to = nullptr;
+    c->pool_cap = 0;
+    void *p = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &p, cap * 4))) return rc;
+    c->d_pool_from = (uint32_t *)p;
+    if ((rc = dalloc(c, &p, cap * 4))) return rc;
+    c->d_pool_to = (uint32_t *)p;
+    if ((rc = dalloc(c, &p, cap * 4))) return rc;
+    c->d_out_from = (uint32_t *)p;
+    if ((rc = dalloc(c, &p, cap * 4))) return rc;
+    c->d_out_to = (uint32_t *)p;
+    HIPCHK(c, hipHostMalloc((void **)&c->h_from, cap * 4, hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void **)&c->h_to, cap * 4, hipHostMallocDefault));
+    c->pool_cap = cap;
+    return EMP_OK;
+}
+
+RoundArgs round_args(const emp_ctx *c, int t, bool active) {
+    RoundArgs r{};
+    r.active = active ? 1 : 0;
+    if (!active) return r;
+    const int L = c->L;
+    r.col_t = (uint32_t)t;
+    r.col_back = (uint32_t)(L - 1 - t);
+    r.m1 = modpow(R1, (uint64_t)(L - t - 1), P1);
+    r.m2 = modpow(R2, (uint64_t)(L - t - 1), P2);
+    return r;
+}
+
+int read_counters(emp_ctx *c) {
+    HIPCHK(c, hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
+    int rc = stage_submit(c);
+    if (rc) return rc;
+    rc = sync(c);
+    if (rc) return rc;
+    for (int s = 0; s < 2; ++s) c->stage_busy[s] = false;
+    uint32_t herr = 0;
+    HIPCHK(c, hipMemcpy(&herr, c->d_err, sizeof herr, hipMemcpyDeviceToHost));
+    if (herr) return fail(c, EMP_ERR_BASE, "a kept read holds a byte other than ACGTacgt");
+
+    c->rounds.clear();
+    emp_summary &S = c->summary;
+    S.n_reads = c->n;
+    S.read_len = c->L;
+    S.n_rounds = 0;
+    S.n_dedup = 0;
+    S.n_records = 0;
+    S.n_edges = 0;
+    const uint64_t n = c->n;
+    if (n == 0) return EMP_OK;  // overlap.cu:231,258: an empty file
+    const int L = c->L;
+    const int rounds = L - c->opts.min_ovlp;
+    const uint64_t N0 = 2 * n;
+    const uint64_t M = c->opts.reduce_mem / 16;
+    if (N0 > M)
+        return fail(c, EMP_ERR_OUT_OF_CORE,
+                    "2n = %llu entries exceed M = reduce_mem/16 = %llu: the reference's out-of-core merge order "
+                    "is not reproduced; raise -r",
+                    (unsigned long long)N0, (unsigned long long)M);
+
+    const int bbD = bucket_bits(N0);
+    const uint32_t nbD = 1u << bbD;
+    if ((rc = ensure_lists(c, nbD))) return rc;
+    if ((rc = ensure_pool(c, std::max<uint64_t>(1u << 20, N0 / 4)))) return rc;
+
+    // dedup input in list 2; round lists: (0,1) current, (2,3) next
+    int cs = 0, cp = 1, ns_ = 2, np_ = 3;
+    HIPCHK(c, hipMemsetAsync(c->d_cnt[2], 0, nbD * sizeof(uint32_t), c->stream));
+    {
+        ScatterArgs a{c->d_entries, N0, c->d_list[2], c->d_cnt[2], (uint32_t)BUCKET_CAP, 29 - bbD};
+        Timed t(c, "k_scatter", 24.0 * N0);
+        HIPCHK(c, launch_scatter(a, c->stream));
+    }
+    int bb = bbD;
+    HIPCHK(c, hipMemsetAsync(c->d_cnt[cs], 0, nbD * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt[cp], 0, nbD * sizeof(uint32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(DevCounters), c->stream));
+    {
+        DedupArgs a{};
+        a.in = c->d_list[2];
+        a.cnt = c->d_cnt[2];
+        a.chunks = c->d_chunk_tab;
+        a.r0 = round_args(c, 0, rounds > 0);
+        a.s_out = c->d_list[cs];
+        a.p_out = c->d_list[cp];
+        a.s_cnt = c->d_cnt[cs];
+        a.p_cnt = c->d_cnt[cp];
+        a.cap = BUCKET_CAP;
+        a.out_shift = 29 - bb;
+        a.big = c->d_big;
+        a.n_big = &c->d_ctr->n_big;
+        a.big_cap = BIG_CAP;
+        a.n_kept = &c->d_ctr->kept;
+        Timed t(c, "k_dedup", 12.0 * N0 + (rounds > 0 ? 24.0 : 0.0) * N0);
+        HIPCHK(c, launch_dedup(a, nbD, c->stream));
+    }
+    if ((rc = read_counters(c))) return rc;
+    if (c->h_ctr->n_big)
+        return fail(c, EMP_ERR_CAPACITY, "%u dedup buckets exceed %d entries", c->h_ctr->n_big, BUCKET_CAP);
+    uint64_t St = c->h_ctr->kept, Pt = St;
+    S.n_dedup = St;
+
+    int t = 0;
+    for (; t < rounds && St > 0; ++t) {
+        if (Pt == 0) break;  // no record can follow: every later round has an empty prefix list
+        if (St >= M / 2 || Pt >= M / 2)
+            return fail(c, EMP_ERR_PARTITION,
+                        "round %d: %llu suffixes / %llu prefixes reach M/2 = %llu: multi-record framing of "
+                        "cqutils.cuh:107-137 is not reproduced; raise -r",
+                        t, (unsigned long long)St, (unsigned long long)Pt, (unsigned long long)(M / 2));
+        const int bbn = bucket_bits(std::max(St, Pt));
+        const uint32_t nbn = 1u << bbn, nbt = 1u << bb;
+        const bool more = t + 1 < rounds;
+        emp_round_stats rs{};
+        rs.s = St;
+        rs.p = Pt;
+        rs.ovlp = L - 1 - t;
+        rs.bucket_bits = bb;
+        for (int attempt = 0;; ++attempt) {
+            if (more) {
+                HIPCHK(c, hipMemsetAsync(c->d_cnt[ns_], 0, nbn * sizeof(uint32_t), c->stream));
+                HIPCHK(c, hipMemsetAsync(c->d_cnt[np_], 0, nbn * sizeof(uint32_t), c->stream));
+            }
+            HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(DevCounters), c->stream));
+            JoinArgs a{};
+            a.s_in = c->d_list[cs];
+            a.p_in = c->d_list[cp];
+            a.s_cnt = c->d_cnt[cs];
+            a.p_cnt = c->d_cnt[cp];
+            a.chunks = c->d_chunk_tab;
+            a.rn = round_args(c, t + 1, more);
+            a.s_out = c->d_list[ns_];
+            a.p_out = c->d_list[np_];
+            a.s_out_cnt = c->d_cnt[ns_];
+            a.p_out_cnt = c->d_cnt[np_];
+            a.cap = BUCKET_CAP;
+            a.out_shift = 29 - bbn;
+            a.pool_from = c->d_pool_from;
+            a.pool_to = c->d_pool_to;
+            a.pool_cursor = &c->d_ctr->pool_cursor;
+            a.pool_cap = c->pool_cap;
+            a.edge_off = c->d_edge_off;
+            a.edge_cnt = c->d_edge_cnt;
+            a.big = c->d_big;
+            a.n_big = &c->d_ctr->n_big;
+            a.big_cap = BIG_CAP;
+            a.surv_s = &c->d_ctr->surv_s;
+            a.surv_p = &c->d_ctr->surv_p;
+            {
+                Timed tm(c, "k_join", 0.0);
+                HIPCHK(c, launch_join(a, nbt, c->stream));
+            }
+            if ((rc = read_counters(c))) return rc;
+            if (c->h_ctr->pool_cursor <= c->pool_cap) break;
+            if (attempt > 2) return fail(c, EMP_ERR_CAPACITY, "edge pool did not settle");
+            if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
+        }
+        const DevCounters &k = *c->h_ctr;
+        if (k.n_big) return fail(c, EMP_ERR_CAPACITY, "round %d: %u buckets exceed the LDS join capacity", t, k.n_big);
+        const uint64_t E = k.pool_cursor;
+        rs.edges = E;
+        rs.s_after = k.surv_s;
+        rs.p_after = k.surv_p;
+        // algorithmic bytes of the join: entries read, survivors written, edges written
+        if (c->opts.profile && !c->kstats.empty()) {
+            int id = kstat_id(c, "k_join");
+            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (k.surv_s + k.surv_p) : 0.0) + 8.0 * E;
+        }
+        // record (overlap.cu:422; written even when E == 0)
+        {
+            EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, nbt,
+                            c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
+            if (E) {
+                Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
+                HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, E * 4, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, E * 4, hipMemcpyDeviceToHost, c->stream));
+                if ((rc = sync(c))) return rc;
+            }
+            if (sink) {
+                int src = sink(user, E, L - 1 - t, c->h_from, c->h_to);
+                if (src) return fail(c, src, "record sink returned %d", src);
+            }
+            rs.records = 1;
+            S.n_records++;
+            S.n_edges += E;
+        }
+        c->rounds.push_back(rs);
+        St = k.surv_s;
+        Pt = k.surv_p;
+        std::swap(cs, ns_);
+        std::swap(cp, np_);
+        bb = bbn;
+    }
+    S.n_rounds = t;
+    return EMP_OK;
+}
+
+int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to) {
+    FILE *f = (FILE *)user;
+    if (fwrite(&n, sizeof n, 1, f) != 1) return EMP_ERR_IO;
+    if (fwrite(&ovlp, sizeof ovlp, 1, f) != 1) return EMP_ERR_IO;
+    if (n && fwrite(from, sizeof(uint32_t), n, f) != n) return EMP_ERR_IO;
+    if (n && fwrite(to, sizeof(uint32_t), n, f) != n) return EMP_ERR_IO;
+    return EMP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *emp_version(void) { return "empanada-mi355x 0.1 (gfx950)"; }
+
+void emp_opts_default(emp_opts *o) {
+    memset(o, 0, sizeof *o);
+    o->min_ovlp = 31;
+    o->map_mem = 1ull << 30;
+    o->reduce_mem = 1ull << 30;
+    o->base_blocksz = 32ull << 20;
+    o->kv_chunksz = 64ull << 20;
+}
+
+int emp_ctx_create(const emp_opts *opts, emp_ctx **out) {
+    if (!out) return EMP_ERR_ARG;
+    *out = nullptr;
+    emp_ctx *c = new emp_ctx();
+    if (opts)
+        c->opts = *opts;
+    else
+        emp_opts_default(&c->opts);
+    if (c->opts.min_ovlp < 0) {
+        delete c;
+        return EMP_ERR_ARG;
+    }
+    c->device = c->opts.device;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) {
+        delete c;
+        return EMP_ERR_HIP;
+    }
+    if (c->opts.stream) {
+        c->stream = (hipStream_t)c->opts.stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return EMP_ERR_HIP;
+        }
+        c->own_stream = true;
+    }
+    void *p = nullptr;
+    if (dalloc(c, &p, sizeof(uint32_t)) || hipMemset(p, 0, sizeof(uint32_t)) != hipSuccess) {
+        emp_ctx_destroy(c);
+        return EMP_ERR_NOMEM;
+    }
+    c->d_err = (uint32_t *)p;
+    if (dalloc(c, &p, sizeof(DevCounters)) || dalloc(c, (void **)&c->d_big, BIG_CAP * sizeof(uint32_t))) {
+        emp_ctx_destroy(c);
+        return EMP_ERR_NOMEM;
+    }
+    c->d_ctr = (DevCounters *)p;
+    if (hipHostMalloc((void **)&c->h_ctr, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
+        emp_ctx_destroy(c);
+        return EMP_ERR_HIP;
+    }
+    *out = c;
+    return EMP_OK;
+}
+
+void emp_ctx_destroy(emp_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &a : c->allocs) (void)hipFree(a.first);
+    c->allocs.clear();
+    for (int i = 0; i < 2; ++i) {
+        if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+        if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+    }
+    if (c->h_from) (void)hipHostFree(c->h_from);
+    if (c->h_to) (void)hipHostFree(c->h_to);
+    if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *emp_last_error(const emp_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int emp_add_fastq(emp_ctx *c, const char *data, size_t size, int final, size_t *consumed) {
+    if (!c || (!data && size) || !consumed) return EMP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    int rc = parse_fastq_block(data, size, final != 0, consumed, read_cb, c);
+    if (rc == EMP_ERR_FORMAT)
+        return fail(c, rc, "malformed FASTQ block: it must start with '@' and hold a complete record");
+    if (rc == EMP_ERR_BASE) return fail(c, rc, "a kept read holds a byte other than ACGTacgt");
+    return rc;
+}
+
+int emp_add_reads(emp_ctx *c, const char *reads, uint64_t n, int32_t read_len) {
+    if (!c || (!reads && n)) return EMP_ERR_ARG;
+    if (n == 0) return EMP_OK;
+    (void)hipSetDevice(c->device);
+    int rc = set_len(c, read_len);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < n; ++i) {
+        rc = stage_read(c, reads + i * (uint64_t)read_len, (size_t)read_len);
+        if (rc) return rc;
+    }
+    return EMP_OK;
+}
+
+int emp_add_reads_device(emp_ctx *c, const void *d_reads, uint64_t n, int32_t read_len) {
+    if (!c || (!d_reads && n)) return EMP_ERR_ARG;
+    if (n == 0) return EMP_OK;
+    if (((uintptr_t)d_reads) & 3) return fail(c, EMP_ERR_ARG, "device reads must be 4-byte aligned");
+    (void)hipSetDevice(c->device);
+    int rc = set_len(c, read_len);
+    if (rc) return rc;
+    if ((rc = stage_submit(c))) return rc;
+    if ((rc = map_batch(c, (const uint8_t *)d_reads, n))) return rc;
+    return sync(c);
+}
+
+int emp_run(emp_ctx *c, emp_record_sink sink, void *user) {
+    if (!c) return EMP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    double t0 = now_ms();
+    int rc = run_impl(c, sink, user);
+    c->summary.overlap_ms = now_ms() - t0;
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        collect_timings(c);
+    }
+    return rc;
+}
+
+int emp_run_to_file(emp_ctx *c, const char *path) {
+    if (!c || !path) return EMP_ERR_ARG;
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(c, EMP_ERR_IO, "Could not open %s", path);
+    int rc = emp_run(c, file_sink, f);
+    if (fclose(f) != 0 && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "closing %s failed", path);
+    return rc;
+}
+
+int emp_reset(emp_ctx *c) {
+    if (!c) return EMP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    c->n = 0;
+    c->stage_n = 0;
+    c->stage_busy[0] = c->stage_busy[1] = false;
+    c->rounds.clear();
+    c->summary = emp_summary{};
+    HIPCHK(c, hipMemset(c->d_err, 0, sizeof(uint32_t)));
+    return EMP_OK;
+}
+
+int emp_get_summary(const emp_ctx *c, emp_summary *out) {
+    if (!c || !out) return EMP_ERR_ARG;
+    *out = c->summary;
+    return EMP_OK;
+}
+
+int emp_get_round(const emp_ctx *c, int32_t round, emp_round_stats *out) {
+    if (!c || !out || round < 0 || (size_t)round >= c->rounds.size()) return EMP_ERR_ARG;
+    *out = c->rounds[(size_t)round];
+    return EMP_OK;
+}
+
+int32_t emp_kernel_count(const emp_ctx *c) { return c ? (int32_t)c->kstats.size() : 0; }
+
+int emp_kernel_info(const emp_ctx *c, int32_t i, const char **name, uint64_t *launches, double *ms, double *bytes) {
+    if (!c || i < 0 || (size_t)i >= c->kstats.size()) return EMP_ERR_ARG;
+    const KStat &k = c->kstats[(size_t)i];
+    if (name) *name = k.name.c_str();
+    if (launches) *launches = k.launches;
+    if (ms) *ms = k.ms;
+    if (bytes) *bytes = k.bytes;
+    return EMP_OK;
+}
+
+void emp_kernel_clear(emp_ctx *c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    c->kstats.clear();
+}
+
+void *emp_stream(const emp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// ---------------------------------------------------------------------------------------------
+// Stage entry points.
+
+int emp_stage_fingerprints(int32_t device, const char *reads, uint64_t n, int32_t len, uint64_t *keys) {
+    if ((!reads || !keys) && n) return EMP_ERR_ARG;
+    if (n == 0) return EMP_OK;
+    emp_opts o;
+    emp_opts_default(&o);
+    o.device = device;
+    emp_ctx *c = nullptr;
+    int rc = emp_ctx_create(&o, &c);
+    if (rc) return rc;
+    void *d = nullptr;
+    rc = dalloc(c, &d, n * (uint64_t)len);
+    if (!rc && hipMemcpy(d, reads, n * (uint64_t)len, hipMemcpyHostToDevice) != hipSuccess) rc = EMP_ERR_HIP;
+    if (!rc) rc = emp_add_reads_device(c, d, n, len);
+    uint32_t herr = 0;
+    if (!rc && hipMemcpy(&herr, c->d_err, 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
+    if (!rc && herr) rc = EMP_ERR_BASE;
+    if (!rc) {
+        std::vector<Entry> e(2 * n);
+        if (hipMemcpy(e.data(), c->d_entries, 2 * n * sizeof(Entry), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = EMP_ERR_HIP;
+        for (uint64_t i = 0; !rc && i < 2 * n; ++i) keys[i] = ((uint64_t)e[i].h1 << 32) | e[i].h2;
+    }
+    emp_ctx_destroy(c);
+    return rc;
+}
+
+int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len, uint32_t *cols) {
+    if ((!reads || !cols) && n) return EMP_ERR_ARG;
+    if (n == 0) return EMP_OK;
+    emp_opts o;
+    emp_opts_default(&o);
+    o.device = device;
+    emp_ctx *c = nullptr;
+    int rc = emp_ctx_create(&o, &c);
+    if (rc) return rc;
+    void *d = nullptr;
+    rc = dalloc(c, &d, n * (uint64_t)len);
+    if (!rc && hipMemcpy(d, reads, n * (uint64_t)len, hipMemcpyHostToDevice) != hipSuccess) rc = EMP_ERR_HIP;
+    if (!rc) rc = emp_add_reads_device(c, d, n, len);
+    const uint64_t W = (n + 15) / 16;
+    for (int j = 0; !rc && j < len; ++j) {
+        for (size_t ch = 0; !rc && ch < c->chunks.size(); ++ch) {
+            uint64_t w0 = (uint64_t)ch * CHUNK_WORDS;
+            if (w0 >= W) break;
+            uint64_t cnt = std::min<uint64_t>(CHUNK_WORDS, W - w0);
+            if (hipMemcpy(cols + (uint64_t)j * W + w0, c->chunks[ch] + (size_t)j * CHUNK_WORDS, cnt * 4,
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                rc = EMP_ERR_HIP;
+        }
+    }
+    emp_ctx_destroy(c);
+    return rc;
+}
+
+int emp_stage_roll(int32_t device, const uint64_t *keys, const char *bases, uint64_t n, int32_t kind, int32_t len,
+                   uint64_t *out) {
+    if ((!keys || !bases || !out) && n) return EMP_ERR_ARG;
+    if (kind != 0 && kind != 1) return EMP_ERR_ARG;
+    if (kind == 0 && (len < 1 || len > MAX_RD_LEN)) return EMP_ERR_ARG;
+    if (n == 0) return EMP_OK;
+    if (hipSetDevice(device) != hipSuccess) return EMP_ERR_HIP;
+    uint64_t *dk = nullptr, *dout = nullptr;
+    uint8_t *db = nullptr;
+    int rc = EMP_OK;
+    if (hipMalloc(&dk, n * 8) != hipSuccess || hipMalloc(&dout, n * 8) != hipSuccess || hipMalloc(&db, n) != hipSuccess)
+        rc = EMP_ERR_NOMEM;
+    if (!rc && (hipMemcpy(dk, keys, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(db, bases, n, hipMemcpyHostToDevice) != hipSuccess))
+        rc = EMP_ERR_HIP;
+    uint32_t m1 = kind == 0 ? modpow(R1, (uint64_t)(len - 1), P1) : 0;
+    uint32_t m2 = kind == 0 ? modpow(R2, (uint64_t)(len - 1), P2) : 0;
+    if (!rc && launch_roll(dk, db, n, kind, m1, m2, dout, nullptr) != hipSuccess) rc = EMP_ERR_HIP;
+    if (!rc && hipMemcpy(out, dout, n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
+    (void)hipFree(dk);
+    (void)hipFree(dout);
+    (void)hipFree(db);
+    return rc;
+}
+
+}  // extern "C"

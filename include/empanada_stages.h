@@ -1,0 +1,43 @@
+/*
+ * empanada-mi355x — stage-level C ABI: single kernels of the path exposed on host buffers, for
+ * callers (and parity tests) that need one stage rather than the whole pipeline. Each call
+ * allocates its device buffers, runs on the default stream of `device`, copies the result back
+ * and frees; they are not the fast path.
+ */
+#ifndef EMPANADA_STAGES_H
+#define EMPANADA_STAGES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * generate_fingerprint_read_pairs (fingerprint.cu:75-115) with get_reverse_complements
+ * (preprocess.cu:246-284): for n kept reads (row-major n x len, ACGTacgt) writes 2n keys,
+ * key(read_k) at 2k and key(revcomp(read_k)) at 2k+1, each HIGH32 = H(R=5,P=536870909),
+ * LOW32 = H(R=7,P=536870879) (assembly.cuh:29-34).
+ */
+int emp_stage_fingerprints(int32_t device, const char *reads, uint64_t n, int32_t len, uint64_t *keys);
+
+/*
+ * transpose_reads + TBlockList::pack (fingerprint.cu:125-153, blocks.cuh:30-47), in this
+ * library's layout: column j of n reads as ceil(n/16) 32-bit words, base k in bits
+ * 2*(k%16)+1 : 2*(k%16) of word k/16, code (c>>1)&3 (A0 C1 T2 G3). cols holds len * ceil(n/16) words.
+ */
+int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len, uint32_t *cols);
+
+/*
+ * The rolling transforms of regenerate_fingerprints (overlap.cu:647,651): kind 0 = NextSfx with
+ * length `len` (overlap.cu:57-75), kind 1 = PrevPfx (overlap.cu:77-91, including its uint64_t
+ * wrap). bases are ASCII (ACGTacgt); out may alias keys.
+ */
+int emp_stage_roll(int32_t device, const uint64_t *keys, const char *bases, uint64_t n, int32_t kind, int32_t len,
+                   uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,102 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — the oracle.
+ *
+ * A CPU restatement, in C11 + OpenMP, of empanada's fingerprint -> sort -> suffix/prefix overlap
+ * path. It follows the reference's steps (sort by read id, even/odd partition, gather, rolling
+ * update, stable sort by key, lower/upper bound, edge expansion, self-edge removal, used-read
+ * removal) so that its output.bin is the reference's. It is the parity checker for the HIP
+ * library and the host-core baseline; the product library never links, loads or calls it.
+ *
+ * Semantics restated: in-core (2n <= M, M = reduce_mem / 16) including the key-aligned search
+ * partitions that split a round into several records when a list reaches M/2. The reference's
+ * out-of-core chunked sort/merge (N > M) is not restated: such runs return EO_ERR_OUT_OF_CORE.
+ */
+#ifndef EMP_ORACLE_H
+#define EMP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    EO_OK = 0,
+    EO_ERR_IO = -1,
+    EO_ERR_FORMAT = -2,      /* first byte of a block is not '@', or a record is longer than a block */
+    EO_ERR_READLEN = -3,     /* kept reads of unequal length, or length > 255 */
+    EO_ERR_BASE = -4,        /* kept read holds a byte other than ACGTacgt */
+    EO_ERR_OUT_OF_CORE = -5, /* 2n > M: the reference's chunked sort/merge order is not restated */
+    EO_ERR_PARTITION = -6,   /* more than M/2 equal keys: the reference loops forever */
+    EO_ERR_NOMEM = -7,
+    EO_ERR_ARG = -8,
+    EO_ERR_TOO_MANY = -9     /* 2n >= 2^32 read ids */
+};
+
+typedef struct {
+    int min_ovlp;        /* -o */
+    uint64_t map_mem;    /* -m; FASTQ block = map_mem / 16 bytes */
+    uint64_t reduce_mem; /* -r; M = reduce_mem / 16 */
+} eo_opts;
+
+typedef struct {
+    uint64_t n_reads;    /* kept reads */
+    int read_len;
+    uint64_t n_dedup;    /* entries after dedup */
+    int n_rounds;        /* rounds executed */
+    uint64_t n_records;
+    uint64_t n_edges;
+} eo_summary;
+
+/* Per-round live counts (before and after removal) and edges written. */
+typedef struct {
+    uint64_t s, p, s_after, p_after, edges;
+    int ovlp;
+    int records;
+} eo_round;
+
+/* Output records are appended to a growable memory buffer. */
+typedef struct {
+    unsigned char *data;
+    size_t size, cap;
+} eo_buf;
+
+void eo_buf_free(eo_buf *b);
+
+/* Constants and rolling functions (unit level). */
+uint32_t eo_modulo_exp(uint64_t a, unsigned n, unsigned p);
+void eo_modulos(uint32_t *m1, uint32_t *m2); /* 256 entries each */
+uint64_t eo_key_of(const char *s, int len);
+uint64_t eo_key_of_rc(const char *s, int len);
+uint64_t eo_next_sfx(uint64_t key, char base, int len);
+uint64_t eo_prev_pfx(uint64_t key, char base);
+char eo_revc(char c);
+
+/*
+ * FASTQ -> kept reads, with the reference's block / seek-back / N-filter rules. On success
+ * *reads (malloc'd, row-major n x L) and *n, *L are set.
+ */
+int eo_parse_fastq_file(const char *path, uint64_t block_bytes, char **reads, uint64_t *n, int *L);
+int eo_parse_fastq_mem(const char *data, uint64_t size, uint64_t block_bytes, char **reads, uint64_t *n,
+                       int *L);
+
+/* Kept reads -> output records (appended to out). rounds may be NULL (else >= 256 entries). */
+int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
+                     eo_summary *sum, eo_round *rounds);
+
+/* Whole pipeline: FASTQ file -> output.bin file. */
+int eo_run_file(const char *in_path, const char *out_path, const eo_opts *opts, eo_summary *sum,
+                eo_round *rounds);
+
+/* Same, with FASTQ bytes in memory and the records returned in memory. */
+int eo_run_mem(const char *fastq, uint64_t size, const eo_opts *opts, eo_buf *out, eo_summary *sum,
+               eo_round *rounds);
+
+const char *eo_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

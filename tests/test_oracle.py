@@ -1,0 +1,222 @@
+"""Pins the oracle: known answers, rolling-update invariants, the FASTQ reader, and agreement with
+a string-level brute force on small read sets. CPU only."""
+import random
+import struct
+
+import pytest
+
+import bruteforce as bf
+import oracle_lib as ol
+
+P1, P2 = bf.P1, bf.P2
+
+
+def is_prime(p):
+    if p < 2:
+        return False
+    i = 2
+    while i * i <= p:
+        if p % i == 0:
+            return False
+        i += 1
+    return True
+
+
+def test_primes_and_inverses():
+    assert is_prime(P1) and is_prime(P2)
+    assert P1 == 2**29 - 3 and P2 == 2**29 - 33
+    # Fermat inverses (overlap.cu:121-122) and the values they must take.
+    assert ol.lib().eo_modulo_exp(5, P1 - 2, P1) == 107374182 == pow(5, -1, P1)
+    assert ol.lib().eo_modulo_exp(7, P2 - 2, P2) == 76695840 == pow(7, -1, P2)
+    assert (5 * 107374182) % P1 == 1 and (7 * 76695840) % P2 == 1
+
+
+def test_modulo_tables():
+    m1, m2 = ol.modulos()
+    assert m1 == [pow(5, i, P1) for i in range(256)]
+    assert m2 == [pow(7, i, P2) for i in range(256)]
+    assert (m1[99], m1[149], m1[255]) == (418017119, 104577844, 121899516)
+    assert (m2[99], m2[149], m2[255]) == (510103779, 377358276, 526359340)
+
+
+def test_known_keys():
+    assert ol.key_of("A") == 0x0000000100000001
+    assert ol.key_of("ACGT") == 0x000000C6000001D8
+    assert ol.key_of("GATTACA") == 0x0001094700079053
+    # homopolymer: geometric series, e(T) = 3
+    h1 = 3 * (5**12 - 1) // 4 % P1
+    h2 = 3 * (7**12 - 1) // 6 % P2
+    assert ol.key_of("T" * 12) == (h1 << 32) | h2 == 0x0AE9F7BC1C80A59C
+    # lower case encodes like upper case
+    assert ol.key_of("gattaca") == ol.key_of("GATTACA")
+
+
+def test_key_matches_polynomial_definition():
+    rng = random.Random(7)
+    for _ in range(300):
+        n = rng.randint(1, 255)
+        s = "".join(rng.choice("ACGT") for _ in range(n))
+        assert ol.key_of(s) == bf.key(s)
+        assert ol.key_of_rc(s) == bf.key(bf.revcomp(s))
+
+
+def test_next_sfx_invariant():
+    rng = random.Random(11)
+    for _ in range(500):
+        n = rng.randint(2, 255)
+        s = "".join(rng.choice("ACGT") for _ in range(n))
+        assert ol.next_sfx(bf.key(s), s[0], n) == bf.key(s[1:])
+
+
+def test_prev_pfx_invariant_and_wrap():
+    rng = random.Random(13)
+    checked = 0
+    for _ in range(500):
+        n = rng.randint(2, 255)
+        s = "".join(rng.choice("ACGT") for _ in range(n))
+        k = bf.key(s)
+        if (k >> 32) >= 4 and (k & 0xFFFFFFFF) >= 4:
+            assert ol.prev_pfx(k, s[-1]) == bf.key(s[:-1])
+            checked += 1
+    assert checked > 450
+    inv1, inv2 = pow(5, -1, P1), pow(7, -1, P2)
+    # the u64 wrap (overlap.cu:87-88): h < e gives ((INV * (h - e mod 2^64)) mod 2^64) mod P
+    for h, base in [(1, "G"), (0, "A"), (2, "T"), (1, "C"), (0, "G"), (3, "G")]:
+        assert h < bf.enc(base)
+        e = bf.enc(base)
+        want1 = ((inv1 * ((h - e) % 2**64)) % 2**64) % P1
+        want2 = ((inv2 * ((h - e) % 2**64)) % 2**64) % P2
+        got = ol.prev_pfx((h << 32) | h, base)
+        assert got == (want1 << 32) | want2
+        true_step = ((h - e) * inv1) % P1
+        assert want1 != true_step  # the wrap really differs from the modular inverse step
+    assert ol.prev_pfx((1 << 32) | 100, "G") >> 32 == 214748939
+
+
+FQ = lambda seqs: b"".join(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)) for i, s in enumerate(seqs))
+
+
+@pytest.mark.parametrize("block", [10**6, 50, 64, 101, 250])
+def test_reader_rules(block):
+    rng = random.Random(block)
+    seqs = []
+    for i in range(60):
+        s = bytearray(rng.choice(b"ACGT") for _ in range(20))
+        if i % 7 == 3:
+            s[rng.randrange(20)] = ord("N")
+        if i % 11 == 5:
+            s[rng.randrange(20)] = ord("n")
+        if i % 5 == 2:
+            s = s.lower()
+        seqs.append(bytes(s))
+    data = FQ(seqs)
+    rc, reads = ol.parse(data, block)
+    assert rc == 0
+    assert reads == bf.kept_reads(data)
+    assert len(reads) == len([s for s in seqs if b"N" not in s and b"n" not in s])
+
+
+def test_reader_edge_cases():
+    # empty sequence line vanishes; last record without newline is kept
+    data = b"@a\n\n+\n\n@b\nACGT\n+\nIIII\n@c\nTTGA\n+\nIIII"
+    rc, reads = ol.parse(data, 1 << 20)
+    assert rc == 0 and reads == ["ACGT", "TTGA"] == bf.kept_reads(data)
+    # IUPAC bytes with bit 3 set drop the read (K, M, Y, H); others are rejected
+    rc, reads = ol.parse(FQ([b"ACGK", b"ACGT", b"AYGT", b"HHHH"]), 1 << 20)
+    assert rc == 0 and reads == ["ACGT"]
+    assert ol.parse(FQ([b"ACGT", b"ACRT"]), 1 << 20)[0] == -4  # R survives the N filter: unsupported
+    assert ol.parse(FQ([b"ACGT", b"ACG"]), 1 << 20)[0] == -3   # unequal lengths
+    assert ol.parse(b"xx\nACGT\n+\nIIII\n", 1 << 20)[0] == -2  # first byte not '@'
+    assert ol.parse(FQ([b"A" * 40]), 16)[0] == -2              # record longer than a block
+    assert ol.parse(FQ([b"A" * 256]), 1 << 20)[0] == -3        # longer than 255
+    assert ol.parse(b"@a\r\nACGT\r\n+\r\nIIII\r\n", 1 << 20)[0] == -4  # CRLF rejected
+
+
+def _random_reads(rng, n, L, genome_len, p_rc, p_sub):
+    g = "".join(rng.choice("ACGT") for _ in range(genome_len))
+    out = []
+    for _ in range(n):
+        st = rng.randrange(genome_len - L + 1)
+        s = g[st:st + L]
+        if rng.random() < p_rc:
+            s = bf.revcomp(s)
+        s = "".join((rng.choice([b for b in "ACGT" if b != c]) if rng.random() < p_sub else c) for c in s)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("seed,n,L,o,glen,p_rc,p_sub", [
+    (1, 40, 20, 8, 300, 0.0, 0.0),
+    (2, 80, 24, 10, 400, 0.5, 0.0),
+    (3, 120, 30, 12, 500, 0.5, 0.01),
+    (4, 150, 32, 15, 250, 0.5, 0.005),   # high coverage: duplicates and large key groups
+    (5, 60, 16, 16, 200, 0.5, 0.0),      # o == L: no rounds, empty output
+    (6, 200, 28, 6, 150, 0.5, 0.02),     # short overlaps, many matches per key
+])
+def test_oracle_matches_bruteforce(seed, n, L, o, glen, p_rc, p_sub):
+    rng = random.Random(seed)
+    reads = _random_reads(rng, n, L, glen, p_rc, p_sub)
+    assert bf.wrap_free(reads, o)
+    data = FQ([r.encode() for r in reads])
+    rc, out, summ, rounds = ol.run_fastq(data, min_ovlp=o)
+    assert rc == 0
+    recs, stats = bf.overlap(reads, o)
+    assert out == bf.encode(recs)
+    assert len(rounds) == len(stats)
+    for a, b in zip(rounds, stats):
+        for f in ("s", "p", "s_after", "p_after", "edges", "ovlp"):
+            assert a[f] == b[f], (f, a, b)
+
+
+def test_block_size_does_not_change_output():
+    rng = random.Random(99)
+    reads = _random_reads(rng, 100, 25, 400, 0.5, 0.01)
+    data = FQ([r.encode() for r in reads])
+    outs = {ol.run_fastq(data, min_ovlp=10, map_mem=m * 16)[1] for m in (10**6, 97, 160, 333)}
+    assert len(outs) == 1
+
+
+def test_search_partitions_split_records():
+    """Lists reaching M/2 split a round into key-aligned records (cqutils.cuh:107-137); the
+    concatenated edges equal the single-record run."""
+    rng = random.Random(5)
+    reads = _random_reads(rng, 150, 30, 600, 0.5, 0.0)
+    data = FQ([r.encode() for r in reads])
+    rc, big, _, _ = ol.run_fastq(data, min_ovlp=12)
+    assert rc == 0
+    rc, small, _, rounds = ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 320)
+    assert rc == 0
+    a, b = bf.decode(big), bf.decode(small)
+    assert len(b) > len(a)
+    merged = {}
+    for ell, e in b:
+        merged.setdefault(ell, []).extend(e)
+    assert [(ell, merged[ell]) for ell, _ in a] == a
+    # out-of-core (2n > M) is not restated
+    assert ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 200)[0] == -5
+
+
+def test_empty_and_trivial():
+    rc, out, summ, _ = ol.run_fastq(b"")
+    assert rc == 0 and out == b""
+    rc, out, _, _ = ol.run_fastq(FQ([b"NNNN"]))
+    assert rc == 0 and out == b""
+    # a single read: its own suffix/prefix pairs are self edges, so every record is empty
+    rc, out, _, rounds = ol.run_fastq(FQ([b"ACGTTGCAACGTTGCA"]), min_ovlp=4)
+    assert rc == 0
+    recs = bf.decode(out)
+    assert all(len(e) == 0 for _, e in recs) and len(recs) == 12
+    assert [r[0] for r in recs] == list(range(15, 3, -1))
+
+
+def test_record_layout():
+    rng = random.Random(3)
+    reads = _random_reads(rng, 50, 20, 200, 0.5, 0.0)
+    rc, out, summ, rounds = ol.run_fastq(FQ([r.encode() for r in reads]), min_ovlp=8)
+    assert rc == 0
+    off = 0
+    for r in rounds:
+        n, ell = struct.unpack_from("<Qi", out, off)
+        assert n == r["edges"] and ell == r["ovlp"]
+        off += 12 + 8 * n
+    assert off == len(out)
