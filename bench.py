@@ -1,0 +1,169 @@
+"""Benchmark of the overlap path on one MI355X.
+
+Workload (default): 50 Mbp i.i.d. genome, 15,000,000 reads of 100 bp, 50 % reverse-complemented,
+0.5 % substitutions, minimum overlap 63 (seeds 1100 / 1101). One step = the whole path over the
+reads, which are resident in HBM when the timed region starts: fingerprints + 2-bit columns,
+partition, dedup, every overlap round, edge compaction and delivery of every output.bin record to
+host memory. Metric: Gbases/s = n * L / step time.
+
+Prints one JSON line with the roofline of the dominant kernel (HIP events recorded by the library
+on its stream around every launch, algorithmic bytes per launch from the live per-round counts)
+and the oracle timed on the host cores over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(scale, min_ovlp, check_gpu):
+    """The oracle over a bounded sample (same genome/read model and coverage, `scale` of the size)."""
+    import oracle_lib
+    from empanada_amd import gen, native
+
+    reads, _ = gen.workload_reads("incore", scale)
+    n, L = reads.shape
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    rc, want, summ, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
+    dt = time.perf_counter() - t0
+    assert rc == 0, rc
+    same = None
+    if check_gpu:
+        with native.Context(min_ovlp=min_ovlp) as c:
+            c.add_reads(reads)
+            same = c.run() == want
+    return {
+        "value": n * L / dt / 1e9,
+        "unit": "Gbases/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%.0f kbp genome, %d reads x %d bp (%.0f%% of the workload, same coverage), %.2f s" % (
+            5e7 * scale / 1e3, n, L, 100 * scale, dt),
+    }, same
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 50 Mbp / 15 M read workload")
+    ap.add_argument("--cpu-scale", type=float, default=0.1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", action="store_true", help="also compare the full-size output with the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this path is measured on a single MI355X (--gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from empanada_amd import gen, hipmem, native
+
+    t_gen = time.perf_counter()
+    reads, min_ovlp = gen.workload_reads("incore", args.scale)
+    n, L = reads.shape
+    t_gen = time.perf_counter() - t_gen
+    ctx = native.Context(min_ovlp=min_ovlp, profile=True)
+    d_reads = hipmem.DeviceBuffer.from_array(reads)
+    hipmem.synchronize()
+    stream = ctx.stream
+
+    def step():
+        ctx.reset()
+        ctx.add_reads_device(d_reads.address, n, L)
+        ctx.run(collect=False)
+
+    for _ in range(args.warmup):
+        step()
+    hipmem.synchronize()
+    ctx.kernel_clear()
+    ev0, ev1 = hipmem.Event(), hipmem.Event()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    ev1.synchronize()
+    hipmem.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    gpu_ms_region = ev0.elapsed_ms(ev1) / args.steps
+    kern = ctx.kernels()
+    summ = ctx.summary()
+    rounds = ctx.rounds()
+
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    kd = kern[dom]
+    avg_ms = kd["ms"] / max(kd["launches"], 1)
+    bytes_per_launch = kd["bytes"] / max(kd["launches"], 1)
+    achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9 if kd["ms"] > 0 else 0.0
+    value = n * L / (ms_per_step * 1e-3) / 1e9
+
+    same_full = None
+    if args.check:
+        import oracle_lib
+        rc, want, _, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
+        ctx.reset()
+        ctx.add_reads(reads)
+        same_full = rc == 0 and ctx.run() == want
+
+    cpu, same_sample = (None, None)
+    if not args.no_cpu:
+        cpu, same_sample = cpu_baseline(args.cpu_scale, min_ovlp, check_gpu=True)
+
+    line = {
+        "metric": "Gbases/s end-to-end overlap detection on 1 GPU",
+        "value": round(value, 4),
+        "unit": "Gbases/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "in-core: %d bp genome, %d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d" % (
+                int(5e7 * args.scale), n, L, min_ovlp),
+            "reads": n, "read_len": L, "min_ovlp": min_ovlp,
+            "dedup": summ["n_dedup"], "rounds": summ["n_rounds"], "edges": summ["n_edges"],
+            "records": summ["n_records"],
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_launch": round(bytes_per_launch),
+            "avg_launch_ms": round(avg_ms, 4),
+        },
+        "kernels": {k: {"launches": v["launches"] // max(args.steps, 1), "ms_per_step": round(v["ms"] / args.steps, 3),
+                        "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                    for k, v in kern.items()},
+        "gpu_event_ms_per_step": round(gpu_ms_region, 3),
+        "cpu_baseline": cpu,
+        "parity_sample_vs_oracle": same_sample,
+        "parity_full_vs_oracle": same_full,
+        "gen_s": round(t_gen, 2),
+    }
+    ctx.close()
+    d_reads.free()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

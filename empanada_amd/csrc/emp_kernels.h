@@ -27,6 +27,7 @@ struct MapArgs {
     const uint8_t *reads;    // n x L row-major, 4-byte aligned
     uint32_t n;              // reads in this batch
     int32_t L, Lpad;         // Lpad: LDS row stride (multiple of 4, odd number of dwords)
+    uint32_t rows;           // reads per work-group (256, or 128 when 256 rows exceed 64 KiB of LDS)
     uint64_t k0;             // global index of the first read (multiple of 16)
     uint32_t *const *chunks; // device table of column chunks
     Entry *entries;          // linear entries, index 2k and 2k+1

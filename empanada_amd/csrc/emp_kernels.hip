@@ -64,8 +64,8 @@ __device__ __forceinline__ void put_entry(Entry *__restrict__ out, uint32_t *__r
 // Map: one work-group of 256 threads per 256 reads.
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     extern __shared__ __align__(16) uint8_t tile[];
-    const uint32_t first = blockIdx.x * 256u;
-    const uint32_t nt = min(256u, a.n - first);
+    const uint32_t first = blockIdx.x * a.rows;
+    const uint32_t nt = min(a.rows, a.n - first);
     const int L = a.L, Lp = a.Lpad;
     const uint8_t *src = a.reads + (size_t)first * L;
     const uint32_t nbytes = nt * (uint32_t)L;
@@ -372,7 +372,8 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
                     if (mpk[mid] < k) lo = mid + 1; else hi = mid;
                 }
                 unsigned long long pos = base + msc[i];
-                for (uint32_t j = lo; j < nmp && mpk[j] == k; ++j) {
+                const unsigned long long end = base + (i + 1 < nms ? msc[i + 1] : tot);
+                for (uint32_t j = lo; j < nmp && mpk[j] == k && pos < end; ++j) {
                     const uint32_t prd = (mpr[j] << 1) | (mpr[j] >> 31);
                     if ((prd >> 1) != (srid >> 1)) {
                         a.pool_from[pos] = srid;
@@ -476,8 +477,8 @@ __global__ void __launch_bounds__(256) k_roll(const uint64_t *__restrict__ keys,
 // Launchers.
 hipError_t launch_map(const MapArgs &a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    uint32_t grid = (a.n + 255) / 256;
-    size_t lds = (size_t)256 * a.Lpad;
+    uint32_t grid = (a.n + a.rows - 1) / a.rows;
+    size_t lds = (size_t)a.rows * a.Lpad;
     hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }

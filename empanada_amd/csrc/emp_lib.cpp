@@ -314,6 +314,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
         a.n = m;
         a.L = c->L;
         a.Lpad = Lpad;
+        a.rows = 256u * (uint32_t)Lpad <= 65536u ? 256u : 128u;
         a.k0 = c->n + done;
         a.chunks = c->d_chunk_tab;
         a.entries = c->d_entries;

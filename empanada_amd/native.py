@@ -1,0 +1,240 @@
+"""ctypes bindings to libempanada_hip.so (the C ABI of include/empanada.h and
+include/empanada_stages.h). Loading fails loudly when the library is not built: there is no
+fallback path."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libempanada_hip.so")
+CLI_PATH = os.path.join(_HERE, "bin", "empanada")
+
+EMP_OK = 0
+ERRORS = {
+    -1: "EMP_ERR_ARG", -2: "EMP_ERR_HIP", -3: "EMP_ERR_FORMAT", -4: "EMP_ERR_READLEN", -5: "EMP_ERR_BASE",
+    -6: "EMP_ERR_OUT_OF_CORE", -7: "EMP_ERR_NOMEM", -8: "EMP_ERR_TOO_MANY", -9: "EMP_ERR_PARTITION",
+    -10: "EMP_ERR_IO", -11: "EMP_ERR_CAPACITY",
+}
+
+# Every symbol the headers in include/ declare.
+EXPORTS = [
+    "emp_opts_default", "emp_ctx_create", "emp_ctx_destroy", "emp_last_error", "emp_add_fastq", "emp_add_reads",
+    "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
+    "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_stream", "emp_version",
+    "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll",
+]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("min_ovlp", ctypes.c_int32), ("map_mem", ctypes.c_uint64), ("reduce_mem", ctypes.c_uint64),
+                ("base_blocksz", ctypes.c_uint64), ("kv_chunksz", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64),
+                ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("stream", ctypes.c_void_p)]
+
+
+class RoundStats(ctypes.Structure):
+    _fields_ = [("s", ctypes.c_uint64), ("p", ctypes.c_uint64), ("s_after", ctypes.c_uint64),
+                ("p_after", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("ovlp", ctypes.c_int32),
+                ("records", ctypes.c_int32), ("bucket_bits", ctypes.c_int32), ("big_buckets", ctypes.c_int32)]
+
+
+class Summary(ctypes.Structure):
+    _fields_ = [("n_reads", ctypes.c_uint64), ("read_len", ctypes.c_int32), ("n_rounds", ctypes.c_int32),
+                ("n_dedup", ctypes.c_uint64), ("n_records", ctypes.c_uint64), ("n_edges", ctypes.c_uint64),
+                ("map_ms", ctypes.c_double), ("overlap_ms", ctypes.c_double)]
+
+
+SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                        ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32))
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libempanada_hip.so is not built (run __graft_entry__.build()); "
+                               "the HIP path has no fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i32, cp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_char_p
+        L.emp_opts_default.argtypes = [ctypes.POINTER(Opts)]
+        L.emp_ctx_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.emp_ctx_destroy.argtypes = [vp]
+        L.emp_last_error.restype = cp
+        L.emp_last_error.argtypes = [vp]
+        L.emp_add_fastq.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+        L.emp_add_reads.argtypes = [vp, vp, u64, i32]
+        L.emp_add_reads_device.argtypes = [vp, vp, u64, i32]
+        L.emp_run.argtypes = [vp, SINK, vp]
+        L.emp_run_to_file.argtypes = [vp, cp]
+        L.emp_reset.argtypes = [vp]
+        L.emp_get_summary.argtypes = [vp, ctypes.POINTER(Summary)]
+        L.emp_get_round.argtypes = [vp, i32, ctypes.POINTER(RoundStats)]
+        L.emp_kernel_count.restype = i32
+        L.emp_kernel_count.argtypes = [vp]
+        L.emp_kernel_info.argtypes = [vp, i32, ctypes.POINTER(cp), ctypes.POINTER(u64),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.emp_kernel_clear.argtypes = [vp]
+        L.emp_stream.restype = vp
+        L.emp_stream.argtypes = [vp]
+        L.emp_version.restype = cp
+        L.emp_stage_fingerprints.argtypes = [i32, vp, u64, i32, vp]
+        L.emp_stage_columns.argtypes = [i32, vp, u64, i32, vp]
+        L.emp_stage_roll.argtypes = [i32, vp, vp, u64, i32, i32, vp]
+        _lib = L
+    return _lib
+
+
+class EmpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Context:
+    """One overlap job on one GPU (wraps emp_ctx)."""
+
+    def __init__(self, min_ovlp=31, map_mem=1 << 30, reduce_mem=1 << 30, hbm_budget=0, device=0, profile=False,
+                 stream=None):
+        o = Opts()
+        lib().emp_opts_default(ctypes.byref(o))
+        o.min_ovlp, o.map_mem, o.reduce_mem = min_ovlp, map_mem, reduce_mem
+        o.hbm_budget, o.device, o.profile = hbm_budget, device, 1 if profile else 0
+        o.stream = stream
+        self._h = ctypes.c_void_p()
+        rc = lib().emp_ctx_create(ctypes.byref(o), ctypes.byref(self._h))
+        if rc:
+            raise EmpError(rc, "emp_ctx_create failed")
+
+    def _check(self, rc):
+        if rc:
+            raise EmpError(rc, lib().emp_last_error(self._h).decode(errors="replace"))
+
+    def close(self):
+        if self._h:
+            lib().emp_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib().emp_stream(self._h)
+
+    def add_fastq(self, data: bytes, block: int = None):
+        """Feeds FASTQ bytes through the block reader (block = map_mem / 16 by default)."""
+        if block is None:
+            block = 64 << 20
+        off = 0
+        buf = (ctypes.c_char * len(data)).from_buffer_copy(data) if data else None
+        base = ctypes.addressof(buf) if buf is not None else 0
+        while off < len(data):
+            got = min(block, len(data) - off)
+            final = 1 if got < block or off + got == len(data) else 0
+            used = ctypes.c_size_t()
+            self._check(lib().emp_add_fastq(self._h, base + off, got, final, ctypes.byref(used)))
+            if final:
+                break
+            off += used.value
+
+    def add_reads(self, reads: np.ndarray):
+        r = np.ascontiguousarray(reads, dtype=np.uint8)
+        n, L = r.shape
+        self._check(lib().emp_add_reads(self._h, r.ctypes.data, n, L))
+
+    def add_reads_device(self, ptr: int, n: int, L: int):
+        self._check(lib().emp_add_reads_device(self._h, ptr, n, L))
+
+    def run(self, collect=True):
+        """Runs the overlap phase. Returns the output.bin bytes (collect=True) or None."""
+        chunks = []
+
+        @SINK
+        def sink(user, n, ovlp, fr, to):
+            if collect:
+                chunks.append(np.array([n], dtype=np.uint64).tobytes())
+                chunks.append(np.array([ovlp], dtype=np.int32).tobytes())
+                if n:
+                    chunks.append(np.ctypeslib.as_array(fr, shape=(n,)).tobytes())
+                    chunks.append(np.ctypeslib.as_array(to, shape=(n,)).tobytes())
+            return 0
+
+        self._check(lib().emp_run(self._h, sink, None))
+        return b"".join(chunks) if collect else None
+
+    def run_to_file(self, path: str):
+        self._check(lib().emp_run_to_file(self._h, path.encode()))
+
+    def reset(self):
+        self._check(lib().emp_reset(self._h))
+
+    def summary(self):
+        s = Summary()
+        self._check(lib().emp_get_summary(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Summary._fields_}
+
+    def rounds(self):
+        out = []
+        i = 0
+        while True:
+            r = RoundStats()
+            if lib().emp_get_round(self._h, i, ctypes.byref(r)):
+                break
+            out.append({f: getattr(r, f) for f, _ in RoundStats._fields_})
+            i += 1
+        return out
+
+    def kernels(self):
+        out = {}
+        for i in range(lib().emp_kernel_count(self._h)):
+            name, launches = ctypes.c_char_p(), ctypes.c_uint64()
+            ms, by = ctypes.c_double(), ctypes.c_double()
+            lib().emp_kernel_info(self._h, i, ctypes.byref(name), ctypes.byref(launches), ctypes.byref(ms),
+                                  ctypes.byref(by))
+            out[name.value.decode()] = dict(launches=launches.value, ms=ms.value, bytes=by.value)
+        return out
+
+    def kernel_clear(self):
+        lib().emp_kernel_clear(self._h)
+
+
+def stage_fingerprints(reads: np.ndarray, device=0) -> np.ndarray:
+    r = np.ascontiguousarray(reads, dtype=np.uint8)
+    n, L = r.shape
+    out = np.empty(2 * n, dtype=np.uint64)
+    rc = lib().emp_stage_fingerprints(device, r.ctypes.data, n, L, out.ctypes.data)
+    if rc:
+        raise EmpError(rc, "emp_stage_fingerprints")
+    return out
+
+
+def stage_columns(reads: np.ndarray, device=0) -> np.ndarray:
+    r = np.ascontiguousarray(reads, dtype=np.uint8)
+    n, L = r.shape
+    W = (n + 15) // 16
+    out = np.empty((L, W), dtype=np.uint32)
+    rc = lib().emp_stage_columns(device, r.ctypes.data, n, L, out.ctypes.data)
+    if rc:
+        raise EmpError(rc, "emp_stage_columns")
+    return out
+
+
+def stage_roll(keys: np.ndarray, bases: np.ndarray, kind: int, length: int, device=0) -> np.ndarray:
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    b = np.ascontiguousarray(bases, dtype=np.uint8)
+    out = np.empty_like(k)
+    rc = lib().emp_stage_roll(device, k.ctypes.data, b.ctypes.data, k.size, kind, length, out.ctypes.data)
+    if rc:
+        raise EmpError(rc, "emp_stage_roll")
+    return out
