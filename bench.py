@@ -59,7 +59,17 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="also compare the full-size output with the oracle")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="only run three device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        from empanada_amd import hipmem
+        a, b = hipmem.DeviceBuffer(args.copy_calibrate), hipmem.DeviceBuffer(args.copy_calibrate)
+        for _ in range(3):
+            hipmem.copy_d2d(b, a, args.copy_calibrate)
+        hipmem.synchronize()
+        print(json.dumps({"copy_calibrate_bytes": args.copy_calibrate, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this path is measured on a single MI355X (--gpus 1)",
               file=sys.stderr)

@@ -106,5 +106,9 @@ class Event:
         return ms.value
 
 
+def copy_d2d(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
+    _check(lib().hipMemcpy(dst.ptr, src.ptr, nbytes, 3), "hipMemcpy D2D")
+
+
 def synchronize():
     _check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
