@@ -14,6 +14,15 @@ constexpr int MATCH_CAP = 512;
 // Target mean bucket load: the partition fan-out is chosen so that lists average this many
 // entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
 constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
+// Entries per work-group tile of a partition pass, and the widest digit of one pass.
+constexpr int PART_TILE = 4096;
+constexpr int PART_MAX_BITS = 8;
+
+// An entry that did not fit its final bucket, with the bucket it belongs to.
+struct SpillRec {
+    Entry e;
+    uint32_t bucket;
+};
 
 // Round description for regeneration: keys become those of length l-1 for round t.
 struct RoundArgs {
@@ -34,13 +43,23 @@ struct MapArgs {
     uint32_t *err;           // set to 1 when a byte is not ACGTacgt
 };
 
-struct ScatterArgs {
+// One partition pass: input groups (keys sharing their top bits) -> output buckets refined by
+// `dbits` more key bits. Output bucket id = h1 >> out_shift.
+struct PartArgs {
     const Entry *in;
-    uint64_t n;
+    const uint32_t *in_cnt;  // per-group counts; null for one linear group of *in_n entries
+    const uint32_t *in_n;
+    uint32_t in_cap;         // group stride in entries (ignored for a linear group)
+    uint32_t tiles_x;        // work-groups per input group
     Entry *out;
     uint32_t *out_cnt;
-    uint32_t cap;
-    int32_t shift;
+    uint32_t out_cap;
+    int32_t out_shift;
+    int32_t dbits;
+    SpillRec *spill;         // final pass: overflowing entries go here; null for intermediate passes
+    uint32_t *spill_n;
+    uint32_t spill_cap;
+    uint32_t *abort;         // set when an intermediate bucket or the spill area overflows
 };
 
 struct DedupArgs {
@@ -48,24 +67,20 @@ struct DedupArgs {
     const uint32_t *cnt;
     const uint32_t *const *chunks;
     RoundArgs r0;
-    Entry *s_out, *p_out;
-    uint32_t *s_cnt, *p_cnt;
-    uint32_t cap;
-    int32_t out_shift;
+    Entry *s_lin, *p_lin;    // survivors with round-0 keys, same positions in both lists
+    uint32_t *lin_n;
     uint32_t *big, *n_big;
     uint32_t big_cap;
-    unsigned long long *n_kept;
+    const uint32_t *abort;
 };
 
 struct JoinArgs {
     const Entry *s_in, *p_in;
     const uint32_t *s_cnt, *p_cnt;
     const uint32_t *const *chunks;
-    RoundArgs rn;  // regeneration for the next round
-    Entry *s_out, *p_out;
-    uint32_t *s_out_cnt, *p_out_cnt;
-    uint32_t cap;
-    int32_t out_shift;
+    RoundArgs rn;            // regeneration for the next round
+    Entry *s_lin, *p_lin;    // survivors, linear
+    uint32_t *s_lin_n, *p_lin_n;
     uint32_t *pool_from, *pool_to;
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
@@ -73,7 +88,33 @@ struct JoinArgs {
     uint32_t *edge_cnt;
     uint32_t *big, *n_big;
     uint32_t big_cap;
-    unsigned long long *surv_s, *surv_p;
+    const uint32_t *abort;
+};
+
+// Large buckets (more than BUCKET_CAP entries, or more matched records than MATCH_CAP): one
+// work-group per bucket sorts the bucket in global scratch memory and joins by binary search.
+struct BigArgs {
+    const uint32_t *ids;           // bucket ids
+    const unsigned long long *soff;// scratch offset (records) of each bucket's suffix side
+    const unsigned long long *poff;// and prefix side (dedup: unused)
+    uint64_t *skey, *pkey;         // scratch keys
+    uint32_t *srank, *prank;       // scratch ranks (join) or read ids (dedup)
+    uint32_t *saux, *paux;         // per-record counts / flags
+    const Entry *s_in, *p_in;
+    const uint32_t *s_cnt, *p_cnt;
+    const SpillRec *s_spill, *p_spill;
+    const uint32_t *s_spill_n, *p_spill_n;
+    uint32_t spill_cap;
+    uint32_t cap;
+    const uint32_t *const *chunks;
+    RoundArgs rn;
+    Entry *s_lin, *p_lin;
+    uint32_t *s_lin_n, *p_lin_n;
+    uint32_t *pool_from, *pool_to;
+    unsigned long long *pool_cursor;
+    unsigned long long pool_cap;
+    unsigned long long *edge_off;
+    uint32_t *edge_cnt;
 };
 
 struct EdgeCopyArgs {
@@ -86,10 +127,13 @@ struct EdgeCopyArgs {
 };
 
 hipError_t launch_map(const MapArgs &a, hipStream_t s);
-hipError_t launch_scatter(const ScatterArgs &a, hipStream_t s);
+hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s);
 hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s);
 hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s);
+hipError_t launch_dedup_big(const BigArgs &a, uint32_t n, hipStream_t s);
+hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,
                        uint64_t *out, hipStream_t s);
 

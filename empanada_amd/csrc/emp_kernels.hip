@@ -3,20 +3,23 @@
 // Data in HBM:
 //   reads    row-major ASCII kept reads (input of the map kernel only)
 //   columns  2-bit codes per read position, chunked by 2^20 reads (emp_device.h)
-//   entries  12-byte (h1, h2, rid) records, first linear (map output), then bucketed:
-//            bucket b of a list holds the entries whose key has top bits b (h1 >> (29 - bits)),
-//            in a fixed-capacity slot array [bucket][CAP] plus a count per bucket. Order inside
-//            a bucket is arbitrary: every order the output depends on is re-established in LDS.
+//   entries  12-byte (h1, h2, rid) records. A list is linear after the kernel that produces it
+//            (map, dedup, join), then one or more partition passes group it by the top bits of
+//            h1 into buckets: bucket b of a pass with `bits` output bits holds the entries with
+//            h1 >> (29 - bits) == b, in a fixed-capacity slot array [bucket][cap] plus a count.
+//            Order inside a bucket is arbitrary: every order the output depends on is
+//            re-established in LDS.
 //
 // Kernels:
 //   k_map      transposition + forward/reverse-complement Rabin-Karp (fingerprint.cu:75-153)
-//   k_scatter  linear entries -> buckets
+//   k_part     tile-sorted partition pass (at most 8 more key bits per pass)
 //   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222), then
-//              regenerate for round 0 and scatter into the suffix and prefix lists
+//              regenerate for round 0 as suffix and prefix entries (linear output)
 //   k_join     per bucket of round t: LDS hash join of suffixes against prefixes, edges of the
 //              matched key groups in (key, rid&1, rid) x (rid&1, rid) order, removal of used
-//              entries, regeneration of the survivors for round t+1 and scatter into its buckets
+//              entries, regeneration of the survivors for round t+1 (linear output)
 //              (overlap.cu:258-290, 361-456, 557-710)
+//   k_*_big    the same for buckets over capacity: global-memory sort + binary-search join
 //   k_edge_*   bucket-ordered compaction of the edges of a round
 #include "emp_kernels.h"
 
@@ -51,13 +54,6 @@ __device__ __forceinline__ void regen(const uint32_t *const *__restrict__ chunks
         next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
     else
         prev_pfx(e.h1, e.h2, ev);
-}
-
-__device__ __forceinline__ void put_entry(Entry *__restrict__ out, uint32_t *__restrict__ cnt, uint32_t cap, int shift,
-                                          const Entry &e) {
-    uint32_t b = e.h1 >> shift;
-    uint32_t slot = atomicAdd(cnt + b, 1u);
-    if (slot < cap) out[(size_t)b * cap + slot] = e;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -131,16 +127,10 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_scatter(ScatterArgs a) {
-    uint64_t i = blockIdx.x * 256ull + threadIdx.x;
-    if (i >= a.n) return;
-    Entry e = a.in[i];
-    put_entry(a.out, a.out_cnt, a.cap, a.shift, e);
-}
-
-// ---------------------------------------------------------------------------------------------
-// Block helpers (256 threads = 4 waves of 64).
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* >= 4 */, uint32_t &total) {
+// Block helpers.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* >= NT/64 */, uint32_t &total) {
+    constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -150,19 +140,24 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* 
     }
     if (lane == 63) sh[wv] = x;
     __syncthreads();
-    uint32_t base = 0;
-    for (int i = 0; i < wv; ++i) base += sh[i];
-    total = sh[0] + sh[1] + sh[2] + sh[3];
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        uint32_t s = sh[i];
+        base += i < wv ? s : 0u;
+        tot += s;
+    }
+    total = tot;
     __syncthreads();
     return base + x - v;
 }
 
-// Bitonic sort of n2 (power of two) records in LDS by (key, rank), ascending; `pay` (may be
-// null) is permuted along.
-__device__ void bitonic_lds(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n2) {
+// Bitonic sort of n2 (power of two) records by (key, rank), ascending; `pay` (may be null) is
+// permuted along. Works on LDS or on global memory private to the work-group.
+__device__ void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n2) {
     for (uint32_t k = 2; k <= n2; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n2; i += 256) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
                 uint32_t ixj = i ^ j;
                 if (ixj > i) {
                     uint64_t ka = key[i], kb = key[ixj];
@@ -181,26 +176,104 @@ __device__ void bitonic_lds(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32
     }
 }
 
+__device__ __forceinline__ uint64_t key_of(const Entry &e) { return ((uint64_t)e.h1 << 32) | e.h2; }
+
+// ---------------------------------------------------------------------------------------------
+// Partition pass: each work-group takes tiles of PART_TILE entries of one input group, ranks them
+// by their next `dbits` key bits in LDS, reserves one output range per (tile, bucket) with a
+// single atomic, and writes each bucket's run contiguously.
+__global__ void __launch_bounds__(256) k_part(PartArgs a) {
+    constexpr int T = PART_TILE, PER = T / 256;
+    __shared__ Entry buf[T];
+    __shared__ uint8_t dig[T];
+    __shared__ uint32_t hist[256], loc[256], base[256], sh[4];
+    const uint32_t g = blockIdx.x / a.tiles_x;
+    const uint32_t F = 1u << a.dbits;
+    uint32_t n_g;
+    const Entry *in;
+    if (a.in_cnt) {
+        n_g = min(a.in_cnt[g], a.in_cap);
+        in = a.in + (size_t)g * a.in_cap;
+    } else {
+        n_g = *a.in_n;
+        in = a.in;
+    }
+    for (uint32_t tile = blockIdx.x % a.tiles_x; (size_t)tile * T < n_g; tile += a.tiles_x) {
+        const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        Entry e[PER];
+        uint32_t r[PER], d[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            uint32_t i = threadIdx.x + 256 * q;
+            if (i < nt) {
+                e[q] = in[t0 + i];
+                d[q] = (e[q].h1 >> a.out_shift) & (F - 1);
+                r[q] = atomicAdd(&hist[d[q]], 1u);
+            }
+        }
+        __syncthreads();
+        const uint32_t h = hist[threadIdx.x];
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<256>(h, sh, tot);
+        loc[threadIdx.x] = ex;
+        base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[(g << a.dbits) + threadIdx.x], h) : 0u;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            uint32_t i = threadIdx.x + 256 * q;
+            if (i < nt) {
+                uint32_t p = loc[d[q]] + r[q];
+                buf[p] = e[q];
+                dig[p] = (uint8_t)d[q];
+            }
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < nt; p += 256) {
+            const uint32_t dd = dig[p];
+            const uint32_t slot = base[dd] + (p - loc[dd]);
+            const uint32_t bucket = (g << a.dbits) + dd;
+            if (slot < a.out_cap) {
+                a.out[(size_t)bucket * a.out_cap + slot] = buf[p];
+            } else if (a.spill) {
+                uint32_t s = atomicAdd(a.spill_n, 1u);
+                if (s < a.spill_cap) a.spill[s] = SpillRec{buf[p], bucket};
+                else atomicOr(a.abort, 1u);
+            } else {
+                atomicOr(a.abort, 1u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Dedup: per bucket keep the entry with the smallest id of each key (stable sort by key of an
-// id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306).
+// id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306), then
+// regenerate it for round 0 as a suffix and as a prefix (overlap.cu:266-267).
 template <int CAP>
 __global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    constexpr int PER = CAP / 256;
     __shared__ uint32_t h1s[CAP], h2s[CAP], rids[CAP];
     __shared__ uint32_t head[CAP];
     __shared__ uint16_t nxt[CAP];
-    __shared__ uint32_t s_kept;
+    __shared__ uint32_t sh[4];
+    __shared__ uint32_t s_base;
+    if (*a.abort) return;
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.cnt[b];
     if (n > (uint32_t)CAP) {
-        if (threadIdx.x == 0) { uint32_t i = atomicAdd(a.n_big, 1u); if (i < a.big_cap) a.big[i] = b; }
+        if (threadIdx.x == 0) {
+            uint32_t i = atomicAdd(a.n_big, 1u);
+            if (i < a.big_cap) a.big[i] = b;
+        }
         return;
     }
     if (n == 0) return;
     const Entry *in = a.in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
-    if (threadIdx.x == 0) s_kept = 0;
     for (uint32_t i = threadIdx.x; i < n; i += 256) {
         Entry e = in[i];
         h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
@@ -208,27 +281,37 @@ __global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += 256) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
     __syncthreads();
-    uint32_t kept = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += 256) {
-        uint32_t h1 = h1s[i], h2 = h2s[i], rid = rids[i];
-        bool keep = true;
-        for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY && keep; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j])) {
-            if (h1s[j] == h1 && h2s[j] == h2 && rids[j] < rid) keep = false;
+    uint32_t keepmask = 0, kept = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        uint32_t i = threadIdx.x + 256 * q;
+        if (i < n) {
+            uint32_t h1 = h1s[i], h2 = h2s[i], rid = rids[i];
+            bool keep = true;
+            for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY && keep; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j]))
+                if (h2s[j] == h2 && h1s[j] == h1 && rids[j] < rid) keep = false;
+            if (keep) { keepmask |= 1u << q; ++kept; }
         }
-        if (keep) {
-            ++kept;
-            if (a.r0.active) {
-                Entry s = {h1, h2, rid}, p = {h1, h2, rid};
+    }
+    uint32_t tot;
+    const uint32_t off = block_excl_scan<256>(kept, sh, tot);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(a.lin_n, tot) : 0u;
+    __syncthreads();
+    if (a.r0.active) {
+        uint32_t pos = s_base + off;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (keepmask & (1u << q)) {
+                uint32_t i = threadIdx.x + 256 * q;
+                Entry s = {h1s[i], h2s[i], rids[i]}, p = s;
                 regen(a.chunks, a.r0, s, true);
                 regen(a.chunks, a.r0, p, false);
-                put_entry(a.s_out, a.s_cnt, a.cap, a.out_shift, s);
-                put_entry(a.p_out, a.p_cnt, a.cap, a.out_shift, p);
+                a.s_lin[pos] = s;
+                a.p_lin[pos] = p;
+                ++pos;
             }
         }
     }
-    if (kept) atomicAdd(&s_kept, kept);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_kept) atomicAdd(a.n_kept, (unsigned long long)s_kept);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -244,11 +327,11 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
     __shared__ uint64_t msk[MCAP], mpk[MCAP];
     __shared__ uint32_t msr[MCAP], mpr[MCAP];
     __shared__ uint32_t msc[MCAP];
-    __shared__ uint32_t sh[8];
-    __shared__ uint32_t s_ms, s_mp, s_ovf;
+    __shared__ uint32_t sh[4];
+    __shared__ uint32_t s_ms, s_mp, s_ovf, s_bs, s_bp;
     __shared__ unsigned long long s_base;
-    __shared__ uint32_t s_surv_s, s_surv_p;
 
+    if (*a.abort) return;
     const uint32_t b = blockIdx.x;
     const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
     if (ns > (uint32_t)CAP || np > (uint32_t)CAP) {
@@ -259,7 +342,7 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
         }
         return;
     }
-    if (threadIdx.x == 0) { s_ms = 0; s_mp = 0; s_ovf = 0; s_surv_s = 0; s_surv_p = 0; }
+    if (threadIdx.x == 0) { s_ms = 0; s_mp = 0; s_ovf = 0; }
     const Entry *sin = a.s_in + (size_t)b * CAP;
     const Entry *pin = a.p_in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
@@ -332,7 +415,6 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
 
     // Edges of the bucket: sort matched suffixes and prefixes by (key, rank), then emit the
     // cross product of each key group skipping same-read pairs (overlap.cu:557-601).
-    uint32_t nedge = 0;
     if (nms) {
         uint32_t n2 = 1;
         while (n2 < nms) n2 <<= 1;
@@ -341,16 +423,13 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
         while (m2 < nmp) m2 <<= 1;
         for (uint32_t i = nmp + threadIdx.x; i < m2; i += 256) { mpk[i] = ~0ull; mpr[i] = ~0u; }
         __syncthreads();
-        bitonic_lds(msk, msr, msc, n2);
-        bitonic_lds(mpk, mpr, nullptr, m2);
-        // exclusive scan of per-suffix edge counts (nms <= MCAP = 2 * 256)
+        bitonic(msk, msr, msc, n2);
+        bitonic(mpk, mpr, nullptr, m2);
         static_assert(MCAP <= 512, "scan covers two records per thread");
         uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
         uint32_t c0 = i0 < nms ? msc[i0] : 0, c1 = i1 < nms ? msc[i1] : 0;
         uint32_t tot;
-        uint32_t off = block_excl_scan(c0 + c1, sh, tot);
-        nedge = tot;
-        __syncthreads();
+        uint32_t off = block_excl_scan<256>(c0 + c1, sh, tot);
         if (i0 < nms) msc[i0] = off;
         if (i1 < nms) msc[i1] = off + c0;
         if (threadIdx.x == 0) {
@@ -365,7 +444,6 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
             for (uint32_t i = threadIdx.x; i < nms; i += 256) {
                 const uint64_t k = msk[i];
                 const uint32_t srid = (msr[i] << 1) | (msr[i] >> 31);
-                // lower bound of k among the matched prefixes
                 uint32_t lo = 0, hi = nmp;
                 while (lo < hi) {
                     uint32_t mid = (lo + hi) >> 1;
@@ -388,38 +466,222 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
         a.edge_off[b] = 0;
     }
 
-    // Survivors: suffixes with no edge, prefixes never a target (overlap.cu:424-429,694-710).
-    uint32_t ks = 0, kp = 0;
+    // Survivors: suffixes with no edge, prefixes never a target (overlap.cu:424-429,694-710),
+    // regenerated for the next round and appended to the linear lists (one reservation per list).
+    uint32_t smask = 0, ks = 0, pmask = 0, kp = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         uint32_t i = threadIdx.x + 256 * q;
-        if (i < ns && sc[q] == 0) {
-            ++ks;
-            if (a.rn.active) {
+        if (i < ns && sc[q] == 0) { smask |= 1u << q; ++ks; }
+        if (i < np && !used[i]) { pmask |= 1u << q; ++kp; }
+    }
+    uint32_t ts, tp;
+    const uint32_t os = block_excl_scan<256>(ks, sh, ts);
+    const uint32_t op = block_excl_scan<256>(kp, sh, tp);
+    if (threadIdx.x == 0) {
+        s_bs = ts ? atomicAdd(a.s_lin_n, ts) : 0u;
+        s_bp = tp ? atomicAdd(a.p_lin_n, tp) : 0u;
+    }
+    __syncthreads();
+    if (a.rn.active) {
+        uint32_t pos = s_bs + os;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (smask & (1u << q)) {
                 Entry e = se[q];
                 regen(a.chunks, a.rn, e, true);
-                put_entry(a.s_out, a.s_out_cnt, a.cap, a.out_shift, e);
+                a.s_lin[pos++] = e;
             }
         }
-    }
-    for (uint32_t i = threadIdx.x; i < np; i += 256) {
-        if (!used[i]) {
-            ++kp;
-            if (a.rn.active) {
+        pos = s_bp + op;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (pmask & (1u << q)) {
+                uint32_t i = threadIdx.x + 256 * q;
                 Entry e = {pk1[i], pk2[i], prid[i]};
                 regen(a.chunks, a.rn, e, false);
-                put_entry(a.p_out, a.p_out_cnt, a.cap, a.out_shift, e);
+                a.p_lin[pos++] = e;
             }
         }
     }
-    if (ks) atomicAdd(&s_surv_s, ks);
-    if (kp) atomicAdd(&s_surv_p, kp);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_surv_s) atomicAdd(a.surv_s, (unsigned long long)s_surv_s);
-        if (s_surv_p) atomicAdd(a.surv_p, (unsigned long long)s_surv_p);
-        (void)nedge;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Large buckets: global-memory sort + binary-search join, one work-group of 1024 per bucket.
+__device__ __forceinline__ uint32_t lower_u64(const uint64_t *k, uint32_t lo, uint32_t hi, uint64_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (k[m] < x) lo = m + 1; else hi = m;
     }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t upper_u64(const uint64_t *k, uint32_t lo, uint32_t hi, uint64_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (x < k[m]) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool has_u32(const uint32_t *r, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (r[m] < x) lo = m + 1; else hi = m;
+    }
+    return lo < hi && r[lo] == x;
+}
+
+// Collects the entries of bucket b (slots + spilled records) into key/value scratch, padded with
+// sentinels to n2. as_rank: store rid_rank(rid) (join) or rid (dedup).
+__device__ void gather_bucket(const Entry *slots, uint32_t cnt, uint32_t cap, const SpillRec *spill,
+                              const uint32_t *spill_n, uint32_t spill_cap, uint32_t b, uint64_t *key, uint32_t *val,
+                              uint32_t n2, bool as_rank, uint32_t *s_pos) {
+    const uint32_t nm = min(cnt, cap);
+    for (uint32_t i = threadIdx.x; i < nm; i += blockDim.x) {
+        Entry e = slots[(size_t)b * cap + i];
+        key[i] = key_of(e);
+        val[i] = as_rank ? rid_rank(e.rid) : e.rid;
+    }
+    if (threadIdx.x == 0) *s_pos = nm;
+    __syncthreads();
+    if (cnt > cap) {
+        const uint32_t sn = min(*spill_n, spill_cap);
+        for (uint32_t j = threadIdx.x; j < sn; j += blockDim.x) {
+            if (spill[j].bucket == b) {
+                uint32_t k = atomicAdd(s_pos, 1u);
+                if (k < cnt) {
+                    Entry e = spill[j].e;
+                    key[k] = key_of(e);
+                    val[k] = as_rank ? rid_rank(e.rid) : e.rid;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = cnt + threadIdx.x; i < n2; i += blockDim.x) { key[i] = ~0ull; val[i] = ~0u; }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n) {
+    uint32_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+__global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
+    __shared__ uint32_t sh[16];
+    __shared__ uint32_t s_pos;
+    __shared__ unsigned long long s_base;
+    const uint32_t b = a.ids[blockIdx.x];
+    const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+    const uint32_t ns2 = pow2_at_least(ns), np2 = pow2_at_least(np);
+    uint64_t *SK = a.skey + a.soff[blockIdx.x], *PK = a.pkey + a.poff[blockIdx.x];
+    uint32_t *SR = a.srank + a.soff[blockIdx.x], *PR = a.prank + a.poff[blockIdx.x];
+    uint32_t *SA = a.saux + a.soff[blockIdx.x], *PA = a.paux + a.poff[blockIdx.x];
+    gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos);
+    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.spill_cap, b, PK, PR, np2, true, &s_pos);
+    bitonic(SK, SR, nullptr, ns2);
+    bitonic(PK, PR, nullptr, np2);
+    // valid partner counts: the key group on the other side minus the same-read members
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint64_t k = SK[i];
+        const uint32_t lo = lower_u64(PK, 0, np, k), hi = upper_u64(PK, lo, np, k);
+        const uint32_t rd = SR[i] & 0x7FFFFFFFu;  // read index of the id
+        uint32_t same = has_u32(PR, lo, hi, rd) + has_u32(PR, lo, hi, rd | 0x80000000u);
+        SA[i] = (hi - lo) - same;
+    }
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+        const uint64_t k = PK[j];
+        const uint32_t lo = lower_u64(SK, 0, ns, k), hi = upper_u64(SK, lo, ns, k);
+        const uint32_t rd = PR[j] & 0x7FFFFFFFu;
+        uint32_t same = has_u32(SR, lo, hi, rd) + has_u32(SR, lo, hi, rd | 0x80000000u);
+        PA[j] = ((hi - lo) - same) ? 1u : 0u;
+    }
+    __syncthreads();
+    // exclusive scan of SA in place: contiguous chunk per thread
+    const uint32_t per = (ns + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = min(ns, threadIdx.x * per), hi = min(ns, lo + per);
+    uint32_t run = 0;
+    for (uint32_t i = lo; i < hi; ++i) run += SA[i];
+    uint32_t tot;
+    uint32_t off = block_excl_scan<1024>(run, sh, tot);
+    for (uint32_t i = lo; i < hi; ++i) {
+        uint32_t c = SA[i];
+        SA[i] = off;
+        off += c;
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long base = atomicAdd(a.pool_cursor, (unsigned long long)tot);
+        s_base = base;
+        a.edge_off[b] = base;
+        a.edge_cnt[b] = tot;
+    }
+    __syncthreads();
+    const unsigned long long base = s_base;
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint32_t o = SA[i], e = (i + 1 < ns) ? SA[i + 1] : tot;
+        const uint32_t srid = (SR[i] << 1) | (SR[i] >> 31);
+        if (e > o && base + tot <= a.pool_cap) {
+            const uint64_t k = SK[i];
+            const uint32_t plo = lower_u64(PK, 0, np, k);
+            unsigned long long pos = base + o;
+            for (uint32_t j = plo; j < np && PK[j] == k && pos < base + e; ++j) {
+                const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
+                if ((prd >> 1) != (srid >> 1)) {
+                    a.pool_from[pos] = srid;
+                    a.pool_to[pos] = prd;
+                    ++pos;
+                }
+            }
+        }
+        if (e == o) {
+            Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
+            uint32_t pos = atomicAdd(a.s_lin_n, 1u);
+            if (a.rn.active) {
+                regen(a.chunks, a.rn, en, true);
+                a.s_lin[pos] = en;
+            }
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+        if (!PA[j]) {
+            const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
+            Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
+            uint32_t pos = atomicAdd(a.p_lin_n, 1u);
+            if (a.rn.active) {
+                regen(a.chunks, a.rn, en, false);
+                a.p_lin[pos] = en;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
+    __shared__ uint32_t s_pos;
+    const uint32_t b = a.ids[blockIdx.x];
+    const uint32_t n = a.s_cnt[b], n2 = pow2_at_least(n);
+    uint64_t *K = a.skey + a.soff[blockIdx.x];
+    uint32_t *R = a.srank + a.soff[blockIdx.x];
+    gather_bucket(a.s_in, n, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
+    bitonic(K, R, nullptr, n2);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (i == 0 || K[i] != K[i - 1]) {
+            Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
+            uint32_t pos = atomicAdd(a.s_lin_n, 1u);
+            if (a.rn.active) {
+                Entry p = e;
+                regen(a.chunks, a.rn, e, true);
+                regen(a.chunks, a.rn, p, false);
+                a.s_lin[pos] = e;
+                a.p_lin[pos] = p;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -483,9 +745,10 @@ hipError_t launch_map(const MapArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const ScatterArgs &a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((a.n + 255) / 256)), dim3(256), 0, s, a);
+hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {
+    uint64_t grid = (uint64_t)groups * a.tiles_x;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_part, dim3((uint32_t)grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -499,11 +762,30 @@ hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_dedup_big(const BigArgs &a, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dedup_big, dim3(n), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_big, dim3(n), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(1024), 0, s, a.cnt, a.nb, dst_off);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_edge_copy, dim3((a.nb + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)(g < 4096 ? g : 4096)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 

@@ -29,10 +29,10 @@ constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large
 
 struct DevCounters {
     unsigned long long pool_cursor;
-    unsigned long long surv_s, surv_p;
-    unsigned long long kept;
-    uint32_t n_big;
-    uint32_t err;
+    uint32_t n_big, abort;
+    uint32_t lin[2];    // linear list sizes written by dedup / join (suffix, prefix)
+    uint32_t spill[2];  // records spilled by the final partition pass
+    uint32_t in[2];     // partition input sizes, set by the host
 };
 
 double now_ms() {
@@ -101,16 +101,29 @@ struct emp_ctx {
     int stage_cur = 0;
     size_t stage_bytes = 0;
 
-    // overlap phase
-    Entry *d_list[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t *d_cnt[4] = {nullptr, nullptr, nullptr, nullptr};
+    // overlap phase, per list side (0 = suffixes, 1 = prefixes)
+    Entry *d_lin[2] = {nullptr, nullptr};  // linear lists; side 0 is the entries buffer
+    uint64_t lin1_cap = 0;
+    Entry *d_inter[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t *d_inter_cnt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint64_t inter_cap = 0;  // entries per intermediate array
+    Entry *d_fine[2] = {nullptr, nullptr};
+    uint32_t *d_fine_cnt[2] = {nullptr, nullptr};
     uint32_t nb_alloc = 0;
+    SpillRec *d_spill[2] = {nullptr, nullptr};
+    uint32_t spill_cap = 0;
+    double slack = 1.125;
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
     uint64_t pool_cap = 0;
     unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr;
     uint32_t *d_edge_cnt = nullptr;
+    // large buckets
     uint32_t *d_big = nullptr;
+    unsigned long long *d_boff[2] = {nullptr, nullptr};
+    uint64_t *d_bkey[2] = {nullptr, nullptr};
+    uint32_t *d_brank[2] = {nullptr, nullptr}, *d_baux[2] = {nullptr, nullptr};
+    uint64_t bscratch_cap = 0;
     DevCounters *d_ctr = nullptr, *h_ctr = nullptr;
 
     // statistics
@@ -371,38 +384,180 @@ int stage_read(emp_ctx *c, const char *seq, size_t len) {
 
 int read_cb(void *user, const char *seq, size_t len) { return stage_read((emp_ctx *)user, seq, len); }
 
-int ensure_lists(emp_ctx *c, uint32_t nb) {
-    if (nb <= c->nb_alloc) return EMP_OK;
-    for (int i = 0; i < 4; ++i) {
-        dfree(c, c->d_list[i]);
-        dfree(c, c->d_cnt[i]);
-        c->d_list[i] = nullptr;
-        c->d_cnt[i] = nullptr;
+// Partition plan of a list of x entries into 2^bb final buckets: passes of at most
+// PART_MAX_BITS bits; intermediate buckets get capacity x / 2^bits * slack + 1024.
+struct Level {
+    int bits_in, dbits;
+    uint32_t cap;
+};
+
+std::vector<Level> plan_levels(uint64_t x, int bb, double slack) {
+    std::vector<Level> v;
+    int done = 0;
+    do {
+        int d = std::min(PART_MAX_BITS, bb - done);
+        Level l{done, d, 0};
+        done += d;
+        if (done == bb) {
+            l.cap = BUCKET_CAP;
+        } else {
+            uint64_t per = (uint64_t)((double)x / (double)(1ull << done) * slack) + 1024;
+            l.cap = (uint32_t)((per + 63) & ~63ull);
+        }
+        v.push_back(l);
+    } while (done < bb);
+    return v;
+}
+
+uint64_t inter_need(uint64_t x, int bb, double slack) {
+    uint64_t need = 0;
+    for (const Level &l : plan_levels(x, bb, slack))
+        if (l.cap != (uint32_t)BUCKET_CAP || l.bits_in + l.dbits != bb)
+            need = std::max<uint64_t>(need, (1ull << (l.bits_in + l.dbits)) * l.cap);
+    return need;
+}
+
+int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
+    dfree(c, *p);
+    *p = nullptr;
+    void *q = nullptr;
+    int rc = dalloc(c, &q, n * sizeof(Entry));
+    if (rc) return rc;
+    *p = (Entry *)q;
+    return EMP_OK;
+}
+
+// Buffers of the overlap phase for lists of up to x entries partitioned into 2^bb buckets.
+int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
+    int rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (x > c->lin1_cap) {
+        if ((rc = realloc_entries(c, &c->d_lin[1], x))) return rc;
+        c->lin1_cap = x;
     }
-    dfree(c, c->d_edge_off);
-    dfree(c, c->d_dst_off);
-    dfree(c, c->d_edge_cnt);
-    c->nb_alloc = 0;
-    for (int i = 0; i < 4; ++i) {
+    const uint32_t nb = 1u << bb;
+    if (nb > c->nb_alloc) {
+        for (int s = 0; s < 2; ++s) {
+            if ((rc = realloc_entries(c, &c->d_fine[s], (uint64_t)nb * BUCKET_CAP))) return rc;
+            dfree(c, c->d_fine_cnt[s]);
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
+            c->d_fine_cnt[s] = (uint32_t *)p;
+        }
+        dfree(c, c->d_edge_off);
+        dfree(c, c->d_dst_off);
+        dfree(c, c->d_edge_cnt);
         void *p = nullptr;
-        int rc = dalloc(c, &p, (size_t)nb * BUCKET_CAP * sizeof(Entry));
-        if (rc) return rc;
-        c->d_list[i] = (Entry *)p;
-        rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t));
-        if (rc) return rc;
-        c->d_cnt[i] = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long)))) return rc;
+        c->d_edge_off = (unsigned long long *)p;
+        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long)))) return rc;
+        c->d_dst_off = (unsigned long long *)p;
+        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
+        c->d_edge_cnt = (uint32_t *)p;
+        c->nb_alloc = nb;
     }
-    void *p = nullptr;
-    int rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long));
+    const uint64_t need = inter_need(x, bb, c->slack);
+    if (need > c->inter_cap) {
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 2; ++k)
+                if ((rc = realloc_entries(c, &c->d_inter[s][k], need))) return rc;
+        c->inter_cap = need;
+    }
+    if (!c->d_inter_cnt[0][0]) {
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 2; ++k) {
+                void *p = nullptr;
+                if ((rc = dalloc(c, &p, (size_t)(1u << 16) * sizeof(uint32_t)))) return rc;
+                c->d_inter_cnt[s][k] = (uint32_t *)p;
+            }
+    }
+    const uint32_t sp = (uint32_t)std::max<uint64_t>(1u << 16, x / 256);
+    if (sp > c->spill_cap) {
+        for (int s = 0; s < 2; ++s) {
+            dfree(c, c->d_spill[s]);
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec)))) return rc;
+            c->d_spill[s] = (SpillRec *)p;
+        }
+        c->spill_cap = sp;
+    }
+    return EMP_OK;
+}
+
+// After an abort (an intermediate bucket or the spill area overflowed): more slack, more spill.
+int grow_after_abort(emp_ctx *c, uint64_t x, int bb) {
+    c->slack *= 2.0;
+    uint32_t sp = c->spill_cap * 4u;
+    c->spill_cap = 0;
+    int rc = ensure_run_buffers(c, x, bb);
     if (rc) return rc;
-    c->d_edge_off = (unsigned long long *)p;
-    rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long));
-    if (rc) return rc;
-    c->d_dst_off = (unsigned long long *)p;
-    rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t));
-    if (rc) return rc;
-    c->d_edge_cnt = (uint32_t *)p;
-    c->nb_alloc = nb;
+    if (c->spill_cap < sp) {
+        for (int s = 0; s < 2; ++s) {
+            dfree(c, c->d_spill[s]);
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec)))) return rc;
+            c->d_spill[s] = (SpillRec *)p;
+        }
+        c->spill_cap = sp;
+    }
+    return EMP_OK;
+}
+
+// Partitions linear list `side` of x entries (x also written to d_ctr->in[side]) into 2^bb buckets.
+int partition(emp_ctx *c, int side, uint64_t x, int bb) {
+    const std::vector<Level> pl = plan_levels(x, bb, c->slack);
+    const Entry *in = c->d_lin[side];
+    const uint32_t *in_cnt = nullptr;
+    uint32_t in_cap = 0, groups = 1;
+    for (size_t i = 0; i < pl.size(); ++i) {
+        const Level &l = pl[i];
+        const bool fin = i + 1 == pl.size();
+        const int bits_out = l.bits_in + l.dbits;
+        Entry *out = fin ? c->d_fine[side] : c->d_inter[side][i & 1];
+        uint32_t *out_cnt = fin ? c->d_fine_cnt[side] : c->d_inter_cnt[side][i & 1];
+        HIPCHK(c, launch_fill_u32(out_cnt, 0, 1ull << bits_out, c->stream));
+        PartArgs a{};
+        a.in = in;
+        a.in_cnt = in_cnt;
+        a.in_n = &c->d_ctr->in[side];
+        a.in_cap = in_cap;
+        a.tiles_x = in_cnt ? std::max<uint32_t>(1, (in_cap + PART_TILE - 1) / PART_TILE)
+                           : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((x + PART_TILE - 1) / PART_TILE, 1u << 16));
+        a.out = out;
+        a.out_cnt = out_cnt;
+        a.out_cap = l.cap;
+        a.out_shift = 29 - bits_out;
+        a.dbits = l.dbits;
+        a.spill = fin ? c->d_spill[side] : nullptr;
+        a.spill_n = &c->d_ctr->spill[side];
+        a.spill_cap = c->spill_cap;
+        a.abort = &c->d_ctr->abort;
+        {
+            Timed t(c, "k_part", 24.0 * (double)x);
+            HIPCHK(c, launch_part(a, groups, c->stream));
+        }
+        in = out;
+        in_cnt = out_cnt;
+        in_cap = l.cap;
+        groups = 1u << bits_out;
+    }
+    return EMP_OK;
+}
+
+// Sets the partition inputs and clears the abort flag and spill counts.
+int reset_part(emp_ctx *c, uint32_t xs, uint32_t xp) {
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->abort, 0, 1, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->spill[0], 0, 2, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->in[0], xs, 1, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->in[1], xp, 1, c->stream));
+    return EMP_OK;
+}
+
+// Clears the counters the dedup / join kernels accumulate.
+int reset_join(emp_ctx *c) {
+    HIPCHK(c, launch_fill_u32((uint32_t *)&c->d_ctr->pool_cursor, 0, 2, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->n_big, 0, 1, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin[0], 0, 2, c->stream));
     return EMP_OK;
 }
 
@@ -450,6 +605,95 @@ int read_counters(emp_ctx *c) {
     return sync(c);
 }
 
+int ensure_big_scratch(emp_ctx *c, uint64_t need) {
+    if (need <= c->bscratch_cap) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    for (int s = 0; s < 2; ++s) {
+        dfree(c, c->d_bkey[s]);
+        dfree(c, c->d_brank[s]);
+        dfree(c, c->d_baux[s]);
+        void *p = nullptr;
+        if ((rc = dalloc(c, &p, need * sizeof(uint64_t)))) return rc;
+        c->d_bkey[s] = (uint64_t *)p;
+        if ((rc = dalloc(c, &p, need * sizeof(uint32_t)))) return rc;
+        c->d_brank[s] = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, need * sizeof(uint32_t)))) return rc;
+        c->d_baux[s] = (uint32_t *)p;
+    }
+    c->bscratch_cap = need;
+    return EMP_OK;
+}
+
+// Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
+// global-memory sort + join (or dedup), one work-group per bucket.
+int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup) {
+    const uint32_t nbig = c->h_ctr->n_big;
+    if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
+    const uint32_t nb = 1u << bb;
+    std::vector<uint32_t> ids(nbig), cs(nb), cp(nb);
+    HIPCHK(c, hipMemcpy(ids.data(), c->d_big, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cs.data(), c->d_fine_cnt[0], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (!dedup) HIPCHK(c, hipMemcpy(cp.data(), c->d_fine_cnt[1], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> so(nbig), po(nbig);
+    uint64_t as = 0, ap = 0;
+    auto p2 = [](uint64_t n) { uint64_t p = 1; while (p < n) p <<= 1; return p; };
+    for (uint32_t i = 0; i < nbig; ++i) {
+        so[i] = as;
+        as += p2(cs[ids[i]]);
+        po[i] = ap;
+        ap += dedup ? 0 : p2(cp[ids[i]]);
+    }
+    int rc = ensure_big_scratch(c, std::max<uint64_t>(std::max(as, ap), 1));
+    if (rc) return rc;
+    if (!c->d_boff[0]) {
+        for (int s = 0; s < 2; ++s) {
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, BIG_CAP * sizeof(unsigned long long)))) return rc;
+            c->d_boff[s] = (unsigned long long *)p;
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_boff[0], so.data(), nbig * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_boff[1], po.data(), nbig * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    BigArgs a{};
+    a.ids = c->d_big;
+    a.soff = c->d_boff[0];
+    a.poff = c->d_boff[1];
+    a.skey = c->d_bkey[0];
+    a.pkey = c->d_bkey[1];
+    a.srank = c->d_brank[0];
+    a.prank = c->d_brank[1];
+    a.saux = c->d_baux[0];
+    a.paux = c->d_baux[1];
+    a.s_in = c->d_fine[0];
+    a.p_in = c->d_fine[1];
+    a.s_cnt = c->d_fine_cnt[0];
+    a.p_cnt = c->d_fine_cnt[1];
+    a.s_spill = c->d_spill[0];
+    a.p_spill = c->d_spill[1];
+    a.s_spill_n = &c->d_ctr->spill[0];
+    a.p_spill_n = &c->d_ctr->spill[1];
+    a.spill_cap = c->spill_cap;
+    a.cap = BUCKET_CAP;
+    a.chunks = c->d_chunk_tab;
+    a.rn = rn;
+    a.s_lin = c->d_lin[0];
+    a.p_lin = c->d_lin[1];
+    a.s_lin_n = &c->d_ctr->lin[0];
+    a.p_lin_n = &c->d_ctr->lin[1];
+    a.pool_from = c->d_pool_from;
+    a.pool_to = c->d_pool_to;
+    a.pool_cursor = &c->d_ctr->pool_cursor;
+    a.pool_cap = c->pool_cap;
+    a.edge_off = c->d_edge_off;
+    a.edge_cnt = c->d_edge_cnt;
+    {
+        Timed t(c, dedup ? "k_dedup_big" : "k_join_big", 0.0);
+        HIPCHK(c, dedup ? launch_dedup_big(a, nbig, c->stream) : launch_join_big(a, nbig, c->stream));
+    }
+    return read_counters(c);
+}
+
 int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     int rc = stage_submit(c);
     if (rc) return rc;
@@ -481,45 +725,40 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                     (unsigned long long)N0, (unsigned long long)M);
 
     const int bbD = bucket_bits(N0);
-    const uint32_t nbD = 1u << bbD;
-    if ((rc = ensure_lists(c, nbD))) return rc;
+    if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
     if ((rc = ensure_pool(c, std::max<uint64_t>(1u << 20, N0 / 4)))) return rc;
+    c->d_lin[0] = c->d_entries;
 
-    // dedup input in list 2; round lists: (0,1) current, (2,3) next
-    int cs = 0, cp = 1, ns_ = 2, np_ = 3;
-    HIPCHK(c, hipMemsetAsync(c->d_cnt[2], 0, nbD * sizeof(uint32_t), c->stream));
-    {
-        ScatterArgs a{c->d_entries, N0, c->d_list[2], c->d_cnt[2], (uint32_t)BUCKET_CAP, 29 - bbD};
-        Timed t(c, "k_scatter", 24.0 * N0);
-        HIPCHK(c, launch_scatter(a, c->stream));
-    }
-    int bb = bbD;
-    HIPCHK(c, hipMemsetAsync(c->d_cnt[cs], 0, nbD * sizeof(uint32_t), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_cnt[cp], 0, nbD * sizeof(uint32_t), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(DevCounters), c->stream));
-    {
+    // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
+    // the survivors with their round-0 suffix and prefix keys.
+    const RoundArgs r0 = round_args(c, 0, rounds > 0);
+    for (int attempt = 0;; ++attempt) {
+        if ((rc = reset_part(c, (uint32_t)N0, 0))) return rc;
+        if ((rc = partition(c, 0, N0, bbD))) return rc;
+        if ((rc = reset_join(c))) return rc;
         DedupArgs a{};
-        a.in = c->d_list[2];
-        a.cnt = c->d_cnt[2];
+        a.in = c->d_fine[0];
+        a.cnt = c->d_fine_cnt[0];
         a.chunks = c->d_chunk_tab;
-        a.r0 = round_args(c, 0, rounds > 0);
-        a.s_out = c->d_list[cs];
-        a.p_out = c->d_list[cp];
-        a.s_cnt = c->d_cnt[cs];
-        a.p_cnt = c->d_cnt[cp];
-        a.cap = BUCKET_CAP;
-        a.out_shift = 29 - bb;
+        a.r0 = r0;
+        a.s_lin = c->d_lin[0];
+        a.p_lin = c->d_lin[1];
+        a.lin_n = &c->d_ctr->lin[0];
         a.big = c->d_big;
         a.n_big = &c->d_ctr->n_big;
         a.big_cap = BIG_CAP;
-        a.n_kept = &c->d_ctr->kept;
-        Timed t(c, "k_dedup", 12.0 * N0 + (rounds > 0 ? 24.0 : 0.0) * N0);
-        HIPCHK(c, launch_dedup(a, nbD, c->stream));
+        a.abort = &c->d_ctr->abort;
+        {
+            Timed t(c, "k_dedup", 12.0 * N0);
+            HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
+        }
+        if ((rc = read_counters(c))) return rc;
+        if (!c->h_ctr->abort) break;
+        if (attempt > 6) return fail(c, EMP_ERR_CAPACITY, "dedup partition keeps overflowing");
+        if ((rc = grow_after_abort(c, N0, bbD))) return rc;
     }
-    if ((rc = read_counters(c))) return rc;
-    if (c->h_ctr->n_big)
-        return fail(c, EMP_ERR_CAPACITY, "%u dedup buckets exceed %d entries", c->h_ctr->n_big, BUCKET_CAP);
-    uint64_t St = c->h_ctr->kept, Pt = St;
+    if (c->h_ctr->n_big && (rc = big_path(c, bbD, r0, true))) return rc;
+    uint64_t St = c->h_ctr->lin[0], Pt = St;
     S.n_dedup = St;
 
     int t = 0;
@@ -530,33 +769,35 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                         "round %d: %llu suffixes / %llu prefixes reach M/2 = %llu: multi-record framing of "
                         "cqutils.cuh:107-137 is not reproduced; raise -r",
                         t, (unsigned long long)St, (unsigned long long)Pt, (unsigned long long)(M / 2));
-        const int bbn = bucket_bits(std::max(St, Pt));
-        const uint32_t nbn = 1u << bbn, nbt = 1u << bb;
+        const int bb = bucket_bits(std::max(St, Pt));
+        const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
+        const RoundArgs rn = round_args(c, t + 1, more);
         emp_round_stats rs{};
         rs.s = St;
         rs.p = Pt;
         rs.ovlp = L - 1 - t;
         rs.bucket_bits = bb;
+        bool need_part = true;
         for (int attempt = 0;; ++attempt) {
-            if (more) {
-                HIPCHK(c, hipMemsetAsync(c->d_cnt[ns_], 0, nbn * sizeof(uint32_t), c->stream));
-                HIPCHK(c, hipMemsetAsync(c->d_cnt[np_], 0, nbn * sizeof(uint32_t), c->stream));
+            if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
+            if (need_part) {
+                if ((rc = reset_part(c, (uint32_t)St, (uint32_t)Pt))) return rc;
+                if ((rc = partition(c, 0, St, bb))) return rc;
+                if ((rc = partition(c, 1, Pt, bb))) return rc;
             }
-            HIPCHK(c, hipMemsetAsync(c->d_ctr, 0, sizeof(DevCounters), c->stream));
+            if ((rc = reset_join(c))) return rc;
             JoinArgs a{};
-            a.s_in = c->d_list[cs];
-            a.p_in = c->d_list[cp];
-            a.s_cnt = c->d_cnt[cs];
-            a.p_cnt = c->d_cnt[cp];
+            a.s_in = c->d_fine[0];
+            a.p_in = c->d_fine[1];
+            a.s_cnt = c->d_fine_cnt[0];
+            a.p_cnt = c->d_fine_cnt[1];
             a.chunks = c->d_chunk_tab;
-            a.rn = round_args(c, t + 1, more);
-            a.s_out = c->d_list[ns_];
-            a.p_out = c->d_list[np_];
-            a.s_out_cnt = c->d_cnt[ns_];
-            a.p_out_cnt = c->d_cnt[np_];
-            a.cap = BUCKET_CAP;
-            a.out_shift = 29 - bbn;
+            a.rn = rn;
+            a.s_lin = c->d_lin[0];
+            a.p_lin = c->d_lin[1];
+            a.s_lin_n = &c->d_ctr->lin[0];
+            a.p_lin_n = &c->d_ctr->lin[1];
             a.pool_from = c->d_pool_from;
             a.pool_to = c->d_pool_to;
             a.pool_cursor = &c->d_ctr->pool_cursor;
@@ -566,53 +807,63 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             a.big = c->d_big;
             a.n_big = &c->d_ctr->n_big;
             a.big_cap = BIG_CAP;
-            a.surv_s = &c->d_ctr->surv_s;
-            a.surv_p = &c->d_ctr->surv_p;
+            a.abort = &c->d_ctr->abort;
             {
                 Timed tm(c, "k_join", 0.0);
                 HIPCHK(c, launch_join(a, nbt, c->stream));
             }
             if ((rc = read_counters(c))) return rc;
-            if (c->h_ctr->pool_cursor <= c->pool_cap) break;
-            if (attempt > 2) return fail(c, EMP_ERR_CAPACITY, "edge pool did not settle");
-            if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
+            if (c->h_ctr->abort) {
+                if ((rc = grow_after_abort(c, std::max(St, Pt), bb))) return rc;
+                need_part = true;
+                continue;
+            }
+            need_part = false;
+            if (c->h_ctr->pool_cursor > c->pool_cap) {
+                if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
+                continue;
+            }
+            rs.big_buckets = (int32_t)c->h_ctr->n_big;
+            if (c->h_ctr->n_big) {
+                if ((rc = big_path(c, bb, rn, false))) return rc;
+                if (c->h_ctr->pool_cursor > c->pool_cap) {
+                    if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
+                    continue;
+                }
+            }
+            break;
         }
         const DevCounters &k = *c->h_ctr;
-        if (k.n_big) return fail(c, EMP_ERR_CAPACITY, "round %d: %u buckets exceed the LDS join capacity", t, k.n_big);
         const uint64_t E = k.pool_cursor;
         rs.edges = E;
-        rs.s_after = k.surv_s;
-        rs.p_after = k.surv_p;
-        // algorithmic bytes of the join: entries read, survivors written, edges written
-        if (c->opts.profile && !c->kstats.empty()) {
+        rs.s_after = k.lin[0];
+        rs.p_after = k.lin[1];
+        if (c->opts.profile) {
             int id = kstat_id(c, "k_join");
-            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (k.surv_s + k.surv_p) : 0.0) + 8.0 * E;
+            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (k.lin[0] + k.lin[1]) : 0.0) + 8.0 * E;
         }
         // record (overlap.cu:422; written even when E == 0)
-        {
+        if (E) {
             EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, nbt,
                             c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
-            if (E) {
-                Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
-                HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
-                HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, E * 4, hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, E * 4, hipMemcpyDeviceToHost, c->stream));
-                if ((rc = sync(c))) return rc;
-            }
-            if (sink) {
-                int src = sink(user, E, L - 1 - t, c->h_from, c->h_to);
-                if (src) return fail(c, src, "record sink returned %d", src);
-            }
-            rs.records = 1;
-            S.n_records++;
-            S.n_edges += E;
+            Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
+            HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
         }
+        if (E) {
+            HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, E * 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, E * 4, hipMemcpyDeviceToHost, c->stream));
+            if ((rc = sync(c))) return rc;
+        }
+        if (sink) {
+            int src = sink(user, E, L - 1 - t, c->h_from, c->h_to);
+            if (src) return fail(c, src, "record sink returned %d", src);
+        }
+        rs.records = 1;
+        S.n_records++;
+        S.n_edges += E;
         c->rounds.push_back(rs);
-        St = k.surv_s;
-        Pt = k.surv_p;
-        std::swap(cs, ns_);
-        std::swap(cp, np_);
-        bb = bbn;
+        St = k.lin[0];
+        Pt = k.lin[1];
     }
     S.n_rounds = t;
     return EMP_OK;

@@ -216,3 +216,42 @@ def test_error_paths():
     with pytest.raises(native.EmpError) as e:
         run_gpu_fastq(FQ([x.encode() for x in reads]), min_ovlp=12, reduce_mem=16 * 200)
     assert e.value.code == -6
+
+
+def _shared_overlap_reads(k, seed):
+    """k reads ending in one common 30-mer and k reads starting with it: a single key group of k
+    suffixes and k prefixes at overlap 30 (k*k edges)."""
+    rng = random.Random(seed)
+    common = "".join(rng.choice("ACGT") for _ in range(30))
+    a = ["".join(rng.choice("ACGT") for _ in range(10)) + common for _ in range(k)]
+    b = [common + "".join(rng.choice("ACGT") for _ in range(10)) for _ in range(k)]
+    return a + b
+
+
+@pytest.mark.parametrize("k", [600, 3000])
+def test_large_key_groups_take_the_global_path(k):
+    """600 x 600 overflows the LDS match records; 3000 x 3000 overflows a bucket (spill)."""
+    reads = _shared_overlap_reads(k, k)
+    r = reads_of(reads)
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 40, min_ovlp=29)
+    assert rc == 0 and osum["n_edges"] >= k * k
+    with native.Context(min_ovlp=29) as c:
+        c.add_reads(r)
+        got = c.run()
+        rounds = c.rounds()
+    assert any(x["big_buckets"] > 0 for x in rounds)
+    assert got == want
+
+
+def test_identical_reads_dedup_big_bucket():
+    rng = random.Random(77)
+    s = "".join(rng.choice("ACGT") for _ in range(60))
+    reads = [s] * 5000 + ["".join(rng.choice("ACGT") for _ in range(60)) for _ in range(300)]
+    r = reads_of(reads)
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 60, min_ovlp=20)
+    assert rc == 0
+    with native.Context(min_ovlp=20) as c:
+        c.add_reads(r)
+        got = c.run()
+        assert c.summary()["n_dedup"] == osum["n_dedup"]
+    assert got == want
