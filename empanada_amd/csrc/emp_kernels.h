@@ -8,15 +8,21 @@
 
 namespace emp {
 
-// Bucket capacity (entries per bucket per list) and matched-record capacity of the LDS join.
+// Bucket capacity (entries per bucket per list), matched-record capacity and work-group size of
+// the LDS join.
 constexpr int BUCKET_CAP = 2048;
-constexpr int MATCH_CAP = 512;
+constexpr int MATCH_CAP = 384;
+constexpr int JOIN_THREADS = 512;
 // Target mean bucket load: the partition fan-out is chosen so that lists average this many
 // entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
 constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
 // Entries per work-group tile of a partition pass, and the widest digit of one pass.
-constexpr int PART_TILE = 4096;
+constexpr int PART_TILE = 2048;
 constexpr int PART_MAX_BITS = 8;
+// Linear outputs (survivor lists, the edge pool) are split into SUBS sub-ranges, each with its
+// own cursor: work-group b reserves in sub-range b % SUBS, so no single counter serialises a
+// launch and, with work-groups dealt round-robin to the 8 XCDs, each cursor stays on one XCD.
+constexpr int SUBS = 256;
 
 // An entry that did not fit its final bucket, with the bucket it belongs to.
 struct SpillRec {
@@ -50,6 +56,8 @@ struct PartArgs {
     const uint32_t *in_cnt;  // per-group counts; null for one linear group of *in_n entries
     const uint32_t *in_n;
     uint32_t in_cap;         // group stride in entries (ignored for a linear group)
+    int32_t keyed;           // groups share their top key bits (bucket = group << dbits | digit),
+                             // or are plain segments of one list (bucket = digit)
     uint32_t tiles_x;        // work-groups per input group
     Entry *out;
     uint32_t *out_cnt;
@@ -68,10 +76,12 @@ struct DedupArgs {
     const uint32_t *const *chunks;
     RoundArgs r0;
     Entry *s_lin, *p_lin;    // survivors with round-0 keys, same positions in both lists
-    uint32_t *lin_n;
+    uint32_t *lin_n;         // [SUBS] counts of the sub-ranges
+    uint32_t lin_cap;        // entries per sub-range
     uint32_t *big, *n_big;
     uint32_t big_cap;
     const uint32_t *abort;
+    uint32_t *lin_ovf;       // set when a sub-range overflows
 };
 
 struct JoinArgs {
@@ -79,16 +89,18 @@ struct JoinArgs {
     const uint32_t *s_cnt, *p_cnt;
     const uint32_t *const *chunks;
     RoundArgs rn;            // regeneration for the next round
-    Entry *s_lin, *p_lin;    // survivors, linear
+    Entry *s_lin, *p_lin;    // survivors, SUBS sub-ranges of lin_cap entries
     uint32_t *s_lin_n, *p_lin_n;
-    uint32_t *pool_from, *pool_to;
-    unsigned long long *pool_cursor;
+    uint32_t lin_cap;
+    uint32_t *pool_from, *pool_to;   // edge pool, SUBS sub-ranges of pool_cap edges
+    unsigned long long *pool_cursor; // [SUBS]
     unsigned long long pool_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
     uint32_t *big, *n_big;
     uint32_t big_cap;
     const uint32_t *abort;
+    uint32_t *lin_ovf;
 };
 
 // Large buckets (more than BUCKET_CAP entries, or more matched records than MATCH_CAP): one
@@ -110,11 +122,13 @@ struct BigArgs {
     RoundArgs rn;
     Entry *s_lin, *p_lin;
     uint32_t *s_lin_n, *p_lin_n;
+    uint32_t lin_cap;
     uint32_t *pool_from, *pool_to;
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
+    uint32_t *lin_ovf;
 };
 
 struct EdgeCopyArgs {

@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
         uint32_t tot;
         const uint32_t ex = block_excl_scan<256>(h, sh, tot);
         loc[threadIdx.x] = ex;
-        base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[(g << a.dbits) + threadIdx.x], h) : 0u;
+        const uint32_t gb = a.keyed ? (g << a.dbits) : 0u;
+        base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[gb + threadIdx.x], h) : 0u;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
         for (uint32_t p = threadIdx.x; p < nt; p += 256) {
             const uint32_t dd = dig[p];
             const uint32_t slot = base[dd] + (p - loc[dd]);
-            const uint32_t bucket = (g << a.dbits) + dd;
+            const uint32_t bucket = gb + dd;
             if (slot < a.out_cap) {
                 a.out[(size_t)bucket * a.out_cap + slot] = buf[p];
             } else if (a.spill) {
@@ -252,14 +253,14 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
 // Dedup: per bucket keep the entry with the smallest id of each key (stable sort by key of an
 // id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306), then
 // regenerate it for round 0 as a suffix and as a prefix (overlap.cu:266-267).
-template <int CAP>
-__global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
+template <int CAP, int NT>
+__global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-    constexpr int PER = CAP / 256;
+    constexpr int PER = CAP / NT;
     __shared__ uint32_t h1s[CAP], h2s[CAP], rids[CAP];
     __shared__ uint32_t head[CAP];
     __shared__ uint16_t nxt[CAP];
-    __shared__ uint32_t sh[4];
+    __shared__ uint32_t sh[NT / 64];
     __shared__ uint32_t s_base;
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
@@ -273,18 +274,18 @@ __global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
     }
     if (n == 0) return;
     const Entry *in = a.in + (size_t)b * CAP;
-    for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
-    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    for (uint32_t i = threadIdx.x; i < CAP; i += NT) head[i] = EMPTY;
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
         Entry e = in[i];
         h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += 256) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
+    for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
     __syncthreads();
     uint32_t keepmask = 0, kept = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        uint32_t i = threadIdx.x + 256 * q;
+        uint32_t i = threadIdx.x + NT * q;
         if (i < n) {
             uint32_t h1 = h1s[i], h2 = h2s[i], rid = rids[i];
             bool keep = true;
@@ -294,15 +295,20 @@ __global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
         }
     }
     uint32_t tot;
-    const uint32_t off = block_excl_scan<256>(kept, sh, tot);
-    if (threadIdx.x == 0) s_base = tot ? atomicAdd(a.lin_n, tot) : 0u;
+    const uint32_t off = block_excl_scan<NT>(kept, sh, tot);
+    const uint32_t sub = b & (SUBS - 1);
+    if (threadIdx.x == 0) {
+        uint32_t c = tot ? atomicAdd(a.lin_n + sub, tot) : 0u;
+        if (c + tot > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
+        s_base = c;
+    }
     __syncthreads();
-    if (a.r0.active) {
-        uint32_t pos = s_base + off;
+    if (a.r0.active && s_base != 0xFFFFFFFFu) {
+        uint32_t pos = sub * a.lin_cap + s_base + off;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             if (keepmask & (1u << q)) {
-                uint32_t i = threadIdx.x + 256 * q;
+                uint32_t i = threadIdx.x + NT * q;
                 Entry s = {h1s[i], h2s[i], rids[i]}, p = s;
                 regen(a.chunks, a.r0, s, true);
                 regen(a.chunks, a.r0, p, false);
@@ -316,20 +322,71 @@ __global__ void __launch_bounds__(256) k_dedup(DedupArgs a) {
 
 // ---------------------------------------------------------------------------------------------
 // Join of round t for one bucket.
-template <int CAP, int MCAP>
-__global__ void __launch_bounds__(256) k_join(JoinArgs a) {
+//
+// LDS: the prefix entries (12 B each), a used flag per prefix, and one region that holds the hash
+// table (chain heads + links) while probing and the matched records afterwards. The suffix
+// entries stay in registers. The base each entry needs for the next round is gathered right
+// after the entries are loaded, so its latency overlaps the hash phase.
+template <int CAP, int MCAP, int NT>
+struct JoinShared {
+    uint32_t pk1[CAP], pk2[CAP], prid[CAP];
+    uint8_t used[CAP];
+    union {
+        struct {
+            uint32_t head[CAP];
+            uint16_t nxt[CAP];
+        } h;
+        struct {
+            uint64_t msk[MCAP], mpk[MCAP];
+            uint32_t msr[MCAP], mpr[MCAP], msc[MCAP];
+        } m;
+    } u;
+    uint32_t sh[NT / 64];
+    uint32_t ms, mp, ovf, bs, bp;
+    unsigned long long base;
+};
+
+// Base code for the next round of a live entry (rid), or 0 when there is no next round.
+__device__ __forceinline__ uint32_t next_code(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, uint32_t rid,
+                                              bool sfx) {
+    return r.active ? entry_code(chunks, r, rid, sfx) : 0u;
+}
+
+__device__ __forceinline__ void regen_code(const RoundArgs &r, Entry &e, uint32_t code, bool sfx) {
+    uint32_t ev = enc_of_code(code);
+    if (sfx)
+        next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
+    else
+        prev_pfx(e.h1, e.h2, ev);
+}
+
+// Sorts n <= NT records (key, rank[, payload]) in place by (key, rank): each thread counts the
+// records smaller than its own, then stores its record at that position. (key, rank) pairs are
+// distinct, so the positions are a permutation.
+template <int NT>
+__device__ __forceinline__ void rank_sort(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n, uint64_t &k,
+                                          uint32_t &r, uint32_t &pv, uint32_t &pos) {
+    const uint32_t i = threadIdx.x;
+    pos = 0;
+    if (i < n) {
+        k = key[i];
+        r = rank[i];
+        pv = pay ? pay[i] : 0u;
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint64_t kj = key[j];
+            pos += (kj < k) || (kj == k && rank[j] < r);
+        }
+    }
+}
+
+template <int CAP, int MCAP, int NT>
+__global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-    constexpr int PER = CAP / 256;
-    __shared__ uint32_t pk1[CAP], pk2[CAP], prid[CAP];
-    __shared__ uint32_t head[CAP];
-    __shared__ uint16_t nxt[CAP];
-    __shared__ uint8_t used[CAP];
-    __shared__ uint64_t msk[MCAP], mpk[MCAP];
-    __shared__ uint32_t msr[MCAP], mpr[MCAP];
-    __shared__ uint32_t msc[MCAP];
-    __shared__ uint32_t sh[4];
-    __shared__ uint32_t s_ms, s_mp, s_ovf, s_bs, s_bp;
-    __shared__ unsigned long long s_base;
+    constexpr int PER = CAP / NT;
+    static_assert(MCAP <= NT, "one matched record per thread in the rank sort");
+    using SM = JoinShared<CAP, MCAP, NT>;
+    static_assert(sizeof(((SM *)nullptr)->u.m) <= sizeof(((SM *)nullptr)->u.h), "matched records fit the hash table");
+    __shared__ SM S;
 
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
@@ -342,68 +399,78 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
         }
         return;
     }
-    if (threadIdx.x == 0) { s_ms = 0; s_mp = 0; s_ovf = 0; }
+    if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; }
     const Entry *sin = a.s_in + (size_t)b * CAP;
     const Entry *pin = a.p_in + (size_t)b * CAP;
-    for (uint32_t i = threadIdx.x; i < CAP; i += 256) head[i] = EMPTY;
-    for (uint32_t i = threadIdx.x; i < np; i += 256) {
-        Entry e = pin[i];
-        pk1[i] = e.h1; pk2[i] = e.h2; prid[i] = e.rid; used[i] = 0;
-    }
+    for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
     Entry se[PER];
-    uint32_t sc[PER];
+    uint32_t sc[PER], scode[PER], pcode[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        uint32_t i = threadIdx.x + 256 * q;
-        if (i < ns) se[q] = sin[i];
+        const uint32_t i = threadIdx.x + NT * q;
+        if (i < np) {
+            Entry e = pin[i];
+            S.pk1[i] = e.h1; S.pk2[i] = e.h2; S.prid[i] = e.rid; S.used[i] = 0;
+            pcode[q] = next_code(a.chunks, a.rn, e.rid, false);
+        }
+        if (i < ns) {
+            se[q] = sin[i];
+            scode[q] = next_code(a.chunks, a.rn, se[q].rid, true);
+        }
         sc[q] = 0;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < np; i += 256) nxt[i] = (uint16_t)atomicExch(head + (pk2[i] & (CAP - 1)), i);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = threadIdx.x + NT * q;
+        if (i < np) S.u.h.nxt[i] = (uint16_t)atomicExch(S.u.h.head + (S.pk2[i] & (CAP - 1)), i);
+    }
     __syncthreads();
 
     // Probe: count valid partners (different read) of every suffix; flag the prefixes used.
     if (np) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            uint32_t i = threadIdx.x + 256 * q;
+            const uint32_t i = threadIdx.x + NT * q;
             if (i < ns) {
                 const uint32_t h1 = se[q].h1, h2 = se[q].h2, rd = se[q].rid >> 1;
                 uint32_t c = 0;
-                for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j])) {
-                    if (pk2[j] == h2 && pk1[j] == h1 && (prid[j] >> 1) != rd) {
+                for (uint32_t j = S.u.h.head[h2 & (CAP - 1)]; j != EMPTY; j = (S.u.h.nxt[j] == 0xFFFFu ? EMPTY : S.u.h.nxt[j])) {
+                    if (S.pk2[j] == h2 && S.pk1[j] == h1 && (S.prid[j] >> 1) != rd) {
                         ++c;
-                        used[j] = 1;
+                        S.used[j] = 1;
                     }
                 }
                 sc[q] = c;
-                if (c) {
-                    uint32_t m = atomicAdd(&s_ms, 1u);
-                    if (m < (uint32_t)MCAP) {
-                        msk[m] = ((uint64_t)h1 << 32) | h2;
-                        msr[m] = rid_rank(se[q].rid);
-                        msc[m] = c;
-                    } else {
-                        s_ovf = 1;
-                    }
-                }
             }
         }
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < np; i += 256) {
-        if (used[i]) {
-            uint32_t m = atomicAdd(&s_mp, 1u);
+    __syncthreads();  // the hash table is dead from here on; its space holds the matched records
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t i = threadIdx.x + NT * q;
+        if (i < ns && sc[q]) {
+            uint32_t m = atomicAdd(&S.ms, 1u);
             if (m < (uint32_t)MCAP) {
-                mpk[m] = ((uint64_t)pk1[i] << 32) | pk2[i];
-                mpr[m] = rid_rank(prid[i]);
+                S.u.m.msk[m] = ((uint64_t)se[q].h1 << 32) | se[q].h2;
+                S.u.m.msr[m] = rid_rank(se[q].rid);
+                S.u.m.msc[m] = sc[q];
             } else {
-                s_ovf = 1;
+                S.ovf = 1;
+            }
+        }
+        if (i < np && S.used[i]) {
+            uint32_t m = atomicAdd(&S.mp, 1u);
+            if (m < (uint32_t)MCAP) {
+                S.u.m.mpk[m] = ((uint64_t)S.pk1[i] << 32) | S.pk2[i];
+                S.u.m.mpr[m] = rid_rank(S.prid[i]);
+            } else {
+                S.ovf = 1;
             }
         }
     }
     __syncthreads();
-    if (s_ovf) {
+    if (S.ovf) {
         if (threadIdx.x == 0) {
             uint32_t i = atomicAdd(a.n_big, 1u);
             if (i < a.big_cap) a.big[i] = b;
@@ -411,48 +478,47 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
         }
         return;
     }
-    const uint32_t nms = s_ms, nmp = s_mp;
+    const uint32_t nms = S.ms, nmp = S.mp;
 
-    // Edges of the bucket: sort matched suffixes and prefixes by (key, rank), then emit the
+    // Edges of the bucket: order matched suffixes and prefixes by (key, rank), then emit the
     // cross product of each key group skipping same-read pairs (overlap.cu:557-601).
     if (nms) {
-        uint32_t n2 = 1;
-        while (n2 < nms) n2 <<= 1;
-        for (uint32_t i = nms + threadIdx.x; i < n2; i += 256) { msk[i] = ~0ull; msr[i] = ~0u; msc[i] = 0; }
-        uint32_t m2 = 1;
-        while (m2 < nmp) m2 <<= 1;
-        for (uint32_t i = nmp + threadIdx.x; i < m2; i += 256) { mpk[i] = ~0ull; mpr[i] = ~0u; }
+        uint64_t k1 = 0, k2 = 0;
+        uint32_t r1 = 0, r2 = 0, c1 = 0, c2 = 0, p1, p2;
+        rank_sort<NT>(S.u.m.msk, S.u.m.msr, S.u.m.msc, nms, k1, r1, c1, p1);
+        rank_sort<NT>(S.u.m.mpk, S.u.m.mpr, nullptr, nmp, k2, r2, c2, p2);
         __syncthreads();
-        bitonic(msk, msr, msc, n2);
-        bitonic(mpk, mpr, nullptr, m2);
-        static_assert(MCAP <= 512, "scan covers two records per thread");
-        uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
-        uint32_t c0 = i0 < nms ? msc[i0] : 0, c1 = i1 < nms ? msc[i1] : 0;
+        if (threadIdx.x < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
+        if (threadIdx.x < nmp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
+        __syncthreads();
+        const uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
+        const uint32_t e0 = i0 < nms ? S.u.m.msc[i0] : 0, e1 = i1 < nms ? S.u.m.msc[i1] : 0;
         uint32_t tot;
-        uint32_t off = block_excl_scan<256>(c0 + c1, sh, tot);
-        if (i0 < nms) msc[i0] = off;
-        if (i1 < nms) msc[i1] = off + c0;
+        const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, tot);
+        if (i0 < nms) S.u.m.msc[i0] = off;
+        if (i1 < nms) S.u.m.msc[i1] = off + e0;
         if (threadIdx.x == 0) {
-            unsigned long long base = atomicAdd(a.pool_cursor, (unsigned long long)tot);
-            s_base = base;
-            a.edge_off[b] = base;
+            const uint32_t sub = b & (SUBS - 1);
+            unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
+            S.base = c + tot <= a.pool_cap ? sub * a.pool_cap + c : ~0ull;
+            a.edge_off[b] = sub * a.pool_cap + c;
             a.edge_cnt[b] = tot;
         }
         __syncthreads();
-        const unsigned long long base = s_base;
-        if (base + tot <= a.pool_cap) {
-            for (uint32_t i = threadIdx.x; i < nms; i += 256) {
-                const uint64_t k = msk[i];
-                const uint32_t srid = (msr[i] << 1) | (msr[i] >> 31);
+        const unsigned long long base = S.base;
+        if (base != ~0ull) {
+            for (uint32_t i = threadIdx.x; i < nms; i += NT) {
+                const uint64_t k = S.u.m.msk[i];
+                const uint32_t srid = (S.u.m.msr[i] << 1) | (S.u.m.msr[i] >> 31);
                 uint32_t lo = 0, hi = nmp;
                 while (lo < hi) {
                     uint32_t mid = (lo + hi) >> 1;
-                    if (mpk[mid] < k) lo = mid + 1; else hi = mid;
+                    if (S.u.m.mpk[mid] < k) lo = mid + 1; else hi = mid;
                 }
-                unsigned long long pos = base + msc[i];
-                const unsigned long long end = base + (i + 1 < nms ? msc[i + 1] : tot);
-                for (uint32_t j = lo; j < nmp && mpk[j] == k && pos < end; ++j) {
-                    const uint32_t prd = (mpr[j] << 1) | (mpr[j] >> 31);
+                unsigned long long pos = base + S.u.m.msc[i];
+                const unsigned long long end = base + (i + 1 < nms ? S.u.m.msc[i + 1] : tot);
+                for (uint32_t j = lo; j < nmp && S.u.m.mpk[j] == k && pos < end; ++j) {
+                    const uint32_t prd = (S.u.m.mpr[j] << 1) | (S.u.m.mpr[j] >> 31);
                     if ((prd >> 1) != (srid >> 1)) {
                         a.pool_from[pos] = srid;
                         a.pool_to[pos] = prd;
@@ -468,38 +534,42 @@ __global__ void __launch_bounds__(256) k_join(JoinArgs a) {
 
     // Survivors: suffixes with no edge, prefixes never a target (overlap.cu:424-429,694-710),
     // regenerated for the next round and appended to the linear lists (one reservation per list).
-    uint32_t smask = 0, ks = 0, pmask = 0, kp = 0;
+    uint32_t smask = 0, pmask = 0, ks = 0, kp = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        uint32_t i = threadIdx.x + 256 * q;
+        const uint32_t i = threadIdx.x + NT * q;
         if (i < ns && sc[q] == 0) { smask |= 1u << q; ++ks; }
-        if (i < np && !used[i]) { pmask |= 1u << q; ++kp; }
+        if (i < np && !S.used[i]) { pmask |= 1u << q; ++kp; }
     }
-    uint32_t ts, tp;
-    const uint32_t os = block_excl_scan<256>(ks, sh, ts);
-    const uint32_t op = block_excl_scan<256>(kp, sh, tp);
+    uint32_t tt;
+    const uint32_t o2 = block_excl_scan<NT>(ks | (kp << 16), S.sh, tt);
+    const uint32_t ts = tt & 0xFFFFu, tp = tt >> 16;
+    const uint32_t sub = b & (SUBS - 1);
     if (threadIdx.x == 0) {
-        s_bs = ts ? atomicAdd(a.s_lin_n, ts) : 0u;
-        s_bp = tp ? atomicAdd(a.p_lin_n, tp) : 0u;
+        uint32_t cs = ts ? atomicAdd(a.s_lin_n + sub, ts) : 0u;
+        uint32_t cp = tp ? atomicAdd(a.p_lin_n + sub, tp) : 0u;
+        if (cs + ts > a.lin_cap || cp + tp > a.lin_cap) atomicOr(a.lin_ovf, 1u);
+        S.bs = cs;
+        S.bp = cp;
     }
     __syncthreads();
-    if (a.rn.active) {
-        uint32_t pos = s_bs + os;
+    if (a.rn.active && S.bs + ts <= a.lin_cap && S.bp + tp <= a.lin_cap) {
+        uint32_t pos = sub * a.lin_cap + S.bs + (o2 & 0xFFFFu);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             if (smask & (1u << q)) {
                 Entry e = se[q];
-                regen(a.chunks, a.rn, e, true);
+                regen_code(a.rn, e, scode[q], true);
                 a.s_lin[pos++] = e;
             }
         }
-        pos = s_bp + op;
+        pos = sub * a.lin_cap + S.bp + (o2 >> 16);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             if (pmask & (1u << q)) {
-                uint32_t i = threadIdx.x + 256 * q;
-                Entry e = {pk1[i], pk2[i], prid[i]};
-                regen(a.chunks, a.rn, e, false);
+                const uint32_t i = threadIdx.x + NT * q;
+                Entry e = {S.pk1[i], S.pk2[i], S.prid[i]};
+                regen_code(a.rn, e, pcode[q], false);
                 a.p_lin[pos++] = e;
             }
         }
@@ -611,10 +681,11 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         SA[i] = off;
         off += c;
     }
+    const uint32_t sub = b & (SUBS - 1);
     if (threadIdx.x == 0) {
-        unsigned long long base = atomicAdd(a.pool_cursor, (unsigned long long)tot);
-        s_base = base;
-        a.edge_off[b] = base;
+        unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
+        s_base = c + tot <= a.pool_cap ? sub * a.pool_cap + c : ~0ull;
+        a.edge_off[b] = sub * a.pool_cap + c;
         a.edge_cnt[b] = tot;
     }
     __syncthreads();
@@ -622,7 +693,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
     for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
         const uint32_t o = SA[i], e = (i + 1 < ns) ? SA[i + 1] : tot;
         const uint32_t srid = (SR[i] << 1) | (SR[i] >> 31);
-        if (e > o && base + tot <= a.pool_cap) {
+        if (e > o && base != ~0ull) {
             const uint64_t k = SK[i];
             const uint32_t plo = lower_u64(PK, 0, np, k);
             unsigned long long pos = base + o;
@@ -637,10 +708,11 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         }
         if (e == o) {
             Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
-            uint32_t pos = atomicAdd(a.s_lin_n, 1u);
-            if (a.rn.active) {
+            uint32_t pos = atomicAdd(a.s_lin_n + sub, 1u);
+            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
+            else if (a.rn.active) {
                 regen(a.chunks, a.rn, en, true);
-                a.s_lin[pos] = en;
+                a.s_lin[sub * a.lin_cap + pos] = en;
             }
         }
     }
@@ -648,10 +720,11 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         if (!PA[j]) {
             const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
             Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
-            uint32_t pos = atomicAdd(a.p_lin_n, 1u);
-            if (a.rn.active) {
+            uint32_t pos = atomicAdd(a.p_lin_n + sub, 1u);
+            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
+            else if (a.rn.active) {
                 regen(a.chunks, a.rn, en, false);
-                a.p_lin[pos] = en;
+                a.p_lin[sub * a.lin_cap + pos] = en;
             }
         }
     }
@@ -668,13 +741,15 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if (i == 0 || K[i] != K[i - 1]) {
             Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
-            uint32_t pos = atomicAdd(a.s_lin_n, 1u);
-            if (a.rn.active) {
+            const uint32_t sub = b & (SUBS - 1);
+            uint32_t pos = atomicAdd(a.s_lin_n + sub, 1u);
+            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
+            else if (a.rn.active) {
                 Entry p = e;
                 regen(a.chunks, a.rn, e, true);
                 regen(a.chunks, a.rn, p, false);
-                a.s_lin[pos] = e;
-                a.p_lin[pos] = p;
+                a.s_lin[sub * a.lin_cap + pos] = e;
+                a.p_lin[sub * a.lin_cap + pos] = p;
             }
         }
     }
@@ -753,12 +828,12 @@ hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {
 }
 
 hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_dedup<BUCKET_CAP>, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_dedup<BUCKET_CAP, JOIN_THREADS>), dim3(nb), dim3(JOIN_THREADS), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s) {
-    hipLaunchKernelGGL((k_join<BUCKET_CAP, MATCH_CAP>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_join<BUCKET_CAP, MATCH_CAP, JOIN_THREADS>), dim3(nb), dim3(JOIN_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

@@ -28,11 +28,11 @@ constexpr uint32_t STAGE_READS = 1u << 20;  // reads per host->device batch (mul
 constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large-bucket path
 
 struct DevCounters {
-    unsigned long long pool_cursor;
-    uint32_t n_big, abort;
-    uint32_t lin[2];    // linear list sizes written by dedup / join (suffix, prefix)
-    uint32_t spill[2];  // records spilled by the final partition pass
-    uint32_t in[2];     // partition input sizes, set by the host
+    unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
+    uint32_t lin[2][2][SUBS];       // [survivor set][side][sub-range] counts
+    uint32_t n_big, abort, lin_ovf, pad;
+    uint32_t spill[2];              // records spilled by the final partition pass
+    uint32_t in[2];                 // partition input sizes of linear lists, set by the host
 };
 
 double now_ms() {
@@ -102,8 +102,8 @@ struct emp_ctx {
     size_t stage_bytes = 0;
 
     // overlap phase, per list side (0 = suffixes, 1 = prefixes)
-    Entry *d_lin[2] = {nullptr, nullptr};  // linear lists; side 0 is the entries buffer
-    uint64_t lin1_cap = 0;
+    Entry *d_lin[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // survivor sets [set][side]
+    uint32_t lin_cap[2] = {0, 0};   // entries per sub-range of each set
     Entry *d_inter[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t *d_inter_cnt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint64_t inter_cap = 0;  // entries per intermediate array
@@ -115,7 +115,8 @@ struct emp_ctx {
     double slack = 1.125;
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
-    uint64_t pool_cap = 0;
+    uint64_t pool_cap = 0;  // edges per sub-range of the pool
+    uint64_t out_cap = 0;   // edges of the compacted arrays
     unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr;
     uint32_t *d_edge_cnt = nullptr;
     // large buckets
@@ -412,9 +413,14 @@ std::vector<Level> plan_levels(uint64_t x, int bb, double slack) {
 uint64_t inter_need(uint64_t x, int bb, double slack) {
     uint64_t need = 0;
     for (const Level &l : plan_levels(x, bb, slack))
-        if (l.cap != (uint32_t)BUCKET_CAP || l.bits_in + l.dbits != bb)
-            need = std::max<uint64_t>(need, (1ull << (l.bits_in + l.dbits)) * l.cap);
+        if (l.bits_in + l.dbits != bb) need = std::max<uint64_t>(need, (1ull << (l.bits_in + l.dbits)) * l.cap);
     return need;
+}
+
+// Entries per sub-range for survivor lists of up to x entries.
+uint32_t lin_cap_for(uint64_t x, double slack) {
+    uint64_t per = (uint64_t)((double)x / SUBS * slack) + 1024;
+    return (uint32_t)((per + 63) & ~63ull);
 }
 
 int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
@@ -427,14 +433,22 @@ int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
     return EMP_OK;
 }
 
-// Buffers of the overlap phase for lists of up to x entries partitioned into 2^bb buckets.
+// Survivor set `set` with at least `cap` entries per sub-range (both sides).
+int ensure_lin_set(emp_ctx *c, int set, uint32_t cap) {
+    if (cap <= c->lin_cap[set]) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int side = 0; side < 2; ++side) {
+        int rc = realloc_entries(c, &c->d_lin[set][side], (uint64_t)SUBS * cap);
+        if (rc) return rc;
+    }
+    c->lin_cap[set] = cap;
+    return EMP_OK;
+}
+
+// Partition and join buffers for lists of up to x entries in 2^bb buckets.
 int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
     int rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (x > c->lin1_cap) {
-        if ((rc = realloc_entries(c, &c->d_lin[1], x))) return rc;
-        c->lin1_cap = x;
-    }
     const uint32_t nb = 1u << bb;
     if (nb > c->nb_alloc) {
         for (int s = 0; s < 2; ++s) {
@@ -488,7 +502,6 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
 int grow_after_abort(emp_ctx *c, uint64_t x, int bb) {
     c->slack *= 2.0;
     uint32_t sp = c->spill_cap * 4u;
-    c->spill_cap = 0;
     int rc = ensure_run_buffers(c, x, bb);
     if (rc) return rc;
     if (c->spill_cap < sp) {
@@ -503,12 +516,20 @@ int grow_after_abort(emp_ctx *c, uint64_t x, int bb) {
     return EMP_OK;
 }
 
-// Partitions linear list `side` of x entries (x also written to d_ctr->in[side]) into 2^bb buckets.
-int partition(emp_ctx *c, int side, uint64_t x, int bb) {
+// A list handed to a partition: one linear array (count in d_ctr->in[side]) or SUBS segments.
+struct ListIn {
+    const Entry *base;
+    const uint32_t *cnt;  // null: linear
+    uint32_t cap;
+};
+
+// Partitions list `side` (x entries) into 2^bb buckets.
+int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb) {
     const std::vector<Level> pl = plan_levels(x, bb, c->slack);
-    const Entry *in = c->d_lin[side];
-    const uint32_t *in_cnt = nullptr;
-    uint32_t in_cap = 0, groups = 1;
+    const Entry *in = li.base;
+    const uint32_t *in_cnt = li.cnt;
+    uint32_t in_cap = li.cap, groups = li.cnt ? (uint32_t)SUBS : 1u;
+    int keyed = 0;
     for (size_t i = 0; i < pl.size(); ++i) {
         const Level &l = pl[i];
         const bool fin = i + 1 == pl.size();
@@ -521,6 +542,7 @@ int partition(emp_ctx *c, int side, uint64_t x, int bb) {
         a.in_cnt = in_cnt;
         a.in_n = &c->d_ctr->in[side];
         a.in_cap = in_cap;
+        a.keyed = keyed;
         a.tiles_x = in_cnt ? std::max<uint32_t>(1, (in_cap + PART_TILE - 1) / PART_TILE)
                            : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((x + PART_TILE - 1) / PART_TILE, 1u << 16));
         a.out = out;
@@ -540,11 +562,12 @@ int partition(emp_ctx *c, int side, uint64_t x, int bb) {
         in_cnt = out_cnt;
         in_cap = l.cap;
         groups = 1u << bits_out;
+        keyed = 1;
     }
     return EMP_OK;
 }
 
-// Sets the partition inputs and clears the abort flag and spill counts.
+// Sets the linear partition inputs and clears the abort flag and spill counts.
 int reset_part(emp_ctx *c, uint32_t xs, uint32_t xp) {
     HIPCHK(c, launch_fill_u32(&c->d_ctr->abort, 0, 1, c->stream));
     HIPCHK(c, launch_fill_u32(&c->d_ctr->spill[0], 0, 2, c->stream));
@@ -553,16 +576,18 @@ int reset_part(emp_ctx *c, uint32_t xs, uint32_t xp) {
     return EMP_OK;
 }
 
-// Clears the counters the dedup / join kernels accumulate.
-int reset_join(emp_ctx *c) {
-    HIPCHK(c, launch_fill_u32((uint32_t *)&c->d_ctr->pool_cursor, 0, 2, c->stream));
+// Clears what a dedup / join launch accumulates: pool cursors, survivor set `set`, flags.
+int reset_join(emp_ctx *c, int set) {
+    HIPCHK(c, launch_fill_u32((uint32_t *)c->d_ctr->pool, 0, 2 * SUBS, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin[set][0][0], 0, 2 * SUBS, c->stream));
     HIPCHK(c, launch_fill_u32(&c->d_ctr->n_big, 0, 1, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin[0], 0, 2, c->stream));
+    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin_ovf, 0, 1, c->stream));
     return EMP_OK;
 }
 
-int ensure_pool(emp_ctx *c, uint64_t cap) {
-    if (cap <= c->pool_cap) return EMP_OK;
+// Edge pool of `per_sub` edges per sub-range and compacted edge arrays of SUBS * per_sub.
+int ensure_pool(emp_ctx *c, uint64_t per_sub) {
+    if (per_sub <= c->pool_cap) return EMP_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, c->d_pool_from);
     dfree(c, c->d_pool_to);
@@ -572,6 +597,7 @@ int ensure_pool(emp_ctx *c, uint64_t cap) {
     if (c->h_to) (void)hipHostFree(c->h_to);
     c->h_from = c->h_to = nullptr;
     c->pool_cap = 0;
+    const uint64_t cap = per_sub * SUBS;
     void *p = nullptr;
     int rc;
     if ((rc = dalloc(c, &p, cap * 4))) return rc;
@@ -584,7 +610,8 @@ int ensure_pool(emp_ctx *c, uint64_t cap) {
     c->d_out_to = (uint32_t *)p;
     HIPCHK(c, hipHostMalloc((void **)&c->h_from, cap * 4, hipHostMallocDefault));
     HIPCHK(c, hipHostMalloc((void **)&c->h_to, cap * 4, hipHostMallocDefault));
-    c->pool_cap = cap;
+    c->pool_cap = per_sub;
+    c->out_cap = cap;
     return EMP_OK;
 }
 
@@ -603,6 +630,23 @@ RoundArgs round_args(const emp_ctx *c, int t, bool active) {
 int read_counters(emp_ctx *c) {
     HIPCHK(c, hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
     return sync(c);
+}
+
+uint64_t sum_subs(const uint32_t *v) {
+    uint64_t s = 0;
+    for (int i = 0; i < SUBS; ++i) s += v[i];
+    return s;
+}
+
+// Edge totals of the last join: sum of the sub-range cursors, largest cursor.
+void pool_totals(const emp_ctx *c, uint64_t *sum, uint64_t *maxc) {
+    uint64_t s = 0, m = 0;
+    for (int i = 0; i < SUBS; ++i) {
+        s += c->h_ctr->pool[i];
+        m = std::max<uint64_t>(m, c->h_ctr->pool[i]);
+    }
+    *sum = s;
+    *maxc = m;
 }
 
 int ensure_big_scratch(emp_ctx *c, uint64_t need) {
@@ -626,8 +670,8 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
 }
 
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
-// global-memory sort + join (or dedup), one work-group per bucket.
-int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup) {
+// global-memory sort + join (or dedup), one work-group per bucket. Survivors go to set `set`.
+int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup, int set) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
@@ -677,16 +721,18 @@ int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup) {
     a.cap = BUCKET_CAP;
     a.chunks = c->d_chunk_tab;
     a.rn = rn;
-    a.s_lin = c->d_lin[0];
-    a.p_lin = c->d_lin[1];
-    a.s_lin_n = &c->d_ctr->lin[0];
-    a.p_lin_n = &c->d_ctr->lin[1];
+    a.s_lin = c->d_lin[set][0];
+    a.p_lin = c->d_lin[set][1];
+    a.s_lin_n = c->d_ctr->lin[set][0];
+    a.p_lin_n = c->d_ctr->lin[set][1];
+    a.lin_cap = c->lin_cap[set];
     a.pool_from = c->d_pool_from;
     a.pool_to = c->d_pool_to;
-    a.pool_cursor = &c->d_ctr->pool_cursor;
+    a.pool_cursor = c->d_ctr->pool;
     a.pool_cap = c->pool_cap;
     a.edge_off = c->d_edge_off;
     a.edge_cnt = c->d_edge_cnt;
+    a.lin_ovf = &c->d_ctr->lin_ovf;
     {
         Timed t(c, dedup ? "k_dedup_big" : "k_join_big", 0.0);
         HIPCHK(c, dedup ? launch_dedup_big(a, nbig, c->stream) : launch_join_big(a, nbig, c->stream));
@@ -726,42 +772,59 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
-    if ((rc = ensure_pool(c, std::max<uint64_t>(1u << 20, N0 / 4)))) return rc;
-    c->d_lin[0] = c->d_entries;
+    if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / 4 / SUBS)))) return rc;
+    if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
-    // the survivors with their round-0 suffix and prefix keys.
+    // the survivors with their round-0 suffix and prefix keys into survivor set 0.
     const RoundArgs r0 = round_args(c, 0, rounds > 0);
+    bool need_part = true;
     for (int attempt = 0;; ++attempt) {
-        if ((rc = reset_part(c, (uint32_t)N0, 0))) return rc;
-        if ((rc = partition(c, 0, N0, bbD))) return rc;
-        if ((rc = reset_join(c))) return rc;
+        if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
+        if (need_part) {
+            if ((rc = reset_part(c, (uint32_t)N0, 0))) return rc;
+            if ((rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD))) return rc;
+        }
+        if ((rc = reset_join(c, 0))) return rc;
         DedupArgs a{};
         a.in = c->d_fine[0];
         a.cnt = c->d_fine_cnt[0];
         a.chunks = c->d_chunk_tab;
         a.r0 = r0;
-        a.s_lin = c->d_lin[0];
-        a.p_lin = c->d_lin[1];
-        a.lin_n = &c->d_ctr->lin[0];
+        a.s_lin = c->d_lin[0][0];
+        a.p_lin = c->d_lin[0][1];
+        a.lin_n = c->d_ctr->lin[0][0];
+        a.lin_cap = c->lin_cap[0];
         a.big = c->d_big;
         a.n_big = &c->d_ctr->n_big;
         a.big_cap = BIG_CAP;
         a.abort = &c->d_ctr->abort;
+        a.lin_ovf = &c->d_ctr->lin_ovf;
         {
             Timed t(c, "k_dedup", 12.0 * N0);
             HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
         }
         if ((rc = read_counters(c))) return rc;
-        if (!c->h_ctr->abort) break;
-        if (attempt > 6) return fail(c, EMP_ERR_CAPACITY, "dedup partition keeps overflowing");
-        if ((rc = grow_after_abort(c, N0, bbD))) return rc;
+        if (c->h_ctr->abort) {
+            if ((rc = grow_after_abort(c, N0, bbD))) return rc;
+            need_part = true;
+            continue;
+        }
+        need_part = false;
+        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, r0, true, 0))) return rc;
+        if (c->h_ctr->lin_ovf) {
+            if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
+            continue;
+        }
+        break;
     }
-    if (c->h_ctr->n_big && (rc = big_path(c, bbD, r0, true))) return rc;
-    uint64_t St = c->h_ctr->lin[0], Pt = St;
+    // the prefix list starts as a copy of the suffix list's counts (same positions)
+    HIPCHK(c, hipMemcpyAsync(c->d_ctr->lin[0][1], c->d_ctr->lin[0][0], SUBS * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, c->stream));
+    uint64_t St = sum_subs(c->h_ctr->lin[0][0]), Pt = St;
     S.n_dedup = St;
 
-    int t = 0;
+    int cur = 0, t = 0;
     for (; t < rounds && St > 0; ++t) {
         if (Pt == 0) break;  // no record can follow: every later round has an empty prefix list
         if (St >= M / 2 || Pt >= M / 2)
@@ -773,20 +836,25 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
         const RoundArgs rn = round_args(c, t + 1, more);
+        const int nxt = cur ^ 1;
+        if ((rc = ensure_lin_set(c, nxt, lin_cap_for(std::max(St, Pt), 1.25)))) return rc;
         emp_round_stats rs{};
         rs.s = St;
         rs.p = Pt;
         rs.ovlp = L - 1 - t;
         rs.bucket_bits = bb;
-        bool need_part = true;
+        need_part = true;
+        uint64_t E = 0, maxc = 0;
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
             if (need_part) {
-                if ((rc = reset_part(c, (uint32_t)St, (uint32_t)Pt))) return rc;
-                if ((rc = partition(c, 0, St, bb))) return rc;
-                if ((rc = partition(c, 1, Pt, bb))) return rc;
+                if ((rc = reset_part(c, 0, 0))) return rc;
+                ListIn ls{c->d_lin[cur][0], c->d_ctr->lin[cur][0], c->lin_cap[cur]};
+                ListIn lp{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
+                if ((rc = partition(c, 0, ls, St, bb))) return rc;
+                if ((rc = partition(c, 1, lp, Pt, bb))) return rc;
             }
-            if ((rc = reset_join(c))) return rc;
+            if ((rc = reset_join(c, nxt))) return rc;
             JoinArgs a{};
             a.s_in = c->d_fine[0];
             a.p_in = c->d_fine[1];
@@ -794,13 +862,14 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             a.p_cnt = c->d_fine_cnt[1];
             a.chunks = c->d_chunk_tab;
             a.rn = rn;
-            a.s_lin = c->d_lin[0];
-            a.p_lin = c->d_lin[1];
-            a.s_lin_n = &c->d_ctr->lin[0];
-            a.p_lin_n = &c->d_ctr->lin[1];
+            a.s_lin = c->d_lin[nxt][0];
+            a.p_lin = c->d_lin[nxt][1];
+            a.s_lin_n = c->d_ctr->lin[nxt][0];
+            a.p_lin_n = c->d_ctr->lin[nxt][1];
+            a.lin_cap = c->lin_cap[nxt];
             a.pool_from = c->d_pool_from;
             a.pool_to = c->d_pool_to;
-            a.pool_cursor = &c->d_ctr->pool_cursor;
+            a.pool_cursor = c->d_ctr->pool;
             a.pool_cap = c->pool_cap;
             a.edge_off = c->d_edge_off;
             a.edge_cnt = c->d_edge_cnt;
@@ -808,6 +877,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             a.n_big = &c->d_ctr->n_big;
             a.big_cap = BIG_CAP;
             a.abort = &c->d_ctr->abort;
+            a.lin_ovf = &c->d_ctr->lin_ovf;
             {
                 Timed tm(c, "k_join", 0.0);
                 HIPCHK(c, launch_join(a, nbt, c->stream));
@@ -819,37 +889,36 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 continue;
             }
             need_part = false;
-            if (c->h_ctr->pool_cursor > c->pool_cap) {
-                if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
+            rs.big_buckets = (int32_t)c->h_ctr->n_big;
+            if (c->h_ctr->n_big && (rc = big_path(c, bb, rn, false, nxt))) return rc;
+            if (c->h_ctr->lin_ovf) {
+                if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
                 continue;
             }
-            rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big) {
-                if ((rc = big_path(c, bb, rn, false))) return rc;
-                if (c->h_ctr->pool_cursor > c->pool_cap) {
-                    if ((rc = ensure_pool(c, c->h_ctr->pool_cursor + c->h_ctr->pool_cursor / 4 + 1024))) return rc;
-                    continue;
-                }
+            pool_totals(c, &E, &maxc);
+            if (maxc > c->pool_cap) {
+                if ((rc = ensure_pool(c, maxc + maxc / 4 + 1024))) return rc;
+                continue;
             }
             break;
         }
         const DevCounters &k = *c->h_ctr;
-        const uint64_t E = k.pool_cursor;
+        const uint64_t Sn = sum_subs(k.lin[nxt][0]), Pn = sum_subs(k.lin[nxt][1]);
         rs.edges = E;
-        rs.s_after = k.lin[0];
-        rs.p_after = k.lin[1];
+        rs.s_after = Sn;
+        rs.p_after = Pn;
         if (c->opts.profile) {
             int id = kstat_id(c, "k_join");
-            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (k.lin[0] + k.lin[1]) : 0.0) + 8.0 * E;
+            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
         }
         // record (overlap.cu:422; written even when E == 0)
         if (E) {
             EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, nbt,
                             c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
-            Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
-            HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
-        }
-        if (E) {
+            {
+                Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
+                HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
+            }
             HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, E * 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, E * 4, hipMemcpyDeviceToHost, c->stream));
             if ((rc = sync(c))) return rc;
@@ -862,8 +931,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         S.n_records++;
         S.n_edges += E;
         c->rounds.push_back(rs);
-        St = k.lin[0];
-        Pt = k.lin[1];
+        St = Sn;
+        Pt = Pn;
+        cur = nxt;
     }
     S.n_rounds = t;
     return EMP_OK;
