@@ -131,10 +131,20 @@ struct BigArgs {
     uint32_t *lin_ovf;
 };
 
+// Up to 16 ranges of 32-bit words set to a value in one launch (per-round counter resets).
+constexpr int ZERO_RANGES = 16;
+struct ZeroArgs {
+    uint32_t *p[ZERO_RANGES];
+    uint32_t n[ZERO_RANGES];
+    uint32_t v[ZERO_RANGES];
+    int32_t count;
+};
+
 struct EdgeCopyArgs {
     const uint32_t *cnt;
     const unsigned long long *src_off;
     const unsigned long long *dst_off;
+    unsigned long long dst_base;  // first free slot of the output arena
     uint32_t nb;
     const uint32_t *pool_from, *pool_to;
     uint32_t *out_from, *out_to;
@@ -148,6 +158,7 @@ hipError_t launch_dedup_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,
                        uint64_t *out, hipStream_t s);
 

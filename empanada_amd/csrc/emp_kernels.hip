@@ -755,6 +755,11 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_zero(ZeroArgs a) {
+    for (int r = 0; r < a.count; ++r)
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.n[r]; i += gridDim.x * 256u) a.p[r][i] = a.v[r];
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
@@ -787,7 +792,7 @@ __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
     const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= a.nb) return;
     const uint32_t c = a.cnt[b];
-    const unsigned long long src = a.src_off[b], dst = a.dst_off[b];
+    const unsigned long long src = a.src_off[b], dst = a.dst_base + a.dst_off[b];
     for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
         a.out_from[dst + i] = a.pool_from[src + i];
         a.out_to[dst + i] = a.pool_to[src + i];
@@ -861,6 +866,15 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint64_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)(g < 4096 ? g : 4096)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(const ZeroArgs &a, hipStream_t s) {
+    uint32_t mx = 0;
+    for (int r = 0; r < a.count; ++r) mx = a.n[r] > mx ? a.n[r] : mx;
+    if (mx == 0) return hipSuccess;
+    uint32_t g = (mx + 255) / 256;
+    hipLaunchKernelGGL(k_zero, dim3(g < 1024 ? g : 1024), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

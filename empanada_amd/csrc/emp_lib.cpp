@@ -116,7 +116,10 @@ struct emp_ctx {
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
     uint64_t pool_cap = 0;  // edges per sub-range of the pool
-    uint64_t out_cap = 0;   // edges of the compacted arrays
+    uint64_t out_cap = 0;   // edges the output arena holds
+    uint64_t out_used = 0;  // edges in the arena not yet passed to the sink
+    struct Record { uint64_t n, off; int32_t ovlp; };
+    std::vector<Record> records;
     unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr;
     uint32_t *d_edge_cnt = nullptr;
     // large buckets
@@ -536,7 +539,6 @@ int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb) {
         const int bits_out = l.bits_in + l.dbits;
         Entry *out = fin ? c->d_fine[side] : c->d_inter[side][i & 1];
         uint32_t *out_cnt = fin ? c->d_fine_cnt[side] : c->d_inter_cnt[side][i & 1];
-        HIPCHK(c, launch_fill_u32(out_cnt, 0, 1ull << bits_out, c->stream));
         PartArgs a{};
         a.in = in;
         a.in_cnt = in_cnt;
@@ -567,21 +569,44 @@ int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb) {
     return EMP_OK;
 }
 
-// Sets the linear partition inputs and clears the abort flag and spill counts.
-int reset_part(emp_ctx *c, uint32_t xs, uint32_t xp) {
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->abort, 0, 1, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->spill[0], 0, 2, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->in[0], xs, 1, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->in[1], xp, 1, c->stream));
-    return EMP_OK;
+// Counter resets batched into one launch.
+struct ZeroList {
+    ZeroArgs a{};
+    void add(uint32_t *p, uint64_t n, uint32_t v = 0) {
+        a.p[a.count] = p;
+        a.n[a.count] = (uint32_t)n;
+        a.v[a.count] = v;
+        ++a.count;
+    }
+};
+
+// Counts a partition of list `side` into 2^bb buckets accumulates.
+void zero_partition(emp_ctx *c, ZeroList &z, int side, uint64_t x, int bb) {
+    const std::vector<Level> pl = plan_levels(x, bb, c->slack);
+    for (size_t i = 0; i < pl.size(); ++i) {
+        const int bits_out = pl[i].bits_in + pl[i].dbits;
+        z.add(i + 1 == pl.size() ? c->d_fine_cnt[side] : c->d_inter_cnt[side][i & 1], 1ull << bits_out);
+    }
 }
 
-// Clears what a dedup / join launch accumulates: pool cursors, survivor set `set`, flags.
-int reset_join(emp_ctx *c, int set) {
-    HIPCHK(c, launch_fill_u32((uint32_t *)c->d_ctr->pool, 0, 2 * SUBS, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin[set][0][0], 0, 2 * SUBS, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->n_big, 0, 1, c->stream));
-    HIPCHK(c, launch_fill_u32(&c->d_ctr->lin_ovf, 0, 1, c->stream));
+// Partition inputs (linear sizes), abort flag, spill counts.
+void zero_part_flags(emp_ctx *c, ZeroList &z, uint32_t xs, uint32_t xp) {
+    z.add(&c->d_ctr->abort, 1);
+    z.add(&c->d_ctr->spill[0], 2);
+    z.add(&c->d_ctr->in[0], 1, xs);
+    z.add(&c->d_ctr->in[1], 1, xp);
+}
+
+// What a dedup / join launch accumulates: pool cursors, survivor set `set`, flags.
+void zero_join(emp_ctx *c, ZeroList &z, int set) {
+    z.add((uint32_t *)c->d_ctr->pool, 2 * SUBS);
+    z.add(&c->d_ctr->lin[set][0][0], 2 * SUBS);
+    z.add(&c->d_ctr->n_big, 1);
+    z.add(&c->d_ctr->lin_ovf, 1);
+}
+
+int run_zero(emp_ctx *c, const ZeroList &z) {
+    HIPCHK(c, launch_zero(z.a, c->stream));
     return EMP_OK;
 }
 
@@ -591,11 +616,6 @@ int ensure_pool(emp_ctx *c, uint64_t per_sub) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, c->d_pool_from);
     dfree(c, c->d_pool_to);
-    dfree(c, c->d_out_from);
-    dfree(c, c->d_out_to);
-    if (c->h_from) (void)hipHostFree(c->h_from);
-    if (c->h_to) (void)hipHostFree(c->h_to);
-    c->h_from = c->h_to = nullptr;
     c->pool_cap = 0;
     const uint64_t cap = per_sub * SUBS;
     void *p = nullptr;
@@ -604,13 +624,48 @@ int ensure_pool(emp_ctx *c, uint64_t per_sub) {
     c->d_pool_from = (uint32_t *)p;
     if ((rc = dalloc(c, &p, cap * 4))) return rc;
     c->d_pool_to = (uint32_t *)p;
+    c->pool_cap = per_sub;
+    return EMP_OK;
+}
+
+// Passes every pending record to the sink (output.bin order) and empties the arena.
+int flush_records(emp_ctx *c, emp_record_sink sink, void *user) {
+    if (c->records.empty()) return EMP_OK;
+    if (c->out_used) {
+        HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, c->out_used * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, c->out_used * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    int rc = sync(c);
+    if (rc) return rc;
+    for (const auto &r : c->records) {
+        if (!sink) continue;
+        int src = sink(user, r.n, r.ovlp, c->h_from + r.off, c->h_to + r.off);
+        if (src) return fail(c, src, "record sink returned %d", src);
+    }
+    c->records.clear();
+    c->out_used = 0;
+    return EMP_OK;
+}
+
+// Output arena of at least `cap` edges (device + pinned host mirror); flushes first.
+int ensure_arena(emp_ctx *c, uint64_t cap, emp_record_sink sink, void *user) {
+    if (cap <= c->out_cap) return EMP_OK;
+    int rc = flush_records(c, sink, user);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->d_out_from);
+    dfree(c, c->d_out_to);
+    if (c->h_from) (void)hipHostFree(c->h_from);
+    if (c->h_to) (void)hipHostFree(c->h_to);
+    c->h_from = c->h_to = nullptr;
+    c->out_cap = 0;
+    void *p = nullptr;
     if ((rc = dalloc(c, &p, cap * 4))) return rc;
     c->d_out_from = (uint32_t *)p;
     if ((rc = dalloc(c, &p, cap * 4))) return rc;
     c->d_out_to = (uint32_t *)p;
     HIPCHK(c, hipHostMalloc((void **)&c->h_from, cap * 4, hipHostMallocDefault));
     HIPCHK(c, hipHostMalloc((void **)&c->h_to, cap * 4, hipHostMallocDefault));
-    c->pool_cap = per_sub;
     c->out_cap = cap;
     return EMP_OK;
 }
@@ -773,6 +828,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
     if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / 4 / SUBS)))) return rc;
+    c->records.clear();
+    c->out_used = 0;
+    if ((rc = ensure_arena(c, std::max<uint64_t>(1u << 24, N0), sink, user))) return rc;
     if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
@@ -781,11 +839,16 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     bool need_part = true;
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
-        if (need_part) {
-            if ((rc = reset_part(c, (uint32_t)N0, 0))) return rc;
-            if ((rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD))) return rc;
+        {
+            ZeroList z;
+            if (need_part) {
+                zero_part_flags(c, z, (uint32_t)N0, 0);
+                zero_partition(c, z, 0, N0, bbD);
+            }
+            zero_join(c, z, 0);
+            if ((rc = run_zero(c, z))) return rc;
         }
-        if ((rc = reset_join(c, 0))) return rc;
+        if (need_part && (rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD))) return rc;
         DedupArgs a{};
         a.in = c->d_fine[0];
         a.cnt = c->d_fine_cnt[0];
@@ -847,14 +910,22 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         uint64_t E = 0, maxc = 0;
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
+            {
+                ZeroList z;
+                if (need_part) {
+                    zero_part_flags(c, z, 0, 0);
+                    zero_partition(c, z, 0, St, bb);
+                    zero_partition(c, z, 1, Pt, bb);
+                }
+                zero_join(c, z, nxt);
+                if ((rc = run_zero(c, z))) return rc;
+            }
             if (need_part) {
-                if ((rc = reset_part(c, 0, 0))) return rc;
                 ListIn ls{c->d_lin[cur][0], c->d_ctr->lin[cur][0], c->lin_cap[cur]};
                 ListIn lp{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
                 if ((rc = partition(c, 0, ls, St, bb))) return rc;
                 if ((rc = partition(c, 1, lp, Pt, bb))) return rc;
             }
-            if ((rc = reset_join(c, nxt))) return rc;
             JoinArgs a{};
             a.s_in = c->d_fine[0];
             a.p_in = c->d_fine[1];
@@ -911,22 +982,19 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             int id = kstat_id(c, "k_join");
             c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
         }
-        // record (overlap.cu:422; written even when E == 0)
+        // record (overlap.cu:422; written even when E == 0), staged in the output arena
+        if (c->out_used + E > c->out_cap) {
+            if ((rc = flush_records(c, sink, user))) return rc;
+            if ((rc = ensure_arena(c, std::max<uint64_t>(E, c->out_cap), sink, user))) return rc;
+        }
         if (E) {
-            EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, nbt,
+            EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->out_used, nbt,
                             c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
-            {
-                Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
-                HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
-            }
-            HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, E * 4, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, E * 4, hipMemcpyDeviceToHost, c->stream));
-            if ((rc = sync(c))) return rc;
+            Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
+            HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
         }
-        if (sink) {
-            int src = sink(user, E, L - 1 - t, c->h_from, c->h_to);
-            if (src) return fail(c, src, "record sink returned %d", src);
-        }
+        c->records.push_back(emp_ctx::Record{E, c->out_used, L - 1 - t});
+        c->out_used += E;
         rs.records = 1;
         S.n_records++;
         S.n_edges += E;
@@ -936,7 +1004,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         cur = nxt;
     }
     S.n_rounds = t;
-    return EMP_OK;
+    return flush_records(c, sink, user);
 }
 
 int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to) {
@@ -1137,6 +1205,13 @@ void emp_kernel_clear(emp_ctx *c) {
 }
 
 void *emp_stream(const emp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+void emp_set_profile(emp_ctx *c, int32_t on) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    collect_timings(c);
+    c->opts.profile = on;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Stage entry points.

@@ -21,7 +21,7 @@ ERRORS = {
 EXPORTS = [
     "emp_opts_default", "emp_ctx_create", "emp_ctx_destroy", "emp_last_error", "emp_add_fastq", "emp_add_reads",
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
-    "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_stream", "emp_version",
+    "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
     "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll",
 ]
 
@@ -76,6 +76,7 @@ def lib():
         L.emp_kernel_info.argtypes = [vp, i32, ctypes.POINTER(cp), ctypes.POINTER(u64),
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.emp_kernel_clear.argtypes = [vp]
+        L.emp_set_profile.argtypes = [vp, i32]
         L.emp_stream.restype = vp
         L.emp_stream.argtypes = [vp]
         L.emp_version.restype = cp
@@ -207,6 +208,9 @@ class Context:
 
     def kernel_clear(self):
         lib().emp_kernel_clear(self._h)
+
+    def set_profile(self, on: bool):
+        lib().emp_set_profile(self._h, 1 if on else 0)
 
 
 def stage_fingerprints(reads: np.ndarray, device=0) -> np.ndarray:

@@ -117,6 +117,9 @@ int emp_kernel_info(const emp_ctx *ctx, int32_t i, const char **name, uint64_t *
                     double *bytes);
 void emp_kernel_clear(emp_ctx *ctx);
 
+/* Turns per-launch event timing on or off for later calls. */
+void emp_set_profile(emp_ctx *ctx, int32_t on);
+
 /* The context's stream (hipStream_t) and device, for callers that record their own events. */
 void *emp_stream(const emp_ctx *ctx);
 
