@@ -13,6 +13,9 @@ namespace emp {
 constexpr int BUCKET_CAP = 2048;
 constexpr int MATCH_CAP = 384;
 constexpr int JOIN_THREADS = 512;
+// Join work-groups loop over buckets with the next bucket's loads in flight (true), or take one
+// bucket each (false).
+constexpr bool JOIN_PERSIST = false;
 // Target mean bucket load: the partition fan-out is chosen so that lists average this many
 // entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
 constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
@@ -23,6 +26,17 @@ constexpr int PART_MAX_BITS = 8;
 // own cursor: work-group b reserves in sub-range b % SUBS, so no single counter serialises a
 // launch and, with work-groups dealt round-robin to the 8 XCDs, each cursor stays on one XCD.
 constexpr int SUBS = 256;
+// Survivor lists are also split by read-id range: the 512 reads of one 128-byte column line
+// (read index k, line k >> 9 of a column) share a range, ranges interleave line by line, and
+// range r owns sub-ranges r * SUBS_PER_RANGE ... The first partition pass of a round runs range r
+// on XCD r (work-group id modulo 8), so each XCD's L2 holds one eighth of the two columns the
+// round's regeneration gathers from.
+constexpr int NRANGE = 8;
+constexpr int SUBS_PER_RANGE = SUBS / NRANGE;
+__host__ __device__ __forceinline__ uint32_t read_range(uint32_t rid) { return (rid >> 10) & (NRANGE - 1); }
+__host__ __device__ __forceinline__ uint32_t range_sub(uint32_t range, uint32_t b) {
+    return range * SUBS_PER_RANGE + (b & (SUBS_PER_RANGE - 1));
+}
 
 // An entry that did not fit its final bucket, with the bucket it belongs to.
 struct SpillRec {
@@ -30,12 +44,13 @@ struct SpillRec {
     uint32_t bucket;
 };
 
-// Round description for regeneration: keys become those of length l-1 for round t.
+// Round description for regeneration: keys of length l become those of length l-1 for round t.
 struct RoundArgs {
-    int32_t active;    // 0: no regeneration (entries are only counted)
+    int32_t active;    // 0: no regeneration
     uint32_t col_t;    // column t
     uint32_t col_back; // column L-1-t
     uint32_t m1, m2;   // R1^(l-1) mod P1, R2^(l-1) mod P2, l = L - t
+    uint32_t L;        // read length: column stride of the column store
 };
 
 struct MapArgs {
@@ -44,7 +59,7 @@ struct MapArgs {
     int32_t L, Lpad;         // Lpad: LDS row stride (multiple of 4, odd number of dwords)
     uint32_t rows;           // reads per work-group (256, or 128 when 256 rows exceed 64 KiB of LDS)
     uint64_t k0;             // global index of the first read (multiple of 16)
-    uint32_t *const *chunks; // device table of column chunks
+    uint32_t *cols;          // column store: chunk c, column j at (c * L + j) * CHUNK_WORDS
     Entry *entries;          // linear entries, index 2k and 2k+1
     uint32_t *err;           // set to 1 when a byte is not ACGTacgt
 };
@@ -68,14 +83,16 @@ struct PartArgs {
     uint32_t *spill_n;
     uint32_t spill_cap;
     uint32_t *abort;         // set when an intermediate bucket or the spill area overflows
+    // First pass of a round: regenerate every entry (suffix or prefix rule) before bucketing.
+    RoundArgs rg;
+    int32_t sfx;
+    const uint32_t *cols;
 };
 
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
-    const uint32_t *const *chunks;
-    RoundArgs r0;
-    Entry *s_lin, *p_lin;    // survivors with round-0 keys, same positions in both lists
+    Entry *lin;              // kept entries (full-length keys), SUBS sub-ranges of lin_cap entries
     uint32_t *lin_n;         // [SUBS] counts of the sub-ranges
     uint32_t lin_cap;        // entries per sub-range
     uint32_t *big, *n_big;
@@ -87,9 +104,9 @@ struct DedupArgs {
 struct JoinArgs {
     const Entry *s_in, *p_in;
     const uint32_t *s_cnt, *p_cnt;
-    const uint32_t *const *chunks;
-    RoundArgs rn;            // regeneration for the next round
-    Entry *s_lin, *p_lin;    // survivors, SUBS sub-ranges of lin_cap entries
+    uint32_t nb;             // buckets
+    int32_t keep;            // write the survivors (a next round exists)
+    Entry *s_lin, *p_lin;    // survivors with this round's keys, SUBS sub-ranges of lin_cap entries
     uint32_t *s_lin_n, *p_lin_n;
     uint32_t lin_cap;
     uint32_t *pool_from, *pool_to;   // edge pool, SUBS sub-ranges of pool_cap edges
@@ -118,8 +135,7 @@ struct BigArgs {
     const uint32_t *s_spill_n, *p_spill_n;
     uint32_t spill_cap;
     uint32_t cap;
-    const uint32_t *const *chunks;
-    RoundArgs rn;
+    int32_t keep;                  // write the survivors
     Entry *s_lin, *p_lin;
     uint32_t *s_lin_n, *p_lin_n;
     uint32_t lin_cap;

@@ -12,13 +12,14 @@
 //
 // Kernels:
 //   k_map      transposition + forward/reverse-complement Rabin-Karp (fingerprint.cu:75-153)
-//   k_part     tile-sorted partition pass (at most 8 more key bits per pass)
-//   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222), then
-//              regenerate for round 0 as suffix and prefix entries (linear output)
+//   k_part     tile-sorted partition pass (at most 8 more key bits per pass); the first pass
+//              of a round regenerates its input for the round (overlap.cu:266-267, 684-690):
+//              one launch per list, so its column gathers touch a single column
+//   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222)
+//              (linear output; round 0 partitions it once as suffixes, once as prefixes)
 //   k_join     per bucket of round t: LDS hash join of suffixes against prefixes, edges of the
 //              matched key groups in (key, rid&1, rid) x (rid&1, rid) order, removal of used
-//              entries, regeneration of the survivors for round t+1 (linear output)
-//              (overlap.cu:258-290, 361-456, 557-710)
+//              entries (linear output, keys of round t) (overlap.cu:258-290, 361-456, 557-710)
 //   k_*_big    the same for buckets over capacity: global-memory sort + binary-search join
 //   k_edge_*   bucket-ordered compaction of the edges of a round
 #include "emp_kernels.h"
@@ -28,32 +29,37 @@
 namespace emp {
 
 // ---------------------------------------------------------------------------------------------
-// Column gather: 2-bit code of read k at column `col`.
-__device__ __forceinline__ uint32_t col_code(const uint32_t *const *__restrict__ chunks, uint32_t col, uint32_t k) {
-    const uint32_t *ch = chunks[k >> CHUNK_SHIFT];
-    uint32_t w = ch[(size_t)col * CHUNK_WORDS + ((k >> 4) & (CHUNK_WORDS - 1))];
-    return (w >> (2 * (k & 15))) & 3u;
+// Column gather: 2-bit code of read k at column `col` (column store of L columns per chunk).
+__device__ __forceinline__ uint32_t col_code(const uint32_t *__restrict__ cols, uint32_t L, uint32_t col, uint32_t k) {
+    const size_t w = ((size_t)(k >> CHUNK_SHIFT) * L + col) * CHUNK_WORDS + ((k >> 4) & (CHUNK_WORDS - 1));
+    return (cols[w] >> (2 * (k & 15))) & 3u;
 }
 
 // Base code of a live entry for the round described by `r`. Suffix list: even ids read column
 // t, odd ids the complement of column L-1-t; prefix list: the other way round (overlap.cu:266-267,
 // 684-690: rid/2 picks the read, odd ids are complemented).
-__device__ __forceinline__ uint32_t entry_code(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, uint32_t rid,
+__device__ __forceinline__ uint32_t entry_code(const uint32_t *__restrict__ cols, const RoundArgs &r, uint32_t rid,
                                                bool sfx) {
     uint32_t k = rid >> 1;
     bool odd = rid & 1;
     uint32_t col = (sfx != odd) ? r.col_t : r.col_back;
-    uint32_t c = col_code(chunks, col, k);
+    uint32_t c = col_code(cols, r.L, col, k);
     return odd ? (c ^ 2u) : c;
 }
 
-__device__ __forceinline__ void regen(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, Entry &e, bool sfx) {
-    uint32_t code = entry_code(chunks, r, e.rid, sfx);
+// Key of length l -> key of length l-1: NextSfx drops the first base of a suffix, PrevPfx the
+// last base of a prefix (fingerprint.cu:37-72); `code` is the 2-bit code of that base.
+__device__ __forceinline__ void regen_code(const RoundArgs &r, Entry &e, uint32_t code, bool sfx) {
     uint32_t ev = enc_of_code(code);
     if (sfx)
         next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
     else
         prev_pfx(e.h1, e.h2, ev);
+}
+
+__device__ __forceinline__ Entry load_stream(const Entry *p) {
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+    return Entry{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1), __builtin_nontemporal_load(q + 2)};
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -121,8 +127,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         uint32_t rmax = min(16u, nt - 16 * w);
         for (uint32_t i = 0; i < rmax; ++i) word |= ((uint32_t)(tile[(16 * w + i) * Lp + j] >> 1) & 3u) << (2 * i);
         uint64_t W = w0 + w;
-        uint32_t *ch = a.chunks[W >> (CHUNK_SHIFT - 4)];
-        ch[(size_t)j * CHUNK_WORDS + (W & (CHUNK_WORDS - 1))] = word;
+        a.cols[((W >> (CHUNK_SHIFT - 4)) * (uint32_t)L + j) * CHUNK_WORDS + (W & (CHUNK_WORDS - 1))] = word;
     }
 }
 
@@ -181,13 +186,26 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) { return ((uint64_t)e
 // ---------------------------------------------------------------------------------------------
 // Partition pass: each work-group takes tiles of PART_TILE entries of one input group, ranks them
 // by their next `dbits` key bits in LDS, reserves one output range per (tile, bucket) with a
-// single atomic, and writes each bucket's run contiguously.
+// single atomic, and writes each bucket's run contiguously. With a.rg.active (first pass of a
+// round) the entries are streamed in with non-temporal loads, so that L2 keeps the one column
+// the launch gathers from, and regenerated before they are bucketed (REGEN = a.rg.active).
+template <bool REGEN>
 __global__ void __launch_bounds__(256) k_part(PartArgs a) {
     constexpr int T = PART_TILE, PER = T / 256;
     __shared__ Entry buf[T];
     __shared__ uint8_t dig[T];
     __shared__ uint32_t hist[256], loc[256], base[256], sh[4];
-    const uint32_t g = blockIdx.x / a.tiles_x;
+    // Regenerating pass: the input groups are the SUBS sub-ranges of a survivor list; the
+    // work-groups of read range r (sub-ranges r * SUBS_PER_RANGE ...) all have id = r mod 8.
+    uint32_t g, tile0;
+    if (REGEN) {
+        const uint32_t x = blockIdx.x & (NRANGE - 1), j = blockIdx.x / NRANGE;
+        g = x * SUBS_PER_RANGE + j / a.tiles_x;
+        tile0 = j % a.tiles_x;
+    } else {
+        g = blockIdx.x / a.tiles_x;
+        tile0 = blockIdx.x % a.tiles_x;
+    }
     const uint32_t F = 1u << a.dbits;
     uint32_t n_g;
     const Entry *in;
@@ -198,17 +216,36 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
         n_g = *a.in_n;
         in = a.in;
     }
-    for (uint32_t tile = blockIdx.x % a.tiles_x; (size_t)tile * T < n_g; tile += a.tiles_x) {
+    for (uint32_t tile = tile0; (size_t)tile * T < n_g; tile += a.tiles_x) {
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
         hist[threadIdx.x] = 0;
         __syncthreads();
         Entry e[PER];
         uint32_t r[PER], d[PER];
+        if (REGEN) {
+            uint32_t code[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                uint32_t i = threadIdx.x + 256 * q;
+                if (i < nt) {
+                    e[q] = load_stream(in + t0 + i);
+                    code[q] = entry_code(a.cols, a.rg, e[q].rid, a.sfx);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PER; ++q)
+                if (threadIdx.x + 256 * q < nt) regen_code(a.rg, e[q], code[q], a.sfx);
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                uint32_t i = threadIdx.x + 256 * q;
+                if (i < nt) e[q] = in[t0 + i];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             uint32_t i = threadIdx.x + 256 * q;
             if (i < nt) {
-                e[q] = in[t0 + i];
                 d[q] = (e[q].h1 >> a.out_shift) & (F - 1);
                 r[q] = atomicAdd(&hist[d[q]], 1u);
             }
@@ -251,8 +288,7 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
 
 // ---------------------------------------------------------------------------------------------
 // Dedup: per bucket keep the entry with the smallest id of each key (stable sort by key of an
-// id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306), then
-// regenerate it for round 0 as a suffix and as a prefix (overlap.cu:266-267).
+// id-ordered list followed by unique_by_key, overlap.cu:212-213,222 / cqutils.cuh:271-306).
 template <int CAP, int NT>
 __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
@@ -260,8 +296,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     __shared__ uint32_t h1s[CAP], h2s[CAP], rids[CAP];
     __shared__ uint32_t head[CAP];
     __shared__ uint16_t nxt[CAP];
-    __shared__ uint32_t sh[NT / 64];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t rcnt[NRANGE], rbase[NRANGE];
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.cnt[b];
@@ -275,6 +310,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     if (n == 0) return;
     const Entry *in = a.in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) head[i] = EMPTY;
+    if (threadIdx.x < NRANGE) rcnt[threadIdx.x] = 0;
     for (uint32_t i = threadIdx.x; i < n; i += NT) {
         Entry e = in[i];
         h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
@@ -282,7 +318,8 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
     __syncthreads();
-    uint32_t keepmask = 0, kept = 0;
+    // kept entries: a slot in their read range's run of this work-group (one reservation per range)
+    uint32_t keepmask = 0, slot[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         uint32_t i = threadIdx.x + NT * q;
@@ -291,31 +328,25 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
             bool keep = true;
             for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY && keep; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j]))
                 if (h2s[j] == h2 && h1s[j] == h1 && rids[j] < rid) keep = false;
-            if (keep) { keepmask |= 1u << q; ++kept; }
+            if (keep) {
+                keepmask |= 1u << q;
+                slot[q] = atomicAdd(&rcnt[read_range(rid)], 1u);
+            }
         }
     }
-    uint32_t tot;
-    const uint32_t off = block_excl_scan<NT>(kept, sh, tot);
-    const uint32_t sub = b & (SUBS - 1);
-    if (threadIdx.x == 0) {
-        uint32_t c = tot ? atomicAdd(a.lin_n + sub, tot) : 0u;
-        if (c + tot > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
-        s_base = c;
+    __syncthreads();
+    if (threadIdx.x < NRANGE) {
+        const uint32_t r = threadIdx.x, cnt = rcnt[r], sub = range_sub(r, b);
+        uint32_t c = cnt ? atomicAdd(a.lin_n + sub, cnt) : 0u;
+        if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
+        rbase[r] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
     }
     __syncthreads();
-    if (a.r0.active && s_base != 0xFFFFFFFFu) {
-        uint32_t pos = sub * a.lin_cap + s_base + off;
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            if (keepmask & (1u << q)) {
-                uint32_t i = threadIdx.x + NT * q;
-                Entry s = {h1s[i], h2s[i], rids[i]}, p = s;
-                regen(a.chunks, a.r0, s, true);
-                regen(a.chunks, a.r0, p, false);
-                a.s_lin[pos] = s;
-                a.p_lin[pos] = p;
-                ++pos;
-            }
+    for (int q = 0; q < PER; ++q) {
+        if (keepmask & (1u << q)) {
+            const uint32_t i = threadIdx.x + NT * q, base = rbase[read_range(rids[i])];
+            if (base != 0xFFFFFFFFu) a.lin[base + slot[q]] = Entry{h1s[i], h2s[i], rids[i]};
         }
     }
 }
@@ -325,8 +356,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
 //
 // LDS: the prefix entries (12 B each), a used flag per prefix, and one region that holds the hash
 // table (chain heads + links) while probing and the matched records afterwards. The suffix
-// entries stay in registers. The base each entry needs for the next round is gathered right
-// after the entries are loaded, so its latency overlaps the hash phase.
+// entries stay in registers.
 template <int CAP, int MCAP, int NT>
 struct JoinShared {
     uint32_t pk1[CAP], pk2[CAP], prid[CAP];
@@ -342,55 +372,57 @@ struct JoinShared {
         } m;
     } u;
     uint32_t sh[NT / 64];
-    uint32_t ms, mp, ovf, bs, bp;
+    uint32_t ms, mp, ovf, etot;
+    uint32_t rcnt[2 * NRANGE], rbase[2 * NRANGE];  // survivors per (side, read range), their slots
     unsigned long long base;
 };
 
-// Base code for the next round of a live entry (rid), or 0 when there is no next round.
-__device__ __forceinline__ uint32_t next_code(const uint32_t *const *__restrict__ chunks, const RoundArgs &r, uint32_t rid,
-                                              bool sfx) {
-    return r.active ? entry_code(chunks, r, rid, sfx) : 0u;
+// Position of record i among n records (key, rank) ordered by (key, rank): the number of records
+// smaller than it. (key, rank) pairs are distinct, so the positions of 0..n-1 are a permutation.
+// Eight independent LDS reads per step (the same address for the whole wave: broadcasts).
+__device__ __forceinline__ uint32_t rank_of(const uint64_t *key, const uint32_t *rank, uint32_t n, uint32_t i,
+                                            uint64_t &k, uint32_t &r) {
+    k = key[i];
+    r = rank[i];
+    uint32_t pos = 0, j = 0;
+    for (; j + 8 <= n; j += 8) {
+        uint64_t kk[8];
+        uint32_t rr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { kk[u] = key[j + u]; rr[u] = rank[j + u]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pos += (uint32_t)(kk[u] < k) | ((uint32_t)(kk[u] == k) & (uint32_t)(rr[u] < r));
+    }
+    for (; j < n; ++j) {
+        const uint64_t kj = key[j];
+        pos += (uint32_t)(kj < k) | ((uint32_t)(kj == k) & (uint32_t)(rank[j] < r));
+    }
+    return pos;
 }
 
-__device__ __forceinline__ void regen_code(const RoundArgs &r, Entry &e, uint32_t code, bool sfx) {
-    uint32_t ev = enc_of_code(code);
-    if (sfx)
-        next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
-    else
-        prev_pfx(e.h1, e.h2, ev);
-}
-
-// Sorts n <= NT records (key, rank[, payload]) in place by (key, rank): each thread counts the
-// records smaller than its own, then stores its record at that position. (key, rank) pairs are
-// distinct, so the positions are a permutation.
-template <int NT>
-__device__ __forceinline__ void rank_sort(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n, uint64_t &k,
-                                          uint32_t &r, uint32_t &pv, uint32_t &pos) {
-    const uint32_t i = threadIdx.x;
-    pos = 0;
-    if (i < n) {
-        k = key[i];
-        r = rank[i];
-        pv = pay ? pay[i] : 0u;
-        for (uint32_t j = 0; j < n; ++j) {
-            const uint64_t kj = key[j];
-            pos += (kj < k) || (kj == k && rank[j] < r);
-        }
+// Loads the suffix and prefix entries of bucket b (nothing for a bucket over capacity).
+template <int CAP, int NT>
+__device__ __forceinline__ void load_bucket(const JoinArgs &a, uint32_t b, uint32_t ns, uint32_t np, Entry (&se)[CAP / NT],
+                                            Entry (&pe)[CAP / NT]) {
+    const Entry *sin = a.s_in + (size_t)b * CAP;
+    const Entry *pin = a.p_in + (size_t)b * CAP;
+    const bool fit = ns <= (uint32_t)CAP && np <= (uint32_t)CAP;
+#pragma unroll
+    for (int q = 0; q < CAP / NT; ++q) {
+        const uint32_t i = threadIdx.x + NT * q;
+        if (fit && i < ns) se[q] = sin[i];
+        if (fit && i < np) pe[q] = pin[i];
     }
 }
 
+// Join of one bucket: suffixes in registers se[] (slot threadIdx.x + NT * q), prefixes already
+// in S.pk1/pk2/prid with S.used cleared. Buckets over capacity, or with more matched records
+// than MCAP, are handed to the large-bucket path.
 template <int CAP, int MCAP, int NT>
-__global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
+__device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t ns,
+                                            uint32_t np, const Entry (&se)[CAP / NT]) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
     constexpr int PER = CAP / NT;
-    static_assert(MCAP <= NT, "one matched record per thread in the rank sort");
-    using SM = JoinShared<CAP, MCAP, NT>;
-    static_assert(sizeof(((SM *)nullptr)->u.m) <= sizeof(((SM *)nullptr)->u.h), "matched records fit the hash table");
-    __shared__ SM S;
-
-    if (*a.abort) return;
-    const uint32_t b = blockIdx.x;
-    const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
     if (ns > (uint32_t)CAP || np > (uint32_t)CAP) {
         if (threadIdx.x == 0) {
             uint32_t i = atomicAdd(a.n_big, 1u);
@@ -399,26 +431,12 @@ __global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
         }
         return;
     }
-    if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; }
-    const Entry *sin = a.s_in + (size_t)b * CAP;
-    const Entry *pin = a.p_in + (size_t)b * CAP;
+    if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; }
+    if (threadIdx.x < 2 * NRANGE) S.rcnt[threadIdx.x] = 0;
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
-    Entry se[PER];
-    uint32_t sc[PER], scode[PER], pcode[PER];
+    uint32_t sc[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const uint32_t i = threadIdx.x + NT * q;
-        if (i < np) {
-            Entry e = pin[i];
-            S.pk1[i] = e.h1; S.pk2[i] = e.h2; S.prid[i] = e.rid; S.used[i] = 0;
-            pcode[q] = next_code(a.chunks, a.rn, e.rid, false);
-        }
-        if (i < ns) {
-            se[q] = sin[i];
-            scode[q] = next_code(a.chunks, a.rn, se[q].rid, true);
-        }
-        sc[q] = 0;
-    }
+    for (int q = 0; q < PER; ++q) sc[q] = 0;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -446,29 +464,48 @@ __global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
         }
     }
     __syncthreads();  // the hash table is dead from here on; its space holds the matched records
+
+    // Matched records (key, rank) of both sides, the bucket's edge total, and a slot for every
+    // survivor (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710) in its
+    // run of (side, read range).
+    uint32_t smask = 0, pmask = 0, slot[PER], ecount = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
-        if (i < ns && sc[q]) {
-            uint32_t m = atomicAdd(&S.ms, 1u);
-            if (m < (uint32_t)MCAP) {
-                S.u.m.msk[m] = ((uint64_t)se[q].h1 << 32) | se[q].h2;
-                S.u.m.msr[m] = rid_rank(se[q].rid);
-                S.u.m.msc[m] = sc[q];
+        slot[q] = 0;
+        if (i < ns) {
+            if (sc[q]) {
+                ecount += sc[q];
+                uint32_t m = atomicAdd(&S.ms, 1u);
+                if (m < (uint32_t)MCAP) {
+                    S.u.m.msk[m] = ((uint64_t)se[q].h1 << 32) | se[q].h2;
+                    S.u.m.msr[m] = rid_rank(se[q].rid);
+                    S.u.m.msc[m] = sc[q];
+                } else {
+                    S.ovf = 1;
+                }
             } else {
-                S.ovf = 1;
+                smask |= 1u << q;
+                slot[q] = atomicAdd(&S.rcnt[read_range(se[q].rid)], 1u);
             }
         }
-        if (i < np && S.used[i]) {
-            uint32_t m = atomicAdd(&S.mp, 1u);
-            if (m < (uint32_t)MCAP) {
-                S.u.m.mpk[m] = ((uint64_t)S.pk1[i] << 32) | S.pk2[i];
-                S.u.m.mpr[m] = rid_rank(S.prid[i]);
+        if (i < np) {
+            const uint32_t rid = S.prid[i];
+            if (S.used[i]) {
+                uint32_t m = atomicAdd(&S.mp, 1u);
+                if (m < (uint32_t)MCAP) {
+                    S.u.m.mpk[m] = ((uint64_t)S.pk1[i] << 32) | S.pk2[i];
+                    S.u.m.mpr[m] = rid_rank(rid);
+                } else {
+                    S.ovf = 1;
+                }
             } else {
-                S.ovf = 1;
+                pmask |= 1u << q;
+                slot[q] |= atomicAdd(&S.rcnt[NRANGE + read_range(rid)], 1u) << 16;
             }
         }
     }
+    if (ecount) atomicAdd(&S.etot, ecount);
     __syncthreads();
     if (S.ovf) {
         if (threadIdx.x == 0) {
@@ -478,32 +515,57 @@ __global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
         }
         return;
     }
-    const uint32_t nms = S.ms, nmp = S.mp;
+    const uint32_t nms = S.ms, nmp = S.mp, tot = S.etot;
+
+    // Reservations, issued now and read back after the sort so that their latency overlaps it:
+    // thread k < 16 reserves the survivor run (side k / 8, read range k % 8), thread 16 the edges.
+    unsigned long long resv = 0;
+    if (threadIdx.x < 2 * NRANGE) {
+        const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
+        resv = cnt ? atomicAdd((k < NRANGE ? a.s_lin_n : a.p_lin_n) + sub, cnt) : 0u;
+    } else if (threadIdx.x == 2 * NRANGE && tot) {
+        resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)), (unsigned long long)tot);
+    }
 
     // Edges of the bucket: order matched suffixes and prefixes by (key, rank), then emit the
     // cross product of each key group skipping same-read pairs (overlap.cu:557-601).
     if (nms) {
+        // one record per thread: suffix t, or prefix t - nms (both sides at once when they fit)
+        const bool both = nms + nmp <= (uint32_t)NT;
         uint64_t k1 = 0, k2 = 0;
-        uint32_t r1 = 0, r2 = 0, c1 = 0, c2 = 0, p1, p2;
-        rank_sort<NT>(S.u.m.msk, S.u.m.msr, S.u.m.msc, nms, k1, r1, c1, p1);
-        rank_sort<NT>(S.u.m.mpk, S.u.m.mpr, nullptr, nmp, k2, r2, c2, p2);
+        uint32_t r1 = 0, r2 = 0, c1 = 0, p1 = 0, p2 = 0;
+        const uint32_t t = threadIdx.x;
+        if (t < nms) {
+            p1 = rank_of(S.u.m.msk, S.u.m.msr, nms, t, k1, r1);
+            c1 = S.u.m.msc[t];
+        } else if (both && t - nms < nmp) {
+            p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t - nms, k2, r2);
+        }
+        if (!both && t < nmp) p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t, k2, r2);
+        const bool hasp = both ? (t >= nms && t - nms < nmp) : t < nmp;
         __syncthreads();
-        if (threadIdx.x < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
-        if (threadIdx.x < nmp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
-        __syncthreads();
+        if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
+        if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
+    }
+    if (threadIdx.x < 2 * NRANGE) {
+        const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
+        uint32_t c = (uint32_t)resv;
+        if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
+        S.rbase[k] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
+    } else if (threadIdx.x == 2 * NRANGE) {
+        const uint32_t sub = b & (SUBS - 1);
+        S.base = tot && resv + tot <= a.pool_cap ? sub * a.pool_cap + resv : ~0ull;
+        a.edge_off[b] = tot ? sub * a.pool_cap + resv : 0ull;
+        a.edge_cnt[b] = tot;
+    }
+    __syncthreads();
+    if (nms) {
         const uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
         const uint32_t e0 = i0 < nms ? S.u.m.msc[i0] : 0, e1 = i1 < nms ? S.u.m.msc[i1] : 0;
-        uint32_t tot;
-        const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, tot);
+        uint32_t stot;
+        const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, stot);
         if (i0 < nms) S.u.m.msc[i0] = off;
         if (i1 < nms) S.u.m.msc[i1] = off + e0;
-        if (threadIdx.x == 0) {
-            const uint32_t sub = b & (SUBS - 1);
-            unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
-            S.base = c + tot <= a.pool_cap ? sub * a.pool_cap + c : ~0ull;
-            a.edge_off[b] = sub * a.pool_cap + c;
-            a.edge_cnt[b] = tot;
-        }
         __syncthreads();
         const unsigned long long base = S.base;
         if (base != ~0ull) {
@@ -527,52 +589,81 @@ __global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
                 }
             }
         }
-    } else if (threadIdx.x == 0) {
-        a.edge_cnt[b] = 0;
-        a.edge_off[b] = 0;
     }
 
-    // Survivors: suffixes with no edge, prefixes never a target (overlap.cu:424-429,694-710),
-    // regenerated for the next round and appended to the linear lists (one reservation per list).
-    uint32_t smask = 0, pmask = 0, ks = 0, kp = 0;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const uint32_t i = threadIdx.x + NT * q;
-        if (i < ns && sc[q] == 0) { smask |= 1u << q; ++ks; }
-        if (i < np && !S.used[i]) { pmask |= 1u << q; ++kp; }
-    }
-    uint32_t tt;
-    const uint32_t o2 = block_excl_scan<NT>(ks | (kp << 16), S.sh, tt);
-    const uint32_t ts = tt & 0xFFFFu, tp = tt >> 16;
-    const uint32_t sub = b & (SUBS - 1);
-    if (threadIdx.x == 0) {
-        uint32_t cs = ts ? atomicAdd(a.s_lin_n + sub, ts) : 0u;
-        uint32_t cp = tp ? atomicAdd(a.p_lin_n + sub, tp) : 0u;
-        if (cs + ts > a.lin_cap || cp + tp > a.lin_cap) atomicOr(a.lin_ovf, 1u);
-        S.bs = cs;
-        S.bp = cp;
-    }
-    __syncthreads();
-    if (a.rn.active && S.bs + ts <= a.lin_cap && S.bp + tp <= a.lin_cap) {
-        uint32_t pos = sub * a.lin_cap + S.bs + (o2 & 0xFFFFu);
+    // Survivors into their runs; the next round's first partition pass regenerates them.
+    if (a.keep) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             if (smask & (1u << q)) {
-                Entry e = se[q];
-                regen_code(a.rn, e, scode[q], true);
-                a.s_lin[pos++] = e;
+                const uint32_t base = S.rbase[read_range(se[q].rid)];
+                if (base != 0xFFFFFFFFu) a.s_lin[base + (slot[q] & 0xFFFFu)] = se[q];
             }
-        }
-        pos = sub * a.lin_cap + S.bp + (o2 >> 16);
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
             if (pmask & (1u << q)) {
-                const uint32_t i = threadIdx.x + NT * q;
-                Entry e = {S.pk1[i], S.pk2[i], S.prid[i]};
-                regen_code(a.rn, e, pcode[q], false);
-                a.p_lin[pos++] = e;
+                const uint32_t i = threadIdx.x + NT * q, rid = S.prid[i];
+                const uint32_t base = S.rbase[NRANGE + read_range(rid)];
+                if (base != 0xFFFFFFFFu) a.p_lin[base + (slot[q] >> 16)] = Entry{S.pk1[i], S.pk2[i], rid};
             }
         }
+    }
+}
+
+// Prefix entries of a bucket (registers) into LDS, used flags cleared.
+template <int CAP, int MCAP, int NT>
+__device__ __forceinline__ void stage_prefixes(JoinShared<CAP, MCAP, NT> &S, uint32_t np, const Entry (&pe)[CAP / NT]) {
+    if (np > (uint32_t)CAP) return;
+#pragma unroll
+    for (int q = 0; q < CAP / NT; ++q) {
+        const uint32_t i = threadIdx.x + NT * q;
+        if (i < np) { S.pk1[i] = pe[q].h1; S.pk2[i] = pe[q].h2; S.prid[i] = pe[q].rid; S.used[i] = 0; }
+    }
+}
+
+// Join kernel. PERSIST: work-group g takes buckets g, g + G, g + 2G, ... (G = grid size); while
+// it joins bucket b, the entries of bucket b + G and the counts of bucket b + 2G are in flight.
+// Otherwise one work-group per bucket.
+template <int CAP, int MCAP, int NT, bool PERSIST>
+__global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
+    constexpr int PER = CAP / NT;
+    static_assert(MCAP <= NT, "one matched record per thread in the rank sort");
+    using SM = JoinShared<CAP, MCAP, NT>;
+    static_assert(sizeof(((SM *)nullptr)->u.m) <= sizeof(((SM *)nullptr)->u.h), "matched records fit the hash table");
+    __shared__ SM S;
+
+    if (*a.abort) return;
+    const uint32_t G = gridDim.x, nb = a.nb;
+    uint32_t b = blockIdx.x;
+    if (b >= nb) return;
+    if (!PERSIST) {
+        const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+        Entry se[PER], pe[PER];
+        load_bucket<CAP, NT>(a, b, ns, np, se, pe);
+        stage_prefixes<CAP, MCAP, NT>(S, np, pe);
+        join_bucket<CAP, MCAP, NT>(a, S, b, ns, np, se);
+        return;
+    }
+    uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+    uint32_t b1 = b + G, ns1 = 0, np1 = 0;
+    if (b1 < nb) { ns1 = a.s_cnt[b1]; np1 = a.p_cnt[b1]; }
+    Entry se[PER], pe[PER];
+    load_bucket<CAP, NT>(a, b, ns, np, se, pe);
+    stage_prefixes<CAP, MCAP, NT>(S, np, pe);
+    for (;;) {
+        Entry sn[PER], pn[PER];
+        const uint32_t b2 = b1 + G;
+        uint32_t ns2 = 0, np2 = 0;
+        if (b1 < nb) {
+            load_bucket<CAP, NT>(a, b1, ns1, np1, sn, pn);
+            if (b2 < nb) { ns2 = a.s_cnt[b2]; np2 = a.p_cnt[b2]; }
+        }
+        join_bucket<CAP, MCAP, NT>(a, S, b, ns, np, se);
+        if (b1 >= nb) break;
+        __syncthreads();  // LDS of bucket b is dead
+        stage_prefixes<CAP, MCAP, NT>(S, np1, pn);
+        b = b1; ns = ns1; np = np1;
+        b1 = b2; ns1 = ns2; np1 = np2;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) se[q] = sn[q];
     }
 }
 
@@ -708,11 +799,11 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         }
         if (e == o) {
             Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
-            uint32_t pos = atomicAdd(a.s_lin_n + sub, 1u);
+            const uint32_t rs = range_sub(read_range(srid), b);
+            uint32_t pos = atomicAdd(a.s_lin_n + rs, 1u);
             if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else if (a.rn.active) {
-                regen(a.chunks, a.rn, en, true);
-                a.s_lin[sub * a.lin_cap + pos] = en;
+            else if (a.keep) {
+                a.s_lin[rs * a.lin_cap + pos] = en;
             }
         }
     }
@@ -720,11 +811,11 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         if (!PA[j]) {
             const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
             Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
-            uint32_t pos = atomicAdd(a.p_lin_n + sub, 1u);
+            const uint32_t rs = range_sub(read_range(prd), b);
+            uint32_t pos = atomicAdd(a.p_lin_n + rs, 1u);
             if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else if (a.rn.active) {
-                regen(a.chunks, a.rn, en, false);
-                a.p_lin[sub * a.lin_cap + pos] = en;
+            else if (a.keep) {
+                a.p_lin[rs * a.lin_cap + pos] = en;
             }
         }
     }
@@ -741,15 +832,11 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if (i == 0 || K[i] != K[i - 1]) {
             Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
-            const uint32_t sub = b & (SUBS - 1);
+            const uint32_t sub = range_sub(read_range(R[i]), b);
             uint32_t pos = atomicAdd(a.s_lin_n + sub, 1u);
             if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else if (a.rn.active) {
-                Entry p = e;
-                regen(a.chunks, a.rn, e, true);
-                regen(a.chunks, a.rn, p, false);
+            else {
                 a.s_lin[sub * a.lin_cap + pos] = e;
-                a.p_lin[sub * a.lin_cap + pos] = p;
             }
         }
     }
@@ -828,7 +915,10 @@ hipError_t launch_map(const MapArgs &a, hipStream_t s) {
 hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {
     uint64_t grid = (uint64_t)groups * a.tiles_x;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_part, dim3((uint32_t)grid), dim3(256), 0, s, a);
+    if (a.rg.active)
+        hipLaunchKernelGGL(k_part<true>, dim3((uint32_t)grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_part<false>, dim3((uint32_t)grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -837,8 +927,27 @@ hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Work-groups of `kernel` that fit on the current device at once (CUs x resident per CU).
+static uint32_t resident_groups(const void *kernel, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+    return (uint32_t)(cus * per);
+}
+
 hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s) {
-    hipLaunchKernelGGL((k_join<BUCKET_CAP, MATCH_CAP, JOIN_THREADS>), dim3(nb), dim3(JOIN_THREADS), 0, s, a);
+    constexpr auto kern = k_join<BUCKET_CAP, MATCH_CAP, JOIN_THREADS, JOIN_PERSIST>;
+    static uint32_t resident[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!resident[dev]) resident[dev] = resident_groups(reinterpret_cast<const void *>(kern), JOIN_THREADS);
+    JoinArgs b = a;
+    b.nb = nb;
+    const uint32_t grid = JOIN_PERSIST ? (nb < resident[dev] ? nb : resident[dev]) : nb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(JOIN_THREADS), 0, s, b);
     return hipGetLastError();
 }
 

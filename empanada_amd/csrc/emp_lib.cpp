@@ -85,9 +85,8 @@ struct emp_ctx {
     // reads / columns / entries
     int L = 0;
     uint64_t n = 0;  // reads mapped (submitted to the device)
-    std::vector<uint32_t *> chunks;
-    uint32_t **d_chunk_tab = nullptr;
-    size_t chunk_tab_cap = 0;
+    uint32_t *d_cols = nullptr;  // column store: chunk c, column j at (c * L + j) * CHUNK_WORDS
+    size_t cols_chunks = 0;      // chunks allocated
     Entry *d_entries = nullptr;
     uint64_t entries_cap = 0;
     uint32_t *d_err = nullptr;
@@ -242,31 +241,23 @@ int sync(emp_ctx *c) {
     return EMP_OK;
 }
 
-int ensure_chunks(emp_ctx *c, uint64_t n_total) {
-    size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
-    if (need <= c->chunks.size()) return EMP_OK;
-    while (c->chunks.size() < need) {
-        void *p = nullptr;
-        int rc = dalloc(c, &p, (size_t)c->L * CHUNK_WORDS * sizeof(uint32_t));
-        if (rc) return rc;
-        c->chunks.push_back((uint32_t *)p);
+// Column store for n_total reads: one allocation of whole chunks, doubled (and copied) when it
+// fills, so that a column gather is a single load.
+int ensure_cols(emp_ctx *c, uint64_t n_total) {
+    const size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
+    if (need <= c->cols_chunks) return EMP_OK;
+    const size_t cap = std::max(need, c->cols_chunks * 2);
+    const size_t chunk_bytes = (size_t)c->L * CHUNK_WORDS * sizeof(uint32_t);
+    void *p = nullptr;
+    int rc = dalloc(c, &p, cap * chunk_bytes);
+    if (rc) return rc;
+    if (c->d_cols) {
+        HIPCHK(c, hipMemcpyAsync(p, c->d_cols, c->cols_chunks * chunk_bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c, c->d_cols);
     }
-    if (c->chunks.size() > c->chunk_tab_cap) {
-        size_t cap = std::max<size_t>(64, c->chunk_tab_cap * 2);
-        while (cap < c->chunks.size()) cap *= 2;
-        if (c->d_chunk_tab) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            dfree(c, c->d_chunk_tab);
-        }
-        void *p = nullptr;
-        int rc = dalloc(c, &p, cap * sizeof(uint32_t *));
-        if (rc) return rc;
-        c->d_chunk_tab = (uint32_t **)p;
-        c->chunk_tab_cap = cap;
-    }
-    HIPCHK(c, hipMemcpyAsync(c->d_chunk_tab, c->chunks.data(), c->chunks.size() * sizeof(uint32_t *),
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->d_cols = (uint32_t *)p;
+    c->cols_chunks = cap;
     return EMP_OK;
 }
 
@@ -289,8 +280,9 @@ int ensure_entries(emp_ctx *c, uint64_t n_entries) {
 
 void free_reads_storage(emp_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
-    for (auto p : c->chunks) dfree(c, p);
-    c->chunks.clear();
+    dfree(c, c->d_cols);
+    c->d_cols = nullptr;
+    c->cols_chunks = 0;
     for (int i = 0; i < 2; ++i) {
         dfree(c, c->d_batch[i]);
         c->d_batch[i] = nullptr;
@@ -317,7 +309,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
     if (nreads == 0) return EMP_OK;
     if (c->n % 16) return fail(c, EMP_ERR_ARG, "a batch follows one whose size is not a multiple of 16");
     if (2 * (c->n + nreads) >= (1ull << 32)) return fail(c, EMP_ERR_TOO_MANY, "more than 2^31 - 1 reads");
-    int rc = ensure_chunks(c, c->n + nreads);
+    int rc = ensure_cols(c, c->n + nreads);
     if (rc) return rc;
     rc = ensure_entries(c, 2 * (c->n + nreads));
     if (rc) return rc;
@@ -333,7 +325,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
         a.Lpad = Lpad;
         a.rows = 256u * (uint32_t)Lpad <= 65536u ? 256u : 128u;
         a.k0 = c->n + done;
-        a.chunks = c->d_chunk_tab;
+        a.cols = c->d_cols;
         a.entries = c->d_entries;
         a.err = c->d_err;
         {
@@ -526,8 +518,10 @@ struct ListIn {
     uint32_t cap;
 };
 
-// Partitions list `side` (x entries) into 2^bb buckets.
-int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb) {
+// Partitions list `side` (x entries) into 2^bb buckets. The first pass regenerates the entries
+// for round `rg` (suffix rule for side 0, prefix rule for side 1) when rg.active.
+int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb, const RoundArgs &rg) {
+    if (rg.active && !li.cnt) return fail(c, EMP_ERR_ARG, "a regenerating partition needs a survivor list");
     const std::vector<Level> pl = plan_levels(x, bb, c->slack);
     const Entry *in = li.base;
     const uint32_t *in_cnt = li.cnt;
@@ -556,6 +550,9 @@ int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb) {
         a.spill_n = &c->d_ctr->spill[side];
         a.spill_cap = c->spill_cap;
         a.abort = &c->d_ctr->abort;
+        if (i == 0) a.rg = rg;
+        a.sfx = side == 0;
+        a.cols = c->d_cols;
         {
             Timed t(c, "k_part", 24.0 * (double)x);
             HIPCHK(c, launch_part(a, groups, c->stream));
@@ -679,6 +676,7 @@ RoundArgs round_args(const emp_ctx *c, int t, bool active) {
     r.col_back = (uint32_t)(L - 1 - t);
     r.m1 = modpow(R1, (uint64_t)(L - t - 1), P1);
     r.m2 = modpow(R2, (uint64_t)(L - t - 1), P2);
+    r.L = (uint32_t)L;
     return r;
 }
 
@@ -726,7 +724,7 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
 
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket. Survivors go to set `set`.
-int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup, int set) {
+int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
@@ -774,8 +772,7 @@ int big_path(emp_ctx *c, int bb, const RoundArgs &rn, bool dedup, int set) {
     a.p_spill_n = &c->d_ctr->spill[1];
     a.spill_cap = c->spill_cap;
     a.cap = BUCKET_CAP;
-    a.chunks = c->d_chunk_tab;
-    a.rn = rn;
+    a.keep = keep ? 1 : 0;
     a.s_lin = c->d_lin[set][0];
     a.p_lin = c->d_lin[set][1];
     a.s_lin_n = c->d_ctr->lin[set][0];
@@ -834,8 +831,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
-    // the survivors with their round-0 suffix and prefix keys into survivor set 0.
-    const RoundArgs r0 = round_args(c, 0, rounds > 0);
+    // the kept entries (full-length keys) into survivor set 0, side 0. Round 0 partitions that
+    // list twice: as suffixes and as prefixes.
+    const RoundArgs none{};
     bool need_part = true;
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
@@ -848,14 +846,11 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             zero_join(c, z, 0);
             if ((rc = run_zero(c, z))) return rc;
         }
-        if (need_part && (rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD))) return rc;
+        if (need_part && (rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD, none))) return rc;
         DedupArgs a{};
         a.in = c->d_fine[0];
         a.cnt = c->d_fine_cnt[0];
-        a.chunks = c->d_chunk_tab;
-        a.r0 = r0;
-        a.s_lin = c->d_lin[0][0];
-        a.p_lin = c->d_lin[0][1];
+        a.lin = c->d_lin[0][0];
         a.lin_n = c->d_ctr->lin[0][0];
         a.lin_cap = c->lin_cap[0];
         a.big = c->d_big;
@@ -874,7 +869,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             continue;
         }
         need_part = false;
-        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, r0, true, 0))) return rc;
+        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0))) return rc;
         if (c->h_ctr->lin_ovf) {
             if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
             continue;
@@ -898,7 +893,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         const int bb = bucket_bits(std::max(St, Pt));
         const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
-        const RoundArgs rn = round_args(c, t + 1, more);
+        const RoundArgs rg = round_args(c, t, true);
         const int nxt = cur ^ 1;
         if ((rc = ensure_lin_set(c, nxt, lin_cap_for(std::max(St, Pt), 1.25)))) return rc;
         emp_round_stats rs{};
@@ -921,18 +916,18 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 if ((rc = run_zero(c, z))) return rc;
             }
             if (need_part) {
+                // round 0: both sides come from the dedup list
                 ListIn ls{c->d_lin[cur][0], c->d_ctr->lin[cur][0], c->lin_cap[cur]};
-                ListIn lp{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
-                if ((rc = partition(c, 0, ls, St, bb))) return rc;
-                if ((rc = partition(c, 1, lp, Pt, bb))) return rc;
+                ListIn lp{c->d_lin[cur][t == 0 ? 0 : 1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
+                if ((rc = partition(c, 0, ls, St, bb, rg))) return rc;
+                if ((rc = partition(c, 1, lp, Pt, bb, rg))) return rc;
             }
             JoinArgs a{};
             a.s_in = c->d_fine[0];
             a.p_in = c->d_fine[1];
             a.s_cnt = c->d_fine_cnt[0];
             a.p_cnt = c->d_fine_cnt[1];
-            a.chunks = c->d_chunk_tab;
-            a.rn = rn;
+            a.keep = more ? 1 : 0;
             a.s_lin = c->d_lin[nxt][0];
             a.p_lin = c->d_lin[nxt][1];
             a.s_lin_n = c->d_ctr->lin[nxt][0];
@@ -961,7 +956,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             }
             need_part = false;
             rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big && (rc = big_path(c, bb, rn, false, nxt))) return rc;
+            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt))) return rc;
             if (c->h_ctr->lin_ovf) {
                 if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
                 continue;
@@ -1257,11 +1252,11 @@ int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len
     if (!rc) rc = emp_add_reads_device(c, d, n, len);
     const uint64_t W = (n + 15) / 16;
     for (int j = 0; !rc && j < len; ++j) {
-        for (size_t ch = 0; !rc && ch < c->chunks.size(); ++ch) {
+        for (size_t ch = 0; !rc && ch < c->cols_chunks; ++ch) {
             uint64_t w0 = (uint64_t)ch * CHUNK_WORDS;
             if (w0 >= W) break;
             uint64_t cnt = std::min<uint64_t>(CHUNK_WORDS, W - w0);
-            if (hipMemcpy(cols + (uint64_t)j * W + w0, c->chunks[ch] + (size_t)j * CHUNK_WORDS, cnt * 4,
+            if (hipMemcpy(cols + (uint64_t)j * W + w0, c->d_cols + (ch * len + j) * CHUNK_WORDS, cnt * 4,
                           hipMemcpyDeviceToHost) != hipSuccess)
                 rc = EMP_ERR_HIP;
         }
