@@ -21,11 +21,20 @@ constexpr uint32_t INV2 = 76695840u;   // 7^(P2-2) mod P2 (overlap.cu:122)
 constexpr uint32_t MASK29 = (1u << 29) - 1;
 constexpr int MAX_RD_LEN = 255;
 
-// Column store: reads are grouped in chunks of 2^CHUNK_SHIFT; a chunk holds, per column j,
-// CHUNK_WORDS 32-bit words of 16 two-bit codes (read k at bits 2*(k%16)+1:2*(k%16)).
+// Column store: 2-bit base codes ((c >> 1) & 3) of every read position. Reads are grouped in
+// chunks of 2^CHUNK_SHIFT and columns in groups of 4; a 32-bit word holds 4 reads x 4 columns:
+// read k, column 4g + c at bits 2 * (4 * (k % 4) + c). Chunk ch, group g holds CHUNK_GWORDS
+// words starting at word (ch * G + g) * CHUNK_GWORDS, G = ceil(L / 4).
 constexpr int CHUNK_SHIFT = 20;
 constexpr uint32_t CHUNK_READS = 1u << CHUNK_SHIFT;
-constexpr uint32_t CHUNK_WORDS = CHUNK_READS / 16;
+constexpr uint32_t CHUNK_GWORDS = CHUNK_READS / 4;
+constexpr int GROUP_COLS = 4;
+__host__ __device__ __forceinline__ uint32_t col_groups(uint32_t L) { return (L + GROUP_COLS - 1) / GROUP_COLS; }
+
+// Keys are below 2^29: bits 29..31 of h1 and h2 carry up to three base codes an entry will need
+// in its next rounds (bits 0-1 of the 6-bit field h1 >> 29 | (h2 >> 29) << 3 first). Every key
+// comparison masks them off with KEY_MASK.
+constexpr uint32_t KEY_MASK = MASK29;
 
 // One (fingerprint, oriented read id) entry, 12 bytes.
 struct Entry {

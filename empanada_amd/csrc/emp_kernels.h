@@ -50,7 +50,7 @@ struct RoundArgs {
     uint32_t col_t;    // column t
     uint32_t col_back; // column L-1-t
     uint32_t m1, m2;   // R1^(l-1) mod P1, R2^(l-1) mod P2, l = L - t
-    uint32_t L;        // read length: column stride of the column store
+    uint32_t G;        // column groups of the column store, ceil(L / 4)
 };
 
 struct MapArgs {
@@ -59,7 +59,7 @@ struct MapArgs {
     int32_t L, Lpad;         // Lpad: LDS row stride (multiple of 4, odd number of dwords)
     uint32_t rows;           // reads per work-group (256, or 128 when 256 rows exceed 64 KiB of LDS)
     uint64_t k0;             // global index of the first read (multiple of 16)
-    uint32_t *cols;          // column store: chunk c, column j at (c * L + j) * CHUNK_WORDS
+    uint32_t *cols;          // column store (grouped layout of emp_device.h)
     Entry *entries;          // linear entries, index 2k and 2k+1
     uint32_t *err;           // set to 1 when a byte is not ACGTacgt
 };
@@ -136,6 +136,8 @@ struct BigArgs {
     uint32_t spill_cap;
     uint32_t cap;
     int32_t keep;                  // write the survivors
+    RoundArgs rg;                  // the round (join): survivors get their carried codes back
+    const uint32_t *cols;
     Entry *s_lin, *p_lin;
     uint32_t *s_lin_n, *p_lin_n;
     uint32_t lin_cap;

@@ -29,32 +29,71 @@
 namespace emp {
 
 // ---------------------------------------------------------------------------------------------
-// Column gather: 2-bit code of read k at column `col` (column store of L columns per chunk).
-__device__ __forceinline__ uint32_t col_code(const uint32_t *__restrict__ cols, uint32_t L, uint32_t col, uint32_t k) {
-    const size_t w = ((size_t)(k >> CHUNK_SHIFT) * L + col) * CHUNK_WORDS + ((k >> 4) & (CHUNK_WORDS - 1));
-    return (cols[w] >> (2 * (k & 15))) & 3u;
+// The 4 codes (8 bits, column 4g + c at bits 2c) of read k in column group g.
+__device__ __forceinline__ uint32_t group_codes(const uint32_t *__restrict__ cols, uint32_t G, uint32_t g, uint32_t k) {
+    const size_t w = ((size_t)(k >> CHUNK_SHIFT) * G + g) * CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
+    return (cols[w] >> (8 * (k & 3))) & 0xFFu;
 }
 
-// Base code of a live entry for the round described by `r`. Suffix list: even ids read column
-// t, odd ids the complement of column L-1-t; prefix list: the other way round (overlap.cu:266-267,
-// 684-690: rid/2 picks the read, odd ids are complemented).
-__device__ __forceinline__ uint32_t entry_code(const uint32_t *__restrict__ cols, const RoundArgs &r, uint32_t rid,
-                                               bool sfx) {
-    uint32_t k = rid >> 1;
-    bool odd = rid & 1;
-    uint32_t col = (sfx != odd) ? r.col_t : r.col_back;
-    uint32_t c = col_code(cols, r.L, col, k);
-    return odd ? (c ^ 2u) : c;
+// Regeneration of a live entry for round t (keys of length L - t -> L - t - 1). Suffix list:
+// even ids drop column t, odd ids the complement of column L-1-t; prefix list: the other way
+// round (overlap.cu:266-267, 684-690: rid/2 picks the read, odd ids are complemented). An entry
+// walks its columns forward (t = 0, 1, ...) or backward (L-1, L-2, ...); it gathers its read's
+// word of a column group when it enters the group (or at round 0) and carries the codes of the
+// group's remaining columns, in the order it uses them, in the spare key bits.
+// Codes an entry carries after its regeneration at column `col` of group word `gw` (8 bits):
+// forward, columns c+1 .. 3; backward, columns c-1 .. 0 (c = col % 4), first-used in bits 0-1.
+__device__ __forceinline__ uint32_t carry_of(uint32_t gw, uint32_t col, bool fwd) {
+    const uint32_t c = col & 3u;
+    if (fwd) return gw >> (2 * (c + 1));
+    uint32_t f = 0;
+    for (uint32_t i = 0; i < c; ++i) f |= ((gw >> (2 * (c - 1 - i))) & 3u) << (2 * i);
+    return f;
 }
 
-// Key of length l -> key of length l-1: NextSfx drops the first base of a suffix, PrevPfx the
-// last base of a prefix (fingerprint.cu:37-72); `code` is the 2-bit code of that base.
-__device__ __forceinline__ void regen_code(const RoundArgs &r, Entry &e, uint32_t code, bool sfx) {
-    uint32_t ev = enc_of_code(code);
+__device__ __forceinline__ void regen_carry(const uint32_t *__restrict__ cols, const RoundArgs &r, Entry &e, bool sfx,
+                                            uint32_t gathered) {
+    const bool odd = e.rid & 1;
+    const bool fwd = sfx != odd;
+    const uint32_t col = fwd ? r.col_t : r.col_back;
+    uint32_t f = (e.h1 >> 29) | ((e.h2 >> 29) << 3), code;
+    if (gathered != 0xFFFFFFFFu) {
+        code = (gathered >> (2 * (col & 3u))) & 3u;
+        f = carry_of(gathered, col, fwd);
+    } else {
+        code = f & 3u;
+        f >>= 2;
+    }
+    uint32_t h1 = e.h1 & KEY_MASK, h2 = e.h2 & KEY_MASK;
+    const uint32_t ev = enc_of_code(odd ? (code ^ 2u) : code);
     if (sfx)
-        next_sfx(e.h1, e.h2, ev, r.m1, r.m2);
+        next_sfx(h1, h2, ev, r.m1, r.m2);
     else
-        prev_pfx(e.h1, e.h2, ev);
+        prev_pfx(h1, h2, ev);
+    e.h1 = h1 | ((f & 7u) << 29);
+    e.h2 = h2 | (((f >> 3) & 7u) << 29);
+}
+
+// The carried-code field of a live entry right after its regeneration for round r (recomputed
+// from its read's group word; used where an entry is rebuilt from its key alone).
+__device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ cols, const RoundArgs &r, uint32_t rid,
+                                                bool sfx) {
+    const bool fwd = sfx != (bool)(rid & 1);
+    const uint32_t col = fwd ? r.col_t : r.col_back;
+    return carry_of(group_codes(cols, r.G, col >> 2, rid >> 1), col, fwd);
+}
+
+__device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
+    e.h1 = (e.h1 & KEY_MASK) | ((f & 7u) << 29);
+    e.h2 = (e.h2 & KEY_MASK) | (((f >> 3) & 7u) << 29);
+}
+
+// Whether the entry must gather its group word this round, and which word (group index).
+__device__ __forceinline__ bool regen_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
+    const bool fwd = sfx != (bool)(rid & 1);
+    const uint32_t col = fwd ? r.col_t : r.col_back;
+    g = col >> 2;
+    return fwd ? (col & 3u) == 0 : ((col & 3u) == 3 || r.col_t == 0);
 }
 
 __device__ __forceinline__ Entry load_stream(const Entry *p) {
@@ -116,18 +155,26 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         a.entries[2 * k + 1] = er;
     }
 
-    // Columns: word w of the tile covers reads 16w..16w+15; task id = w * L + j keeps a wave on
-    // consecutive columns of the same 16 rows (conflict-free LDS byte reads).
-    const uint32_t nw = (nt + 15) / 16;
-    const uint32_t ntask = nw * (uint32_t)L;
-    const uint64_t w0 = (a.k0 + first) >> 4;
+    // Columns: word w of the tile covers reads 4w..4w+3; task id = g * nw + w keeps a wave on
+    // consecutive words of one column group (contiguous stores).
+    const uint32_t G = col_groups((uint32_t)L);
+    const uint32_t nw = (nt + 3) / 4;
+    const uint32_t ntask = nw * G;
+    const uint64_t w0 = (a.k0 + first) >> 2;
     for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
-        uint32_t w = task / (uint32_t)L, j = task - w * (uint32_t)L;
+        const uint32_t g = task / nw, w = task - g * nw;
         uint32_t word = 0;
-        uint32_t rmax = min(16u, nt - 16 * w);
-        for (uint32_t i = 0; i < rmax; ++i) word |= ((uint32_t)(tile[(16 * w + i) * Lp + j] >> 1) & 3u) << (2 * i);
-        uint64_t W = w0 + w;
-        a.cols[((W >> (CHUNK_SHIFT - 4)) * (uint32_t)L + j) * CHUNK_WORDS + (W & (CHUNK_WORDS - 1))] = word;
+        const uint32_t rmax = min(4u, nt - 4 * w);
+        for (uint32_t i = 0; i < rmax; ++i) {
+            const uint8_t *row = tile + (4 * w + i) * Lp;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; ++c) {
+                const uint32_t j = 4 * g + c;
+                if (j < (uint32_t)L) word |= ((uint32_t)(row[j] >> 1) & 3u) << (2 * (4 * i + c));
+            }
+        }
+        const uint64_t W = w0 + w;
+        a.cols[((W >> (CHUNK_SHIFT - 2)) * G + g) * CHUNK_GWORDS + (W & (CHUNK_GWORDS - 1))] = word;
     }
 }
 
@@ -181,7 +228,9 @@ __device__ void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n
     }
 }
 
-__device__ __forceinline__ uint64_t key_of(const Entry &e) { return ((uint64_t)e.h1 << 32) | e.h2; }
+__device__ __forceinline__ uint64_t key_of(const Entry &e) {
+    return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Partition pass: each work-group takes tiles of PART_TILE entries of one input group, ranks them
@@ -223,18 +272,20 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
         Entry e[PER];
         uint32_t r[PER], d[PER];
         if (REGEN) {
-            uint32_t code[PER];
+            uint32_t word[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 uint32_t i = threadIdx.x + 256 * q;
+                word[q] = 0xFFFFFFFFu;
                 if (i < nt) {
                     e[q] = load_stream(in + t0 + i);
-                    code[q] = entry_code(a.cols, a.rg, e[q].rid, a.sfx);
+                    uint32_t g;
+                    if (regen_needs_gather(a.rg, e[q].rid, a.sfx, g)) word[q] = group_codes(a.cols, a.rg.G, g, e[q].rid >> 1);
                 }
             }
 #pragma unroll
             for (int q = 0; q < PER; ++q)
-                if (threadIdx.x + 256 * q < nt) regen_code(a.rg, e[q], code[q], a.sfx);
+                if (threadIdx.x + 256 * q < nt) regen_carry(a.cols, a.rg, e[q], a.sfx, word[q]);
         } else {
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
@@ -327,7 +378,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
             uint32_t h1 = h1s[i], h2 = h2s[i], rid = rids[i];
             bool keep = true;
             for (uint32_t j = head[h2 & (CAP - 1)]; j != EMPTY && keep; j = (nxt[j] == 0xFFFFu ? EMPTY : nxt[j]))
-                if (h2s[j] == h2 && h1s[j] == h1 && rids[j] < rid) keep = false;
+                if (((h2s[j] ^ h2) & KEY_MASK) == 0 && ((h1s[j] ^ h1) & KEY_MASK) == 0 && rids[j] < rid) keep = false;
             if (keep) {
                 keepmask |= 1u << q;
                 slot[q] = atomicAdd(&rcnt[read_range(rid)], 1u);
@@ -451,10 +502,10 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         for (int q = 0; q < PER; ++q) {
             const uint32_t i = threadIdx.x + NT * q;
             if (i < ns) {
-                const uint32_t h1 = se[q].h1, h2 = se[q].h2, rd = se[q].rid >> 1;
+                const uint32_t h1 = se[q].h1 & KEY_MASK, h2 = se[q].h2 & KEY_MASK, rd = se[q].rid >> 1;
                 uint32_t c = 0;
                 for (uint32_t j = S.u.h.head[h2 & (CAP - 1)]; j != EMPTY; j = (S.u.h.nxt[j] == 0xFFFFu ? EMPTY : S.u.h.nxt[j])) {
-                    if (S.pk2[j] == h2 && S.pk1[j] == h1 && (S.prid[j] >> 1) != rd) {
+                    if ((S.pk2[j] & KEY_MASK) == h2 && (S.pk1[j] & KEY_MASK) == h1 && (S.prid[j] >> 1) != rd) {
                         ++c;
                         S.used[j] = 1;
                     }
@@ -478,7 +529,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                 ecount += sc[q];
                 uint32_t m = atomicAdd(&S.ms, 1u);
                 if (m < (uint32_t)MCAP) {
-                    S.u.m.msk[m] = ((uint64_t)se[q].h1 << 32) | se[q].h2;
+                    S.u.m.msk[m] = ((uint64_t)(se[q].h1 & KEY_MASK) << 32) | (se[q].h2 & KEY_MASK);
                     S.u.m.msr[m] = rid_rank(se[q].rid);
                     S.u.m.msc[m] = sc[q];
                 } else {
@@ -494,7 +545,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             if (S.used[i]) {
                 uint32_t m = atomicAdd(&S.mp, 1u);
                 if (m < (uint32_t)MCAP) {
-                    S.u.m.mpk[m] = ((uint64_t)S.pk1[i] << 32) | S.pk2[i];
+                    S.u.m.mpk[m] = ((uint64_t)(S.pk1[i] & KEY_MASK) << 32) | (S.pk2[i] & KEY_MASK);
                     S.u.m.mpr[m] = rid_rank(rid);
                 } else {
                     S.ovf = 1;
@@ -803,6 +854,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
             uint32_t pos = atomicAdd(a.s_lin_n + rs, 1u);
             if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
             else if (a.keep) {
+                set_carry(en, carry_after(a.cols, a.rg, srid, true));
                 a.s_lin[rs * a.lin_cap + pos] = en;
             }
         }
@@ -815,6 +867,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
             uint32_t pos = atomicAdd(a.p_lin_n + rs, 1u);
             if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
             else if (a.keep) {
+                set_carry(en, carry_after(a.cols, a.rg, prd, false));
                 a.p_lin[rs * a.lin_cap + pos] = en;
             }
         }

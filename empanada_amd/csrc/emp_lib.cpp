@@ -85,7 +85,7 @@ struct emp_ctx {
     // reads / columns / entries
     int L = 0;
     uint64_t n = 0;  // reads mapped (submitted to the device)
-    uint32_t *d_cols = nullptr;  // column store: chunk c, column j at (c * L + j) * CHUNK_WORDS
+    uint32_t *d_cols = nullptr;  // column store (grouped layout of emp_device.h)
     size_t cols_chunks = 0;      // chunks allocated
     Entry *d_entries = nullptr;
     uint64_t entries_cap = 0;
@@ -247,7 +247,7 @@ int ensure_cols(emp_ctx *c, uint64_t n_total) {
     const size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
     if (need <= c->cols_chunks) return EMP_OK;
     const size_t cap = std::max(need, c->cols_chunks * 2);
-    const size_t chunk_bytes = (size_t)c->L * CHUNK_WORDS * sizeof(uint32_t);
+    const size_t chunk_bytes = (size_t)col_groups((uint32_t)c->L) * CHUNK_GWORDS * sizeof(uint32_t);
     void *p = nullptr;
     int rc = dalloc(c, &p, cap * chunk_bytes);
     if (rc) return rc;
@@ -676,7 +676,7 @@ RoundArgs round_args(const emp_ctx *c, int t, bool active) {
     r.col_back = (uint32_t)(L - 1 - t);
     r.m1 = modpow(R1, (uint64_t)(L - t - 1), P1);
     r.m2 = modpow(R2, (uint64_t)(L - t - 1), P2);
-    r.L = (uint32_t)L;
+    r.G = col_groups((uint32_t)L);
     return r;
 }
 
@@ -724,7 +724,7 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
 
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket. Survivors go to set `set`.
-int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set) {
+int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
@@ -773,6 +773,8 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set) {
     a.spill_cap = c->spill_cap;
     a.cap = BUCKET_CAP;
     a.keep = keep ? 1 : 0;
+    a.rg = rg;
+    a.cols = c->d_cols;
     a.s_lin = c->d_lin[set][0];
     a.p_lin = c->d_lin[set][1];
     a.s_lin_n = c->d_ctr->lin[set][0];
@@ -869,7 +871,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             continue;
         }
         need_part = false;
-        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0))) return rc;
+        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0, none))) return rc;
         if (c->h_ctr->lin_ovf) {
             if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
             continue;
@@ -956,7 +958,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             }
             need_part = false;
             rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt))) return rc;
+            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt, rg))) return rc;
             if (c->h_ctr->lin_ovf) {
                 if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
                 continue;
@@ -1250,15 +1252,23 @@ int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len
     rc = dalloc(c, &d, n * (uint64_t)len);
     if (!rc && hipMemcpy(d, reads, n * (uint64_t)len, hipMemcpyHostToDevice) != hipSuccess) rc = EMP_ERR_HIP;
     if (!rc) rc = emp_add_reads_device(c, d, n, len);
+    // device layout (4 reads x 4 columns per word) -> columns of 16 reads per word
     const uint64_t W = (n + 15) / 16;
-    for (int j = 0; !rc && j < len; ++j) {
-        for (size_t ch = 0; !rc && ch < c->cols_chunks; ++ch) {
-            uint64_t w0 = (uint64_t)ch * CHUNK_WORDS;
-            if (w0 >= W) break;
-            uint64_t cnt = std::min<uint64_t>(CHUNK_WORDS, W - w0);
-            if (hipMemcpy(cols + (uint64_t)j * W + w0, c->d_cols + (ch * len + j) * CHUNK_WORDS, cnt * 4,
-                          hipMemcpyDeviceToHost) != hipSuccess)
-                rc = EMP_ERR_HIP;
+    const uint32_t G = col_groups((uint32_t)len);
+    std::vector<uint32_t> dev;
+    if (!rc) {
+        dev.resize(c->cols_chunks * G * (size_t)CHUNK_GWORDS);
+        if (hipMemcpy(dev.data(), c->d_cols, dev.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
+    }
+    if (!rc) {
+        memset(cols, 0, (size_t)len * W * 4);
+        for (uint64_t k = 0; k < n; ++k) {
+            const size_t base = ((k >> CHUNK_SHIFT) * G) * (size_t)CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
+            for (int j = 0; j < len; ++j) {
+                const uint32_t w = dev[base + (size_t)(j >> 2) * CHUNK_GWORDS];
+                const uint32_t code = (w >> (2 * (4 * (k & 3) + (j & 3)))) & 3u;
+                cols[(uint64_t)j * W + (k >> 4)] |= code << (2 * (k & 15));
+            }
         }
     }
     emp_ctx_destroy(c);
