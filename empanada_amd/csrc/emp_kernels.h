@@ -109,9 +109,11 @@ struct JoinArgs {
     Entry *s_lin, *p_lin;    // survivors with this round's keys, SUBS sub-ranges of lin_cap entries
     uint32_t *s_lin_n, *p_lin_n;
     uint32_t lin_cap;
-    uint32_t *pool_from, *pool_to;   // edge pool, SUBS sub-ranges of pool_cap edges
-    unsigned long long *pool_cursor; // [SUBS]
-    unsigned long long pool_cap;
+    uint32_t *pool_from, *pool_to;   // edge pool: SUBS sub-ranges of pool_cap edges, then the
+    unsigned long long *pool_cursor; // overflow region [SUBS * pool_cap, + ovf_cap) with its
+    unsigned long long pool_cap;     // own cursor for reservations a sub-range cannot take
+    unsigned long long *ovf_cursor;
+    unsigned long long ovf_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
     uint32_t *big, *n_big;
@@ -144,6 +146,8 @@ struct BigArgs {
     uint32_t *pool_from, *pool_to;
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
+    unsigned long long *ovf_cursor;
+    unsigned long long ovf_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
     uint32_t *lin_ovf;
@@ -158,12 +162,14 @@ struct ZeroArgs {
     int32_t count;
 };
 
+// Edge compaction over n (round, bucket) slots: counts, pool offsets, and scratch for the
+// exclusive offsets (dst_off: n, blk: ceil(n / 4096)).
 struct EdgeCopyArgs {
     const uint32_t *cnt;
     const unsigned long long *src_off;
-    const unsigned long long *dst_off;
-    unsigned long long dst_base;  // first free slot of the output arena
-    uint32_t nb;
+    unsigned long long *dst_off;
+    unsigned long long *blk;
+    uint64_t n;
     const uint32_t *pool_from, *pool_to;
     uint32_t *out_from, *out_to;
 };
@@ -174,7 +180,7 @@ hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s);
 hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s);
 hipError_t launch_dedup_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
-hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s);
+hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,

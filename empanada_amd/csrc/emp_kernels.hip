@@ -204,6 +204,31 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* 
     return base + x - v;
 }
 
+template <int NT>
+__device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long long v, unsigned long long *sh,
+                                                               unsigned long long &total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    unsigned long long base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        unsigned long long s = sh[i];
+        base += i < wv ? s : 0ull;
+        tot += s;
+    }
+    total = tot;
+    __syncthreads();
+    return base + x - v;
+}
+
 // Bitonic sort of n2 (power of two) records by (key, rank), ascending; `pay` (may be null) is
 // permuted along. Works on LDS or on global memory private to the work-group.
 __device__ void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n2) {
@@ -226,6 +251,18 @@ __device__ void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n
             __syncthreads();
         }
     }
+}
+
+// Edge-pool slot of n edges whose sub-range reservation returned c (atomicAdd on the cursor
+// of sub-range `sub`): inside the sub-range when it fits, else the reservation is returned and
+// the edges go to the overflow region; ~0 when that is full too (the host grows it and reruns).
+__device__ __forceinline__ unsigned long long place_edges(unsigned long long c, uint32_t n, uint32_t sub,
+                                                          unsigned long long *cursor, unsigned long long cap,
+                                                          unsigned long long *ovf_cursor, unsigned long long ovf_cap) {
+    if (c + n <= cap) return (unsigned long long)sub * cap + c;
+    atomicAdd(cursor, 0ull - (unsigned long long)n);
+    const unsigned long long o = atomicAdd(ovf_cursor, (unsigned long long)n);
+    return o + n <= ovf_cap ? (unsigned long long)SUBS * cap + o : ~0ull;
 }
 
 __device__ __forceinline__ uint64_t key_of(const Entry &e) {
@@ -605,8 +642,10 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         S.rbase[k] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
     } else if (threadIdx.x == 2 * NRANGE) {
         const uint32_t sub = b & (SUBS - 1);
-        S.base = tot && resv + tot <= a.pool_cap ? sub * a.pool_cap + resv : ~0ull;
-        a.edge_off[b] = tot ? sub * a.pool_cap + resv : 0ull;
+        const unsigned long long base =
+            tot ? place_edges(resv, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap) : ~0ull;
+        S.base = base;
+        a.edge_off[b] = base;
         a.edge_cnt[b] = tot;
     }
     __syncthreads();
@@ -825,9 +864,13 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
     }
     const uint32_t sub = b & (SUBS - 1);
     if (threadIdx.x == 0) {
-        unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
-        s_base = c + tot <= a.pool_cap ? sub * a.pool_cap + c : ~0ull;
-        a.edge_off[b] = sub * a.pool_cap + c;
+        unsigned long long base = ~0ull;
+        if (tot) {
+            const unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
+            base = place_edges(c, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap);
+        }
+        s_base = base;
+        a.edge_off[b] = base;
         a.edge_cnt[b] = tot;
     }
     __syncthreads();
@@ -905,34 +948,60 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t v, uint6
 }
 
 // ---------------------------------------------------------------------------------------------
-// Edge compaction: exclusive scan of per-bucket edge counts in bucket (= key) order, then copy.
-__global__ void __launch_bounds__(1024) k_edge_scan(const uint32_t *__restrict__ cnt, uint32_t nb,
-                                                    unsigned long long *__restrict__ dst_off) {
-    __shared__ unsigned long long part[1024];
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t lo = min(nb, threadIdx.x * per), hi = min(nb, lo + per);
-    unsigned long long s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += cnt[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t o = 1; o < 1024; o <<= 1) {
-        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+// Edge compaction at the end of a run: the per-bucket edge counts of all rounds, in (round,
+// bucket) order (= output.bin order), get exclusive offsets in two levels (blocks of
+// ESCAN_BLOCK counts, then the block totals), and every bucket's edges are copied from the pool
+// to their offset in the output arena.
+constexpr int ESCAN_BLOCK = 4096;
+
+__global__ void __launch_bounds__(1024) k_escan_local(const uint32_t *__restrict__ cnt, uint64_t n,
+                                                     unsigned long long *__restrict__ dst,
+                                                     unsigned long long *__restrict__ blk) {
+    __shared__ unsigned long long sh[16];
+    const uint64_t i0 = (uint64_t)blockIdx.x * ESCAN_BLOCK + 4ull * threadIdx.x;
+    uint32_t v[4];
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        v[u] = i0 + u < n ? cnt[i0 + u] : 0u;
+        sum += v[u];
     }
-    unsigned long long run = part[threadIdx.x] - s;
-    for (uint32_t i = lo; i < hi; ++i) {
-        dst_off[i] = run;
-        run += cnt[i];
+    unsigned long long tot;
+    unsigned long long off = block_excl_scan64<1024>(sum, sh, tot);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (i0 + u < n) dst[i0 + u] = off;
+        off += v[u];
+    }
+    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_escan_top(unsigned long long *__restrict__ blk, uint32_t nblk) {
+    __shared__ unsigned long long part[1024];
+    unsigned long long carry = 0;
+    for (uint32_t base = 0; base < nblk; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const unsigned long long v = i < nblk ? blk[i] : 0ull;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024; o <<= 1) {
+            const unsigned long long w = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+            __syncthreads();
+            part[threadIdx.x] += w;
+            __syncthreads();
+        }
+        if (i < nblk) blk[i] = carry + part[threadIdx.x] - v;
+        carry += part[1023];
+        __syncthreads();
     }
 }
 
 __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
-    const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= a.nb) return;
+    const uint64_t b = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= a.n) return;
     const uint32_t c = a.cnt[b];
-    const unsigned long long src = a.src_off[b], dst = a.dst_base + a.dst_off[b];
+    if (c == 0) return;
+    const unsigned long long src = a.src_off[b], dst = a.dst_off[b] + a.blk[b / ESCAN_BLOCK];
     for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
         a.out_from[dst + i] = a.pool_from[src + i];
         a.out_to[dst + i] = a.pool_to[src + i];
@@ -1016,11 +1085,16 @@ hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_edge_compact(const EdgeCopyArgs &a, unsigned long long *dst_off, hipStream_t s) {
-    hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(1024), 0, s, a.cnt, a.nb, dst_off);
+hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint64_t nblk = (a.n + ESCAN_BLOCK - 1) / ESCAN_BLOCK;
+    hipLaunchKernelGGL(k_escan_local, dim3((uint32_t)nblk), dim3(1024), 0, s, a.cnt, a.n, a.dst_off, a.blk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_edge_copy, dim3((a.nb + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_escan_top, dim3(1), dim3(1024), 0, s, a.blk, (uint32_t)nblk);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_edge_copy, dim3((uint32_t)((a.n + 3) / 4)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

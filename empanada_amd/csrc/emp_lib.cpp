@@ -29,6 +29,7 @@ constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large
 
 struct DevCounters {
     unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
+    unsigned long long pool_ovf;    // cursor of the pool's overflow region
     uint32_t lin[2][2][SUBS];       // [survivor set][side][sub-range] counts
     uint32_t n_big, abort, lin_ovf, pad;
     uint32_t spill[2];              // records spilled by the final partition pass
@@ -115,12 +116,16 @@ struct emp_ctx {
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
     uint64_t pool_cap = 0;  // edges per sub-range of the pool
+    uint64_t pool_ovf_cap = 0;  // edges of the pool's overflow region
     uint64_t out_cap = 0;   // edges the output arena holds
     uint64_t out_used = 0;  // edges in the arena not yet passed to the sink
     struct Record { uint64_t n, off; int32_t ovlp; };
     std::vector<Record> records;
-    unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr;
+    // per (round, bucket) slot: pool offset and count of the bucket's edges, output offset
+    unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr, *d_eblk = nullptr;
     uint32_t *d_edge_cnt = nullptr;
+    uint64_t eslots_cap = 0;
+    std::vector<unsigned long long> pool_prev;  // edge-pool cursors after the last completed round
     // large buckets
     uint32_t *d_big = nullptr;
     unsigned long long *d_boff[2] = {nullptr, nullptr};
@@ -453,16 +458,6 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
             if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
             c->d_fine_cnt[s] = (uint32_t *)p;
         }
-        dfree(c, c->d_edge_off);
-        dfree(c, c->d_dst_off);
-        dfree(c, c->d_edge_cnt);
-        void *p = nullptr;
-        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long)))) return rc;
-        c->d_edge_off = (unsigned long long *)p;
-        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long)))) return rc;
-        c->d_dst_off = (unsigned long long *)p;
-        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
-        c->d_edge_cnt = (uint32_t *)p;
         c->nb_alloc = nb;
     }
     const uint64_t need = inter_need(x, bb, c->slack);
@@ -596,7 +591,6 @@ void zero_part_flags(emp_ctx *c, ZeroList &z, uint32_t xs, uint32_t xp) {
 
 // What a dedup / join launch accumulates: pool cursors, survivor set `set`, flags.
 void zero_join(emp_ctx *c, ZeroList &z, int set) {
-    z.add((uint32_t *)c->d_ctr->pool, 2 * SUBS);
     z.add(&c->d_ctr->lin[set][0][0], 2 * SUBS);
     z.add(&c->d_ctr->n_big, 1);
     z.add(&c->d_ctr->lin_ovf, 1);
@@ -607,21 +601,65 @@ int run_zero(emp_ctx *c, const ZeroList &z) {
     return EMP_OK;
 }
 
-// Edge pool of `per_sub` edges per sub-range and compacted edge arrays of SUBS * per_sub.
-int ensure_pool(emp_ctx *c, uint64_t per_sub) {
-    if (per_sub <= c->pool_cap) return EMP_OK;
+// Edge pool: SUBS sub-ranges of `per_sub` edges, then an overflow region of `ovf` edges. The
+// pool holds the edges of every round until the end-of-run compaction. With `keep` only the
+// overflow region grows and everything already written stays at its offset.
+int ensure_pool(emp_ctx *c, uint64_t per_sub, uint64_t ovf, bool keep) {
+    if (keep) per_sub = c->pool_cap;
+    if (per_sub <= c->pool_cap && ovf <= c->pool_ovf_cap) return EMP_OK;
+    per_sub = std::max<uint64_t>(per_sub, c->pool_cap);
+    ovf = std::max<uint64_t>(ovf, c->pool_ovf_cap);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    dfree(c, c->d_pool_from);
-    dfree(c, c->d_pool_to);
-    c->pool_cap = 0;
-    const uint64_t cap = per_sub * SUBS;
-    void *p = nullptr;
-    int rc;
-    if ((rc = dalloc(c, &p, cap * 4))) return rc;
-    c->d_pool_from = (uint32_t *)p;
-    if ((rc = dalloc(c, &p, cap * 4))) return rc;
-    c->d_pool_to = (uint32_t *)p;
+    uint32_t *old[2] = {c->d_pool_from, c->d_pool_to};
+    const uint64_t old_total = c->pool_cap * SUBS + c->pool_ovf_cap;
+    uint32_t *nu[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k) {
+        void *p = nullptr;
+        int rc = dalloc(c, &p, (per_sub * SUBS + ovf) * 4);
+        if (rc) {
+            if (nu[0]) dfree(c, nu[0]);
+            return rc;
+        }
+        nu[k] = (uint32_t *)p;
+    }
+    if (keep && old[0]) {
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(c, hipMemcpyAsync(nu[k], old[k], old_total * 4, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    dfree(c, old[0]);
+    dfree(c, old[1]);
+    c->d_pool_from = nu[0];
+    c->d_pool_to = nu[1];
     c->pool_cap = per_sub;
+    c->pool_ovf_cap = ovf;
+    return EMP_OK;
+}
+
+// Edge slots for n (round, bucket) pairs, all counts zero.
+int ensure_edge_slots(emp_ctx *c, uint64_t n) {
+    if (n > c->eslots_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c, c->d_edge_off);
+        dfree(c, c->d_dst_off);
+        dfree(c, c->d_edge_cnt);
+        dfree(c, c->d_eblk);
+        c->d_edge_off = c->d_dst_off = c->d_eblk = nullptr;
+        c->d_edge_cnt = nullptr;
+        c->eslots_cap = 0;
+        void *p = nullptr;
+        int rc;
+        if ((rc = dalloc(c, &p, n * sizeof(unsigned long long)))) return rc;
+        c->d_edge_off = (unsigned long long *)p;
+        if ((rc = dalloc(c, &p, n * sizeof(unsigned long long)))) return rc;
+        c->d_dst_off = (unsigned long long *)p;
+        if ((rc = dalloc(c, &p, n * sizeof(uint32_t)))) return rc;
+        c->d_edge_cnt = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, ((n + 4095) / 4096) * sizeof(unsigned long long)))) return rc;
+        c->d_eblk = (unsigned long long *)p;
+        c->eslots_cap = n;
+    }
+    HIPCHK(c, launch_fill_u32(c->d_edge_cnt, 0u, n, c->stream));
     return EMP_OK;
 }
 
@@ -691,16 +729,6 @@ uint64_t sum_subs(const uint32_t *v) {
     return s;
 }
 
-// Edge totals of the last join: sum of the sub-range cursors, largest cursor.
-void pool_totals(const emp_ctx *c, uint64_t *sum, uint64_t *maxc) {
-    uint64_t s = 0, m = 0;
-    for (int i = 0; i < SUBS; ++i) {
-        s += c->h_ctr->pool[i];
-        m = std::max<uint64_t>(m, c->h_ctr->pool[i]);
-    }
-    *sum = s;
-    *maxc = m;
-}
 
 int ensure_big_scratch(emp_ctx *c, uint64_t need) {
     if (need <= c->bscratch_cap) return EMP_OK;
@@ -724,7 +752,7 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
 
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket. Survivors go to set `set`.
-int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg) {
+int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg, uint64_t slot0) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
@@ -784,8 +812,10 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
     a.pool_to = c->d_pool_to;
     a.pool_cursor = c->d_ctr->pool;
     a.pool_cap = c->pool_cap;
-    a.edge_off = c->d_edge_off;
-    a.edge_cnt = c->d_edge_cnt;
+    a.ovf_cursor = &c->d_ctr->pool_ovf;
+    a.ovf_cap = c->pool_ovf_cap;
+    a.edge_off = c->d_edge_off + slot0;
+    a.edge_cnt = c->d_edge_cnt + slot0;
     a.lin_ovf = &c->d_ctr->lin_ovf;
     {
         Timed t(c, dedup ? "k_dedup_big" : "k_join_big", 0.0);
@@ -826,10 +856,11 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
-    if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / 4 / SUBS)))) return rc;
+    if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
+        return rc;
     c->records.clear();
     c->out_used = 0;
-    if ((rc = ensure_arena(c, std::max<uint64_t>(1u << 24, N0), sink, user))) return rc;
+    c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
     if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
@@ -846,6 +877,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 zero_partition(c, z, 0, N0, bbD);
             }
             zero_join(c, z, 0);
+            if (attempt == 0) z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors, once per run
             if ((rc = run_zero(c, z))) return rc;
         }
         if (need_part && (rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD, none))) return rc;
@@ -871,7 +903,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             continue;
         }
         need_part = false;
-        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0, none))) return rc;
+        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0, none, 0))) return rc;
         if (c->h_ctr->lin_ovf) {
             if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
             continue;
@@ -883,6 +915,11 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                              hipMemcpyDeviceToDevice, c->stream));
     uint64_t St = sum_subs(c->h_ctr->lin[0][0]), Pt = St;
     S.n_dedup = St;
+
+    // Edge slots: round t owns slots [t * stride, t * stride + 2^bb); bucket counts only shrink.
+    const uint64_t stride = 1ull << bucket_bits(St);
+    if (St > 0 && rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)rounds * stride))) return rc;
+    std::vector<uint64_t> round_edges;
 
     int cur = 0, t = 0;
     for (; t < rounds && St > 0; ++t) {
@@ -904,9 +941,13 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         rs.ovlp = L - 1 - t;
         rs.bucket_bits = bb;
         need_part = true;
-        uint64_t E = 0, maxc = 0;
+        uint64_t E = 0;
+        const uint64_t slot0 = (uint64_t)t * stride;
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
+            if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
+                HIPCHK(c, hipMemcpyAsync(c->d_ctr->pool, c->pool_prev.data(), (SUBS + 1) * sizeof(unsigned long long),
+                                         hipMemcpyHostToDevice, c->stream));
             {
                 ZeroList z;
                 if (need_part) {
@@ -939,8 +980,10 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             a.pool_to = c->d_pool_to;
             a.pool_cursor = c->d_ctr->pool;
             a.pool_cap = c->pool_cap;
-            a.edge_off = c->d_edge_off;
-            a.edge_cnt = c->d_edge_cnt;
+            a.ovf_cursor = &c->d_ctr->pool_ovf;
+            a.ovf_cap = c->pool_ovf_cap;
+            a.edge_off = c->d_edge_off + slot0;
+            a.edge_cnt = c->d_edge_cnt + slot0;
             a.big = c->d_big;
             a.n_big = &c->d_ctr->n_big;
             a.big_cap = BIG_CAP;
@@ -958,17 +1001,24 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             }
             need_part = false;
             rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt, rg))) return rc;
+            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt, rg, slot0))) return rc;
             if (c->h_ctr->lin_ovf) {
                 if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
                 continue;
             }
-            pool_totals(c, &E, &maxc);
-            if (maxc > c->pool_cap) {
-                if ((rc = ensure_pool(c, maxc + maxc / 4 + 1024))) return rc;
+            const uint64_t ovf = c->h_ctr->pool_ovf;
+            if (ovf > c->pool_ovf_cap) {
+                if ((rc = ensure_pool(c, 0, std::max<uint64_t>(2 * c->pool_ovf_cap, ovf + ovf / 4 + 1024), true)))
+                    return rc;
                 continue;
             }
             break;
+        }
+        E = 0;
+        for (int i = 0; i <= SUBS; ++i) {
+            const unsigned long long v = i < SUBS ? c->h_ctr->pool[i] : c->h_ctr->pool_ovf;
+            E += v - c->pool_prev[i];
+            c->pool_prev[i] = v;
         }
         const DevCounters &k = *c->h_ctr;
         const uint64_t Sn = sum_subs(k.lin[nxt][0]), Pn = sum_subs(k.lin[nxt][1]);
@@ -979,19 +1029,8 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             int id = kstat_id(c, "k_join");
             c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
         }
-        // record (overlap.cu:422; written even when E == 0), staged in the output arena
-        if (c->out_used + E > c->out_cap) {
-            if ((rc = flush_records(c, sink, user))) return rc;
-            if ((rc = ensure_arena(c, std::max<uint64_t>(E, c->out_cap), sink, user))) return rc;
-        }
-        if (E) {
-            EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->out_used, nbt,
-                            c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
-            Timed tm(c, "k_edge_compact", 16.0 * E + 12.0 * nbt);
-            HIPCHK(c, launch_edge_compact(ea, c->d_dst_off, c->stream));
-        }
-        c->records.push_back(emp_ctx::Record{E, c->out_used, L - 1 - t});
-        c->out_used += E;
+        // one record per round (overlap.cu:422; written even when E == 0)
+        round_edges.push_back(E);
         rs.records = 1;
         S.n_records++;
         S.n_edges += E;
@@ -1001,6 +1040,24 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         cur = nxt;
     }
     S.n_rounds = t;
+
+    // Output: compact the edges of every round from the pool into the arena in (round, bucket)
+    // order, then hand the records to the sink.
+    uint64_t total = 0;
+    for (uint64_t e : round_edges) total += e;
+    if ((rc = ensure_arena(c, std::max<uint64_t>(total, 1), sink, user))) return rc;
+    if (total) {
+        EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->d_eblk, (uint64_t)t * stride,
+                        c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
+        Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)t * stride);
+        HIPCHK(c, launch_edge_compact(ea, c->stream));
+    }
+    uint64_t off = 0;
+    for (int r = 0; r < t; ++r) {
+        c->records.push_back(emp_ctx::Record{round_edges[r], off, L - 1 - r});
+        off += round_edges[r];
+    }
+    c->out_used = total;
     return flush_records(c, sink, user);
 }
 

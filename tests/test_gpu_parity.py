@@ -255,3 +255,25 @@ def test_identical_reads_dedup_big_bucket():
         got = c.run()
         assert c.summary()["n_dedup"] == osum["n_dedup"]
     assert got == want
+
+
+def test_edge_pool_growth_keeps_earlier_rounds():
+    """Reads of a genome give edges in every round; a 1500 x 1500 group sharing a 25-base overlap
+    adds 2.25M edges in a later round, more than the pool's first overflow region holds, so the
+    pool grows after earlier rounds have written their edges."""
+    rng = random.Random(5)
+    genome = "".join(rng.choice("ACGT") for _ in range(4000))
+    reads = [genome[p:p + 40] for p in (rng.randrange(0, 4000 - 40) for _ in range(3000))]
+    common = "".join(rng.choice("ACGT") for _ in range(25))
+    reads += ["".join(rng.choice("ACGT") for _ in range(15)) + common for _ in range(1500)]
+    reads += [common + "".join(rng.choice("ACGT") for _ in range(15)) for _ in range(1500)]
+    r = reads_of(reads)
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 40, min_ovlp=20)
+    assert rc == 0 and osum["n_edges"] > (1 << 21)
+    with native.Context(min_ovlp=20) as c:
+        c.add_reads(r)
+        got = c.run()
+        rounds = c.rounds()
+    big = max(range(len(rounds)), key=lambda i: rounds[i]["edges"])
+    assert rounds[big]["edges"] >= 1500 * 1500 and sum(x["edges"] for x in rounds[:big]) > 0
+    assert got == want
