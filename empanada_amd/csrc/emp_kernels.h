@@ -19,8 +19,10 @@ constexpr bool JOIN_PERSIST = false;
 // Target mean bucket load: the partition fan-out is chosen so that lists average this many
 // entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
 constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
-// Entries per work-group tile of a partition pass, and the widest digit of one pass.
-constexpr int PART_TILE = 2048;
+// Entries per work-group tile of a partition pass, its work-group size, and the widest digit of
+// one pass. A 4096-entry tile gives runs of 16 entries (192 bytes) per bucket at 256 buckets.
+constexpr int PART_TILE = 4096;
+constexpr int PART_THREADS = 512;
 constexpr int PART_MAX_BITS = 8;
 // Linear outputs (survivor lists, the edge pool) are split into SUBS sub-ranges, each with its
 // own cursor: work-group b reserves in sub-range b % SUBS, so no single counter serialises a

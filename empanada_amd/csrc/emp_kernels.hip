@@ -276,11 +276,11 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) {
 // round) the entries are streamed in with non-temporal loads, so that L2 keeps the one column
 // the launch gathers from, and regenerated before they are bucketed (REGEN = a.rg.active).
 template <bool REGEN>
-__global__ void __launch_bounds__(256) k_part(PartArgs a) {
-    constexpr int T = PART_TILE, PER = T / 256;
+__global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_part(PartArgs a) {
+    constexpr int NT = PART_THREADS, T = PART_TILE, PER = T / NT, NB = 1 << PART_MAX_BITS;
+    static_assert(NT >= NB && T % NT == 0, "one bin per thread");
     __shared__ Entry buf[T];
-    __shared__ uint8_t dig[T];
-    __shared__ uint32_t hist[256], loc[256], base[256], sh[4];
+    __shared__ uint32_t hist[NB], loc[NB], base[NB], sh[NT / 64];
     // Regenerating pass: the input groups are the SUBS sub-ranges of a survivor list; the
     // work-groups of read range r (sub-ranges r * SUBS_PER_RANGE ...) all have id = r mod 8.
     uint32_t g, tile0;
@@ -302,9 +302,10 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
         n_g = *a.in_n;
         in = a.in;
     }
+    const uint32_t gb = a.keyed ? (g << a.dbits) : 0u;
     for (uint32_t tile = tile0; (size_t)tile * T < n_g; tile += a.tiles_x) {
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
-        hist[threadIdx.x] = 0;
+        if (threadIdx.x < NB) hist[threadIdx.x] = 0;
         __syncthreads();
         Entry e[PER];
         uint32_t r[PER], d[PER];
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
             uint32_t word[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
-                uint32_t i = threadIdx.x + 256 * q;
+                uint32_t i = threadIdx.x + NT * q;
                 word[q] = 0xFFFFFFFFu;
                 if (i < nt) {
                     e[q] = load_stream(in + t0 + i);
@@ -322,49 +323,47 @@ __global__ void __launch_bounds__(256) k_part(PartArgs a) {
             }
 #pragma unroll
             for (int q = 0; q < PER; ++q)
-                if (threadIdx.x + 256 * q < nt) regen_carry(a.cols, a.rg, e[q], a.sfx, word[q]);
+                if (threadIdx.x + NT * q < nt) regen_carry(a.cols, a.rg, e[q], a.sfx, word[q]);
         } else {
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
-                uint32_t i = threadIdx.x + 256 * q;
+                uint32_t i = threadIdx.x + NT * q;
                 if (i < nt) e[q] = in[t0 + i];
             }
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            uint32_t i = threadIdx.x + 256 * q;
+            uint32_t i = threadIdx.x + NT * q;
             if (i < nt) {
                 d[q] = (e[q].h1 >> a.out_shift) & (F - 1);
                 r[q] = atomicAdd(&hist[d[q]], 1u);
             }
         }
         __syncthreads();
-        const uint32_t h = hist[threadIdx.x];
+        const uint32_t h = threadIdx.x < NB ? hist[threadIdx.x] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_scan<256>(h, sh, tot);
-        loc[threadIdx.x] = ex;
-        const uint32_t gb = a.keyed ? (g << a.dbits) : 0u;
-        base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[gb + threadIdx.x], h) : 0u;
+        const uint32_t ex = block_excl_scan<NT>(h, sh, tot);
+        if (threadIdx.x < NB) {
+            loc[threadIdx.x] = ex;
+            base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[gb + threadIdx.x], h) : 0u;
+        }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            uint32_t i = threadIdx.x + 256 * q;
-            if (i < nt) {
-                uint32_t p = loc[d[q]] + r[q];
-                buf[p] = e[q];
-                dig[p] = (uint8_t)d[q];
-            }
+            uint32_t i = threadIdx.x + NT * q;
+            if (i < nt) buf[loc[d[q]] + r[q]] = e[q];
         }
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < nt; p += 256) {
-            const uint32_t dd = dig[p];
+        for (uint32_t p = threadIdx.x; p < nt; p += NT) {
+            const Entry v = buf[p];
+            const uint32_t dd = (v.h1 >> a.out_shift) & (F - 1);
             const uint32_t slot = base[dd] + (p - loc[dd]);
             const uint32_t bucket = gb + dd;
             if (slot < a.out_cap) {
-                a.out[(size_t)bucket * a.out_cap + slot] = buf[p];
+                a.out[(size_t)bucket * a.out_cap + slot] = v;
             } else if (a.spill) {
                 uint32_t s = atomicAdd(a.spill_n, 1u);
-                if (s < a.spill_cap) a.spill[s] = SpillRec{buf[p], bucket};
+                if (s < a.spill_cap) a.spill[s] = SpillRec{v, bucket};
                 else atomicOr(a.abort, 1u);
             } else {
                 atomicOr(a.abort, 1u);
@@ -1038,9 +1037,9 @@ hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {
     uint64_t grid = (uint64_t)groups * a.tiles_x;
     if (grid == 0) return hipSuccess;
     if (a.rg.active)
-        hipLaunchKernelGGL(k_part<true>, dim3((uint32_t)grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_part<true>, dim3((uint32_t)grid), dim3(PART_THREADS), 0, s, a);
     else
-        hipLaunchKernelGGL(k_part<false>, dim3((uint32_t)grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_part<false>, dim3((uint32_t)grid), dim3(PART_THREADS), 0, s, a);
     return hipGetLastError();
 }
 
