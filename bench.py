@@ -9,6 +9,14 @@ host memory. Metric: Gbases/s = n * L / step time.
 Prints one JSON line with the roofline of the dominant kernel (HIP events recorded by the library
 on its stream around every launch, algorithmic bytes per launch from the live per-round counts)
 and the oracle timed on the host cores over a bounded sample of the same workload.
+
+Two more measurements ride on the same line (skip them with --batched-sets 0 / --served-jobs 0):
+
+- ``batched``: read sets of the batched workload (genome log-uniform over 5 kbp-5 Mbp, L from
+  {100, 150, 250}, coverage 10-50x, -o 31) run through one batched engine, against the plain
+  single-job path run once per set on the same GPU; jobs/s of both, reads in host memory.
+- ``served``: the same distribution arriving as a Poisson stream at stepped offered loads, served
+  by the engine with continuous batching; achieved jobs/s and p50/p99 latency per load.
 """
 import argparse
 import json
@@ -50,15 +58,81 @@ def cpu_baseline(scale, min_ovlp, check_gpu):
     }, same
 
 
+def batched_bench(n_sets):
+    """The first n_sets sets of the batched workload: one batched engine vs the plain path per set."""
+    from empanada_amd import gen, native
+    t = time.perf_counter()
+    sets = [gen.batched_set(i) for i in range(n_sets)]
+    t_gen = time.perf_counter() - t
+    reads = sum(r.shape[0] for r, _ in sets)
+    bases = sum(r.size for r, _ in sets)
+    with native.Batch(collect=False) as b:
+        for r, o in sets[:8]:  # warm-up: allocations, code objects
+            b.submit(r, o)
+        b.run()
+        b.reset()
+        t = time.perf_counter()
+        for r, o in sets:
+            b.submit(r, o)
+        b.run()
+        t_b = time.perf_counter() - t
+        edges_b = sum(b.info(j)["n_edges"] for j in range(n_sets))
+    with native.Context(min_ovlp=sets[0][1]) as c:
+        c.add_reads(sets[0][0])
+        c.run(collect=False)
+        edges_p = 0
+        t = time.perf_counter()
+        for r, o in sets:
+            c.reset()
+            c.add_reads(r)
+            c.run(collect=False)
+            edges_p += c.summary()["n_edges"]
+        t_p = time.perf_counter() - t
+    return {
+        "workload": "%d sets: genome log-uniform 5 kbp-5 Mbp, L in {100,150,250}, coverage 10-50x, 50%% RC, "
+                    "0.5%% substitutions, -o 31 (seeds 1300+i, 1300+i+4096)" % n_sets,
+        "reads": reads, "bases": bases,
+        "batched_s": round(t_b, 4), "plain_s": round(t_p, 4),
+        "batched_jobs_per_s": round(n_sets / t_b, 2), "plain_jobs_per_s": round(n_sets / t_p, 2),
+        "speedup": round(t_p / t_b, 2),
+        "batched_gbases_per_s": round(bases / t_b / 1e9, 3),
+        "edges_equal": edges_b == edges_p,
+        "reads_in": "host memory (both paths copy them to HBM inside the timed region)",
+        "gen_s": round(t_gen, 2),
+    }
+
+
+def served_bench(n_jobs, loads):
+    """Poisson arrivals of batched-workload sets at each offered load (jobs/s)."""
+    from empanada_amd import gen, serve
+    jobs = [gen.batched_set(i) for i in range(n_jobs)]
+    out = []
+    with serve.Server() as srv:
+        srv.run(jobs[:4], [0.0] * 4)  # warm-up
+        for k, rate in enumerate(loads):
+            srv.engine.reset()
+            arr = serve.poisson_arrivals(n_jobs, rate, 1400 + k)
+            res = srv.run(jobs, arr)
+            st = serve.latency_stats(res["latency"])
+            out.append(dict(offered_jobs_per_s=round(rate, 2), achieved_jobs_per_s=round(n_jobs / res["wall"], 2),
+                            p50_ms=round(st["p50_ms"], 2), p99_ms=round(st["p99_ms"], 2),
+                            all_ok=all(s == 0 for s in res["status"])))
+    return {"jobs_per_load": n_jobs, "arrivals": "Poisson, seed 1400 + load index", "loads": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 50 Mbp / 15 M read workload")
-    ap.add_argument("--cpu-scale", type=float, default=0.1)
+    ap.add_argument("--cpu-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="also compare the full-size output with the oracle")
+    ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
+    ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
+    ap.add_argument("--served-loads", default="0.5,1,2,4,8",
+                    help="offered loads as multiples of the plain path's jobs/s")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -136,6 +210,14 @@ def main():
     cpu, same_sample = (None, None)
     if not args.no_cpu:
         cpu, same_sample = cpu_baseline(args.cpu_scale, min_ovlp, check_gpu=True)
+    ctx.close()
+    d_reads.free()
+
+    batched = batched_bench(args.batched_sets) if args.batched_sets > 0 else None
+    served = None
+    if args.served_jobs > 0 and batched:
+        loads = [float(x) * batched["plain_jobs_per_s"] for x in args.served_loads.split(",")]
+        served = served_bench(args.served_jobs, loads)
 
     line = {
         "metric": "Gbases/s end-to-end overlap detection on 1 GPU",
@@ -177,9 +259,9 @@ def main():
         "parity_sample_vs_oracle": same_sample,
         "parity_full_vs_oracle": same_full,
         "gen_s": round(t_gen, 2),
+        "batched": batched,
+        "served": served,
     }
-    ctx.close()
-    d_reads.free()
     print(json.dumps(line), flush=True)
 
 

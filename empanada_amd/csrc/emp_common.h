@@ -1,0 +1,268 @@
+// Device helpers shared by the single-job kernels (emp_kernels.hip) and the batched kernels
+// (emp_batch.hip): regeneration of an entry for a round, block scans, sorts and searches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emp_kernels.h"
+
+namespace emp {
+
+// Regeneration of a live entry for round t (keys of length L - t -> L - t - 1). Suffix list:
+// even ids drop column t, odd ids the complement of column L-1-t; prefix list: the other way
+// round (overlap.cu:266-267, 684-690: rid/2 picks the read, odd ids are complemented). An entry
+// walks its columns forward (t = 0, 1, ...) or backward (L-1, L-2, ...); it gathers its read's
+// word of a column group when it enters the group (or at round 0) and carries the codes of the
+// group's remaining columns, in the order it uses them, in the spare key bits.
+// Codes an entry carries after its regeneration at column `col` of group word `gw` (8 bits):
+// forward, columns c+1 .. 3; backward, columns c-1 .. 0 (c = col % 4), first-used in bits 0-1.
+__device__ __forceinline__ uint32_t carry_of(uint32_t gw, uint32_t col, bool fwd) {
+    const uint32_t c = col & 3u;
+    if (fwd) return gw >> (2 * (c + 1));
+    uint32_t f = 0;
+    for (uint32_t i = 0; i < c; ++i) f |= ((gw >> (2 * (c - 1 - i))) & 3u) << (2 * i);
+    return f;
+}
+
+__device__ __forceinline__ void regen_carry(const uint32_t *__restrict__ cols, const RoundArgs &r, Entry &e, bool sfx,
+                                            uint32_t gathered) {
+    const bool odd = e.rid & 1;
+    const bool fwd = sfx != odd;
+    const uint32_t col = fwd ? r.col_t : r.col_back;
+    uint32_t f = (e.h1 >> 29) | ((e.h2 >> 29) << 3), code;
+    if (gathered != 0xFFFFFFFFu) {
+        code = (gathered >> (2 * (col & 3u))) & 3u;
+        f = carry_of(gathered, col, fwd);
+    } else {
+        code = f & 3u;
+        f >>= 2;
+    }
+    uint32_t h1 = e.h1 & KEY_MASK, h2 = e.h2 & KEY_MASK;
+    const uint32_t ev = enc_of_code(odd ? (code ^ 2u) : code);
+    if (sfx)
+        next_sfx(h1, h2, ev, r.m1, r.m2);
+    else
+        prev_pfx(h1, h2, ev);
+    e.h1 = h1 | ((f & 7u) << 29);
+    e.h2 = h2 | (((f >> 3) & 7u) << 29);
+}
+
+__device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
+    e.h1 = (e.h1 & KEY_MASK) | ((f & 7u) << 29);
+    e.h2 = (e.h2 & KEY_MASK) | (((f >> 3) & 7u) << 29);
+}
+
+// Whether the entry must gather its group word this round, and which word (group index).
+__device__ __forceinline__ bool regen_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
+    const bool fwd = sfx != (bool)(rid & 1);
+    const uint32_t col = fwd ? r.col_t : r.col_back;
+    g = col >> 2;
+    return fwd ? (col & 3u) == 0 : ((col & 3u) == 3 || r.col_t == 0);
+}
+
+__device__ __forceinline__ Entry load_stream(const Entry *p) {
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+    return Entry{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1), __builtin_nontemporal_load(q + 2)};
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block helpers.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* >= NT/64 */, uint32_t &total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        uint32_t s = sh[i];
+        base += i < wv ? s : 0u;
+        tot += s;
+    }
+    total = tot;
+    __syncthreads();
+    return base + x - v;
+}
+
+template <int NT>
+__device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long long v, unsigned long long *sh,
+                                                               unsigned long long &total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    unsigned long long base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        unsigned long long s = sh[i];
+        base += i < wv ? s : 0ull;
+        tot += s;
+    }
+    total = tot;
+    __syncthreads();
+    return base + x - v;
+}
+
+// Bitonic sort of n2 (power of two) records by (key, rank), ascending; `pay` (may be null) is
+// permuted along. Works on LDS or on global memory private to the work-group.
+__device__ inline void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uint32_t n2) {
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t ka = key[i], kb = key[ixj];
+                    uint32_t ra = rank[i], rb = rank[ixj];
+                    bool a_gt = (ka > kb) || (ka == kb && ra > rb);
+                    bool up = (i & k) == 0;
+                    if (a_gt == up) {
+                        key[i] = kb; key[ixj] = ka;
+                        rank[i] = rb; rank[ixj] = ra;
+                        if (pay) { uint32_t t = pay[i]; pay[i] = pay[ixj]; pay[ixj] = t; }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Edge-pool slot of n edges whose sub-range reservation returned c (atomicAdd on the cursor
+// of sub-range `sub`): inside the sub-range when it fits, else the reservation is returned and
+// the edges go to the overflow region; ~0 when that is full too (the host grows it and reruns).
+__device__ __forceinline__ unsigned long long place_edges(unsigned long long c, uint32_t n, uint32_t sub,
+                                                          unsigned long long *cursor, unsigned long long cap,
+                                                          unsigned long long *ovf_cursor, unsigned long long ovf_cap) {
+    if (c + n <= cap) return (unsigned long long)sub * cap + c;
+    atomicAdd(cursor, 0ull - (unsigned long long)n);
+    const unsigned long long o = atomicAdd(ovf_cursor, (unsigned long long)n);
+    return o + n <= ovf_cap ? (unsigned long long)SUBS * cap + o : ~0ull;
+}
+
+__device__ __forceinline__ uint64_t key_of(const Entry &e) {
+    return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK);
+}
+
+// LDS of the join of one bucket: the prefix entries (12 B each), a used flag per prefix, and one
+// region that holds the hash table (chain heads + links) while probing and the matched records
+// afterwards; survivor counts and slots per run.
+template <int CAP, int MCAP, int NT>
+struct JoinShared {
+    uint32_t pk1[CAP], pk2[CAP], prid[CAP];
+    uint8_t used[CAP];
+    union {
+        struct {
+            uint32_t head[CAP];
+            uint16_t nxt[CAP];
+        } h;
+        struct {
+            uint64_t msk[MCAP], mpk[MCAP];
+            uint32_t msr[MCAP], mpr[MCAP], msc[MCAP];
+        } m;
+    } u;
+    uint32_t sh[NT / 64];
+    uint32_t ms, mp, ovf, etot;
+    uint32_t rcnt[2 * NRANGE], rbase[2 * NRANGE];  // survivors per (side, read range), their slots
+    unsigned long long base;
+};
+
+// Position of record i among n records (key, rank) ordered by (key, rank): the number of records
+// smaller than it. (key, rank) pairs are distinct, so the positions of 0..n-1 are a permutation.
+// Eight independent LDS reads per step (the same address for the whole wave: broadcasts).
+__device__ __forceinline__ uint32_t rank_of(const uint64_t *key, const uint32_t *rank, uint32_t n, uint32_t i,
+                                            uint64_t &k, uint32_t &r) {
+    k = key[i];
+    r = rank[i];
+    uint32_t pos = 0, j = 0;
+    for (; j + 8 <= n; j += 8) {
+        uint64_t kk[8];
+        uint32_t rr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { kk[u] = key[j + u]; rr[u] = rank[j + u]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pos += (uint32_t)(kk[u] < k) | ((uint32_t)(kk[u] == k) & (uint32_t)(rr[u] < r));
+    }
+    for (; j < n; ++j) {
+        const uint64_t kj = key[j];
+        pos += (uint32_t)(kj < k) | ((uint32_t)(kj == k) & (uint32_t)(rank[j] < r));
+    }
+    return pos;
+}
+
+// Binary searches over sorted global-memory arrays.
+__device__ __forceinline__ uint32_t lower_u64(const uint64_t *k, uint32_t lo, uint32_t hi, uint64_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (k[m] < x) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t upper_u64(const uint64_t *k, uint32_t lo, uint32_t hi, uint64_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (x < k[m]) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool has_u32(const uint32_t *r, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        uint32_t m = (lo + hi) >> 1;
+        if (r[m] < x) lo = m + 1; else hi = m;
+    }
+    return lo < hi && r[lo] == x;
+}
+
+// Collects the entries of bucket b (slots + spilled records) into key/value scratch, padded with
+// sentinels to n2. as_rank: store rid_rank(rid) (join) or rid (dedup).
+__device__ inline void gather_bucket(const Entry *slots, uint32_t cnt, uint32_t cap, const SpillRec *spill,
+                              const uint32_t *spill_n, uint32_t spill_cap, uint32_t b, uint64_t *key, uint32_t *val,
+                              uint32_t n2, bool as_rank, uint32_t *s_pos) {
+    const uint32_t nm = min(cnt, cap);
+    for (uint32_t i = threadIdx.x; i < nm; i += blockDim.x) {
+        Entry e = slots[(size_t)b * cap + i];
+        key[i] = key_of(e);
+        val[i] = as_rank ? rid_rank(e.rid) : e.rid;
+    }
+    if (threadIdx.x == 0) *s_pos = nm;
+    __syncthreads();
+    if (cnt > cap) {
+        const uint32_t sn = min(*spill_n, spill_cap);
+        for (uint32_t j = threadIdx.x; j < sn; j += blockDim.x) {
+            if (spill[j].bucket == b) {
+                uint32_t k = atomicAdd(s_pos, 1u);
+                if (k < cnt) {
+                    Entry e = spill[j].e;
+                    key[k] = key_of(e);
+                    val[k] = as_rank ? rid_rank(e.rid) : e.rid;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = cnt + threadIdx.x; i < n2; i += blockDim.x) { key[i] = ~0ull; val[i] = ~0u; }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n) {
+    uint32_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+}  // namespace emp

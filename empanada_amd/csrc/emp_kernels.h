@@ -64,6 +64,10 @@ struct MapArgs {
     uint32_t *cols;          // column store (grouped layout of emp_device.h)
     Entry *entries;          // linear entries, index 2k and 2k+1
     uint32_t *err;           // set to 1 when a byte is not ACGTacgt
+    // gs != 0: a batched job's column region instead of the chunked store: word of read k,
+    // group g at cbase + g * gs + k / 4 (k and entry ids local to the job)
+    uint64_t cbase;
+    uint32_t gs;
 };
 
 // One partition pass: input groups (keys sharing their top bits) -> output buckets refined by

@@ -323,7 +323,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
     uint64_t done = 0;
     while (done < nreads) {
         uint32_t m = (uint32_t)std::min<uint64_t>(nreads - done, 1u << 30);
-        MapArgs a;
+        MapArgs a{};
         a.reads = d + done * (uint64_t)c->L;
         a.n = m;
         a.L = c->L;

@@ -83,6 +83,42 @@ WORKLOADS = {
 }
 
 
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64(state: int):
+    """One SplitMix64 step: (new state, output)."""
+    state = (state + 0x9E3779B97F4A7C15) & _M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return state, z ^ (z >> 31)
+
+
+def batched_set_params(i: int, base: int = 1300, n_sets: int = 4096):
+    """Parameters of read set i of the batched workload: genome length log-uniform over
+    [5 kbp, 5 Mbp], read length from {100, 150, 250}, coverage uniform over [10, 50], all drawn by
+    SplitMix64 from seed base + i. Returns dict(G, L, n, cov, o, seeds)."""
+    st = base + i
+    st, a = _splitmix64(st)
+    st, b = _splitmix64(st)
+    st, c = _splitmix64(st)
+    u = (a >> 11) * 2.0 ** -53
+    G = int(round(5e3 * 1000.0 ** u))
+    L = (100, 150, 250)[b % 3]
+    cov = 10.0 + 40.0 * ((c >> 11) * 2.0 ** -53)
+    n = max(1, int(round(G * cov / L)))
+    return dict(G=G, L=L, n=n, cov=cov, o=31, seeds=(base + i, base + i + n_sets))
+
+
+def batched_set(i: int, base: int = 1300, n_sets: int = 4096):
+    """Reads of set i of the batched workload (50 % reverse-complemented, 0.5 % substitutions).
+    Returns (reads, min_ovlp)."""
+    p = batched_set_params(i, base, n_sets)
+    g = genome(p["G"], p["seeds"][0])
+    return reads(g, p["n"], p["L"], 0.5, 0.005, p["seeds"][1]), p["o"]
+
+
 def workload_reads(name: str, scale: float = 1.0):
     """Reads of a named workload; scale < 1 shrinks genome and read count together (same
     coverage, same per-base work), for bounded samples. Returns (reads, min_ovlp)."""

@@ -23,6 +23,10 @@ EXPORTS = [
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
     "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
     "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll",
+    "emp_batch_opts_default", "emp_batch_create", "emp_batch_destroy", "emp_batch_last_error", "emp_batch_submit",
+    "emp_batch_submit_fastq", "emp_batch_job_bytes", "emp_batch_step", "emp_batch_run", "emp_batch_job_info",
+    "emp_batch_reset", "emp_batch_clock_ms", "emp_batch_kernel_count", "emp_batch_kernel_info",
+    "emp_batch_kernel_clear", "emp_batch_set_profile", "emp_batch_stream",
 ]
 
 
@@ -43,6 +47,22 @@ class Summary(ctypes.Structure):
                 ("n_dedup", ctypes.c_uint64), ("n_records", ctypes.c_uint64), ("n_edges", ctypes.c_uint64),
                 ("map_ms", ctypes.c_double), ("overlap_ms", ctypes.c_double)]
 
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("stream", ctypes.c_void_p), ("max_active", ctypes.c_uint32)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("read_len", ctypes.c_int32),
+                ("min_ovlp", ctypes.c_int32), ("rounds", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("n_reads", ctypes.c_uint64), ("n_dedup", ctypes.c_uint64), ("n_edges", ctypes.c_uint64),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
+
+
+JOB_SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
+                            ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32))
+JOB_DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32)
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32))
@@ -83,6 +103,29 @@ def lib():
         L.emp_stage_fingerprints.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_columns.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_roll.argtypes = [i32, vp, vp, u64, i32, i32, vp]
+        L.emp_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
+        L.emp_batch_create.argtypes = [ctypes.POINTER(BatchOpts), ctypes.POINTER(vp)]
+        L.emp_batch_destroy.argtypes = [vp]
+        L.emp_batch_last_error.restype = cp
+        L.emp_batch_last_error.argtypes = [vp]
+        L.emp_batch_submit.argtypes = [vp, vp, u64, i32, i32, i32, ctypes.POINTER(i32)]
+        L.emp_batch_submit_fastq.argtypes = [vp, vp, ctypes.c_size_t, i32, ctypes.POINTER(i32)]
+        L.emp_batch_job_bytes.restype = u64
+        L.emp_batch_job_bytes.argtypes = [u64, i32]
+        L.emp_batch_step.argtypes = [vp, JOB_SINK, JOB_DONE, vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.emp_batch_run.argtypes = [vp, JOB_SINK, JOB_DONE, vp]
+        L.emp_batch_job_info.argtypes = [vp, i32, ctypes.POINTER(JobInfo)]
+        L.emp_batch_reset.argtypes = [vp]
+        L.emp_batch_clock_ms.restype = ctypes.c_double
+        L.emp_batch_clock_ms.argtypes = [vp]
+        L.emp_batch_kernel_count.restype = i32
+        L.emp_batch_kernel_count.argtypes = [vp]
+        L.emp_batch_kernel_info.argtypes = [vp, i32, ctypes.POINTER(cp), ctypes.POINTER(u64),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.emp_batch_kernel_clear.argtypes = [vp]
+        L.emp_batch_set_profile.argtypes = [vp, i32]
+        L.emp_batch_stream.restype = vp
+        L.emp_batch_stream.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -211,6 +254,144 @@ class Context:
 
     def set_profile(self, on: bool):
         lib().emp_set_profile(self._h, 1 if on else 0)
+
+
+def records_bytes(records):
+    """output.bin bytes of a list of (n, ovlp, from, to) records (overlap.cu:148-161)."""
+    out = []
+    for n, ovlp, fr, to in records:
+        out.append(np.array([n], dtype=np.uint64).tobytes())
+        out.append(np.array([ovlp], dtype=np.int32).tobytes())
+        if n:
+            out.append(fr.tobytes())
+            out.append(to.tobytes())
+    return b"".join(out)
+
+
+class Batch:
+    """Many overlap jobs sharing one engine (wraps emp_batch). Each job's output is collected as
+    its output.bin bytes."""
+
+    def __init__(self, hbm_budget=0, max_active=0, device=0, profile=False, stream=None, collect=True):
+        o = BatchOpts()
+        lib().emp_batch_opts_default(ctypes.byref(o))
+        o.hbm_budget, o.max_active, o.device, o.profile, o.stream = hbm_budget, max_active, device, \
+            1 if profile else 0, stream
+        self._h = ctypes.c_void_p()
+        rc = lib().emp_batch_create(ctypes.byref(o), ctypes.byref(self._h))
+        if rc:
+            raise EmpError(rc, "emp_batch_create failed")
+        self.collect = collect
+        self.records = {}      # job -> list of (n, ovlp, from, to)
+        self.status = {}       # job -> status of finished jobs
+        self.finished = []     # job ids in completion order
+        self._keep = {}        # job -> host reads kept alive until admission
+
+        @JOB_SINK
+        def sink(user, job, n, ovlp, fr, to):
+            if self.collect:
+                a = np.ctypeslib.as_array(fr, shape=(n,)).copy() if n else None
+                b = np.ctypeslib.as_array(to, shape=(n,)).copy() if n else None
+                self.records.setdefault(job, []).append((n, ovlp, a, b))
+            return 0
+
+        @JOB_DONE
+        def done(user, job, status):
+            self.status[job] = status
+            self.finished.append(job)
+            self._keep.pop(job, None)
+
+        self._sink, self._done = sink, done
+
+    def _check(self, rc):
+        if rc:
+            raise EmpError(rc, lib().emp_batch_last_error(self._h).decode(errors="replace"))
+
+    def close(self):
+        if self._h:
+            lib().emp_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib().emp_batch_stream(self._h)
+
+    def submit(self, reads: np.ndarray, min_ovlp: int) -> int:
+        r = np.ascontiguousarray(reads, dtype=np.uint8)
+        n, L = r.shape if r.ndim == 2 else (0, 0)
+        job = ctypes.c_int32()
+        self._check(lib().emp_batch_submit(self._h, r.ctypes.data if n else None, n, L, min_ovlp, 0,
+                                           ctypes.byref(job)))
+        self._keep[job.value] = r
+        return job.value
+
+    def submit_device(self, ptr: int, n: int, L: int, min_ovlp: int) -> int:
+        job = ctypes.c_int32()
+        self._check(lib().emp_batch_submit(self._h, ptr, n, L, min_ovlp, 1, ctypes.byref(job)))
+        return job.value
+
+    def submit_fastq(self, data: bytes, min_ovlp: int) -> int:
+        job = ctypes.c_int32()
+        self._check(lib().emp_batch_submit_fastq(self._h, data, len(data), min_ovlp, ctypes.byref(job)))
+        return job.value
+
+    def step(self):
+        """One step; returns (active, queued)."""
+        a, q = ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().emp_batch_step(self._h, self._sink, self._done, None, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def run(self):
+        self._check(lib().emp_batch_run(self._h, self._sink, self._done, None))
+
+    def output(self, job: int) -> bytes:
+        return records_bytes(self.records.get(job, []))
+
+    def info(self, job: int):
+        i = JobInfo()
+        self._check(lib().emp_batch_job_info(self._h, job, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in JobInfo._fields_ if f != "pad"}
+
+    def clock_ms(self) -> float:
+        return lib().emp_batch_clock_ms(self._h)
+
+    def reset(self):
+        self._check(lib().emp_batch_reset(self._h))
+        self.records.clear()
+        self.status.clear()
+        self.finished.clear()
+
+    def kernels(self):
+        out = {}
+        for i in range(lib().emp_batch_kernel_count(self._h)):
+            name, launches = ctypes.c_char_p(), ctypes.c_uint64()
+            ms, by = ctypes.c_double(), ctypes.c_double()
+            lib().emp_batch_kernel_info(self._h, i, ctypes.byref(name), ctypes.byref(launches), ctypes.byref(ms),
+                                        ctypes.byref(by))
+            out[name.value.decode()] = dict(launches=launches.value, ms=ms.value, bytes=by.value)
+        return out
+
+    def kernel_clear(self):
+        lib().emp_batch_kernel_clear(self._h)
+
+    def set_profile(self, on: bool):
+        lib().emp_batch_set_profile(self._h, 1 if on else 0)
+
+
+def job_bytes(n: int, L: int) -> int:
+    return lib().emp_batch_job_bytes(n, L)
 
 
 def stage_fingerprints(reads: np.ndarray, device=0) -> np.ndarray:
