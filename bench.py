@@ -31,12 +31,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(scale, min_ovlp, check_gpu):
+def cpu_baseline(workload, scale, min_ovlp, check_gpu):
     """The oracle over a bounded sample (same genome/read model and coverage, `scale` of the size)."""
     import oracle_lib
     from empanada_amd import gen, native
 
-    reads, _ = gen.workload_reads("incore", scale)
+    reads, _ = gen.workload_reads(workload, scale)
     n, L = reads.shape
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
@@ -53,8 +53,8 @@ def cpu_baseline(scale, min_ovlp, check_gpu):
         "unit": "Gbases/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%.0f kbp genome, %d reads x %d bp (%.0f%% of the workload, same coverage), %.2f s" % (
-            5e7 * scale / 1e3, n, L, 100 * scale, dt),
+        "sample": "%.0f kbp genome, %d reads x %d bp (%g%% of the workload, same coverage), %.2f s" % (
+            gen.WORKLOADS[workload]["G"] * scale / 1e3, n, L, 100 * scale, dt),
     }, same
 
 
@@ -125,8 +125,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 50 Mbp / 15 M read workload")
-    ap.add_argument("--cpu-scale", type=float, default=1.0)
+    ap.add_argument("--workload", choices=("incore", "ooc"), default="incore",
+                    help="incore: 50 Mbp, 15 M x 100 bp, -o 63; ooc: 3.1 Gbp with repeats, 413 M x 150 bp, -o 75")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload (genome and reads)")
+    ap.add_argument("--cpu-scale", type=float, default=None,
+                    help="fraction of the workload the oracle runs on (default 1.0 in-core, 0.01 ooc)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="also compare the full-size output with the oracle")
     ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
@@ -151,38 +154,55 @@ def main():
 
     from empanada_amd import gen, hipmem, native
 
+    ooc = args.workload == "ooc"
+    cpu_scale = args.cpu_scale if args.cpu_scale is not None else (0.01 if ooc else 1.0)
+    wl = gen.WORKLOADS[args.workload]
     t_gen = time.perf_counter()
-    reads, min_ovlp = gen.workload_reads("incore", args.scale)
+    reads, min_ovlp = gen.workload_reads(args.workload, args.scale)
     n, L = reads.shape
     t_gen = time.perf_counter() - t_gen
-    ctx = native.Context(min_ovlp=min_ovlp, profile=False)
-    d_reads = hipmem.DeviceBuffer.from_array(reads)
-    hipmem.synchronize()
+    # 2n entries must stay below M = reduce_mem / 16 (in-core order): -r 96g for the 3.1 Gbp case
+    ctx = native.Context(min_ovlp=min_ovlp, profile=False, reduce_mem=(96 << 30) if ooc else (1 << 30))
     stream = ctx.stream
+    dev = {"reads": None}
+
+    def upload():
+        if dev["reads"] is None:
+            dev["reads"] = hipmem.DeviceBuffer.from_array(reads)
+            hipmem.synchronize()
 
     def step():
         ctx.reset()
-        ctx.add_reads_device(d_reads.address, n, L)
+        ctx.add_reads_device(dev["reads"].address, n, L)
+        if ooc:  # the 3.1 Gbp reads leave HBM once mapped (62 GB)
+            dev["reads"].free()
+            dev["reads"] = None
         ctx.run(collect=False)
 
     for _ in range(args.warmup):
+        upload()
         step()
     hipmem.synchronize()
 
     def timed(profile):
+        """Per step: reads uploaded (outside the timed region), then the step bracketed by device
+        synchronisation and HIP events on the library stream."""
         ctx.set_profile(profile)
         ctx.kernel_clear()
         ev0, ev1 = hipmem.Event(), hipmem.Event()
-        hipmem.synchronize()
-        t0 = time.perf_counter()
-        ev0.record(stream)
+        wall = gpu = 0.0
         for _ in range(args.steps):
+            upload()
+            hipmem.synchronize()
+            t0 = time.perf_counter()
+            ev0.record(stream)
             step()
-        ev1.record(stream)
-        ev1.synchronize()
-        hipmem.synchronize()
-        t1 = time.perf_counter()
-        return (t1 - t0) * 1e3 / args.steps, ev0.elapsed_ms(ev1) / args.steps
+            ev1.record(stream)
+            ev1.synchronize()
+            hipmem.synchronize()
+            wall += time.perf_counter() - t0
+            gpu += ev0.elapsed_ms(ev1)
+        return wall * 1e3 / args.steps, gpu / args.steps
 
     # the metric: a timed region without per-launch events
     ms_per_step, gpu_ms_region = timed(False)
@@ -200,7 +220,7 @@ def main():
     value = n * L / (ms_per_step * 1e-3) / 1e9
 
     same_full = None
-    if args.check:
+    if args.check and not ooc:
         import oracle_lib
         rc, want, _, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
         ctx.reset()
@@ -209,11 +229,13 @@ def main():
 
     cpu, same_sample = (None, None)
     if not args.no_cpu:
-        cpu, same_sample = cpu_baseline(args.cpu_scale, min_ovlp, check_gpu=True)
+        cpu, same_sample = cpu_baseline(args.workload, cpu_scale, min_ovlp, check_gpu=True)
     ctx.close()
-    d_reads.free()
+    if dev["reads"] is not None:
+        dev["reads"].free()
+    del reads
 
-    batched = batched_bench(args.batched_sets) if args.batched_sets > 0 else None
+    batched = batched_bench(args.batched_sets) if args.batched_sets > 0 and not ooc else None
     served = None
     if args.served_jobs > 0 and batched:
         loads = [float(x) * batched["plain_jobs_per_s"] for x in args.served_loads.split(",")]
@@ -233,8 +255,10 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": "in-core: %d bp genome, %d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d" % (
-                int(5e7 * args.scale), n, L, min_ovlp),
+            "workload": ("%s: %d bp genome%s, %d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d%s" % (
+                args.workload, int(wl["G"] * args.scale),
+                " (10% repeats: 300 bp and 6 kbp families, 16 copies, 1% divergence)" if ooc else "",
+                n, L, min_ovlp, ", -r 96g" if ooc else "")),
             "reads": n, "read_len": L, "min_ovlp": min_ovlp,
             "dedup": summ["n_dedup"], "rounds": summ["n_rounds"], "edges": summ["n_edges"],
             "records": summ["n_records"],

@@ -80,6 +80,10 @@ def write_fastq(path: str, r: np.ndarray) -> None:
 WORKLOADS = {
     "small": dict(G=10**6, n=20000, L=100, p_rc=0.0, p_sub=0.0, o=31, seeds=(1000, 1001)),
     "incore": dict(G=5 * 10**7, n=15_000_000, L=100, p_rc=0.5, p_sub=0.005, o=63, seeds=(1100, 1101)),
+    # 10 % repeats: 5 % in 300 bp families and 5 % in 6 kbp families, 16 copies each, 1 % per-copy
+    # divergence (repeat seeds 1207 and 1208); 20x coverage
+    "ooc": dict(G=3_100_000_000, n=413_333_333, L=150, p_rc=0.5, p_sub=0.005, o=75, seeds=(1200, 1201),
+                repeats=((0.05, 300, 16, 0.01, 1207), (0.05, 6000, 16, 0.01, 1208))),
 }
 
 
@@ -126,4 +130,8 @@ def workload_reads(name: str, scale: float = 1.0):
     G = max(int(w["G"] * scale), 10 * w["L"])
     n = max(int(w["n"] * scale), 1)
     g = genome(G, w["seeds"][0])
+    for frac, len_f, copies, div, seed in w.get("repeats", ()):
+        # each family size on its own grid of slots; copies of the two sizes may overlap
+        used = np.zeros(G // len_f, dtype=np.uint8)
+        add_repeats(g, seed, frac, len_f, copies, div, used)
     return reads(g, n, w["L"], w["p_rc"], w["p_sub"], w["seeds"][1]), w["o"]

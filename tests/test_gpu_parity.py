@@ -277,3 +277,17 @@ def test_edge_pool_growth_keeps_earlier_rounds():
     big = max(range(len(rounds)), key=lambda i: rounds[i]["edges"])
     assert rounds[big]["edges"] >= 1500 * 1500 and sum(x["edges"] for x in rounds[:big]) > 0
     assert got == want
+
+
+def test_repeat_genome_sample_vs_oracle():
+    """A 0.05 % sample of the 3.1 Gbp workload: 1.55 Mbp genome with 300 bp and 6 kbp repeat
+    families (16 copies, 1 % divergence), 150 bp reads at 20x, -o 75: repeat copies give equal keys
+    across distant reads."""
+    r, o = gen.workload_reads("ooc", 0.0005)
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], r.shape[1], min_ovlp=o)
+    assert rc == 0
+    with native.Context(min_ovlp=o) as c:
+        c.add_reads(r)
+        got = c.run()
+        assert c.summary()["n_edges"] == osum["n_edges"]
+    assert got == want
