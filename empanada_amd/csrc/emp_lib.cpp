@@ -18,6 +18,7 @@
 #include "../../include/empanada.h"
 #include "../../include/empanada_stages.h"
 #include "emp_fastq.h"
+#include "emp_fastq_gpu.h"
 #include "emp_kernels.h"
 
 using namespace emp;
@@ -100,6 +101,22 @@ struct emp_ctx {
     uint32_t stage_n = 0;
     int stage_cur = 0;
     size_t stage_bytes = 0;
+
+    // FASTQ blocks parsed on the device: the block, its newline positions and per-record
+    // arrays, and the kept reads not yet mapped (fewer than 16 between calls)
+    uint8_t *d_fq = nullptr;
+    uint64_t fq_cap = 0;
+    uint32_t *d_fq_cnt = nullptr;
+    unsigned long long *d_fq_choff = nullptr;
+    uint64_t fq_chunk_cap = 0, fq_choff_cap = 0;
+    uint64_t *d_fq_nl = nullptr;
+    uint64_t fq_nl_cap = 0;
+    uint32_t *d_fq_kept = nullptr, *d_fq_len = nullptr;
+    unsigned long long *d_fq_rank = nullptr;
+    uint64_t fq_rec_cap = 0, fq_len_cap = 0, fq_rank_cap = 0;
+    FqResult *d_fq_res = nullptr, *h_fq_res = nullptr;
+    uint8_t *d_acc = nullptr;
+    uint64_t acc_bytes = 0, acc_n = 0;
 
     // overlap phase, per list side (0 = suffixes, 1 = prefixes)
     Entry *d_lin[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // survivor sets [set][side]
@@ -300,7 +317,7 @@ void free_reads_storage(emp_ctx *c) {
 int set_len(emp_ctx *c, int L) {
     if (L <= 0 || L > MAX_RD_LEN) return fail(c, EMP_ERR_READLEN, "read length %d outside [1, 255]", L);
     if (c->L == L) return EMP_OK;
-    if (c->n == 0 && c->stage_n == 0) {
+    if (c->n == 0 && c->stage_n == 0 && c->acc_n == 0) {
         // first read of a run: (re)size the per-length storage
         if (c->L != 0) free_reads_storage(c);
         c->L = L;
@@ -368,8 +385,12 @@ int stage_submit(emp_ctx *c) {
     return EMP_OK;
 }
 
+int acc_to_stage(emp_ctx *c);
+
 int stage_read(emp_ctx *c, const char *seq, size_t len) {
-    int rc = set_len(c, (int)std::min<size_t>(len, 1u << 20));
+    int rc = acc_to_stage(c);
+    if (rc) return rc;
+    rc = set_len(c, (int)std::min<size_t>(len, 1u << 20));
     if (rc) return rc;
     if (!c->h_stage[0]) {
         c->stage_bytes = (size_t)STAGE_READS * c->L;
@@ -383,7 +404,166 @@ int stage_read(emp_ctx *c, const char *seq, size_t len) {
     return EMP_OK;
 }
 
-int read_cb(void *user, const char *seq, size_t len) { return stage_read((emp_ctx *)user, seq, len); }
+
+// Grows a device buffer to at least `need` elements (contents dropped).
+template <class T>
+int ensure_dev(emp_ctx *c, T **p, uint64_t &cap, uint64_t need) {
+    if (need <= cap) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, *p);
+    *p = nullptr;
+    const uint64_t n = std::max<uint64_t>(need + need / 4, 1024);
+    void *q = nullptr;
+    int rc = dalloc(c, &q, n * sizeof(T));
+    if (rc) return rc;
+    *p = (T *)q;
+    cap = n;
+    return EMP_OK;
+}
+
+// Grows a device byte buffer to at least `need` bytes, keeping its contents.
+int ensure_dev_keep(emp_ctx *c, uint8_t **p, uint64_t &cap, uint64_t need) {
+    if (need <= cap) return EMP_OK;
+    const uint64_t n = std::max<uint64_t>(need + need / 4, 1 << 20);
+    void *q = nullptr;
+    int rc = dalloc(c, &q, n);
+    if (rc) return rc;
+    if (*p && cap) HIPCHK(c, hipMemcpyAsync(q, *p, cap, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, *p);
+    *p = (uint8_t *)q;
+    cap = n;
+    return EMP_OK;
+}
+
+// Parsed reads left in the device accumulation go to the host stage, ahead of reads added there
+// (they are fewer than 16).
+int acc_to_stage(emp_ctx *c) {
+    if (c->acc_n == 0) return EMP_OK;
+    const uint64_t m = c->acc_n;
+    c->acc_n = 0;
+    std::vector<char> tmp(m * (uint64_t)c->L);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->d_acc, tmp.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint64_t i = 0; i < m; ++i) {
+        int rc = stage_read(c, tmp.data() + i * (uint64_t)c->L, (size_t)c->L);
+        if (rc) return rc;
+    }
+    return EMP_OK;
+}
+
+int read_result(emp_ctx *c) {
+    HIPCHK(c, hipMemcpyAsync(c->h_fq_res, c->d_fq_res, sizeof(FqResult), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMP_OK;
+}
+
+// read_block + the map phase on the device (preprocess.cu:195-237, fingerprint.cu:75-153): the
+// block is copied to HBM and parsed there; its kept reads join the device accumulation, which is
+// mapped in multiples of 16 reads (the rest waits for the next block or the run).
+int add_fastq_gpu(emp_ctx *c, const char *data, size_t size, bool final, size_t *consumed) {
+    *consumed = 0;
+    if (size == 0) return EMP_OK;
+    int rc;
+    if (c->stage_n) {  // reads staged on the host come first
+        const uint64_t b = (uint64_t)c->stage_n * c->L;
+        if ((rc = ensure_dev(c, &c->d_acc, c->acc_bytes, c->acc_n * c->L + b))) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_acc + c->acc_n * c->L, c->h_stage[c->stage_cur], b, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->acc_n += c->stage_n;
+        c->stage_n = 0;
+    }
+    const uint64_t nch = fq_chunks(size);
+    if ((rc = ensure_dev(c, &c->d_fq, c->fq_cap, size))) return rc;
+    if ((rc = ensure_dev(c, &c->d_fq_cnt, c->fq_chunk_cap, nch + 1))) return rc;
+    if ((rc = ensure_dev(c, &c->d_fq_choff, c->fq_choff_cap, nch + 1))) return rc;
+    if (!c->d_fq_res) {
+        void *q = nullptr;
+        if ((rc = dalloc(c, &q, sizeof(FqResult)))) return rc;
+        c->d_fq_res = (FqResult *)q;
+        HIPCHK(c, hipHostMalloc((void **)&c->h_fq_res, sizeof(FqResult), hipHostMallocDefault));
+    }
+    FqResult &R = *c->h_fq_res;
+    R = FqResult{};
+    R.first_bad = R.first_kept = R.first_len = ~0ull;
+    HIPCHK(c, hipMemcpyAsync(c->d_fq_res, c->h_fq_res, sizeof(FqResult), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_fq, data, size, hipMemcpyHostToDevice, c->stream));
+    {
+        Timed t(c, "k_fastq", 2.0 * size);
+        HIPCHK(c, launch_fq_count(c->d_fq, size, c->d_fq_cnt, c->d_fq_choff, c->stream));
+    }
+    unsigned long long nl = 0;
+    HIPCHK(c, hipMemcpyAsync(&nl, c->d_fq_choff + nch, sizeof nl, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = ensure_dev(c, &c->d_fq_nl, c->fq_nl_cap, nl + 1))) return rc;
+    {
+        Timed t(c, "k_fastq", 1.0 * size);
+        HIPCHK(c, launch_fq_frame(c->d_fq, size, final ? 1 : 0, c->d_fq_choff, c->d_fq_nl, c->d_fq_res, c->stream));
+    }
+    if ((rc = read_result(c))) return rc;
+    if (R.err == FQ_ERR_FORMAT)
+        return fail(c, EMP_ERR_FORMAT, "malformed FASTQ block: it must start with '@' and hold a complete record");
+    const uint64_t recs = R.records;
+    if (recs) {
+        if ((rc = ensure_dev(c, &c->d_fq_kept, c->fq_rec_cap, recs))) return rc;
+        if ((rc = ensure_dev(c, &c->d_fq_len, c->fq_len_cap, recs))) return rc;
+        if ((rc = ensure_dev(c, &c->d_fq_rank, c->fq_rank_cap, recs + 1))) return rc;
+        {
+            Timed t(c, "k_fastq", 1.0 * size);
+            HIPCHK(c, launch_fq_classify(c->d_fq, size, c->d_fq_nl, recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len,
+                                         c->d_fq_rank, c->stream));
+        }
+        if ((rc = read_result(c))) return rc;
+    }
+    uint64_t kept = 0;
+    uint32_t L = (uint32_t)c->L;
+    if (R.first_kept != ~0ull) {
+        if (c->n == 0 && c->stage_n == 0 && c->acc_n == 0) {  // the first read sets the length
+            HIPCHK(c, hipMemcpyAsync(&L, c->d_fq_len + R.first_kept, sizeof L, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        HIPCHK(c, launch_fq_lencheck(c->d_fq_kept, c->d_fq_len, recs, L, c->d_fq_res, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&kept, c->d_fq_rank + recs, sizeof kept, hipMemcpyDeviceToHost, c->stream));
+        if ((rc = read_result(c))) return rc;
+    }
+    // the first offending record decides the error, as the host reader meets them in order
+    if (R.first_bad != ~0ull && R.first_bad < R.first_len)
+        return fail(c, EMP_ERR_BASE, "a kept read holds a byte other than ACGTacgt");
+    if (R.first_len != ~0ull) {
+        if (L > MAX_RD_LEN) return fail(c, EMP_ERR_READLEN, "read length %u outside [1, 255]", L);
+        return fail(c, EMP_ERR_READLEN, "All reads must be of equal length (%d vs other)", (int)L);
+    }
+    if (kept) {
+        if ((rc = set_len(c, (int)L))) return rc;
+        if ((rc = ensure_dev_keep(c, &c->d_acc, c->acc_bytes, (c->acc_n + kept) * (uint64_t)L))) return rc;
+        {
+            Timed t(c, "k_fastq", 2.0 * kept * L);
+            HIPCHK(c, launch_fq_copy(c->d_fq, c->d_fq_nl, recs, c->d_fq_kept, c->d_fq_rank, L,
+                                     c->d_acc + c->acc_n * (uint64_t)L, c->stream));
+        }
+        c->acc_n += kept;
+        const uint64_t m = c->acc_n & ~15ull;
+        if (m) {
+            if ((rc = map_batch(c, c->d_acc, m))) return rc;
+            const uint64_t rest = c->acc_n - m;  // fewer than 16 and m >= 16: no overlap
+            if (rest)
+                HIPCHK(c, hipMemcpyAsync(c->d_acc, c->d_acc + m * (uint64_t)L, rest * (uint64_t)L,
+                                         hipMemcpyDeviceToDevice, c->stream));
+            c->acc_n = rest;
+        }
+    }
+    *consumed = (size_t)R.use;
+    return EMP_OK;
+}
+
+// Maps what the device accumulation still holds (the run's last batch).
+int flush_acc(emp_ctx *c) {
+    if (c->acc_n == 0) return EMP_OK;
+    const uint64_t m = c->acc_n;
+    c->acc_n = 0;
+    return map_batch(c, c->d_acc, m);
+}
 
 // Partition plan of a list of x entries into 2^bb final buckets: passes of at most
 // PART_MAX_BITS bits; intermediate buckets get capacity x / 2^bits * slack + 1024.
@@ -825,7 +1005,9 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
 }
 
 int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
-    int rc = stage_submit(c);
+    int rc = flush_acc(c);
+    if (rc) return rc;
+    rc = stage_submit(c);
     if (rc) return rc;
     rc = sync(c);
     if (rc) return rc;
@@ -1145,6 +1327,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     if (c->h_from) (void)hipHostFree(c->h_from);
     if (c->h_to) (void)hipHostFree(c->h_to);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->h_fq_res) (void)hipHostFree(c->h_fq_res);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -1159,11 +1342,7 @@ const char *emp_last_error(const emp_ctx *c) { return c ? c->err.c_str() : "no c
 int emp_add_fastq(emp_ctx *c, const char *data, size_t size, int final, size_t *consumed) {
     if (!c || (!data && size) || !consumed) return EMP_ERR_ARG;
     (void)hipSetDevice(c->device);
-    int rc = parse_fastq_block(data, size, final != 0, consumed, read_cb, c);
-    if (rc == EMP_ERR_FORMAT)
-        return fail(c, rc, "malformed FASTQ block: it must start with '@' and hold a complete record");
-    if (rc == EMP_ERR_BASE) return fail(c, rc, "a kept read holds a byte other than ACGTacgt");
-    return rc;
+    return add_fastq_gpu(c, data, size, final != 0, consumed);
 }
 
 int emp_add_reads(emp_ctx *c, const char *reads, uint64_t n, int32_t read_len) {
@@ -1186,6 +1365,7 @@ int emp_add_reads_device(emp_ctx *c, const void *d_reads, uint64_t n, int32_t re
     (void)hipSetDevice(c->device);
     int rc = set_len(c, read_len);
     if (rc) return rc;
+    if ((rc = flush_acc(c))) return rc;
     if ((rc = stage_submit(c))) return rc;
     if ((rc = map_batch(c, (const uint8_t *)d_reads, n))) return rc;
     return sync(c);
@@ -1220,6 +1400,7 @@ int emp_reset(emp_ctx *c) {
     collect_timings(c);
     c->n = 0;
     c->stage_n = 0;
+    c->acc_n = 0;
     c->stage_busy[0] = c->stage_busy[1] = false;
     c->rounds.clear();
     c->summary = emp_summary{};

@@ -291,3 +291,46 @@ def test_repeat_genome_sample_vs_oracle():
         got = c.run()
         assert c.summary()["n_edges"] == osum["n_edges"]
     assert got == want
+
+
+EO2EMP = {0: 0, -2: -3, -3: -4, -4: -5}  # oracle status -> C ABI status
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_fastq_parser_rules_and_error_order(seed):
+    """The device FASTQ parser against the oracle's reader: dropped reads (a byte in [64,127]
+    with bit 3 set), empty sequence lines, lowercase, blocks cut mid-record, and which error a
+    block reports when it holds a bad base and a length mismatch (the first in file order)."""
+    rng = random.Random(seed)
+    L = rng.choice([20, 33, 64])
+    seqs = []
+    for _ in range(rng.randrange(60, 300)):
+        u = rng.random()
+        if u < 0.08:
+            seqs.append("")
+        elif u < 0.25:
+            s = [rng.choice("ACGT") for _ in range(L)]
+            s[rng.randrange(L)] = rng.choice("NnKkMmHX")
+            seqs.append("".join(s))
+        else:
+            seqs.append("".join(rng.choice("ACGTacgt") for _ in range(L)))
+    if seed % 4 == 1:
+        seqs.insert(rng.randrange(len(seqs)), "ACGR" + "A" * (L - 4))   # bad base
+    if seed % 4 == 2:
+        seqs.insert(rng.randrange(len(seqs)), "A" * (L + 1))            # other length
+    if seed % 4 == 3:
+        k = rng.randrange(len(seqs) // 2)
+        seqs.insert(k, "ACGR" + "A" * (L - 4))
+        seqs.insert(k + rng.randrange(1, 5), "A" * (L - 1))
+    data = FQ([s.encode() for s in seqs])
+    for block in (1 << 20, 997, 4103):
+        rc_o, want, _, _ = ol.run_fastq(data, min_ovlp=L // 2, map_mem=16 * block)
+        try:
+            with native.Context(min_ovlp=L // 2, map_mem=16 * block) as c:
+                c.add_fastq(data, block=block)
+                got, rc_g = c.run(), 0
+        except native.EmpError as e:
+            got, rc_g = None, e.code
+        assert rc_g == EO2EMP[rc_o], (block, rc_o, rc_g)
+        if rc_g == 0:
+            assert got == want
