@@ -58,6 +58,17 @@ def cpu_baseline(workload, scale, min_ovlp, check_gpu):
     }, same
 
 
+def traffic_of(kernel, workload):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
+    1 GiB copy's counters) and recorded in profiles/roofline_traffic.json; None when not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "roofline_traffic.json")) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return t["traffic_bytes_per_launch"] if t and t.get("workload") == workload else None
+
+
 def batched_bench(n_sets):
     """The first n_sets sets of the batched workload: one batched engine vs the plain path per set."""
     from empanada_amd import gen, native
@@ -270,7 +281,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_of(dom, args.workload),
             "alg_bytes_per_launch": round(bytes_per_launch),
             "avg_launch_ms": round(avg_ms, 4),
         },

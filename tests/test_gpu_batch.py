@@ -81,9 +81,10 @@ def test_jobs_admitted_in_later_steps_and_budget_waves():
             assert b.output(j) == want
         starts = [b.info(j)["admit_ms"] for j in ids]
         assert starts == sorted(starts)
-    # a budget that holds about two jobs: admitted in waves, same outputs
+    # a budget that holds the largest job alone: admitted in waves, same outputs
     need = [native.job_bytes(r.shape[0], r.shape[1]) for r, _ in jobs]
-    budget = 2 * max(need)
+    budget = max(need)
+    assert sum(need) > budget
     with native.Batch(hbm_budget=budget) as b:
         ids = [b.submit(r, o) for r, o in jobs]
         b.run()
