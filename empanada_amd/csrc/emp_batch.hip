@@ -67,28 +67,24 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart1(BPartArgs a) {
     __syncthreads();
     Entry e[PER];
     uint32_t r[PER], d[PER];
+    // every lane loads (lanes past the tile re-read its last entry): all loads in flight together
+    const uint32_t last = nt - 1;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) e[q] = load_stream(in + min(threadIdx.x + NT * q, last));
     if (REGEN) {
         const RoundArgs rg = J.rg;
         uint32_t word[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t i = threadIdx.x + NT * q;
-            word[q] = 0xFFFFFFFFu;
-            if (i < nt) {
-                e[q] = load_stream(in + i);
-                uint32_t g;
-                if (regen_needs_gather(rg, e[q].rid, side == 0, g)) word[q] = bgroup_codes(a.cols, J, g, e[q].rid >> 1);
-            }
+            uint32_t g;
+            const bool need = regen_needs_gather(rg, e[q].rid, side == 0, g);
+            // entries without a gather this round read the job's first column word (cached)
+            const uint32_t w = bgroup_codes(a.cols, J, need ? g : 0u, need ? e[q].rid >> 1 : 0u);
+            word[q] = need ? w : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q)
             if (threadIdx.x + NT * q < nt) regen_carry(a.cols, rg, e[q], side == 0, word[q]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const uint32_t i = threadIdx.x + NT * q;
-            if (i < nt) e[q] = load_stream(in + i);
-        }
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -138,10 +134,10 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
     Entry e[PER];
     uint32_t r[PER], d[PER];
 #pragma unroll
+    for (int q = 0; q < PER; ++q) e[q] = in[min(threadIdx.x + NT * q, nt - 1)];
+#pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t i = threadIdx.x + NT * q;
-        if (i < nt) {
-            e[q] = in[i];
+        if (threadIdx.x + NT * q < nt) {
             d[q] = range_of(e[q].h1, nb) - f0;
             r[q] = atomicAdd(&hist[d[q]], 1u);
         }
@@ -204,9 +200,15 @@ __global__ void __launch_bounds__(NT) k_bdedup(BJoinArgs a) {
     const Entry *in = a.s_in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) head[i] = EMPTY;
     if (threadIdx.x == 0) kcnt = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-        const Entry e = in[i];
-        h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
+    {
+        Entry e[PER];  // all loads in flight together (lanes past n re-read the last entry)
+#pragma unroll
+        for (int q = 0; q < PER; ++q) e[q] = in[min(threadIdx.x + NT * q, n - 1)];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t i = threadIdx.x + NT * q;
+            if (i < n) { h1s[i] = e[q].h1; h2s[i] = e[q].h2; rids[i] = e[q].rid; }
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
@@ -259,15 +261,16 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
     }
     const uint32_t j = job_of(a.fpre, a.njobs, b);
     const Entry *sin = a.s_in + (size_t)b * CAP, *pin = a.p_in + (size_t)b * CAP;
-    Entry se[PER];
+    Entry se[PER], pe[PER];  // all loads in flight together (empty sides read slot 0, unused)
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        se[q] = sin[ns ? min(threadIdx.x + NT * q, ns - 1) : 0u];
+        pe[q] = pin[np ? min(threadIdx.x + NT * q, np - 1) : 0u];
+    }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
-        if (i < ns) se[q] = sin[i];
-        if (i < np) {
-            const Entry p = pin[i];
-            S.pk1[i] = p.h1; S.pk2[i] = p.h2; S.prid[i] = p.rid; S.used[i] = 0;
-        }
+        if (i < np) { S.pk1[i] = pe[q].h1; S.pk2[i] = pe[q].h2; S.prid[i] = pe[q].rid; S.used[i] = 0; }
     }
     if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; S.rcnt[0] = 0; S.rcnt[1] = 0; }
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;

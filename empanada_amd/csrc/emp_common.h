@@ -160,6 +160,37 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) {
 // LDS of the join of one bucket: the prefix entries (12 B each), a used flag per prefix, and one
 // region that holds the hash table (chain heads + links) while probing and the matched records
 // afterwards; survivor counts and slots per run.
+// Target buckets one work-group writes in the next round's suffix list (NextSfxOut): an
+// open-addressed LDS table of bucket -> (entries, run base).
+constexpr int NXT_SLOTS = 32;
+struct NxtTable {
+    uint32_t key[NXT_SLOTS], cnt[NXT_SLOTS], base[NXT_SLOTS];
+    uint32_t total;
+};
+
+// Threads below NXT_SLOTS clear the table; a barrier must follow before it is used.
+__device__ __forceinline__ void nxt_init(NxtTable &T) {
+    if (threadIdx.x < NXT_SLOTS) {
+        T.key[threadIdx.x] = 0xFFFFFFFFu;
+        T.cnt[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) T.total = 0;
+}
+
+// Slot of target bucket nb (NXT_SLOTS when the table is full) and the entry's rank in it.
+__device__ __forceinline__ uint32_t nxt_slot(NxtTable &T, uint32_t nb, uint32_t &rank) {
+    uint32_t h = (nb * 0x9E3779B1u) >> 27;
+    for (int k = 0; k < NXT_SLOTS; ++k) {
+        const uint32_t prev = atomicCAS(&T.key[h], 0xFFFFFFFFu, nb);
+        if (prev == 0xFFFFFFFFu || prev == nb) {
+            rank = atomicAdd(&T.cnt[h], 1u);
+            return h;
+        }
+        h = (h + 1) & (NXT_SLOTS - 1);
+    }
+    return NXT_SLOTS;
+}
+
 template <int CAP, int MCAP, int NT>
 struct JoinShared {
     uint32_t pk1[CAP], pk2[CAP], prid[CAP];
@@ -174,6 +205,7 @@ struct JoinShared {
             uint32_t msr[MCAP], mpr[MCAP], msc[MCAP];
         } m;
     } u;
+    NxtTable nt;
     uint32_t sh[NT / 64];
     uint32_t ms, mp, ovf, etot;
     uint32_t rcnt[2 * NRANGE], rbase[2 * NRANGE];  // survivors per (side, read range), their slots

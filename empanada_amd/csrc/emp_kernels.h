@@ -55,6 +55,24 @@ struct RoundArgs {
     uint32_t G;        // column groups of the column store, ceil(L / 4)
 };
 
+// Suffix entries regenerated for the next round and written straight into that round's suffix
+// buckets. NextSfx shifts h1 by e * R1^(l-1) (mod P1), e in 1..4, so the entries of one bucket
+// land in at most two buckets per base of a round with as many or fewer buckets: a work-group
+// collects its entries per target bucket in LDS and reserves one run per target.
+struct NextSfxOut {
+    int32_t active;          // 0: no next round
+    int32_t shift;           // target bucket = (h1 & KEY_MASK) >> shift
+    RoundArgs rg;            // the next round
+    Entry *fine;             // its suffix buckets, BUCKET_CAP slots each
+    uint32_t *cnt;           // their counts
+    SpillRec *spill;         // entries past a bucket's capacity
+    uint32_t *spill_n;
+    uint32_t spill_cap;
+    uint32_t *total;         // suffix entries of the next round
+    uint32_t *ovf;           // set when the spill area overflows
+    const uint32_t *cols;
+};
+
 struct MapArgs {
     const uint8_t *reads;    // n x L row-major, 4-byte aligned
     uint32_t n;              // reads in this batch
@@ -98,7 +116,8 @@ struct PartArgs {
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
-    Entry *lin;              // kept entries (full-length keys), SUBS sub-ranges of lin_cap entries
+    NextSfxOut nx;           // round 0's suffix list (the kept entries, regenerated)
+    Entry *lin;              // kept entries (full-length keys): round 0's prefix list, SUBS sub-ranges
     uint32_t *lin_n;         // [SUBS] counts of the sub-ranges
     uint32_t lin_cap;        // entries per sub-range
     uint32_t *big, *n_big;
@@ -112,8 +131,9 @@ struct JoinArgs {
     const uint32_t *s_cnt, *p_cnt;
     uint32_t nb;             // buckets
     int32_t keep;            // write the survivors (a next round exists)
-    Entry *s_lin, *p_lin;    // survivors with this round's keys, SUBS sub-ranges of lin_cap entries
-    uint32_t *s_lin_n, *p_lin_n;
+    NextSfxOut nx;           // suffix survivors -> the next round's suffix buckets
+    Entry *p_lin;            // prefix survivors with this round's keys, SUBS sub-ranges of lin_cap
+    uint32_t *p_lin_n;
     uint32_t lin_cap;
     uint32_t *pool_from, *pool_to;   // edge pool: SUBS sub-ranges of pool_cap edges, then the
     unsigned long long *pool_cursor; // overflow region [SUBS * pool_cap, + ovf_cap) with its
@@ -141,12 +161,13 @@ struct BigArgs {
     const uint32_t *s_cnt, *p_cnt;
     const SpillRec *s_spill, *p_spill;
     const uint32_t *s_spill_n, *p_spill_n;
-    uint32_t spill_cap;
+    uint32_t spill_cap, p_spill_cap;  // capacities of s_spill / p_spill
     uint32_t cap;
     int32_t keep;                  // write the survivors
     RoundArgs rg;                  // the round (join): survivors get their carried codes back
     const uint32_t *cols;
-    Entry *s_lin, *p_lin;
+    NextSfxOut nx;                 // suffix survivors (join) / kept entries (dedup) for the next round
+    Entry *s_lin, *p_lin;          // dedup: kept entries (s_lin); join: prefix survivors (p_lin)
     uint32_t *s_lin_n, *p_lin_n;
     uint32_t lin_cap;
     uint32_t *pool_from, *pool_to;

@@ -46,6 +46,96 @@ __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ col
 }
 
 // ---------------------------------------------------------------------------------------------
+// Next round's suffix list written by the kernel that finds its entries (NextSfxOut).
+
+// Work-group form, in three steps so that a kernel can overlap the reservation with other work:
+//   next_targets  every thread: gathers, regenerates its entries e[q] (bit q of `mask`) for nx.rg
+//                 and enters each target bucket in the LDS table (tgt[q] = slot << 16 | rank);
+//   (barrier)     then threads below NXT_SLOTS reserve one run per target (next_reserve) and one
+//                 thread adds the entries to nx.total;
+//   (barrier)     next_write: every thread rebuilds its entries from the gathered words and writes.
+template <int PER>
+__device__ __forceinline__ void next_targets(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask,
+                                             uint32_t (&word)[PER], uint32_t (&tgt)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {  // all gathers in flight together; others read one cached word
+        uint32_t g = 0;
+        const bool need = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, true, g);
+        const uint32_t w = group_codes(nx.cols, nx.rg.G, need ? g : 0u, need ? e[q].rid >> 1 : 0u);
+        word[q] = need ? w : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        tgt[q] = 0;
+        if ((mask >> q) & 1u) {
+            Entry ne = e[q];
+            regen_carry(nx.cols, nx.rg, ne, true, word[q]);
+            uint32_t rk = 0;
+            const uint32_t sl = nxt_slot(T, (ne.h1 & KEY_MASK) >> nx.shift, rk);
+            tgt[q] = (sl << 16) | rk;
+        }
+    }
+    if (mask) atomicAdd(&T.total, (uint32_t)__popc(mask));
+}
+
+// Run base of table slot k (thread k < NXT_SLOTS, after a barrier that follows next_targets).
+__device__ __forceinline__ uint32_t next_reserve(const NextSfxOut &nx, const NxtTable &T, uint32_t k) {
+    const uint32_t key = T.key[k], c = T.cnt[k];
+    return (key != 0xFFFFFFFFu && c) ? atomicAdd(nx.cnt + key, c) : 0u;
+}
+
+template <int PER>
+__device__ __forceinline__ void next_write(const NextSfxOut &nx, const NxtTable &T, const Entry (&e)[PER], uint32_t mask,
+                                           const uint32_t (&word)[PER], const uint32_t (&tgt)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if ((mask >> q) & 1u) {
+            Entry ne = e[q];
+            regen_carry(nx.cols, nx.rg, ne, true, word[q]);
+            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = tgt[q] >> 16;
+            const uint32_t slot = sl < NXT_SLOTS ? T.base[sl] + (tgt[q] & 0xFFFFu) : atomicAdd(nx.cnt + nb, 1u);
+            if (slot < (uint32_t)BUCKET_CAP) {
+                nx.fine[(size_t)nb * BUCKET_CAP + slot] = ne;
+            } else {
+                const uint32_t sp = atomicAdd(nx.spill_n, 1u);
+                if (sp < nx.spill_cap) nx.spill[sp] = SpillRec{ne, nb};
+                else atomicOr(nx.ovf, 1u);
+            }
+        }
+    }
+}
+
+// The three steps back to back (every thread of the work-group calls it).
+template <int PER>
+__device__ __forceinline__ void scatter_next(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask) {
+    uint32_t word[PER], tgt[PER];
+    next_targets<PER>(nx, T, e, mask, word, tgt);
+    __syncthreads();
+    if (threadIdx.x < NXT_SLOTS) T.base[threadIdx.x] = next_reserve(nx, T, threadIdx.x);
+    else if (threadIdx.x == NXT_SLOTS && T.total) atomicAdd(nx.total, T.total);
+    __syncthreads();
+    next_write<PER>(nx, T, e, mask, word, tgt);
+}
+
+// Single-entry form (large-bucket kernels): e is a suffix entry of this round with its carried
+// codes in place.
+__device__ __forceinline__ void put_next(const NextSfxOut &nx, Entry e) {
+    uint32_t g, word = 0xFFFFFFFFu;
+    if (regen_needs_gather(nx.rg, e.rid, true, g)) word = group_codes(nx.cols, nx.rg.G, g, e.rid >> 1);
+    regen_carry(nx.cols, nx.rg, e, true, word);
+    const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift;
+    const uint32_t slot = atomicAdd(nx.cnt + nb, 1u);
+    atomicAdd(nx.total, 1u);
+    if (slot < (uint32_t)BUCKET_CAP) {
+        nx.fine[(size_t)nb * BUCKET_CAP + slot] = e;
+    } else {
+        const uint32_t sp = atomicAdd(nx.spill_n, 1u);
+        if (sp < nx.spill_cap) nx.spill[sp] = SpillRec{e, nb};
+        else atomicOr(nx.ovf, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Map: one work-group of 256 threads per 256 reads.
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     extern __shared__ __align__(16) uint8_t tile[];
@@ -163,9 +253,14 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
         if (threadIdx.x < NB) hist[threadIdx.x] = 0;
         __syncthreads();
+        // Without regeneration every lane loads (lanes past the tile re-read its last entry), so the
+        // loads of all PER entries are in flight together.
         Entry e[PER];
         uint32_t r[PER], d[PER];
+        const uint32_t last = nt - 1;
         if (REGEN) {
+            // entry and gather per slot: within the 80 registers of 6 waves per SIMD, loading all
+            // 8 entries first spills
             uint32_t word[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
@@ -182,10 +277,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
                 if (threadIdx.x + NT * q < nt) regen_carry(a.cols, a.rg, e[q], a.sfx, word[q]);
         } else {
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                uint32_t i = threadIdx.x + NT * q;
-                if (i < nt) e[q] = in[t0 + i];
-            }
+            for (int q = 0; q < PER; ++q) e[q] = in[t0 + min(threadIdx.x + NT * q, last)];
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -240,6 +332,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     __shared__ uint32_t head[CAP];
     __shared__ uint16_t nxt[CAP];
     __shared__ uint32_t rcnt[NRANGE], rbase[NRANGE];
+    __shared__ NxtTable T;
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.cnt[b];
@@ -254,9 +347,16 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     const Entry *in = a.in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) head[i] = EMPTY;
     if (threadIdx.x < NRANGE) rcnt[threadIdx.x] = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-        Entry e = in[i];
-        h1s[i] = e.h1; h2s[i] = e.h2; rids[i] = e.rid;
+    nxt_init(T);
+    {
+        Entry e[PER];  // all loads in flight together (lanes past n re-read the last entry)
+#pragma unroll
+        for (int q = 0; q < PER; ++q) e[q] = in[min(threadIdx.x + NT * q, n - 1)];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t i = threadIdx.x + NT * q;
+            if (i < n) { h1s[i] = e[q].h1; h2s[i] = e[q].h2; rids[i] = e[q].rid; }
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
@@ -285,13 +385,17 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         rbase[r] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
     }
     __syncthreads();
+    Entry kept[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         if (keepmask & (1u << q)) {
             const uint32_t i = threadIdx.x + NT * q, base = rbase[read_range(rids[i])];
-            if (base != 0xFFFFFFFFu) a.lin[base + slot[q]] = Entry{h1s[i], h2s[i], rids[i]};
+            kept[q] = Entry{h1s[i], h2s[i], rids[i]};
+            if (base != 0xFFFFFFFFu) a.lin[base + slot[q]] = kept[q];
         }
     }
+    // the kept entries are also round 0's suffix list
+    if (a.nx.active) scatter_next<PER>(a.nx, T, kept, keepmask);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -334,6 +438,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     }
     if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; }
     if (threadIdx.x < 2 * NRANGE) S.rcnt[threadIdx.x] = 0;
+    nxt_init(S.nt);
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
     uint32_t sc[PER];
 #pragma unroll
@@ -386,8 +491,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                     S.ovf = 1;
                 }
             } else {
-                smask |= 1u << q;
-                slot[q] = atomicAdd(&S.rcnt[read_range(se[q].rid)], 1u);
+                smask |= 1u << q;  // a survivor: into the next round's buckets below
             }
         }
         if (i < np) {
@@ -406,6 +510,10 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
+    // Suffix survivors: their next-round targets (the entries are rebuilt when written, below).
+    uint32_t word[PER], tgt[PER];
+    if (a.nx.active) next_targets<PER>(a.nx, S.nt, se, smask, word, tgt);
+    else if (smask) atomicAdd(&S.nt.total, (uint32_t)__popc(smask));  // last round: only counted
     if (ecount) atomicAdd(&S.etot, ecount);
     __syncthreads();
     if (S.ovf) {
@@ -419,13 +527,20 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     const uint32_t nms = S.ms, nmp = S.mp, tot = S.etot;
 
     // Reservations, issued now and read back after the sort so that their latency overlaps it:
-    // thread k < 16 reserves the survivor run (side k / 8, read range k % 8), thread 16 the edges.
+    // thread k in [8, 16) reserves the prefix survivor run of read range k % 8, thread 16 the edges,
+    // threads 64 + k the next-round suffix run of table slot k.
+    constexpr uint32_t NXT0 = 64;
+    static_assert(NT >= NXT0 + NXT_SLOTS + 1, "reservation threads");
     unsigned long long resv = 0;
-    if (threadIdx.x < 2 * NRANGE) {
+    if (threadIdx.x >= NRANGE && threadIdx.x < 2 * NRANGE) {
         const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
-        resv = cnt ? atomicAdd((k < NRANGE ? a.s_lin_n : a.p_lin_n) + sub, cnt) : 0u;
+        resv = cnt ? atomicAdd(a.p_lin_n + sub, cnt) : 0u;
     } else if (threadIdx.x == 2 * NRANGE && tot) {
         resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)), (unsigned long long)tot);
+    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NXT_SLOTS && a.nx.active) {
+        resv = next_reserve(a.nx, S.nt, threadIdx.x - NXT0);
+    } else if (threadIdx.x == NXT0 + NXT_SLOTS && S.nt.total) {
+        atomicAdd(a.nx.total, S.nt.total);
     }
 
     // Edges of the bucket: order matched suffixes and prefixes by (key, rank), then emit the
@@ -448,7 +563,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
         if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
     }
-    if (threadIdx.x < 2 * NRANGE) {
+    if (threadIdx.x >= NRANGE && threadIdx.x < 2 * NRANGE) {
         const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
         uint32_t c = (uint32_t)resv;
         if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
@@ -460,6 +575,8 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         S.base = base;
         a.edge_off[b] = base;
         a.edge_cnt[b] = tot;
+    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NXT_SLOTS) {
+        S.nt.base[threadIdx.x - NXT0] = (uint32_t)resv;
     }
     __syncthreads();
     if (nms) {
@@ -494,14 +611,11 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
     }
 
-    // Survivors into their runs; the next round's first partition pass regenerates them.
+    // Survivors: prefixes into their runs (the next round's first partition pass regenerates
+    // them), suffixes regenerated here into the next round's suffix buckets.
     if (a.keep) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            if (smask & (1u << q)) {
-                const uint32_t base = S.rbase[read_range(se[q].rid)];
-                if (base != 0xFFFFFFFFu) a.s_lin[base + (slot[q] & 0xFFFFu)] = se[q];
-            }
             if (pmask & (1u << q)) {
                 const uint32_t i = threadIdx.x + NT * q, rid = S.prid[i];
                 const uint32_t base = S.rbase[NRANGE + read_range(rid)];
@@ -509,6 +623,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
+    if (a.nx.active) next_write<PER>(a.nx, S.nt, se, smask, word, tgt);
 }
 
 // Prefix entries of a bucket (registers) into LDS, used flags cleared.
@@ -583,7 +698,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
     uint32_t *SR = a.srank + a.soff[blockIdx.x], *PR = a.prank + a.poff[blockIdx.x];
     uint32_t *SA = a.saux + a.soff[blockIdx.x], *PA = a.paux + a.poff[blockIdx.x];
     gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos);
-    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.spill_cap, b, PK, PR, np2, true, &s_pos);
+    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, PK, PR, np2, true, &s_pos);
     bitonic(SK, SR, nullptr, ns2);
     bitonic(PK, PR, nullptr, np2);
     // valid partner counts: the key group on the other side minus the same-read members
@@ -643,14 +758,13 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
                 }
             }
         }
-        if (e == o) {
-            Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
-            const uint32_t rs = range_sub(read_range(srid), b);
-            uint32_t pos = atomicAdd(a.s_lin_n + rs, 1u);
-            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else if (a.keep) {
+        if (e == o) {  // a survivor
+            if (a.nx.active) {
+                Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
                 set_carry(en, carry_after(a.cols, a.rg, srid, true));
-                a.s_lin[rs * a.lin_cap + pos] = en;
+                put_next(a.nx, en);
+            } else {
+                atomicAdd(a.nx.total, 1u);
             }
         }
     }
@@ -686,6 +800,7 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
             else {
                 a.s_lin[sub * a.lin_cap + pos] = e;
             }
+            if (a.nx.active) put_next(a.nx, e);
         }
     }
 }

@@ -35,6 +35,9 @@ struct DevCounters {
     uint32_t n_big, abort, lin_ovf, pad;
     uint32_t spill[2];              // records spilled by the final partition pass
     uint32_t in[2];                 // partition input sizes of linear lists, set by the host
+    uint32_t sspill[2];             // records spilled into suffix bucket set 0 / 1
+    uint32_t stotal[2];             // entries written into suffix bucket set 0 / 1
+    uint32_t nx_ovf, pad2;          // a suffix spill area overflowed
 };
 
 double now_ms() {
@@ -129,6 +132,11 @@ struct emp_ctx {
     uint32_t nb_alloc = 0;
     SpillRec *d_spill[2] = {nullptr, nullptr};
     uint32_t spill_cap = 0;
+    // suffix buckets of rounds t (set t % 2), written by the dedup (round 0) or round t-1's join
+    Entry *d_sfine[2] = {nullptr, nullptr};
+    uint32_t *d_sfine_cnt[2] = {nullptr, nullptr};
+    SpillRec *d_sspill[2] = {nullptr, nullptr};
+    uint32_t sspill_cap[2] = {0, 0}, nb_s_alloc = 0;
     double slack = 1.125;
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
@@ -625,6 +633,20 @@ int ensure_lin_set(emp_ctx *c, int set, uint32_t cap) {
     return EMP_OK;
 }
 
+// Suffix spill area of set `set` with `sp` records. Its contents are dropped: call it only when
+// the set holds no live round (before the kernel that fills it runs, or reruns).
+int grow_sspill(emp_ctx *c, int set, uint32_t sp) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->d_sspill[set]);
+    c->d_sspill[set] = nullptr;
+    void *p = nullptr;
+    int rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec));
+    if (rc) return rc;
+    c->d_sspill[set] = (SpillRec *)p;
+    c->sspill_cap[set] = sp;
+    return EMP_OK;
+}
+
 // Partition and join buffers for lists of up to x entries in 2^bb buckets.
 int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
     int rc;
@@ -639,6 +661,16 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
             c->d_fine_cnt[s] = (uint32_t *)p;
         }
         c->nb_alloc = nb;
+    }
+    if (nb > c->nb_s_alloc) {
+        for (int s = 0; s < 2; ++s) {
+            if ((rc = realloc_entries(c, &c->d_sfine[s], (uint64_t)nb * BUCKET_CAP))) return rc;
+            dfree(c, c->d_sfine_cnt[s]);
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
+            c->d_sfine_cnt[s] = (uint32_t *)p;
+        }
+        c->nb_s_alloc = nb;
     }
     const uint64_t need = inter_need(x, bb, c->slack);
     if (need > c->inter_cap) {
@@ -774,6 +806,14 @@ void zero_join(emp_ctx *c, ZeroList &z, int set) {
     z.add(&c->d_ctr->lin[set][0][0], 2 * SUBS);
     z.add(&c->d_ctr->n_big, 1);
     z.add(&c->d_ctr->lin_ovf, 1);
+}
+
+// What a launch writing suffix bucket set `set` (2^bb buckets) accumulates.
+void zero_next(emp_ctx *c, ZeroList &z, int set, int bb) {
+    z.add(c->d_sfine_cnt[set], 1ull << bb);
+    z.add(&c->d_ctr->sspill[set], 1);
+    z.add(&c->d_ctr->stotal[set], 1);
+    z.add(&c->d_ctr->nx_ovf, 1);
 }
 
 int run_zero(emp_ctx *c, const ZeroList &z) {
@@ -930,15 +970,43 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
     return EMP_OK;
 }
 
+// Suffix-side input of a dedup or join launch: buckets, counts and spill area with its count.
+struct SfxIn {
+    const Entry *fine;
+    const uint32_t *cnt;
+    const SpillRec *spill;
+    const uint32_t *spill_n;
+    uint32_t spill_cap;
+};
+
+// The next round's suffix buckets: set `set`, 2^bb buckets, for round rg (inactive: none).
+NextSfxOut next_out(emp_ctx *c, int set, int bb, const RoundArgs &rg) {
+    NextSfxOut o{};
+    o.active = rg.active;
+    o.shift = 29 - bb;
+    o.rg = rg;
+    o.fine = c->d_sfine[set];
+    o.cnt = c->d_sfine_cnt[set];
+    o.spill = c->d_sspill[set];
+    o.spill_n = &c->d_ctr->sspill[set];
+    o.spill_cap = c->sspill_cap[set];
+    o.total = &c->d_ctr->stotal[set];
+    o.ovf = &c->d_ctr->nx_ovf;
+    o.cols = c->d_cols;
+    return o;
+}
+
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
-// global-memory sort + join (or dedup), one work-group per bucket. Survivors go to set `set`.
-int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg, uint64_t slot0) {
+// global-memory sort + join (or dedup), one work-group per bucket. Dedup: kept entries go to
+// survivor set 0 (side 0) and round 0's suffix buckets; join: prefix survivors to set `set`.
+int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg, uint64_t slot0,
+             const SfxIn &sin, const NextSfxOut &nx) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
     std::vector<uint32_t> ids(nbig), cs(nb), cp(nb);
     HIPCHK(c, hipMemcpy(ids.data(), c->d_big, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(cs.data(), c->d_fine_cnt[0], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cs.data(), sin.cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (!dedup) HIPCHK(c, hipMemcpy(cp.data(), c->d_fine_cnt[1], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
     std::vector<unsigned long long> so(nbig), po(nbig);
     uint64_t as = 0, ap = 0;
@@ -970,24 +1038,26 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
     a.prank = c->d_brank[1];
     a.saux = c->d_baux[0];
     a.paux = c->d_baux[1];
-    a.s_in = c->d_fine[0];
+    a.s_in = sin.fine;
     a.p_in = c->d_fine[1];
-    a.s_cnt = c->d_fine_cnt[0];
+    a.s_cnt = sin.cnt;
     a.p_cnt = c->d_fine_cnt[1];
-    a.s_spill = c->d_spill[0];
+    a.s_spill = sin.spill;
     a.p_spill = c->d_spill[1];
-    a.s_spill_n = &c->d_ctr->spill[0];
+    a.s_spill_n = sin.spill_n;
     a.p_spill_n = &c->d_ctr->spill[1];
-    a.spill_cap = c->spill_cap;
+    a.spill_cap = sin.spill_cap;
+    a.p_spill_cap = c->spill_cap;
     a.cap = BUCKET_CAP;
     a.keep = keep ? 1 : 0;
     a.rg = rg;
     a.cols = c->d_cols;
-    a.s_lin = c->d_lin[set][0];
+    a.nx = nx;
+    a.s_lin = c->d_lin[0][0];
+    a.s_lin_n = c->d_ctr->lin[0][0];
     a.p_lin = c->d_lin[set][1];
-    a.s_lin_n = c->d_ctr->lin[set][0];
     a.p_lin_n = c->d_ctr->lin[set][1];
-    a.lin_cap = c->lin_cap[set];
+    a.lin_cap = dedup ? c->lin_cap[0] : c->lin_cap[set];
     a.pool_from = c->d_pool_from;
     a.pool_to = c->d_pool_to;
     a.pool_cursor = c->d_ctr->pool;
@@ -1038,6 +1108,8 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
+    for (int s = 0; s < 2; ++s)  // suffix spill areas as large as the partition's
+        if (c->sspill_cap[s] < c->spill_cap && (rc = grow_sspill(c, s, c->spill_cap))) return rc;
     if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
         return rc;
     c->records.clear();
@@ -1045,10 +1117,14 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
     if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
 
-    // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key, write
-    // the kept entries (full-length keys) into survivor set 0, side 0. Round 0 partitions that
-    // list twice: as suffixes and as prefixes.
+    // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key. The kept
+    // entries (full-length keys) go to survivor set 0, side 0, which round 0 partitions as its
+    // prefix list, and, regenerated for round 0, into suffix bucket set 0: round 0's suffix list.
+    // Suffix bucket set t % 2 holds round t's suffixes; bbS[set] is the bucket bits it was
+    // written with (the round's prefix partition uses the same).
     const RoundArgs none{};
+    const RoundArgs rg0 = rounds > 0 ? round_args(c, 0, true) : none;
+    int bbS[2] = {bbD, bbD};
     bool need_part = true;
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
@@ -1059,6 +1135,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 zero_partition(c, z, 0, N0, bbD);
             }
             zero_join(c, z, 0);
+            zero_next(c, z, 0, bbD);
             if (attempt == 0) z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors, once per run
             if ((rc = run_zero(c, z))) return rc;
         }
@@ -1066,6 +1143,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         DedupArgs a{};
         a.in = c->d_fine[0];
         a.cnt = c->d_fine_cnt[0];
+        a.nx = next_out(c, 0, bbD, rg0);
         a.lin = c->d_lin[0][0];
         a.lin_n = c->d_ctr->lin[0][0];
         a.lin_cap = c->lin_cap[0];
@@ -1085,21 +1163,27 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             continue;
         }
         need_part = false;
-        if (!c->h_ctr->lin_ovf && c->h_ctr->n_big && (rc = big_path(c, bbD, true, true, 0, none, 0))) return rc;
+        if (!c->h_ctr->lin_ovf && !c->h_ctr->nx_ovf && c->h_ctr->n_big &&
+            (rc = big_path(c, bbD, true, true, 0, none, 0,
+                           SfxIn{c->d_fine[0], c->d_fine_cnt[0], c->d_spill[0], &c->d_ctr->spill[0], c->spill_cap},
+                           next_out(c, 0, bbD, rg0))))
+            return rc;
         if (c->h_ctr->lin_ovf) {
             if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
             continue;
         }
+        if (c->h_ctr->nx_ovf) {
+            if ((rc = grow_sspill(c, 0, c->sspill_cap[0] * 4))) return rc;
+            continue;
+        }
         break;
     }
-    // the prefix list starts as a copy of the suffix list's counts (same positions)
-    HIPCHK(c, hipMemcpyAsync(c->d_ctr->lin[0][1], c->d_ctr->lin[0][0], SUBS * sizeof(uint32_t),
-                             hipMemcpyDeviceToDevice, c->stream));
-    uint64_t St = sum_subs(c->h_ctr->lin[0][0]), Pt = St;
-    S.n_dedup = St;
+    const uint64_t D = sum_subs(c->h_ctr->lin[0][0]);
+    uint64_t St = rounds > 0 ? c->h_ctr->stotal[0] : D, Pt = D;
+    S.n_dedup = D;
 
     // Edge slots: round t owns slots [t * stride, t * stride + 2^bb); bucket counts only shrink.
-    const uint64_t stride = 1ull << bucket_bits(St);
+    const uint64_t stride = 1ull << bbD;
     if (St > 0 && rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)rounds * stride))) return rc;
     std::vector<uint64_t> round_edges;
 
@@ -1111,12 +1195,16 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                         "round %d: %llu suffixes / %llu prefixes reach M/2 = %llu: multi-record framing of "
                         "cqutils.cuh:107-137 is not reproduced; raise -r",
                         t, (unsigned long long)St, (unsigned long long)Pt, (unsigned long long)(M / 2));
-        const int bb = bucket_bits(std::max(St, Pt));
+        const int sset = t & 1, nset = sset ^ 1;
+        const int bb = bbS[sset];
         const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
+        // the next round's buckets, chosen now because this round's join writes its suffixes
+        const int bbn = more ? std::min(bb, bucket_bits(std::max(St, Pt))) : bb;
         const RoundArgs rg = round_args(c, t, true);
+        const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
         const int nxt = cur ^ 1;
-        if ((rc = ensure_lin_set(c, nxt, lin_cap_for(std::max(St, Pt), 1.25)))) return rc;
+        if ((rc = ensure_lin_set(c, nxt, lin_cap_for(Pt, 1.25)))) return rc;
         emp_round_stats rs{};
         rs.s = St;
         rs.p = Pt;
@@ -1125,6 +1213,8 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         need_part = true;
         uint64_t E = 0;
         const uint64_t slot0 = (uint64_t)t * stride;
+        const SfxIn sin{c->d_sfine[sset], c->d_sfine_cnt[sset], c->d_sspill[sset], &c->d_ctr->sspill[sset],
+                        c->sspill_cap[sset]};
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
             if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
@@ -1134,28 +1224,26 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 ZeroList z;
                 if (need_part) {
                     zero_part_flags(c, z, 0, 0);
-                    zero_partition(c, z, 0, St, bb);
                     zero_partition(c, z, 1, Pt, bb);
                 }
                 zero_join(c, z, nxt);
+                zero_next(c, z, nset, bbn);
                 if ((rc = run_zero(c, z))) return rc;
             }
             if (need_part) {
-                // round 0: both sides come from the dedup list
-                ListIn ls{c->d_lin[cur][0], c->d_ctr->lin[cur][0], c->lin_cap[cur]};
-                ListIn lp{c->d_lin[cur][t == 0 ? 0 : 1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
-                if ((rc = partition(c, 0, ls, St, bb, rg))) return rc;
+                // round 0: the prefixes are the dedup list
+                const ListIn lp = t == 0 ? ListIn{c->d_lin[0][0], c->d_ctr->lin[0][0], c->lin_cap[0]}
+                                         : ListIn{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
                 if ((rc = partition(c, 1, lp, Pt, bb, rg))) return rc;
             }
             JoinArgs a{};
-            a.s_in = c->d_fine[0];
+            a.s_in = sin.fine;
             a.p_in = c->d_fine[1];
-            a.s_cnt = c->d_fine_cnt[0];
+            a.s_cnt = sin.cnt;
             a.p_cnt = c->d_fine_cnt[1];
             a.keep = more ? 1 : 0;
-            a.s_lin = c->d_lin[nxt][0];
+            a.nx = next_out(c, nset, bbn, rgn);
             a.p_lin = c->d_lin[nxt][1];
-            a.s_lin_n = c->d_ctr->lin[nxt][0];
             a.p_lin_n = c->d_ctr->lin[nxt][1];
             a.lin_cap = c->lin_cap[nxt];
             a.pool_from = c->d_pool_from;
@@ -1183,9 +1271,15 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             }
             need_part = false;
             rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big && (rc = big_path(c, bb, more, false, nxt, rg, slot0))) return rc;
+            if (c->h_ctr->n_big &&
+                (rc = big_path(c, bb, more, false, nxt, rg, slot0, sin, next_out(c, nset, bbn, rgn))))
+                return rc;
             if (c->h_ctr->lin_ovf) {
                 if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
+                continue;
+            }
+            if (c->h_ctr->nx_ovf) {
+                if ((rc = grow_sspill(c, nset, c->sspill_cap[nset] * 4))) return rc;
                 continue;
             }
             const uint64_t ovf = c->h_ctr->pool_ovf;
@@ -1203,7 +1297,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             c->pool_prev[i] = v;
         }
         const DevCounters &k = *c->h_ctr;
-        const uint64_t Sn = sum_subs(k.lin[nxt][0]), Pn = sum_subs(k.lin[nxt][1]);
+        const uint64_t Sn = k.stotal[nset], Pn = sum_subs(k.lin[nxt][1]);
         rs.edges = E;
         rs.s_after = Sn;
         rs.p_after = Pn;
@@ -1220,6 +1314,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         St = Sn;
         Pt = Pn;
         cur = nxt;
+        bbS[nset] = bbn;
     }
     S.n_rounds = t;
 
