@@ -50,29 +50,32 @@ __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ col
 
 // Work-group form, in three steps so that a kernel can overlap the reservation with other work:
 //   next_targets  every thread: gathers, regenerates its entries e[q] (bit q of `mask`) for nx.rg
-//                 and enters each target bucket in the LDS table (tgt[q] = slot << 16 | rank);
+//                 and enters each target bucket in the LDS table;
 //   (barrier)     then threads below NXT_SLOTS reserve one run per target (next_reserve) and one
 //                 thread adds the entries to nx.total;
-//   (barrier)     next_write: every thread rebuilds its entries from the gathered words and writes.
+//   (barrier)     next_write: every thread rebuilds its entries and writes them.
+// Between the steps an entry is kept as one word: its gathered codes (9 bits, 0x100 = none), its
+// table slot (6 bits, NXT_SLOTS = table full) and its rank in the slot (12 bits).
+__device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
+
 template <int PER>
 __device__ __forceinline__ void next_targets(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask,
-                                             uint32_t (&word)[PER], uint32_t (&tgt)[PER]) {
+                                             uint32_t (&tgt)[PER]) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {  // all gathers in flight together; others read one cached word
         uint32_t g = 0;
         const bool need = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, true, g);
         const uint32_t w = group_codes(nx.cols, nx.rg.G, need ? g : 0u, need ? e[q].rid >> 1 : 0u);
-        word[q] = need ? w : 0xFFFFFFFFu;
+        tgt[q] = need ? w : 0x100u;
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        tgt[q] = 0;
         if ((mask >> q) & 1u) {
             Entry ne = e[q];
-            regen_carry(nx.cols, nx.rg, ne, true, word[q]);
+            regen_carry(nx.cols, nx.rg, ne, true, tgt_word(tgt[q]));
             uint32_t rk = 0;
             const uint32_t sl = nxt_slot(T, (ne.h1 & KEY_MASK) >> nx.shift, rk);
-            tgt[q] = (sl << 16) | rk;
+            tgt[q] |= (sl << 9) | (rk << 15);
         }
     }
     if (mask) atomicAdd(&T.total, (uint32_t)__popc(mask));
@@ -86,14 +89,14 @@ __device__ __forceinline__ uint32_t next_reserve(const NextSfxOut &nx, const Nxt
 
 template <int PER>
 __device__ __forceinline__ void next_write(const NextSfxOut &nx, const NxtTable &T, const Entry (&e)[PER], uint32_t mask,
-                                           const uint32_t (&word)[PER], const uint32_t (&tgt)[PER]) {
+                                           const uint32_t (&tgt)[PER]) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         if ((mask >> q) & 1u) {
             Entry ne = e[q];
-            regen_carry(nx.cols, nx.rg, ne, true, word[q]);
-            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = tgt[q] >> 16;
-            const uint32_t slot = sl < NXT_SLOTS ? T.base[sl] + (tgt[q] & 0xFFFFu) : atomicAdd(nx.cnt + nb, 1u);
+            regen_carry(nx.cols, nx.rg, ne, true, tgt_word(tgt[q]));
+            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 63u;
+            const uint32_t slot = sl < NXT_SLOTS ? T.base[sl] + (tgt[q] >> 15) : atomicAdd(nx.cnt + nb, 1u);
             if (slot < (uint32_t)BUCKET_CAP) {
                 nx.fine[(size_t)nb * BUCKET_CAP + slot] = ne;
             } else {
@@ -108,13 +111,13 @@ __device__ __forceinline__ void next_write(const NextSfxOut &nx, const NxtTable 
 // The three steps back to back (every thread of the work-group calls it).
 template <int PER>
 __device__ __forceinline__ void scatter_next(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask) {
-    uint32_t word[PER], tgt[PER];
-    next_targets<PER>(nx, T, e, mask, word, tgt);
+    uint32_t tgt[PER];
+    next_targets<PER>(nx, T, e, mask, tgt);
     __syncthreads();
     if (threadIdx.x < NXT_SLOTS) T.base[threadIdx.x] = next_reserve(nx, T, threadIdx.x);
     else if (threadIdx.x == NXT_SLOTS && T.total) atomicAdd(nx.total, T.total);
     __syncthreads();
-    next_write<PER>(nx, T, e, mask, word, tgt);
+    next_write<PER>(nx, T, e, mask, tgt);
 }
 
 // Single-entry form (large-bucket kernels): e is a suffix entry of this round with its carried
@@ -511,8 +514,8 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
     }
     // Suffix survivors: their next-round targets (the entries are rebuilt when written, below).
-    uint32_t word[PER], tgt[PER];
-    if (a.nx.active) next_targets<PER>(a.nx, S.nt, se, smask, word, tgt);
+    uint32_t tgt[PER];
+    if (a.nx.active) next_targets<PER>(a.nx, S.nt, se, smask, tgt);
     else if (smask) atomicAdd(&S.nt.total, (uint32_t)__popc(smask));  // last round: only counted
     if (ecount) atomicAdd(&S.etot, ecount);
     __syncthreads();
@@ -623,7 +626,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
-    if (a.nx.active) next_write<PER>(a.nx, S.nt, se, smask, word, tgt);
+    if (a.nx.active) next_write<PER>(a.nx, S.nt, se, smask, tgt);
 }
 
 // Prefix entries of a bucket (registers) into LDS, used flags cleared.
@@ -641,7 +644,7 @@ __device__ __forceinline__ void stage_prefixes(JoinShared<CAP, MCAP, NT> &S, uin
 // it joins bucket b, the entries of bucket b + G and the counts of bucket b + 2G are in flight.
 // Otherwise one work-group per bucket.
 template <int CAP, int MCAP, int NT, bool PERSIST>
-__global__ void __launch_bounds__(NT) k_join(JoinArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_join(JoinArgs a) {
     constexpr int PER = CAP / NT;
     static_assert(MCAP <= NT, "one matched record per thread in the rank sort");
     using SM = JoinShared<CAP, MCAP, NT>;
