@@ -10,19 +10,40 @@ namespace emp {
 
 // Bucket capacity (entries per bucket per list), matched-record capacity and work-group size of
 // the LDS join.
-constexpr int BUCKET_CAP = 2048;
-constexpr int MATCH_CAP = 384;
-constexpr int JOIN_THREADS = 512;
+// Build-time tuning knobs: EMP_BUCKET_CAP, EMP_MATCH_CAP, EMP_JOIN_THREADS.
+#ifndef EMP_BUCKET_CAP
+#define EMP_BUCKET_CAP 2048
+#endif
+#ifndef EMP_MATCH_CAP
+#define EMP_MATCH_CAP 384
+#endif
+#ifndef EMP_JOIN_THREADS
+#define EMP_JOIN_THREADS 512
+#endif
+constexpr int BUCKET_CAP = EMP_BUCKET_CAP;
+constexpr int MATCH_CAP = EMP_MATCH_CAP;
+constexpr int JOIN_THREADS = EMP_JOIN_THREADS;
 // Join work-groups loop over buckets with the next bucket's loads in flight (true), or take one
-// bucket each (false).
-constexpr bool JOIN_PERSIST = false;
+// bucket each (false). Build-time tuning knob (EMP_JOIN_PERSIST).
+#ifndef EMP_JOIN_PERSIST
+#define EMP_JOIN_PERSIST 0
+#endif
+constexpr bool JOIN_PERSIST = EMP_JOIN_PERSIST != 0;
 // Target mean bucket load: the partition fan-out is chosen so that lists average this many
 // entries per bucket, far enough below BUCKET_CAP that a hash-uniform key never overflows one.
 constexpr uint32_t BUCKET_TARGET = BUCKET_CAP / 2;
 // Entries per work-group tile of a partition pass, its work-group size, and the widest digit of
 // one pass. A 4096-entry tile gives runs of 16 entries (192 bytes) per bucket at 256 buckets.
+// Build-time tuning knobs: EMP_PART_THREADS (work-group size) and EMP_PART_WAVES (waves per SIMD
+// the kernel is compiled for, which caps its registers).
+#ifndef EMP_PART_THREADS
+#define EMP_PART_THREADS 512
+#endif
+#ifndef EMP_PART_WAVES
+#define EMP_PART_WAVES 6
+#endif
 constexpr int PART_TILE = 4096;
-constexpr int PART_THREADS = 512;
+constexpr int PART_THREADS = EMP_PART_THREADS;
 constexpr int PART_MAX_BITS = 8;
 // Linear outputs (survivor lists, the edge pool) are split into SUBS sub-ranges, each with its
 // own cursor: work-group b reserves in sub-range b % SUBS, so no single counter serialises a

@@ -61,12 +61,24 @@ __device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u)
 template <int PER>
 __device__ __forceinline__ void next_targets(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask,
                                              uint32_t (&tgt)[PER]) {
+    // An entry gathers when it enters a column group (one round in four): rounds where no lane
+    // of the wave needs a word skip the loads; otherwise all of the wave's gathers are in flight
+    // together (lanes without one read the store's first word, a cached line).
+    bool need[PER];
+    uint32_t grp[PER], any = 0;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {  // all gathers in flight together; others read one cached word
-        uint32_t g = 0;
-        const bool need = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, true, g);
-        const uint32_t w = group_codes(nx.cols, nx.rg.G, need ? g : 0u, need ? e[q].rid >> 1 : 0u);
-        tgt[q] = need ? w : 0x100u;
+    for (int q = 0; q < PER; ++q) {
+        grp[q] = 0;
+        need[q] = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, true, grp[q]);
+        any |= need[q];
+        tgt[q] = 0x100u;
+    }
+    if (__any(any)) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t w = group_codes(nx.cols, nx.rg.G, need[q] ? grp[q] : 0u, need[q] ? e[q].rid >> 1 : 0u);
+            if (need[q]) tgt[q] = w;
+        }
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -225,7 +237,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
 // round) the entries are streamed in with non-temporal loads, so that L2 keeps the one column
 // the launch gathers from, and regenerated before they are bucketed (REGEN = a.rg.active).
 template <bool REGEN>
-__global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_part(PartArgs a) {
+__global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_eu(EMP_PART_WAVES))) k_part(PartArgs a) {
     constexpr int NT = PART_THREADS, T = PART_TILE, PER = T / NT, NB = 1 << PART_MAX_BITS;
     static_assert(NT >= NB && T % NT == 0, "one bin per thread");
     __shared__ Entry buf[T];

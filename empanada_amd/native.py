@@ -7,7 +7,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libempanada_hip.so")
+# EMPANADA_LIB selects another build of the library (a tuning variant); the default is the in-tree one.
+LIB_PATH = os.environ.get("EMPANADA_LIB") or os.path.join(_HERE, "libempanada_hip.so")
 CLI_PATH = os.path.join(_HERE, "bin", "empanada")
 
 EMP_OK = 0
