@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart1(BPartArgs a) {
     const uint32_t nt = min((uint32_t)T, n - t0), nc = J.nc, ccap = J.ccap[side];
     const Entry *in = a.lists[J.src[side]] + J.seg + t0;
     if (threadIdx.x < NB) hist[threadIdx.x] = 0;
-    __syncthreads();
+    lds_barrier();
     Entry e[PER];
     uint32_t r[PER], d[PER];
     // every lane loads (lanes past the tile re-read its last entry): all loads in flight together
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart1(BPartArgs a) {
             r[q] = atomicAdd(&hist[d[q]], 1u);
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t h = threadIdx.x < NB ? hist[threadIdx.x] : 0u;
     uint32_t tot;
     const uint32_t ex = block_excl_scan<NT>(h, sh, tot);
@@ -101,11 +101,11 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart1(BPartArgs a) {
         loc[threadIdx.x] = ex;
         base[threadIdx.x] = h ? atomicAdd(&a.ccnt[J.cb + threadIdx.x], h) : 0u;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < PER; ++q)
         if (threadIdx.x + NT * q < nt) buf[loc[d[q]] + r[q]] = e[q];
-    __syncthreads();
+    lds_barrier();
     Entry *out = a.coarse + J.coff[side];
     for (uint32_t p = threadIdx.x; p < nt; p += NT) {
         const Entry v = buf[p];
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
     const uint32_t nt = min((uint32_t)T, n - t0), nb = J.nb, f0 = c * J.m;
     const Entry *in = a.coarse + J.coff[side] + (size_t)c * ccap + t0;
     if (threadIdx.x < NB) hist[threadIdx.x] = 0;
-    __syncthreads();
+    lds_barrier();
     Entry e[PER];
     uint32_t r[PER], d[PER];
 #pragma unroll
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
             r[q] = atomicAdd(&hist[d[q]], 1u);
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t h = threadIdx.x < NB ? hist[threadIdx.x] : 0u;
     uint32_t tot;
     const uint32_t ex = block_excl_scan<NT>(h, sh, tot);
@@ -151,11 +151,11 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
         loc[threadIdx.x] = ex;
         base[threadIdx.x] = h ? atomicAdd(&a.fcnt[fb + threadIdx.x], h) : 0u;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < PER; ++q)
         if (threadIdx.x + NT * q < nt) buf[loc[d[q]] + r[q]] = e[q];
-    __syncthreads();
+    lds_barrier();
     for (uint32_t p = threadIdx.x; p < nt; p += NT) {
         const Entry v = buf[p];
         const uint32_t dd = range_of(v.h1, nb) - f0;
@@ -210,9 +210,9 @@ __global__ void __launch_bounds__(NT) k_bdedup(BJoinArgs a) {
             if (i < n) { h1s[i] = e[q].h1; h2s[i] = e[q].h2; rids[i] = e[q].rid; }
         }
     }
-    __syncthreads();
+    lds_barrier();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
-    __syncthreads();
+    lds_barrier();
     uint32_t keepmask = 0, slot[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -229,9 +229,9 @@ __global__ void __launch_bounds__(NT) k_bdedup(BJoinArgs a) {
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) kbase = a.jobs[j].seg + atomicAdd(a.lcnt + j, kcnt);
-    __syncthreads();
+    lds_barrier();
     Entry *out = a.lists[0] + kbase;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -274,13 +274,13 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
     }
     if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; S.rcnt[0] = 0; S.rcnt[1] = 0; }
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
         if (i < np) S.u.h.nxt[i] = (uint16_t)atomicExch(S.u.h.head + (S.pk2[i] & (CAP - 1)), i);
     }
-    __syncthreads();
+    lds_barrier();
 
     // Probe: valid partners (equal key, different read) of every suffix; flag the prefixes used.
     uint32_t sc[PER];
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
             sc[q] = c;
         }
     }
-    __syncthreads();  // the hash table is dead; its space holds the matched records
+    lds_barrier();  // the hash table is dead; its space holds the matched records
 
     uint32_t smask = 0, pmask = 0, slot[PER], ecount = 0;
 #pragma unroll
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
         }
     }
     if (ecount) atomicAdd(&S.etot, ecount);
-    __syncthreads();
+    lds_barrier();
     if (S.ovf) {
         if (threadIdx.x == 0) to_big(a, b);
         return;
@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
         }
         if (!both && t < nmp) p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t, k2, r2);
         const bool hasp = both ? (t >= nms && t - nms < nmp) : t < nmp;
-        __syncthreads();
+        lds_barrier();
         if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
         if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
     }
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
         a.edge_cnt[b] = tot;
         if (tot) atomicAdd(a.ejob + j, tot);
     }
-    __syncthreads();
+    lds_barrier();
     if (nms) {
         const uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
         const uint32_t e0 = i0 < nms ? S.u.m.msc[i0] : 0, e1 = i1 < nms ? S.u.m.msc[i1] : 0;
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
         const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, stot);
         if (i0 < nms) S.u.m.msc[i0] = off;
         if (i1 < nms) S.u.m.msc[i1] = off + e0;
-        __syncthreads();
+        lds_barrier();
         const unsigned long long base = S.base;
         if (base != ~0ull) {
             for (uint32_t i = threadIdx.x; i < nms; i += NT) {

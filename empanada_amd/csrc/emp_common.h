@@ -9,6 +9,15 @@
 
 namespace emp {
 
+// Barrier for threads that share data only through LDS: each wave waits for its own LDS
+// operations, not for its global loads, stores or atomics in flight (a result is waited for where
+// it is used). Kernels whose threads exchange data through global memory use __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Regeneration of a live entry for round t (keys of length L - t -> L - t - 1). Suffix list:
 // even ids drop column t, odd ids the complement of column L-1-t; prefix list: the other way
 // round (overlap.cu:266-267, 684-690: rid/2 picks the read, odd ids are complemented). An entry
@@ -79,7 +88,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* 
         if (lane >= o) x += y;
     }
     if (lane == 63) sh[wv] = x;
-    __syncthreads();
+    lds_barrier();
     uint32_t base = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
@@ -88,7 +97,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh /* 
         tot += s;
     }
     total = tot;
-    __syncthreads();
+    lds_barrier();
     return base + x - v;
 }
 
@@ -104,7 +113,7 @@ __device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long lo
         if (lane >= o) x += y;
     }
     if (lane == 63) sh[wv] = x;
-    __syncthreads();
+    lds_barrier();
     unsigned long long base = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
@@ -113,7 +122,7 @@ __device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long lo
         tot += s;
     }
     total = tot;
-    __syncthreads();
+    lds_barrier();
     return base + x - v;
 }
 

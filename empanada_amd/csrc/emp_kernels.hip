@@ -125,10 +125,10 @@ template <int PER>
 __device__ __forceinline__ void scatter_next(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask) {
     uint32_t tgt[PER];
     next_targets<PER>(nx, T, e, mask, tgt);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < NXT_SLOTS) T.base[threadIdx.x] = next_reserve(nx, T, threadIdx.x);
     else if (threadIdx.x == NXT_SLOTS && T.total) atomicAdd(nx.total, T.total);
-    __syncthreads();
+    lds_barrier();
     next_write<PER>(nx, T, e, mask, tgt);
 }
 
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         tile[r * Lp + (g - r * (uint32_t)L)] = c;
     }
     if (bad) atomicOr(a.err, 1u);
-    __syncthreads();
+    lds_barrier();
 
     const uint32_t t = threadIdx.x;
     if (t < nt) {
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
     for (uint32_t tile = tile0; (size_t)tile * T < n_g; tile += a.tiles_x) {
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
         if (threadIdx.x < NB) hist[threadIdx.x] = 0;
-        __syncthreads();
+        lds_barrier();
         // Without regeneration every lane loads (lanes past the tile re-read its last entry), so the
         // loads of all PER entries are in flight together.
         Entry e[PER];
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
                 r[q] = atomicAdd(&hist[d[q]], 1u);
             }
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t h = threadIdx.x < NB ? hist[threadIdx.x] : 0u;
         uint32_t tot;
         const uint32_t ex = block_excl_scan<NT>(h, sh, tot);
@@ -310,13 +310,13 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
             loc[threadIdx.x] = ex;
             base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[gb + threadIdx.x], h) : 0u;
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             uint32_t i = threadIdx.x + NT * q;
             if (i < nt) buf[loc[d[q]] + r[q]] = e[q];
         }
-        __syncthreads();
+        lds_barrier();
         for (uint32_t p = threadIdx.x; p < nt; p += NT) {
             const Entry v = buf[p];
             const uint32_t dd = (v.h1 >> a.out_shift) & (F - 1);
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
                 atomicOr(a.abort, 1u);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -373,9 +373,9 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
             if (i < n) { h1s[i] = e[q].h1; h2s[i] = e[q].h2; rids[i] = e[q].rid; }
         }
     }
-    __syncthreads();
+    lds_barrier();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
-    __syncthreads();
+    lds_barrier();
     // kept entries: a slot in their read range's run of this work-group (one reservation per range)
     uint32_t keepmask = 0, slot[PER];
 #pragma unroll
@@ -392,14 +392,14 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < NRANGE) {
         const uint32_t r = threadIdx.x, cnt = rcnt[r], sub = range_sub(r, b);
         uint32_t c = cnt ? atomicAdd(a.lin_n + sub, cnt) : 0u;
         if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
         rbase[r] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
     }
-    __syncthreads();
+    lds_barrier();
     Entry kept[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -458,13 +458,13 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     uint32_t sc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) sc[q] = 0;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
         if (i < np) S.u.h.nxt[i] = (uint16_t)atomicExch(S.u.h.head + (S.pk2[i] & (CAP - 1)), i);
     }
-    __syncthreads();
+    lds_barrier();
 
     // Probe: count valid partners (different read) of every suffix; flag the prefixes used.
     if (np) {
@@ -484,7 +484,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
-    __syncthreads();  // the hash table is dead from here on; its space holds the matched records
+    lds_barrier();  // the hash table is dead from here on; its space holds the matched records
 
     // Matched records (key, rank) of both sides, the bucket's edge total, and a slot for every
     // survivor (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710) in its
@@ -530,7 +530,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     if (a.nx.active) next_targets<PER>(a.nx, S.nt, se, smask, tgt);
     else if (smask) atomicAdd(&S.nt.total, (uint32_t)__popc(smask));  // last round: only counted
     if (ecount) atomicAdd(&S.etot, ecount);
-    __syncthreads();
+    lds_barrier();
     if (S.ovf) {
         if (threadIdx.x == 0) {
             uint32_t i = atomicAdd(a.n_big, 1u);
@@ -574,7 +574,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
         if (!both && t < nmp) p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t, k2, r2);
         const bool hasp = both ? (t >= nms && t - nms < nmp) : t < nmp;
-        __syncthreads();
+        lds_barrier();
         if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
         if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
     }
@@ -593,7 +593,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NXT_SLOTS) {
         S.nt.base[threadIdx.x - NXT0] = (uint32_t)resv;
     }
-    __syncthreads();
+    lds_barrier();
     if (nms) {
         const uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
         const uint32_t e0 = i0 < nms ? S.u.m.msc[i0] : 0, e1 = i1 < nms ? S.u.m.msc[i1] : 0;
@@ -601,7 +601,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, stot);
         if (i0 < nms) S.u.m.msc[i0] = off;
         if (i1 < nms) S.u.m.msc[i1] = off + e0;
-        __syncthreads();
+        lds_barrier();
         const unsigned long long base = S.base;
         if (base != ~0ull) {
             for (uint32_t i = threadIdx.x; i < nms; i += NT) {
@@ -691,7 +691,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
         }
         join_bucket<CAP, MCAP, NT>(a, S, b, ns, np, se);
         if (b1 >= nb) break;
-        __syncthreads();  // LDS of bucket b is dead
+        lds_barrier();  // LDS of bucket b is dead
         stage_prefixes<CAP, MCAP, NT>(S, np1, pn);
         b = b1; ns = ns1; np = np1;
         b1 = b2; ns1 = ns2; np1 = np2;

@@ -122,8 +122,12 @@ struct emp_ctx {
     uint64_t acc_bytes = 0, acc_n = 0;
 
     // overlap phase, per list side (0 = suffixes, 1 = prefixes)
-    Entry *d_lin[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // survivor sets [set][side]
+    // prefix survivor sets [set][1] (side 0 unused), and the dedup's kept entries (round 0's
+    // prefix list; counts in DevCounters::lin[0][0])
+    Entry *d_lin[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t lin_cap[2] = {0, 0};   // entries per sub-range of each set
+    Entry *d_dedup = nullptr;
+    uint32_t dedup_cap = 0;
     Entry *d_inter[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t *d_inter_cnt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint64_t inter_cap = 0;  // entries per intermediate array
@@ -621,15 +625,23 @@ int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
     return EMP_OK;
 }
 
-// Survivor set `set` with at least `cap` entries per sub-range (both sides).
+// Prefix survivor set `set` with at least `cap` entries per sub-range.
 int ensure_lin_set(emp_ctx *c, int set, uint32_t cap) {
     if (cap <= c->lin_cap[set]) return EMP_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int side = 0; side < 2; ++side) {
-        int rc = realloc_entries(c, &c->d_lin[set][side], (uint64_t)SUBS * cap);
-        if (rc) return rc;
-    }
+    int rc = realloc_entries(c, &c->d_lin[set][1], (uint64_t)SUBS * cap);
+    if (rc) return rc;
     c->lin_cap[set] = cap;
+    return EMP_OK;
+}
+
+// The dedup's kept-entry list with at least `cap` entries per sub-range.
+int ensure_dedup_list(emp_ctx *c, uint32_t cap) {
+    if (cap <= c->dedup_cap) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = realloc_entries(c, &c->d_dedup, (uint64_t)SUBS * cap);
+    if (rc) return rc;
+    c->dedup_cap = cap;
     return EMP_OK;
 }
 
@@ -652,9 +664,11 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
     int rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const uint32_t nb = 1u << bb;
+    // Slot arrays: the prefix side's (d_fine[1]) and the two suffix sets. The dedup partition's
+    // output (d_fine[0]) lives in suffix set 1, which round 0's join fills only after the dedup.
     if (nb > c->nb_alloc) {
         for (int s = 0; s < 2; ++s) {
-            if ((rc = realloc_entries(c, &c->d_fine[s], (uint64_t)nb * BUCKET_CAP))) return rc;
+            if (s == 1 && (rc = realloc_entries(c, &c->d_fine[1], (uint64_t)nb * BUCKET_CAP))) return rc;
             dfree(c, c->d_fine_cnt[s]);
             void *p = nullptr;
             if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
@@ -672,11 +686,14 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
         }
         c->nb_s_alloc = nb;
     }
+    c->d_fine[0] = c->d_sfine[1];
+    // Intermediate partition buckets: the dedup (side 0) and the rounds (side 1) never overlap.
     const uint64_t need = inter_need(x, bb, c->slack);
     if (need > c->inter_cap) {
-        for (int s = 0; s < 2; ++s)
-            for (int k = 0; k < 2; ++k)
-                if ((rc = realloc_entries(c, &c->d_inter[s][k], need))) return rc;
+        for (int k = 0; k < 2; ++k) {
+            if ((rc = realloc_entries(c, &c->d_inter[1][k], need))) return rc;
+            c->d_inter[0][k] = c->d_inter[1][k];
+        }
         c->inter_cap = need;
     }
     if (!c->d_inter_cnt[0][0]) {
@@ -1053,11 +1070,11 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
     a.rg = rg;
     a.cols = c->d_cols;
     a.nx = nx;
-    a.s_lin = c->d_lin[0][0];
+    a.s_lin = c->d_dedup;
     a.s_lin_n = c->d_ctr->lin[0][0];
     a.p_lin = c->d_lin[set][1];
     a.p_lin_n = c->d_ctr->lin[set][1];
-    a.lin_cap = dedup ? c->lin_cap[0] : c->lin_cap[set];
+    a.lin_cap = dedup ? c->dedup_cap : c->lin_cap[set];
     a.pool_from = c->d_pool_from;
     a.pool_to = c->d_pool_to;
     a.pool_cursor = c->d_ctr->pool;
@@ -1115,7 +1132,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     c->records.clear();
     c->out_used = 0;
     c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
-    if ((rc = ensure_lin_set(c, 0, lin_cap_for(N0, 1.25)))) return rc;
+    if ((rc = ensure_dedup_list(c, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key. The kept
     // entries (full-length keys) go to survivor set 0, side 0, which round 0 partitions as its
@@ -1144,9 +1161,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         a.in = c->d_fine[0];
         a.cnt = c->d_fine_cnt[0];
         a.nx = next_out(c, 0, bbD, rg0);
-        a.lin = c->d_lin[0][0];
+        a.lin = c->d_dedup;
         a.lin_n = c->d_ctr->lin[0][0];
-        a.lin_cap = c->lin_cap[0];
+        a.lin_cap = c->dedup_cap;
         a.big = c->d_big;
         a.n_big = &c->d_ctr->n_big;
         a.big_cap = BIG_CAP;
@@ -1169,7 +1186,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                            next_out(c, 0, bbD, rg0))))
             return rc;
         if (c->h_ctr->lin_ovf) {
-            if ((rc = ensure_lin_set(c, 0, c->lin_cap[0] * 2))) return rc;
+            if ((rc = ensure_dedup_list(c, c->dedup_cap * 2))) return rc;
             continue;
         }
         if (c->h_ctr->nx_ovf) {
@@ -1232,7 +1249,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             }
             if (need_part) {
                 // round 0: the prefixes are the dedup list
-                const ListIn lp = t == 0 ? ListIn{c->d_lin[0][0], c->d_ctr->lin[0][0], c->lin_cap[0]}
+                const ListIn lp = t == 0 ? ListIn{c->d_dedup, c->d_ctr->lin[0][0], c->dedup_cap}
                                          : ListIn{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
                 if ((rc = partition(c, 1, lp, Pt, bb, rg))) return rc;
             }
