@@ -12,7 +12,9 @@
 #include <time.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/empanada.h"
@@ -1013,6 +1015,148 @@ NextSfxOut next_out(emp_ctx *c, int set, int bb, const RoundArgs &rg) {
     return o;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Multi-record framing of a round (cq_get_search_partitions, cqutils.cuh:107-137, used by
+// emit_edges, overlap.cu:395-447). When the suffix or the prefix list reaches M/2 entries, the
+// reference cuts both key-sorted lists into partitions of at most M/2 entries per side at a common
+// key m (the smaller of the two sides' last keys in the window, then lower_bound on each side),
+// and writes one record per partition with both sides non-empty. The lists here are in key order
+// bucket by bucket, so every cut falls inside one bucket: positions and counts below a key come
+// from the bucket counts plus that one bucket's entries, read back and sorted on the host. The
+// round's edges are already in key order, so the records are consecutive slices of them.
+struct FrameSide {
+    const Entry *fine = nullptr;  // final buckets (BUCKET_CAP slots each)
+    const SpillRec *spill = nullptr;
+    uint32_t nspill = 0;
+    std::vector<uint32_t> cnt;
+    std::vector<uint64_t> pre;  // exclusive prefix sums of cnt, nb + 1 of them
+    std::vector<SpillRec> hspill;
+    bool spill_read = false;
+    std::map<uint32_t, std::vector<std::pair<uint64_t, uint32_t>>> sorted;  // bucket -> (key, rid)
+};
+
+uint64_t entry_key(const Entry &e) { return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK); }
+
+int frame_bucket(emp_ctx *c, FrameSide &f, uint32_t b, const std::vector<std::pair<uint64_t, uint32_t>> **out) {
+    auto it = f.sorted.find(b);
+    if (it == f.sorted.end()) {
+        const uint32_t n = f.cnt[b], in_fine = std::min<uint32_t>(n, BUCKET_CAP);
+        std::vector<Entry> h(in_fine);
+        if (in_fine)
+            HIPCHK(c, hipMemcpyAsync(h.data(), f.fine + (size_t)b * BUCKET_CAP, in_fine * sizeof(Entry),
+                                     hipMemcpyDeviceToHost, c->stream));
+        if (n > in_fine && !f.spill_read) {  // the rest of a large bucket is in the spill area
+            f.hspill.resize(f.nspill);
+            if (f.nspill)
+                HIPCHK(c, hipMemcpyAsync(f.hspill.data(), f.spill, f.nspill * sizeof(SpillRec), hipMemcpyDeviceToHost,
+                                         c->stream));
+            f.spill_read = true;
+        }
+        if (int rc = sync(c)) return rc;
+        if (n > in_fine) {
+            for (const SpillRec &r : f.hspill)
+                if (r.bucket == b) h.push_back(r.e);
+            if (h.size() != n)
+                return fail(c, EMP_ERR_CAPACITY, "framing: bucket %u holds %zu entries, counted %u", b, h.size(), n);
+        }
+        std::vector<std::pair<uint64_t, uint32_t>> v(h.size());
+        for (size_t i = 0; i < h.size(); ++i) v[i] = {entry_key(h[i]), h[i].rid};
+        std::sort(v.begin(), v.end());
+        it = f.sorted.emplace(b, std::move(v)).first;
+    }
+    *out = &it->second;
+    return EMP_OK;
+}
+
+int frame_side(emp_ctx *c, FrameSide &f, const Entry *fine, const uint32_t *d_cnt, const SpillRec *spill,
+               uint32_t nspill, uint32_t nb, uint64_t want) {
+    f.fine = fine;
+    f.spill = spill;
+    f.nspill = nspill;
+    f.cnt.resize(nb);
+    HIPCHK(c, hipMemcpyAsync(f.cnt.data(), d_cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if (int rc = sync(c)) return rc;
+    f.pre.assign(nb + 1, 0);
+    for (uint32_t b = 0; b < nb; ++b) f.pre[b + 1] = f.pre[b] + f.cnt[b];
+    if (f.pre[nb] != want)
+        return fail(c, EMP_ERR_CAPACITY, "framing: %llu entries in buckets, %llu in the list",
+                    (unsigned long long)f.pre[nb], (unsigned long long)want);
+    return EMP_OK;
+}
+
+// Records of one round: the edge count of every partition with both sides non-empty, in order.
+int frame_round(emp_ctx *c, int bb, FrameSide &S, FrameSide &P, const uint32_t *d_ecnt, uint64_t E, uint64_t M,
+                std::vector<uint64_t> &recs) {
+    const uint32_t nb = 1u << bb;
+    const int shift = 29 - bb;
+    std::vector<uint32_t> ecnt(nb);
+    HIPCHK(c, hipMemcpyAsync(ecnt.data(), d_ecnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if (int rc = sync(c)) return rc;
+    std::vector<uint64_t> epre(nb + 1, 0);
+    for (uint32_t b = 0; b < nb; ++b) epre[b + 1] = epre[b] + ecnt[b];
+    if (epre[nb] != E) return fail(c, EMP_ERR_CAPACITY, "framing: %llu edges in buckets, %llu in the round",
+                                   (unsigned long long)epre[nb], (unsigned long long)E);
+    const uint64_t nA = S.pre[nb], nB = P.pre[nb], H = M / 2;
+    auto bucket_of = [&](uint64_t m) { return (uint32_t)((m >> 32) >> shift); };
+    auto key_at = [&](FrameSide &f, uint64_t r, uint64_t &key) -> int {
+        const uint32_t b = (uint32_t)(std::upper_bound(f.pre.begin(), f.pre.end(), r) - f.pre.begin()) - 1;
+        const std::vector<std::pair<uint64_t, uint32_t>> *v;
+        int rc = frame_bucket(c, f, b, &v);
+        if (rc) return rc;
+        key = (*v)[r - f.pre[b]].first;
+        return EMP_OK;
+    };
+    auto below = [&](FrameSide &f, uint64_t m, uint64_t &n) -> int {
+        const uint32_t b = bucket_of(m);
+        const std::vector<std::pair<uint64_t, uint32_t>> *v;
+        int rc = frame_bucket(c, f, b, &v);
+        if (rc) return rc;
+        n = f.pre[b] + (uint64_t)(std::lower_bound(v->begin(), v->end(), std::make_pair(m, 0u)) - v->begin());
+        return EMP_OK;
+    };
+    // edges whose key is below m: whole buckets below m's, then the valid pairs of m's bucket
+    auto edges_below = [&](uint64_t m, uint64_t &n) -> int {
+        const uint32_t b = bucket_of(m);
+        const std::vector<std::pair<uint64_t, uint32_t>> *sv, *pv;
+        int rc = frame_bucket(c, S, b, &sv);
+        if (!rc) rc = frame_bucket(c, P, b, &pv);
+        if (rc) return rc;
+        n = epre[b];
+        for (const auto &s : *sv) {
+            if (s.first >= m) break;
+            auto lo = std::lower_bound(pv->begin(), pv->end(), std::make_pair(s.first, 0u));
+            for (; lo != pv->end() && lo->first == s.first; ++lo) n += (lo->second >> 1) != (s.second >> 1);
+        }
+        return EMP_OK;
+    };
+    uint64_t offA = 0, offB = 0, ebase = 0;
+    int rc;
+    while (offA < nA && offB < nB) {
+        uint64_t nPa = std::min(nA - offA, H), nPb = std::min(nB - offB, H);
+        if (nPa < H && nPb < H) {
+            recs.push_back(E - ebase);  // the closing partition: everything left, both sides non-empty
+            return EMP_OK;
+        }
+        uint64_t ka, kb, ca, cb, eb;
+        if ((rc = key_at(S, offA + nPa - 1, ka)) || (rc = key_at(P, offB + nPb - 1, kb))) return rc;
+        const uint64_t m = std::min(ka, kb);
+        if ((rc = below(S, m, ca)) || (rc = below(P, m, cb)) || (rc = edges_below(m, eb))) return rc;
+        nPa = ca - offA;
+        nPb = cb - offB;
+        if (nPa == 0 && nPb == 0)  // more than M/2 equal keys: the reference never leaves this loop
+            return fail(c, EMP_ERR_PARTITION, "more than M/2 = %llu entries share one key", (unsigned long long)H);
+        if (nPa && nPb) recs.push_back(eb - ebase);
+        else if (eb != ebase) return fail(c, EMP_ERR_CAPACITY, "framing: edges in a one-sided partition");
+        ebase = eb;
+        offA += nPa;
+        offB += nPb;
+    }
+    // one side is used up: what is left of the other forms a partition without a record
+    if (ebase != E) return fail(c, EMP_ERR_CAPACITY, "framing: %llu edges outside the records",
+                                (unsigned long long)(E - ebase));
+    return EMP_OK;
+}
+
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket. Dedup: kept entries go to
 // survivor set 0 (side 0) and round 0's suffix buckets; join: prefix survivors to set `set`.
@@ -1202,16 +1346,13 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     // Edge slots: round t owns slots [t * stride, t * stride + 2^bb); bucket counts only shrink.
     const uint64_t stride = 1ull << bbD;
     if (St > 0 && rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)rounds * stride))) return rc;
-    std::vector<uint64_t> round_edges;
+    std::vector<uint64_t> round_edges, rec_edges;  // edges per round; per record
+    std::vector<int> rec_round;                      // round of each record
 
     int cur = 0, t = 0;
     for (; t < rounds && St > 0; ++t) {
         if (Pt == 0) break;  // no record can follow: every later round has an empty prefix list
-        if (St >= M / 2 || Pt >= M / 2)
-            return fail(c, EMP_ERR_PARTITION,
-                        "round %d: %llu suffixes / %llu prefixes reach M/2 = %llu: multi-record framing of "
-                        "cqutils.cuh:107-137 is not reproduced; raise -r",
-                        t, (unsigned long long)St, (unsigned long long)Pt, (unsigned long long)(M / 2));
+        const bool framed = St >= M / 2 || Pt >= M / 2;  // several records (frame_round)
         const int sset = t & 1, nset = sset ^ 1;
         const int bb = bbS[sset];
         const uint32_t nbt = 1u << bb;
@@ -1322,10 +1463,29 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             int id = kstat_id(c, "k_join");
             c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
         }
-        // one record per round (overlap.cu:422; written even when E == 0)
+        // one record per partition with both sides non-empty (overlap.cu:398-422; written even when
+        // E == 0): one per round unless a list reaches M/2
+        size_t nrec0 = rec_edges.size();
+        if (framed) {
+            FrameSide fs, fp;
+            const uint32_t nss = std::min(c->h_ctr->sspill[sset], c->sspill_cap[sset]);
+            const uint32_t nps = std::min(c->h_ctr->spill[1], c->spill_cap);
+            if ((rc = frame_side(c, fs, sin.fine, sin.cnt, sin.spill, nss, nbt, St)) ||
+                (rc = frame_side(c, fp, c->d_fine[1], c->d_fine_cnt[1], c->d_spill[1], nps, nbt, Pt)) ||
+                (rc = frame_round(c, bb, fs, fp, c->d_edge_cnt + slot0, E, M, rec_edges)))
+                return rc;
+        } else {
+            rec_edges.push_back(E);
+        }
+        for (size_t i = nrec0; i < rec_edges.size(); ++i) {
+            if (rec_edges[i] >= (1ull << 31))  // the reference's int edge offsets (overlap.cu:561,567)
+                return fail(c, EMP_ERR_PARTITION, "round %d: %llu edges in one record reach 2^31", t,
+                            (unsigned long long)rec_edges[i]);
+            rec_round.push_back(t);
+        }
         round_edges.push_back(E);
-        rs.records = 1;
-        S.n_records++;
+        rs.records = (int32_t)(rec_edges.size() - nrec0);
+        S.n_records += rs.records;
         S.n_edges += E;
         c->rounds.push_back(rs);
         St = Sn;
@@ -1346,10 +1506,10 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)t * stride);
         HIPCHK(c, launch_edge_compact(ea, c->stream));
     }
-    uint64_t off = 0;
-    for (int r = 0; r < t; ++r) {
-        c->records.push_back(emp_ctx::Record{round_edges[r], off, L - 1 - r});
-        off += round_edges[r];
+    uint64_t off = 0;  // the arena holds the rounds' edges back to back; records slice them in order
+    for (size_t i = 0; i < rec_edges.size(); ++i) {
+        c->records.push_back(emp_ctx::Record{rec_edges[i], off, L - 1 - rec_round[i]});
+        off += rec_edges[i];
     }
     c->out_used = total;
     return flush_records(c, sink, user);

@@ -218,6 +218,67 @@ def test_error_paths():
     assert e.value.code == -6
 
 
+@pytest.mark.parametrize("m", [300, 320, 450, 540])
+def test_search_partitions_records_vs_oracle(m):
+    """Lists reaching M/2 = m/2 split each round into key-aligned records (cqutils.cuh:107-137)."""
+    rng = random.Random(5)
+    reads = _random_reads(rng, 150, 30, 600, 0.5, 0.0)
+    data = FQ([r.encode() for r in reads])
+    rc, want, osum, orounds = ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * m)
+    assert rc == 0 and osum["n_records"] > len(orounds)
+    got, summ, rounds = run_gpu_fastq(data, min_ovlp=12, reduce_mem=16 * m)
+    assert got == want
+    assert summ["n_records"] == osum["n_records"]
+    assert [r["records"] for r in rounds] == [r["records"] for r in orounds]
+
+
+@pytest.mark.parametrize("m", [60000, 61000, 90000, 119999])
+def test_search_partitions_generated_vs_oracle(m):
+    """30,000 x 100 bp reads (both strands, 0.5 % substitutions) at M from 2n to 4n: two or three
+    records per round while a list holds M/2 entries, cuts inside buckets."""
+    g = gen.genome(2 * 10**5, 7)
+    r = gen.reads(g, 30000, 100, 0.5, 0.005, 8)
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), 30000, 100, min_ovlp=31, reduce_mem=16 * m)
+    assert rc == 0
+    with native.Context(min_ovlp=31, reduce_mem=16 * m) as c:
+        c.add_reads(r)
+        got = c.run()
+        summ, rounds = c.summary(), c.rounds()
+    assert got == want
+    assert summ["n_records"] == osum["n_records"]
+    assert [x["records"] for x in rounds] == [x["records"] for x in orounds]
+
+
+def test_search_partitions_with_large_buckets():
+    """A 3000 x 3000 key group (a bucket over capacity, spilled) in rounds framed at M = 2n."""
+    reads = _shared_overlap_reads(3000, 3000)
+    r = reads_of(reads)
+    m = 2 * r.shape[0]
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 40, min_ovlp=29, reduce_mem=16 * m)
+    assert rc == 0 and osum["n_records"] > 11
+    with native.Context(min_ovlp=29, reduce_mem=16 * m) as c:
+        c.add_reads(r)
+        got = c.run()
+        rounds = c.rounds()
+    assert any(x["big_buckets"] > 0 for x in rounds)
+    assert got == want
+
+
+def test_search_partition_that_cannot_form_is_an_error():
+    """Reads rc(B) + 10 random bases + B: at overlap 45 every suffix is B and every prefix rc(B), so
+    a window of M/2 equal keys leaves no cut (the reference loops forever); the oracle and the
+    library both stop with a partition error."""
+    rng = random.Random(21)
+    b = "".join(rng.choice("ACGT") for _ in range(45))
+    reads = [bf.revcomp(b) + "".join(rng.choice("ACGT") for _ in range(10)) + b for _ in range(200)]
+    data = FQ([x.encode() for x in reads])
+    assert ol.run_fastq(data, min_ovlp=31, reduce_mem=16 * 400)[0] == -6
+    with pytest.raises(native.EmpError) as e:
+        run_gpu_fastq(data, min_ovlp=31, reduce_mem=16 * 400)
+    assert e.value.code == -9
+    assert run_gpu_fastq(data, min_ovlp=31)[0] == ol.run_fastq(data, min_ovlp=31)[1]
+
+
 def _shared_overlap_reads(k, seed):
     """k reads ending in one common 30-mer and k reads starting with it: a single key group of k
     suffixes and k prefixes at overlap 30 (k*k edges)."""

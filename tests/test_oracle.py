@@ -196,6 +196,23 @@ def test_search_partitions_split_records():
     assert ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 200)[0] == -5
 
 
+def test_search_partition_without_a_cut_is_an_error():
+    """rc(B) + 10 random bases + B: at overlap 45 all 400 suffix keys are key(B) and all prefix keys
+    key(rc B); with M = 400 the first window is one key on both sides, lower_bound cuts nothing and
+    the reference's loop (cqutils.cuh:113-131) never ends. The oracle reports it; with M large the
+    same reads run (one record per round) and end with no edge at 45 (B is not its own rc)."""
+    rng = random.Random(21)
+    b = "".join(rng.choice("ACGT") for _ in range(45))
+    assert bf.revcomp(b) != b
+    reads = [bf.revcomp(b) + "".join(rng.choice("ACGT") for _ in range(10)) + b for _ in range(200)]
+    data = FQ([x.encode() for x in reads])
+    assert ol.run_fastq(data, min_ovlp=31, reduce_mem=16 * 400)[0] == -6
+    rc, out, summ, rounds = ol.run_fastq(data, min_ovlp=31)
+    assert rc == 0 and summ["n_records"] == 69
+    r45 = [r for r in rounds if r["ovlp"] == 45][0]
+    assert r45["s"] == 400 and r45["edges"] == 0
+
+
 def test_empty_and_trivial():
     rc, out, summ, _ = ol.run_fastq(b"")
     assert rc == 0 and out == b""
