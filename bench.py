@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
     ap.add_argument("--served-loads", default="0.5,1,2,4,8",
                     help="offered loads as multiples of the plain path's jobs/s")
+    ap.add_argument("--audit", action="store_true",
+                    help="after the timed steps, one more step with the collision audit on (reported apart)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -230,6 +232,31 @@ def main():
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9 if kd["ms"] > 0 else 0.0
     value = n * L / (ms_per_step * 1e-3) / 1e9
 
+    audit = None
+    if args.audit:  # not part of the metric: the audit adds a sort and string checks per list
+        upload()
+        ctx.set_profile(False)
+        ctx.set_audit(True)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        step()
+        hipmem.synchronize()
+        t_audit = time.perf_counter() - t0
+        ctx.set_audit(False)
+        au = ctx.audits()
+        coll = [a["strings"] - a["keys"] for a in au]
+        audit = {
+            "seconds": round(t_audit, 3),
+            "lists": len(au),
+            "collisions": sum(coll),
+            "max_per_list": max(coll) if coll else 0,
+            "by_list": {name: {"lists": sum(1 for a in au if a["list"] == k),
+                               "entries": sum(a["entries"] for a in au if a["list"] == k),
+                               "collisions": sum(a["strings"] - a["keys"] for a in au if a["list"] == k)}
+                        for k, name in ((0, "reads"), (1, "suffixes"), (2, "prefixes"))},
+            "expected_per_list_round_0": round(au[1]["entries"] ** 2 / 2 / 2.0**58, 4) if len(au) > 1 else None,
+        }
+
     same_full = None
     if args.check and not ooc:
         import oracle_lib
@@ -297,6 +324,8 @@ def main():
         "batched": batched,
         "served": served,
     }
+    if audit is not None:
+        line["audit"] = audit
     print(json.dumps(line), flush=True)
 
 

@@ -100,6 +100,9 @@ int main(int argc, char **argv) {
     }
     const char *env = getenv("EMPANADA_PROFILE");
     o.profile = env && *env && *env != '0';
+    // EMPANADA_AUDIT=1: the collision audit the reference builds with FPCHECK (overlap.cu:295-359)
+    const char *aenv = getenv("EMPANADA_AUDIT");
+    const bool audit = aenv && *aenv && *aenv != '0';
 
     double t_init = now_s();
     emp_ctx *ctx = nullptr;
@@ -108,6 +111,7 @@ int main(int argc, char **argv) {
         fprintf(stderr, "empanada: cannot create a context (%d)\n", rc);
         return 1;
     }
+    emp_set_audit(ctx, audit ? 1 : 0);
     double t0 = now_s();
     FILE *f = fopen(argv[1], "rb");
     if (!f) {
@@ -146,6 +150,15 @@ int main(int argc, char **argv) {
             (unsigned long long)s.n_reads, s.read_len, (unsigned long long)s.n_dedup, s.n_rounds,
             (unsigned long long)s.n_records, (unsigned long long)s.n_edges, t0 - t_init, t1 - t0, t2 - t1, t2 - t0,
             (t2 - t0) > 0 ? (double)s.n_reads * s.read_len / (t2 - t0) / 1e9 : 0.0);
+    for (int i = 0; audit && i < emp_audit_count(ctx); ++i) {
+        emp_audit a;
+        emp_get_audit(ctx, i, &a);
+        static const char *names[3] = {"reads", "suffixes", "prefixes"};
+        fprintf(stderr, "  audit %-8s len=%3d entries=%llu keys=%llu strings=%llu collision rate = %f%%\n",
+                names[a.list < 0 || a.list > 2 ? 0 : a.list], a.ovlp, (unsigned long long)a.entries,
+                (unsigned long long)a.keys, (unsigned long long)a.strings,
+                a.strings ? (1.0 - (double)a.keys / (double)a.strings) * 100.0 : 0.0);
+    }
     if (o.profile) {
         for (int i = 0; i < emp_kernel_count(ctx); ++i) {
             const char *name;

@@ -222,6 +222,25 @@ struct EdgeCopyArgs {
     uint32_t *out_from, *out_to;
 };
 
+// Collision audit of one key-bucketed list (FPCHECK, overlap.cu:295-359): the distinct keys and,
+// summed over keys, the distinct strings [start, start + len) of the oriented reads.
+struct AuditArgs {
+    const Entry *fine;              // final buckets of cap slots, with counts and spilled records
+    const uint32_t *cnt;
+    uint32_t cap;
+    const SpillRec *spill;
+    const uint32_t *spill_n;
+    uint32_t spill_cap;
+    const uint32_t *cols;
+    uint32_t G;
+    int32_t L, start, len;
+    const uint32_t *ids;            // large-bucket launch: bucket of each work-group; else null
+    const unsigned long long *off;  // large-bucket launch: scratch offset of each work-group
+    uint64_t *skey;
+    uint32_t *srid;
+    unsigned long long *out;        // [0] += distinct keys, [1] += distinct strings
+};
+
 hipError_t launch_map(const MapArgs &a, hipStream_t s);
 hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s);
 hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s);
@@ -231,6 +250,8 @@ hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
+// n work-groups: all buckets (a.ids null; buckets over a.cap are skipped) or the listed ones.
+hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,
                        uint64_t *out, hipStream_t s);
 

@@ -820,6 +820,71 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Collision audit (FPCHECK: fpcheck and print_false_positive_rate, overlap.cu:320-359). One
+// work-group per bucket sorts its entries by key (LDS, or global scratch for a bucket over
+// capacity); every entry then counts as a new key when it opens its key group, and as a new
+// string when no earlier entry of its group spells the same string.
+constexpr int AUDIT_THREADS = 256;
+
+// 2-bit code of base `pos` of oriented read `rid` (odd ids: reverse complement, code ^ 2).
+__device__ __forceinline__ uint32_t oriented_code(const AuditArgs &a, uint32_t rid, int pos) {
+    const int c = (rid & 1) ? a.L - 1 - pos : pos;
+    const uint32_t v = (group_codes(a.cols, a.G, (uint32_t)c >> 2, rid >> 1) >> (2 * (c & 3))) & 3u;
+    return (rid & 1) ? v ^ 2u : v;
+}
+
+__device__ bool same_string(const AuditArgs &a, uint32_t r1, uint32_t r2) {
+    if (((r1 ^ r2) & 1u) == 0) {
+        // one orientation: the same columns of both reads ([start, start+len) forward,
+        // [L-start-len, L-start) reverse-complemented), four at a time
+        const int c0 = (r1 & 1) ? a.L - a.start - a.len : a.start, c1 = c0 + a.len;
+        for (int g = c0 >> 2; g <= (c1 - 1) >> 2; ++g) {
+            const int lo = max(c0 - 4 * g, 0), hi = min(c1 - 4 * g, 4);
+            const uint32_t m = ((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u);
+            if ((group_codes(a.cols, a.G, (uint32_t)g, r1 >> 1) ^ group_codes(a.cols, a.G, (uint32_t)g, r2 >> 1)) & m)
+                return false;
+        }
+        return true;
+    }
+    for (int p = 0; p < a.len; ++p)
+        if (oriented_code(a, r1, a.start + p) != oriented_code(a, r2, a.start + p)) return false;
+    return true;
+}
+
+__global__ void __launch_bounds__(AUDIT_THREADS) k_audit(AuditArgs a) {
+    __shared__ uint64_t lk[BUCKET_CAP];
+    __shared__ uint32_t lr[BUCKET_CAP];
+    __shared__ uint32_t s_pos;
+    __shared__ unsigned long long red[2];
+    const bool big = a.ids != nullptr;
+    const uint32_t b = big ? a.ids[blockIdx.x] : blockIdx.x;
+    const uint32_t n = a.cnt[b];
+    if (n == 0 || (!big && n > a.cap)) return;  // a bucket over capacity comes in the large-bucket launch
+    uint64_t *K = big ? a.skey + a.off[blockIdx.x] : lk;
+    uint32_t *R = big ? a.srid + a.off[blockIdx.x] : lr;
+    const uint32_t n2 = pow2_at_least(n);
+    if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
+    gather_bucket(a.fine, n, a.cap, a.spill, a.spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
+    bitonic(K, R, nullptr, n2);
+    unsigned long long keys = 0, strs = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += AUDIT_THREADS) {
+        const uint64_t k = K[i];
+        keys += (i == 0 || K[i - 1] != k) ? 1u : 0u;
+        bool fresh = true;
+        for (uint32_t j = i; j-- > 0 && K[j] == k;)
+            if (same_string(a, R[i], R[j])) { fresh = false; break; }
+        strs += fresh ? 1u : 0u;
+    }
+    if (keys) atomicAdd(&red[0], keys);
+    if (strs) atomicAdd(&red[1], strs);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(a.out, red[0]);
+        atomicAdd(a.out + 1, red[1]);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_zero(ZeroArgs a) {
     for (int r = 0; r < a.count; ++r)
         for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.n[r]; i += gridDim.x * 256u) a.p[r][i] = a.v[r];
@@ -984,6 +1049,12 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint64_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)(g < 4096 ? g : 4096)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_audit, dim3(n), dim3(AUDIT_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

@@ -171,6 +171,10 @@ struct emp_ctx {
 
     // kernel timing
     std::vector<KStat> kstats;
+    // collision audit (emp_set_audit): one record per audited list, and its device counters
+    bool audit = false;
+    std::vector<emp_audit> audits;
+    unsigned long long *d_audit = nullptr;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
 };
@@ -1157,6 +1161,80 @@ int frame_round(emp_ctx *c, int bb, FrameSide &S, FrameSide &P, const uint32_t *
     return EMP_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Collision audit of one bucketed list (FPCHECK, overlap.cu:215-219,274-284,295-359): distinct
+// keys and distinct strings [start, start + len) of the oriented reads, summed over keys. The LDS
+// launch takes every bucket within capacity; the buckets over it follow in a launch over global
+// scratch. Appends one record to c->audits.
+int audit_list(emp_ctx *c, int list, int ovlp, uint64_t entries, const Entry *fine, const uint32_t *d_cnt,
+               const SpillRec *spill, const uint32_t *d_spill_n, uint32_t spill_cap, uint32_t nb, int start, int len) {
+    int rc;
+    if (!c->d_audit) {
+        void *p = nullptr;
+        if ((rc = dalloc(c, &p, 2 * sizeof(unsigned long long)))) return rc;
+        c->d_audit = (unsigned long long *)p;
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_audit, 0, 2 * sizeof(unsigned long long), c->stream));
+    AuditArgs a{};
+    a.fine = fine;
+    a.cnt = d_cnt;
+    a.cap = BUCKET_CAP;
+    a.spill = spill;
+    a.spill_n = d_spill_n;
+    a.spill_cap = spill_cap;
+    a.cols = c->d_cols;
+    a.G = col_groups((uint32_t)c->L);
+    a.L = c->L;
+    a.start = start;
+    a.len = len;
+    a.out = c->d_audit;
+    HIPCHK(c, launch_audit(a, nb, c->stream));
+    std::vector<uint32_t> cnt(nb);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), d_cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = sync(c))) return rc;
+    std::vector<uint32_t> ids;
+    std::vector<unsigned long long> off;
+    uint64_t tot = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (cnt[b] > BUCKET_CAP) {
+            uint64_t p2 = 1;
+            while (p2 < cnt[b]) p2 <<= 1;
+            ids.push_back(b);
+            off.push_back(tot);
+            tot += p2;
+        }
+    if (!ids.empty()) {
+        if (ids.size() > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "audit: %zu buckets over capacity", ids.size());
+        if ((rc = ensure_big_scratch(c, tot))) return rc;
+        if (!c->d_boff[0]) {
+            for (int s2 = 0; s2 < 2; ++s2) {
+                void *p = nullptr;
+                if ((rc = dalloc(c, &p, BIG_CAP * sizeof(unsigned long long)))) return rc;
+                c->d_boff[s2] = (unsigned long long *)p;
+            }
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_big, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_boff[0], off.data(), off.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                                 c->stream));
+        a.ids = c->d_big;
+        a.off = c->d_boff[0];
+        a.skey = c->d_bkey[0];
+        a.srid = c->d_brank[0];
+        HIPCHK(c, launch_audit(a, (uint32_t)ids.size(), c->stream));
+    }
+    unsigned long long h[2];
+    HIPCHK(c, hipMemcpyAsync(h, c->d_audit, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = sync(c))) return rc;
+    emp_audit r{};
+    r.list = list;
+    r.ovlp = ovlp;
+    r.entries = entries;
+    r.keys = h[0];
+    r.strings = h[1];
+    c->audits.push_back(r);
+    return EMP_OK;
+}
+
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket. Dedup: kept entries go to
 // survivor set 0 (side 0) and round 0's suffix buckets; join: prefix survivors to set `set`.
@@ -1248,6 +1326,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     if (herr) return fail(c, EMP_ERR_BASE, "a kept read holds a byte other than ACGTacgt");
 
     c->rounds.clear();
+    c->audits.clear();
     emp_summary &S = c->summary;
     S.n_reads = c->n;
     S.read_len = c->L;
@@ -1339,6 +1418,10 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         }
         break;
     }
+    // the 2n entries before dedup, in the dedup partition's buckets (overlap.cu:215-219)
+    if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_fine[0], c->d_fine_cnt[0], c->d_spill[0], &c->d_ctr->spill[0],
+                                     c->spill_cap, 1u << bbD, 0, L)))
+        return rc;
     const uint64_t D = sum_subs(c->h_ctr->lin[0][0]);
     uint64_t St = rounds > 0 ? c->h_ctr->stotal[0] : D, Pt = D;
     S.n_dedup = D;
@@ -1351,7 +1434,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     int cur = 0, t = 0;
     for (; t < rounds && St > 0; ++t) {
-        if (Pt == 0) break;  // no record can follow: every later round has an empty prefix list
+        // No record can follow an empty prefix list. (The lists mirror each other under reverse
+        // complement, so they empty together; the audit's records end here too.)
+        if (Pt == 0) break;
         const bool framed = St >= M / 2 || Pt >= M / 2;  // several records (frame_round)
         const int sset = t & 1, nset = sset ^ 1;
         const int bb = bbS[sset];
@@ -1476,6 +1561,14 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                 return rc;
         } else {
             rec_edges.push_back(E);
+        }
+        if (c->audit) {  // the lists as the round joins them (overlap.cu:274-284)
+            const int ell = L - 1 - t;
+            if ((rc = audit_list(c, 1, ell, St, sin.fine, sin.cnt, sin.spill, sin.spill_n, sin.spill_cap, nbt, t + 1,
+                                 ell)) ||
+                (rc = audit_list(c, 2, ell, Pt, c->d_fine[1], c->d_fine_cnt[1], c->d_spill[1], &c->d_ctr->spill[1],
+                                 c->spill_cap, nbt, 0, ell)))
+                return rc;
         }
         for (size_t i = nrec0; i < rec_edges.size(); ++i) {
             if (rec_edges[i] >= (1ull << 31))  // the reference's int edge offsets (overlap.cu:561,567)
@@ -1712,6 +1805,18 @@ void emp_kernel_clear(emp_ctx *c) {
 }
 
 void *emp_stream(const emp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+void emp_set_audit(emp_ctx *c, int32_t on) {
+    if (c) c->audit = on != 0;
+}
+
+int32_t emp_audit_count(const emp_ctx *c) { return c ? (int32_t)c->audits.size() : 0; }
+
+int emp_get_audit(const emp_ctx *c, int32_t i, emp_audit *out) {
+    if (!c || !out || i < 0 || (size_t)i >= c->audits.size()) return EMP_ERR_ARG;
+    *out = c->audits[(size_t)i];
+    return EMP_OK;
+}
 
 void emp_set_profile(emp_ctx *c, int32_t on) {
     if (!c) return;

@@ -23,6 +23,7 @@ EXPORTS = [
     "emp_opts_default", "emp_ctx_create", "emp_ctx_destroy", "emp_last_error", "emp_add_fastq", "emp_add_reads",
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
     "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
+    "emp_set_audit", "emp_audit_count", "emp_get_audit",
     "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll",
     "emp_batch_opts_default", "emp_batch_create", "emp_batch_destroy", "emp_batch_last_error", "emp_batch_submit",
     "emp_batch_submit_fastq", "emp_batch_job_bytes", "emp_batch_step", "emp_batch_run", "emp_batch_job_info",
@@ -41,6 +42,11 @@ class RoundStats(ctypes.Structure):
     _fields_ = [("s", ctypes.c_uint64), ("p", ctypes.c_uint64), ("s_after", ctypes.c_uint64),
                 ("p_after", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("ovlp", ctypes.c_int32),
                 ("records", ctypes.c_int32), ("bucket_bits", ctypes.c_int32), ("big_buckets", ctypes.c_int32)]
+
+
+class Audit(ctypes.Structure):
+    _fields_ = [("list", ctypes.c_int32), ("ovlp", ctypes.c_int32), ("entries", ctypes.c_uint64),
+                ("keys", ctypes.c_uint64), ("strings", ctypes.c_uint64)]
 
 
 class Summary(ctypes.Structure):
@@ -98,6 +104,10 @@ def lib():
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.emp_kernel_clear.argtypes = [vp]
         L.emp_set_profile.argtypes = [vp, i32]
+        L.emp_set_audit.argtypes = [vp, i32]
+        L.emp_audit_count.restype = i32
+        L.emp_audit_count.argtypes = [vp]
+        L.emp_get_audit.argtypes = [vp, i32, ctypes.POINTER(Audit)]
         L.emp_stream.restype = vp
         L.emp_stream.argtypes = [vp]
         L.emp_version.restype = cp
@@ -255,6 +265,20 @@ class Context:
 
     def set_profile(self, on: bool):
         lib().emp_set_profile(self._h, 1 if on else 0)
+
+    def set_audit(self, on: bool):
+        """Collision audit (FPCHECK) for later runs: see audits()."""
+        lib().emp_set_audit(self._h, 1 if on else 0)
+
+    def audits(self):
+        """Audit records of the last run: list (0 before dedup, 1 suffixes, 2 prefixes), ovlp,
+        entries, distinct keys, distinct strings."""
+        out = []
+        for i in range(lib().emp_audit_count(self._h)):
+            a = Audit()
+            self._check(lib().emp_get_audit(self._h, i, ctypes.byref(a)))
+            out.append({f: getattr(a, f) for f, _ in Audit._fields_})
+        return out
 
 
 def records_bytes(records):

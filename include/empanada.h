@@ -111,6 +111,26 @@ typedef struct {
 int emp_get_summary(const emp_ctx *ctx, emp_summary *out);
 int emp_get_round(const emp_ctx *ctx, int32_t round, emp_round_stats *out);
 
+/*
+ * Collision audit: the FPCHECK diagnostic (overlap.cu:215-219,274-284,295-359), a compile-time
+ * option of the reference, here switched per context. With it on, emp_run records, for the 2n
+ * entries before dedup and then for every round's suffix and prefix list as they are joined, the
+ * distinct fingerprints (n_uniq_fp) and the distinct strings summed over fingerprints (n_uniq_rd);
+ * the reference logs 1 - keys/strings as "Collision rate". Bases compare by their 2-bit code. The
+ * audit adds work and host synchronisation to every round; output.bin is unchanged.
+ */
+typedef struct {
+    int32_t list;             /* 0: the 2n entries before dedup (full reads), 1: suffixes, 2: prefixes */
+    int32_t ovlp;             /* string length: L for list 0, the round's overlap otherwise */
+    uint64_t entries;         /* list size */
+    uint64_t keys;            /* distinct fingerprints */
+    uint64_t strings;         /* distinct strings, summed over fingerprints */
+} emp_audit;
+
+void emp_set_audit(emp_ctx *ctx, int32_t on);
+int32_t emp_audit_count(const emp_ctx *ctx);
+int emp_get_audit(const emp_ctx *ctx, int32_t i, emp_audit *out);
+
 /* Kernel timing (profile != 0): launches and summed HIP-event milliseconds per kernel name. */
 int32_t emp_kernel_count(const emp_ctx *ctx);
 int emp_kernel_info(const emp_ctx *ctx, int32_t i, const char **name, uint64_t *launches, double *ms,

@@ -527,9 +527,55 @@ static int emit_edges(list_t *S, list_t *P, int ovlp, size_t M, eo_buf *out, eo_
  * transpose_reads (fingerprint.cu:125-153) gives the columns; generate_fingerprint_read_pairs
  * (fingerprint.cu:75-115) gives key(read_k) with id 2k and key(revcomp(read_k)) with id 2k+1.
  */
+/* 2-bit code of base `pos` of oriented read `rid` (odd ids: reverse complement, code ^ 2). */
+static inline unsigned ocode(const char *reads, int L, uint32_t rid, int pos) {
+    const char *r = reads + (size_t)(rid >> 1) * (size_t)L;
+    return (rid & 1) ? ((((unsigned char)r[L - 1 - pos] >> 1) & 3u) ^ 2u) : (((unsigned char)r[pos] >> 1) & 3u);
+}
+
+/*
+ * print_false_positive_rate (overlap.cu:342-359) with fpcheck (overlap.cu:320-340): over the
+ * key-sorted list, count the distinct keys, and in every key group of more than one entry the
+ * distinct substrings [start, start+len) of the oriented reads (a group of one counts one).
+ */
+static void audit_list(const list_t *l, const char *reads, int L, int start, int len, int list, int ovlp,
+                       eo_audit *audit, int audit_cap, int *n_audit) {
+    if (!audit || !n_audit || *n_audit >= audit_cap) return;
+    uint64_t keys = 0, strings = 0;
+    size_t i = 0;
+    while (i < l->n) {
+        size_t e = i + 1;
+        while (e < l->n && l->fp[e] == l->fp[i]) ++e;
+        ++keys;
+        for (size_t a = i; a < e; ++a) {
+            int fresh = 1;
+            for (size_t b = i; b < a && fresh; ++b) {
+                int same = 1;
+                for (int p = 0; p < len && same; ++p)
+                    same = ocode(reads, L, l->rid[a], start + p) == ocode(reads, L, l->rid[b], start + p);
+                if (same) fresh = 0;
+            }
+            strings += (uint64_t)fresh;
+        }
+        i = e;
+    }
+    eo_audit *r = &audit[(*n_audit)++];
+    r->list = list;
+    r->ovlp = ovlp;
+    r->entries = l->n;
+    r->keys = keys;
+    r->strings = strings;
+}
+
 int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
                      eo_summary *sum, eo_round *rounds) {
+    return eo_overlap_reads_audit(reads, n, L, min_ovlp, reduce_mem, out, sum, rounds, NULL, 0, NULL);
+}
+
+int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
+                           eo_summary *sum, eo_round *rounds, eo_audit *audit, int audit_cap, int *n_audit) {
     eo_init();
+    if (n_audit) *n_audit = 0;
     if (sum) memset(sum, 0, sizeof *sum);
     if (sum) { sum->n_reads = n; sum->read_len = L; }
     if (n == 0) return EO_OK;
@@ -562,6 +608,7 @@ int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_
     /* overlap.cu:212-213: sort by id (already id-ordered), then stable sort by key */
     sort_u32_u64(S.rid, S.fp, S.n);
     sort_u64_u32(S.fp, S.rid, S.n);
+    audit_list(&S, reads, L, 0, L, 0, L, audit, audit_cap, n_audit); /* overlap.cu:215-219 */
     /* cq_unique_by_key (cqutils.cuh:271-306), one partition since N0 <= M: keep the first. */
     {
         size_t w = 0;
@@ -596,6 +643,9 @@ int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_
         /* overlap.cu:271-272 */
         sort_u64_u32(S.fp, S.rid, S.n);
         sort_u64_u32(P.fp, P.rid, P.n);
+        /* overlap.cu:274-284 */
+        audit_list(&S, reads, L, round + 1, L - 1 - round, 1, L - 1 - round, audit, audit_cap, n_audit);
+        audit_list(&P, reads, L, 0, L - 1 - round, 2, L - 1 - round, audit, audit_cap, n_audit);
         /* overlap.cu:288-289 */
         rc = emit_edges(&S, &P, L - 1 - round, M, out, &st);
         if (rc) break;

@@ -85,6 +85,25 @@ int eo_parse_fastq_mem(const char *data, uint64_t size, uint64_t block_bytes, ch
 int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
                      eo_summary *sum, eo_round *rounds);
 
+/*
+ * Collision audit (the FPCHECK diagnostic, overlap.cu:215-219,274-284,295-359): for a key-sorted
+ * list, the distinct fingerprints (n_uniq_fp) and the distinct strings summed over fingerprints
+ * (n_uniq_rd), the strings being the oriented reads' substrings the keys stand for. One record for
+ * the 2n entries before dedup (full reads), then per round the suffix list and the prefix list
+ * as they are joined. Bases compare by their 2-bit code (case-insensitive).
+ */
+typedef struct {
+    int32_t list;      /* 0: the 2n entries before dedup, 1: suffixes, 2: prefixes */
+    int32_t ovlp;      /* string length: L for list 0, the round's overlap otherwise */
+    uint64_t entries;  /* list size */
+    uint64_t keys;     /* distinct fingerprints */
+    uint64_t strings;  /* distinct strings, summed over fingerprints */
+} eo_audit;
+
+/* eo_overlap_reads that also fills up to audit_cap audit records (*n_audit of them). */
+int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
+                           eo_summary *sum, eo_round *rounds, eo_audit *audit, int audit_cap, int *n_audit);
+
 /* Whole pipeline: FASTQ file -> output.bin file. */
 int eo_run_file(const char *in_path, const char *out_path, const eo_opts *opts, eo_summary *sum,
                 eo_round *rounds);

@@ -134,3 +134,48 @@ def wrap_free(reads, min_ovlp):
                 if (k >> 32) < 5 or (k & 0xFFFFFFFF) < 5:
                     return False
     return True
+
+
+def audit(reads, min_ovlp):
+    """The collision audit from strings: for the 2n oriented reads, then per round for the live
+    suffix and prefix lists, (list, ovlp, entries, distinct keys, distinct strings). The live lists
+    follow string-equality matching, so callers use inputs where no key collision joins a suffix
+    to a prefix (a collision inside one list is what the audit is for)."""
+    if not reads:
+        return []
+    L = len(reads[0])
+    orient = []
+    for r in reads:
+        orient.append(r)
+        orient.append(revcomp(r))
+
+    def rec(lst, ell, strs):
+        return dict(list=lst, ovlp=ell, entries=len(strs), keys=len({key(w) for w in strs}), strings=len(set(strs)))
+
+    out = [rec(0, L, orient)]
+    first = {}
+    for rid, s in enumerate(orient):
+        first.setdefault(key(s), rid)
+    S = set(first.values())
+    P = set(S)
+    t = 0
+    while t < L - min_ovlp and S:
+        ell = L - 1 - t
+        out.append(rec(1, ell, [orient[r][t + 1:] for r in S]))
+        out.append(rec(2, ell, [orient[r][:ell] for r in P]))
+        sg, pg = {}, {}
+        for r in S:
+            sg.setdefault(orient[r][t + 1:], []).append(r)
+        for r in P:
+            pg.setdefault(orient[r][:ell], []).append(r)
+        used_s, used_p = set(), set()
+        for w in set(sg) & set(pg):
+            for a in sg[w]:
+                for b in pg[w]:
+                    if a >> 1 != b >> 1:
+                        used_s.add(a)
+                        used_p.add(b)
+        S -= used_s
+        P -= used_p
+        t += 1
+    return out

@@ -24,6 +24,11 @@ class Round(ctypes.Structure):
                 ("records", ctypes.c_int)]
 
 
+class Audit(ctypes.Structure):
+    _fields_ = [("list", ctypes.c_int32), ("ovlp", ctypes.c_int32), ("entries", ctypes.c_uint64),
+                ("keys", ctypes.c_uint64), ("strings", ctypes.c_uint64)]
+
+
 class Buf(ctypes.Structure):
     _fields_ = [("data", ctypes.POINTER(ctypes.c_ubyte)), ("size", ctypes.c_size_t), ("cap", ctypes.c_size_t)]
 
@@ -53,6 +58,10 @@ def lib():
         L.eo_overlap_reads.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_uint64, ctypes.POINTER(Buf), ctypes.POINTER(Summary),
                                        ctypes.POINTER(Round)]
+        L.eo_overlap_reads_audit.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_uint64, ctypes.POINTER(Buf), ctypes.POINTER(Summary),
+                                             ctypes.POINTER(Round), ctypes.POINTER(Audit), ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int)]
         L.eo_parse_fastq_mem.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_int)]
@@ -124,3 +133,18 @@ def run_reads(reads: bytes, n: int, L: int, min_ovlp=31, reduce_mem=1 << 30):
     summ = {f: getattr(s, f) for f, _ in Summary._fields_}
     rounds = [{f: getattr(rr[i], f) for f, _ in Round._fields_} for i in range(min(s.n_rounds, 256))]
     return rc, out, summ, rounds
+
+
+def run_reads_audit(reads: bytes, n: int, L: int, min_ovlp=31, reduce_mem=1 << 30):
+    """run_reads plus the collision audit: (rc, output bytes, summary, list of audit dicts)."""
+    b = Buf()
+    s = Summary()
+    cap = 2 * 256 + 1
+    au = (Audit * cap)()
+    na = ctypes.c_int(0)
+    rc = lib().eo_overlap_reads_audit(reads, n, L, min_ovlp, reduce_mem, ctypes.byref(b), ctypes.byref(s), None,
+                                      au, cap, ctypes.byref(na))
+    out = ctypes.string_at(b.data, b.size) if b.size else b""
+    lib().eo_buf_free(ctypes.byref(b))
+    summ = {f: getattr(s, f) for f, _ in Summary._fields_}
+    return rc, out, summ, [{f: getattr(au[i], f) for f, _ in Audit._fields_} for i in range(na.value)]
