@@ -34,8 +34,10 @@ __device__ __forceinline__ uint32_t carry_of(uint32_t gw, uint32_t col, bool fwd
     return f;
 }
 
-__device__ __forceinline__ void regen_carry(const uint32_t *__restrict__ cols, const RoundArgs &r, Entry &e, bool sfx,
-                                            uint32_t gathered) {
+// Regenerates e for round r (suffix rule: NextSfx, else PrevPfx) from the gathered group word
+// (0xFFFFFFFF: none, the carried codes supply the base) and returns the base's encoding (1..4).
+__device__ __forceinline__ uint32_t regen_carry(const uint32_t *__restrict__ cols, const RoundArgs &r, Entry &e, bool sfx,
+                                                uint32_t gathered) {
     const bool odd = e.rid & 1;
     const bool fwd = sfx != odd;
     const uint32_t col = fwd ? r.col_t : r.col_back;
@@ -55,6 +57,7 @@ __device__ __forceinline__ void regen_carry(const uint32_t *__restrict__ cols, c
         prev_pfx(h1, h2, ev);
     e.h1 = h1 | ((f & 7u) << 29);
     e.h2 = h2 | (((f >> 3) & 7u) << 29);
+    return ev;
 }
 
 __device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
@@ -166,40 +169,47 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) {
     return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK);
 }
 
-// LDS of the join of one bucket: the prefix entries (12 B each), a used flag per prefix, and one
-// region that holds the hash table (chain heads + links) while probing and the matched records
-// afterwards; survivor counts and slots per run.
-// Target buckets one work-group writes in the next round's suffix list (NextSfxOut): an
-// open-addressed LDS table of bucket -> (entries, run base).
-constexpr int NXT_SLOTS = 32;
-struct NxtTable {
-    uint32_t key[NXT_SLOTS], cnt[NXT_SLOTS], base[NXT_SLOTS];
-    uint32_t total;
-};
+// Next-round target slots (NextOut). The entries of one bucket of width w = 2^src_shift land in a
+// few buckets of the next round (width 2^shift >= w), found from the entry's removed base e
+// (encoding 1..4) and, for prefixes, k = (h1 - e) mod 5:
+//   slots 0..11   suffixes: 3 (e - 1) + d, d = target - first target of base e (mod the bucket
+//                 count; d <= 2, the third only when the shifted range wraps past P1);
+//   slots 12..21  prefixes: 12 + 2 k + d, d <= 1;
+//   slots 30, 31  suffixes / prefixes outside these (the reference's u64 wrap for h1 < e): one
+//                 global atomic each.
+// Work-groups rank their entries per slot with wave ballots (no LDS atomics per entry) and
+// reserve one run per slot.
+constexpr int NSLOT = 32;
+constexpr uint32_t SLOT_SFX_OTHER = 30, SLOT_PFX_OTHER = 31;
 
-// Threads below NXT_SLOTS clear the table; a barrier must follow before it is used.
-__device__ __forceinline__ void nxt_init(NxtTable &T) {
-    if (threadIdx.x < NXT_SLOTS) {
-        T.key[threadIdx.x] = 0xFFFFFFFFu;
-        T.cnt[threadIdx.x] = 0;
-    }
-    if (threadIdx.x == 0) T.total = 0;
-}
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
-// Slot of target bucket nb (NXT_SLOTS when the table is full) and the entry's rank in it.
-__device__ __forceinline__ uint32_t nxt_slot(NxtTable &T, uint32_t nb, uint32_t &rank) {
-    uint32_t h = (nb * 0x9E3779B1u) >> 27;
-    for (int k = 0; k < NXT_SLOTS; ++k) {
-        const uint32_t prev = atomicCAS(&T.key[h], 0xFFFFFFFFu, nb);
-        if (prev == 0xFFFFFFFFu || prev == nb) {
-            rank = atomicAdd(&T.cnt[h], 1u);
-            return h;
+// Rank of this lane's entry among the wave's entries of its slot, counting earlier calls: `row`
+// is the wave's own NSLOT counters in LDS (no other wave touches them). Lanes with act false
+// take part in the ballots only. One iteration per distinct slot in the wave.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t *row, bool act, uint32_t slot) {
+    uint32_t rank = 0;
+    bool pend = act;
+    for (;;) {
+        const unsigned long long todo = __ballot(pend);
+        if (!todo) break;
+        const int first = __builtin_ctzll(todo);
+        const uint32_t s = (uint32_t)__builtin_amdgcn_readlane((int)slot, first);
+        const bool mine = pend && slot == s;
+        const unsigned long long m = __ballot(mine);
+        const uint32_t base = row[s];
+        if (mine) {
+            rank = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            pend = false;
         }
-        h = (h + 1) & (NXT_SLOTS - 1);
+        if (lane_id() == (uint32_t)first) row[s] = base + (uint32_t)__popcll(m);
     }
-    return NXT_SLOTS;
+    return rank;
 }
 
+// LDS of the join of one bucket: the prefix entries (12 B each), a used flag per prefix (then,
+// for prefix survivors, the column-group byte they gathered), and one region that holds the hash
+// table (chain heads + links) while probing and the matched records afterwards.
 template <int CAP, int MCAP, int NT>
 struct JoinShared {
     uint32_t pk1[CAP], pk2[CAP], prid[CAP];
@@ -212,12 +222,13 @@ struct JoinShared {
         struct {
             uint64_t msk[MCAP], mpk[MCAP];
             uint32_t msr[MCAP], mpr[MCAP], msc[MCAP];
+            uint32_t wcnt[NT / 64][NSLOT];  // per wave and slot: entries, then their run start
         } m;
     } u;
-    NxtTable nt;
+    uint32_t ntot[2];  // survivors per side
     uint32_t sh[NT / 64];
     uint32_t ms, mp, ovf, etot;
-    uint32_t rcnt[2 * NRANGE], rbase[2 * NRANGE];  // survivors per (side, read range), their slots
+    uint32_t rcnt[2];  // batched join: survivors per side
     unsigned long long base;
 };
 

@@ -52,21 +52,10 @@ constexpr uint32_t BUCKET_TARGET = EMP_BUCKET_TARGET;
 constexpr int PART_TILE = 4096;
 constexpr int PART_THREADS = EMP_PART_THREADS;
 constexpr int PART_MAX_BITS = 8;
-// Linear outputs (survivor lists, the edge pool) are split into SUBS sub-ranges, each with its
-// own cursor: work-group b reserves in sub-range b % SUBS, so no single counter serialises a
-// launch and, with work-groups dealt round-robin to the 8 XCDs, each cursor stays on one XCD.
+// The edge pool is split into SUBS sub-ranges, each with its own cursor: work-group b reserves in
+// sub-range b % SUBS, so no single counter serialises a launch and, with work-groups dealt
+// round-robin to the 8 XCDs, each cursor stays on one XCD.
 constexpr int SUBS = 256;
-// Survivor lists are also split by read-id range: the 512 reads of one 128-byte column line
-// (read index k, line k >> 9 of a column) share a range, ranges interleave line by line, and
-// range r owns sub-ranges r * SUBS_PER_RANGE ... The first partition pass of a round runs range r
-// on XCD r (work-group id modulo 8), so each XCD's L2 holds one eighth of the two columns the
-// round's regeneration gathers from.
-constexpr int NRANGE = 8;
-constexpr int SUBS_PER_RANGE = SUBS / NRANGE;
-__host__ __device__ __forceinline__ uint32_t read_range(uint32_t rid) { return (rid >> 10) & (NRANGE - 1); }
-__host__ __device__ __forceinline__ uint32_t range_sub(uint32_t range, uint32_t b) {
-    return range * SUBS_PER_RANGE + (b & (SUBS_PER_RANGE - 1));
-}
 
 // An entry that did not fit its final bucket, with the bucket it belongs to.
 struct SpillRec {
@@ -83,21 +72,28 @@ struct RoundArgs {
     uint32_t G;        // column groups of the column store, ceil(L / 4)
 };
 
-// Suffix entries regenerated for the next round and written straight into that round's suffix
-// buckets. NextSfx shifts h1 by e * R1^(l-1) (mod P1), e in 1..4, so the entries of one bucket
-// land in at most two buckets per base of a round with as many or fewer buckets: a work-group
-// collects its entries per target bucket in LDS and reserves one run per target.
-struct NextSfxOut {
-    int32_t active;          // 0: no next round
+// The next round's two lists (side 0 suffixes, side 1 prefixes), written by the kernel that finds
+// their entries, already regenerated and in their final buckets (bucket = top bits of h1). A
+// bucket of width w feeds few buckets of the next round (as many buckets or fewer):
+//   - NextSfx subtracts e * R1^(l-1) mod P1 from h1 (e in 1..4): at most 2 buckets per base;
+//   - PrevPfx multiplies h1 - e by R1^-1 = (P1 + 1) / 5, which is y = (h1 - e + k P1) / 5 with
+//     k = (h1 - e) mod 5: 5 ranges of width w / 5, at most 2 buckets each (6 in practice; the
+//     reference's u64 wrap for h1 < e sends an entry anywhere).
+// A work-group therefore collects its entries per (side, target bucket) in a small LDS table and
+// reserves one run per target; an entry whose target does not fit the table takes one global
+// atomic of its own.
+struct NextOut {
+    int32_t active;          // 0: no next round (survivors are only counted)
     int32_t shift;           // target bucket = (h1 & KEY_MASK) >> shift
+    int32_t src_shift;       // source buckets of the writing launch: h1 >> src_shift
     RoundArgs rg;            // the next round
-    Entry *fine;             // its suffix buckets, BUCKET_CAP slots each
-    uint32_t *cnt;           // their counts
-    SpillRec *spill;         // entries past a bucket's capacity
-    uint32_t *spill_n;
-    uint32_t spill_cap;
-    uint32_t *total;         // suffix entries of the next round
-    uint32_t *ovf;           // set when the spill area overflows
+    Entry *fine[2];          // its buckets per side, BUCKET_CAP slots each
+    uint32_t *cnt[2];        // their counts
+    SpillRec *spill[2];      // entries past a bucket's capacity
+    uint32_t *spill_n[2];
+    uint32_t spill_cap[2];
+    uint32_t *total[2];      // entries of the next round per side
+    uint32_t *ovf;           // set when a spill area overflows
     const uint32_t *cols;
 };
 
@@ -135,34 +131,22 @@ struct PartArgs {
     uint32_t *spill_n;
     uint32_t spill_cap;
     uint32_t *abort;         // set when an intermediate bucket or the spill area overflows
-    // First pass of a round: regenerate every entry (suffix or prefix rule) before bucketing.
-    RoundArgs rg;
-    int32_t sfx;
-    const uint32_t *cols;
 };
 
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
-    NextSfxOut nx;           // round 0's suffix list (the kept entries, regenerated)
-    Entry *lin;              // kept entries (full-length keys): round 0's prefix list, SUBS sub-ranges
-    uint32_t *lin_n;         // [SUBS] counts of the sub-ranges
-    uint32_t lin_cap;        // entries per sub-range
+    NextOut nx;              // the kept entries, regenerated: round 0's suffix and prefix lists
     uint32_t *big, *n_big;
     uint32_t big_cap;
     const uint32_t *abort;
-    uint32_t *lin_ovf;       // set when a sub-range overflows
 };
 
 struct JoinArgs {
     const Entry *s_in, *p_in;
     const uint32_t *s_cnt, *p_cnt;
     uint32_t nb;             // buckets
-    int32_t keep;            // write the survivors (a next round exists)
-    NextSfxOut nx;           // suffix survivors -> the next round's suffix buckets
-    Entry *p_lin;            // prefix survivors with this round's keys, SUBS sub-ranges of lin_cap
-    uint32_t *p_lin_n;
-    uint32_t lin_cap;
+    NextOut nx;              // survivors -> the next round's buckets (nx.active: a next round exists)
     uint32_t *pool_from, *pool_to;   // edge pool: SUBS sub-ranges of pool_cap edges, then the
     unsigned long long *pool_cursor; // overflow region [SUBS * pool_cap, + ovf_cap) with its
     unsigned long long pool_cap;     // own cursor for reservations a sub-range cannot take
@@ -173,7 +157,6 @@ struct JoinArgs {
     uint32_t *big, *n_big;
     uint32_t big_cap;
     const uint32_t *abort;
-    uint32_t *lin_ovf;
 };
 
 // Large buckets (more than BUCKET_CAP entries, or more matched records than MATCH_CAP): one
@@ -191,13 +174,9 @@ struct BigArgs {
     const uint32_t *s_spill_n, *p_spill_n;
     uint32_t spill_cap, p_spill_cap;  // capacities of s_spill / p_spill
     uint32_t cap;
-    int32_t keep;                  // write the survivors
     RoundArgs rg;                  // the round (join): survivors get their carried codes back
     const uint32_t *cols;
-    NextSfxOut nx;                 // suffix survivors (join) / kept entries (dedup) for the next round
-    Entry *s_lin, *p_lin;          // dedup: kept entries (s_lin); join: prefix survivors (p_lin)
-    uint32_t *s_lin_n, *p_lin_n;
-    uint32_t lin_cap;
+    NextOut nx;                    // survivors (join) / kept entries (dedup) for the next round
     uint32_t *pool_from, *pool_to;
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
@@ -205,7 +184,6 @@ struct BigArgs {
     unsigned long long ovf_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
-    uint32_t *lin_ovf;
 };
 
 // Up to 16 ranges of 32-bit words set to a value in one launch (per-round counter resets).

@@ -3,25 +3,25 @@
 // Data in HBM:
 //   reads    row-major ASCII kept reads (input of the map kernel only)
 //   columns  2-bit codes per read position, chunked by 2^20 reads (emp_device.h)
-//   entries  12-byte (h1, h2, rid) records. A list is linear after the kernel that produces it
-//            (map, dedup, join), then one or more partition passes group it by the top bits of
-//            h1 into buckets: bucket b of a pass with `bits` output bits holds the entries with
-//            h1 >> (29 - bits) == b, in a fixed-capacity slot array [bucket][cap] plus a count.
+//   entries  12-byte (h1, h2, rid) records, in buckets by the top bits of h1: bucket b of a list
+//            with `bits` bucket bits holds the entries with h1 >> (29 - bits) == b, in a
+//            fixed-capacity slot array [bucket][cap] plus a count (spilled records past it).
 //            Order inside a bucket is arbitrary: every order the output depends on is
 //            re-established in LDS.
 //
 // Kernels:
 //   k_map      transposition + forward/reverse-complement Rabin-Karp (fingerprint.cu:75-153)
-//   k_part     tile-sorted partition pass (at most 8 more key bits per pass); the first pass
-//              of a round regenerates its input for the round (overlap.cu:266-267, 684-690):
-//              one launch per list, so its column gathers touch a single column
-//   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222)
-//              (linear output; round 0 partitions it once as suffixes, once as prefixes)
+//   k_part     tile-sorted partition pass (at most 8 more key bits per pass): the dedup's sort of
+//              the 2n map entries (overlap.cu:211-212)
+//   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222); the kept
+//              entries, regenerated, go straight into round 0's suffix and prefix buckets
 //   k_join     per bucket of round t: LDS hash join of suffixes against prefixes, edges of the
 //              matched key groups in (key, rid&1, rid) x (rid&1, rid) order, removal of used
-//              entries (linear output, keys of round t) (overlap.cu:258-290, 361-456, 557-710)
+//              entries (overlap.cu:258-290, 361-456, 557-710); the survivors, regenerated for
+//              round t+1 (overlap.cu:266-267, 605-692), go straight into that round's buckets
 //   k_*_big    the same for buckets over capacity: global-memory sort + binary-search join
 //   k_edge_*   bucket-ordered compaction of the edges of a round
+//   k_audit    collision audit of one list (FPCHECK)
 #include "emp_common.h"
 #include "emp_kernels.h"
 
@@ -46,30 +46,74 @@ __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ col
 }
 
 // ---------------------------------------------------------------------------------------------
-// Next round's suffix list written by the kernel that finds its entries (NextSfxOut).
+// The next round's lists written by the kernel that finds their entries (NextOut), by slot (see
+// NSLOT): count (every thread: gathers, regenerates, finds its entries' slots and ranks them per
+// wave), reserve (one thread per slot: one run per target bucket), write.
 
-// Work-group form, in three steps so that a kernel can overlap the reservation with other work:
-//   next_targets  every thread: gathers, regenerates its entries e[q] (bit q of `mask`) for nx.rg
-//                 and enters each target bucket in the LDS table;
-//   (barrier)     then threads below NXT_SLOTS reserve one run per target (next_reserve) and one
-//                 thread adds the entries to nx.total;
-//   (barrier)     next_write: every thread rebuilds its entries and writes them.
-// Between the steps an entry is kept as one word: its gathered codes (9 bits, 0x100 = none), its
-// table slot (6 bits, NXT_SLOTS = table full) and its rank in the slot (12 bits).
+// First target bucket of the suffixes of source bucket b whose removed base has encoding ev:
+// the bucket of its smallest key minus NextSfx's subtrahend e * R1^(l-1) mod P1.
+__device__ __forceinline__ uint32_t sfx_first(const NextOut &nx, uint32_t b, uint32_t ev) {
+    const uint32_t lo = b << nx.src_shift, c = red1(ev * nx.rg.m1);
+    return (lo >= c ? lo - c : lo + P1 - c) >> nx.shift;
+}
+
+// First target bucket of the prefixes of source bucket b with (h1 - e) mod 5 == k: PrevPfx gives
+// (h1 - e + k P1) / 5, smallest for the smallest h1 - e (h1 >= lo, e <= 4).
+__device__ __forceinline__ uint32_t pfx_first(const NextOut &nx, uint32_t b, uint32_t k) {
+    const uint32_t lo = b << nx.src_shift, u = lo >= 4u ? lo - 4u : 0u;
+    return ((u + k * P1) / 5u) >> nx.shift;
+}
+
+__device__ __forceinline__ uint32_t sfx_slot(const NextOut &nx, uint32_t b, uint32_t ev, uint32_t h1n) {
+    const uint32_t nbmask = (1u << (29 - nx.shift)) - 1u;
+    const uint32_t d = ((h1n >> nx.shift) - sfx_first(nx, b, ev)) & nbmask;
+    return d < 3u ? 3u * (ev - 1u) + d : SLOT_SFX_OTHER;
+}
+
+// h1o: the prefix's key before PrevPfx (the u64 wrap for h1o < e lands anywhere).
+__device__ __forceinline__ uint32_t pfx_slot(const NextOut &nx, uint32_t b, uint32_t h1o, uint32_t ev, uint32_t h1n) {
+    if (h1o < ev) return SLOT_PFX_OTHER;
+    const uint32_t k = (h1o - ev) % 5u;
+    const uint32_t d = (h1n >> nx.shift) - pfx_first(nx, b, k);
+    return d < 2u ? 12u + 2u * k + d : SLOT_PFX_OTHER;
+}
+
+// Target bucket of direct slot s (< 22) of source bucket b.
+__device__ __forceinline__ uint32_t slot_bucket(const NextOut &nx, uint32_t b, uint32_t s) {
+    if (s < 12u) return (sfx_first(nx, b, s / 3u + 1u) + s % 3u) & ((1u << (29 - nx.shift)) - 1u);
+    return pfx_first(nx, b, (s - 12u) / 2u) + (s - 12u) % 2u;
+}
+
+// Stores entry e of side `side` at position pos of bucket nb (past the capacity: the spill area).
+__device__ __forceinline__ void next_put(const NextOut &nx, uint32_t side, uint32_t nb, uint32_t pos, const Entry &e) {
+    if (pos < (uint32_t)BUCKET_CAP) {
+        nx.fine[side][(size_t)nb * BUCKET_CAP + pos] = e;
+    } else {
+        const uint32_t sp = atomicAdd(nx.spill_n[side], 1u);
+        if (sp < nx.spill_cap[side]) nx.spill[side][sp] = SpillRec{e, nb};
+        else atomicOr(nx.ovf, 1u);
+    }
+}
+
+// Between count and write an entry is kept as one word: its gathered codes (9 bits, 0x100 =
+// none), its slot (5 bits) and its rank among the wave's entries of the slot (9 bits).
 __device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
 
+// Count step for register-resident entries e[q] (bit q of mask) of side `side` from source bucket
+// b; row: this wave's NSLOT counters (zeroed), ntot: the work-group's per-side totals.
 template <int PER>
-__device__ __forceinline__ void next_targets(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask,
-                                             uint32_t (&tgt)[PER]) {
+__device__ __forceinline__ void next_count(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot,
+                                           const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
     // An entry gathers when it enters a column group (one round in four): rounds where no lane
     // of the wave needs a word skip the loads; otherwise all of the wave's gathers are in flight
     // together (lanes without one read the store's first word, a cached line).
+    const bool sfx = side == 0;
     bool need[PER];
     uint32_t grp[PER], any = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         grp[q] = 0;
-        need[q] = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, true, grp[q]);
+        need[q] = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, sfx, grp[q]);
         any |= need[q];
         tgt[q] = 0x100u;
     }
@@ -82,72 +126,72 @@ __device__ __forceinline__ void next_targets(const NextSfxOut &nx, NxtTable &T, 
     }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        if ((mask >> q) & 1u) {
+        const bool act = (mask >> q) & 1u;
+        uint32_t slot = 0;
+        if (act) {
             Entry ne = e[q];
-            regen_carry(nx.cols, nx.rg, ne, true, tgt_word(tgt[q]));
-            uint32_t rk = 0;
-            const uint32_t sl = nxt_slot(T, (ne.h1 & KEY_MASK) >> nx.shift, rk);
-            tgt[q] |= (sl << 9) | (rk << 15);
+            const uint32_t ev = regen_carry(nx.cols, nx.rg, ne, sfx, tgt_word(tgt[q]));
+            const uint32_t h1n = ne.h1 & KEY_MASK;
+            slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, e[q].h1 & KEY_MASK, ev, h1n);
         }
+        tgt[q] |= (slot << 9) | (wave_rank(row, act, slot) << 14);
     }
-    if (mask) atomicAdd(&T.total, (uint32_t)__popc(mask));
+    uint32_t wn = 0;  // the wave's entries, added by one lane
+#pragma unroll
+    for (int q = 0; q < PER; ++q) wn += (uint32_t)__popcll(__ballot((mask >> q) & 1u));
+    if (lane_id() == 0 && wn) atomicAdd(ntot + side, wn);
 }
 
-// Run base of table slot k (thread k < NXT_SLOTS, after a barrier that follows next_targets).
-__device__ __forceinline__ uint32_t next_reserve(const NextSfxOut &nx, const NxtTable &T, uint32_t k) {
-    const uint32_t key = T.key[k], c = T.cnt[k];
-    return (key != 0xFFFFFFFFu && c) ? atomicAdd(nx.cnt + key, c) : 0u;
-}
-
+// Write step: `row` now holds this wave's run start per slot.
 template <int PER>
-__device__ __forceinline__ void next_write(const NextSfxOut &nx, const NxtTable &T, const Entry (&e)[PER], uint32_t mask,
-                                           const uint32_t (&tgt)[PER]) {
+__device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *row, const Entry (&e)[PER], uint32_t mask,
+                                           const uint32_t (&tgt)[PER], uint32_t side) {
+    const bool sfx = side == 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         if ((mask >> q) & 1u) {
             Entry ne = e[q];
-            regen_carry(nx.cols, nx.rg, ne, true, tgt_word(tgt[q]));
-            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 63u;
-            const uint32_t slot = sl < NXT_SLOTS ? T.base[sl] + (tgt[q] >> 15) : atomicAdd(nx.cnt + nb, 1u);
-            if (slot < (uint32_t)BUCKET_CAP) {
-                nx.fine[(size_t)nb * BUCKET_CAP + slot] = ne;
-            } else {
-                const uint32_t sp = atomicAdd(nx.spill_n, 1u);
-                if (sp < nx.spill_cap) nx.spill[sp] = SpillRec{ne, nb};
-                else atomicOr(nx.ovf, 1u);
-            }
+            regen_carry(nx.cols, nx.rg, ne, sfx, tgt_word(tgt[q]));
+            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
+            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[side] + nb, 1u);
+            next_put(nx, side, nb, pos, ne);
         }
     }
 }
 
-// The three steps back to back (every thread of the work-group calls it).
-template <int PER>
-__device__ __forceinline__ void scatter_next(const NextSfxOut &nx, NxtTable &T, const Entry (&e)[PER], uint32_t mask) {
-    uint32_t tgt[PER];
-    next_targets<PER>(nx, T, e, mask, tgt);
-    lds_barrier();
-    if (threadIdx.x < NXT_SLOTS) T.base[threadIdx.x] = next_reserve(nx, T, threadIdx.x);
-    else if (threadIdx.x == NXT_SLOTS && T.total) atomicAdd(nx.total, T.total);
-    lds_barrier();
-    next_write<PER>(nx, T, e, mask, tgt);
+// Reserve step, thread for slot s of an NW-wave work-group: the slot's entries over all waves,
+// one run in its target bucket (returned: the run start), to be turned into per-wave starts by
+// slot_starts once the reservation is back.
+template <int NW>
+__device__ __forceinline__ uint32_t slot_reserve(const NextOut &nx, uint32_t b, const uint32_t (*wcnt)[NSLOT], uint32_t s) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) tot += wcnt[w][s];
+    if (s >= 22u || tot == 0) return 0u;
+    return atomicAdd(nx.cnt[s < 12u ? 0 : 1] + slot_bucket(nx, b, s), tot);
 }
 
-// Single-entry form (large-bucket kernels): e is a suffix entry of this round with its carried
-// codes in place.
-__device__ __forceinline__ void put_next(const NextSfxOut &nx, Entry e) {
-    uint32_t g, word = 0xFFFFFFFFu;
-    if (regen_needs_gather(nx.rg, e.rid, true, g)) word = group_codes(nx.cols, nx.rg.G, g, e.rid >> 1);
-    regen_carry(nx.cols, nx.rg, e, true, word);
-    const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift;
-    const uint32_t slot = atomicAdd(nx.cnt + nb, 1u);
-    atomicAdd(nx.total, 1u);
-    if (slot < (uint32_t)BUCKET_CAP) {
-        nx.fine[(size_t)nb * BUCKET_CAP + slot] = e;
-    } else {
-        const uint32_t sp = atomicAdd(nx.spill_n, 1u);
-        if (sp < nx.spill_cap) nx.spill[sp] = SpillRec{e, nb};
-        else atomicOr(nx.ovf, 1u);
+template <int NW>
+__device__ __forceinline__ void slot_starts(uint32_t (*wcnt)[NSLOT], uint32_t s, uint32_t base) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const uint32_t c = wcnt[w][s];
+        wcnt[w][s] = base;
+        base += c;
     }
+}
+
+// Single-entry form (large-bucket kernels): e is an entry of this round (side `side`) with its
+// carried codes in place.
+__device__ __forceinline__ void put_next(const NextOut &nx, Entry e, uint32_t side) {
+    const bool sfx = side == 0;
+    uint32_t g, word = 0xFFFFFFFFu;
+    if (regen_needs_gather(nx.rg, e.rid, sfx, g)) word = group_codes(nx.cols, nx.rg.G, g, e.rid >> 1);
+    regen_carry(nx.cols, nx.rg, e, sfx, word);
+    const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift;
+    const uint32_t pos = atomicAdd(nx.cnt[side] + nb, 1u);
+    atomicAdd(nx.total[side], 1u);
+    next_put(nx, side, nb, pos, e);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -231,28 +275,15 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Partition pass: each work-group takes tiles of PART_TILE entries of one input group, ranks them
-// by their next `dbits` key bits in LDS, reserves one output range per (tile, bucket) with a
-// single atomic, and writes each bucket's run contiguously. With a.rg.active (first pass of a
-// round) the entries are streamed in with non-temporal loads, so that L2 keeps the one column
-// the launch gathers from, and regenerated before they are bucketed (REGEN = a.rg.active).
-template <bool REGEN>
+// Partition pass (the dedup's sort, overlap.cu:211-212): each work-group takes tiles of PART_TILE
+// entries of one input group, ranks them by their next `dbits` key bits in LDS, reserves one output
+// range per (tile, bucket) with a single atomic, and writes each bucket's run contiguously.
 __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_eu(EMP_PART_WAVES))) k_part(PartArgs a) {
     constexpr int NT = PART_THREADS, T = PART_TILE, PER = T / NT, NB = 1 << PART_MAX_BITS;
     static_assert(NT >= NB && T % NT == 0, "one bin per thread");
     __shared__ Entry buf[T];
     __shared__ uint32_t hist[NB], loc[NB], base[NB], sh[NT / 64];
-    // Regenerating pass: the input groups are the SUBS sub-ranges of a survivor list; the
-    // work-groups of read range r (sub-ranges r * SUBS_PER_RANGE ...) all have id = r mod 8.
-    uint32_t g, tile0;
-    if (REGEN) {
-        const uint32_t x = blockIdx.x & (NRANGE - 1), j = blockIdx.x / NRANGE;
-        g = x * SUBS_PER_RANGE + j / a.tiles_x;
-        tile0 = j % a.tiles_x;
-    } else {
-        g = blockIdx.x / a.tiles_x;
-        tile0 = blockIdx.x % a.tiles_x;
-    }
+    const uint32_t g = blockIdx.x / a.tiles_x, tile0 = blockIdx.x % a.tiles_x;
     const uint32_t F = 1u << a.dbits;
     uint32_t n_g;
     const Entry *in;
@@ -268,32 +299,13 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
         if (threadIdx.x < NB) hist[threadIdx.x] = 0;
         lds_barrier();
-        // Without regeneration every lane loads (lanes past the tile re-read its last entry), so the
-        // loads of all PER entries are in flight together.
+        // every lane loads (lanes past the tile re-read its last entry), so the loads of all PER
+        // entries are in flight together
         Entry e[PER];
         uint32_t r[PER], d[PER];
         const uint32_t last = nt - 1;
-        if (REGEN) {
-            // entry and gather per slot: within the 80 registers of 6 waves per SIMD, loading all
-            // 8 entries first spills
-            uint32_t word[PER];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                uint32_t i = threadIdx.x + NT * q;
-                word[q] = 0xFFFFFFFFu;
-                if (i < nt) {
-                    e[q] = load_stream(in + t0 + i);
-                    uint32_t g;
-                    if (regen_needs_gather(a.rg, e[q].rid, a.sfx, g)) word[q] = group_codes(a.cols, a.rg.G, g, e[q].rid >> 1);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < PER; ++q)
-                if (threadIdx.x + NT * q < nt) regen_carry(a.cols, a.rg, e[q], a.sfx, word[q]);
-        } else {
-#pragma unroll
-            for (int q = 0; q < PER; ++q) e[q] = in[t0 + min(threadIdx.x + NT * q, last)];
-        }
+        for (int q = 0; q < PER; ++q) e[q] = in[t0 + min(threadIdx.x + NT * q, last)];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             uint32_t i = threadIdx.x + NT * q;
@@ -346,8 +358,8 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     __shared__ uint32_t h1s[CAP], h2s[CAP], rids[CAP];
     __shared__ uint32_t head[CAP];
     __shared__ uint16_t nxt[CAP];
-    __shared__ uint32_t rcnt[NRANGE], rbase[NRANGE];
-    __shared__ NxtTable T;
+    __shared__ uint32_t wcnt[NT / 64][NSLOT];
+    __shared__ uint32_t ntot[2];
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
     const uint32_t n = a.cnt[b];
@@ -361,8 +373,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     if (n == 0) return;
     const Entry *in = a.in + (size_t)b * CAP;
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) head[i] = EMPTY;
-    if (threadIdx.x < NRANGE) rcnt[threadIdx.x] = 0;
-    nxt_init(T);
+    if (threadIdx.x < 2) ntot[threadIdx.x] = 0;
     {
         Entry e[PER];  // all loads in flight together (lanes past n re-read the last entry)
 #pragma unroll
@@ -376,8 +387,8 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
     lds_barrier();
     for (uint32_t i = threadIdx.x; i < n; i += NT) nxt[i] = (uint16_t)atomicExch(head + (h2s[i] & (CAP - 1)), i);
     lds_barrier();
-    // kept entries: a slot in their read range's run of this work-group (one reservation per range)
-    uint32_t keepmask = 0, slot[PER];
+    uint32_t keepmask = 0;
+    Entry kept[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         uint32_t i = threadIdx.x + NT * q;
@@ -388,29 +399,31 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
                 if (((h2s[j] ^ h2) & KEY_MASK) == 0 && ((h1s[j] ^ h1) & KEY_MASK) == 0 && rids[j] < rid) keep = false;
             if (keep) {
                 keepmask |= 1u << q;
-                slot[q] = atomicAdd(&rcnt[read_range(rid)], 1u);
+                kept[q] = Entry{h1, h2, rid};
             }
         }
     }
-    lds_barrier();
-    if (threadIdx.x < NRANGE) {
-        const uint32_t r = threadIdx.x, cnt = rcnt[r], sub = range_sub(r, b);
-        uint32_t c = cnt ? atomicAdd(a.lin_n + sub, cnt) : 0u;
-        if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
-        rbase[r] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
-    }
-    lds_barrier();
-    Entry kept[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        if (keepmask & (1u << q)) {
-            const uint32_t i = threadIdx.x + NT * q, base = rbase[read_range(rids[i])];
-            kept[q] = Entry{h1s[i], h2s[i], rids[i]};
-            if (base != 0xFFFFFFFFu) a.lin[base + slot[q]] = kept[q];
+    // the kept entries, regenerated, are round 0's suffix list and prefix list (overlap.cu:235-242)
+    if (a.nx.active) {
+        uint32_t *row = wcnt[threadIdx.x >> 6];
+        if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
+        uint32_t ts[PER], tp[PER];
+        next_count<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
+        next_count<PER>(a.nx, b, row, ntot, kept, keepmask, tp, 1);
+        lds_barrier();
+        if (threadIdx.x < NSLOT) {
+            slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64>(a.nx, b, wcnt, threadIdx.x));
+        } else if (threadIdx.x < NSLOT + 2) {
+            const uint32_t sd = threadIdx.x - NSLOT;
+            if (ntot[sd]) atomicAdd(a.nx.total[sd], ntot[sd]);
         }
+        lds_barrier();
+        next_write<PER>(a.nx, row, kept, keepmask, ts, 0);
+        next_write<PER>(a.nx, row, kept, keepmask, tp, 1);
+    } else if (keepmask) {  // no round follows: the kept entries are only counted
+        atomicAdd(a.nx.total[0], (uint32_t)__popc(keepmask));
+        atomicAdd(a.nx.total[1], (uint32_t)__popc(keepmask));
     }
-    // the kept entries are also round 0's suffix list
-    if (a.nx.active) scatter_next<PER>(a.nx, T, kept, keepmask);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -435,6 +448,64 @@ __device__ __forceinline__ void load_bucket(const JoinArgs &a, uint32_t b, uint3
     }
 }
 
+// Prefix survivors of the join (in LDS, slot threadIdx.x + NT * q for bit q of pmask): the count
+// and write steps of next_count / next_write with the entries read from LDS.
+template <int CAP, int MCAP, int NT>
+__device__ __forceinline__ void pfx_count(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t *row,
+                                          uint32_t pmask, uint32_t (&tgt)[CAP / NT]) {
+    constexpr int PER = CAP / NT;
+    bool need[PER];
+    uint32_t grp[PER], any = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        grp[q] = 0;
+        need[q] = ((pmask >> q) & 1u) && regen_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q]);
+        any |= need[q];
+        tgt[q] = 0x100u;
+    }
+    if (__any(any)) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t w = group_codes(nx.cols, nx.rg.G, need[q] ? grp[q] : 0u,
+                                           need[q] ? S.prid[threadIdx.x + NT * q] >> 1 : 0u);
+            if (need[q]) tgt[q] = w;
+        }
+    }
+    uint32_t wn = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const bool act = (pmask >> q) & 1u;
+        uint32_t slot = 0;
+        if (act) {
+            const uint32_t i = threadIdx.x + NT * q;
+            Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
+            const uint32_t h1o = e.h1 & KEY_MASK;
+            const uint32_t ev = regen_carry(nx.cols, nx.rg, e, false, tgt_word(tgt[q]));
+            slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
+        }
+        tgt[q] |= (slot << 9) | (wave_rank(row, act, slot) << 14);
+        wn += (uint32_t)__popcll(__ballot(act));
+    }
+    if (lane_id() == 0 && wn) atomicAdd(&S.ntot[1], wn);
+}
+
+template <int CAP, int MCAP, int NT>
+__device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CAP, MCAP, NT> &S, const uint32_t *row,
+                                          uint32_t pmask, const uint32_t (&tgt)[CAP / NT]) {
+    constexpr int PER = CAP / NT;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if ((pmask >> q) & 1u) {
+            const uint32_t i = threadIdx.x + NT * q;
+            Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
+            regen_carry(nx.cols, nx.rg, e, false, tgt_word(tgt[q]));
+            const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
+            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[1] + nb, 1u);
+            next_put(nx, 1, nb, pos, e);
+        }
+    }
+}
+
 // Join of one bucket: suffixes in registers se[] (slot threadIdx.x + NT * q), prefixes already
 // in S.pk1/pk2/prid with S.used cleared. Buckets over capacity, or with more matched records
 // than MCAP, are handed to the large-bucket path.
@@ -451,9 +522,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
         return;
     }
-    if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; }
-    if (threadIdx.x < 2 * NRANGE) S.rcnt[threadIdx.x] = 0;
-    nxt_init(S.nt);
+    if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; S.ntot[0] = 0; S.ntot[1] = 0; }
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
     uint32_t sc[PER];
 #pragma unroll
@@ -486,14 +555,12 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     }
     lds_barrier();  // the hash table is dead from here on; its space holds the matched records
 
-    // Matched records (key, rank) of both sides, the bucket's edge total, and a slot for every
-    // survivor (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710) in its
-    // run of (side, read range).
-    uint32_t smask = 0, pmask = 0, slot[PER], ecount = 0;
+    // Matched records (key, rank) of both sides, the bucket's edge total, and the survivors
+    // (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710).
+    uint32_t smask = 0, pmask = 0, ecount = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
-        slot[q] = 0;
         if (i < ns) {
             if (sc[q]) {
                 ecount += sc[q];
@@ -506,7 +573,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                     S.ovf = 1;
                 }
             } else {
-                smask |= 1u << q;  // a survivor: into the next round's buckets below
+                smask |= 1u << q;
             }
         }
         if (i < np) {
@@ -521,14 +588,21 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                 }
             } else {
                 pmask |= 1u << q;
-                slot[q] |= atomicAdd(&S.rcnt[NRANGE + read_range(rid)], 1u) << 16;
             }
         }
     }
-    // Suffix survivors: their next-round targets (the entries are rebuilt when written, below).
-    uint32_t tgt[PER];
-    if (a.nx.active) next_targets<PER>(a.nx, S.nt, se, smask, tgt);
-    else if (smask) atomicAdd(&S.nt.total, (uint32_t)__popc(smask));  // last round: only counted
+    // Survivors: their next-round slots and ranks (the entries are rebuilt when written, below).
+    // The wave's slot counters live past the matched records (the hash table is dead).
+    uint32_t tgt[PER], ptg[PER];
+    uint32_t *row = S.u.m.wcnt[threadIdx.x >> 6];
+    if (a.nx.active) {
+        if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
+        next_count<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0);
+        pfx_count<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg);
+    } else {  // last round: only counted
+        if (smask) atomicAdd(&S.ntot[0], (uint32_t)__popc(smask));
+        if (pmask) atomicAdd(&S.ntot[1], (uint32_t)__popc(pmask));
+    }
     if (ecount) atomicAdd(&S.etot, ecount);
     lds_barrier();
     if (S.ovf) {
@@ -542,20 +616,18 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     const uint32_t nms = S.ms, nmp = S.mp, tot = S.etot;
 
     // Reservations, issued now and read back after the sort so that their latency overlaps it:
-    // thread k in [8, 16) reserves the prefix survivor run of read range k % 8, thread 16 the edges,
-    // threads 64 + k the next-round suffix run of table slot k.
+    // thread 16 reserves the edges, threads 64 + k the next-round run of table slot k, and the two
+    // threads after them add the side totals.
     constexpr uint32_t NXT0 = 64;
-    static_assert(NT >= NXT0 + NXT_SLOTS + 1, "reservation threads");
+    static_assert(NT >= NXT0 + NSLOT + 2, "reservation threads");
     unsigned long long resv = 0;
-    if (threadIdx.x >= NRANGE && threadIdx.x < 2 * NRANGE) {
-        const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
-        resv = cnt ? atomicAdd(a.p_lin_n + sub, cnt) : 0u;
-    } else if (threadIdx.x == 2 * NRANGE && tot) {
+    if (threadIdx.x == 16 && tot) {
         resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)), (unsigned long long)tot);
-    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NXT_SLOTS && a.nx.active) {
-        resv = next_reserve(a.nx, S.nt, threadIdx.x - NXT0);
-    } else if (threadIdx.x == NXT0 + NXT_SLOTS && S.nt.total) {
-        atomicAdd(a.nx.total, S.nt.total);
+    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NSLOT && a.nx.active) {
+        resv = slot_reserve<NT / 64>(a.nx, b, S.u.m.wcnt, threadIdx.x - NXT0);
+    } else if (threadIdx.x >= NXT0 + NSLOT && threadIdx.x < NXT0 + NSLOT + 2) {
+        const uint32_t sd = threadIdx.x - NXT0 - NSLOT;
+        if (S.ntot[sd]) atomicAdd(a.nx.total[sd], S.ntot[sd]);
     }
 
     // Edges of the bucket: order matched suffixes and prefixes by (key, rank), then emit the
@@ -578,20 +650,15 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
         if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
     }
-    if (threadIdx.x >= NRANGE && threadIdx.x < 2 * NRANGE) {
-        const uint32_t k = threadIdx.x, cnt = S.rcnt[k], sub = range_sub(k & (NRANGE - 1), b);
-        uint32_t c = (uint32_t)resv;
-        if (c + cnt > a.lin_cap) { atomicOr(a.lin_ovf, 1u); c = 0xFFFFFFFFu; }
-        S.rbase[k] = c == 0xFFFFFFFFu ? c : sub * a.lin_cap + c;
-    } else if (threadIdx.x == 2 * NRANGE) {
+    if (threadIdx.x == 16) {
         const uint32_t sub = b & (SUBS - 1);
         const unsigned long long base =
             tot ? place_edges(resv, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap) : ~0ull;
         S.base = base;
         a.edge_off[b] = base;
         a.edge_cnt[b] = tot;
-    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NXT_SLOTS) {
-        S.nt.base[threadIdx.x - NXT0] = (uint32_t)resv;
+    } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NSLOT && a.nx.active) {
+        slot_starts<NT / 64>(S.u.m.wcnt, threadIdx.x - NXT0, (uint32_t)resv);
     }
     lds_barrier();
     if (nms) {
@@ -626,19 +693,11 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
     }
 
-    // Survivors: prefixes into their runs (the next round's first partition pass regenerates
-    // them), suffixes regenerated here into the next round's suffix buckets.
-    if (a.keep) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            if (pmask & (1u << q)) {
-                const uint32_t i = threadIdx.x + NT * q, rid = S.prid[i];
-                const uint32_t base = S.rbase[NRANGE + read_range(rid)];
-                if (base != 0xFFFFFFFFu) a.p_lin[base + (slot[q] >> 16)] = Entry{S.pk1[i], S.pk2[i], rid};
-            }
-        }
+    // Survivors into the next round's buckets.
+    if (a.nx.active) {
+        pfx_write<CAP, MCAP, NT>(a.nx, S, row, pmask, ptg);
+        next_write<PER>(a.nx, row, se, smask, tgt, 0);
     }
-    if (a.nx.active) next_write<PER>(a.nx, S.nt, se, smask, tgt);
 }
 
 // Prefix entries of a bucket (registers) into LDS, used flags cleared.
@@ -777,22 +836,21 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
             if (a.nx.active) {
                 Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
                 set_carry(en, carry_after(a.cols, a.rg, srid, true));
-                put_next(a.nx, en);
+                put_next(a.nx, en, 0);
             } else {
-                atomicAdd(a.nx.total, 1u);
+                atomicAdd(a.nx.total[0], 1u);
             }
         }
     }
     for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
         if (!PA[j]) {
             const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
-            Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
-            const uint32_t rs = range_sub(read_range(prd), b);
-            uint32_t pos = atomicAdd(a.p_lin_n + rs, 1u);
-            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else if (a.keep) {
+            if (a.nx.active) {
+                Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
                 set_carry(en, carry_after(a.cols, a.rg, prd, false));
-                a.p_lin[rs * a.lin_cap + pos] = en;
+                put_next(a.nx, en, 1);
+            } else {
+                atomicAdd(a.nx.total[1], 1u);
             }
         }
     }
@@ -807,15 +865,15 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     gather_bucket(a.s_in, n, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
     bitonic(K, R, nullptr, n2);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        if (i == 0 || K[i] != K[i - 1]) {
-            Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
-            const uint32_t sub = range_sub(read_range(R[i]), b);
-            uint32_t pos = atomicAdd(a.s_lin_n + sub, 1u);
-            if (pos >= a.lin_cap) atomicOr(a.lin_ovf, 1u);
-            else {
-                a.s_lin[sub * a.lin_cap + pos] = e;
+        if (i == 0 || K[i] != K[i - 1]) {  // the smallest id of its key: round 0's suffix and prefix
+            const Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
+            if (a.nx.active) {
+                put_next(a.nx, e, 0);
+                put_next(a.nx, e, 1);
+            } else {
+                atomicAdd(a.nx.total[0], 1u);
+                atomicAdd(a.nx.total[1], 1u);
             }
-            if (a.nx.active) put_next(a.nx, e);
         }
     }
 }
@@ -984,10 +1042,7 @@ hipError_t launch_map(const MapArgs &a, hipStream_t s) {
 hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {
     uint64_t grid = (uint64_t)groups * a.tiles_x;
     if (grid == 0) return hipSuccess;
-    if (a.rg.active)
-        hipLaunchKernelGGL(k_part<true>, dim3((uint32_t)grid), dim3(PART_THREADS), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_part<false>, dim3((uint32_t)grid), dim3(PART_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_part, dim3((uint32_t)grid), dim3(PART_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

@@ -33,13 +33,12 @@ constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large
 struct DevCounters {
     unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
     unsigned long long pool_ovf;    // cursor of the pool's overflow region
-    uint32_t lin[2][2][SUBS];       // [survivor set][side][sub-range] counts
-    uint32_t n_big, abort, lin_ovf, pad;
-    uint32_t spill[2];              // records spilled by the final partition pass
-    uint32_t in[2];                 // partition input sizes of linear lists, set by the host
-    uint32_t sspill[2];             // records spilled into suffix bucket set 0 / 1
-    uint32_t stotal[2];             // entries written into suffix bucket set 0 / 1
-    uint32_t nx_ovf, pad2;          // a suffix spill area overflowed
+    uint32_t n_big, abort;
+    uint32_t spill;                 // records spilled by the dedup partition's final pass
+    uint32_t in;                    // dedup partition input size, set by the host
+    uint32_t bspill[2][2];          // [set][side] records spilled into bucket set `set`
+    uint32_t btotal[2][2];          // [set][side] entries written into bucket set `set`
+    uint32_t nx_ovf, pad;           // a bucket set's spill area overflowed
 };
 
 double now_ms() {
@@ -123,26 +122,22 @@ struct emp_ctx {
     uint8_t *d_acc = nullptr;
     uint64_t acc_bytes = 0, acc_n = 0;
 
-    // overlap phase, per list side (0 = suffixes, 1 = prefixes)
-    // prefix survivor sets [set][1] (side 0 unused), and the dedup's kept entries (round 0's
-    // prefix list; counts in DevCounters::lin[0][0])
-    Entry *d_lin[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    uint32_t lin_cap[2] = {0, 0};   // entries per sub-range of each set
-    Entry *d_dedup = nullptr;
-    uint32_t dedup_cap = 0;
-    Entry *d_inter[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    uint32_t *d_inter_cnt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    uint64_t inter_cap = 0;  // entries per intermediate array
-    Entry *d_fine[2] = {nullptr, nullptr};
-    uint32_t *d_fine_cnt[2] = {nullptr, nullptr};
+    // overlap phase. Bucket set `set` holds round t's lists for t % 2 == set, per side (0 =
+    // suffixes, 1 = prefixes): the dedup writes set 0, round t's join writes set (t + 1) % 2.
+    Entry *d_bfine[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t *d_bcnt[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    SpillRec *d_bspill[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t bspill_cap[2][2] = {{0, 0}, {0, 0}};
     uint32_t nb_alloc = 0;
-    SpillRec *d_spill[2] = {nullptr, nullptr};
-    uint32_t spill_cap = 0;
-    // suffix buckets of rounds t (set t % 2), written by the dedup (round 0) or round t-1's join
-    Entry *d_sfine[2] = {nullptr, nullptr};
-    uint32_t *d_sfine_cnt[2] = {nullptr, nullptr};
-    SpillRec *d_sspill[2] = {nullptr, nullptr};
-    uint32_t sspill_cap[2] = {0, 0}, nb_s_alloc = 0;
+    // The dedup partition: intermediate buckets, and final buckets that live in bucket set 1's
+    // suffix slots (free until round 0's join writes them), with their counts and spill area.
+    Entry *d_inter[2] = {nullptr, nullptr};
+    uint32_t *d_inter_cnt[2] = {nullptr, nullptr};
+    uint64_t inter_cap = 0;  // entries per intermediate array
+    Entry *d_dfine = nullptr;
+    uint32_t *d_dcnt = nullptr;
+    SpillRec *d_dspill = nullptr;
+    uint32_t dspill_cap = 0;
     double slack = 1.125;
     uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
@@ -615,12 +610,6 @@ uint64_t inter_need(uint64_t x, int bb, double slack) {
     return need;
 }
 
-// Entries per sub-range for survivor lists of up to x entries.
-uint32_t lin_cap_for(uint64_t x, double slack) {
-    uint64_t per = (uint64_t)((double)x / SUBS * slack) + 1024;
-    return (uint32_t)((per + 63) & ~63ull);
-}
-
 int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
     dfree(c, *p);
     *p = nullptr;
@@ -631,142 +620,104 @@ int realloc_entries(emp_ctx *c, Entry **p, uint64_t n) {
     return EMP_OK;
 }
 
-// Prefix survivor set `set` with at least `cap` entries per sub-range.
-int ensure_lin_set(emp_ctx *c, int set, uint32_t cap) {
-    if (cap <= c->lin_cap[set]) return EMP_OK;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = realloc_entries(c, &c->d_lin[set][1], (uint64_t)SUBS * cap);
+int realloc_spill(emp_ctx *c, SpillRec **p, uint32_t n) {
+    dfree(c, *p);
+    *p = nullptr;
+    void *q = nullptr;
+    int rc = dalloc(c, &q, (size_t)n * sizeof(SpillRec));
     if (rc) return rc;
-    c->lin_cap[set] = cap;
+    *p = (SpillRec *)q;
     return EMP_OK;
 }
 
-// The dedup's kept-entry list with at least `cap` entries per sub-range.
-int ensure_dedup_list(emp_ctx *c, uint32_t cap) {
-    if (cap <= c->dedup_cap) return EMP_OK;
+// Spill area of bucket set `set`, side `side`, with `sp` records. Its contents are dropped: call it
+// only when the set holds no live round (before the kernel that fills it runs, or reruns).
+int grow_bspill(emp_ctx *c, int set, int side, uint32_t sp) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = realloc_entries(c, &c->d_dedup, (uint64_t)SUBS * cap);
+    int rc = realloc_spill(c, &c->d_bspill[set][side], sp);
     if (rc) return rc;
-    c->dedup_cap = cap;
+    c->bspill_cap[set][side] = sp;
     return EMP_OK;
 }
 
-// Suffix spill area of set `set` with `sp` records. Its contents are dropped: call it only when
-// the set holds no live round (before the kernel that fills it runs, or reruns).
-int grow_sspill(emp_ctx *c, int set, uint32_t sp) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    dfree(c, c->d_sspill[set]);
-    c->d_sspill[set] = nullptr;
-    void *p = nullptr;
-    int rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec));
-    if (rc) return rc;
-    c->d_sspill[set] = (SpillRec *)p;
-    c->sspill_cap[set] = sp;
-    return EMP_OK;
-}
-
-// Partition and join buffers for lists of up to x entries in 2^bb buckets.
+// Buffers for lists of up to x entries in 2^bb buckets: the two bucket sets, and the dedup
+// partition's intermediate buckets and spill area.
 int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
     int rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const uint32_t nb = 1u << bb;
-    // Slot arrays: the prefix side's (d_fine[1]) and the two suffix sets. The dedup partition's
-    // output (d_fine[0]) lives in suffix set 1, which round 0's join fills only after the dedup.
     if (nb > c->nb_alloc) {
-        for (int s = 0; s < 2; ++s) {
-            if (s == 1 && (rc = realloc_entries(c, &c->d_fine[1], (uint64_t)nb * BUCKET_CAP))) return rc;
-            dfree(c, c->d_fine_cnt[s]);
-            void *p = nullptr;
-            if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
-            c->d_fine_cnt[s] = (uint32_t *)p;
-        }
+        for (int set = 0; set < 2; ++set)
+            for (int side = 0; side < 2; ++side) {
+                if ((rc = realloc_entries(c, &c->d_bfine[set][side], (uint64_t)nb * BUCKET_CAP))) return rc;
+                dfree(c, c->d_bcnt[set][side]);
+                void *p = nullptr;
+                if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
+                c->d_bcnt[set][side] = (uint32_t *)p;
+            }
+        dfree(c, c->d_dcnt);
+        void *p = nullptr;
+        if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
+        c->d_dcnt = (uint32_t *)p;
         c->nb_alloc = nb;
     }
-    if (nb > c->nb_s_alloc) {
-        for (int s = 0; s < 2; ++s) {
-            if ((rc = realloc_entries(c, &c->d_sfine[s], (uint64_t)nb * BUCKET_CAP))) return rc;
-            dfree(c, c->d_sfine_cnt[s]);
-            void *p = nullptr;
-            if ((rc = dalloc(c, &p, (size_t)nb * sizeof(uint32_t)))) return rc;
-            c->d_sfine_cnt[s] = (uint32_t *)p;
-        }
-        c->nb_s_alloc = nb;
-    }
-    c->d_fine[0] = c->d_sfine[1];
-    // Intermediate partition buckets: the dedup (side 0) and the rounds (side 1) never overlap.
+    c->d_dfine = c->d_bfine[1][0];
     const uint64_t need = inter_need(x, bb, c->slack);
     if (need > c->inter_cap) {
-        for (int k = 0; k < 2; ++k) {
-            if ((rc = realloc_entries(c, &c->d_inter[1][k], need))) return rc;
-            c->d_inter[0][k] = c->d_inter[1][k];
-        }
+        for (int k = 0; k < 2; ++k)
+            if ((rc = realloc_entries(c, &c->d_inter[k], need))) return rc;
         c->inter_cap = need;
     }
-    if (!c->d_inter_cnt[0][0]) {
-        for (int s = 0; s < 2; ++s)
-            for (int k = 0; k < 2; ++k) {
-                void *p = nullptr;
-                if ((rc = dalloc(c, &p, (size_t)(1u << 16) * sizeof(uint32_t)))) return rc;
-                c->d_inter_cnt[s][k] = (uint32_t *)p;
-            }
+    if (!c->d_inter_cnt[0]) {
+        for (int k = 0; k < 2; ++k) {
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)(1u << 16) * sizeof(uint32_t)))) return rc;
+            c->d_inter_cnt[k] = (uint32_t *)p;
+        }
     }
     const uint32_t sp = (uint32_t)std::max<uint64_t>(1u << 16, x / 256);
-    if (sp > c->spill_cap) {
-        for (int s = 0; s < 2; ++s) {
-            dfree(c, c->d_spill[s]);
-            void *p = nullptr;
-            if ((rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec)))) return rc;
-            c->d_spill[s] = (SpillRec *)p;
-        }
-        c->spill_cap = sp;
+    if (sp > c->dspill_cap) {
+        if ((rc = realloc_spill(c, &c->d_dspill, sp))) return rc;
+        c->dspill_cap = sp;
     }
+    for (int set = 0; set < 2; ++set)
+        for (int side = 0; side < 2; ++side)
+            if (c->bspill_cap[set][side] < sp && (rc = grow_bspill(c, set, side, sp))) return rc;
     return EMP_OK;
 }
 
-// After an abort (an intermediate bucket or the spill area overflowed): more slack, more spill.
+// After an abort of the dedup partition (an intermediate bucket or the spill area overflowed):
+// more slack, more spill.
 int grow_after_abort(emp_ctx *c, uint64_t x, int bb) {
     c->slack *= 2.0;
-    uint32_t sp = c->spill_cap * 4u;
+    const uint32_t sp = c->dspill_cap * 4u;
     int rc = ensure_run_buffers(c, x, bb);
     if (rc) return rc;
-    if (c->spill_cap < sp) {
-        for (int s = 0; s < 2; ++s) {
-            dfree(c, c->d_spill[s]);
-            void *p = nullptr;
-            if ((rc = dalloc(c, &p, (size_t)sp * sizeof(SpillRec)))) return rc;
-            c->d_spill[s] = (SpillRec *)p;
-        }
-        c->spill_cap = sp;
+    if (c->dspill_cap < sp) {
+        if ((rc = realloc_spill(c, &c->d_dspill, sp))) return rc;
+        c->dspill_cap = sp;
     }
     return EMP_OK;
 }
 
-// A list handed to a partition: one linear array (count in d_ctr->in[side]) or SUBS segments.
-struct ListIn {
-    const Entry *base;
-    const uint32_t *cnt;  // null: linear
-    uint32_t cap;
-};
-
-// Partitions list `side` (x entries) into 2^bb buckets. The first pass regenerates the entries
-// for round `rg` (suffix rule for side 0, prefix rule for side 1) when rg.active.
-int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb, const RoundArgs &rg) {
-    if (rg.active && !li.cnt) return fail(c, EMP_ERR_ARG, "a regenerating partition needs a survivor list");
+// The dedup's sort (overlap.cu:211-212): the x map entries (linear, count in d_ctr->in) into
+// 2^bb buckets by passes of at most PART_MAX_BITS bits.
+int partition(emp_ctx *c, uint64_t x, int bb) {
     const std::vector<Level> pl = plan_levels(x, bb, c->slack);
-    const Entry *in = li.base;
-    const uint32_t *in_cnt = li.cnt;
-    uint32_t in_cap = li.cap, groups = li.cnt ? (uint32_t)SUBS : 1u;
+    const Entry *in = c->d_entries;
+    const uint32_t *in_cnt = nullptr;
+    uint32_t in_cap = 0, groups = 1;
     int keyed = 0;
     for (size_t i = 0; i < pl.size(); ++i) {
         const Level &l = pl[i];
         const bool fin = i + 1 == pl.size();
         const int bits_out = l.bits_in + l.dbits;
-        Entry *out = fin ? c->d_fine[side] : c->d_inter[side][i & 1];
-        uint32_t *out_cnt = fin ? c->d_fine_cnt[side] : c->d_inter_cnt[side][i & 1];
+        Entry *out = fin ? c->d_dfine : c->d_inter[i & 1];
+        uint32_t *out_cnt = fin ? c->d_dcnt : c->d_inter_cnt[i & 1];
         PartArgs a{};
         a.in = in;
         a.in_cnt = in_cnt;
-        a.in_n = &c->d_ctr->in[side];
+        a.in_n = &c->d_ctr->in;
         a.in_cap = in_cap;
         a.keyed = keyed;
         a.tiles_x = in_cnt ? std::max<uint32_t>(1, (in_cap + PART_TILE - 1) / PART_TILE)
@@ -776,13 +727,10 @@ int partition(emp_ctx *c, int side, const ListIn &li, uint64_t x, int bb, const 
         a.out_cap = l.cap;
         a.out_shift = 29 - bits_out;
         a.dbits = l.dbits;
-        a.spill = fin ? c->d_spill[side] : nullptr;
-        a.spill_n = &c->d_ctr->spill[side];
-        a.spill_cap = c->spill_cap;
+        a.spill = fin ? c->d_dspill : nullptr;
+        a.spill_n = &c->d_ctr->spill;
+        a.spill_cap = c->dspill_cap;
         a.abort = &c->d_ctr->abort;
-        if (i == 0) a.rg = rg;
-        a.sfx = side == 0;
-        a.cols = c->d_cols;
         {
             Timed t(c, "k_part", 24.0 * (double)x);
             HIPCHK(c, launch_part(a, groups, c->stream));
@@ -807,36 +755,26 @@ struct ZeroList {
     }
 };
 
-// Counts a partition of list `side` into 2^bb buckets accumulates.
-void zero_partition(emp_ctx *c, ZeroList &z, int side, uint64_t x, int bb) {
+// What the dedup partition of x entries into 2^bb buckets accumulates, its input size and flags.
+void zero_partition(emp_ctx *c, ZeroList &z, uint64_t x, int bb) {
     const std::vector<Level> pl = plan_levels(x, bb, c->slack);
     for (size_t i = 0; i < pl.size(); ++i) {
         const int bits_out = pl[i].bits_in + pl[i].dbits;
-        z.add(i + 1 == pl.size() ? c->d_fine_cnt[side] : c->d_inter_cnt[side][i & 1], 1ull << bits_out);
+        z.add(i + 1 == pl.size() ? c->d_dcnt : c->d_inter_cnt[i & 1], 1ull << bits_out);
     }
-}
-
-// Partition inputs (linear sizes), abort flag, spill counts.
-void zero_part_flags(emp_ctx *c, ZeroList &z, uint32_t xs, uint32_t xp) {
     z.add(&c->d_ctr->abort, 1);
-    z.add(&c->d_ctr->spill[0], 2);
-    z.add(&c->d_ctr->in[0], 1, xs);
-    z.add(&c->d_ctr->in[1], 1, xp);
+    z.add(&c->d_ctr->spill, 1);
+    z.add(&c->d_ctr->in, 1, (uint32_t)x);
 }
 
-// What a dedup / join launch accumulates: pool cursors, survivor set `set`, flags.
-void zero_join(emp_ctx *c, ZeroList &z, int set) {
-    z.add(&c->d_ctr->lin[set][0][0], 2 * SUBS);
-    z.add(&c->d_ctr->n_big, 1);
-    z.add(&c->d_ctr->lin_ovf, 1);
-}
-
-// What a launch writing suffix bucket set `set` (2^bb buckets) accumulates.
+// What a launch writing bucket set `set` (2^bb buckets per side) accumulates, and its flags.
 void zero_next(emp_ctx *c, ZeroList &z, int set, int bb) {
-    z.add(c->d_sfine_cnt[set], 1ull << bb);
-    z.add(&c->d_ctr->sspill[set], 1);
-    z.add(&c->d_ctr->stotal[set], 1);
+    z.add(c->d_bcnt[set][0], 1ull << bb);
+    z.add(c->d_bcnt[set][1], 1ull << bb);
+    z.add(&c->d_ctr->bspill[set][0], 2);
+    z.add(&c->d_ctr->btotal[set][0], 2);
     z.add(&c->d_ctr->nx_ovf, 1);
+    z.add(&c->d_ctr->n_big, 1);
 }
 
 int run_zero(emp_ctx *c, const ZeroList &z) {
@@ -966,13 +904,6 @@ int read_counters(emp_ctx *c) {
     return sync(c);
 }
 
-uint64_t sum_subs(const uint32_t *v) {
-    uint64_t s = 0;
-    for (int i = 0; i < SUBS; ++i) s += v[i];
-    return s;
-}
-
-
 int ensure_big_scratch(emp_ctx *c, uint64_t need) {
     if (need <= c->bscratch_cap) return EMP_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -993,8 +924,8 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
     return EMP_OK;
 }
 
-// Suffix-side input of a dedup or join launch: buckets, counts and spill area with its count.
-struct SfxIn {
+// One side of a bucket set as the input of a launch: buckets, counts, spill area with its count.
+struct SideIn {
     const Entry *fine;
     const uint32_t *cnt;
     const SpillRec *spill;
@@ -1002,18 +933,27 @@ struct SfxIn {
     uint32_t spill_cap;
 };
 
-// The next round's suffix buckets: set `set`, 2^bb buckets, for round rg (inactive: none).
-NextSfxOut next_out(emp_ctx *c, int set, int bb, const RoundArgs &rg) {
-    NextSfxOut o{};
+SideIn side_in(emp_ctx *c, int set, int side) {
+    return SideIn{c->d_bfine[set][side], c->d_bcnt[set][side], c->d_bspill[set][side], &c->d_ctr->bspill[set][side],
+                  c->bspill_cap[set][side]};
+}
+
+// Bucket set `set` as the output of a launch over 2^src_bb source buckets: 2^bb buckets per side,
+// for round rg (inactive: none).
+NextOut next_out(emp_ctx *c, int set, int bb, int src_bb, const RoundArgs &rg) {
+    NextOut o{};
     o.active = rg.active;
     o.shift = 29 - bb;
+    o.src_shift = 29 - src_bb;
     o.rg = rg;
-    o.fine = c->d_sfine[set];
-    o.cnt = c->d_sfine_cnt[set];
-    o.spill = c->d_sspill[set];
-    o.spill_n = &c->d_ctr->sspill[set];
-    o.spill_cap = c->sspill_cap[set];
-    o.total = &c->d_ctr->stotal[set];
+    for (int side = 0; side < 2; ++side) {
+        o.fine[side] = c->d_bfine[set][side];
+        o.cnt[side] = c->d_bcnt[set][side];
+        o.spill[side] = c->d_bspill[set][side];
+        o.spill_n[side] = &c->d_ctr->bspill[set][side];
+        o.spill_cap[side] = c->bspill_cap[set][side];
+        o.total[side] = &c->d_ctr->btotal[set][side];
+    }
     o.ovf = &c->d_ctr->nx_ovf;
     o.cols = c->d_cols;
     return o;
@@ -1236,17 +1176,19 @@ int audit_list(emp_ctx *c, int list, int ovlp, uint64_t entries, const Entry *fi
 }
 
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
-// global-memory sort + join (or dedup), one work-group per bucket. Dedup: kept entries go to
-// survivor set 0 (side 0) and round 0's suffix buckets; join: prefix survivors to set `set`.
-int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs &rg, uint64_t slot0,
-             const SfxIn &sin, const NextSfxOut &nx) {
+// global-memory sort + join (or dedup), one work-group per bucket, over the inputs `sin` / `pin`
+// (dedup: sin only); survivors (kept entries) go to nx.
+int big_path(emp_ctx *c, int bb, bool dedup, const RoundArgs &rg, uint64_t slot0, const SideIn &sin,
+             const SideIn &pin, const NextOut &nx) {
     const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
     std::vector<uint32_t> ids(nbig), cs(nb), cp(nb);
-    HIPCHK(c, hipMemcpy(ids.data(), c->d_big, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(cs.data(), sin.cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (!dedup) HIPCHK(c, hipMemcpy(cp.data(), c->d_fine_cnt[1], nb * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(ids.data(), c->d_big, nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cs.data(), sin.cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if (!dedup) HIPCHK(c, hipMemcpyAsync(cp.data(), pin.cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    int rc = sync(c);
+    if (rc) return rc;
     std::vector<unsigned long long> so(nbig), po(nbig);
     uint64_t as = 0, ap = 0;
     auto p2 = [](uint64_t n) { uint64_t p = 1; while (p < n) p <<= 1; return p; };
@@ -1256,8 +1198,7 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
         po[i] = ap;
         ap += dedup ? 0 : p2(cp[ids[i]]);
     }
-    int rc = ensure_big_scratch(c, std::max<uint64_t>(std::max(as, ap), 1));
-    if (rc) return rc;
+    if ((rc = ensure_big_scratch(c, std::max<uint64_t>(std::max(as, ap), 1)))) return rc;
     if (!c->d_boff[0]) {
         for (int s = 0; s < 2; ++s) {
             void *p = nullptr;
@@ -1278,25 +1219,21 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
     a.saux = c->d_baux[0];
     a.paux = c->d_baux[1];
     a.s_in = sin.fine;
-    a.p_in = c->d_fine[1];
     a.s_cnt = sin.cnt;
-    a.p_cnt = c->d_fine_cnt[1];
     a.s_spill = sin.spill;
-    a.p_spill = c->d_spill[1];
     a.s_spill_n = sin.spill_n;
-    a.p_spill_n = &c->d_ctr->spill[1];
     a.spill_cap = sin.spill_cap;
-    a.p_spill_cap = c->spill_cap;
+    if (!dedup) {
+        a.p_in = pin.fine;
+        a.p_cnt = pin.cnt;
+        a.p_spill = pin.spill;
+        a.p_spill_n = pin.spill_n;
+        a.p_spill_cap = pin.spill_cap;
+    }
     a.cap = BUCKET_CAP;
-    a.keep = keep ? 1 : 0;
     a.rg = rg;
     a.cols = c->d_cols;
     a.nx = nx;
-    a.s_lin = c->d_dedup;
-    a.s_lin_n = c->d_ctr->lin[0][0];
-    a.p_lin = c->d_lin[set][1];
-    a.p_lin_n = c->d_ctr->lin[set][1];
-    a.lin_cap = dedup ? c->dedup_cap : c->lin_cap[set];
     a.pool_from = c->d_pool_from;
     a.pool_to = c->d_pool_to;
     a.pool_cursor = c->d_ctr->pool;
@@ -1305,7 +1242,6 @@ int big_path(emp_ctx *c, int bb, bool keep, bool dedup, int set, const RoundArgs
     a.ovf_cap = c->pool_ovf_cap;
     a.edge_off = c->d_edge_off + slot0;
     a.edge_cnt = c->d_edge_cnt + slot0;
-    a.lin_ovf = &c->d_ctr->lin_ovf;
     {
         Timed t(c, dedup ? "k_dedup_big" : "k_join_big", 0.0);
         HIPCHK(c, dedup ? launch_dedup_big(a, nbig, c->stream) : launch_join_big(a, nbig, c->stream));
@@ -1348,20 +1284,16 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
-    for (int s = 0; s < 2; ++s)  // suffix spill areas as large as the partition's
-        if (c->sspill_cap[s] < c->spill_cap && (rc = grow_sspill(c, s, c->spill_cap))) return rc;
     if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
         return rc;
     c->records.clear();
     c->out_used = 0;
     c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
-    if ((rc = ensure_dedup_list(c, lin_cap_for(N0, 1.25)))) return rc;
 
     // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key. The kept
-    // entries (full-length keys) go to survivor set 0, side 0, which round 0 partitions as its
-    // prefix list, and, regenerated for round 0, into suffix bucket set 0: round 0's suffix list.
-    // Suffix bucket set t % 2 holds round t's suffixes; bbS[set] is the bucket bits it was
-    // written with (the round's prefix partition uses the same).
+    // entries (full-length keys), regenerated for round 0, go into bucket set 0 as round 0's
+    // suffix and prefix lists (overlap.cu:235-242). Bucket set t % 2 holds round t's lists;
+    // bbS[set] is the bucket bits it was written with.
     const RoundArgs none{};
     const RoundArgs rg0 = rounds > 0 ? round_args(c, 0, true) : none;
     int bbS[2] = {bbD, bbD};
@@ -1370,28 +1302,20 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
         {
             ZeroList z;
-            if (need_part) {
-                zero_part_flags(c, z, (uint32_t)N0, 0);
-                zero_partition(c, z, 0, N0, bbD);
-            }
-            zero_join(c, z, 0);
+            if (need_part) zero_partition(c, z, N0, bbD);
             zero_next(c, z, 0, bbD);
             if (attempt == 0) z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors, once per run
             if ((rc = run_zero(c, z))) return rc;
         }
-        if (need_part && (rc = partition(c, 0, ListIn{c->d_entries, nullptr, 0}, N0, bbD, none))) return rc;
+        if (need_part && (rc = partition(c, N0, bbD))) return rc;
         DedupArgs a{};
-        a.in = c->d_fine[0];
-        a.cnt = c->d_fine_cnt[0];
-        a.nx = next_out(c, 0, bbD, rg0);
-        a.lin = c->d_dedup;
-        a.lin_n = c->d_ctr->lin[0][0];
-        a.lin_cap = c->dedup_cap;
+        a.in = c->d_dfine;
+        a.cnt = c->d_dcnt;
+        a.nx = next_out(c, 0, bbD, bbD, rg0);
         a.big = c->d_big;
         a.n_big = &c->d_ctr->n_big;
         a.big_cap = BIG_CAP;
         a.abort = &c->d_ctr->abort;
-        a.lin_ovf = &c->d_ctr->lin_ovf;
         {
             Timed t(c, "k_dedup", 12.0 * N0);
             HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
@@ -1403,27 +1327,23 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             continue;
         }
         need_part = false;
-        if (!c->h_ctr->lin_ovf && !c->h_ctr->nx_ovf && c->h_ctr->n_big &&
-            (rc = big_path(c, bbD, true, true, 0, none, 0,
-                           SfxIn{c->d_fine[0], c->d_fine_cnt[0], c->d_spill[0], &c->d_ctr->spill[0], c->spill_cap},
-                           next_out(c, 0, bbD, rg0))))
+        const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
+        if (!c->h_ctr->nx_ovf && c->h_ctr->n_big &&
+            (rc = big_path(c, bbD, true, none, 0, din, din, next_out(c, 0, bbD, bbD, rg0))))
             return rc;
-        if (c->h_ctr->lin_ovf) {
-            if ((rc = ensure_dedup_list(c, c->dedup_cap * 2))) return rc;
-            continue;
-        }
         if (c->h_ctr->nx_ovf) {
-            if ((rc = grow_sspill(c, 0, c->sspill_cap[0] * 4))) return rc;
+            for (int side = 0; side < 2; ++side)
+                if ((rc = grow_bspill(c, 0, side, c->bspill_cap[0][side] * 4))) return rc;
             continue;
         }
         break;
     }
     // the 2n entries before dedup, in the dedup partition's buckets (overlap.cu:215-219)
-    if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_fine[0], c->d_fine_cnt[0], c->d_spill[0], &c->d_ctr->spill[0],
-                                     c->spill_cap, 1u << bbD, 0, L)))
+    if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill,
+                                     c->dspill_cap, 1u << bbD, 0, L)))
         return rc;
-    const uint64_t D = sum_subs(c->h_ctr->lin[0][0]);
-    uint64_t St = rounds > 0 ? c->h_ctr->stotal[0] : D, Pt = D;
+    const uint64_t D = c->h_ctr->btotal[0][0];
+    uint64_t St = D, Pt = c->h_ctr->btotal[0][1];
     S.n_dedup = D;
 
     // Edge slots: round t owns slots [t * stride, t * stride + 2^bb); bucket counts only shrink.
@@ -1432,7 +1352,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     std::vector<uint64_t> round_edges, rec_edges;  // edges per round; per record
     std::vector<int> rec_round;                      // round of each record
 
-    int cur = 0, t = 0;
+    int t = 0;
     for (; t < rounds && St > 0; ++t) {
         // No record can follow an empty prefix list. (The lists mirror each other under reverse
         // complement, so they empty together; the audit's records end here too.)
@@ -1442,22 +1362,18 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         const int bb = bbS[sset];
         const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
-        // the next round's buckets, chosen now because this round's join writes its suffixes
+        // the next round's buckets, chosen now because this round's join writes them
         const int bbn = more ? std::min(bb, bucket_bits(std::max(St, Pt))) : bb;
         const RoundArgs rg = round_args(c, t, true);
         const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
-        const int nxt = cur ^ 1;
-        if ((rc = ensure_lin_set(c, nxt, lin_cap_for(Pt, 1.25)))) return rc;
         emp_round_stats rs{};
         rs.s = St;
         rs.p = Pt;
         rs.ovlp = L - 1 - t;
         rs.bucket_bits = bb;
-        need_part = true;
         uint64_t E = 0;
         const uint64_t slot0 = (uint64_t)t * stride;
-        const SfxIn sin{c->d_sfine[sset], c->d_sfine_cnt[sset], c->d_sspill[sset], &c->d_ctr->sspill[sset],
-                        c->sspill_cap[sset]};
+        const SideIn sin = side_in(c, sset, 0), pin = side_in(c, sset, 1);
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
             if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
@@ -1465,30 +1381,15 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                                          hipMemcpyHostToDevice, c->stream));
             {
                 ZeroList z;
-                if (need_part) {
-                    zero_part_flags(c, z, 0, 0);
-                    zero_partition(c, z, 1, Pt, bb);
-                }
-                zero_join(c, z, nxt);
                 zero_next(c, z, nset, bbn);
                 if ((rc = run_zero(c, z))) return rc;
             }
-            if (need_part) {
-                // round 0: the prefixes are the dedup list
-                const ListIn lp = t == 0 ? ListIn{c->d_dedup, c->d_ctr->lin[0][0], c->dedup_cap}
-                                         : ListIn{c->d_lin[cur][1], c->d_ctr->lin[cur][1], c->lin_cap[cur]};
-                if ((rc = partition(c, 1, lp, Pt, bb, rg))) return rc;
-            }
             JoinArgs a{};
             a.s_in = sin.fine;
-            a.p_in = c->d_fine[1];
+            a.p_in = pin.fine;
             a.s_cnt = sin.cnt;
-            a.p_cnt = c->d_fine_cnt[1];
-            a.keep = more ? 1 : 0;
-            a.nx = next_out(c, nset, bbn, rgn);
-            a.p_lin = c->d_lin[nxt][1];
-            a.p_lin_n = c->d_ctr->lin[nxt][1];
-            a.lin_cap = c->lin_cap[nxt];
+            a.p_cnt = pin.cnt;
+            a.nx = next_out(c, nset, bbn, bb, rgn);
             a.pool_from = c->d_pool_from;
             a.pool_to = c->d_pool_to;
             a.pool_cursor = c->d_ctr->pool;
@@ -1501,28 +1402,18 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             a.n_big = &c->d_ctr->n_big;
             a.big_cap = BIG_CAP;
             a.abort = &c->d_ctr->abort;
-            a.lin_ovf = &c->d_ctr->lin_ovf;
             {
                 Timed tm(c, "k_join", 0.0);
                 HIPCHK(c, launch_join(a, nbt, c->stream));
             }
             if ((rc = read_counters(c))) return rc;
-            if (c->h_ctr->abort) {
-                if ((rc = grow_after_abort(c, std::max(St, Pt), bb))) return rc;
-                need_part = true;
-                continue;
-            }
-            need_part = false;
             rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (c->h_ctr->n_big &&
-                (rc = big_path(c, bb, more, false, nxt, rg, slot0, sin, next_out(c, nset, bbn, rgn))))
+            if (!c->h_ctr->nx_ovf && c->h_ctr->n_big &&
+                (rc = big_path(c, bb, false, rg, slot0, sin, pin, next_out(c, nset, bbn, bb, rgn))))
                 return rc;
-            if (c->h_ctr->lin_ovf) {
-                if ((rc = ensure_lin_set(c, nxt, c->lin_cap[nxt] * 2))) return rc;
-                continue;
-            }
             if (c->h_ctr->nx_ovf) {
-                if ((rc = grow_sspill(c, nset, c->sspill_cap[nset] * 4))) return rc;
+                for (int side = 0; side < 2; ++side)
+                    if ((rc = grow_bspill(c, nset, side, c->bspill_cap[nset][side] * 4))) return rc;
                 continue;
             }
             const uint64_t ovf = c->h_ctr->pool_ovf;
@@ -1540,7 +1431,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             c->pool_prev[i] = v;
         }
         const DevCounters &k = *c->h_ctr;
-        const uint64_t Sn = k.stotal[nset], Pn = sum_subs(k.lin[nxt][1]);
+        const uint64_t Sn = k.btotal[nset][0], Pn = k.btotal[nset][1];
         rs.edges = E;
         rs.s_after = Sn;
         rs.p_after = Pn;
@@ -1553,10 +1444,10 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         size_t nrec0 = rec_edges.size();
         if (framed) {
             FrameSide fs, fp;
-            const uint32_t nss = std::min(c->h_ctr->sspill[sset], c->sspill_cap[sset]);
-            const uint32_t nps = std::min(c->h_ctr->spill[1], c->spill_cap);
+            const uint32_t nss = std::min(k.bspill[sset][0], sin.spill_cap);
+            const uint32_t nps = std::min(k.bspill[sset][1], pin.spill_cap);
             if ((rc = frame_side(c, fs, sin.fine, sin.cnt, sin.spill, nss, nbt, St)) ||
-                (rc = frame_side(c, fp, c->d_fine[1], c->d_fine_cnt[1], c->d_spill[1], nps, nbt, Pt)) ||
+                (rc = frame_side(c, fp, pin.fine, pin.cnt, pin.spill, nps, nbt, Pt)) ||
                 (rc = frame_round(c, bb, fs, fp, c->d_edge_cnt + slot0, E, M, rec_edges)))
                 return rc;
         } else {
@@ -1566,8 +1457,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
             const int ell = L - 1 - t;
             if ((rc = audit_list(c, 1, ell, St, sin.fine, sin.cnt, sin.spill, sin.spill_n, sin.spill_cap, nbt, t + 1,
                                  ell)) ||
-                (rc = audit_list(c, 2, ell, Pt, c->d_fine[1], c->d_fine_cnt[1], c->d_spill[1], &c->d_ctr->spill[1],
-                                 c->spill_cap, nbt, 0, ell)))
+                (rc = audit_list(c, 2, ell, Pt, pin.fine, pin.cnt, pin.spill, pin.spill_n, pin.spill_cap, nbt, 0, ell)))
                 return rc;
         }
         for (size_t i = nrec0; i < rec_edges.size(); ++i) {
@@ -1583,7 +1473,6 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         c->rounds.push_back(rs);
         St = Sn;
         Pt = Pn;
-        cur = nxt;
         bbS[nset] = bbn;
     }
     S.n_rounds = t;
