@@ -144,7 +144,8 @@ struct DedupArgs {
 
 struct JoinArgs {
     const Entry *s_in, *p_in;
-    const uint32_t *s_cnt, *p_cnt;
+    uint32_t *s_cnt, *p_cnt;
+    int32_t zero_in;         // clear the input counts once read (no later launch reads them)
     uint32_t nb;             // buckets
     NextOut nx;              // survivors -> the next round's buckets (nx.active: a next round exists)
     uint32_t *pool_from, *pool_to;   // edge pool: SUBS sub-ranges of pool_cap edges, then the
@@ -235,6 +236,9 @@ hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
+// out[t] = sum of cnt[t * stride, (t + 1) * stride) for t < rounds.
+hipError_t launch_round_sums(const uint32_t *cnt, uint64_t stride, uint32_t rounds, unsigned long long *out,
+                             hipStream_t s);
 // n work-groups: all buckets (a.ids null; buckets over a.cap are skipped) or the listed ones.
 hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,

@@ -99,14 +99,14 @@ __device__ __forceinline__ void next_put(const NextOut &nx, uint32_t side, uint3
 // none), its slot (5 bits) and its rank among the wave's entries of the slot (9 bits).
 __device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
 
-// Count step for register-resident entries e[q] (bit q of mask) of side `side` from source bucket
-// b; row: this wave's NSLOT counters (zeroed), ntot: the work-group's per-side totals.
+// Gather step for register-resident entries e[q] (bit q of mask) of side `side`: tgt[q] = the
+// column-group word of entries that enter a group this round (one round in four), else 0x100.
+// Rounds where no lane of the wave needs a word skip the loads; otherwise all the wave's gathers
+// are in flight together (lanes without one read the store's first word, a cached line). The
+// caller may issue several sides' gathers before it waits for any.
 template <int PER>
-__device__ __forceinline__ void next_count(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot,
-                                           const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
-    // An entry gathers when it enters a column group (one round in four): rounds where no lane
-    // of the wave needs a word skip the loads; otherwise all of the wave's gathers are in flight
-    // together (lanes without one read the store's first word, a cached line).
+__device__ __forceinline__ void next_gather(const NextOut &nx, const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER],
+                                            uint32_t side) {
     const bool sfx = side == 0;
     bool need[PER];
     uint32_t grp[PER], any = 0;
@@ -124,6 +124,16 @@ __device__ __forceinline__ void next_count(const NextOut &nx, uint32_t b, uint32
             if (need[q]) tgt[q] = w;
         }
     }
+}
+
+// Rank step (after next_gather): regenerates each entry, finds its slot for source bucket b and
+// ranks it among the wave's entries of the slot; row: this wave's NSLOT counters (zeroed), ntot:
+// the work-group's per-side totals.
+template <int PER>
+__device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot,
+                                          const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
+    const bool sfx = side == 0;
+    uint32_t wn = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const bool act = (mask >> q) & 1u;
@@ -135,10 +145,8 @@ __device__ __forceinline__ void next_count(const NextOut &nx, uint32_t b, uint32
             slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, e[q].h1 & KEY_MASK, ev, h1n);
         }
         tgt[q] |= (slot << 9) | (wave_rank(row, act, slot) << 14);
+        wn += (uint32_t)__popcll(__ballot(act));
     }
-    uint32_t wn = 0;  // the wave's entries, added by one lane
-#pragma unroll
-    for (int q = 0; q < PER; ++q) wn += (uint32_t)__popcll(__ballot((mask >> q) & 1u));
     if (lane_id() == 0 && wn) atomicAdd(ntot + side, wn);
 }
 
@@ -408,8 +416,10 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         uint32_t *row = wcnt[threadIdx.x >> 6];
         if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
         uint32_t ts[PER], tp[PER];
-        next_count<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
-        next_count<PER>(a.nx, b, row, ntot, kept, keepmask, tp, 1);
+        next_gather<PER>(a.nx, kept, keepmask, ts, 0);
+        next_gather<PER>(a.nx, kept, keepmask, tp, 1);
+        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
+        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, tp, 1);
         lds_barrier();
         if (threadIdx.x < NSLOT) {
             slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64>(a.nx, b, wcnt, threadIdx.x));
@@ -448,11 +458,11 @@ __device__ __forceinline__ void load_bucket(const JoinArgs &a, uint32_t b, uint3
     }
 }
 
-// Prefix survivors of the join (in LDS, slot threadIdx.x + NT * q for bit q of pmask): the count
-// and write steps of next_count / next_write with the entries read from LDS.
+// Prefix survivors of the join (in LDS, slot threadIdx.x + NT * q for bit q of pmask): the gather
+// and rank steps of next_gather / next_rank with the entries read from LDS.
 template <int CAP, int MCAP, int NT>
-__device__ __forceinline__ void pfx_count(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t *row,
-                                          uint32_t pmask, uint32_t (&tgt)[CAP / NT]) {
+__device__ __forceinline__ void pfx_gather(const NextOut &nx, const JoinShared<CAP, MCAP, NT> &S, uint32_t pmask,
+                                           uint32_t (&tgt)[CAP / NT]) {
     constexpr int PER = CAP / NT;
     bool need[PER];
     uint32_t grp[PER], any = 0;
@@ -471,6 +481,12 @@ __device__ __forceinline__ void pfx_count(const NextOut &nx, JoinShared<CAP, MCA
             if (need[q]) tgt[q] = w;
         }
     }
+}
+
+template <int CAP, int MCAP, int NT>
+__device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t *row,
+                                         uint32_t pmask, uint32_t (&tgt)[CAP / NT]) {
+    constexpr int PER = CAP / NT;
     uint32_t wn = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -597,8 +613,10 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     uint32_t *row = S.u.m.wcnt[threadIdx.x >> 6];
     if (a.nx.active) {
         if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
-        next_count<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0);
-        pfx_count<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg);
+        next_gather<PER>(a.nx, se, smask, tgt, 0);  // both sides' gathers in flight together
+        pfx_gather<CAP, MCAP, NT>(a.nx, S, pmask, ptg);
+        next_rank<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0);
+        pfx_rank<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg);
     } else {  // last round: only counted
         if (smask) atomicAdd(&S.ntot[0], (uint32_t)__popc(smask));
         if (pmask) atomicAdd(&S.ntot[1], (uint32_t)__popc(pmask));
@@ -728,6 +746,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     if (b >= nb) return;
     if (!PERSIST) {
         const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+        // the counts of this bucket are read only here: clear them for the launch that next
+        // writes this bucket set
+        if (a.zero_in && threadIdx.x == 0) { a.s_cnt[b] = 0; a.p_cnt[b] = 0; }
         Entry se[PER], pe[PER];
         load_bucket<CAP, NT>(a, b, ns, np, se, pe);
         stage_prefixes<CAP, MCAP, NT>(S, np, pe);
@@ -1001,6 +1022,18 @@ __global__ void __launch_bounds__(1024) k_escan_top(unsigned long long *__restri
     }
 }
 
+// Edges per round: the sum of round t's `stride` bucket slots (one work-group per round).
+__global__ void __launch_bounds__(1024) k_round_sums(const uint32_t *__restrict__ cnt, uint64_t stride,
+                                                    unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long sh[16];
+    const uint32_t *c = cnt + (uint64_t)blockIdx.x * stride;
+    unsigned long long v = 0;
+    for (uint64_t i = threadIdx.x; i < stride; i += 1024) v += c[i];
+    unsigned long long tot;
+    block_excl_scan64<1024>(v, sh, tot);
+    if (threadIdx.x == 0) out[blockIdx.x] = tot;
+}
+
 __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
     const uint64_t b = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= a.n) return;
@@ -1104,6 +1137,13 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint64_t g = (n + 255) / 256;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)(g < 4096 ? g : 4096)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_sums(const uint32_t *cnt, uint64_t stride, uint32_t rounds, unsigned long long *out,
+                             hipStream_t s) {
+    if (rounds == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_round_sums, dim3(rounds), dim3(1024), 0, s, cnt, stride, out);
     return hipGetLastError();
 }
 

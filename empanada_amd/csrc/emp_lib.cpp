@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <time.h>
 
@@ -30,15 +31,26 @@ namespace {
 constexpr uint32_t STAGE_READS = 1u << 20;  // reads per host->device batch (multiple of 16)
 constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large-bucket path
 
+// Counters of one round's lists, filled by the launch that writes the round's bucket set (the
+// dedup for round 0, round t-1's join for round t).
+struct ListCtr {
+    uint32_t total[2];  // entries per side
+    uint32_t spill[2];  // records past a bucket's capacity, per side
+    uint32_t n_big;     // buckets the writing launch handed to the large-bucket path
+    uint32_t nx_ovf;    // a spill area overflowed
+    uint32_t pad[2];
+};
+constexpr int MAX_ROUNDS = 256;  // L - o <= 255
+
 struct DevCounters {
     unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
     unsigned long long pool_ovf;    // cursor of the pool's overflow region
-    uint32_t n_big, abort;
+    uint32_t abort;                 // the dedup partition overflowed an intermediate bucket or its spill
     uint32_t spill;                 // records spilled by the dedup partition's final pass
     uint32_t in;                    // dedup partition input size, set by the host
-    uint32_t bspill[2][2];          // [set][side] records spilled into bucket set `set`
-    uint32_t btotal[2][2];          // [set][side] entries written into bucket set `set`
-    uint32_t nx_ovf, pad;           // a bucket set's spill area overflowed
+    uint32_t pad;
+    ListCtr r[MAX_ROUNDS + 1];      // r[t]: round t's lists
+    unsigned long long redges[MAX_ROUNDS];  // edges per round (pipelined rounds, at the end)
 };
 
 double now_ms() {
@@ -159,6 +171,10 @@ struct emp_ctx {
     uint32_t *d_brank[2] = {nullptr, nullptr}, *d_baux[2] = {nullptr, nullptr};
     uint64_t bscratch_cap = 0;
     DevCounters *d_ctr = nullptr, *h_ctr = nullptr;
+    // pipelined rounds: each round's counter record read back (pinned) with an event per round
+    bool pipeline = true;
+    ListCtr *h_rec = nullptr;
+    std::vector<hipEvent_t> rec_ev;
 
     // statistics
     std::vector<emp_round_stats> rounds;
@@ -767,14 +783,12 @@ void zero_partition(emp_ctx *c, ZeroList &z, uint64_t x, int bb) {
     z.add(&c->d_ctr->in, 1, (uint32_t)x);
 }
 
-// What a launch writing bucket set `set` (2^bb buckets per side) accumulates, and its flags.
-void zero_next(emp_ctx *c, ZeroList &z, int set, int bb) {
-    z.add(c->d_bcnt[set][0], 1ull << bb);
-    z.add(c->d_bcnt[set][1], 1ull << bb);
-    z.add(&c->d_ctr->bspill[set][0], 2);
-    z.add(&c->d_ctr->btotal[set][0], 2);
-    z.add(&c->d_ctr->nx_ovf, 1);
-    z.add(&c->d_ctr->n_big, 1);
+// What a launch writing round t's bucket set (2^bb buckets per side) accumulates: its counts and
+// the round's counter record.
+void zero_next(emp_ctx *c, ZeroList &z, int t, int bb) {
+    z.add(c->d_bcnt[t & 1][0], 1ull << bb);
+    z.add(c->d_bcnt[t & 1][1], 1ull << bb);
+    z.add(reinterpret_cast<uint32_t *>(&c->d_ctr->r[t]), sizeof(ListCtr) / 4);
 }
 
 int run_zero(emp_ctx *c, const ZeroList &z) {
@@ -924,23 +938,25 @@ int ensure_big_scratch(emp_ctx *c, uint64_t need) {
     return EMP_OK;
 }
 
-// One side of a bucket set as the input of a launch: buckets, counts, spill area with its count.
+// One side of round t's lists as the input of a launch: buckets, counts, spill area with its count.
 struct SideIn {
     const Entry *fine;
-    const uint32_t *cnt;
+    uint32_t *cnt;
     const SpillRec *spill;
     const uint32_t *spill_n;
     uint32_t spill_cap;
 };
 
-SideIn side_in(emp_ctx *c, int set, int side) {
-    return SideIn{c->d_bfine[set][side], c->d_bcnt[set][side], c->d_bspill[set][side], &c->d_ctr->bspill[set][side],
+SideIn side_in(emp_ctx *c, int t, int side) {
+    const int set = t & 1;
+    return SideIn{c->d_bfine[set][side], c->d_bcnt[set][side], c->d_bspill[set][side], &c->d_ctr->r[t].spill[side],
                   c->bspill_cap[set][side]};
 }
 
-// Bucket set `set` as the output of a launch over 2^src_bb source buckets: 2^bb buckets per side,
-// for round rg (inactive: none).
-NextOut next_out(emp_ctx *c, int set, int bb, int src_bb, const RoundArgs &rg) {
+// Round t's bucket set as the output of a launch over 2^src_bb source buckets: 2^bb buckets per
+// side, regenerated for round rg (inactive: none, survivors only counted).
+NextOut next_out(emp_ctx *c, int t, int bb, int src_bb, const RoundArgs &rg) {
+    const int set = t & 1;
     NextOut o{};
     o.active = rg.active;
     o.shift = 29 - bb;
@@ -950,11 +966,11 @@ NextOut next_out(emp_ctx *c, int set, int bb, int src_bb, const RoundArgs &rg) {
         o.fine[side] = c->d_bfine[set][side];
         o.cnt[side] = c->d_bcnt[set][side];
         o.spill[side] = c->d_bspill[set][side];
-        o.spill_n[side] = &c->d_ctr->bspill[set][side];
+        o.spill_n[side] = &c->d_ctr->r[t].spill[side];
         o.spill_cap[side] = c->bspill_cap[set][side];
-        o.total[side] = &c->d_ctr->btotal[set][side];
+        o.total[side] = &c->d_ctr->r[t].total[side];
     }
-    o.ovf = &c->d_ctr->nx_ovf;
+    o.ovf = &c->d_ctr->r[t].nx_ovf;
     o.cols = c->d_cols;
     return o;
 }
@@ -1178,9 +1194,8 @@ int audit_list(emp_ctx *c, int list, int ovlp, uint64_t entries, const Entry *fi
 // Buckets the LDS kernels handed over (more than BUCKET_CAP entries or MATCH_CAP matches):
 // global-memory sort + join (or dedup), one work-group per bucket, over the inputs `sin` / `pin`
 // (dedup: sin only); survivors (kept entries) go to nx.
-int big_path(emp_ctx *c, int bb, bool dedup, const RoundArgs &rg, uint64_t slot0, const SideIn &sin,
+int big_path(emp_ctx *c, uint32_t nbig, int bb, bool dedup, const RoundArgs &rg, uint64_t slot0, const SideIn &sin,
              const SideIn &pin, const NextOut &nx) {
-    const uint32_t nbig = c->h_ctr->n_big;
     if (nbig > BIG_CAP) return fail(c, EMP_ERR_CAPACITY, "%u buckets over capacity (limit %u)", nbig, BIG_CAP);
     const uint32_t nb = 1u << bb;
     std::vector<uint32_t> ids(nbig), cs(nb), cp(nb);
@@ -1249,6 +1264,342 @@ int big_path(emp_ctx *c, int bb, bool dedup, const RoundArgs &rg, uint64_t slot0
     return read_counters(c);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The overlap phase: dedup, then the rounds (synchronous or pipelined), then the output.
+
+constexpr int RUN_FALLBACK = 1;  // pipelined rounds met an exceptional round: rerun synchronously
+
+struct RunState {
+    int rounds = 0, L = 0, bbD = 0;
+    uint64_t M = 0, stride = 0, D = 0;
+    int bbS[2] = {0, 0};  // bucket bits of the bucket set written for rounds of each parity
+    int t = 0;            // rounds done
+    std::vector<uint64_t> round_edges, rec_edges;  // edges per round; per record
+    std::vector<int> rec_round;                      // round of each record
+};
+
+// Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key. The kept
+// entries (full-length keys), regenerated for round 0, go into bucket set 0 as round 0's suffix
+// and prefix lists (overlap.cu:235-242). Round t's lists live in bucket set t % 2.
+int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
+    int rc;
+    const int bbD = st.bbD, L = st.L;
+    c->records.clear();
+    c->out_used = 0;
+    c->rounds.clear();
+    c->audits.clear();
+    st.round_edges.clear();
+    st.rec_edges.clear();
+    st.rec_round.clear();
+    st.t = 0;
+    st.bbS[0] = st.bbS[1] = bbD;
+    if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
+        return rc;
+    c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
+    if (st.rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)st.rounds * st.stride))) return rc;
+    const RoundArgs none{};
+    const RoundArgs rg0 = st.rounds > 0 ? round_args(c, 0, true) : none;
+    bool need_part = true;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
+        {
+            ZeroList z;
+            if (need_part) zero_partition(c, z, N0, bbD);
+            zero_next(c, z, 0, bbD);
+            z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors
+            if ((rc = run_zero(c, z))) return rc;
+        }
+        if (need_part && (rc = partition(c, N0, bbD))) return rc;
+        DedupArgs a{};
+        a.in = c->d_dfine;
+        a.cnt = c->d_dcnt;
+        a.nx = next_out(c, 0, bbD, bbD, rg0);
+        a.big = c->d_big;
+        a.n_big = &c->d_ctr->r[0].n_big;
+        a.big_cap = BIG_CAP;
+        a.abort = &c->d_ctr->abort;
+        {
+            Timed t(c, "k_dedup", 12.0 * N0);
+            HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
+        }
+        if ((rc = read_counters(c))) return rc;
+        if (c->h_ctr->abort) {
+            if ((rc = grow_after_abort(c, N0, bbD))) return rc;
+            need_part = true;
+            continue;
+        }
+        need_part = false;
+        const ListCtr &r0 = c->h_ctr->r[0];
+        const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
+        if (!r0.nx_ovf && r0.n_big &&
+            (rc = big_path(c, r0.n_big, bbD, true, none, 0, din, din, next_out(c, 0, bbD, bbD, rg0))))
+            return rc;
+        if (c->h_ctr->r[0].nx_ovf) {
+            for (int side = 0; side < 2; ++side)
+                if ((rc = grow_bspill(c, 0, side, c->bspill_cap[0][side] * 4))) return rc;
+            continue;
+        }
+        break;
+    }
+    // the 2n entries before dedup, in the dedup partition's buckets (overlap.cu:215-219)
+    if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill,
+                                     c->dspill_cap, 1u << bbD, 0, L)))
+        return rc;
+    st.D = c->h_ctr->r[0].total[0];
+    return EMP_OK;
+}
+
+// Rounds with the host between them: after every join the counters come back, so that a large
+// bucket, a spill or pool overflow is handled (and the round rerun) at once, and framing and the
+// audit can read the round's lists.
+int rounds_sync(emp_ctx *c, RunState &st) {
+    int rc;
+    const int L = st.L, rounds = st.rounds;
+    const uint64_t M = st.M, stride = st.stride;
+    const RoundArgs none{};
+    uint64_t St = c->h_ctr->r[0].total[0], Pt = c->h_ctr->r[0].total[1];
+    int t = 0;
+    for (; t < rounds && St > 0; ++t) {
+        // No record can follow an empty prefix list. (The lists mirror each other under reverse
+        // complement, so they empty together; the audit's records end here too.)
+        if (Pt == 0) break;
+        const bool framed = St >= M / 2 || Pt >= M / 2;  // several records (frame_round)
+        const int sset = t & 1, nset = sset ^ 1;
+        const int bb = st.bbS[sset];
+        const uint32_t nbt = 1u << bb;
+        const bool more = t + 1 < rounds;
+        // the next round's buckets, chosen now because this round's join writes them
+        const int bbn = more ? std::min(bb, bucket_bits(std::max(St, Pt))) : bb;
+        const RoundArgs rg = round_args(c, t, true);
+        const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
+        emp_round_stats rs{};
+        rs.s = St;
+        rs.p = Pt;
+        rs.ovlp = L - 1 - t;
+        rs.bucket_bits = bb;
+        const uint64_t slot0 = (uint64_t)t * stride;
+        const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
+        for (int attempt = 0;; ++attempt) {
+            if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
+            if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
+                HIPCHK(c, hipMemcpyAsync(c->d_ctr->pool, c->pool_prev.data(), (SUBS + 1) * sizeof(unsigned long long),
+                                         hipMemcpyHostToDevice, c->stream));
+            {
+                ZeroList z;
+                zero_next(c, z, t + 1, bbn);
+                if ((rc = run_zero(c, z))) return rc;
+            }
+            JoinArgs a{};
+            a.s_in = sin.fine;
+            a.p_in = pin.fine;
+            a.s_cnt = sin.cnt;
+            a.p_cnt = pin.cnt;
+            a.zero_in = 0;
+            a.nx = next_out(c, t + 1, bbn, bb, rgn);
+            a.pool_from = c->d_pool_from;
+            a.pool_to = c->d_pool_to;
+            a.pool_cursor = c->d_ctr->pool;
+            a.pool_cap = c->pool_cap;
+            a.ovf_cursor = &c->d_ctr->pool_ovf;
+            a.ovf_cap = c->pool_ovf_cap;
+            a.edge_off = c->d_edge_off + slot0;
+            a.edge_cnt = c->d_edge_cnt + slot0;
+            a.big = c->d_big;
+            a.n_big = &c->d_ctr->r[t + 1].n_big;
+            a.big_cap = BIG_CAP;
+            a.abort = &c->d_ctr->abort;
+            {
+                Timed tm(c, "k_join", 0.0);
+                HIPCHK(c, launch_join(a, nbt, c->stream));
+            }
+            if ((rc = read_counters(c))) return rc;
+            const ListCtr &rn = c->h_ctr->r[t + 1];
+            rs.big_buckets = (int32_t)rn.n_big;
+            if (!rn.nx_ovf && rn.n_big &&
+                (rc = big_path(c, rn.n_big, bb, false, rg, slot0, sin, pin, next_out(c, t + 1, bbn, bb, rgn))))
+                return rc;
+            if (c->h_ctr->r[t + 1].nx_ovf) {
+                for (int side = 0; side < 2; ++side)
+                    if ((rc = grow_bspill(c, nset, side, c->bspill_cap[nset][side] * 4))) return rc;
+                continue;
+            }
+            const uint64_t ovf = c->h_ctr->pool_ovf;
+            if (ovf > c->pool_ovf_cap) {
+                if ((rc = ensure_pool(c, 0, std::max<uint64_t>(2 * c->pool_ovf_cap, ovf + ovf / 4 + 1024), true)))
+                    return rc;
+                continue;
+            }
+            break;
+        }
+        uint64_t E = 0;
+        for (int i = 0; i <= SUBS; ++i) {
+            const unsigned long long v = i < SUBS ? c->h_ctr->pool[i] : c->h_ctr->pool_ovf;
+            E += v - c->pool_prev[i];
+            c->pool_prev[i] = v;
+        }
+        const DevCounters &k = *c->h_ctr;
+        const uint64_t Sn = k.r[t + 1].total[0], Pn = k.r[t + 1].total[1];
+        rs.edges = E;
+        rs.s_after = Sn;
+        rs.p_after = Pn;
+        if (c->opts.profile) {
+            int id = kstat_id(c, "k_join");
+            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
+        }
+        // one record per partition with both sides non-empty (overlap.cu:398-422; written even when
+        // E == 0): one per round unless a list reaches M/2
+        size_t nrec0 = st.rec_edges.size();
+        if (framed) {
+            FrameSide fs, fp;
+            const uint32_t nss = std::min(k.r[t].spill[0], sin.spill_cap);
+            const uint32_t nps = std::min(k.r[t].spill[1], pin.spill_cap);
+            if ((rc = frame_side(c, fs, sin.fine, sin.cnt, sin.spill, nss, nbt, St)) ||
+                (rc = frame_side(c, fp, pin.fine, pin.cnt, pin.spill, nps, nbt, Pt)) ||
+                (rc = frame_round(c, bb, fs, fp, c->d_edge_cnt + slot0, E, M, st.rec_edges)))
+                return rc;
+        } else {
+            st.rec_edges.push_back(E);
+        }
+        if (c->audit) {  // the lists as the round joins them (overlap.cu:274-284)
+            const int ell = L - 1 - t;
+            if ((rc = audit_list(c, 1, ell, St, sin.fine, sin.cnt, sin.spill, sin.spill_n, sin.spill_cap, nbt, t + 1,
+                                 ell)) ||
+                (rc = audit_list(c, 2, ell, Pt, pin.fine, pin.cnt, pin.spill, pin.spill_n, pin.spill_cap, nbt, 0, ell)))
+                return rc;
+        }
+        for (size_t i = nrec0; i < st.rec_edges.size(); ++i) {
+            if (st.rec_edges[i] >= (1ull << 31))  // the reference's int edge offsets (overlap.cu:561,567)
+                return fail(c, EMP_ERR_PARTITION, "round %d: %llu edges in one record reach 2^31", t,
+                            (unsigned long long)st.rec_edges[i]);
+            st.rec_round.push_back(t);
+        }
+        st.round_edges.push_back(E);
+        rs.records = (int32_t)(st.rec_edges.size() - nrec0);
+        c->rounds.push_back(rs);
+        St = Sn;
+        Pt = Pn;
+        st.bbS[nset] = bbn;
+    }
+    st.t = t;
+    return EMP_OK;
+}
+
+// Rounds back to back: no host synchronisation between joins. Each join clears the bucket counts
+// it has read (the next writer of that set needs them zero) and fills round t+1's counter record;
+// the host reads each record back one round late, only to size the buckets of the round after
+// next (from an upper bound of the list sizes: lists only shrink) and to stop at an empty list.
+// Any large bucket, spill or pool overflow returns RUN_FALLBACK: the rounds after it read
+// incomplete lists, so the run restarts from the dedup on the synchronous path.
+int rounds_pipelined(emp_ctx *c, RunState &st) {
+    int rc;
+    const int L = st.L, rounds = st.rounds;
+    const uint64_t stride = st.stride;
+    const RoundArgs none{};
+    // round 1's bucket set (set 1) and every round's counter record start at zero; set 0 was
+    // cleared before the dedup wrote it
+    {
+        ZeroList z;
+        z.add(c->d_bcnt[1][0], 1ull << st.bbD);
+        z.add(c->d_bcnt[1][1], 1ull << st.bbD);
+        z.add(reinterpret_cast<uint32_t *>(&c->d_ctr->r[1]), (uint64_t)rounds * sizeof(ListCtr) / 4);
+        if ((rc = run_zero(c, z))) return rc;
+    }
+    if (!c->h_rec) {
+        void *p = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, (MAX_ROUNDS + 1) * sizeof(ListCtr)));
+        c->h_rec = (ListCtr *)p;
+    }
+    if (c->rec_ev.empty()) {
+        c->rec_ev.resize(MAX_ROUNDS);
+        for (auto &e : c->rec_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    std::vector<int> bits(rounds + 1, st.bbD);  // bucket bits of round t's lists
+    c->h_rec[0] = c->h_ctr->r[0];
+    uint64_t Sk = c->h_rec[0].total[0], Pk = c->h_rec[0].total[1];  // newest sizes known
+    int launched = 0;
+    for (int t = 0; t < rounds; ++t) {
+        // the record of round t-1 (written by join t-2) is complete: check it, size round t+1
+        if (t >= 2) {
+            HIPCHK(c, hipEventSynchronize(c->rec_ev[t - 2]));
+            const ListCtr &r = c->h_rec[t - 1];
+            if (r.n_big || r.nx_ovf) return RUN_FALLBACK;
+            Sk = r.total[0];
+            Pk = r.total[1];
+            if (Sk == 0 || Pk == 0) break;  // round t-1 writes no record, nor any later round
+        }
+        const bool more = t + 1 < rounds;
+        const int bb = bits[t];
+        bits[t + 1] = more ? std::min(bb, bucket_bits(std::max(Sk, Pk))) : bb;
+        const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
+        const uint64_t slot0 = (uint64_t)t * stride;
+        const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
+        JoinArgs a{};
+        a.s_in = sin.fine;
+        a.p_in = pin.fine;
+        a.s_cnt = sin.cnt;
+        a.p_cnt = pin.cnt;
+        a.zero_in = 1;
+        a.nx = next_out(c, t + 1, bits[t + 1], bb, rgn);
+        a.pool_from = c->d_pool_from;
+        a.pool_to = c->d_pool_to;
+        a.pool_cursor = c->d_ctr->pool;
+        a.pool_cap = c->pool_cap;
+        a.ovf_cursor = &c->d_ctr->pool_ovf;
+        a.ovf_cap = c->pool_ovf_cap;
+        a.edge_off = c->d_edge_off + slot0;
+        a.edge_cnt = c->d_edge_cnt + slot0;
+        a.big = c->d_big;
+        a.n_big = &c->d_ctr->r[t + 1].n_big;
+        a.big_cap = BIG_CAP;
+        a.abort = &c->d_ctr->abort;
+        {
+            Timed tm(c, "k_join", 0.0);
+            HIPCHK(c, launch_join(a, 1u << bb, c->stream));
+        }
+        HIPCHK(c, hipMemcpyAsync(&c->h_rec[t + 1], &c->d_ctr->r[t + 1], sizeof(ListCtr), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipEventRecord(c->rec_ev[t], c->stream));
+        launched = t + 1;
+    }
+    // every launched round's record, the pool cursors and the edges per round
+    HIPCHK(c, launch_round_sums(c->d_edge_cnt, stride, (uint32_t)launched, c->d_ctr->redges, c->stream));
+    if ((rc = read_counters(c))) return rc;
+    const DevCounters &k = *c->h_ctr;
+    if (k.pool_ovf > c->pool_ovf_cap) return RUN_FALLBACK;
+    int t = 0;
+    for (; t < launched; ++t) {
+        const ListCtr &r = k.r[t + 1];
+        if (r.n_big || r.nx_ovf) return RUN_FALLBACK;
+    }
+    for (t = 0; t < launched; ++t) {
+        const uint64_t St = k.r[t].total[0], Pt = k.r[t].total[1];
+        if (St == 0 || Pt == 0) break;
+        emp_round_stats rs{};
+        rs.s = St;
+        rs.p = Pt;
+        rs.s_after = k.r[t + 1].total[0];
+        rs.p_after = k.r[t + 1].total[1];
+        rs.edges = k.redges[t];
+        rs.ovlp = L - 1 - t;
+        rs.records = 1;
+        rs.bucket_bits = bits[t];
+        if (c->opts.profile) {
+            int id = kstat_id(c, "k_join");
+            c->kstats[id].bytes += 12.0 * (St + Pt) + (t + 1 < rounds ? 12.0 * (rs.s_after + rs.p_after) : 0.0) +
+                                   8.0 * rs.edges;
+        }
+        if (rs.edges >= (1ull << 31))  // the reference's int edge offsets (overlap.cu:561,567)
+            return fail(c, EMP_ERR_PARTITION, "round %d: %llu edges in one record reach 2^31", t,
+                        (unsigned long long)rs.edges);
+        st.rec_edges.push_back(rs.edges);
+        st.rec_round.push_back(t);
+        st.round_edges.push_back(rs.edges);
+        c->rounds.push_back(rs);
+    }
+    st.t = t;
+    return EMP_OK;
+}
+
 int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     int rc = flush_acc(c);
     if (rc) return rc;
@@ -1284,214 +1635,53 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
 
     const int bbD = bucket_bits(N0);
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
-    if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
-        return rc;
-    c->records.clear();
-    c->out_used = 0;
-    c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
-
-    // Dedup (overlap.cu:212-225): partition all 2n entries, keep the smallest id per key. The kept
-    // entries (full-length keys), regenerated for round 0, go into bucket set 0 as round 0's
-    // suffix and prefix lists (overlap.cu:235-242). Bucket set t % 2 holds round t's lists;
-    // bbS[set] is the bucket bits it was written with.
-    const RoundArgs none{};
-    const RoundArgs rg0 = rounds > 0 ? round_args(c, 0, true) : none;
-    int bbS[2] = {bbD, bbD};
-    bool need_part = true;
-    for (int attempt = 0;; ++attempt) {
-        if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
-        {
-            ZeroList z;
-            if (need_part) zero_partition(c, z, N0, bbD);
-            zero_next(c, z, 0, bbD);
-            if (attempt == 0) z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors, once per run
-            if ((rc = run_zero(c, z))) return rc;
+    const uint64_t stride = 1ull << bbD;  // edge slots per round
+    RunState st{};
+    st.rounds = rounds;
+    st.L = L;
+    st.M = M;
+    st.bbD = bbD;
+    st.stride = stride;
+    // Rounds run back to back when nothing needs the host between them: no audit and no
+    // framing (lists below M/2; they only shrink). An exceptional round (a bucket over capacity,
+    // a spill or pool overflow) sends the run to the synchronous rounds, from the dedup on.
+    bool pipelined = !c->audit && !JOIN_PERSIST && c->pipeline;
+    for (int pass = 0;; ++pass) {
+        if ((rc = dedup_phase(c, st, N0))) return rc;
+        S.n_dedup = st.D;
+        if (pipelined && st.D < M / 2 && st.D > 0 && rounds > 0) {
+            rc = rounds_pipelined(c, st);
+            if (rc == RUN_FALLBACK && pass == 0) {
+                pipelined = false;
+                continue;
+            }
+        } else {
+            rc = rounds_sync(c, st);
         }
-        if (need_part && (rc = partition(c, N0, bbD))) return rc;
-        DedupArgs a{};
-        a.in = c->d_dfine;
-        a.cnt = c->d_dcnt;
-        a.nx = next_out(c, 0, bbD, bbD, rg0);
-        a.big = c->d_big;
-        a.n_big = &c->d_ctr->n_big;
-        a.big_cap = BIG_CAP;
-        a.abort = &c->d_ctr->abort;
-        {
-            Timed t(c, "k_dedup", 12.0 * N0);
-            HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
-        }
-        if ((rc = read_counters(c))) return rc;
-        if (c->h_ctr->abort) {
-            if ((rc = grow_after_abort(c, N0, bbD))) return rc;
-            need_part = true;
-            continue;
-        }
-        need_part = false;
-        const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
-        if (!c->h_ctr->nx_ovf && c->h_ctr->n_big &&
-            (rc = big_path(c, bbD, true, none, 0, din, din, next_out(c, 0, bbD, bbD, rg0))))
-            return rc;
-        if (c->h_ctr->nx_ovf) {
-            for (int side = 0; side < 2; ++side)
-                if ((rc = grow_bspill(c, 0, side, c->bspill_cap[0][side] * 4))) return rc;
-            continue;
-        }
+        if (rc) return rc;
         break;
     }
-    // the 2n entries before dedup, in the dedup partition's buckets (overlap.cu:215-219)
-    if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill,
-                                     c->dspill_cap, 1u << bbD, 0, L)))
-        return rc;
-    const uint64_t D = c->h_ctr->btotal[0][0];
-    uint64_t St = D, Pt = c->h_ctr->btotal[0][1];
-    S.n_dedup = D;
-
-    // Edge slots: round t owns slots [t * stride, t * stride + 2^bb); bucket counts only shrink.
-    const uint64_t stride = 1ull << bbD;
-    if (St > 0 && rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)rounds * stride))) return rc;
-    std::vector<uint64_t> round_edges, rec_edges;  // edges per round; per record
-    std::vector<int> rec_round;                      // round of each record
-
-    int t = 0;
-    for (; t < rounds && St > 0; ++t) {
-        // No record can follow an empty prefix list. (The lists mirror each other under reverse
-        // complement, so they empty together; the audit's records end here too.)
-        if (Pt == 0) break;
-        const bool framed = St >= M / 2 || Pt >= M / 2;  // several records (frame_round)
-        const int sset = t & 1, nset = sset ^ 1;
-        const int bb = bbS[sset];
-        const uint32_t nbt = 1u << bb;
-        const bool more = t + 1 < rounds;
-        // the next round's buckets, chosen now because this round's join writes them
-        const int bbn = more ? std::min(bb, bucket_bits(std::max(St, Pt))) : bb;
-        const RoundArgs rg = round_args(c, t, true);
-        const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
-        emp_round_stats rs{};
-        rs.s = St;
-        rs.p = Pt;
-        rs.ovlp = L - 1 - t;
-        rs.bucket_bits = bb;
-        uint64_t E = 0;
-        const uint64_t slot0 = (uint64_t)t * stride;
-        const SideIn sin = side_in(c, sset, 0), pin = side_in(c, sset, 1);
-        for (int attempt = 0;; ++attempt) {
-            if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
-            if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
-                HIPCHK(c, hipMemcpyAsync(c->d_ctr->pool, c->pool_prev.data(), (SUBS + 1) * sizeof(unsigned long long),
-                                         hipMemcpyHostToDevice, c->stream));
-            {
-                ZeroList z;
-                zero_next(c, z, nset, bbn);
-                if ((rc = run_zero(c, z))) return rc;
-            }
-            JoinArgs a{};
-            a.s_in = sin.fine;
-            a.p_in = pin.fine;
-            a.s_cnt = sin.cnt;
-            a.p_cnt = pin.cnt;
-            a.nx = next_out(c, nset, bbn, bb, rgn);
-            a.pool_from = c->d_pool_from;
-            a.pool_to = c->d_pool_to;
-            a.pool_cursor = c->d_ctr->pool;
-            a.pool_cap = c->pool_cap;
-            a.ovf_cursor = &c->d_ctr->pool_ovf;
-            a.ovf_cap = c->pool_ovf_cap;
-            a.edge_off = c->d_edge_off + slot0;
-            a.edge_cnt = c->d_edge_cnt + slot0;
-            a.big = c->d_big;
-            a.n_big = &c->d_ctr->n_big;
-            a.big_cap = BIG_CAP;
-            a.abort = &c->d_ctr->abort;
-            {
-                Timed tm(c, "k_join", 0.0);
-                HIPCHK(c, launch_join(a, nbt, c->stream));
-            }
-            if ((rc = read_counters(c))) return rc;
-            rs.big_buckets = (int32_t)c->h_ctr->n_big;
-            if (!c->h_ctr->nx_ovf && c->h_ctr->n_big &&
-                (rc = big_path(c, bb, false, rg, slot0, sin, pin, next_out(c, nset, bbn, bb, rgn))))
-                return rc;
-            if (c->h_ctr->nx_ovf) {
-                for (int side = 0; side < 2; ++side)
-                    if ((rc = grow_bspill(c, nset, side, c->bspill_cap[nset][side] * 4))) return rc;
-                continue;
-            }
-            const uint64_t ovf = c->h_ctr->pool_ovf;
-            if (ovf > c->pool_ovf_cap) {
-                if ((rc = ensure_pool(c, 0, std::max<uint64_t>(2 * c->pool_ovf_cap, ovf + ovf / 4 + 1024), true)))
-                    return rc;
-                continue;
-            }
-            break;
-        }
-        E = 0;
-        for (int i = 0; i <= SUBS; ++i) {
-            const unsigned long long v = i < SUBS ? c->h_ctr->pool[i] : c->h_ctr->pool_ovf;
-            E += v - c->pool_prev[i];
-            c->pool_prev[i] = v;
-        }
-        const DevCounters &k = *c->h_ctr;
-        const uint64_t Sn = k.btotal[nset][0], Pn = k.btotal[nset][1];
-        rs.edges = E;
-        rs.s_after = Sn;
-        rs.p_after = Pn;
-        if (c->opts.profile) {
-            int id = kstat_id(c, "k_join");
-            c->kstats[id].bytes += 12.0 * (St + Pt) + (more ? 12.0 * (Sn + Pn) : 0.0) + 8.0 * E;
-        }
-        // one record per partition with both sides non-empty (overlap.cu:398-422; written even when
-        // E == 0): one per round unless a list reaches M/2
-        size_t nrec0 = rec_edges.size();
-        if (framed) {
-            FrameSide fs, fp;
-            const uint32_t nss = std::min(k.bspill[sset][0], sin.spill_cap);
-            const uint32_t nps = std::min(k.bspill[sset][1], pin.spill_cap);
-            if ((rc = frame_side(c, fs, sin.fine, sin.cnt, sin.spill, nss, nbt, St)) ||
-                (rc = frame_side(c, fp, pin.fine, pin.cnt, pin.spill, nps, nbt, Pt)) ||
-                (rc = frame_round(c, bb, fs, fp, c->d_edge_cnt + slot0, E, M, rec_edges)))
-                return rc;
-        } else {
-            rec_edges.push_back(E);
-        }
-        if (c->audit) {  // the lists as the round joins them (overlap.cu:274-284)
-            const int ell = L - 1 - t;
-            if ((rc = audit_list(c, 1, ell, St, sin.fine, sin.cnt, sin.spill, sin.spill_n, sin.spill_cap, nbt, t + 1,
-                                 ell)) ||
-                (rc = audit_list(c, 2, ell, Pt, pin.fine, pin.cnt, pin.spill, pin.spill_n, pin.spill_cap, nbt, 0, ell)))
-                return rc;
-        }
-        for (size_t i = nrec0; i < rec_edges.size(); ++i) {
-            if (rec_edges[i] >= (1ull << 31))  // the reference's int edge offsets (overlap.cu:561,567)
-                return fail(c, EMP_ERR_PARTITION, "round %d: %llu edges in one record reach 2^31", t,
-                            (unsigned long long)rec_edges[i]);
-            rec_round.push_back(t);
-        }
-        round_edges.push_back(E);
-        rs.records = (int32_t)(rec_edges.size() - nrec0);
-        S.n_records += rs.records;
-        S.n_edges += E;
-        c->rounds.push_back(rs);
-        St = Sn;
-        Pt = Pn;
-        bbS[nset] = bbn;
+    for (const emp_round_stats &rs : c->rounds) {
+        S.n_records += (uint64_t)rs.records;
+        S.n_edges += rs.edges;
     }
-    S.n_rounds = t;
+    S.n_rounds = st.t;
 
     // Output: compact the edges of every round from the pool into the arena in (round, bucket)
     // order, then hand the records to the sink.
     uint64_t total = 0;
-    for (uint64_t e : round_edges) total += e;
+    for (uint64_t e : st.round_edges) total += e;
     if ((rc = ensure_arena(c, std::max<uint64_t>(total, 1), sink, user))) return rc;
     if (total) {
-        EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->d_eblk, (uint64_t)t * stride,
+        EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->d_eblk, (uint64_t)st.t * stride,
                         c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
-        Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)t * stride);
+        Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)st.t * stride);
         HIPCHK(c, launch_edge_compact(ea, c->stream));
     }
     uint64_t off = 0;  // the arena holds the rounds' edges back to back; records slice them in order
-    for (size_t i = 0; i < rec_edges.size(); ++i) {
-        c->records.push_back(emp_ctx::Record{rec_edges[i], off, L - 1 - rec_round[i]});
-        off += rec_edges[i];
+    for (size_t i = 0; i < st.rec_edges.size(); ++i) {
+        c->records.push_back(emp_ctx::Record{st.rec_edges[i], off, L - 1 - st.rec_round[i]});
+        off += st.rec_edges[i];
     }
     c->out_used = total;
     return flush_records(c, sink, user);
@@ -1535,6 +1725,10 @@ int emp_ctx_create(const emp_opts *opts, emp_ctx **out) {
         return EMP_ERR_ARG;
     }
     c->device = c->opts.device;
+    // EMPANADA_SYNC_ROUNDS=1: the host between every two rounds (the path that large buckets,
+    // framing and the audit take anyway)
+    const char *sync_env = getenv("EMPANADA_SYNC_ROUNDS");
+    c->pipeline = !(sync_env && *sync_env && *sync_env != '0');
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) {
         delete c;
@@ -1582,6 +1776,8 @@ void emp_ctx_destroy(emp_ctx *c) {
     if (c->h_to) (void)hipHostFree(c->h_to);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
     if (c->h_fq_res) (void)hipHostFree(c->h_fq_res);
+    if (c->h_rec) (void)hipHostFree(c->h_rec);
+    for (auto e : c->rec_ev) (void)hipEventDestroy(e);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
