@@ -544,6 +544,9 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
 #pragma unroll
     for (int q = 0; q < PER; ++q) sc[q] = 0;
     lds_barrier();
+    // Every thread has read the bucket's counts (before this barrier) and nothing else reads
+    // them: with zero_in they are cleared for the launch that next writes this bucket set.
+    if (a.zero_in && threadIdx.x == 0) { a.s_cnt[b] = 0; a.p_cnt[b] = 0; }
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
@@ -746,9 +749,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     if (b >= nb) return;
     if (!PERSIST) {
         const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
-        // the counts of this bucket are read only here: clear them for the launch that next
-        // writes this bucket set
-        if (a.zero_in && threadIdx.x == 0) { a.s_cnt[b] = 0; a.p_cnt[b] = 0; }
         Entry se[PER], pe[PER];
         load_bucket<CAP, NT>(a, b, ns, np, se, pe);
         stage_prefixes<CAP, MCAP, NT>(S, np, pe);

@@ -139,6 +139,29 @@ def test_pipeline_generated_vs_oracle(G, n, L, o, p_rc, p_sub, seeds):
     assert s["n_dedup"] == osum["n_dedup"] and s["n_edges"] == osum["n_edges"]
 
 
+def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
+    """2,000,000 x 100 bp reads (4 M entries, 12-bit buckets): the back-to-back rounds (each join
+    clears the counts it read) and the rounds with the host between them give the oracle's bytes
+    and the same per-round counts."""
+    g = gen.genome(5 * 10**6, 21)
+    r = gen.reads(g, 2_000_000, 100, 0.5, 0.005, 22)
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), r.shape[0], 100, min_ovlp=40)
+    assert rc == 0
+    got = {}
+    for mode in ("pipelined", "sync"):
+        if mode == "sync":
+            monkeypatch.setenv("EMPANADA_SYNC_ROUNDS", "1")
+        else:
+            monkeypatch.delenv("EMPANADA_SYNC_ROUNDS", raising=False)
+        with native.Context(min_ovlp=40) as c:
+            c.add_reads(r)
+            got[mode] = (c.run(), c.rounds())
+    for mode, (out, rounds) in got.items():
+        assert out == want, mode
+        assert [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in rounds] == \
+               [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in orounds], mode
+
+
 def test_fastq_and_read_entry_points_agree_and_block_size_is_irrelevant():
     g = gen.genome(3 * 10**5, 21)
     r = gen.reads(g, 30000, 100, 0.5, 0.005, 22)
