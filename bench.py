@@ -142,7 +142,9 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=None,
                     help="fraction of the workload the oracle runs on (default 1.0 in-core, 0.01 ooc)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check", action="store_true", help="also compare the full-size output with the oracle")
+    ap.add_argument("--check", action="store_true",
+                    help="also compare the full-size output with the oracle (in-core), or the pipelined rounds' "
+                         "output with the synchronous rounds' (3.1 Gbp)")
     ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
     ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
     ap.add_argument("--served-loads", default="0.5,1,2,4,8",
@@ -257,7 +259,7 @@ def main():
             "expected_per_list_round_0": round(au[1]["entries"] ** 2 / 2 / 2.0**58, 4) if len(au) > 1 else None,
         }
 
-    same_full = None
+    same_full = same_sync = None
     if args.check and not ooc:
         import oracle_lib
         rc, want, _, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
@@ -269,6 +271,21 @@ def main():
     if not args.no_cpu:
         cpu, same_sample = cpu_baseline(args.workload, cpu_scale, min_ovlp, check_gpu=True)
     ctx.close()
+    if args.check and ooc:
+        # full size, where the oracle would take hours: the back-to-back rounds against the rounds
+        # with the host between them (counters read after every join), byte for byte; one
+        # context at a time (each holds about 110 GB)
+        upload()
+        outs = []
+        for sync in (False, True):
+            if sync:
+                os.environ["EMPANADA_SYNC_ROUNDS"] = "1"
+            with native.Context(min_ovlp=min_ovlp, reduce_mem=96 << 30) as cc:
+                cc.add_reads_device(dev["reads"].address, n, L)
+                outs.append(cc.run())
+            os.environ.pop("EMPANADA_SYNC_ROUNDS", None)
+        same_sync = outs[0] == outs[1]
+        del outs
     if dev["reads"] is not None:
         dev["reads"].free()
     del reads
@@ -320,6 +337,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_sample_vs_oracle": same_sample,
         "parity_full_vs_oracle": same_full,
+        "parity_full_pipelined_vs_synchronous": same_sync,
         "gen_s": round(t_gen, 2),
         "batched": batched,
         "served": served,

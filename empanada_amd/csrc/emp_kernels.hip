@@ -1022,16 +1022,39 @@ __global__ void __launch_bounds__(1024) k_escan_top(unsigned long long *__restri
     }
 }
 
-// Edges per round: the sum of round t's `stride` bucket slots (one work-group per round).
-__global__ void __launch_bounds__(1024) k_round_sums(const uint32_t *__restrict__ cnt, uint64_t stride,
-                                                    unsigned long long *__restrict__ out) {
+// One round's edge offsets in the output, in bucket order: dst[i] = base + the edges of slots
+// before i, with base = *run (the edges of the rounds before) ; then *run += the round's edges and
+// rec = {edges, base}. One work-group; rounds of one run are scanned in order on one stream.
+__global__ void __launch_bounds__(1024) k_round_scan(const uint32_t *__restrict__ cnt, uint32_t n,
+                                                    unsigned long long *__restrict__ dst,
+                                                    unsigned long long *__restrict__ run,
+                                                    unsigned long long *__restrict__ rec) {
     __shared__ unsigned long long sh[16];
-    const uint32_t *c = cnt + (uint64_t)blockIdx.x * stride;
-    unsigned long long v = 0;
-    for (uint64_t i = threadIdx.x; i < stride; i += 1024) v += c[i];
-    unsigned long long tot;
-    block_excl_scan64<1024>(v, sh, tot);
-    if (threadIdx.x == 0) out[blockIdx.x] = tot;
+    const unsigned long long base = *run;
+    unsigned long long carry = base;
+    for (uint32_t c0 = 0; c0 < n; c0 += 4096) {
+        const uint32_t i0 = c0 + 4 * threadIdx.x;
+        uint32_t v[4];
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u] = i0 + u < n ? cnt[i0 + u] : 0u;
+            sum += v[u];
+        }
+        unsigned long long tot;
+        unsigned long long off = carry + block_excl_scan64<1024>(sum, sh, tot);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + u < n) dst[i0 + u] = off;
+            off += v[u];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        rec[0] = carry - base;
+        rec[1] = base;
+        *run = carry;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
@@ -1039,7 +1062,7 @@ __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
     if (b >= a.n) return;
     const uint32_t c = a.cnt[b];
     if (c == 0) return;
-    const unsigned long long src = a.src_off[b], dst = a.dst_off[b] + a.blk[b / ESCAN_BLOCK];
+    const unsigned long long src = a.src_off[b], dst = a.dst_off[b] + (a.blk ? a.blk[b / ESCAN_BLOCK] : 0ull);
     for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
         a.out_from[dst + i] = a.pool_from[src + i];
         a.out_to[dst + i] = a.pool_to[src + i];
@@ -1140,10 +1163,15 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_round_sums(const uint32_t *cnt, uint64_t stride, uint32_t rounds, unsigned long long *out,
-                             hipStream_t s) {
-    if (rounds == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_round_sums, dim3(rounds), dim3(1024), 0, s, cnt, stride, out);
+hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
+                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint32_t *pool_from,
+                              const uint32_t *pool_to, uint32_t *out_from, uint32_t *out_to, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    EdgeCopyArgs a{cnt, src_off, dst_off, nullptr, n, pool_from, pool_to, out_from, out_to};
+    hipLaunchKernelGGL(k_edge_copy, dim3((n + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

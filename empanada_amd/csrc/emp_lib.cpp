@@ -50,7 +50,10 @@ struct DevCounters {
     uint32_t in;                    // dedup partition input size, set by the host
     uint32_t pad;
     ListCtr r[MAX_ROUNDS + 1];      // r[t]: round t's lists
-    unsigned long long redges[MAX_ROUNDS];  // edges per round (pipelined rounds, at the end)
+    // pipelined rounds stream their edges out: output position after the rounds copied so far,
+    // and per round {edges, first position}
+    unsigned long long obase;
+    unsigned long long orec[MAX_ROUNDS][2];
 };
 
 double now_ms() {
@@ -175,6 +178,12 @@ struct emp_ctx {
     bool pipeline = true;
     ListCtr *h_rec = nullptr;
     std::vector<hipEvent_t> rec_ev;
+    // their edges go straight to the pinned output arena (device view of h_from / h_to) on a
+    // second stream, while later rounds run
+    hipStream_t ostream = nullptr;
+    bool out_streamed = false;
+    unsigned long long (*h_orec)[2] = nullptr;  // per round {edges, first position}, pinned
+    std::vector<hipEvent_t> orec_ev;
 
     // statistics
     std::vector<emp_round_stats> rounds;
@@ -861,7 +870,7 @@ int ensure_edge_slots(emp_ctx *c, uint64_t n) {
 // Passes every pending record to the sink (output.bin order) and empties the arena.
 int flush_records(emp_ctx *c, emp_record_sink sink, void *user) {
     if (c->records.empty()) return EMP_OK;
-    if (c->out_used) {
+    if (c->out_used && !c->out_streamed) {
         HIPCHK(c, hipMemcpyAsync(c->h_from, c->d_out_from, c->out_used * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->h_to, c->d_out_to, c->out_used * 4, hipMemcpyDeviceToHost, c->stream));
     }
@@ -874,6 +883,7 @@ int flush_records(emp_ctx *c, emp_record_sink sink, void *user) {
     }
     c->records.clear();
     c->out_used = 0;
+    c->out_streamed = false;
     return EMP_OK;
 }
 
@@ -1274,6 +1284,7 @@ struct RunState {
     uint64_t M = 0, stride = 0, D = 0;
     int bbS[2] = {0, 0};  // bucket bits of the bucket set written for rounds of each parity
     int t = 0;            // rounds done
+    bool streamed = false;  // the edges are already in the pinned arena, in output order
     std::vector<uint64_t> round_edges, rec_edges;  // edges per round; per record
     std::vector<int> rec_round;                      // round of each record
 };
@@ -1292,6 +1303,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
     st.rec_edges.clear();
     st.rec_round.clear();
     st.t = 0;
+    st.streamed = false;
     st.bbS[0] = st.bbS[1] = bbD;
     if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
         return rc;
@@ -1495,15 +1507,44 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     const int L = st.L, rounds = st.rounds;
     const uint64_t stride = st.stride;
     const RoundArgs none{};
-    // round 1's bucket set (set 1) and every round's counter record start at zero; set 0 was
-    // cleared before the dedup wrote it
+    // Output while the rounds run, on a second stream: each round's edges are compacted into the
+    // device arena in output order as soon as its join is done, and every finished prefix of the
+    // arena goes on to the pinned host arena by DMA. The arena holds as many edges as the pool;
+    // nothing is pending in it at the start of a run.
+    if ((rc = ensure_arena(c, c->pool_cap * SUBS + c->pool_ovf_cap, nullptr, nullptr))) return rc;
+    if (!c->ostream) HIPCHK(c, hipStreamCreateWithFlags(&c->ostream, hipStreamNonBlocking));
+    if (!c->h_orec) {
+        void *p = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, MAX_ROUNDS * 2 * sizeof(unsigned long long)));
+        c->h_orec = (unsigned long long(*)[2])p;
+    }
+    if (c->orec_ev.empty()) {
+        c->orec_ev.resize(MAX_ROUNDS);
+        for (auto &e : c->orec_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    uint64_t copied = 0;  // arena edges already sent to the host
+    auto send = [&](uint64_t upto) -> int {
+        if (upto <= copied) return EMP_OK;
+        const size_t bytes = (size_t)(upto - copied) * 4;
+        HIPCHK(c, hipMemcpyAsync(c->h_from + copied, c->d_out_from + copied, bytes, hipMemcpyDeviceToHost, c->ostream));
+        HIPCHK(c, hipMemcpyAsync(c->h_to + copied, c->d_out_to + copied, bytes, hipMemcpyDeviceToHost, c->ostream));
+        copied = upto;
+        return EMP_OK;
+    };
+    // round 1's bucket set (set 1), every round's counter record and the output position start
+    // at zero; set 0 was cleared before the dedup wrote it
     {
         ZeroList z;
         z.add(c->d_bcnt[1][0], 1ull << st.bbD);
         z.add(c->d_bcnt[1][1], 1ull << st.bbD);
         z.add(reinterpret_cast<uint32_t *>(&c->d_ctr->r[1]), (uint64_t)rounds * sizeof(ListCtr) / 4);
+        z.add(reinterpret_cast<uint32_t *>(&c->d_ctr->obase), 2);
         if ((rc = run_zero(c, z))) return rc;
     }
+    auto fallback = [&]() {
+        (void)hipStreamSynchronize(c->ostream);
+        return RUN_FALLBACK;
+    };
     if (!c->h_rec) {
         void *p = nullptr;
         HIPCHK(c, hipHostMalloc(&p, (MAX_ROUNDS + 1) * sizeof(ListCtr)));
@@ -1522,10 +1563,16 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         if (t >= 2) {
             HIPCHK(c, hipEventSynchronize(c->rec_ev[t - 2]));
             const ListCtr &r = c->h_rec[t - 1];
-            if (r.n_big || r.nx_ovf) return RUN_FALLBACK;
+            if (r.n_big || r.nx_ovf) return fallback();
             Sk = r.total[0];
             Pk = r.total[1];
             if (Sk == 0 || Pk == 0) break;  // round t-1 writes no record, nor any later round
+        }
+        // round t-3's edges are in the device arena: its compaction started when join t-3 ended,
+        // two joins ago (join t-2 has ended too: its record is in)
+        if (t >= 3) {
+            HIPCHK(c, hipEventSynchronize(c->orec_ev[t - 3]));
+            if ((rc = send(c->h_orec[t - 3][1] + c->h_orec[t - 3][0]))) return rc;
         }
         const bool more = t + 1 < rounds;
         const int bb = bits[t];
@@ -1559,17 +1606,28 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         HIPCHK(c, hipMemcpyAsync(&c->h_rec[t + 1], &c->d_ctr->r[t + 1], sizeof(ListCtr), hipMemcpyDeviceToHost,
                                  c->stream));
         HIPCHK(c, hipEventRecord(c->rec_ev[t], c->stream));
+        // the round's edges, in output order, into the device arena while later rounds run
+        HIPCHK(c, hipStreamWaitEvent(c->ostream, c->rec_ev[t], 0));
+        HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, 1u << bb,
+                                     &c->d_ctr->obase, c->d_ctr->orec[t], c->d_pool_from, c->d_pool_to,
+                                     c->d_out_from, c->d_out_to, c->ostream));
+        HIPCHK(c, hipMemcpyAsync(c->h_orec[t], c->d_ctr->orec[t], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->ostream));
+        HIPCHK(c, hipEventRecord(c->orec_ev[t], c->ostream));
         launched = t + 1;
     }
-    // every launched round's record, the pool cursors and the edges per round
-    HIPCHK(c, launch_round_sums(c->d_edge_cnt, stride, (uint32_t)launched, c->d_ctr->redges, c->stream));
+    // every launched round's record, the pool cursors and the edges per round; the rest of the
+    // arena to the host
+    HIPCHK(c, hipStreamSynchronize(c->ostream));
     if ((rc = read_counters(c))) return rc;
+    if (launched && (rc = send(c->h_ctr->obase))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->ostream));
     const DevCounters &k = *c->h_ctr;
-    if (k.pool_ovf > c->pool_ovf_cap) return RUN_FALLBACK;
+    if (k.pool_ovf > c->pool_ovf_cap) return fallback();
     int t = 0;
     for (; t < launched; ++t) {
         const ListCtr &r = k.r[t + 1];
-        if (r.n_big || r.nx_ovf) return RUN_FALLBACK;
+        if (r.n_big || r.nx_ovf) return fallback();
     }
     for (t = 0; t < launched; ++t) {
         const uint64_t St = k.r[t].total[0], Pt = k.r[t].total[1];
@@ -1579,7 +1637,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         rs.p = Pt;
         rs.s_after = k.r[t + 1].total[0];
         rs.p_after = k.r[t + 1].total[1];
-        rs.edges = k.redges[t];
+        rs.edges = k.orec[t][0];
         rs.ovlp = L - 1 - t;
         rs.records = 1;
         rs.bucket_bits = bits[t];
@@ -1597,6 +1655,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         c->rounds.push_back(rs);
     }
     st.t = t;
+    st.streamed = true;
     return EMP_OK;
 }
 
@@ -1671,8 +1730,8 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     // order, then hand the records to the sink.
     uint64_t total = 0;
     for (uint64_t e : st.round_edges) total += e;
-    if ((rc = ensure_arena(c, std::max<uint64_t>(total, 1), sink, user))) return rc;
-    if (total) {
+    if (!st.streamed && (rc = ensure_arena(c, std::max<uint64_t>(total, 1), sink, user))) return rc;
+    if (total && !st.streamed) {
         EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->d_eblk, (uint64_t)st.t * stride,
                         c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
         Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)st.t * stride);
@@ -1684,6 +1743,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         off += st.rec_edges[i];
     }
     c->out_used = total;
+    c->out_streamed = st.streamed;
     return flush_records(c, sink, user);
 }
 
@@ -1766,6 +1826,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ostream) (void)hipStreamSynchronize(c->ostream);
     for (auto &a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     for (int i = 0; i < 2; ++i) {
@@ -1778,12 +1839,15 @@ void emp_ctx_destroy(emp_ctx *c) {
     if (c->h_fq_res) (void)hipHostFree(c->h_fq_res);
     if (c->h_rec) (void)hipHostFree(c->h_rec);
     for (auto e : c->rec_ev) (void)hipEventDestroy(e);
+    if (c->h_orec) (void)hipHostFree(c->h_orec);
+    for (auto e : c->orec_ev) (void)hipEventDestroy(e);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->ostream) (void)hipStreamDestroy(c->ostream);
     delete c;
 }
 
