@@ -204,6 +204,19 @@ __device__ __forceinline__ void put_next(const NextOut &nx, Entry e, uint32_t si
 
 // ---------------------------------------------------------------------------------------------
 // Map: one work-group of 256 threads per 256 reads.
+// Bytes of w outside ACGTacgt (non-zero when any).
+__device__ __forceinline__ uint32_t bad_bases(uint32_t w) {
+    uint32_t bad = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t lc = ((w >> (8 * i)) & 0xFFu) | 0x20u;
+        bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
+    }
+    return bad;
+}
+
+constexpr int MAP_Q = 7;  // 16-byte loads per thread in flight (a 256 x 100 bp block is 1,600)
+
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     extern __shared__ __align__(16) uint8_t tile[];
     const uint32_t first = blockIdx.x * a.rows;
@@ -211,43 +224,79 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     const int L = a.L, Lp = a.Lpad;
     const uint8_t *src = a.reads + (size_t)first * L;
     const uint32_t nbytes = nt * (uint32_t)L;
-    const uint32_t ndw = nbytes >> 2;
     uint32_t bad = 0;
-    for (uint32_t d = threadIdx.x; d < ndw; d += 256) {
-        uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(src) + d);
-        uint32_t g = d * 4;
-        uint32_t r = g / (uint32_t)L, p = g - r * (uint32_t)L;
+    // Reads -> LDS tile (row stride Lp). A 16-byte aligned block is read with 16-byte loads, up to
+    // MAP_Q per thread in flight; when Lp == L the tile is the block itself.
+    uint32_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
+        const uint32_t nq = nbytes >> 4;
+        for (uint32_t q0 = 0; q0 < nq; q0 += 256 * MAP_Q) {
+            uint4 v[MAP_Q];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint8_t c = (uint8_t)(w >> (8 * i));
-            uint8_t lc = c | 0x20;
-            bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
-            tile[r * Lp + p] = c;
-            if (++p == (uint32_t)L) { p = 0; ++r; }
+            for (int u = 0; u < MAP_Q; ++u) {
+                const uint32_t q = q0 + (uint32_t)u * 256 + threadIdx.x;
+                if (q < nq) v[u] = reinterpret_cast<const uint4 *>(src)[q];
+            }
+#pragma unroll
+            for (int u = 0; u < MAP_Q; ++u) {
+                const uint32_t q = q0 + (uint32_t)u * 256 + threadIdx.x;
+                if (q < nq) {
+                    bad |= bad_bases(v[u].x) | bad_bases(v[u].y) | bad_bases(v[u].z) | bad_bases(v[u].w);
+                    const uint32_t g = q * 16;
+                    if (Lp == L) {
+                        *reinterpret_cast<uint4 *>(tile + g) = v[u];
+                    } else {
+                        const uint32_t w4[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                        uint32_t r = g / (uint32_t)L, p = g - r * (uint32_t)L;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            tile[r * Lp + p] = (uint8_t)(w4[i >> 2] >> (8 * (i & 3)));
+                            if (++p == (uint32_t)L) { p = 0; ++r; }
+                        }
+                    }
+                }
+            }
         }
+        done = nq * 16;
     }
-    for (uint32_t g = ndw * 4 + threadIdx.x; g < nbytes; g += 256) {
-        uint8_t c = src[g];
-        uint8_t lc = c | 0x20;
-        bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
-        uint32_t r = g / (uint32_t)L;
+    for (uint32_t g = done + threadIdx.x; g < nbytes; g += 256) {
+        const uint8_t c = src[g];
+        bad |= bad_bases(c | 0x61616100u);
+        const uint32_t r = g / (uint32_t)L;
         tile[r * Lp + (g - r * (uint32_t)L)] = c;
     }
     if (bad) atomicOr(a.err, 1u);
     lds_barrier();
 
+    // Fingerprints of the read and of its reverse complement, four Horner chains side by side,
+    // the row read four bases at a time (the reverse complement takes the row from its end).
     const uint32_t t = threadIdx.x;
     if (t < nt) {
         const uint8_t *row = tile + t * Lp;
+        const uint32_t *row4 = reinterpret_cast<const uint32_t *>(row);
+        const int nd = L >> 2, tl = L & 3;
         uint32_t f1 = 0, f2 = 0, r1 = 0, r2 = 0;
-        for (int p = 0; p < L; ++p) {
-            uint32_t code = (row[p] >> 1) & 3u;
-            uint32_t e = enc_of_code(code);
-            f1 = step1(f1, e);
-            f2 = step2(f2, e);
-            uint32_t ce = enc_of_code(((row[L - 1 - p] >> 1) & 3u) ^ 2u);
+        for (int i = tl - 1; i >= 0; --i) {  // the reverse complement starts with the last bases
+            const uint32_t ce = enc_of_code(((row[4 * nd + i] >> 1) & 3u) ^ 2u);
             r1 = step1(r1, ce);
             r2 = step2(r2, ce);
+        }
+        for (int d = 0; d < nd; ++d) {
+            const uint32_t wf = row4[d], wr = row4[nd - 1 - d];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t e = enc_of_code((wf >> (8 * i + 1)) & 3u);
+                f1 = step1(f1, e);
+                f2 = step2(f2, e);
+                const uint32_t ce = enc_of_code(((wr >> (8 * (3 - i) + 1)) & 3u) ^ 2u);
+                r1 = step1(r1, ce);
+                r2 = step2(r2, ce);
+            }
+        }
+        for (int i = 0; i < tl; ++i) {
+            const uint32_t e = enc_of_code((row[4 * nd + i] >> 1) & 3u);
+            f1 = step1(f1, e);
+            f2 = step2(f2, e);
         }
         uint64_t k = a.k0 + first + t;
         Entry ef = {f1, f2, (uint32_t)(2 * k)};
