@@ -204,15 +204,14 @@ __device__ __forceinline__ void put_next(const NextOut &nx, Entry e, uint32_t si
 
 // ---------------------------------------------------------------------------------------------
 // Map: one work-group of 256 threads per 256 reads.
-// Bytes of w outside ACGTacgt (non-zero when any).
+// Bytes of w outside ACGTacgt (non-zero when any), four bytes at a time: after setting the
+// lower-case bit, a byte is a base when it equals one of a, c, g, t; nz(v) has bit 7 of each
+// byte set where that byte of v is not zero.
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t v) { return (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u; }
 __device__ __forceinline__ uint32_t bad_bases(uint32_t w) {
-    uint32_t bad = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t lc = ((w >> (8 * i)) & 0xFFu) | 0x20u;
-        bad |= !(lc == 'a' || lc == 'c' || lc == 'g' || lc == 't');
-    }
-    return bad;
+    const uint32_t x = w | 0x20202020u;
+    return nz_bytes(x ^ 0x61616161u) & nz_bytes(x ^ 0x63636363u) & nz_bytes(x ^ 0x67676767u) &
+           nz_bytes(x ^ 0x74747474u);
 }
 
 constexpr int MAP_Q = 7;  // 16-byte loads per thread in flight (a 256 x 100 bp block is 1,600)
@@ -231,12 +230,12 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     if ((reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
         const uint32_t nq = nbytes >> 4;
         for (uint32_t q0 = 0; q0 < nq; q0 += 256 * MAP_Q) {
+            // every lane loads (past the block: its last 16 bytes again), so that no branch
+            // separates the loads and all of them are in flight together
             uint4 v[MAP_Q];
 #pragma unroll
-            for (int u = 0; u < MAP_Q; ++u) {
-                const uint32_t q = q0 + (uint32_t)u * 256 + threadIdx.x;
-                if (q < nq) v[u] = reinterpret_cast<const uint4 *>(src)[q];
-            }
+            for (int u = 0; u < MAP_Q; ++u)
+                v[u] = reinterpret_cast<const uint4 *>(src)[min(q0 + (uint32_t)u * 256 + threadIdx.x, nq - 1)];
 #pragma unroll
             for (int u = 0; u < MAP_Q; ++u) {
                 const uint32_t q = q0 + (uint32_t)u * 256 + threadIdx.x;
@@ -278,26 +277,30 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         uint32_t f1 = 0, f2 = 0, r1 = 0, r2 = 0;
         for (int i = tl - 1; i >= 0; --i) {  // the reverse complement starts with the last bases
             const uint32_t ce = enc_of_code(((row[4 * nd + i] >> 1) & 3u) ^ 2u);
-            r1 = step1(r1, ce);
-            r2 = step2(r2, ce);
+            r1 = lazy_step1(r1, ce);
+            r2 = lazy_step2(r2, ce);
         }
         for (int d = 0; d < nd; ++d) {
             const uint32_t wf = row4[d], wr = row4[nd - 1 - d];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint32_t e = enc_of_code((wf >> (8 * i + 1)) & 3u);
-                f1 = step1(f1, e);
-                f2 = step2(f2, e);
+                f1 = lazy_step1(f1, e);
+                f2 = lazy_step2(f2, e);
                 const uint32_t ce = enc_of_code(((wr >> (8 * (3 - i) + 1)) & 3u) ^ 2u);
-                r1 = step1(r1, ce);
-                r2 = step2(r2, ce);
+                r1 = lazy_step1(r1, ce);
+                r2 = lazy_step2(r2, ce);
             }
         }
         for (int i = 0; i < tl; ++i) {
             const uint32_t e = enc_of_code((row[4 * nd + i] >> 1) & 3u);
-            f1 = step1(f1, e);
-            f2 = step2(f2, e);
+            f1 = lazy_step1(f1, e);
+            f2 = lazy_step2(f2, e);
         }
+        f1 = fin1(f1);
+        f2 = fin2(f2);
+        r1 = fin1(r1);
+        r2 = fin2(r2);
         uint64_t k = a.k0 + first + t;
         Entry ef = {f1, f2, (uint32_t)(2 * k)};
         Entry er = {r1, r2, (uint32_t)(2 * k + 1)};
