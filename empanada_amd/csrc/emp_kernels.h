@@ -110,6 +110,9 @@ struct MapArgs {
     // group g at cbase + g * gs + k / 4 (k and entry ids local to the job)
     uint64_t cbase;
     uint32_t gs;
+    // non-null: per read k, the codes of its first and last column groups (group 0 in bits 0-7,
+    // group ceil(L/4) - 1 in bits 8-15): everything round 0's regeneration reads from the columns
+    uint16_t *ends;
 };
 
 // One partition pass: input groups (keys sharing their top bits) -> output buckets refined by
@@ -136,6 +139,7 @@ struct PartArgs {
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
+    const uint16_t *ends;    // per read: first and last column groups (MapArgs::ends)
     NextOut nx;              // the kept entries, regenerated: round 0's suffix and prefix lists
     uint32_t *big, *n_big;
     uint32_t big_cap;

@@ -306,6 +306,16 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         Entry er = {r1, r2, (uint32_t)(2 * k + 1)};
         a.entries[2 * k] = ef;
         a.entries[2 * k + 1] = er;
+        if (a.ends) {
+            const uint32_t gl = 4u * (col_groups((uint32_t)L) - 1u);
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; ++c) {
+                if (c < (uint32_t)L) lo |= ((uint32_t)(row[c] >> 1) & 3u) << (2 * c);
+                if (gl + c < (uint32_t)L) hi |= ((uint32_t)(row[gl + c] >> 1) & 3u) << (2 * c);
+            }
+            a.ends[k] = (uint16_t)(lo | (hi << 8));
+        }
     }
 
     // Columns: word w of the tile covers reads 4w..4w+3; task id = g * nw + w keeps a wave on
@@ -468,8 +478,25 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         uint32_t *row = wcnt[threadIdx.x >> 6];
         if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
         uint32_t ts[PER], tp[PER];
-        next_gather<PER>(a.nx, kept, keepmask, ts, 0);
-        next_gather<PER>(a.nx, kept, keepmask, tp, 1);
+        if (a.ends && a.nx.rg.col_t == 0) {
+            // round 0 gathers column group 0 or the last one (suffix and prefix in turn): both
+            // are in the read's ends word, one 2-byte load per kept entry
+            uint32_t ew[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) ew[q] = a.ends[((keepmask >> q) & 1u) ? kept[q].rid >> 1 : 0u];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                ts[q] = tp[q] = 0x100u;
+                if ((keepmask >> q) & 1u) {
+                    uint32_t g;
+                    if (regen_needs_gather(a.nx.rg, kept[q].rid, true, g)) ts[q] = (g ? ew[q] >> 8 : ew[q]) & 0xFFu;
+                    if (regen_needs_gather(a.nx.rg, kept[q].rid, false, g)) tp[q] = (g ? ew[q] >> 8 : ew[q]) & 0xFFu;
+                }
+            }
+        } else {
+            next_gather<PER>(a.nx, kept, keepmask, ts, 0);
+            next_gather<PER>(a.nx, kept, keepmask, tp, 1);
+        }
         next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
         next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, tp, 1);
         lds_barrier();

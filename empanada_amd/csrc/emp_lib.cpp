@@ -110,6 +110,7 @@ struct emp_ctx {
     size_t cols_chunks = 0;      // chunks allocated
     Entry *d_entries = nullptr;
     uint64_t entries_cap = 0;
+    uint16_t *d_ends = nullptr;  // per read: first and last column groups (entries_cap / 2 of them)
     uint32_t *d_err = nullptr;
 
     // host staging (pinned, double-buffered)
@@ -325,15 +326,19 @@ int ensure_entries(emp_ctx *c, uint64_t n_entries) {
     if (n_entries <= c->entries_cap) return EMP_OK;
     uint64_t cap = std::max<uint64_t>(c->entries_cap * 2, 1u << 16);
     while (cap < n_entries) cap *= 2;
-    void *p = nullptr;
+    void *p = nullptr, *q = nullptr;
     int rc = dalloc(c, &p, cap * sizeof(Entry));
     if (rc) return rc;
+    if ((rc = dalloc(c, &q, cap / 2 * sizeof(uint16_t)))) return rc;
     if (c->d_entries) {
         HIPCHK(c, hipMemcpyAsync(p, c->d_entries, 2 * c->n * sizeof(Entry), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(q, c->d_ends, c->n * sizeof(uint16_t), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         dfree(c, c->d_entries);
+        dfree(c, c->d_ends);
     }
     c->d_entries = (Entry *)p;
+    c->d_ends = (uint16_t *)q;
     c->entries_cap = cap;
     return EMP_OK;
 }
@@ -387,6 +392,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
         a.k0 = c->n + done;
         a.cols = c->d_cols;
         a.entries = c->d_entries;
+        a.ends = c->d_ends;
         a.err = c->d_err;
         {
             Timed t(c, "k_map", (double)m * c->L * 1.25 + 24.0 * m);
@@ -1325,6 +1331,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         DedupArgs a{};
         a.in = c->d_dfine;
         a.cnt = c->d_dcnt;
+        a.ends = c->d_ends;
         a.nx = next_out(c, 0, bbD, bbD, rg0);
         a.big = c->d_big;
         a.n_big = &c->d_ctr->r[0].n_big;
