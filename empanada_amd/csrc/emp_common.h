@@ -177,35 +177,12 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) {
 //   slots 12..21  prefixes: 12 + 2 k + d, d <= 1;
 //   slots 30, 31  suffixes / prefixes outside these (the reference's u64 wrap for h1 < e): one
 //                 global atomic each.
-// Work-groups rank their entries per slot with wave ballots (no LDS atomics per entry) and
-// reserve one run per slot.
+// Work-groups rank their entries per slot with one LDS atomic per entry on a per-wave counter
+// row, and reserve one run per slot.
 constexpr int NSLOT = 32;
 constexpr uint32_t SLOT_SFX_OTHER = 30, SLOT_PFX_OTHER = 31;
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-// Rank of this lane's entry among the wave's entries of its slot, counting earlier calls: `row`
-// is the wave's own NSLOT counters in LDS (no other wave touches them). Lanes with act false
-// take part in the ballots only. One iteration per distinct slot in the wave.
-__device__ __forceinline__ uint32_t wave_rank(uint32_t *row, bool act, uint32_t slot) {
-    uint32_t rank = 0;
-    bool pend = act;
-    for (;;) {
-        const unsigned long long todo = __ballot(pend);
-        if (!todo) break;
-        const int first = __builtin_ctzll(todo);
-        const uint32_t s = (uint32_t)__builtin_amdgcn_readlane((int)slot, first);
-        const bool mine = pend && slot == s;
-        const unsigned long long m = __ballot(mine);
-        const uint32_t base = row[s];
-        if (mine) {
-            rank = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            pend = false;
-        }
-        if (lane_id() == (uint32_t)first) row[s] = base + (uint32_t)__popcll(m);
-    }
-    return rank;
-}
 
 // LDS of the join of one bucket: the prefix entries (12 B each), a used flag per prefix (then,
 // for prefix survivors, the column-group byte they gathered), and one region that holds the hash

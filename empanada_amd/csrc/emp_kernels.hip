@@ -126,43 +126,41 @@ __device__ __forceinline__ void next_gather(const NextOut &nx, const Entry (&e)[
     }
 }
 
-// Rank step (after next_gather): regenerates each entry, finds its slot for source bucket b and
-// ranks it among the wave's entries of the slot; row: this wave's NSLOT counters (zeroed), ntot:
-// the work-group's per-side totals.
+// Rank step (after next_gather): regenerates each entry in place (e[q] becomes its next-round
+// entry), finds its slot for source bucket b and takes a rank among the wave's entries of that
+// slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches them).
+// ntot: the work-group's per-side totals. tgt[q] becomes the slot (bits 9-13) and the rank
+// (bits 14 up).
 template <int PER>
-__device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot,
-                                          const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
+__device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
+                                          uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
     const bool sfx = side == 0;
     uint32_t wn = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const bool act = (mask >> q) & 1u;
-        uint32_t slot = 0;
         if (act) {
-            Entry ne = e[q];
-            const uint32_t ev = regen_carry(nx.cols, nx.rg, ne, sfx, tgt_word(tgt[q]));
-            const uint32_t h1n = ne.h1 & KEY_MASK;
-            slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, e[q].h1 & KEY_MASK, ev, h1n);
+            const uint32_t h1o = e[q].h1 & KEY_MASK;
+            const uint32_t ev = regen_carry(nx.cols, nx.rg, e[q], sfx, tgt_word(tgt[q]));
+            const uint32_t h1n = e[q].h1 & KEY_MASK;
+            const uint32_t slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, h1o, ev, h1n);
+            tgt[q] = (slot << 9) | (atomicAdd(row + slot, 1u) << 14);
         }
-        tgt[q] |= (slot << 9) | (wave_rank(row, act, slot) << 14);
         wn += (uint32_t)__popcll(__ballot(act));
     }
     if (lane_id() == 0 && wn) atomicAdd(ntot + side, wn);
 }
 
-// Write step: `row` now holds this wave's run start per slot.
+// Write step: `row` now holds this wave's run start per slot; e[q] are the next-round entries.
 template <int PER>
 __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *row, const Entry (&e)[PER], uint32_t mask,
                                            const uint32_t (&tgt)[PER], uint32_t side) {
-    const bool sfx = side == 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         if ((mask >> q) & 1u) {
-            Entry ne = e[q];
-            regen_carry(nx.cols, nx.rg, ne, sfx, tgt_word(tgt[q]));
-            const uint32_t nb = (ne.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
+            const uint32_t nb = (e[q].h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[side] + nb, 1u);
-            next_put(nx, side, nb, pos, ne);
+            next_put(nx, side, nb, pos, e[q]);
         }
     }
 }
@@ -497,8 +495,11 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
             next_gather<PER>(a.nx, kept, keepmask, ts, 0);
             next_gather<PER>(a.nx, kept, keepmask, tp, 1);
         }
+        Entry kp[PER];  // the prefix side's copy (each side is regenerated in place)
+#pragma unroll
+        for (int q = 0; q < PER; ++q) kp[q] = kept[q];
         next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
-        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, tp, 1);
+        next_rank<PER>(a.nx, b, row, ntot, kp, keepmask, tp, 1);
         lds_barrier();
         if (threadIdx.x < NSLOT) {
             slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64>(a.nx, b, wcnt, threadIdx.x));
@@ -508,7 +509,7 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         }
         lds_barrier();
         next_write<PER>(a.nx, row, kept, keepmask, ts, 0);
-        next_write<PER>(a.nx, row, kept, keepmask, tp, 1);
+        next_write<PER>(a.nx, row, kp, keepmask, tp, 1);
     } else if (keepmask) {  // no round follows: the kept entries are only counted
         atomicAdd(a.nx.total[0], (uint32_t)__popc(keepmask));
         atomicAdd(a.nx.total[1], (uint32_t)__popc(keepmask));
@@ -570,15 +571,18 @@ __device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const bool act = (pmask >> q) & 1u;
-        uint32_t slot = 0;
         if (act) {
+            // the regenerated entry replaces the old key in LDS (the matched records hold copies
+            // of the keys the edges need)
             const uint32_t i = threadIdx.x + NT * q;
             Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
             const uint32_t h1o = e.h1 & KEY_MASK;
             const uint32_t ev = regen_carry(nx.cols, nx.rg, e, false, tgt_word(tgt[q]));
-            slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
+            S.pk1[i] = e.h1;
+            S.pk2[i] = e.h2;
+            const uint32_t slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
+            tgt[q] = (slot << 9) | (atomicAdd(row + slot, 1u) << 14);
         }
-        tgt[q] |= (slot << 9) | (wave_rank(row, act, slot) << 14);
         wn += (uint32_t)__popcll(__ballot(act));
     }
     if (lane_id() == 0 && wn) atomicAdd(&S.ntot[1], wn);
@@ -592,8 +596,7 @@ __device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CA
     for (int q = 0; q < PER; ++q) {
         if ((pmask >> q) & 1u) {
             const uint32_t i = threadIdx.x + NT * q;
-            Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
-            regen_carry(nx.cols, nx.rg, e, false, tgt_word(tgt[q]));
+            const Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
             const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[1] + nb, 1u);
             next_put(nx, 1, nb, pos, e);
@@ -606,7 +609,7 @@ __device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CA
 // than MCAP, are handed to the large-bucket path.
 template <int CAP, int MCAP, int NT>
 __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t ns,
-                                            uint32_t np, const Entry (&se)[CAP / NT]) {
+                                            uint32_t np, Entry (&se)[CAP / NT]) {
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
     constexpr int PER = CAP / NT;
     if (ns > (uint32_t)CAP || np > (uint32_t)CAP) {
