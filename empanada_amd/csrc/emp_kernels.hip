@@ -96,7 +96,8 @@ __device__ __forceinline__ void next_put(const NextOut &nx, uint32_t side, uint3
 }
 
 // Between count and write an entry is kept as one word: its gathered codes (9 bits, 0x100 =
-// none), its slot (5 bits) and its rank among the wave's entries of the slot (9 bits).
+// none), its slot (5 bits) and, from bit 14, its rank among the wave's entries of the slot (for a
+// fallback slot: its position in the target bucket).
 __device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
 
 // Gather step for register-resident entries e[q] (bit q of mask) of side `side`: tgt[q] = the
@@ -126,11 +127,21 @@ __device__ __forceinline__ void next_gather(const NextOut &nx, const Entry (&e)[
     }
 }
 
+// An entry's rank in its slot (table slots: LDS counter of the wave), or for the two fallback slots
+// its position in the target bucket, taken here with a global atomic (clamped to BUCKET_CAP: any
+// position from there on spills). Taking the fallback positions in the rank step keeps atomics
+// out of the write step, whose stores would otherwise all wait for them.
+__device__ __forceinline__ uint32_t slot_rank(const NextOut &nx, uint32_t *row, uint32_t slot, uint32_t h1n, uint32_t side) {
+    if (slot < 22u) return atomicAdd(row + slot, 1u);
+    const uint32_t pos = atomicAdd((side ? nx.cnt[1] : nx.cnt[0]) + (h1n >> nx.shift), 1u);
+    return pos < (uint32_t)BUCKET_CAP ? pos : (uint32_t)BUCKET_CAP;
+}
+
 // Rank step (after next_gather): regenerates each entry in place (e[q] becomes its next-round
 // entry), finds its slot for source bucket b and takes a rank among the wave's entries of that
 // slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches them).
 // ntot: the work-group's per-side totals. tgt[q] becomes the slot (bits 9-13) and the rank
-// (bits 14 up).
+// (bits 14 up, slot_rank).
 template <int PER>
 __device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
                                           uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
@@ -144,7 +155,7 @@ __device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_
             const uint32_t ev = regen_carry(nx.cols, nx.rg, e[q], sfx, tgt_word(tgt[q]));
             const uint32_t h1n = e[q].h1 & KEY_MASK;
             const uint32_t slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, h1o, ev, h1n);
-            tgt[q] = (slot << 9) | (atomicAdd(row + slot, 1u) << 14);
+            tgt[q] = (slot << 9) | (slot_rank(nx, row, slot, h1n, side) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
     }
@@ -159,7 +170,7 @@ __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *ro
     for (int q = 0; q < PER; ++q) {
         if ((mask >> q) & 1u) {
             const uint32_t nb = (e[q].h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
-            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[side] + nb, 1u);
+            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
             next_put(nx, side, nb, pos, e[q]);
         }
     }
@@ -168,13 +179,17 @@ __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *ro
 // Reserve step, thread for slot s of an NW-wave work-group: the slot's entries over all waves,
 // one run in its target bucket (returned: the run start), to be turned into per-wave starts by
 // slot_starts once the reservation is back.
-template <int NW>
+// SEL: pick the side's count array with a select of the two pointers instead of an index into
+// the argument (which the compiler loads per lane); the select holds more scalar registers, which
+// pays in the dedup and not in the join.
+template <int NW, bool SEL = false>
 __device__ __forceinline__ uint32_t slot_reserve(const NextOut &nx, uint32_t b, const uint32_t (*wcnt)[NSLOT], uint32_t s) {
     uint32_t tot = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) tot += wcnt[w][s];
     if (s >= 22u || tot == 0) return 0u;
-    return atomicAdd(nx.cnt[s < 12u ? 0 : 1] + slot_bucket(nx, b, s), tot);
+    uint32_t *const cnt = SEL ? (s < 12u ? nx.cnt[0] : nx.cnt[1]) : nx.cnt[s < 12u ? 0 : 1];
+    return atomicAdd(cnt + slot_bucket(nx, b, s), tot);
 }
 
 template <int NW>
@@ -502,10 +517,10 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         next_rank<PER>(a.nx, b, row, ntot, kp, keepmask, tp, 1);
         lds_barrier();
         if (threadIdx.x < NSLOT) {
-            slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64>(a.nx, b, wcnt, threadIdx.x));
+            slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64, true>(a.nx, b, wcnt, threadIdx.x));
         } else if (threadIdx.x < NSLOT + 2) {
             const uint32_t sd = threadIdx.x - NSLOT;
-            if (ntot[sd]) atomicAdd(a.nx.total[sd], ntot[sd]);
+            if (ntot[sd]) atomicAdd(sd ? a.nx.total[1] : a.nx.total[0], ntot[sd]);
         }
         lds_barrier();
         next_write<PER>(a.nx, row, kept, keepmask, ts, 0);
@@ -581,7 +596,7 @@ __device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP
             S.pk1[i] = e.h1;
             S.pk2[i] = e.h2;
             const uint32_t slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
-            tgt[q] = (slot << 9) | (atomicAdd(row + slot, 1u) << 14);
+            tgt[q] = (slot << 9) | (slot_rank(nx, row, slot, e.h1 & KEY_MASK, 1) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
     }
@@ -598,7 +613,7 @@ __device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CA
             const uint32_t i = threadIdx.x + NT * q;
             const Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
             const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
-            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : atomicAdd(nx.cnt[1] + nb, 1u);
+            const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
             next_put(nx, 1, nb, pos, e);
         }
     }
@@ -719,17 +734,21 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     const uint32_t nms = S.ms, nmp = S.mp, tot = S.etot;
 
     // Reservations, issued now and read back after the sort so that their latency overlaps it:
-    // thread 16 reserves the edges, threads 64 + k the next-round run of table slot k, and the two
-    // threads after them add the side totals.
-    constexpr uint32_t NXT0 = 64;
-    static_assert(NT >= NXT0 + NSLOT + 2, "reservation threads");
+    // thread 16 reserves the edges, threads 64 + k the next-round run of table slot k, and threads
+    // 128 and 129 (another wave: nothing in it waits for the returns) add the side totals.
+    // The edge reservation's address is made lane-dependent (lane 16: offset 0) so that it stays
+    // one lane's atomic; the compiler's wave-wide form of an atomic at a uniform address reads
+    // the result back at once.
+    constexpr uint32_t NXT0 = 64, TOT0 = 128;
+    static_assert(NT >= TOT0 + 2, "reservation threads");
     unsigned long long resv = 0;
     if (threadIdx.x == 16 && tot) {
-        resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)), (unsigned long long)tot);
+        resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)) + (lane_id() - 16u), (unsigned long long)tot);
     } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NSLOT && a.nx.active) {
         resv = slot_reserve<NT / 64>(a.nx, b, S.u.m.wcnt, threadIdx.x - NXT0);
-    } else if (threadIdx.x >= NXT0 + NSLOT && threadIdx.x < NXT0 + NSLOT + 2) {
-        const uint32_t sd = threadIdx.x - NXT0 - NSLOT;
+    }
+    if (threadIdx.x >= TOT0 && threadIdx.x < TOT0 + 2) {
+        const uint32_t sd = threadIdx.x - TOT0;
         if (S.ntot[sd]) atomicAdd(a.nx.total[sd], S.ntot[sd]);
     }
 

@@ -1711,7 +1711,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     // Rounds run back to back when nothing needs the host between them: no audit and no
     // framing (lists below M/2; they only shrink). An exceptional round (a bucket over capacity,
     // a spill or pool overflow) sends the run to the synchronous rounds, from the dedup on.
-    bool pipelined = !c->audit && !JOIN_PERSIST && c->pipeline;
+    bool pipelined = !c->audit && c->pipeline;
     for (int pass = 0;; ++pass) {
         if ((rc = dedup_phase(c, st, N0))) return rc;
         S.n_dedup = st.D;

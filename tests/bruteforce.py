@@ -136,6 +136,27 @@ def wrap_free(reads, min_ovlp):
     return True
 
 
+def wrap_read(L, ln, rng):
+    """A read of length L whose prefix of length ln has H1 < e(its last base): stepping that
+    prefix back takes PrevPfx's u64-wrap branch (overlap.cu:87-88). The first ln - 13 bases are
+    random; the next 12 are solved for by base-5 digits (H is linear in them), the last one is G
+    (e = 4): H1(prefix[:ln-1]) * 5 + 4 = v (mod P1) for a random v < 4."""
+    dec = {1: "A", 2: "C", 3: "T", 4: "G"}
+    k, inv5, p5k = 12, pow(5, -1, P1), pow(5, 12, P1)
+    while True:
+        a = "".join(rng.choice("ACGT") for _ in range(ln - 1 - k))
+        tgt = (rng.randrange(4) - 4) * inv5 % P1
+        x = (tgt - (key(a) >> 32) * p5k) % P1
+        digits = []
+        for _ in range(k):
+            digits.append(x % 5)
+            x //= 5
+        if x == 0 and all(1 <= d <= 4 for d in digits):
+            pre = a + "".join(dec[d] for d in reversed(digits)) + "G"
+            assert (key(pre) >> 32) < 4
+            return pre + "".join(rng.choice("ACGT") for _ in range(L - ln))
+
+
 def audit(reads, min_ovlp):
     """The collision audit from strings: for the 2n oriented reads, then per round for the live
     suffix and prefix lists, (list, ovlp, entries, distinct keys, distinct strings). The live lists

@@ -139,6 +139,26 @@ def test_pipeline_generated_vs_oracle(G, n, L, o, p_rc, p_sub, seeds):
     assert s["n_dedup"] == osum["n_dedup"] and s["n_edges"] == osum["n_edges"]
 
 
+def test_prefix_wrap_takes_the_fallback_slot():
+    """Reads whose prefix fingerprints reach PrevPfx's u64 wrap at rounds of every phase of the
+    column-group cycle: the wrapped key is outside the prefix's computed target buckets, so the
+    entry takes the fallback slot. Output and per-round list sizes equal the oracle's."""
+    rng = random.Random(77)
+    g = gen.genome(2 * 10**5, 31)
+    base = gen.reads(g, 20000, 100, 0.5, 0.005, 32)
+    crafted = [bf.wrap_read(100, ln, rng) for ln in (70, 75, 80, 81, 82, 83, 99)]
+    assert not bf.wrap_free(crafted, 63)
+    r = np.concatenate([base, np.frombuffer("".join(crafted).encode(), np.uint8).reshape(-1, 100)])
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), r.shape[0], 100, min_ovlp=63)
+    assert rc == 0
+    with native.Context(min_ovlp=63) as c:
+        c.add_reads(r)
+        got, rounds = c.run(), c.rounds()
+    assert got == want
+    assert [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in rounds] == \
+           [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in orounds]
+
+
 def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
     """2,000,000 x 100 bp reads (4 M entries, 12-bit buckets): the back-to-back rounds (each join
     clears the counts it read) and the rounds with the host between them give the oracle's bytes

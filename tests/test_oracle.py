@@ -93,6 +93,17 @@ def test_prev_pfx_invariant_and_wrap():
     assert ol.prev_pfx((1 << 32) | 100, "G") >> 32 == 214748939
 
 
+def test_wrap_read_steps_through_the_wrap():
+    """bf.wrap_read's prefix of length ln has H1 below its last base's encoding, so the oracle's
+    PrevPfx step from it takes the wrap and leaves the true key of the shorter prefix."""
+    rng = random.Random(5)
+    for ln in (40, 63, 80, 99):
+        r = bf.wrap_read(100, ln, rng)
+        k = bf.key(r[:ln])
+        assert (k >> 32) < bf.enc(r[ln - 1])
+        assert ol.prev_pfx(k, r[ln - 1]) >> 32 != bf.key(r[:ln - 1]) >> 32
+
+
 FQ = lambda seqs: b"".join(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)) for i, s in enumerate(seqs))
 
 
