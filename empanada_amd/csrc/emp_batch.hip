@@ -38,7 +38,7 @@ __device__ __forceinline__ uint32_t bgroup_codes(const uint32_t *__restrict__ co
 __device__ __forceinline__ uint32_t bcarry_after(const uint32_t *__restrict__ cols, const BJob &J, uint32_t rid,
                                                  bool sfx) {
     const bool fwd = sfx != (bool)(rid & 1);
-    const uint32_t col = fwd ? J.rg.col_t : J.rg.col_back;
+    const uint32_t col = round_col(J.rg, fwd);
     return carry_of(bgroup_codes(cols, J, col >> 2, rid >> 1), col, fwd);
 }
 

@@ -36,11 +36,21 @@ __device__ __forceinline__ uint32_t carry_of(uint32_t gw, uint32_t col, bool fwd
 
 // Regenerates e for round r (suffix rule: NextSfx, else PrevPfx) from the gathered group word
 // (0xFFFFFFFF: none, the carried codes supply the base) and returns the base's encoding (1..4).
+// Column that round r consumes for an entry read forward (column t) or backward (column L-1-t).
+// r is the same for every lane (kernel arguments, or one job's per work-group); taking both
+// fields into scalar registers first makes the choice a select of values. A select of the two
+// fields' addresses is what the compiler makes otherwise: a load from the argument per lane,
+// waited for inside every regeneration.
+__device__ __forceinline__ uint32_t round_col(const RoundArgs &r, bool fwd) {
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(r.col_t), cb = __builtin_amdgcn_readfirstlane(r.col_back);
+    return fwd ? ct : cb;
+}
+
 __device__ __forceinline__ uint32_t regen_carry(const uint32_t *__restrict__ cols, const RoundArgs &r, Entry &e, bool sfx,
                                                 uint32_t gathered) {
     const bool odd = e.rid & 1;
     const bool fwd = sfx != odd;
-    const uint32_t col = fwd ? r.col_t : r.col_back;
+    const uint32_t col = round_col(r, fwd);
     uint32_t f = (e.h1 >> 29) | ((e.h2 >> 29) << 3), code;
     if (gathered != 0xFFFFFFFFu) {
         code = (gathered >> (2 * (col & 3u))) & 3u;
@@ -68,7 +78,7 @@ __device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
 // Whether the entry must gather its group word this round, and which word (group index).
 __device__ __forceinline__ bool regen_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
     const bool fwd = sfx != (bool)(rid & 1);
-    const uint32_t col = fwd ? r.col_t : r.col_back;
+    const uint32_t col = round_col(r, fwd);
     g = col >> 2;
     return fwd ? (col & 3u) == 0 : ((col & 3u) == 3 || r.col_t == 0);
 }

@@ -30,10 +30,14 @@
 namespace emp {
 
 // ---------------------------------------------------------------------------------------------
+// The word of the column store holding read k's 4 codes of column group g (4 reads per word).
+__device__ __forceinline__ size_t group_word(uint32_t G, uint32_t g, uint32_t k) {
+    return ((size_t)(k >> CHUNK_SHIFT) * G + g) * CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
+}
+
 // The 4 codes (8 bits, column 4g + c at bits 2c) of read k in column group g.
 __device__ __forceinline__ uint32_t group_codes(const uint32_t *__restrict__ cols, uint32_t G, uint32_t g, uint32_t k) {
-    const size_t w = ((size_t)(k >> CHUNK_SHIFT) * G + g) * CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
-    return (cols[w] >> (8 * (k & 3))) & 0xFFu;
+    return (cols[group_word(G, g, k)] >> (8 * (k & 3))) & 0xFFu;
 }
 
 // The carried-code field of a live entry right after its regeneration for round r (recomputed
@@ -41,7 +45,7 @@ __device__ __forceinline__ uint32_t group_codes(const uint32_t *__restrict__ col
 __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ cols, const RoundArgs &r, uint32_t rid,
                                                 bool sfx) {
     const bool fwd = sfx != (bool)(rid & 1);
-    const uint32_t col = fwd ? r.col_t : r.col_back;
+    const uint32_t col = round_col(r, fwd);
     return carry_of(group_codes(cols, r.G, col >> 2, rid >> 1), col, fwd);
 }
 
@@ -100,31 +104,45 @@ __device__ __forceinline__ void next_put(const NextOut &nx, uint32_t side, uint3
 // fallback slot: its position in the target bucket).
 __device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
 
-// Gather step for register-resident entries e[q] (bit q of mask) of side `side`: tgt[q] = the
-// column-group word of entries that enter a group this round (one round in four), else 0x100.
-// Rounds where no lane of the wave needs a word skip the loads; otherwise all the wave's gathers
-// are in flight together (lanes without one read the store's first word, a cached line). The
-// caller may issue several sides' gathers before it waits for any.
+// Gather step for register-resident entries e[q] (bit q of mask) of side `side`, in two halves.
+// The issue half loads, into tgt[q], the column-store word of each entry that enters a column
+// group this round (one round in four) and returns those entries as a mask; the finish half turns
+// the words into tgt form (the read's 8 bits, or bit 8 set: no word). Between the two the caller
+// issues the other side's gathers, so that all of them are in flight together. Rounds where no
+// lane of the wave needs a word skip the loads; otherwise lanes without one read the store's first
+// word (a cached line). The finish half uses every loaded word whether it is needed or not: a
+// word used only under its condition would let the compiler move each load into a branch of its
+// own and wait for each in turn.
 template <int PER>
-__device__ __forceinline__ void next_gather(const NextOut &nx, const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER],
-                                            uint32_t side) {
+__device__ __forceinline__ uint32_t next_gather(const NextOut &nx, const Entry (&e)[PER], uint32_t mask, uint32_t (&tgt)[PER],
+                                                uint32_t side) {
     const bool sfx = side == 0;
-    bool need[PER];
-    uint32_t grp[PER], any = 0;
+    uint32_t nmask = 0, grp[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         grp[q] = 0;
-        need[q] = ((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, sfx, grp[q]);
-        any |= need[q];
-        tgt[q] = 0x100u;
+        if (((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, sfx, grp[q])) nmask |= 1u << q;
+        tgt[q] = 0u;
     }
-    if (__any(any)) {
+    if (__any(nmask != 0)) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t w = group_codes(nx.cols, nx.rg.G, need[q] ? grp[q] : 0u, need[q] ? e[q].rid >> 1 : 0u);
-            if (need[q]) tgt[q] = w;
+            const bool need = (nmask >> q) & 1u;
+            tgt[q] = nx.cols[need ? group_word(nx.rg.G, grp[q], e[q].rid >> 1) : 0];
         }
     }
+    return nmask;
+}
+
+// The read's byte of a loaded word (need), else the word with bit 8 set.
+__device__ __forceinline__ uint32_t gathered(uint32_t w, uint32_t rid, bool need) {
+    return need ? (w >> (8 * ((rid >> 1) & 3u))) & 0xFFu : w | 0x100u;
+}
+
+template <int PER>
+__device__ __forceinline__ void next_gather_finish(const Entry (&e)[PER], uint32_t nmask, uint32_t (&tgt)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) tgt[q] = gathered(tgt[q], e[q].rid, (nmask >> q) & 1u);
 }
 
 // An entry's rank in its slot (table slots: LDS counter of the wave), or for the two fallback slots
@@ -507,8 +525,10 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
                 }
             }
         } else {
-            next_gather<PER>(a.nx, kept, keepmask, ts, 0);
-            next_gather<PER>(a.nx, kept, keepmask, tp, 1);
+            const uint32_t gs = next_gather<PER>(a.nx, kept, keepmask, ts, 0);
+            const uint32_t gp = next_gather<PER>(a.nx, kept, keepmask, tp, 1);
+            next_gather_finish<PER>(kept, gs, ts);
+            next_gather_finish<PER>(kept, gp, tp);
         }
         Entry kp[PER];  // the prefix side's copy (each side is regenerated in place)
 #pragma unroll
@@ -556,26 +576,30 @@ __device__ __forceinline__ void load_bucket(const JoinArgs &a, uint32_t b, uint3
 // Prefix survivors of the join (in LDS, slot threadIdx.x + NT * q for bit q of pmask): the gather
 // and rank steps of next_gather / next_rank with the entries read from LDS.
 template <int CAP, int MCAP, int NT>
-__device__ __forceinline__ void pfx_gather(const NextOut &nx, const JoinShared<CAP, MCAP, NT> &S, uint32_t pmask,
-                                           uint32_t (&tgt)[CAP / NT]) {
+__device__ __forceinline__ uint32_t pfx_gather(const NextOut &nx, const JoinShared<CAP, MCAP, NT> &S, uint32_t pmask,
+                                               uint32_t (&tgt)[CAP / NT]) {
     constexpr int PER = CAP / NT;
-    bool need[PER];
-    uint32_t grp[PER], any = 0;
+    uint32_t nmask = 0, grp[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         grp[q] = 0;
-        need[q] = ((pmask >> q) & 1u) && regen_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q]);
-        any |= need[q];
-        tgt[q] = 0x100u;
+        if (((pmask >> q) & 1u) && regen_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q])) nmask |= 1u << q;
+        tgt[q] = 0u;
     }
-    if (__any(any)) {
+    if (__any(nmask != 0)) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t w = group_codes(nx.cols, nx.rg.G, need[q] ? grp[q] : 0u,
-                                           need[q] ? S.prid[threadIdx.x + NT * q] >> 1 : 0u);
-            if (need[q]) tgt[q] = w;
+            const bool need = (nmask >> q) & 1u;
+            tgt[q] = nx.cols[need ? group_word(nx.rg.G, grp[q], S.prid[threadIdx.x + NT * q] >> 1) : 0];
         }
     }
+    return nmask;
+}
+
+template <int CAP, int MCAP, int NT>
+__device__ __forceinline__ void pfx_gather_finish(const JoinShared<CAP, MCAP, NT> &S, uint32_t nmask, uint32_t (&tgt)[CAP / NT]) {
+#pragma unroll
+    for (int q = 0; q < CAP / NT; ++q) tgt[q] = gathered(tgt[q], S.prid[threadIdx.x + NT * q], (nmask >> q) & 1u);
 }
 
 template <int CAP, int MCAP, int NT>
@@ -713,8 +737,10 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     uint32_t *row = S.u.m.wcnt[threadIdx.x >> 6];
     if (a.nx.active) {
         if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
-        next_gather<PER>(a.nx, se, smask, tgt, 0);  // both sides' gathers in flight together
-        pfx_gather<CAP, MCAP, NT>(a.nx, S, pmask, ptg);
+        const uint32_t gs = next_gather<PER>(a.nx, se, smask, tgt, 0);  // both sides' gathers in flight
+        const uint32_t gp = pfx_gather<CAP, MCAP, NT>(a.nx, S, pmask, ptg);
+        next_gather_finish<PER>(se, gs, tgt);
+        pfx_gather_finish<CAP, MCAP, NT>(S, gp, ptg);
         next_rank<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0);
         pfx_rank<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg);
     } else {  // last round: only counted
