@@ -69,33 +69,23 @@ __host__ __device__ __forceinline__ uint32_t enc_of_code(uint32_t code) { return
 __host__ __device__ __forceinline__ uint32_t step1(uint32_t h, uint32_t e) { return red1(h * R1 + e); }
 __host__ __device__ __forceinline__ uint32_t step2(uint32_t h, uint32_t e) { return red2(h * R2 + e); }
 
-// Horner steps of the map kernel with a partial reduction: for h < 2^29 + 231 and e <= 4,
-// h*R + e < 2^32, and y = (x >> 29)*c + (x & (2^29 - 1)) is congruent to x mod 2^29 - c and below
-// 2^29 + 7c <= 2^29 + 231, so the chain stays in range without a compare per step; fin1 / fin2
-// bring the result into [0, P). Five full-rate ALU operations per step: the multiplications by
-// 5 and 7 are a shift-and-add, the reduction a 24-bit multiply-add.
-// (a << s) + b as one instruction (written out so that the compiler does not fold it into a
-// quarter-rate 32-bit multiplication).
-__device__ __forceinline__ uint32_t lshl_add(uint32_t a, uint32_t b, int s) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t r;
-    if (s == 2) asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(r) : "v"(a), "v"(b));
-    else asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-#else
-    return (a << s) + b;
-#endif
+// Fingerprints as byte dot products (the map kernel): with W_i = R^(L-1-i) mod P split into bytes
+// W_i = sum_j b_ij 256^j, H = sum_i e_i W_i = sum_j S_j 256^j (mod P) for S_j = sum_i e_i b_ij.
+// For L <= 255, e <= 4 and b <= 255, S_j < 2^18 and x = sum_j S_j 256^j < 2^43; fold1 / fold2
+// reduce x with 2^29 = C (mod 2^29 - C): (x >> 29) C + (x mod 2^29) < 2^29 + 2^20 < 2P, so one
+// conditional subtraction ends in [0, P).
+__host__ __device__ __forceinline__ uint32_t fold_bytes(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, uint32_t c,
+                                                        uint32_t p) {
+    const uint64_t x = (uint64_t)s0 + ((uint64_t)s1 << 8) + ((uint64_t)s2 << 16) + ((uint64_t)s3 << 24);
+    const uint32_t y = (uint32_t)(x >> 29) * c + (uint32_t)(x & MASK29);
+    return y >= p ? y - p : y;
 }
-__device__ __forceinline__ uint32_t lazy_step1(uint32_t h, uint32_t e) {
-    const uint32_t x = lshl_add(h, h + e, 2);  // 5h + e
-    return __umul24(x >> 29, C1) + (x & MASK29);
+__host__ __device__ __forceinline__ uint32_t fold1(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3) {
+    return fold_bytes(s0, s1, s2, s3, C1, P1);
 }
-__device__ __forceinline__ uint32_t lazy_step2(uint32_t h, uint32_t e) {
-    const uint32_t x = lshl_add(h, e, 3) - h;  // 7h + e
-    return __umul24(x >> 29, C2) + (x & MASK29);
+__host__ __device__ __forceinline__ uint32_t fold2(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3) {
+    return fold_bytes(s0, s1, s2, s3, C2, P2);
 }
-__host__ __device__ __forceinline__ uint32_t fin1(uint32_t y) { return y >= P1 ? y - P1 : y; }
-__host__ __device__ __forceinline__ uint32_t fin2(uint32_t y) { return y >= P2 ? y - P2 : y; }
 
 // NextSfx (overlap.cu:57-75): drop the first base of a length-l suffix. m1 = R1^(l-1) mod P1,
 // m2 = R2^(l-1) mod P2. e*m < 4*2^29, so the 32-bit product is exact as in the reference.

@@ -113,6 +113,7 @@ struct MapArgs {
     // non-null: per read k, the codes of its first and last column groups (group 0 in bits 0-7,
     // group ceil(L/4) - 1 in bits 8-15): everything round 0's regeneration reads from the columns
     uint16_t *ends;
+    const uint32_t *wt;      // set by launch_map: the dot-product weights for L
 };
 
 // One partition pass: input groups (keys sharing their top bits) -> output buckets refined by

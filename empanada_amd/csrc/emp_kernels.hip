@@ -27,6 +27,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 namespace emp {
 
 // ---------------------------------------------------------------------------------------------
@@ -245,6 +248,17 @@ __device__ __forceinline__ uint32_t bad_bases(uint32_t w) {
            nz_bytes(x ^ 0x74747474u);
 }
 
+// The 2-bit codes of the 4 bases in dword v of a row (column 4g + c at bits 2c), columns past the
+// read cleared by keep.
+__device__ __forceinline__ uint32_t pack_codes(uint32_t v, uint32_t keep) {
+    const uint32_t c = (v >> 1) & 0x03030303u;
+    return (c | (c >> 6) | (c >> 12) | (c >> 18)) & keep;
+}
+__device__ __forceinline__ uint32_t group_keep(uint32_t g, int L) {
+    const int rest = L - 4 * (int)g;
+    return rest >= 4 ? 0xFFu : (1u << (2 * rest)) - 1u;
+}
+
 constexpr int MAP_Q = 7;  // 16-byte loads per thread in flight (a 256 x 100 bp block is 1,600)
 
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
@@ -298,55 +312,37 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     if (bad) atomicOr(a.err, 1u);
     lds_barrier();
 
-    // Fingerprints of the read and of its reverse complement, four Horner chains side by side,
-    // the row read four bases at a time (the reverse complement takes the row from its end).
+    // Fingerprints of the read and of its reverse complement as byte dot products, four bases per
+    // instruction: the 2-bit codes of a dword of the row give the encodings e (the read) and e'
+    // (their complements, the reverse complement's bases in reverse order); a.wt holds the bytes
+    // of the position weights, 16 dwords per 4 positions (map_weights), read as scalars.
     const uint32_t t = threadIdx.x;
+    const int ng = (L + 3) >> 2;
     if (t < nt) {
-        const uint8_t *row = tile + t * Lp;
-        const uint32_t *row4 = reinterpret_cast<const uint32_t *>(row);
-        const int nd = L >> 2, tl = L & 3;
-        uint32_t f1 = 0, f2 = 0, r1 = 0, r2 = 0;
-        for (int i = tl - 1; i >= 0; --i) {  // the reverse complement starts with the last bases
-            const uint32_t ce = enc_of_code(((row[4 * nd + i] >> 1) & 3u) ^ 2u);
-            r1 = lazy_step1(r1, ce);
-            r2 = lazy_step2(r2, ce);
-        }
-        for (int d = 0; d < nd; ++d) {
-            const uint32_t wf = row4[d], wr = row4[nd - 1 - d];
+        const uint32_t *row4 = reinterpret_cast<const uint32_t *>(tile + t * Lp);
+        const __attribute__((address_space(4))) uint32_t *wt =
+            (const __attribute__((address_space(4))) uint32_t *)a.wt;
+        uint32_t sd[16];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t e = enc_of_code((wf >> (8 * i + 1)) & 3u);
-                f1 = lazy_step1(f1, e);
-                f2 = lazy_step2(f2, e);
-                const uint32_t ce = enc_of_code(((wr >> (8 * (3 - i) + 1)) & 3u) ^ 2u);
-                r1 = lazy_step1(r1, ce);
-                r2 = lazy_step2(r2, ce);
+        for (int i = 0; i < 16; ++i) sd[i] = 0;
+        for (int d = 0; d < ng; ++d) {
+            const uint32_t c = (row4[d] >> 1) & 0x03030303u;  // bytes past L: weight 0
+            const uint32_t e = c + 0x01010101u, ec = (c ^ 0x02020202u) + 0x01010101u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                sd[j] = __builtin_amdgcn_udot4(e, wt[16 * d + j], sd[j], false);
+                sd[4 + j] = __builtin_amdgcn_udot4(e, wt[16 * d + 4 + j], sd[4 + j], false);
+                sd[8 + j] = __builtin_amdgcn_udot4(ec, wt[16 * d + 8 + j], sd[8 + j], false);
+                sd[12 + j] = __builtin_amdgcn_udot4(ec, wt[16 * d + 12 + j], sd[12 + j], false);
             }
         }
-        for (int i = 0; i < tl; ++i) {
-            const uint32_t e = enc_of_code((row[4 * nd + i] >> 1) & 3u);
-            f1 = lazy_step1(f1, e);
-            f2 = lazy_step2(f2, e);
-        }
-        f1 = fin1(f1);
-        f2 = fin2(f2);
-        r1 = fin1(r1);
-        r2 = fin2(r2);
         uint64_t k = a.k0 + first + t;
-        Entry ef = {f1, f2, (uint32_t)(2 * k)};
-        Entry er = {r1, r2, (uint32_t)(2 * k + 1)};
+        Entry ef = {fold1(sd[0], sd[1], sd[2], sd[3]), fold2(sd[4], sd[5], sd[6], sd[7]), (uint32_t)(2 * k)};
+        Entry er = {fold1(sd[8], sd[9], sd[10], sd[11]), fold2(sd[12], sd[13], sd[14], sd[15]), (uint32_t)(2 * k + 1)};
         a.entries[2 * k] = ef;
         a.entries[2 * k + 1] = er;
-        if (a.ends) {
-            const uint32_t gl = 4u * (col_groups((uint32_t)L) - 1u);
-            uint32_t lo = 0, hi = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < 4; ++c) {
-                if (c < (uint32_t)L) lo |= ((uint32_t)(row[c] >> 1) & 3u) << (2 * c);
-                if (gl + c < (uint32_t)L) hi |= ((uint32_t)(row[gl + c] >> 1) & 3u) << (2 * c);
-            }
-            a.ends[k] = (uint16_t)(lo | (hi << 8));
-        }
+        if (a.ends)
+            a.ends[k] = (uint16_t)(pack_codes(row4[0], group_keep(0, L)) | (pack_codes(row4[ng - 1], group_keep(ng - 1, L)) << 8));
     }
 
     // Columns: word w of the tile covers reads 4w..4w+3; task id = g * nw + w keeps a wave on
@@ -358,15 +354,9 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
         const uint32_t g = task / nw, w = task - g * nw;
         uint32_t word = 0;
-        const uint32_t rmax = min(4u, nt - 4 * w);
-        for (uint32_t i = 0; i < rmax; ++i) {
-            const uint8_t *row = tile + (4 * w + i) * Lp;
-#pragma unroll
-            for (uint32_t c = 0; c < 4; ++c) {
-                const uint32_t j = 4 * g + c;
-                if (j < (uint32_t)L) word |= ((uint32_t)(row[j] >> 1) & 3u) << (2 * (4 * i + c));
-            }
-        }
+        const uint32_t rmax = min(4u, nt - 4 * w), keep = group_keep(g, L);
+        for (uint32_t i = 0; i < rmax; ++i)
+            word |= pack_codes(reinterpret_cast<const uint32_t *>(tile + (4 * w + i) * Lp)[g], keep) << (8 * i);
         const uint64_t W = w0 + w;
         if (a.gs)
             a.cols[a.cbase + (uint64_t)g * a.gs + W] = word;
@@ -1214,11 +1204,54 @@ __global__ void __launch_bounds__(256) k_roll(const uint64_t *__restrict__ keys,
 
 // ---------------------------------------------------------------------------------------------
 // Launchers.
+// Weights of the map kernel's dot products for read length L (1..255): dword 16 d + 4 h + j holds,
+// in byte u, byte j of the weight of position k = 4d + u (0 past the read), for h = 0: R1^(L-1-k)
+// mod P1 and h = 1: R2^(L-1-k) mod P2 (the read), h = 2: R1^k mod P1 and h = 3: R2^k mod P2 (its
+// reverse complement: its base L-1-k is the complement of the read's base k). One table per
+// device and length, built at its first use and kept until the process ends.
+static hipError_t map_weights(int L, const uint32_t **out) {
+    static std::mutex mu;
+    static const uint32_t *tab[64][256];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64 || L < 1 || L > 255) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!tab[dev][L]) {
+        const int ng = (L + 3) / 4;
+        std::vector<uint32_t> h((size_t)ng * 16, 0u);
+        std::vector<uint64_t> p1(L), p2(L);
+        p1[0] = p2[0] = 1;
+        for (int k = 1; k < L; ++k) {
+            p1[k] = p1[k - 1] * R1 % P1;
+            p2[k] = p2[k - 1] * R2 % P2;
+        }
+        for (int k = 0; k < L; ++k) {
+            const uint64_t w[4] = {p1[L - 1 - k], p2[L - 1 - k], p1[k], p2[k]};
+            for (int q = 0; q < 4; ++q)
+                for (int j = 0; j < 4; ++j)
+                    h[16 * (k / 4) + 4 * q + j] |= (uint32_t)((w[q] >> (8 * j)) & 0xFFu) << (8 * (k % 4));
+        }
+        void *d = nullptr;
+        if ((e = hipMalloc(&d, h.size() * 4)) != hipSuccess) return e;
+        if ((e = hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+            (void)hipFree(d);
+            return e;
+        }
+        tab[dev][L] = static_cast<const uint32_t *>(d);
+    }
+    *out = tab[dev][L];
+    return hipSuccess;
+}
+
 hipError_t launch_map(const MapArgs &a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
+    MapArgs b = a;
+    hipError_t e = map_weights(a.L, &b.wt);
+    if (e != hipSuccess) return e;
     uint32_t grid = (a.n + a.rows - 1) / a.rows;
     size_t lds = (size_t)a.rows * a.Lpad;
-    hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, b);
     return hipGetLastError();
 }
 
