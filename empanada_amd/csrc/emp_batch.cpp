@@ -122,6 +122,7 @@ struct HJob {
 struct BCounters {
     unsigned long long pool[SUBS];
     unsigned long long pool_ovf;
+    unsigned long long pool_lost;  // abandoned sub-range reservations (place_edges)
     uint32_t n_big, abort, spill[2];
 };
 
@@ -571,7 +572,7 @@ int zero_step(emp_batch *b, const StepLayout &sl, bool part, bool dedup) {
     }
     add(b->d_lcnt, 3 * b->tab_cap);
     add(&b->d_ctr->n_big, 1);
-    add((uint32_t *)b->d_ctr->pool, 2 * (SUBS + 1));
+    add((uint32_t *)b->d_ctr->pool, 2 * (SUBS + 2));
     BCHK(b, launch_zero(z, b->stream));
     return EMP_OK;
 }

@@ -164,13 +164,17 @@ __device__ inline void bitonic(uint64_t *key, uint32_t *rank, uint32_t *pay, uin
 }
 
 // Edge-pool slot of n edges whose sub-range reservation returned c (atomicAdd on the cursor
-// of sub-range `sub`): inside the sub-range when it fits, else the reservation is returned and
-// the edges go to the overflow region; ~0 when that is full too (the host grows it and reruns).
+// of sub-range `sub`): inside the sub-range when it fits, else the edges go to the overflow
+// region; ~0 when that is full too (the host grows it and reruns). A reservation past the end is
+// abandoned, not returned: the cursor only grows (moving it back could let a later reservation
+// overlap one handed out in between); ovf_cursor[1] counts the abandoned edges, which the host
+// subtracts when it counts edges from the cursors.
 __device__ __forceinline__ unsigned long long place_edges(unsigned long long c, uint32_t n, uint32_t sub,
                                                           unsigned long long *cursor, unsigned long long cap,
                                                           unsigned long long *ovf_cursor, unsigned long long ovf_cap) {
+    (void)cursor;
     if (c + n <= cap) return (unsigned long long)sub * cap + c;
-    atomicAdd(cursor, 0ull - (unsigned long long)n);
+    atomicAdd(ovf_cursor + 1, (unsigned long long)n);
     const unsigned long long o = atomicAdd(ovf_cursor, (unsigned long long)n);
     return o + n <= ovf_cap ? (unsigned long long)SUBS * cap + o : ~0ull;
 }

@@ -45,6 +45,7 @@ constexpr int MAX_ROUNDS = 256;  // L - o <= 255
 struct DevCounters {
     unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
     unsigned long long pool_ovf;    // cursor of the pool's overflow region
+    unsigned long long pool_lost;   // edges of sub-range reservations abandoned (past the end)
     uint32_t abort;                 // the dedup partition overflowed an intermediate bucket or its spill
     uint32_t spill;                 // records spilled by the dedup partition's final pass
     uint32_t in;                    // dedup partition input size, set by the host
@@ -1313,7 +1314,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
     st.bbS[0] = st.bbS[1] = bbD;
     if ((rc = ensure_pool(c, std::max<uint64_t>(4096, N0 / SUBS), std::max<uint64_t>(1u << 20, N0 / 8), false)))
         return rc;
-    c->pool_prev.assign(SUBS + 1, 0ull);  // sub-range cursors, overflow cursor
+    c->pool_prev.assign(SUBS + 2, 0ull);  // sub-range cursors, overflow cursor, abandoned edges
     if (st.rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)st.rounds * st.stride))) return rc;
     const RoundArgs none{};
     const RoundArgs rg0 = st.rounds > 0 ? round_args(c, 0, true) : none;
@@ -1324,7 +1325,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
             ZeroList z;
             if (need_part) zero_partition(c, z, N0, bbD);
             zero_next(c, z, 0, bbD);
-            z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 1));  // edge-pool cursors
+            z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 2));  // edge-pool cursors
             if ((rc = run_zero(c, z))) return rc;
         }
         if (need_part && (rc = partition(c, N0, bbD))) return rc;
@@ -1401,7 +1402,7 @@ int rounds_sync(emp_ctx *c, RunState &st) {
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
             if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
-                HIPCHK(c, hipMemcpyAsync(c->d_ctr->pool, c->pool_prev.data(), (SUBS + 1) * sizeof(unsigned long long),
+                HIPCHK(c, hipMemcpyAsync(c->d_ctr->pool, c->pool_prev.data(), (SUBS + 2) * sizeof(unsigned long long),
                                          hipMemcpyHostToDevice, c->stream));
             {
                 ZeroList z;
@@ -1451,9 +1452,13 @@ int rounds_sync(emp_ctx *c, RunState &st) {
             break;
         }
         uint64_t E = 0;
-        for (int i = 0; i <= SUBS; ++i) {
-            const unsigned long long v = i < SUBS ? c->h_ctr->pool[i] : c->h_ctr->pool_ovf;
-            E += v - c->pool_prev[i];
+        for (int i = 0; i <= SUBS + 1; ++i) {  // placed = reserved - abandoned
+            const unsigned long long v =
+                i < SUBS ? c->h_ctr->pool[i] : i == SUBS ? c->h_ctr->pool_ovf : c->h_ctr->pool_lost;
+            if (i <= SUBS)
+                E += v - c->pool_prev[i];
+            else
+                E -= v - c->pool_prev[i];
             c->pool_prev[i] = v;
         }
         const DevCounters &k = *c->h_ctr;
