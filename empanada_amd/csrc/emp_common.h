@@ -75,6 +75,59 @@ __device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
     e.h2 = (e.h2 & KEY_MASK) | (((f >> 3) & 7u) << 29);
 }
 
+// Wide carry (the single-job engine's round buckets, 14 bits: see WIDE_SHIFT). An entry gathers its
+// read's 16-bit unit of 8 columns when it enters a group: every eighth round per direction.
+__device__ __forceinline__ uint32_t wide_carry_of(uint32_t u, uint32_t col, bool fwd) {
+    const uint32_t c = col & 7u;
+    if (fwd) return (u & 0xFFFFu) >> (2 * (c + 1));
+    uint32_t f = 0;
+    for (uint32_t i = 0; i < c; ++i) f |= ((u >> (2 * (c - 1 - i))) & 3u) << (2 * i);
+    return f;
+}
+
+__device__ __forceinline__ bool wide_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
+    const bool fwd = sfx != (bool)(rid & 1);
+    const uint32_t col = round_col(r, fwd);
+    g = col >> 3;
+    return fwd ? (col & 7u) == 0 : ((col & 7u) == 7 || r.col_t == 0);
+}
+
+// The 14-bit carry of a stored entry, and its h1 key with bits 21..28 restored from top8 (the top
+// 8 bits of its bucket's index).
+__device__ __forceinline__ uint32_t wide_carry(const Entry &e) {
+    return (e.h1 >> 29) | ((e.h2 >> 29) << 3) | (((e.h1 >> WIDE_SHIFT) & 0xFFu) << 6);
+}
+__device__ __forceinline__ uint32_t wide_key1(uint32_t h1, uint32_t top8) { return (h1 & LOW_MASK) | (top8 << WIDE_SHIFT); }
+
+// Regenerates keys h1, h2 (this round's, bits 0..28) of oriented read rid for round r and its carry
+// f (in: this round's, out: the next round's), from the gathered 16-bit unit (0xFFFFFFFF: none, the
+// carry supplies the base). Returns the base's encoding (1..4).
+__device__ __forceinline__ uint32_t regen_wide(const RoundArgs &r, uint32_t &h1, uint32_t &h2, uint32_t rid, uint32_t &f,
+                                               bool sfx, uint32_t gathered) {
+    const bool odd = rid & 1;
+    const bool fwd = sfx != odd;
+    const uint32_t col = round_col(r, fwd);
+    uint32_t code;
+    if (gathered != 0xFFFFFFFFu) {
+        code = (gathered >> (2 * (col & 7u))) & 3u;
+        f = wide_carry_of(gathered, col, fwd);
+    } else {
+        code = f & 3u;
+        f >>= 2;
+    }
+    const uint32_t ev = enc_of_code(odd ? (code ^ 2u) : code);
+    if (sfx)
+        next_sfx(h1, h2, ev, r.m1, r.m2);
+    else
+        prev_pfx(h1, h2, ev);
+    return ev;
+}
+
+// Stored form of an entry with keys h1, h2 (bits 0..28) and carry f.
+__device__ __forceinline__ Entry wide_entry(uint32_t h1, uint32_t h2, uint32_t rid, uint32_t f) {
+    return Entry{(h1 & LOW_MASK) | (((f >> 6) & 0xFFu) << WIDE_SHIFT) | ((f & 7u) << 29), h2 | (((f >> 3) & 7u) << 29), rid};
+}
+
 // Whether the entry must gather its group word this round, and which word (group index).
 __device__ __forceinline__ bool regen_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
     const bool fwd = sfx != (bool)(rid & 1);
@@ -273,13 +326,19 @@ __device__ __forceinline__ bool has_u32(const uint32_t *r, uint32_t lo, uint32_t
 
 // Collects the entries of bucket b (slots + spilled records) into key/value scratch, padded with
 // sentinels to n2. as_rank: store rid_rank(rid) (join) or rid (dedup).
+// wshift >= 0: the entries are wide (round buckets of the single-job engine, bucket = h1 >> wshift)
+// and their keys are restored from the bucket index.
 __device__ inline void gather_bucket(const Entry *slots, uint32_t cnt, uint32_t cap, const SpillRec *spill,
                               const uint32_t *spill_n, uint32_t spill_cap, uint32_t b, uint64_t *key, uint32_t *val,
-                              uint32_t n2, bool as_rank, uint32_t *s_pos) {
+                              uint32_t n2, bool as_rank, uint32_t *s_pos, int32_t wshift = -1) {
     const uint32_t nm = min(cnt, cap);
+    const uint32_t top8 = wshift >= 0 ? b >> (WIDE_SHIFT - wshift) : 0u;
+    auto kof = [&](const Entry &e) {
+        return wshift >= 0 ? ((uint64_t)wide_key1(e.h1, top8) << 32) | (e.h2 & KEY_MASK) : key_of(e);
+    };
     for (uint32_t i = threadIdx.x; i < nm; i += blockDim.x) {
         Entry e = slots[(size_t)b * cap + i];
-        key[i] = key_of(e);
+        key[i] = kof(e);
         val[i] = as_rank ? rid_rank(e.rid) : e.rid;
     }
     if (threadIdx.x == 0) *s_pos = nm;
@@ -291,7 +350,7 @@ __device__ inline void gather_bucket(const Entry *slots, uint32_t cnt, uint32_t 
                 uint32_t k = atomicAdd(s_pos, 1u);
                 if (k < cnt) {
                     Entry e = spill[j].e;
-                    key[k] = key_of(e);
+                    key[k] = kof(e);
                     val[k] = as_rank ? rid_rank(e.rid) : e.rid;
                 }
             }

@@ -31,10 +31,26 @@ constexpr uint32_t CHUNK_GWORDS = CHUNK_READS / 4;
 constexpr int GROUP_COLS = 4;
 __host__ __device__ __forceinline__ uint32_t col_groups(uint32_t L) { return (L + GROUP_COLS - 1) / GROUP_COLS; }
 
+// Wide column store (the single-job engine): columns in groups of 8, a 32-bit word holds 2 reads x
+// 8 columns: read k, column 8g + c at bits 2 * (8 * (k % 2) + c). Chunk ch, group g holds
+// CHUNK_WWORDS words starting at word (ch * G8 + g) * CHUNK_WWORDS, G8 = ceil(L / 8).
+constexpr uint32_t CHUNK_WWORDS = CHUNK_READS / 2;
+constexpr int WIDE_COLS = 8;
+__host__ __device__ __forceinline__ uint32_t col_groups8(uint32_t L) { return (L + WIDE_COLS - 1) / WIDE_COLS; }
+
 // Keys are below 2^29: bits 29..31 of h1 and h2 carry up to three base codes an entry will need
 // in its next rounds (bits 0-1 of the 6-bit field h1 >> 29 | (h2 >> 29) << 3 first). Every key
 // comparison masks them off with KEY_MASK.
 constexpr uint32_t KEY_MASK = MASK29;
+
+// Entries of the single-job engine's round buckets carry 8 more codes' bits in bits 21..28 of h1:
+// a bucket set has at least 2^WIDE_MIN_BITS buckets (bucket = h1 >> (29 - bits)), so those key
+// bits are the bucket index's top 8 and are restored from it (wide_key1). The 14-bit carry is
+// h1 >> 29 | (h2 >> 29) << 3 | ((h1 >> 21) & 0xFF) << 6, the next codes two bits each, first at
+// bits 0-1. Keys of one bucket compare on h1 & LOW_MASK.
+constexpr int WIDE_MIN_BITS = 8;
+constexpr int WIDE_SHIFT = 21;
+constexpr uint32_t LOW_MASK = (1u << WIDE_SHIFT) - 1;
 
 // One (fingerprint, oriented read id) entry, 12 bytes.
 struct Entry {

@@ -69,7 +69,7 @@ struct RoundArgs {
     uint32_t col_t;    // column t
     uint32_t col_back; // column L-1-t
     uint32_t m1, m2;   // R1^(l-1) mod P1, R2^(l-1) mod P2, l = L - t
-    uint32_t G;        // column groups of the column store, ceil(L / 4)
+    uint32_t G;        // column groups of the column store: ceil(L / 8) (wide store), ceil(L / 4) (batched)
 };
 
 // The next round's two lists (side 0 suffixes, side 1 prefixes), written by the kernel that finds
@@ -110,9 +110,9 @@ struct MapArgs {
     // group g at cbase + g * gs + k / 4 (k and entry ids local to the job)
     uint64_t cbase;
     uint32_t gs;
-    // non-null: per read k, the codes of its first and last column groups (group 0 in bits 0-7,
-    // group ceil(L/4) - 1 in bits 8-15): everything round 0's regeneration reads from the columns
-    uint16_t *ends;
+    // non-null: per read k, the codes of its first and last 8-column groups (group 0 in bits 0-15,
+    // group ceil(L/8) - 1 in bits 16-31): everything round 0's regeneration reads from the columns
+    uint32_t *ends;
     const uint32_t *wt;      // set by launch_map: the dot-product weights for L
 };
 
@@ -140,7 +140,7 @@ struct PartArgs {
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
-    const uint16_t *ends;    // per read: first and last column groups (MapArgs::ends)
+    const uint32_t *ends;    // per read: first and last column groups (MapArgs::ends)
     NextOut nx;              // the kept entries, regenerated: round 0's suffix and prefix lists
     uint32_t *big, *n_big;
     uint32_t big_cap;
@@ -222,9 +222,10 @@ struct AuditArgs {
     const SpillRec *spill;
     const uint32_t *spill_n;
     uint32_t spill_cap;
-    const uint32_t *cols;
+    const uint32_t *cols;           // the wide column store, G = ceil(L / 8) groups
     uint32_t G;
     int32_t L, start, len;
+    int32_t wshift;                 // >= 0: wide entries of buckets h1 >> wshift (a round's list)
     const uint32_t *ids;            // large-bucket launch: bucket of each work-group; else null
     const unsigned long long *off;  // large-bucket launch: scratch offset of each work-group
     uint64_t *skey;

@@ -33,23 +33,24 @@
 namespace emp {
 
 // ---------------------------------------------------------------------------------------------
-// The word of the column store holding read k's 4 codes of column group g (4 reads per word).
-__device__ __forceinline__ size_t group_word(uint32_t G, uint32_t g, uint32_t k) {
-    return ((size_t)(k >> CHUNK_SHIFT) * G + g) * CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
+// The word of the wide column store holding read k's 8 codes of column group g (2 reads per word;
+// G8 groups per chunk).
+__device__ __forceinline__ size_t wide_word(uint32_t G8, uint32_t g, uint32_t k) {
+    return ((size_t)(k >> CHUNK_SHIFT) * G8 + g) * CHUNK_WWORDS + ((k >> 1) & (CHUNK_WWORDS - 1));
 }
 
-// The 4 codes (8 bits, column 4g + c at bits 2c) of read k in column group g.
-__device__ __forceinline__ uint32_t group_codes(const uint32_t *__restrict__ cols, uint32_t G, uint32_t g, uint32_t k) {
-    return (cols[group_word(G, g, k)] >> (8 * (k & 3))) & 0xFFu;
+// The 8 codes (16 bits, column 8g + c at bits 2c) of read k in column group g.
+__device__ __forceinline__ uint32_t wide_codes(const uint32_t *__restrict__ cols, uint32_t G8, uint32_t g, uint32_t k) {
+    return (cols[wide_word(G8, g, k)] >> (16 * (k & 1))) & 0xFFFFu;
 }
 
-// The carried-code field of a live entry right after its regeneration for round r (recomputed
-// from its read's group word; used where an entry is rebuilt from its key alone).
+// The carry of a live entry of round r (the codes it holds after that round's regeneration),
+// recomputed from its read's unit: used where an entry is rebuilt from its key alone.
 __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ cols, const RoundArgs &r, uint32_t rid,
                                                 bool sfx) {
     const bool fwd = sfx != (bool)(rid & 1);
     const uint32_t col = round_col(r, fwd);
-    return carry_of(group_codes(cols, r.G, col >> 2, rid >> 1), col, fwd);
+    return wide_carry_of(wide_codes(cols, r.G, col >> 3, rid >> 1), col, fwd);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -102,15 +103,16 @@ __device__ __forceinline__ void next_put(const NextOut &nx, uint32_t side, uint3
     }
 }
 
-// Between count and write an entry is kept as one word: its gathered codes (9 bits, 0x100 =
-// none), its slot (5 bits) and, from bit 14, its rank among the wave's entries of the slot (for a
+// Between gather and rank an entry's word is its gathered unit (16 bits; bit 16 set: none). After
+// the rank it holds the entry's carry bits 6..13 (bits 0-7, stored in h1 bits 21..28 when written),
+// its slot (bits 9-13) and, from bit 14, its rank among the wave's entries of the slot (for a
 // fallback slot: its position in the target bucket).
-__device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x100u) ? 0xFFFFFFFFu : (tg & 0xFFu); }
+__device__ __forceinline__ uint32_t tgt_word(uint32_t tg) { return (tg & 0x10000u) ? 0xFFFFFFFFu : (tg & 0xFFFFu); }
 
 // Gather step for register-resident entries e[q] (bit q of mask) of side `side`, in two halves.
-// The issue half loads, into tgt[q], the column-store word of each entry that enters a column
-// group this round (one round in four) and returns those entries as a mask; the finish half turns
-// the words into tgt form (the read's 8 bits, or bit 8 set: no word). Between the two the caller
+// The issue half loads, into tgt[q], the wide-store word of each entry that enters a column group
+// this round (one round in eight per direction) and returns those entries as a mask; the finish
+// half turns the words into tgt form (the read's 16 bits, or bit 16 set: no word). Between the two the caller
 // issues the other side's gathers, so that all of them are in flight together. Rounds where no
 // lane of the wave needs a word skip the loads; otherwise lanes without one read the store's first
 // word (a cached line). The finish half uses every loaded word whether it is needed or not: a
@@ -124,22 +126,22 @@ __device__ __forceinline__ uint32_t next_gather(const NextOut &nx, const Entry (
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         grp[q] = 0;
-        if (((mask >> q) & 1u) && regen_needs_gather(nx.rg, e[q].rid, sfx, grp[q])) nmask |= 1u << q;
+        if (((mask >> q) & 1u) && wide_needs_gather(nx.rg, e[q].rid, sfx, grp[q])) nmask |= 1u << q;
         tgt[q] = 0u;
     }
     if (__any(nmask != 0)) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const bool need = (nmask >> q) & 1u;
-            tgt[q] = nx.cols[need ? group_word(nx.rg.G, grp[q], e[q].rid >> 1) : 0];
+            tgt[q] = nx.cols[need ? wide_word(nx.rg.G, grp[q], e[q].rid >> 1) : 0];
         }
     }
     return nmask;
 }
 
-// The read's byte of a loaded word (need), else the word with bit 8 set.
+// The read's 16 bits of a loaded word (need), else the word with bit 16 set.
 __device__ __forceinline__ uint32_t gathered(uint32_t w, uint32_t rid, bool need) {
-    return need ? (w >> (8 * ((rid >> 1) & 3u))) & 0xFFu : w | 0x100u;
+    return need ? (w >> (16 * ((rid >> 1) & 1u))) & 0xFFFFu : w | 0x10000u;
 }
 
 template <int PER>
@@ -158,29 +160,51 @@ __device__ __forceinline__ uint32_t slot_rank(const NextOut &nx, uint32_t *row, 
     return pos < (uint32_t)BUCKET_CAP ? pos : (uint32_t)BUCKET_CAP;
 }
 
+// Regenerates entry e of this round in place: in, its stored form (wide: key bits 21..28 restored
+// from top8, the top 8 bits of its bucket's index) or, from the dedup (wide false), its full keys
+// with no carry; out, the next round's keys in bits 0..28 with carry bits 0..5 above them. Returns
+// the removed base's encoding; x and h1o get the carry bits 6..13 and the old h1 key.
+__device__ __forceinline__ uint32_t regen_entry(const RoundArgs &r, Entry &e, bool wide, uint32_t top8, bool sfx,
+                                                uint32_t gathered_unit, uint32_t &x, uint32_t &h1o) {
+    uint32_t h1 = wide ? wide_key1(e.h1, top8) : e.h1 & KEY_MASK, h2 = e.h2 & KEY_MASK;
+    uint32_t f = wide ? wide_carry(e) : 0u;
+    h1o = h1;
+    const uint32_t ev = regen_wide(r, h1, h2, e.rid, f, sfx, gathered_unit);
+    e.h1 = h1 | ((f & 7u) << 29);
+    e.h2 = h2 | (((f >> 3) & 7u) << 29);
+    x = (f >> 6) & 0xFFu;
+    return ev;
+}
+
 // Rank step (after next_gather): regenerates each entry in place (e[q] becomes its next-round
-// entry), finds its slot for source bucket b and takes a rank among the wave's entries of that
-// slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches them).
-// ntot: the work-group's per-side totals. tgt[q] becomes the slot (bits 9-13) and the rank
-// (bits 14 up, slot_rank).
+// entry, regen_entry), finds its slot for source bucket b and takes a rank among the wave's entries
+// of that slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches
+// them). ntot: the work-group's per-side totals. tgt[q] becomes the carry's top bits, the slot and
+// the rank (tgt_word).
 template <int PER>
 __device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
-                                          uint32_t mask, uint32_t (&tgt)[PER], uint32_t side) {
+                                          uint32_t mask, uint32_t (&tgt)[PER], uint32_t side, bool wide, uint32_t top8) {
     const bool sfx = side == 0;
     uint32_t wn = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const bool act = (mask >> q) & 1u;
         if (act) {
-            const uint32_t h1o = e[q].h1 & KEY_MASK;
-            const uint32_t ev = regen_carry(nx.cols, nx.rg, e[q], sfx, tgt_word(tgt[q]));
+            uint32_t x, h1o;
+            const uint32_t ev = regen_entry(nx.rg, e[q], wide, top8, sfx, tgt_word(tgt[q]), x, h1o);
             const uint32_t h1n = e[q].h1 & KEY_MASK;
             const uint32_t slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, h1o, ev, h1n);
-            tgt[q] = (slot << 9) | (slot_rank(nx, row, slot, h1n, side) << 14);
+            tgt[q] = x | (slot << 9) | (slot_rank(nx, row, slot, h1n, side) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
     }
     if (lane_id() == 0 && wn) atomicAdd(ntot + side, wn);
+}
+
+// Stored form of a regenerated entry (keys in bits 0..28): h1's bits 21..28, implied by the
+// bucket, hold the carry bits 6..13 from tgt.
+__device__ __forceinline__ Entry stored(const Entry &e, uint32_t tg) {
+    return Entry{(e.h1 & ~(0xFFu << WIDE_SHIFT)) | ((tg & 0xFFu) << WIDE_SHIFT), e.h2, e.rid};
 }
 
 // Write step: `row` now holds this wave's run start per slot; e[q] are the next-round entries.
@@ -192,7 +216,7 @@ __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *ro
         if ((mask >> q) & 1u) {
             const uint32_t nb = (e[q].h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
-            next_put(nx, side, nb, pos, e[q]);
+            next_put(nx, side, nb, pos, stored(e[q], tgt[q]));
         }
     }
 }
@@ -223,17 +247,18 @@ __device__ __forceinline__ void slot_starts(uint32_t (*wcnt)[NSLOT], uint32_t s,
     }
 }
 
-// Single-entry form (large-bucket kernels): e is an entry of this round (side `side`) with its
-// carried codes in place.
-__device__ __forceinline__ void put_next(const NextOut &nx, Entry e, uint32_t side) {
+// Single-entry form (large-bucket kernels): keys h1, h2 (bits 0..28) and carry f of an entry of
+// this round (side `side`).
+__device__ __forceinline__ void put_next(const NextOut &nx, uint32_t h1, uint32_t h2, uint32_t rid, uint32_t f,
+                                         uint32_t side) {
     const bool sfx = side == 0;
-    uint32_t g, word = 0xFFFFFFFFu;
-    if (regen_needs_gather(nx.rg, e.rid, sfx, g)) word = group_codes(nx.cols, nx.rg.G, g, e.rid >> 1);
-    regen_carry(nx.cols, nx.rg, e, sfx, word);
-    const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift;
+    uint32_t g, unit = 0xFFFFFFFFu;
+    if (wide_needs_gather(nx.rg, rid, sfx, g)) unit = wide_codes(nx.cols, nx.rg.G, g, rid >> 1);
+    regen_wide(nx.rg, h1, h2, rid, f, sfx, unit);
+    const uint32_t nb = h1 >> nx.shift;
     const uint32_t pos = atomicAdd(nx.cnt[side] + nb, 1u);
     atomicAdd(nx.total[side], 1u);
-    next_put(nx, side, nb, pos, e);
+    next_put(nx, side, nb, pos, wide_entry(h1, h2, rid, f));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -257,6 +282,12 @@ __device__ __forceinline__ uint32_t pack_codes(uint32_t v, uint32_t keep) {
 __device__ __forceinline__ uint32_t group_keep(uint32_t g, int L) {
     const int rest = L - 4 * (int)g;
     return rest >= 4 ? 0xFFu : (1u << (2 * rest)) - 1u;
+}
+// The 16-bit unit of 8-column group g of a row of ng dwords (no dword past the row is read).
+__device__ __forceinline__ uint32_t wide_unit(const uint32_t *row4, uint32_t g, int ng, int L) {
+    const uint32_t lo = pack_codes(row4[2 * g], group_keep(2 * g, L));
+    const uint32_t hi = 2 * (int)g + 1 < ng ? pack_codes(row4[2 * g + 1], group_keep(2 * g + 1, L)) : 0u;
+    return lo | (hi << 8);
 }
 
 constexpr int MAP_Q = 7;  // 16-byte loads per thread in flight (a 256 x 100 bp block is 1,600)
@@ -341,27 +372,42 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         Entry er = {fold1(sd[8], sd[9], sd[10], sd[11]), fold2(sd[12], sd[13], sd[14], sd[15]), (uint32_t)(2 * k + 1)};
         a.entries[2 * k] = ef;
         a.entries[2 * k + 1] = er;
-        if (a.ends)
-            a.ends[k] = (uint16_t)(pack_codes(row4[0], group_keep(0, L)) | (pack_codes(row4[ng - 1], group_keep(ng - 1, L)) << 8));
+        if (a.ends) {
+            const uint32_t gl = col_groups8((uint32_t)L) - 1u;
+            a.ends[k] = wide_unit(row4, 0, ng, L) | (wide_unit(row4, gl, ng, L) << 16);
+        }
     }
 
-    // Columns: word w of the tile covers reads 4w..4w+3; task id = g * nw + w keeps a wave on
-    // consecutive words of one column group (contiguous stores).
-    const uint32_t G = col_groups((uint32_t)L);
-    const uint32_t nw = (nt + 3) / 4;
-    const uint32_t ntask = nw * G;
-    const uint64_t w0 = (a.k0 + first) >> 2;
-    for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
-        const uint32_t g = task / nw, w = task - g * nw;
-        uint32_t word = 0;
-        const uint32_t rmax = min(4u, nt - 4 * w), keep = group_keep(g, L);
-        for (uint32_t i = 0; i < rmax; ++i)
-            word |= pack_codes(reinterpret_cast<const uint32_t *>(tile + (4 * w + i) * Lp)[g], keep) << (8 * i);
-        const uint64_t W = w0 + w;
-        if (a.gs)
-            a.cols[a.cbase + (uint64_t)g * a.gs + W] = word;
-        else
-            a.cols[((W >> (CHUNK_SHIFT - 2)) * G + g) * CHUNK_GWORDS + (W & (CHUNK_GWORDS - 1))] = word;
+    // Columns. A batched job's region (gs != 0): 4-column groups, word w of the tile covers reads
+    // 4w..4w+3. The wide store: 8-column groups, word w covers reads 2w, 2w+1. Task id = g * nw + w
+    // keeps a wave on consecutive words of one column group (contiguous stores).
+    if (a.gs) {
+        const uint32_t G = col_groups((uint32_t)L);
+        const uint32_t nw = (nt + 3) / 4;
+        const uint32_t ntask = nw * G;
+        const uint64_t w0 = (a.k0 + first) >> 2;
+        for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
+            const uint32_t g = task / nw, w = task - g * nw;
+            uint32_t word = 0;
+            const uint32_t rmax = min(4u, nt - 4 * w), keep = group_keep(g, L);
+            for (uint32_t i = 0; i < rmax; ++i)
+                word |= pack_codes(reinterpret_cast<const uint32_t *>(tile + (4 * w + i) * Lp)[g], keep) << (8 * i);
+            a.cols[a.cbase + (uint64_t)g * a.gs + w0 + w] = word;
+        }
+    } else {
+        const uint32_t G8 = col_groups8((uint32_t)L);
+        const uint32_t nw = (nt + 1) / 2;
+        const uint32_t ntask = nw * G8;
+        const uint64_t w0 = (a.k0 + first) >> 1;
+        for (uint32_t task = threadIdx.x; task < ntask; task += 256) {
+            const uint32_t g = task / nw, w = task - g * nw;
+            uint32_t word = 0;
+            const uint32_t rmax = min(2u, nt - 2 * w);
+            for (uint32_t i = 0; i < rmax; ++i)
+                word |= wide_unit(reinterpret_cast<const uint32_t *>(tile + (2 * w + i) * Lp), g, ng, L) << (16 * i);
+            const uint64_t W = w0 + w;
+            a.cols[((W >> (CHUNK_SHIFT - 1)) * G8 + g) * CHUNK_WWORDS + (W & (CHUNK_WWORDS - 1))] = word;
+        }
     }
 }
 
@@ -501,17 +547,17 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         uint32_t ts[PER], tp[PER];
         if (a.ends && a.nx.rg.col_t == 0) {
             // round 0 gathers column group 0 or the last one (suffix and prefix in turn): both
-            // are in the read's ends word, one 2-byte load per kept entry
+            // are in the read's ends word, one 4-byte load per kept entry
             uint32_t ew[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) ew[q] = a.ends[((keepmask >> q) & 1u) ? kept[q].rid >> 1 : 0u];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
-                ts[q] = tp[q] = 0x100u;
+                ts[q] = tp[q] = 0x10000u;
                 if ((keepmask >> q) & 1u) {
                     uint32_t g;
-                    if (regen_needs_gather(a.nx.rg, kept[q].rid, true, g)) ts[q] = (g ? ew[q] >> 8 : ew[q]) & 0xFFu;
-                    if (regen_needs_gather(a.nx.rg, kept[q].rid, false, g)) tp[q] = (g ? ew[q] >> 8 : ew[q]) & 0xFFu;
+                    if (wide_needs_gather(a.nx.rg, kept[q].rid, true, g)) ts[q] = (g ? ew[q] >> 16 : ew[q]) & 0xFFFFu;
+                    if (wide_needs_gather(a.nx.rg, kept[q].rid, false, g)) tp[q] = (g ? ew[q] >> 16 : ew[q]) & 0xFFFFu;
                 }
             }
         } else {
@@ -523,8 +569,8 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         Entry kp[PER];  // the prefix side's copy (each side is regenerated in place)
 #pragma unroll
         for (int q = 0; q < PER; ++q) kp[q] = kept[q];
-        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0);
-        next_rank<PER>(a.nx, b, row, ntot, kp, keepmask, tp, 1);
+        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0, false, 0u);
+        next_rank<PER>(a.nx, b, row, ntot, kp, keepmask, tp, 1, false, 0u);
         lds_barrier();
         if (threadIdx.x < NSLOT) {
             slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64, true>(a.nx, b, wcnt, threadIdx.x));
@@ -573,14 +619,14 @@ __device__ __forceinline__ uint32_t pfx_gather(const NextOut &nx, const JoinShar
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         grp[q] = 0;
-        if (((pmask >> q) & 1u) && regen_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q])) nmask |= 1u << q;
+        if (((pmask >> q) & 1u) && wide_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q])) nmask |= 1u << q;
         tgt[q] = 0u;
     }
     if (__any(nmask != 0)) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const bool need = (nmask >> q) & 1u;
-            tgt[q] = nx.cols[need ? group_word(nx.rg.G, grp[q], S.prid[threadIdx.x + NT * q] >> 1) : 0];
+            tgt[q] = nx.cols[need ? wide_word(nx.rg.G, grp[q], S.prid[threadIdx.x + NT * q] >> 1) : 0];
         }
     }
     return nmask;
@@ -594,7 +640,7 @@ __device__ __forceinline__ void pfx_gather_finish(const JoinShared<CAP, MCAP, NT
 
 template <int CAP, int MCAP, int NT>
 __device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t *row,
-                                         uint32_t pmask, uint32_t (&tgt)[CAP / NT]) {
+                                         uint32_t pmask, uint32_t (&tgt)[CAP / NT], uint32_t top8) {
     constexpr int PER = CAP / NT;
     uint32_t wn = 0;
 #pragma unroll
@@ -605,12 +651,12 @@ __device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP
             // of the keys the edges need)
             const uint32_t i = threadIdx.x + NT * q;
             Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
-            const uint32_t h1o = e.h1 & KEY_MASK;
-            const uint32_t ev = regen_carry(nx.cols, nx.rg, e, false, tgt_word(tgt[q]));
+            uint32_t x, h1o;
+            const uint32_t ev = regen_entry(nx.rg, e, true, top8, false, tgt_word(tgt[q]), x, h1o);
             S.pk1[i] = e.h1;
             S.pk2[i] = e.h2;
             const uint32_t slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
-            tgt[q] = (slot << 9) | (slot_rank(nx, row, slot, e.h1 & KEY_MASK, 1) << 14);
+            tgt[q] = x | (slot << 9) | (slot_rank(nx, row, slot, e.h1 & KEY_MASK, 1) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
     }
@@ -628,7 +674,7 @@ __device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CA
             const Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
             const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
-            next_put(nx, 1, nb, pos, e);
+            next_put(nx, 1, nb, pos, stored(e, tgt[q]));
         }
     }
 }
@@ -671,10 +717,11 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         for (int q = 0; q < PER; ++q) {
             const uint32_t i = threadIdx.x + NT * q;
             if (i < ns) {
-                const uint32_t h1 = se[q].h1 & KEY_MASK, h2 = se[q].h2 & KEY_MASK, rd = se[q].rid >> 1;
+                // keys of one bucket differ only in h1's low bits (its bits 21..28 carry codes)
+                const uint32_t h1 = se[q].h1 & LOW_MASK, h2 = se[q].h2 & KEY_MASK, rd = se[q].rid >> 1;
                 uint32_t c = 0;
                 for (uint32_t j = S.u.h.head[h2 & (CAP - 1)]; j != EMPTY; j = (S.u.h.nxt[j] == 0xFFFFu ? EMPTY : S.u.h.nxt[j])) {
-                    if ((S.pk2[j] & KEY_MASK) == h2 && (S.pk1[j] & KEY_MASK) == h1 && (S.prid[j] >> 1) != rd) {
+                    if ((S.pk2[j] & KEY_MASK) == h2 && (S.pk1[j] & LOW_MASK) == h1 && (S.prid[j] >> 1) != rd) {
                         ++c;
                         S.used[j] = 1;
                     }
@@ -696,7 +743,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                 ecount += sc[q];
                 uint32_t m = atomicAdd(&S.ms, 1u);
                 if (m < (uint32_t)MCAP) {
-                    S.u.m.msk[m] = ((uint64_t)(se[q].h1 & KEY_MASK) << 32) | (se[q].h2 & KEY_MASK);
+                    S.u.m.msk[m] = ((uint64_t)(se[q].h1 & LOW_MASK) << 32) | (se[q].h2 & KEY_MASK);
                     S.u.m.msr[m] = rid_rank(se[q].rid);
                     S.u.m.msc[m] = sc[q];
                 } else {
@@ -711,7 +758,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             if (S.used[i]) {
                 uint32_t m = atomicAdd(&S.mp, 1u);
                 if (m < (uint32_t)MCAP) {
-                    S.u.m.mpk[m] = ((uint64_t)(S.pk1[i] & KEY_MASK) << 32) | (S.pk2[i] & KEY_MASK);
+                    S.u.m.mpk[m] = ((uint64_t)(S.pk1[i] & LOW_MASK) << 32) | (S.pk2[i] & KEY_MASK);
                     S.u.m.mpr[m] = rid_rank(rid);
                 } else {
                     S.ovf = 1;
@@ -731,8 +778,9 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         const uint32_t gp = pfx_gather<CAP, MCAP, NT>(a.nx, S, pmask, ptg);
         next_gather_finish<PER>(se, gs, tgt);
         pfx_gather_finish<CAP, MCAP, NT>(S, gp, ptg);
-        next_rank<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0);
-        pfx_rank<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg);
+        const uint32_t top8 = b >> (WIDE_SHIFT - a.nx.src_shift);
+        next_rank<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0, true, top8);
+        pfx_rank<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg, top8);
     } else {  // last round: only counted
         if (smask) atomicAdd(&S.ntot[0], (uint32_t)__popc(smask));
         if (pmask) atomicAdd(&S.ntot[1], (uint32_t)__popc(pmask));
@@ -909,8 +957,9 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
     uint64_t *SK = a.skey + a.soff[blockIdx.x], *PK = a.pkey + a.poff[blockIdx.x];
     uint32_t *SR = a.srank + a.soff[blockIdx.x], *PR = a.prank + a.poff[blockIdx.x];
     uint32_t *SA = a.saux + a.soff[blockIdx.x], *PA = a.paux + a.poff[blockIdx.x];
-    gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos);
-    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, PK, PR, np2, true, &s_pos);
+    // round buckets hold wide entries: their keys are restored from the bucket index
+    gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos, a.nx.src_shift);
+    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, PK, PR, np2, true, &s_pos, a.nx.src_shift);
     bitonic(SK, SR, nullptr, ns2);
     bitonic(PK, PR, nullptr, np2);
     // valid partner counts: the key group on the other side minus the same-read members
@@ -972,9 +1021,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         }
         if (e == o) {  // a survivor
             if (a.nx.active) {
-                Entry en = {(uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid};
-                set_carry(en, carry_after(a.cols, a.rg, srid, true));
-                put_next(a.nx, en, 0);
+                put_next(a.nx, (uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid, carry_after(a.cols, a.rg, srid, true), 0);
             } else {
                 atomicAdd(a.nx.total[0], 1u);
             }
@@ -984,9 +1031,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         if (!PA[j]) {
             const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
             if (a.nx.active) {
-                Entry en = {(uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd};
-                set_carry(en, carry_after(a.cols, a.rg, prd, false));
-                put_next(a.nx, en, 1);
+                put_next(a.nx, (uint32_t)(PK[j] >> 32), (uint32_t)PK[j], prd, carry_after(a.cols, a.rg, prd, false), 1);
             } else {
                 atomicAdd(a.nx.total[1], 1u);
             }
@@ -1004,10 +1049,9 @@ __global__ void __launch_bounds__(1024) k_dedup_big(BigArgs a) {
     bitonic(K, R, nullptr, n2);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if (i == 0 || K[i] != K[i - 1]) {  // the smallest id of its key: round 0's suffix and prefix
-            const Entry e = {(uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i]};
-            if (a.nx.active) {
-                put_next(a.nx, e, 0);
-                put_next(a.nx, e, 1);
+            if (a.nx.active) {  // the dedup's buckets hold full keys and no carry
+                put_next(a.nx, (uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i], 0u, 0);
+                put_next(a.nx, (uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i], 0u, 1);
             } else {
                 atomicAdd(a.nx.total[0], 1u);
                 atomicAdd(a.nx.total[1], 1u);
@@ -1026,19 +1070,19 @@ constexpr int AUDIT_THREADS = 256;
 // 2-bit code of base `pos` of oriented read `rid` (odd ids: reverse complement, code ^ 2).
 __device__ __forceinline__ uint32_t oriented_code(const AuditArgs &a, uint32_t rid, int pos) {
     const int c = (rid & 1) ? a.L - 1 - pos : pos;
-    const uint32_t v = (group_codes(a.cols, a.G, (uint32_t)c >> 2, rid >> 1) >> (2 * (c & 3))) & 3u;
+    const uint32_t v = (wide_codes(a.cols, a.G, (uint32_t)c >> 3, rid >> 1) >> (2 * (c & 7))) & 3u;
     return (rid & 1) ? v ^ 2u : v;
 }
 
 __device__ bool same_string(const AuditArgs &a, uint32_t r1, uint32_t r2) {
     if (((r1 ^ r2) & 1u) == 0) {
         // one orientation: the same columns of both reads ([start, start+len) forward,
-        // [L-start-len, L-start) reverse-complemented), four at a time
+        // [L-start-len, L-start) reverse-complemented), eight at a time
         const int c0 = (r1 & 1) ? a.L - a.start - a.len : a.start, c1 = c0 + a.len;
-        for (int g = c0 >> 2; g <= (c1 - 1) >> 2; ++g) {
-            const int lo = max(c0 - 4 * g, 0), hi = min(c1 - 4 * g, 4);
+        for (int g = c0 >> 3; g <= (c1 - 1) >> 3; ++g) {
+            const int lo = max(c0 - 8 * g, 0), hi = min(c1 - 8 * g, 8);
             const uint32_t m = ((1u << (2 * hi)) - 1u) & ~((1u << (2 * lo)) - 1u);
-            if ((group_codes(a.cols, a.G, (uint32_t)g, r1 >> 1) ^ group_codes(a.cols, a.G, (uint32_t)g, r2 >> 1)) & m)
+            if ((wide_codes(a.cols, a.G, (uint32_t)g, r1 >> 1) ^ wide_codes(a.cols, a.G, (uint32_t)g, r2 >> 1)) & m)
                 return false;
         }
         return true;
@@ -1061,7 +1105,7 @@ __global__ void __launch_bounds__(AUDIT_THREADS) k_audit(AuditArgs a) {
     uint32_t *R = big ? a.srid + a.off[blockIdx.x] : lr;
     const uint32_t n2 = pow2_at_least(n);
     if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
-    gather_bucket(a.fine, n, a.cap, a.spill, a.spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
+    gather_bucket(a.fine, n, a.cap, a.spill, a.spill_n, a.spill_cap, b, K, R, n2, false, &s_pos, a.wshift);
     bitonic(K, R, nullptr, n2);
     unsigned long long keys = 0, strs = 0;
     for (uint32_t i = threadIdx.x; i < n; i += AUDIT_THREADS) {

@@ -81,6 +81,10 @@ int bucket_bits(uint64_t x) {
     return b;
 }
 
+// Bucket bits of a round's bucket set: at least WIDE_MIN_BITS, so that h1's bits 21..28 of every
+// entry are implied by its bucket and free to carry codes.
+int round_bits(uint64_t x) { return std::max(WIDE_MIN_BITS, bucket_bits(x)); }
+
 struct KStat {
     std::string name;
     uint64_t launches = 0;
@@ -111,7 +115,7 @@ struct emp_ctx {
     size_t cols_chunks = 0;      // chunks allocated
     Entry *d_entries = nullptr;
     uint64_t entries_cap = 0;
-    uint16_t *d_ends = nullptr;  // per read: first and last column groups (entries_cap / 2 of them)
+    uint32_t *d_ends = nullptr;  // per read: first and last column groups (entries_cap / 2 of them)
     uint32_t *d_err = nullptr;
 
     // host staging (pinned, double-buffered)
@@ -309,7 +313,7 @@ int ensure_cols(emp_ctx *c, uint64_t n_total) {
     const size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
     if (need <= c->cols_chunks) return EMP_OK;
     const size_t cap = std::max(need, c->cols_chunks * 2);
-    const size_t chunk_bytes = (size_t)col_groups((uint32_t)c->L) * CHUNK_GWORDS * sizeof(uint32_t);
+    const size_t chunk_bytes = (size_t)col_groups8((uint32_t)c->L) * CHUNK_WWORDS * sizeof(uint32_t);
     void *p = nullptr;
     int rc = dalloc(c, &p, cap * chunk_bytes);
     if (rc) return rc;
@@ -330,16 +334,16 @@ int ensure_entries(emp_ctx *c, uint64_t n_entries) {
     void *p = nullptr, *q = nullptr;
     int rc = dalloc(c, &p, cap * sizeof(Entry));
     if (rc) return rc;
-    if ((rc = dalloc(c, &q, cap / 2 * sizeof(uint16_t)))) return rc;
+    if ((rc = dalloc(c, &q, cap / 2 * sizeof(uint32_t)))) return rc;
     if (c->d_entries) {
         HIPCHK(c, hipMemcpyAsync(p, c->d_entries, 2 * c->n * sizeof(Entry), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(q, c->d_ends, c->n * sizeof(uint16_t), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(q, c->d_ends, c->n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         dfree(c, c->d_entries);
         dfree(c, c->d_ends);
     }
     c->d_entries = (Entry *)p;
-    c->d_ends = (uint16_t *)q;
+    c->d_ends = (uint32_t *)q;
     c->entries_cap = cap;
     return EMP_OK;
 }
@@ -926,7 +930,7 @@ RoundArgs round_args(const emp_ctx *c, int t, bool active) {
     r.col_back = (uint32_t)(L - 1 - t);
     r.m1 = modpow(R1, (uint64_t)(L - t - 1), P1);
     r.m2 = modpow(R2, (uint64_t)(L - t - 1), P2);
-    r.G = col_groups((uint32_t)L);
+    r.G = col_groups8((uint32_t)L);
     return r;
 }
 
@@ -1010,9 +1014,14 @@ struct FrameSide {
     std::vector<SpillRec> hspill;
     bool spill_read = false;
     std::map<uint32_t, std::vector<std::pair<uint64_t, uint32_t>>> sorted;  // bucket -> (key, rid)
+    int bb = 0;                 // bucket bits of the list's set
 };
 
-uint64_t entry_key(const Entry &e) { return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK); }
+// Key of a wide entry of bucket b of a set of 2^bb buckets: h1's bits 21..28 are the bucket's top 8.
+uint64_t entry_key(const Entry &e, uint32_t b, int bb) {
+    const uint32_t h1 = (e.h1 & LOW_MASK) | ((b >> (bb - WIDE_MIN_BITS)) << WIDE_SHIFT);
+    return ((uint64_t)h1 << 32) | (e.h2 & KEY_MASK);
+}
 
 int frame_bucket(emp_ctx *c, FrameSide &f, uint32_t b, const std::vector<std::pair<uint64_t, uint32_t>> **out) {
     auto it = f.sorted.find(b);
@@ -1037,7 +1046,7 @@ int frame_bucket(emp_ctx *c, FrameSide &f, uint32_t b, const std::vector<std::pa
                 return fail(c, EMP_ERR_CAPACITY, "framing: bucket %u holds %zu entries, counted %u", b, h.size(), n);
         }
         std::vector<std::pair<uint64_t, uint32_t>> v(h.size());
-        for (size_t i = 0; i < h.size(); ++i) v[i] = {entry_key(h[i]), h[i].rid};
+        for (size_t i = 0; i < h.size(); ++i) v[i] = {entry_key(h[i], b, f.bb), h[i].rid};
         std::sort(v.begin(), v.end());
         it = f.sorted.emplace(b, std::move(v)).first;
     }
@@ -1050,6 +1059,8 @@ int frame_side(emp_ctx *c, FrameSide &f, const Entry *fine, const uint32_t *d_cn
     f.fine = fine;
     f.spill = spill;
     f.nspill = nspill;
+    f.bb = 0;
+    while ((1u << f.bb) < nb) ++f.bb;
     f.cnt.resize(nb);
     HIPCHK(c, hipMemcpyAsync(f.cnt.data(), d_cnt, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     if (int rc = sync(c)) return rc;
@@ -1140,7 +1151,8 @@ int frame_round(emp_ctx *c, int bb, FrameSide &S, FrameSide &P, const uint32_t *
 // launch takes every bucket within capacity; the buckets over it follow in a launch over global
 // scratch. Appends one record to c->audits.
 int audit_list(emp_ctx *c, int list, int ovlp, uint64_t entries, const Entry *fine, const uint32_t *d_cnt,
-               const SpillRec *spill, const uint32_t *d_spill_n, uint32_t spill_cap, uint32_t nb, int start, int len) {
+               const SpillRec *spill, const uint32_t *d_spill_n, uint32_t spill_cap, uint32_t nb, int start, int len,
+               bool wide) {
     int rc;
     if (!c->d_audit) {
         void *p = nullptr;
@@ -1156,10 +1168,13 @@ int audit_list(emp_ctx *c, int list, int ovlp, uint64_t entries, const Entry *fi
     a.spill_n = d_spill_n;
     a.spill_cap = spill_cap;
     a.cols = c->d_cols;
-    a.G = col_groups((uint32_t)c->L);
+    a.G = col_groups8((uint32_t)c->L);
     a.L = c->L;
     a.start = start;
     a.len = len;
+    int bbl = 0;
+    while ((1u << bbl) < nb) ++bbl;
+    a.wshift = wide ? 29 - bbl : -1;  // a round's list holds wide entries; the dedup's, full keys
     a.out = c->d_audit;
     HIPCHK(c, launch_audit(a, nb, c->stream));
     std::vector<uint32_t> cnt(nb);
@@ -1363,7 +1378,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
     }
     // the 2n entries before dedup, in the dedup partition's buckets (overlap.cu:215-219)
     if (c->audit && (rc = audit_list(c, 0, L, N0, c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill,
-                                     c->dspill_cap, 1u << bbD, 0, L)))
+                                     c->dspill_cap, 1u << bbD, 0, L, false)))
         return rc;
     st.D = c->h_ctr->r[0].total[0];
     return EMP_OK;
@@ -1389,7 +1404,7 @@ int rounds_sync(emp_ctx *c, RunState &st) {
         const uint32_t nbt = 1u << bb;
         const bool more = t + 1 < rounds;
         // the next round's buckets, chosen now because this round's join writes them
-        const int bbn = more ? std::min(bb, bucket_bits(std::max(St, Pt))) : bb;
+        const int bbn = more ? std::min(bb, round_bits(std::max(St, Pt))) : bb;
         const RoundArgs rg = round_args(c, t, true);
         const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
         emp_round_stats rs{};
@@ -1487,8 +1502,9 @@ int rounds_sync(emp_ctx *c, RunState &st) {
         if (c->audit) {  // the lists as the round joins them (overlap.cu:274-284)
             const int ell = L - 1 - t;
             if ((rc = audit_list(c, 1, ell, St, sin.fine, sin.cnt, sin.spill, sin.spill_n, sin.spill_cap, nbt, t + 1,
-                                 ell)) ||
-                (rc = audit_list(c, 2, ell, Pt, pin.fine, pin.cnt, pin.spill, pin.spill_n, pin.spill_cap, nbt, 0, ell)))
+                                 ell, true)) ||
+                (rc = audit_list(c, 2, ell, Pt, pin.fine, pin.cnt, pin.spill, pin.spill_n, pin.spill_cap, nbt, 0, ell,
+                                 true)))
                 return rc;
         }
         for (size_t i = nrec0; i < st.rec_edges.size(); ++i) {
@@ -1588,7 +1604,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         }
         const bool more = t + 1 < rounds;
         const int bb = bits[t];
-        bits[t + 1] = more ? std::min(bb, bucket_bits(std::max(Sk, Pk))) : bb;
+        bits[t + 1] = more ? std::min(bb, round_bits(std::max(Sk, Pk))) : bb;
         const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
         const uint64_t slot0 = (uint64_t)t * stride;
         const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
@@ -1704,7 +1720,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
                     "is not reproduced; raise -r",
                     (unsigned long long)N0, (unsigned long long)M);
 
-    const int bbD = bucket_bits(N0);
+    const int bbD = round_bits(N0);  // the dedup's buckets and round 0's
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;
     const uint64_t stride = 1ull << bbD;  // edge slots per round
     RunState st{};
@@ -2028,21 +2044,21 @@ int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len
     rc = dalloc(c, &d, n * (uint64_t)len);
     if (!rc && hipMemcpy(d, reads, n * (uint64_t)len, hipMemcpyHostToDevice) != hipSuccess) rc = EMP_ERR_HIP;
     if (!rc) rc = emp_add_reads_device(c, d, n, len);
-    // device layout (4 reads x 4 columns per word) -> columns of 16 reads per word
+    // device layout (wide store: 2 reads x 8 columns per word) -> columns of 16 reads per word
     const uint64_t W = (n + 15) / 16;
-    const uint32_t G = col_groups((uint32_t)len);
+    const uint32_t G8 = col_groups8((uint32_t)len);
     std::vector<uint32_t> dev;
     if (!rc) {
-        dev.resize(c->cols_chunks * G * (size_t)CHUNK_GWORDS);
+        dev.resize(c->cols_chunks * G8 * (size_t)CHUNK_WWORDS);
         if (hipMemcpy(dev.data(), c->d_cols, dev.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
     }
     if (!rc) {
         memset(cols, 0, (size_t)len * W * 4);
         for (uint64_t k = 0; k < n; ++k) {
-            const size_t base = ((k >> CHUNK_SHIFT) * G) * (size_t)CHUNK_GWORDS + ((k >> 2) & (CHUNK_GWORDS - 1));
+            const size_t base = ((k >> CHUNK_SHIFT) * G8) * (size_t)CHUNK_WWORDS + ((k >> 1) & (CHUNK_WWORDS - 1));
             for (int j = 0; j < len; ++j) {
-                const uint32_t w = dev[base + (size_t)(j >> 2) * CHUNK_GWORDS];
-                const uint32_t code = (w >> (2 * (4 * (k & 3) + (j & 3)))) & 3u;
+                const uint32_t w = dev[base + (size_t)(j >> 3) * CHUNK_WWORDS];
+                const uint32_t code = (w >> (2 * (8 * (k & 1) + (j & 7)))) & 3u;
                 cols[(uint64_t)j * W + (k >> 4)] |= code << (2 * (k & 15));
             }
         }
