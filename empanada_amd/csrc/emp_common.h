@@ -278,25 +278,34 @@ struct JoinShared {
 
 // Position of record i among n records (key, rank) ordered by (key, rank): the number of records
 // smaller than it. (key, rank) pairs are distinct, so the positions of 0..n-1 are a permutation.
-// Eight independent LDS reads per step (the same address for the whole wave: broadcasts).
+// Eight independent LDS reads per step (the same address for the whole wave: broadcasts), two
+// 64-bit compares per record; ranks are compared only for a key that occurs more than once.
 __device__ __forceinline__ uint32_t rank_of(const uint64_t *key, const uint32_t *rank, uint32_t n, uint32_t i,
                                             uint64_t &k, uint32_t &r) {
     k = key[i];
     r = rank[i];
-    uint32_t pos = 0, j = 0;
+    // records below in key order, and records with the key at most k (the record itself
+    // included): two compares per record, no rank compare unless the key occurs twice
+    uint32_t lt = 0, le = 0, j = 0;
     for (; j + 8 <= n; j += 8) {
         uint64_t kk[8];
-        uint32_t rr[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { kk[u] = key[j + u]; rr[u] = rank[j + u]; }
+        for (int u = 0; u < 8; ++u) kk[u] = key[j + u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) pos += (uint32_t)(kk[u] < k) | ((uint32_t)(kk[u] == k) & (uint32_t)(rr[u] < r));
+        for (int u = 0; u < 8; ++u) {
+            lt += kk[u] < k ? 1u : 0u;
+            le += kk[u] <= k ? 1u : 0u;
+        }
     }
     for (; j < n; ++j) {
-        const uint64_t kj = key[j];
-        pos += (uint32_t)(kj < k) | ((uint32_t)(kj == k) & (uint32_t)(rank[j] < r));
+        const uint64_t kk = key[j];
+        lt += kk < k ? 1u : 0u;
+        le += kk <= k ? 1u : 0u;
     }
-    return pos;
+    if (le - lt > 1u) {  // equal keys: the smaller ranks among them come first
+        for (j = 0; j < n; ++j) lt += (key[j] == k && rank[j] < r) ? 1u : 0u;
+    }
+    return lt;
 }
 
 // Binary searches over sorted global-memory arrays.
