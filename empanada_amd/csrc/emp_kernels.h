@@ -30,13 +30,13 @@ constexpr int JOIN_THREADS = EMP_JOIN_THREADS;
 #endif
 constexpr bool JOIN_PERSIST = EMP_JOIN_PERSIST != 0;
 // Target mean bucket load: the partition fan-out is chosen so that lists average at most this
-// many entries per bucket. 13/16 of BUCKET_CAP keeps a hash-uniform bucket more than 8 standard
-// deviations (sqrt of the mean, about 41) below the capacity; fuller buckets spread the join's
-// fixed per-work-group costs over more entries (measured: 1024 -> 1664 takes the 15 M-read join
-// from 12.7 to 10.6 ms). A bucket that still overflows takes the large-bucket path.
-// Build-time tuning knob (EMP_BUCKET_TARGET).
+// many entries per bucket. 7/8 of BUCKET_CAP keeps a hash-uniform bucket 6 standard deviations
+// (sqrt of the mean, about 42) below the capacity; fuller buckets spread the join's fixed
+// per-work-group costs over more entries (measured on the 15 M-read join: 1024 -> 1664 took it
+// from 12.7 to 10.6 ms, 1664 -> 1792 from 9.65 to 9.44 ms). A bucket that still overflows takes
+// the large-bucket path. Build-time tuning knob (EMP_BUCKET_TARGET).
 #ifndef EMP_BUCKET_TARGET
-#define EMP_BUCKET_TARGET (EMP_BUCKET_CAP / 16 * 13)
+#define EMP_BUCKET_TARGET (EMP_BUCKET_CAP / 8 * 7)
 #endif
 constexpr uint32_t BUCKET_TARGET = EMP_BUCKET_TARGET;
 // Entries per work-group tile of a partition pass, its work-group size, and the widest digit of
