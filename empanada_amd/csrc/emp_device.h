@@ -112,13 +112,27 @@ __host__ __device__ __forceinline__ void next_sfx(uint32_t &h1, uint32_t &h2, ui
     h2 = v2 >= P2 ? v2 - P2 : v2;
 }
 
+// u * R^-1 mod P for u < P as an exact division: P = -1 (mod R) for (R1, P1) and (R2, P2), so
+// u + (u mod R) P is a multiple of R below R 2^29 < 2^32, and its quotient by R (a product with R's
+// inverse mod 2^32) is u R^-1 mod P, below P.
+__host__ __device__ __forceinline__ uint32_t div_r1(uint32_t u) { return (u + (u % R1) * P1) * 0xCCCCCCCDu; }
+__host__ __device__ __forceinline__ uint32_t div_r2(uint32_t u) { return (u + (u % R2) * P2) * 0xB6DB6DB7u; }
+
 // PrevPfx (overlap.cu:77-91): ((INV * ((h - e) mod 2^64)) mod 2^64) mod P, keeping the
-// reference's uint64_t wrap when h < e.
+// reference's uint64_t wrap when h < e (the 64-bit product then; else the exact division above).
 __host__ __device__ __forceinline__ void prev_pfx(uint32_t &h1, uint32_t &h2, uint32_t e) {
-    uint64_t x1 = (uint64_t)INV1 * ((uint64_t)h1 - (uint64_t)e);
-    uint64_t x2 = (uint64_t)INV2 * ((uint64_t)h2 - (uint64_t)e);
-    h1 = red64(x1, C1, P1);
-    h2 = red64(x2, C2, P2);
+    if (h1 >= e) {
+        h1 = div_r1(h1 - e);
+    } else {
+        const uint64_t x1 = (uint64_t)INV1 * ((uint64_t)h1 - (uint64_t)e);
+        h1 = red64(x1, C1, P1);
+    }
+    if (h2 >= e) {
+        h2 = div_r2(h2 - e);
+    } else {
+        const uint64_t x2 = (uint64_t)INV2 * ((uint64_t)h2 - (uint64_t)e);
+        h2 = red64(x2, C2, P2);
+    }
 }
 
 // Sort rank of a read id inside a group of equal keys: (rid & 1, rid) as one 32-bit value.
