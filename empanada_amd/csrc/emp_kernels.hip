@@ -847,17 +847,37 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         slot_starts<NT / 64>(S.u.m.wcnt, threadIdx.x - NXT0, (uint32_t)resv);
     }
     lds_barrier();
-    if (nms) {
+    // Edge offsets (an exclusive scan of the partner counts, two records per thread) and the
+    // edges. Up to 128 matched suffixes (nearly every bucket) wave 0 does both alone: a wave scan,
+    // its own LDS writes read back in order, no work-group barrier; the other waves go on to the
+    // survivors.
+    const bool one_wave = nms <= 128u;
+    if (nms && (!one_wave || threadIdx.x < 64)) {
         const uint32_t i0 = 2 * threadIdx.x, i1 = i0 + 1;
         const uint32_t e0 = i0 < nms ? S.u.m.msc[i0] : 0, e1 = i1 < nms ? S.u.m.msc[i1] : 0;
-        uint32_t stot;
-        const uint32_t off = block_excl_scan<NT>(e0 + e1, S.sh, stot);
+        uint32_t off;
+        if (one_wave) {
+            uint32_t x = e0 + e1;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if ((int)lane_id() >= o) x += y;
+            }
+            off = x - (e0 + e1);
+        } else {
+            uint32_t stot;
+            off = block_excl_scan<NT>(e0 + e1, S.sh, stot);
+        }
         if (i0 < nms) S.u.m.msc[i0] = off;
         if (i1 < nms) S.u.m.msc[i1] = off + e0;
-        lds_barrier();
+        if (one_wave)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
+        else
+            lds_barrier();
         const unsigned long long base = S.base;
+        const uint32_t nthr = one_wave ? 64u : (uint32_t)NT;
         if (base != ~0ull) {
-            for (uint32_t i = threadIdx.x; i < nms; i += NT) {
+            for (uint32_t i = threadIdx.x; i < nms; i += nthr) {
                 const uint64_t k = S.u.m.msk[i];
                 const uint32_t srid = (S.u.m.msr[i] << 1) | (S.u.m.msr[i] >> 31);
                 uint32_t lo = 0, hi = nmp;
