@@ -267,6 +267,7 @@ struct JoinShared {
             uint64_t msk[MCAP], mpk[MCAP];
             uint32_t msr[MCAP], mpr[MCAP], msc[MCAP];
             uint32_t wcnt[NT / 64][NSLOT];  // per wave and slot: entries, then their run start
+            uint32_t hist[2][64];           // binned record sort: per side and key bin, count then start
         } m;
     } u;
     uint32_t ntot[2];  // survivors per side

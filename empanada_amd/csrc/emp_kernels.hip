@@ -731,6 +731,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         }
     }
     lds_barrier();  // the hash table is dead from here on; its space holds the matched records
+    if (threadIdx.x < 128) S.u.m.hist[threadIdx.x >> 6][threadIdx.x & 63] = 0;
 
     // Matched records (key, rank) of both sides, the bucket's edge total, and the survivors
     // (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710).
@@ -823,18 +824,65 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         const bool both = nms + nmp <= (uint32_t)NT;
         uint64_t k1 = 0, k2 = 0;
         uint32_t r1 = 0, r2 = 0, c1 = 0, p1 = 0, p2 = 0;
-        const uint32_t t = threadIdx.x;
-        if (t < nms) {
-            p1 = rank_of(S.u.m.msk, S.u.m.msr, nms, t, k1, r1);
-            c1 = S.u.m.msc[t];
-        } else if (both && t - nms < nmp) {
-            p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t - nms, k2, r2);
+        const uint32_t t = threadIdx.x, j2 = both ? t - nms : t;
+        const bool has1 = t < nms, has2 = both ? (t >= nms && t - nms < nmp) : t < nmp;
+        if (has1) { k1 = S.u.m.msk[t]; r1 = S.u.m.msr[t]; c1 = S.u.m.msc[t]; }
+        if (has2) { k2 = S.u.m.mpk[j2]; r2 = S.u.m.mpr[j2]; }
+        if (nms + nmp > 96u) {
+            // Binned: 64 bins per side on the highest key bits that vary inside the bucket (h1's
+            // bits below its bucket bits; the bins are in key order), counted, scanned by wave 0,
+            // record indices scattered into bin order (in the dead used flags), then each record
+            // ranked only against its own bin.
+            static_assert(4 * MCAP <= CAP, "record indices in the used flags");
+            const uint32_t bsh = 32u + (a.nx.src_shift >= 6u ? a.nx.src_shift - 6u : 0u);
+            uint32_t b1 = 0, b2 = 0, l1 = 0, l2 = 0;
+            if (has1) { b1 = (uint32_t)(k1 >> bsh) & 63u; l1 = atomicAdd(&S.u.m.hist[0][b1], 1u); }
+            if (has2) { b2 = (uint32_t)(k2 >> bsh) & 63u; l2 = atomicAdd(&S.u.m.hist[1][b2], 1u); }
+            lds_barrier();
+            if (t < 64) {
+                const uint32_t x0 = S.u.m.hist[0][t], x1 = S.u.m.hist[1][t];
+                uint32_t s0 = x0, s1 = x1;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t y0 = __shfl_up(s0, o, 64), y1 = __shfl_up(s1, o, 64);
+                    if ((int)t >= o) { s0 += y0; s1 += y1; }
+                }
+                S.u.m.hist[0][t] = s0 - x0;
+                S.u.m.hist[1][t] = s1 - x1;
+            }
+            lds_barrier();
+            uint16_t *const idx = reinterpret_cast<uint16_t *>(S.used);  // suffixes, then prefixes from MCAP
+            uint32_t st1 = 0, st2 = 0;
+            if (has1) { st1 = S.u.m.hist[0][b1]; idx[st1 + l1] = (uint16_t)t; }
+            if (has2) { st2 = S.u.m.hist[1][b2]; idx[MCAP + st2 + l2] = (uint16_t)j2; }
+            lds_barrier();
+            if (has1) {
+                const uint32_t e = b1 == 63u ? nms : S.u.m.hist[0][b1 + 1];
+                uint32_t lt = 0;
+                for (uint32_t q = st1; q < e; ++q) {
+                    const uint32_t o = idx[q];
+                    const uint64_t kk = S.u.m.msk[o];
+                    lt += (kk < k1 || (kk == k1 && S.u.m.msr[o] < r1)) ? 1u : 0u;
+                }
+                p1 = st1 + lt;
+            }
+            if (has2) {
+                const uint32_t e = b2 == 63u ? nmp : S.u.m.hist[1][b2 + 1];
+                uint32_t lt = 0;
+                for (uint32_t q = st2; q < e; ++q) {
+                    const uint32_t o = idx[MCAP + q];
+                    const uint64_t kk = S.u.m.mpk[o];
+                    lt += (kk < k2 || (kk == k2 && S.u.m.mpr[o] < r2)) ? 1u : 0u;
+                }
+                p2 = st2 + lt;
+            }
+        } else {
+            if (has1) p1 = rank_of(S.u.m.msk, S.u.m.msr, nms, t, k1, r1);
+            if (has2) p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, j2, k2, r2);
         }
-        if (!both && t < nmp) p2 = rank_of(S.u.m.mpk, S.u.m.mpr, nmp, t, k2, r2);
-        const bool hasp = both ? (t >= nms && t - nms < nmp) : t < nmp;
         lds_barrier();
-        if (t < nms) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
-        if (hasp) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
+        if (has1) { S.u.m.msk[p1] = k1; S.u.m.msr[p1] = r1; S.u.m.msc[p1] = c1; }
+        if (has2) { S.u.m.mpk[p2] = k2; S.u.m.mpr[p2] = r2; }
     }
     if (threadIdx.x == 16) {
         const uint32_t sub = b & (SUBS - 1);
