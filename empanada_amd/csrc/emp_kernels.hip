@@ -828,7 +828,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         const bool has1 = t < nms, has2 = both ? (t >= nms && t - nms < nmp) : t < nmp;
         if (has1) { k1 = S.u.m.msk[t]; r1 = S.u.m.msr[t]; c1 = S.u.m.msc[t]; }
         if (has2) { k2 = S.u.m.mpk[j2]; r2 = S.u.m.mpr[j2]; }
-        if (nms + nmp > 96u) {
+        if (nms + nmp > 48u) {
             // Binned: 64 bins per side on the highest key bits that vary inside the bucket (h1's
             // bits below its bucket bits; the bins are in key order), counted, scanned by wave 0,
             // record indices scattered into bin order (in the dead used flags), then each record
