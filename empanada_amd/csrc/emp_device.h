@@ -21,10 +21,9 @@ constexpr uint32_t INV2 = 76695840u;   // 7^(P2-2) mod P2 (overlap.cu:122)
 constexpr uint32_t MASK29 = (1u << 29) - 1;
 constexpr int MAX_RD_LEN = 255;
 
-// Column store: 2-bit base codes ((c >> 1) & 3) of every read position. Reads are grouped in
-// chunks of 2^CHUNK_SHIFT and columns in groups of 4; a 32-bit word holds 4 reads x 4 columns:
-// read k, column 4g + c at bits 2 * (4 * (k % 4) + c). Chunk ch, group g holds CHUNK_GWORDS
-// words starting at word (ch * G + g) * CHUNK_GWORDS, G = ceil(L / 4).
+// Column stores: 2-bit base codes ((c >> 1) & 3) of every read position, reads grouped in chunks of
+// 2^CHUNK_SHIFT. The batched engine's job regions use columns in groups of 4: a 32-bit word holds
+// 4 reads x 4 columns, read k, column 4g + c at bits 2 * (4 * (k % 4) + c) (G = ceil(L / 4) groups).
 constexpr int CHUNK_SHIFT = 20;
 constexpr uint32_t CHUNK_READS = 1u << CHUNK_SHIFT;
 constexpr uint32_t CHUNK_GWORDS = CHUNK_READS / 4;
@@ -38,9 +37,10 @@ constexpr uint32_t CHUNK_WWORDS = CHUNK_READS / 2;
 constexpr int WIDE_COLS = 8;
 __host__ __device__ __forceinline__ uint32_t col_groups8(uint32_t L) { return (L + WIDE_COLS - 1) / WIDE_COLS; }
 
-// Keys are below 2^29: bits 29..31 of h1 and h2 carry up to three base codes an entry will need
-// in its next rounds (bits 0-1 of the 6-bit field h1 >> 29 | (h2 >> 29) << 3 first). Every key
-// comparison masks them off with KEY_MASK.
+// Keys are below 2^29: bits 29..31 of h1 and h2 carry base codes an entry will need in its next
+// rounds (the batched engine: up to three, the 6-bit field h1 >> 29 | (h2 >> 29) << 3 with the first
+// in bits 0-1; the single-job engine's round entries: seven, below). Key comparisons mask them off
+// with KEY_MASK.
 constexpr uint32_t KEY_MASK = MASK29;
 
 // Entries of the single-job engine's round buckets carry 8 more codes' bits in bits 21..28 of h1:
@@ -81,9 +81,6 @@ __host__ __device__ __forceinline__ uint32_t red64(uint64_t x, uint32_t c, uint3
 // (preprocess.cu:18: A<->T, C<->G) of a code is code ^ 2.
 __host__ __device__ __forceinline__ uint32_t enc_of_code(uint32_t code) { return code + 1; }
 
-// Horner step h*R + e (mod P); h < P, e <= 4 keeps the sum below 2^32.
-__host__ __device__ __forceinline__ uint32_t step1(uint32_t h, uint32_t e) { return red1(h * R1 + e); }
-__host__ __device__ __forceinline__ uint32_t step2(uint32_t h, uint32_t e) { return red2(h * R2 + e); }
 
 // Fingerprints as byte dot products (the map kernel): with W_i = R^(L-1-i) mod P split into bytes
 // W_i = sum_j b_ij 256^j, H = sum_i e_i W_i = sum_j S_j 256^j (mod P) for S_j = sum_i e_i b_ij.

@@ -4,13 +4,15 @@
 //   reads    row-major ASCII kept reads (input of the map kernel only)
 //   columns  2-bit codes per read position, chunked by 2^20 reads (emp_device.h)
 //   entries  12-byte (h1, h2, rid) records, in buckets by the top bits of h1: bucket b of a list
-//            with `bits` bucket bits holds the entries with h1 >> (29 - bits) == b, in a
-//            fixed-capacity slot array [bucket][cap] plus a count (spilled records past it).
-//            Order inside a bucket is arbitrary: every order the output depends on is
-//            re-established in LDS.
+//            with `bits` bucket bits (at least 8 for a round's lists) holds the entries with
+//            h1 >> (29 - bits) == b, in a fixed-capacity slot array [bucket][cap] plus a count
+//            (spilled records past it). A round's entries keep carried codes in h1's bits 21..28
+//            (implied by the bucket: wide_key1) and 29..31 and h2's bits 29..31. Order inside a
+//            bucket is arbitrary: every order the output depends on is re-established in LDS.
 //
 // Kernels:
-//   k_map      transposition + forward/reverse-complement Rabin-Karp (fingerprint.cu:75-153)
+//   k_map      transposition + forward/reverse-complement Rabin-Karp as byte dot products
+//              (fingerprint.cu:75-153)
 //   k_part     tile-sorted partition pass (at most 8 more key bits per pass): the dedup's sort of
 //              the 2n map entries (overlap.cu:211-212)
 //   k_dedup    per bucket: keep the smallest read id of every key (overlap.cu:212-222); the kept
