@@ -103,6 +103,8 @@ def _random_reads(rng, n, L, glen, p_rc, p_sub):
     (5, 60, 16, 16, 200, 0.5, 0.0),
     (6, 200, 28, 6, 150, 0.5, 0.02),
     (7, 500, 40, 10, 120, 0.5, 0.0),     # extreme coverage: large equal-key groups
+    (8, 300, 7, 3, 60, 0.5, 0.0),        # one 8-column group per read: first and last the same
+    (9, 300, 9, 4, 80, 0.5, 0.01),       # a full group and a one-column group
 ])
 def test_pipeline_small_vs_oracle_and_bruteforce(seed, n, L, o, glen, p_rc, p_sub):
     rng = random.Random(seed)
