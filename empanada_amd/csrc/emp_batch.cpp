@@ -186,7 +186,7 @@ struct emp_batch {
     SpillRec *d_spill[2] = {nullptr, nullptr};
     uint32_t spill_cap = 0, spill_min = 1u << 16;
     double slack = 1.3;
-    uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr;
+    uint2 *d_pool = nullptr;  // edge pool: (from, to) pairs
     uint64_t pool_cap = 0, pool_ovf_cap = 0;
     unsigned long long *d_edge_off = nullptr, *d_dst_off = nullptr, *d_eblk = nullptr;
     uint32_t *d_edge_cnt = nullptr;
@@ -503,8 +503,7 @@ int ensure_pool(emp_batch *b, uint64_t per_sub, uint64_t ovf) {
     ovf = std::max(ovf, b->pool_ovf_cap);
     BCHK(b, hipStreamSynchronize(b->stream));
     int rc;
-    if ((rc = realloc_dev(b, &b->d_pool_from, per_sub * SUBS + ovf))) return rc;
-    if ((rc = realloc_dev(b, &b->d_pool_to, per_sub * SUBS + ovf))) return rc;
+    if ((rc = realloc_dev(b, &b->d_pool, per_sub * SUBS + ovf))) return rc;
     b->pool_cap = per_sub;
     b->pool_ovf_cap = ovf;
     return EMP_OK;
@@ -621,8 +620,7 @@ BJoinArgs join_args(emp_batch *b, uint32_t nj) {
     a.lists[1] = b->d_list[1];
     a.lcnt = b->d_lcnt;
     a.ejob = b->d_lcnt + 2 * nj;
-    a.pool_from = b->d_pool_from;
-    a.pool_to = b->d_pool_to;
+    a.pool = b->d_pool;
     a.pool_cursor = b->d_ctr->pool;
     a.pool_cap = b->pool_cap;
     a.ovf_cursor = &b->d_ctr->pool_ovf;
@@ -687,8 +685,7 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
     a.lists[1] = b->d_list[1];
     a.lcnt = b->d_lcnt;
     a.ejob = b->d_lcnt + 2 * nj;
-    a.pool_from = b->d_pool_from;
-    a.pool_to = b->d_pool_to;
+    a.pool = b->d_pool;
     a.pool_cursor = b->d_ctr->pool;
     a.pool_cap = b->pool_cap;
     a.ovf_cursor = &b->d_ctr->pool_ovf;
@@ -949,7 +946,7 @@ int round_step(emp_batch *b, emp_job_sink sink, emp_job_done done, void *user) {
     if ((rc = ensure_out(b, std::max<uint64_t>(total, 1)))) return rc;
     if (total) {
         EdgeCopyArgs ea{b->d_edge_cnt, b->d_edge_off, b->d_dst_off, b->d_eblk, sl.nfine,
-                        b->d_pool_from, b->d_pool_to, b->d_out_from, b->d_out_to};
+                        b->d_pool, b->d_out_from, b->d_out_to};
         {
             Timed t(b, "k_edge_compact", 16.0 * total + 20.0 * (double)sl.nfine);
             BCHK(b, launch_edge_compact(ea, b->stream));

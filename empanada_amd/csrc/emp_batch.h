@@ -73,7 +73,7 @@ struct BJoinArgs {
     Entry *lists[2];         // survivors go to the job's segment of lists[side]
     uint32_t *lcnt;          // [2][njobs] survivor cursors
     uint32_t *ejob;          // [njobs] edges per job
-    uint32_t *pool_from, *pool_to;
+    uint2 *pool;  // edge pool: (from, to) pairs
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
     unsigned long long *ovf_cursor;
@@ -102,7 +102,7 @@ struct BBigArgs {
     const uint32_t *cols;
     Entry *lists[2];
     uint32_t *lcnt, *ejob;
-    uint32_t *pool_from, *pool_to;
+    uint2 *pool;  // edge pool: (from, to) pairs
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
     unsigned long long *ovf_cursor;

@@ -410,8 +410,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
                 for (uint32_t q = lo; q < nmp && S.u.m.mpk[q] == k && pos < end; ++q) {
                     const uint32_t prd = (S.u.m.mpr[q] << 1) | (S.u.m.mpr[q] >> 31);
                     if ((prd >> 1) != (srid >> 1)) {
-                        a.pool_from[pos] = srid;
-                        a.pool_to[pos] = prd;
+                        a.pool[pos] = make_uint2(srid, prd);
                         ++pos;
                     }
                 }
@@ -500,8 +499,7 @@ __global__ void __launch_bounds__(1024) k_bjoin_big(BBigArgs a) {
             for (uint32_t q = lower_u64(PK, 0, np, k); q < np && PK[q] == k && pos < base + e; ++q) {
                 const uint32_t prd = (PR[q] << 1) | (PR[q] >> 31);
                 if ((prd >> 1) != (srid >> 1)) {
-                    a.pool_from[pos] = srid;
-                    a.pool_to[pos] = prd;
+                    a.pool[pos] = make_uint2(srid, prd);
                     ++pos;
                 }
             }

@@ -153,7 +153,7 @@ struct JoinArgs {
     int32_t zero_in;         // clear the input counts once read (no later launch reads them)
     uint32_t nb;             // buckets
     NextOut nx;              // survivors -> the next round's buckets (nx.active: a next round exists)
-    uint32_t *pool_from, *pool_to;   // edge pool: SUBS sub-ranges of pool_cap edges, then the
+    uint2 *pool;                     // edge pool, (from, to) pairs: SUBS sub-ranges of pool_cap edges, then the
     unsigned long long *pool_cursor; // overflow region [SUBS * pool_cap, + ovf_cap) with its
     unsigned long long pool_cap;     // own cursor for reservations a sub-range cannot take
     unsigned long long *ovf_cursor;
@@ -183,7 +183,7 @@ struct BigArgs {
     RoundArgs rg;                  // the round (join): survivors get their carried codes back
     const uint32_t *cols;
     NextOut nx;                    // survivors (join) / kept entries (dedup) for the next round
-    uint32_t *pool_from, *pool_to;
+    uint2 *pool;                     // edge pool, (from, to) pairs
     unsigned long long *pool_cursor;
     unsigned long long pool_cap;
     unsigned long long *ovf_cursor;
@@ -209,7 +209,7 @@ struct EdgeCopyArgs {
     unsigned long long *dst_off;
     unsigned long long *blk;  // block offsets added to dst_off (null: dst_off is final)
     uint64_t n;
-    const uint32_t *pool_from, *pool_to;
+    const uint2 *pool;               // edge pool: (from, to) pairs
     uint32_t *out_from, *out_to;
 };
 
@@ -247,8 +247,8 @@ hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 // edges; rec = {edges, first position}), then copied from the pool to out_from / out_to (which may
 // be pinned host memory). Two launches on stream s.
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
-                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint32_t *pool_from,
-                              const uint32_t *pool_to, uint32_t *out_from, uint32_t *out_to, hipStream_t s);
+                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint2 *pool,
+                              uint32_t *out_from, uint32_t *out_to, hipStream_t s);
 // n work-groups: all buckets (a.ids null; buckets over a.cap are skipped) or the listed ones.
 hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,

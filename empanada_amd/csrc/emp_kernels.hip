@@ -940,8 +940,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
                 for (uint32_t j = lo; j < nmp && S.u.m.mpk[j] == k && pos < end; ++j) {
                     const uint32_t prd = (S.u.m.mpr[j] << 1) | (S.u.m.mpr[j] >> 31);
                     if ((prd >> 1) != (srid >> 1)) {
-                        a.pool_from[pos] = srid;
-                        a.pool_to[pos] = prd;
+                        a.pool[pos] = make_uint2(srid, prd);
                         ++pos;
                     }
                 }
@@ -1083,8 +1082,7 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
             for (uint32_t j = plo; j < np && PK[j] == k && pos < base + e; ++j) {
                 const uint32_t prd = (PR[j] << 1) | (PR[j] >> 31);
                 if ((prd >> 1) != (srid >> 1)) {
-                    a.pool_from[pos] = srid;
-                    a.pool_to[pos] = prd;
+                    a.pool[pos] = make_uint2(srid, prd);
                     ++pos;
                 }
             }
@@ -1295,8 +1293,9 @@ __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
     if (c == 0) return;
     const unsigned long long src = a.src_off[b], dst = a.dst_off[b] + (a.blk ? a.blk[b / ESCAN_BLOCK] : 0ull);
     for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
-        a.out_from[dst + i] = a.pool_from[src + i];
-        a.out_to[dst + i] = a.pool_to[src + i];
+        const uint2 v = a.pool[src + i];
+        a.out_from[dst + i] = v.x;
+        a.out_to[dst + i] = v.y;
     }
 }
 
@@ -1438,13 +1437,13 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
 }
 
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
-                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint32_t *pool_from,
-                              const uint32_t *pool_to, uint32_t *out_from, uint32_t *out_to, hipStream_t s) {
+                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint2 *pool,
+                              uint32_t *out_from, uint32_t *out_to, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    EdgeCopyArgs a{cnt, src_off, dst_off, nullptr, n, pool_from, pool_to, out_from, out_to};
+    EdgeCopyArgs a{cnt, src_off, dst_off, nullptr, n, pool, out_from, out_to};
     hipLaunchKernelGGL(k_edge_copy, dim3((n + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }

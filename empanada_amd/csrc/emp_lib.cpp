@@ -160,7 +160,8 @@ struct emp_ctx {
     SpillRec *d_dspill = nullptr;
     uint32_t dspill_cap = 0;
     double slack = 1.125;
-    uint32_t *d_pool_from = nullptr, *d_pool_to = nullptr, *d_out_from = nullptr, *d_out_to = nullptr;
+    uint2 *d_pool = nullptr;  // edge pool: (from, to) pairs
+    uint32_t *d_out_from = nullptr, *d_out_to = nullptr;
     uint32_t *h_from = nullptr, *h_to = nullptr;
     uint64_t pool_cap = 0;  // edges per sub-range of the pool
     uint64_t pool_ovf_cap = 0;  // edges of the pool's overflow region
@@ -825,27 +826,15 @@ int ensure_pool(emp_ctx *c, uint64_t per_sub, uint64_t ovf, bool keep) {
     per_sub = std::max<uint64_t>(per_sub, c->pool_cap);
     ovf = std::max<uint64_t>(ovf, c->pool_ovf_cap);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    uint32_t *old[2] = {c->d_pool_from, c->d_pool_to};
     const uint64_t old_total = c->pool_cap * SUBS + c->pool_ovf_cap;
-    uint32_t *nu[2] = {nullptr, nullptr};
-    for (int k = 0; k < 2; ++k) {
-        void *p = nullptr;
-        int rc = dalloc(c, &p, (per_sub * SUBS + ovf) * 4);
-        if (rc) {
-            if (nu[0]) dfree(c, nu[0]);
-            return rc;
-        }
-        nu[k] = (uint32_t *)p;
-    }
-    if (keep && old[0]) {
-        for (int k = 0; k < 2; ++k)
-            HIPCHK(c, hipMemcpyAsync(nu[k], old[k], old_total * 4, hipMemcpyDeviceToDevice, c->stream));
+    void *p = nullptr;
+    if (int rc = dalloc(c, &p, (per_sub * SUBS + ovf) * sizeof(uint2))) return rc;
+    if (keep && c->d_pool) {
+        HIPCHK(c, hipMemcpyAsync(p, c->d_pool, old_total * sizeof(uint2), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    dfree(c, old[0]);
-    dfree(c, old[1]);
-    c->d_pool_from = nu[0];
-    c->d_pool_to = nu[1];
+    dfree(c, c->d_pool);
+    c->d_pool = (uint2 *)p;
     c->pool_cap = per_sub;
     c->pool_ovf_cap = ovf;
     return EMP_OK;
@@ -1281,8 +1270,7 @@ int big_path(emp_ctx *c, uint32_t nbig, int bb, bool dedup, const RoundArgs &rg,
     a.rg = rg;
     a.cols = c->d_cols;
     a.nx = nx;
-    a.pool_from = c->d_pool_from;
-    a.pool_to = c->d_pool_to;
+    a.pool = c->d_pool;
     a.pool_cursor = c->d_ctr->pool;
     a.pool_cap = c->pool_cap;
     a.ovf_cursor = &c->d_ctr->pool_ovf;
@@ -1431,8 +1419,7 @@ int rounds_sync(emp_ctx *c, RunState &st) {
             a.p_cnt = pin.cnt;
             a.zero_in = 0;
             a.nx = next_out(c, t + 1, bbn, bb, rgn);
-            a.pool_from = c->d_pool_from;
-            a.pool_to = c->d_pool_to;
+            a.pool = c->d_pool;
             a.pool_cursor = c->d_ctr->pool;
             a.pool_cap = c->pool_cap;
             a.ovf_cursor = &c->d_ctr->pool_ovf;
@@ -1615,8 +1602,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         a.p_cnt = pin.cnt;
         a.zero_in = 1;
         a.nx = next_out(c, t + 1, bits[t + 1], bb, rgn);
-        a.pool_from = c->d_pool_from;
-        a.pool_to = c->d_pool_to;
+        a.pool = c->d_pool;
         a.pool_cursor = c->d_ctr->pool;
         a.pool_cap = c->pool_cap;
         a.ovf_cursor = &c->d_ctr->pool_ovf;
@@ -1637,8 +1623,8 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         // the round's edges, in output order, into the device arena while later rounds run
         HIPCHK(c, hipStreamWaitEvent(c->ostream, c->rec_ev[t], 0));
         HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, 1u << bb,
-                                     &c->d_ctr->obase, c->d_ctr->orec[t], c->d_pool_from, c->d_pool_to,
-                                     c->d_out_from, c->d_out_to, c->ostream));
+                                     &c->d_ctr->obase, c->d_ctr->orec[t], c->d_pool, c->d_out_from,
+                                     c->d_out_to, c->ostream));
         HIPCHK(c, hipMemcpyAsync(c->h_orec[t], c->d_ctr->orec[t], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                  c->ostream));
         HIPCHK(c, hipEventRecord(c->orec_ev[t], c->ostream));
@@ -1761,7 +1747,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     if (!st.streamed && (rc = ensure_arena(c, std::max<uint64_t>(total, 1), sink, user))) return rc;
     if (total && !st.streamed) {
         EdgeCopyArgs ea{c->d_edge_cnt, c->d_edge_off, c->d_dst_off, c->d_eblk, (uint64_t)st.t * stride,
-                        c->d_pool_from, c->d_pool_to, c->d_out_from, c->d_out_to};
+                        c->d_pool, c->d_out_from, c->d_out_to};
         Timed tm(c, "k_edge_compact", 16.0 * total + 20.0 * (double)st.t * stride);
         HIPCHK(c, launch_edge_compact(ea, c->stream));
     }
