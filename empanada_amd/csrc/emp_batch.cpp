@@ -124,6 +124,8 @@ struct BCounters {
     unsigned long long pool_ovf;
     unsigned long long pool_lost;  // abandoned sub-range reservations (place_edges)
     uint32_t n_big, abort, spill[2];
+    uint32_t too_many;  // a large bucket held 2^31 edges or more
+    uint32_t pad;
 };
 
 struct KStat {
@@ -571,6 +573,7 @@ int zero_step(emp_batch *b, const StepLayout &sl, bool part, bool dedup) {
     }
     add(b->d_lcnt, 3 * b->tab_cap);
     add(&b->d_ctr->n_big, 1);
+    add(&b->d_ctr->too_many, 1);
     add((uint32_t *)b->d_ctr->pool, 2 * (SUBS + 2));
     BCHK(b, launch_zero(z, b->stream));
     return EMP_OK;
@@ -692,11 +695,15 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
     a.ovf_cap = b->pool_ovf_cap;
     a.edge_off = b->d_edge_off;
     a.edge_cnt = b->d_edge_cnt;
+    a.too_many = &b->d_ctr->too_many;
     {
         Timed t(b, dedup ? "k_bdedup_big" : "k_bjoin_big", 0.0);
         BCHK(b, dedup ? launch_bdedup_big(a, nbig, b->stream) : launch_bjoin_big(a, nbig, b->stream));
     }
-    return read_counters(b, nj);
+    if ((rc = read_counters(b, nj))) return rc;
+    if (b->h_ctr->too_many)  // the reference's int edge offsets (overlap.cu:561,567)
+        return fail(b, EMP_ERR_PARTITION, "a key group of a batched job holds 2^31 edges or more");
+    return EMP_OK;
 }
 
 // Map phase of job h: its reads (host, through the pinned double buffer, or device) become the

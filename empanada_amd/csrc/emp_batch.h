@@ -109,6 +109,7 @@ struct BBigArgs {
     unsigned long long ovf_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
+    uint32_t *too_many;  // set when a bucket holds 2^31 edges or more
 };
 
 hipError_t launch_bpart(const BPartArgs &a, int pass, bool regen, uint32_t groups, hipStream_t s);

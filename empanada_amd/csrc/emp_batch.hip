@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
             tot ? place_edges(resv, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap) : ~0ull;
         S.base = base;
         a.edge_off[b] = base;
-        a.edge_cnt[b] = tot;
+        a.edge_cnt[b] = base == ~0ull ? 0u : tot;  // unplaced edges are not written; the host reruns the step
         if (tot) atomicAdd(a.ejob + j, tot);
     }
     lds_barrier();
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
 // ---------------------------------------------------------------------------------------------
 // Large buckets.
 __global__ void __launch_bounds__(1024) k_bjoin_big(BBigArgs a) {
-    __shared__ uint32_t sh[16];
+    __shared__ unsigned long long sh64[16];
     __shared__ uint32_t s_pos;
     __shared__ unsigned long long s_base;
     const uint32_t b = a.ids[blockIdx.x];
@@ -465,20 +465,28 @@ __global__ void __launch_bounds__(1024) k_bjoin_big(BBigArgs a) {
         PA[i] = ((hi - lo) - same) ? 1u : 0u;
     }
     __syncthreads();
+    // exclusive scan in 64 bits: a bucket of 2^31 edges or more (the reference's int edge
+    // offsets, overlap.cu:561,567) writes none and sets the step's error flag
     const uint32_t per = (ns + blockDim.x - 1) / blockDim.x;
     const uint32_t lo = min(ns, threadIdx.x * per), hi = min(ns, lo + per);
-    uint32_t run = 0;
+    unsigned long long run = 0;
     for (uint32_t i = lo; i < hi; ++i) run += SA[i];
-    uint32_t tot;
-    uint32_t off = block_excl_scan<1024>(run, sh, tot);
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t c = SA[i];
-        SA[i] = off;
-        off += c;
+    unsigned long long tot64;
+    const unsigned long long off64 = block_excl_scan64<1024>(run, sh64, tot64);
+    const bool too_many = tot64 >= (1ull << 31);
+    const uint32_t tot = too_many ? 0u : (uint32_t)tot64;
+    if (!too_many) {
+        uint32_t off = (uint32_t)off64;
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint32_t c = SA[i];
+            SA[i] = off;
+            off += c;
+        }
     }
     const uint32_t sub = b & (SUBS - 1);
     if (threadIdx.x == 0) {
         unsigned long long base = ~0ull;
+        if (too_many) atomicOr(a.too_many, 1u);
         if (tot) {
             const unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
             base = place_edges(c, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap);
@@ -486,9 +494,10 @@ __global__ void __launch_bounds__(1024) k_bjoin_big(BBigArgs a) {
         }
         s_base = base;
         a.edge_off[b] = base;
-        a.edge_cnt[b] = tot;
+        a.edge_cnt[b] = base == ~0ull ? 0u : tot;  // unplaced edges are not written (see k_bjoin)
     }
     __syncthreads();
+    if (too_many) return;
     const unsigned long long base = s_base;
     for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
         const uint32_t o = SA[i], e = (i + 1 < ns) ? SA[i + 1] : tot;

@@ -190,6 +190,7 @@ struct BigArgs {
     unsigned long long ovf_cap;
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
+    uint32_t *too_many;            // set when a bucket holds 2^31 edges or more (join)
 };
 
 // Up to 16 ranges of 32-bit words set to a value in one launch (per-round counter resets).

@@ -732,18 +732,26 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
+    // The bucket's edge total. It bounds both sides' matched-record counts (every matched record
+    // has an edge), so a bucket whose total is at most MCAP cannot overflow the records.
+    {
+        uint32_t ecount = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) ecount += sc[q];
+        if (ecount) atomicAdd(&S.etot, ecount);
+    }
     lds_barrier();  // the hash table is dead from here on; its space holds the matched records
     if (threadIdx.x < 128) S.u.m.hist[threadIdx.x >> 6][threadIdx.x & 63] = 0;
+    const bool may_overflow = __builtin_amdgcn_readfirstlane(S.etot) > (uint32_t)MCAP;
 
-    // Matched records (key, rank) of both sides, the bucket's edge total, and the survivors
-    // (suffixes with no edge, prefixes never a target: overlap.cu:424-429,694-710).
-    uint32_t smask = 0, pmask = 0, ecount = 0;
+    // Matched records (key, rank) of both sides and the survivors (suffixes with no edge, prefixes
+    // never a target: overlap.cu:424-429,694-710).
+    uint32_t smask = 0, pmask = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
         if (i < ns) {
             if (sc[q]) {
-                ecount += sc[q];
                 uint32_t m = atomicAdd(&S.ms, 1u);
                 if (m < (uint32_t)MCAP) {
                     S.u.m.msk[m] = ((uint64_t)(se[q].h1 & LOW_MASK) << 32) | (se[q].h2 & KEY_MASK);
@@ -771,6 +779,20 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             }
         }
     }
+    // A bucket with more matched records than MCAP goes to the large-bucket path, which writes
+    // its survivors itself: decided before the rank step, whose fallback-slot positions are
+    // global reservations in the next round's buckets that this launch would then never fill.
+    if (may_overflow) {
+        lds_barrier();
+        if (S.ovf) {
+            if (threadIdx.x == 0) {
+                uint32_t i = atomicAdd(a.n_big, 1u);
+                if (i < a.big_cap) a.big[i] = b;
+                a.edge_cnt[b] = 0;
+            }
+            return;
+        }
+    }
     // Survivors: their next-round slots and ranks (the entries are rebuilt when written, below).
     // The wave's slot counters live past the matched records (the hash table is dead).
     uint32_t tgt[PER], ptg[PER];
@@ -788,16 +810,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         if (smask) atomicAdd(&S.ntot[0], (uint32_t)__popc(smask));
         if (pmask) atomicAdd(&S.ntot[1], (uint32_t)__popc(pmask));
     }
-    if (ecount) atomicAdd(&S.etot, ecount);
     lds_barrier();
-    if (S.ovf) {
-        if (threadIdx.x == 0) {
-            uint32_t i = atomicAdd(a.n_big, 1u);
-            if (i < a.big_cap) a.big[i] = b;
-            a.edge_cnt[b] = 0;
-        }
-        return;
-    }
     const uint32_t nms = S.ms, nmp = S.mp, tot = S.etot;
 
     // Reservations, issued now and read back after the sort so that their latency overlaps it:
@@ -892,7 +905,9 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
             tot ? place_edges(resv, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap) : ~0ull;
         S.base = base;
         a.edge_off[b] = base;
-        a.edge_cnt[b] = tot;
+        // edges that found no room are not written: the slot counts none, so that nothing copies
+        // from the pool for it (the host sees the overflow cursor past the end and reruns)
+        a.edge_cnt[b] = base == ~0ull ? 0u : tot;
     } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NSLOT && a.nx.active) {
         slot_starts<NT / 64>(S.u.m.wcnt, threadIdx.x - NXT0, (uint32_t)resv);
     }
@@ -1017,7 +1032,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
 // ---------------------------------------------------------------------------------------------
 // Large buckets: global-memory sort + binary-search join, one work-group of 1024 per bucket.
 __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
-    __shared__ uint32_t sh[16];
+    __shared__ unsigned long long sh64[16];
     __shared__ uint32_t s_pos;
     __shared__ unsigned long long s_base;
     const uint32_t b = a.ids[blockIdx.x];
@@ -1047,30 +1062,39 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
         PA[j] = ((hi - lo) - same) ? 1u : 0u;
     }
     __syncthreads();
-    // exclusive scan of SA in place: contiguous chunk per thread
+    // exclusive scan of SA in place: contiguous chunk per thread, in 64 bits; a bucket of 2^31
+    // edges or more (the reference's int edge offsets, overlap.cu:561,567) writes none and sets
+    // the run's error flag
     const uint32_t per = (ns + blockDim.x - 1) / blockDim.x;
     const uint32_t lo = min(ns, threadIdx.x * per), hi = min(ns, lo + per);
-    uint32_t run = 0;
+    unsigned long long run = 0;
     for (uint32_t i = lo; i < hi; ++i) run += SA[i];
-    uint32_t tot;
-    uint32_t off = block_excl_scan<1024>(run, sh, tot);
-    for (uint32_t i = lo; i < hi; ++i) {
-        uint32_t c = SA[i];
-        SA[i] = off;
-        off += c;
+    unsigned long long tot64;
+    unsigned long long off64 = block_excl_scan64<1024>(run, sh64, tot64);
+    const bool too_many = tot64 >= (1ull << 31);
+    const uint32_t tot = too_many ? 0u : (uint32_t)tot64;
+    if (!too_many) {
+        uint32_t off = (uint32_t)off64;
+        for (uint32_t i = lo; i < hi; ++i) {
+            uint32_t c = SA[i];
+            SA[i] = off;
+            off += c;
+        }
     }
     const uint32_t sub = b & (SUBS - 1);
     if (threadIdx.x == 0) {
         unsigned long long base = ~0ull;
+        if (too_many) atomicOr(a.too_many, 1u);
         if (tot) {
             const unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
             base = place_edges(c, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap);
         }
         s_base = base;
         a.edge_off[b] = base;
-        a.edge_cnt[b] = tot;
+        a.edge_cnt[b] = base == ~0ull ? 0u : tot;  // unplaced edges are not written (see k_join)
     }
     __syncthreads();
+    if (too_many) return;
     const unsigned long long base = s_base;
     for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
         const uint32_t o = SA[i], e = (i + 1 < ns) ? SA[i + 1] : tot;
@@ -1292,6 +1316,7 @@ __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
     const uint32_t c = a.cnt[b];
     if (c == 0) return;
     const unsigned long long src = a.src_off[b], dst = a.dst_off[b] + (a.blk ? a.blk[b / ESCAN_BLOCK] : 0ull);
+    if (src == ~0ull) return;  // edges that found no room in the pool (their count is zero too)
     for (uint32_t i = threadIdx.x & 63; i < c; i += 64) {
         const uint2 v = a.pool[src + i];
         a.out_from[dst + i] = v.x;

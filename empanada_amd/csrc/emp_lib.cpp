@@ -49,7 +49,7 @@ struct DevCounters {
     uint32_t abort;                 // the dedup partition overflowed an intermediate bucket or its spill
     uint32_t spill;                 // records spilled by the dedup partition's final pass
     uint32_t in;                    // dedup partition input size, set by the host
-    uint32_t pad;
+    uint32_t too_many;              // a large bucket held 2^31 edges or more
     ListCtr r[MAX_ROUNDS + 1];      // r[t]: round t's lists
     // pipelined rounds stream their edges out: output position after the rounds copied so far,
     // and per round {edges, first position}
@@ -802,6 +802,7 @@ void zero_partition(emp_ctx *c, ZeroList &z, uint64_t x, int bb) {
     z.add(&c->d_ctr->abort, 1);
     z.add(&c->d_ctr->spill, 1);
     z.add(&c->d_ctr->in, 1, (uint32_t)x);
+    z.add(&c->d_ctr->too_many, 1);
 }
 
 // What a launch writing round t's bucket set (2^bb buckets per side) accumulates: its counts and
@@ -1277,11 +1278,15 @@ int big_path(emp_ctx *c, uint32_t nbig, int bb, bool dedup, const RoundArgs &rg,
     a.ovf_cap = c->pool_ovf_cap;
     a.edge_off = c->d_edge_off + slot0;
     a.edge_cnt = c->d_edge_cnt + slot0;
+    a.too_many = &c->d_ctr->too_many;
     {
         Timed t(c, dedup ? "k_dedup_big" : "k_join_big", 0.0);
         HIPCHK(c, dedup ? launch_dedup_big(a, nbig, c->stream) : launch_join_big(a, nbig, c->stream));
     }
-    return read_counters(c);
+    if (int rc = read_counters(c)) return rc;
+    if (c->h_ctr->too_many)  // the reference's int edge offsets (overlap.cu:561,567)
+        return fail(c, EMP_ERR_PARTITION, "a key group holds 2^31 edges or more");
+    return EMP_OK;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1539,6 +1544,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     }
     uint64_t copied = 0;  // arena edges already sent to the host
     auto send = [&](uint64_t upto) -> int {
+        upto = std::min<uint64_t>(upto, c->out_cap);  // placed edges never exceed the arena (the pool's size)
         if (upto <= copied) return EMP_OK;
         const size_t bytes = (size_t)(upto - copied) * 4;
         HIPCHK(c, hipMemcpyAsync(c->h_from + copied, c->d_out_from + copied, bytes, hipMemcpyDeviceToHost, c->ostream));

@@ -36,7 +36,7 @@ typedef struct emp_ctx emp_ctx;
 
 /* Options: the reference's Config (config.h:120-140) plus context-level settings. */
 typedef struct {
-    int32_t min_ovlp;        /* -o, default 31 (config.h:379) */
+    int32_t min_ovlp;        /* -o, default 31 (config.h:124) */
     uint64_t map_mem;        /* -m, default 1 GiB; FASTQ block = map_mem / 16 bytes (main.cu:16) */
     uint64_t reduce_mem;     /* -r, default 1 GiB; M = reduce_mem / 16 (main.cu:69) */
     uint64_t base_blocksz;   /* -b, accepted; sizes host blocks in the reference, no effect here */
@@ -47,7 +47,7 @@ typedef struct {
     void *stream;            /* hipStream_t to run on; NULL = the context creates one */
 } emp_opts;
 
-/* Fills the reference's defaults (config.h:376-384). */
+/* Fills the reference's defaults (config.h:124-128). */
 void emp_opts_default(emp_opts *opts);
 
 /* Replaces the Config/Kmp/Vmp singletons and init_match globals (main.cu:12-27, overlap.cu:117-127). */

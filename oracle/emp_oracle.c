@@ -404,7 +404,48 @@ static size_t remove_used(uint64_t *fp, uint32_t *rid, size_t n, uint32_t *used,
 }
 
 /*
- * emit_edges (overlap.cu:361-456) with cq_get_search_partitions (cqutils.cuh:107-137) and
+ * cq_get_search_partitions (cqutils.cuh:107-137): key-aligned partitions of the key-sorted lists a
+ * (suffixes) and b (prefixes), at most M/2 entries per side. While a side holds M/2 entries or
+ * more, the cut is at m = min(a[offA + M/2' - 1], b[offB + M/2' - 1]) (M/2' = the side's window),
+ * lower_bound of m on each side; when both sides hold fewer than M/2, the rest is the last
+ * partition. A window where neither side has an entry below m (for example more than M/2 equal
+ * keys, or a side whose remaining window is one key not above the other side's first) gives an
+ * empty partition and the reference loops forever: that is EO_ERR_PARTITION here. A side that is
+ * empty from the start gives one partition of the other side alone. parts receives (na, nb) pairs.
+ */
+int eo_search_partitions(const uint64_t *a, size_t nA, const uint64_t *b, size_t nB, size_t M, size_t *parts,
+                         size_t cap, size_t *n_parts) {
+    size_t np = 0, offA = 0, offB = 0;
+    const size_t H = M / 2;
+    *n_parts = 0;
+    while (offA < nA && offB < nB) {
+        size_t nPa = nA - offA < H ? nA - offA : H;
+        size_t nPb = nB - offB < H ? nB - offB : H;
+        if (np >= cap) return EO_ERR_ARG;
+        if (nPa < H && nPb < H) {
+            parts[2 * np] = nPa; parts[2 * np + 1] = nPb; ++np;
+            offA += nPa; offB += nPb;
+            break;
+        }
+        uint64_t ka = a[offA + nPa - 1], kb = b[offB + nPb - 1];
+        uint64_t m = ka < kb ? ka : kb;
+        nPa = lower_bound_u64(a + offA, nPa, m);
+        nPb = lower_bound_u64(b + offB, nPb, m);
+        if (nPa == 0 && nPb == 0) return EO_ERR_PARTITION;
+        parts[2 * np] = nPa; parts[2 * np + 1] = nPb; ++np;
+        offA += nPa; offB += nPb;
+    }
+    if (offA == nA || offB == nB) {
+        if (np >= cap) return EO_ERR_ARG;
+        if (offA == nA && offB < nB) { parts[2 * np] = 0; parts[2 * np + 1] = nB - offB; ++np; }
+        else if (offB == nB && offA < nA) { parts[2 * np] = nA - offA; parts[2 * np + 1] = 0; ++np; }
+    }
+    *n_parts = np;
+    return EO_OK;
+}
+
+/*
+ * emit_edges (overlap.cu:361-456) with cq_get_search_partitions (eo_search_partitions) and
  * get_edges (overlap.cu:557-603). Suffix keys are the needles, prefix keys the haystack.
  */
 static int emit_edges(list_t *S, list_t *P, int ovlp, size_t M, eo_buf *out, eo_round *st) {
@@ -414,32 +455,9 @@ static int emit_edges(list_t *S, list_t *P, int ovlp, size_t M, eo_buf *out, eo_
     size_t *parts = (size_t *)malloc(2 * cap * sizeof(size_t));
     size_t np = 0;
     if (!parts) return EO_ERR_NOMEM;
-    /* cq_get_search_partitions */
     {
-        size_t offA = 0, offB = 0;
-        while (offA < nA && offB < nB) {
-            size_t nPa = nA - offA < M / 2 ? nA - offA : M / 2;
-            size_t nPb = nB - offB < M / 2 ? nB - offB : M / 2;
-            if (np >= cap) { free(parts); return EO_ERR_PARTITION; }
-            if (nPa < M / 2 && nPb < M / 2) {
-                parts[2 * np] = nPa; parts[2 * np + 1] = nPb; ++np;
-                offA += nPa; offB += nPb;
-                break;
-            } else {
-                uint64_t a = S->fp[offA + nPa - 1], b = P->fp[offB + nPb - 1];
-                uint64_t m = a < b ? a : b;
-                nPa = lower_bound_u64(S->fp + offA, nPa, m);
-                nPb = lower_bound_u64(P->fp + offB, nPb, m);
-                if (nPa == 0 && nPb == 0) { free(parts); return EO_ERR_PARTITION; }
-                parts[2 * np] = nPa; parts[2 * np + 1] = nPb; ++np;
-                offA += nPa; offB += nPb;
-            }
-        }
-        if (offA == nA || offB == nB) {
-            if (np >= cap) { free(parts); return EO_ERR_PARTITION; }
-            if (offA == nA && offB < nB) { parts[2 * np] = 0; parts[2 * np + 1] = nB - offB; ++np; }
-            else if (offB == nB && offA < nA) { parts[2 * np] = nA - offA; parts[2 * np + 1] = 0; ++np; }
-        }
+        int prc = eo_search_partitions(S->fp, nA, P->fp, nB, M, parts, cap, &np);
+        if (prc) { free(parts); return prc == EO_ERR_ARG ? EO_ERR_PARTITION : prc; }
     }
 
     size_t offA = 0, offB = 0, wA = 0, wB = 0;

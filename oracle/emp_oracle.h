@@ -81,6 +81,14 @@ int eo_parse_fastq_file(const char *path, uint64_t block_bytes, char **reads, ui
 int eo_parse_fastq_mem(const char *data, uint64_t size, uint64_t block_bytes, char **reads, uint64_t *n,
                        int *L);
 
+/*
+ * The key-aligned search partitions of one round (cq_get_search_partitions, cqutils.cuh:107-137)
+ * over key-sorted suffix keys a and prefix keys b: up to cap (na, nb) pairs into parts, *n_parts
+ * of them. EO_ERR_PARTITION where the reference loops forever (a window with no cut).
+ */
+int eo_search_partitions(const uint64_t *a, size_t nA, const uint64_t *b, size_t nB, size_t M, size_t *parts,
+                         size_t cap, size_t *n_parts);
+
 /* Kept reads -> output records (appended to out). rounds may be NULL (else >= 256 entries). */
 int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
                      eo_summary *sum, eo_round *rounds);

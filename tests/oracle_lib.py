@@ -65,6 +65,10 @@ def lib():
         L.eo_parse_fastq_mem.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_int)]
+        L.eo_search_partitions.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_size_t)]
         L.eo_buf_free.argtypes = [ctypes.POINTER(Buf)]
         L.eo_strerror.restype = ctypes.c_char_p
         L.eo_strerror.argtypes = [ctypes.c_int]
@@ -93,6 +97,17 @@ def modulos():
     m2 = (ctypes.c_uint32 * 256)()
     lib().eo_modulos(m1, m2)
     return list(m1), list(m2)
+
+
+def search_partitions(a, b, M):
+    """cq_get_search_partitions over key-sorted lists a, b: (rc, [(na, nb), ...])."""
+    A = (ctypes.c_uint64 * max(len(a), 1))(*a)
+    B = (ctypes.c_uint64 * max(len(b), 1))(*b)
+    cap = 4 + 2 * (len(a) + len(b))
+    parts = (ctypes.c_size_t * (2 * cap))()
+    n = ctypes.c_size_t()
+    rc = lib().eo_search_partitions(A, len(a), B, len(b), M, parts, cap, ctypes.byref(n))
+    return rc, [(parts[2 * i], parts[2 * i + 1]) for i in range(n.value)]
 
 
 def parse(fastq: bytes, block_bytes: int):

@@ -161,6 +161,60 @@ def test_prefix_wrap_takes_the_fallback_slot():
            [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in orounds]
 
 
+def test_match_overflow_bucket_with_a_wrapped_prefix_survivor():
+    """A bucket with more matched records than the LDS join holds (400 reads ending in C and 400
+    starting with C, key(C) in bucket 0) that also holds surviving prefixes whose next step takes
+    PrevPfx's u64 wrap (the fallback slot of the next round's buckets). The whole bucket goes to
+    the large-bucket path, which writes its survivors once: per-round list sizes and output equal
+    the oracle's."""
+    rng = random.Random(91)
+    L, ln = 40, 30
+    C = bf.wrap_read(ln, ln, rng)  # key(C) has H1 < 4: bucket 0 at any bucket count
+    reads = ["".join(rng.choice("ACGT") for _ in range(L - ln)) + C for _ in range(400)]
+    reads += [C + "".join(rng.choice("ACGT") for _ in range(L - ln)) for _ in range(400)]
+    reads += [bf.wrap_read(L, ln, rng) for _ in range(5)]  # length-30 prefixes with H1 < 4, unmatched
+    g = gen.genome(2 * 10**5, 93)
+    r = np.concatenate([reads_of(reads), gen.reads(g, 5000, L, 0.5, 0.0, 94)])
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), r.shape[0], L, min_ovlp=25)
+    assert rc == 0
+    with native.Context(min_ovlp=25) as c:
+        c.add_reads(r)
+        got, rounds = c.run(), c.rounds()
+    assert rounds[L - 1 - ln]["big_buckets"] > 0
+    assert [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in rounds] == \
+           [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in orounds]
+    assert got == want
+
+
+def test_pool_overflow_in_pipelined_rounds(monkeypatch):
+    """Edges of one round beyond the whole first edge pool (sub-ranges and overflow region): 6,000
+    key groups of 30 x 30 reads sharing a 30-mer (10.8 M edges with their reverse complements at
+    overlap 30) among 2 M reads, on the back-to-back rounds. Buckets whose edges find no room
+    count none, nothing is copied for them, and the run restarts on the synchronous rounds with a
+    larger pool: the output equals the oracle's."""
+    monkeypatch.delenv("EMPANADA_SYNC_ROUNDS", raising=False)
+    rng = np.random.default_rng(17)
+    L, ln, k, groups = 40, 30, 30, 6000
+    A = np.frombuffer(b"ACGT", np.uint8)
+    common = np.repeat(rng.choice(A, size=(groups, 1, ln)), k, axis=1)
+    sfx = np.concatenate([rng.choice(A, size=(groups, k, L - ln)), common], axis=2).reshape(-1, L)
+    pfx = np.concatenate([common, rng.choice(A, size=(groups, k, L - ln))], axis=2).reshape(-1, L)
+    g = gen.genome(10**8, 61)
+    r = np.ascontiguousarray(np.concatenate([gen.reads(g, 2_000_000, L, 0.5, 0.0, 62), sfx, pfx]))
+    n = r.shape[0]
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), n, L, min_ovlp=29)
+    assert rc == 0
+    # the first pool: 256 sub-ranges of max(4096, 2n/256) edges and an overflow region of
+    # max(2^20, 2n/8)
+    first_pool = 256 * max(4096, 2 * n // 256) + max(1 << 20, 2 * n // 8)
+    assert orounds[L - 1 - ln]["edges"] > first_pool
+    with native.Context(min_ovlp=29) as c:
+        c.add_reads(r)
+        got = c.run()
+        assert c.summary()["n_edges"] == osum["n_edges"]
+    assert got == want
+
+
 def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
     """2,000,000 x 100 bp reads (4 M entries, 12-bit buckets): the back-to-back rounds (each join
     clears the counts it read) and the rounds with the host between them give the oracle's bytes

@@ -1,0 +1,73 @@
+"""GPU parity on the measured workloads' models and through the reference's executable path.
+
+- The in-core workload (50 Mbp i.i.d. genome, 100 bp reads at 30x, 50 % reverse-complemented,
+  0.5 % substitutions, -o 63) on a 2 % sample of the same coverage: output.bin byte for byte and
+  every round's live counts (S, P before and after removal, E) against the oracle, through the
+  kept-read and the FASTQ entry points.
+- The empanada CLI (main.cu:42-71: the block loop of read_block with its seek-back, then
+  build_graph writing output.bin) run as a child process with a block size that cuts the FASTQ
+  into dozens of blocks, against the oracle's reader and overlap path on the same file."""
+import os
+import subprocess
+
+import pytest
+
+import oracle_lib as ol
+from empanada_amd import gen, native
+
+pytestmark = pytest.mark.gpu
+
+COUNTS = ("s", "p", "edges", "s_after", "p_after")
+
+
+def _counts(rounds):
+    return [tuple(x[k] for k in COUNTS) for x in rounds]
+
+
+def test_incore_workload_sample_vs_oracle():
+    r, o = gen.workload_reads("incore", 0.02)
+    n, L = r.shape
+    assert (n, L, o) == (300_000, 100, 63)
+    rc, want, osum, orounds = ol.run_reads(r.tobytes(), n, L, min_ovlp=o)
+    assert rc == 0 and osum["n_rounds"] == 37 and osum["n_edges"] > 0
+    with native.Context(min_ovlp=o) as c:
+        c.add_reads(r)
+        got = c.run()
+        summ, rounds = c.summary(), c.rounds()
+    assert got == want
+    assert summ["n_dedup"] == osum["n_dedup"] and summ["n_records"] == osum["n_records"] == 37
+    assert _counts(rounds) == _counts(orounds)
+    fq = gen.fastq_bytes(r)
+    with native.Context(min_ovlp=o) as c:
+        c.add_fastq(fq, block=16 << 20)
+        assert c.run() == want
+
+
+def test_cli_block_loop_vs_oracle(tmp_path):
+    g = gen.genome(3 * 10**5, 51)
+    r = gen.reads(g, 30000, 100, 0.5, 0.005, 52)
+    fq = tmp_path / "in.fastq"
+    gen.write_fastq(str(fq), r)
+    data = fq.read_bytes()
+    out = tmp_path / "output.bin"
+    # -m 1600k: blocks of 100 KiB, about 65 of them, each resumed after its last complete record
+    p = subprocess.run([native.CLI_PATH, str(fq), str(out), "-o", "40", "-m", "1600k"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert len(data) > 60 * (100 << 10)
+    rc, want, osum, _ = ol.run_fastq(data, min_ovlp=40, map_mem=1600 << 10)
+    assert rc == 0 and osum["n_edges"] > 0
+    assert out.read_bytes() == want
+    assert "edges=%d" % osum["n_edges"] in p.stderr
+
+
+def test_cli_errors(tmp_path):
+    missing = subprocess.run([native.CLI_PATH, str(tmp_path / "none.fastq"), str(tmp_path / "o.bin")],
+                             capture_output=True, text=True, timeout=60)
+    assert missing.returncode == 1 and "cannot open" in missing.stderr
+    bad = tmp_path / "bad.fastq"
+    bad.write_bytes(b"@r0\nACGT\n+\nIIII\n@r1\nACG\n+\nIII\n")
+    p = subprocess.run([native.CLI_PATH, str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 1 and "error -4" in p.stderr
+    assert not os.path.exists(tmp_path / "o.bin") or os.path.getsize(tmp_path / "o.bin") == 0

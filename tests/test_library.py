@@ -44,7 +44,7 @@ def test_version_and_defaults():
     assert b"gfx950" in lib.emp_version()
     o = native.Opts()
     lib.emp_opts_default(ctypes.byref(o))
-    # config.h:376-384 defaults
+    # config.h:124-128 defaults
     assert (o.min_ovlp, o.map_mem, o.reduce_mem, o.base_blocksz, o.kv_chunksz) == (31, 1 << 30, 1 << 30, 32 << 20,
                                                                                    64 << 20)
 

@@ -207,6 +207,70 @@ def test_search_partitions_split_records():
     assert ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 200)[0] == -5
 
 
+# Search partitions worked by hand from cqutils.cuh:113-135 (window w = min(rest, M/2) per side;
+# unless both windows are below M/2, cut at m = min(last of A's window, last of B's window) with
+# lower_bound on each side). Each case notes the steps.
+SEARCH_CASES = [
+    # M = 6, H = 3. (0,0): windows 10 20 30 | 15 25 35, m = 30 -> lb 2 | 2.  (2,2): 30 40 50 |
+    # 35 45 55, m = 50 -> 2 | 2.  (4,4): 50 60 70 | 55 65 75, m = 70 -> 2 | 2.  (6,6): 70 80 |
+    # 75 85, both below 3: the rest.  upper_bound would cut (3, 2) first; windows of M = 6 would
+    # give (5, 5), (3, 3).
+    ([10, 20, 30, 40, 50, 60, 70, 80], [15, 25, 35, 45, 55, 65, 75, 85], 6, 0,
+     [(2, 2), (2, 2), (2, 2), (2, 2)]),
+    # M = 6. (0,0): 1 2 3 | 7 8 9, m = 3 -> 2 | 0 (a partition without prefixes: no record).
+    # (2,0): 3 4 5 | 7 8 9, m = 5 -> 2 | 0.  (4,0): 5 6 7 | 7 8 9, m = 7 -> 2 | 0.  (6,0): 7 8 |
+    # 7 8 9 (B's window is full), m = min(8, 9) = 8 -> 1 | 1.  (7,1): 8 | 8 9, both below 3: 1 | 2.
+    # upper_bound would cut (3, 0) first.
+    ([1, 2, 3, 4, 5, 6, 7, 8], [7, 8, 9], 6, 0, [(2, 0), (2, 0), (2, 0), (1, 1), (1, 2)]),
+    # M = 7, H = 3 (integer half). (0,0): 1 1 2 | 1 3 3, m = min(2, 3) = 2 -> 2 | 1.  (2,1): 2 4 4 |
+    # 3 3 5, m = min(4, 5) = 4 -> 1 | 2.  (3,3): 4 4 6 | 5 6, m = 6 -> 2 | 1 (B's window 5 6 is
+    # below 3 but A's is not).  (5,4): 6 | 6, both below 3: 1 | 1.
+    ([1, 1, 2, 4, 4, 6], [1, 3, 3, 5, 6], 7, 0, [(2, 1), (1, 2), (2, 1), (1, 1)]),
+    # M = 4, H = 2. (0,0): 5 6 | 1 2, m = 2 -> 0 | 1.  (0,1): 5 6 | 2, m = min(6, 2) = 2 -> 0 | 0:
+    # no cut, the reference pushes empty partitions forever.
+    ([5, 6, 7, 8, 9, 10, 11], [1, 2], 4, -6, None),
+    # one side empty from the start: one partition of the other side alone
+    ([3, 4, 5], [], 4, 0, [(3, 0)]),
+    ([], [3, 4, 5], 4, 0, [(0, 3)]),
+    # both windows below H from the start: one partition
+    ([1, 2, 3], [2, 3, 4], 8, 0, [(3, 3)]),
+]
+
+
+@pytest.mark.parametrize("a,b,M,rc,parts", SEARCH_CASES)
+def test_search_partitions_hand_worked(a, b, M, rc, parts):
+    got_rc, got = ol.search_partitions(a, b, M)
+    assert got_rc == rc
+    if rc == 0:
+        assert got == parts
+
+
+def test_search_partitions_of_a_run_match_the_records():
+    """The records of a framed run are the partitions with both sides non-empty: per round, the
+    oracle's record count equals the partitions computed from its own sorted lists' keys, which are
+    rebuilt here from the reads (the suffix and prefix strings of the live reads)."""
+    rng = random.Random(5)
+    reads = _random_reads(rng, 150, 30, 600, 0.5, 0.0)
+    data = FQ([r.encode() for r in reads])
+    M = 320
+    rc, out, _, rounds = ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * M)
+    assert rc == 0
+    # round 0: every kept read (deduplicated by key, both orientations) is live on both sides
+    oriented = {}
+    for k, r in enumerate(reads):
+        for rid, s in ((2 * k, r), (2 * k + 1, bf.revcomp(r))):
+            key = ol.key_of(s)
+            if key not in oriented or rid < oriented[key][0]:
+                oriented[key] = (rid, s)
+    live = list(oriented.values())
+    L = len(reads[0])
+    sfx = sorted(ol.key_of(s[1:]) for _, s in live)
+    pfx = sorted(ol.key_of(s[:L - 1]) for _, s in live)
+    prc, parts = ol.search_partitions(sfx, pfx, M)
+    assert prc == 0
+    assert rounds[0]["records"] == sum(1 for na, nb in parts if na and nb) > 1
+
+
 def test_search_partition_without_a_cut_is_an_error():
     """rc(B) + 10 random bases + B: at overlap 45 all 400 suffix keys are key(B) and all prefix keys
     key(rc B); with M = 400 the first window is one key on both sides, lower_bound cuts nothing and
