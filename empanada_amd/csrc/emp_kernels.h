@@ -103,7 +103,7 @@ struct MapArgs {
     int32_t L, Lpad;         // Lpad: LDS row stride (multiple of 4, odd number of dwords)
     uint32_t rows;           // reads per work-group (256, or 128 when 256 rows exceed 64 KiB of LDS)
     uint64_t k0;             // global index of the first read (multiple of 16)
-    uint32_t *cols;          // column store (grouped layout of emp_device.h)
+    uint32_t *cols;          // column store (grouped layout of emp_device.h); null: no columns written
     Entry *entries;          // linear entries, index 2k and 2k+1
     uint32_t *err;           // set to 1 when a byte is not ACGTacgt
     // gs != 0: a batched job's column region instead of the chunked store: word of read k,
@@ -235,6 +235,18 @@ struct AuditArgs {
 };
 
 hipError_t launch_map(const MapArgs &a, hipStream_t s);
+// The map kernel with the dot-product weights a.wt supplied by the caller (launch_map builds and
+// keeps one table per device and read length): ceil(L / 4) * 16 dwords, dword 16 d + 4 h + j
+// holding in byte u byte j of the weight of position 4 d + u (h = 0: R1^(L-1-k) mod P1, 1: R2^(L-1-k)
+// mod P2, 2: R1^k mod P1, 3: R2^k mod P2; 0 past the read).
+hipError_t launch_map_wt(const MapArgs &a, hipStream_t s);
+// LDS row stride and reads per work-group of the map kernel for read length L.
+inline void map_shape(int L, int32_t *Lpad, uint32_t *rows) {
+    int lp = (L + 3) & ~3;
+    if (((lp / 4) & 1) == 0) lp += 4;
+    *Lpad = lp;
+    *rows = 256u * (uint32_t)lp <= 65536u ? 256u : 128u;
+}
 hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s);
 hipError_t launch_dedup(const DedupArgs &a, uint32_t nb, hipStream_t s);
 hipError_t launch_join(const JoinArgs &a, uint32_t nb, hipStream_t s);

@@ -382,8 +382,11 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
 
     // Columns. A batched job's region (gs != 0): 4-column groups, word w of the tile covers reads
     // 4w..4w+3. The wide store: 8-column groups, word w covers reads 2w, 2w+1. Task id = g * nw + w
-    // keeps a wave on consecutive words of one column group (contiguous stores).
-    if (a.gs) {
+    // keeps a wave on consecutive words of one column group (contiguous stores). No store (cols
+    // null): fingerprints only.
+    if (!a.cols) {
+        return;
+    } else if (a.gs) {
         const uint32_t G = col_groups((uint32_t)L);
         const uint32_t nw = (nt + 3) / 4;
         const uint32_t ntask = nw * G;
@@ -1382,15 +1385,20 @@ static hipError_t map_weights(int L, const uint32_t **out) {
     return hipSuccess;
 }
 
+hipError_t launch_map_wt(const MapArgs &a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    uint32_t grid = (a.n + a.rows - 1) / a.rows;
+    size_t lds = (size_t)a.rows * a.Lpad;
+    hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_map(const MapArgs &a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     MapArgs b = a;
     hipError_t e = map_weights(a.L, &b.wt);
     if (e != hipSuccess) return e;
-    uint32_t grid = (a.n + a.rows - 1) / a.rows;
-    size_t lds = (size_t)a.rows * a.Lpad;
-    hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), lds, s, b);
-    return hipGetLastError();
+    return launch_map_wt(b, s);
 }
 
 hipError_t launch_part(const PartArgs &a, uint32_t groups, hipStream_t s) {

@@ -384,8 +384,9 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
     if (rc) return rc;
     rc = ensure_entries(c, 2 * (c->n + nreads));
     if (rc) return rc;
-    int Lpad = (c->L + 3) & ~3;
-    if (((Lpad / 4) & 1) == 0) Lpad += 4;
+    int32_t Lpad;
+    uint32_t rows;
+    map_shape(c->L, &Lpad, &rows);
     uint64_t done = 0;
     while (done < nreads) {
         uint32_t m = (uint32_t)std::min<uint64_t>(nreads - done, 1u << 30);
@@ -394,7 +395,7 @@ int map_batch(emp_ctx *c, const uint8_t *d, uint64_t nreads) {
         a.n = m;
         a.L = c->L;
         a.Lpad = Lpad;
-        a.rows = 256u * (uint32_t)Lpad <= 65536u ? 256u : 128u;
+        a.rows = rows;
         a.k0 = c->n + done;
         a.cols = c->d_cols;
         a.entries = c->d_entries;

@@ -24,7 +24,7 @@ EXPORTS = [
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
     "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
     "emp_set_audit", "emp_audit_count", "emp_get_audit",
-    "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll",
+    "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll", "emp_stage_fingerprints_noalloc",
     "emp_batch_opts_default", "emp_batch_create", "emp_batch_destroy", "emp_batch_last_error", "emp_batch_submit",
     "emp_batch_submit_fastq", "emp_batch_job_bytes", "emp_batch_step", "emp_batch_run", "emp_batch_job_info",
     "emp_batch_reset", "emp_batch_clock_ms", "emp_batch_kernel_count", "emp_batch_kernel_info",
@@ -114,6 +114,7 @@ def lib():
         L.emp_stage_fingerprints.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_columns.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_roll.argtypes = [i32, vp, vp, u64, i32, i32, vp]
+        L.emp_stage_fingerprints_noalloc.argtypes = [vp, u64, i32, vp, vp, vp]
         L.emp_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
         L.emp_batch_create.argtypes = [ctypes.POINTER(BatchOpts), ctypes.POINTER(vp)]
         L.emp_batch_destroy.argtypes = [vp]

@@ -36,6 +36,22 @@ int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len
 int emp_stage_roll(int32_t device, const uint64_t *keys, const char *bases, uint64_t n, int32_t kind, int32_t len,
                    uint64_t *out);
 
+/*
+ * Allocation-free form of emp_stage_fingerprints. It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Input and output are the caller's device memory: d_reads (n x
+ * len row-major, 4-byte aligned), d_keys (2n keys, as emp_stage_fingerprints writes them) and
+ * d_status (one word: 0, or 1 when a read holds a byte other than ACGTacgt, the case where
+ * emp_stage_fingerprints returns EMP_ERR_BASE). Everything else lives in a device array the
+ * library holds on each GPU for the life of the process (about 24 MiB). It runs asynchronously on
+ * `stream` (a hipStream_t of the current device; NULL = its null stream), calls no allocating
+ * routine and never synchronises. At most EMP_NOALLOC_MAX_READS reads per call (EMP_ERR_TOO_MANY
+ * above). Calls on one GPU share its array: calls that overlap on one GPU race.
+ */
+#define EMP_NOALLOC_MAX_READS 1048576
+int emp_stage_fingerprints_noalloc(const void *d_reads, uint64_t n, int32_t len, uint64_t *d_keys, uint32_t *d_status,
+                                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif
