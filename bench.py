@@ -1,14 +1,22 @@
 """Benchmark of the overlap path on one MI355X.
 
 Workload (default): 50 Mbp i.i.d. genome, 15,000,000 reads of 100 bp, 50 % reverse-complemented,
-0.5 % substitutions, minimum overlap 63 (seeds 1100 / 1101). One step = the whole path over the
-reads, which are resident in HBM when the timed region starts: fingerprints + 2-bit columns,
-partition, dedup, every overlap round, edge compaction and delivery of every output.bin record to
-host memory. Metric: Gbases/s = n * L / step time.
+0.5 % substitutions, minimum overlap 63 (seeds 1100 / 1101), as FASTQ (3.2 GB). One step is the
+whole job from FASTQ bytes already in HBM to output.bin written and closed: the FASTQ parsed on the
+device in blocks of map_mem/16 bytes (read_block's rules), fingerprints + 2-bit columns, partition,
+dedup, every overlap round, edge compaction, the records to host memory and into the file.
+Metric: Gbases/s = n * L / step time.
 
-Prints one JSON line with the roofline of the dominant kernel (HIP events recorded by the library
-on its stream around every launch, algorithmic bytes per launch from the live per-round counts)
-and the oracle timed on the host cores over a bounded sample of the same workload.
+Beside it, on the same line:
+
+- ``e2e_from_page_cache``: the reference's t_e2e, FASTQ file open (page-cached) to output.bin
+  closed, through emp_add_fastq_file (the file streamed over PCIe while blocks are parsed) and
+  emp_run_to_file: the PCIe-inclusive rate;
+- ``reads_in_hbm``: the overlap path alone over kept reads already in HBM, records to host memory;
+- ``roofline`` of the dominant kernel (HIP events recorded by the library on its stream around
+  every launch, algorithmic bytes per launch from the live per-round counts);
+- ``cpu_baseline``: the oracle on the host cores over the same FASTQ file, file to file, with the
+  host described from lscpu; its output.bin is compared with the GPU's (``parity_full_vs_oracle``).
 
 Two more measurements ride on the same line (skip them with --batched-sets 0 / --served-jobs 0):
 
@@ -24,6 +32,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -31,30 +41,69 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(workload, scale, min_ovlp, check_gpu):
-    """The oracle over a bounded sample (same genome/read model and coverage, `scale` of the size)."""
-    import oracle_lib
-    from empanada_amd import gen, native
+def host_cpu():
+    """The host's CPU from lscpu: model, sockets, cores per socket, threads per core."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for ln in out.splitlines():
+        k, _, v = ln.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    return info
 
-    reads, _ = gen.workload_reads(workload, scale)
-    n, L = reads.shape
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+def cpu_baseline(fq_path, n, L, min_ovlp, reduce_mem, gpu_out):
+    """The oracle on the host cores, FASTQ file to output.bin (the t_e2e boundaries), over the whole
+    workload; returns the baseline and whether its output.bin equals the GPU's."""
+    import oracle_lib
+    host = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or host.get("cpus") or os.cpu_count() or 1
+    out_path = fq_path + ".oracle.bin"
     t0 = time.perf_counter()
-    rc, want, summ, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
+    rc, summ = oracle_lib.run_file(fq_path, out_path, min_ovlp=min_ovlp, reduce_mem=reduce_mem)
     dt = time.perf_counter() - t0
     assert rc == 0, rc
-    same = None
-    if check_gpu:
-        with native.Context(min_ovlp=min_ovlp) as c:
-            c.add_reads(reads)
-            same = c.run() == want
+    with open(out_path, "rb") as f:
+        same = f.read() == gpu_out
+    os.unlink(out_path)
     return {
         "value": n * L / dt / 1e9,
         "unit": "Gbases/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%.0f kbp genome, %d reads x %d bp (%g%% of the workload, same coverage), %.2f s" % (
-            gen.WORKLOADS[workload]["G"] * scale / 1e3, n, L, 100 * scale, dt),
+        "host": host,
+        "sample": "the whole workload: FASTQ file (page-cached) to output.bin, %d reads x %d bp, %.2f s "
+                  "(FASTQ parse single-threaded, overlap path on %d OpenMP threads)" % (n, L, dt, threads),
+    }, same
+
+
+def cpu_baseline_sample(workload, scale, min_ovlp, reduce_mem):
+    """The oracle over a bounded sample of a workload too large for it (same genome/read model and
+    coverage, `scale` of the size), reads in memory; its output is compared with the GPU's."""
+    import oracle_lib
+    from empanada_amd import gen, native
+    reads, _ = gen.workload_reads(workload, scale)
+    n, L = reads.shape
+    host = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or host.get("cpus") or os.cpu_count() or 1
+    t0 = time.perf_counter()
+    rc, want, _, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp, reduce_mem=reduce_mem)
+    dt = time.perf_counter() - t0
+    assert rc == 0, rc
+    with native.Context(min_ovlp=min_ovlp, reduce_mem=reduce_mem) as c:
+        c.add_reads(reads)
+        same = c.run() == want
+    return {
+        "value": n * L / dt / 1e9, "unit": "Gbases/s", "cores": threads, "kind": "port", "host": host,
+        "sample": "%.0f kbp genome, %d reads x %d bp (%g%% of the workload, same coverage), kept reads in memory "
+                  "to records, %.2f s" % (gen.WORKLOADS[workload]["G"] * scale / 1e3, n, L, 100 * scale, dt),
     }, same
 
 
@@ -140,11 +189,12 @@ def main():
                     help="incore: 50 Mbp, 15 M x 100 bp, -o 63; ooc: 3.1 Gbp with repeats, 413 M x 150 bp, -o 75")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload (genome and reads)")
     ap.add_argument("--cpu-scale", type=float, default=None,
-                    help="fraction of the workload the oracle runs on (default 1.0 in-core, 0.01 ooc)")
+                    help="fraction of the 3.1 Gbp workload the oracle runs on (default 0.01); the in-core "
+                         "baseline is the whole workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true",
-                    help="also compare the full-size output with the oracle (in-core), or the pipelined rounds' "
-                         "output with the synchronous rounds' (3.1 Gbp)")
+                    help="3.1 Gbp: also compare the pipelined rounds' output with the synchronous rounds' at full "
+                         "size (in-core: the full-size output is always compared with the oracle's)")
     ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
     ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
     ap.add_argument("--served-loads", default="0.5,1,2,4,8",
@@ -170,23 +220,49 @@ def main():
     from empanada_amd import gen, hipmem, native
 
     ooc = args.workload == "ooc"
-    cpu_scale = args.cpu_scale if args.cpu_scale is not None else (0.01 if ooc else 1.0)
     wl = gen.WORKLOADS[args.workload]
     t_gen = time.perf_counter()
     reads, min_ovlp = gen.workload_reads(args.workload, args.scale)
     n, L = reads.shape
     t_gen = time.perf_counter() - t_gen
     # 2n entries must stay below M = reduce_mem / 16 (in-core order): -r 96g for the 3.1 Gbp case
-    ctx = native.Context(min_ovlp=min_ovlp, profile=False, reduce_mem=(96 << 30) if ooc else (1 << 30))
+    reduce_mem = (96 << 30) if ooc else (1 << 30)
+    map_mem = 1 << 30
+    ctx = native.Context(min_ovlp=min_ovlp, profile=False, reduce_mem=reduce_mem, map_mem=map_mem)
     stream = ctx.stream
-    dev = {"reads": None}
+    tmp = os.environ.get("TMPDIR", "/tmp")
+    fq_path = os.path.join(tmp, "emp_bench_%s.fastq" % args.workload)
+    out_path = os.path.join(tmp, "emp_bench_%s.output.bin" % args.workload)
+    dev = {"reads": None, "fq": None}
+    fq_size = 0
+    if not ooc:
+        t = time.perf_counter()
+        gen.write_fastq(fq_path, reads)
+        fq_size = os.path.getsize(fq_path)
+        with open(fq_path, "rb") as f:
+            dev["fq"] = hipmem.DeviceBuffer.from_array(np.frombuffer(f.read(), np.uint8))
+        t_gen += time.perf_counter() - t
 
     def upload():
         if dev["reads"] is None:
             dev["reads"] = hipmem.DeviceBuffer.from_array(reads)
             hipmem.synchronize()
 
-    def step():
+    def step_fastq():
+        """FASTQ in HBM -> output.bin closed (the metric)."""
+        ctx.reset()
+        ctx.add_fastq_device(dev["fq"].address, fq_size)
+        ctx.run_to_file(out_path)
+
+    def step_file():
+        """FASTQ file (page cache) -> output.bin closed: the PCIe-inclusive t_e2e."""
+        ctx.reset()
+        ctx.add_fastq_file(fq_path)
+        ctx.run_to_file(out_path)
+
+    def step_reads():
+        """Kept reads in HBM -> records in host memory: the overlap path alone."""
+        upload()
         ctx.reset()
         ctx.add_reads_device(dev["reads"].address, n, L)
         if ooc:  # the 3.1 Gbp reads leave HBM once mapped (62 GB)
@@ -194,20 +270,16 @@ def main():
             dev["reads"] = None
         ctx.run(collect=False)
 
-    for _ in range(args.warmup):
-        upload()
-        step()
-    hipmem.synchronize()
-
-    def timed(profile):
-        """Per step: reads uploaded (outside the timed region), then the step bracketed by device
-        synchronisation and HIP events on the library stream."""
+    def timed(step, steps, profile, before=None):
+        """Mean wall and HIP-event milliseconds of `steps` runs of step, each bracketed by device
+        synchronisation and events on the library stream."""
         ctx.set_profile(profile)
         ctx.kernel_clear()
         ev0, ev1 = hipmem.Event(), hipmem.Event()
         wall = gpu = 0.0
-        for _ in range(args.steps):
-            upload()
+        for _ in range(steps):
+            if before:
+                before()
             hipmem.synchronize()
             t0 = time.perf_counter()
             ev0.record(stream)
@@ -217,31 +289,54 @@ def main():
             hipmem.synchronize()
             wall += time.perf_counter() - t0
             gpu += ev0.elapsed_ms(ev1)
-        return wall * 1e3 / args.steps, gpu / args.steps
+        return wall * 1e3 / steps, gpu / steps
 
+    main_step = step_reads if ooc else step_fastq
+    for _ in range(args.warmup):
+        main_step()
+    hipmem.synchronize()
     # the metric: a timed region without per-launch events
-    ms_per_step, gpu_ms_region = timed(False)
-    # the roofline: the same steps again with HIP events around every launch on the library stream
-    ms_per_step_prof, _ = timed(True)
-    kern = ctx.kernels()
+    ms_per_step, gpu_ms_region = timed(main_step, args.steps, False, upload if ooc else None)
     summ = ctx.summary()
+    # the roofline: the same steps again with HIP events around every launch on the library stream
+    ms_per_step_prof, _ = timed(main_step, args.steps, True, upload if ooc else None)
+    kern = ctx.kernels()
     rounds = ctx.rounds()
+    value = n * L / (ms_per_step * 1e-3) / 1e9
 
-    dom = max(kern, key=lambda k: kern[k]["ms"])
+    e2e = reads_hbm = None
+    gpu_out = None
+    if not ooc:
+        with open(out_path, "rb") as f:
+            gpu_out = f.read()
+        file_steps = max(1, min(args.steps, 5))
+        step_file()  # warm-up of the streaming buffers
+        ms_file, _ = timed(step_file, file_steps, False)
+        with open(out_path, "rb") as f:
+            same_file = f.read() == gpu_out
+        e2e = {"ms_per_step": round(ms_file, 3), "gbases_per_s": round(n * L / (ms_file * 1e-3) / 1e9, 4),
+               "steps": file_steps, "fastq_bytes": fq_size,
+               "pcie_gbytes_per_s": round(fq_size / (ms_file * 1e-3) / 1e9, 2),
+               "output_equal_to_hbm_step": same_file,
+               "path": "emp_add_fastq_file + emp_run_to_file: FASTQ file open (page-cached) to output.bin closed"}
+        step_reads()
+        ms_reads, _ = timed(step_reads, args.steps, False)
+        reads_hbm = {"ms_per_step": round(ms_reads, 3), "gbases_per_s": round(n * L / (ms_reads * 1e-3) / 1e9, 4),
+                     "path": "emp_add_reads_device + emp_run: kept reads in HBM to records in host memory"}
+
+    dom = max((k for k in kern if k.startswith("k_")), key=lambda k: kern[k]["ms"])
     kd = kern[dom]
     avg_ms = kd["ms"] / max(kd["launches"], 1)
     bytes_per_launch = kd["bytes"] / max(kd["launches"], 1)
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9 if kd["ms"] > 0 else 0.0
-    value = n * L / (ms_per_step * 1e-3) / 1e9
 
     audit = None
     if args.audit:  # not part of the metric: the audit adds a sort and string checks per list
-        upload()
         ctx.set_profile(False)
         ctx.set_audit(True)
         hipmem.synchronize()
         t0 = time.perf_counter()
-        step()
+        step_reads()
         hipmem.synchronize()
         t_audit = time.perf_counter() - t0
         ctx.set_audit(False)
@@ -259,18 +354,17 @@ def main():
             "expected_per_list_round_0": round(au[1]["entries"] ** 2 / 2 / 2.0**58, 4) if len(au) > 1 else None,
         }
 
-    same_full = same_sync = None
-    if args.check and not ooc:
-        import oracle_lib
-        rc, want, _, _ = oracle_lib.run_reads(reads.tobytes(), n, L, min_ovlp=min_ovlp)
-        ctx.reset()
-        ctx.add_reads(reads)
-        same_full = rc == 0 and ctx.run() == want
-
-    cpu, same_sample = (None, None)
-    if not args.no_cpu:
-        cpu, same_sample = cpu_baseline(args.workload, cpu_scale, min_ovlp, check_gpu=True)
     ctx.close()
+    for k in ("reads", "fq"):
+        if dev[k] is not None:
+            dev[k].free()
+            dev[k] = None
+    cpu, same_full = (None, None)
+    if not args.no_cpu and not ooc:
+        cpu, same_full = cpu_baseline(fq_path, n, L, min_ovlp, reduce_mem, gpu_out)
+    same_sample = same_sync = None
+    if ooc and not args.no_cpu:
+        cpu, same_sample = cpu_baseline_sample(args.workload, args.cpu_scale or 0.01, min_ovlp, reduce_mem)
     if args.check and ooc:
         # full size, where the oracle would take hours: the back-to-back rounds against the rounds
         # with the host between them (counters read after every join), byte for byte; one
@@ -288,7 +382,10 @@ def main():
         del outs
     if dev["reads"] is not None:
         dev["reads"].free()
-    del reads
+    for pth in (fq_path, out_path):
+        if os.path.exists(pth):
+            os.unlink(pth)
+    del reads, gpu_out
 
     batched = batched_bench(args.batched_sets) if args.batched_sets > 0 and not ooc else None
     served = None
@@ -310,10 +407,12 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": ("%s: %d bp genome%s, %d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d%s" % (
+            "workload": ("%s: %d bp genome%s, %d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d%s; %s" % (
                 args.workload, int(wl["G"] * args.scale),
                 " (10% repeats: 300 bp and 6 kbp families, 16 copies, 1% divergence)" if ooc else "",
-                n, L, min_ovlp, ", -r 96g" if ooc else "")),
+                n, L, min_ovlp, ", -r 96g" if ooc else "",
+                "kept reads in HBM to records in host memory" if ooc else
+                "%d bytes of FASTQ in HBM to output.bin closed" % fq_size)),
             "reads": n, "read_len": L, "min_ovlp": min_ovlp,
             "dedup": summ["n_dedup"], "rounds": summ["n_rounds"], "edges": summ["n_edges"],
             "records": summ["n_records"],
@@ -334,6 +433,8 @@ def main():
                     for k, v in kern.items()},
         "gpu_event_ms_per_step": round(gpu_ms_region, 3),
         "ms_per_step_profiled": round(ms_per_step_prof, 3),
+        "e2e_from_page_cache": e2e,
+        "reads_in_hbm": reads_hbm,
         "cpu_baseline": cpu,
         "parity_sample_vs_oracle": same_sample,
         "parity_full_vs_oracle": same_full,

@@ -4,7 +4,8 @@
 //
 // Sizes take k/m/g suffixes (config.h:75-100). The map phase reads the input in blocks of
 // map_mem/16 bytes and resumes after the last complete record of each block, as read_block does
-// (preprocess.cu:195-237); the overlap phase writes output.bin (overlap.cu:148-161).
+// (preprocess.cu:195-237), through emp_add_fastq_file; the overlap phase writes output.bin
+// (overlap.cu:148-161).
 // Differences from the reference: errors return a non-zero status instead of exit(0) or a
 // release-mode assert, and timings go to stderr instead of timings.csv.
 #include <stdio.h>
@@ -119,21 +120,10 @@ int main(int argc, char **argv) {
         emp_ctx_destroy(ctx);
         return 1;
     }
-    std::vector<char> buf(block);
-    for (;;) {
-        size_t got = fread(buf.data(), 1, block, f);
-        bool final = got != block;
-        if (got == 0) break;
-        size_t used = 0;
-        rc = emp_add_fastq(ctx, buf.data(), got, final, &used);
-        if (rc) break;
-        if (final) break;
-        if (used < got && fseeko(f, -(off_t)(got - used), SEEK_CUR) != 0) {
-            rc = EMP_ERR_IO;
-            break;
-        }
-    }
     fclose(f);
+    // the block loop of main.cu:42-58 (read_block in blocks of map_mem/16 bytes, each resumed after
+    // its last complete record), with the file streamed into HBM while earlier blocks are parsed
+    rc = emp_add_fastq_file(ctx, argv[1]);
     double t1 = now_s();
     if (!rc) rc = emp_run_to_file(ctx, argv[2]);
     double t2 = now_s();

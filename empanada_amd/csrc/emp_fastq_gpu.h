@@ -1,4 +1,4 @@
-// GPU FASTQ block parser (emp_fastq_gpu.hip): launchers and the result block they fill.
+// GPU FASTQ parser (emp_fastq_gpu.hip): launchers and the result block they fill.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -6,35 +6,44 @@
 
 namespace emp {
 
-constexpr uint32_t FQ_ERR_FORMAT = 1;  // no leading '@', or a record longer than the block
+constexpr uint32_t FQ_ERR_FORMAT = 1;  // a block without a leading '@', or holding no complete record
 
 struct FqResult {
-    uint64_t newlines;    // '\n' bytes in the block
-    uint64_t use;         // bytes of complete records used (the caller resumes there)
-    uint64_t lines;       // lines in use
-    uint64_t records;     // sequence lines in use
+    uint64_t newlines;              // '\n' bytes in the range
+    uint64_t use;                   // bytes of the range in the blocks parsed (the caller resumes there)
+    uint64_t records;               // sequence lines in those blocks
+    uint64_t blocks;                // blocks parsed (the block after them failed when err is set)
     unsigned long long first_bad;   // first record whose kept-class read holds a byte outside ACGTacgt
     unsigned long long first_kept;  // first kept record
     unsigned long long first_len;   // first kept record of another length than L, or over 255
-    uint32_t err;
-    uint32_t pad;
+    uint64_t kept;                  // kept records
+    uint32_t err;                   // FQ_ERR_FORMAT
+    uint32_t kept_len;              // length of the first kept record
 };
 
 uint64_t fq_chunks(uint64_t size);
+// Block totals a scan of n counts needs (the blk scratch of launch_fq_count / launch_fq_classify).
+uint64_t fq_scan_blocks(uint64_t n);
 
-// Newline counts per chunk (cnt: fq_chunks(size) words) and their exclusive offsets (choff:
-// fq_chunks(size) + 1, the last one the block's newline total).
+// Newline counts per 4 KiB chunk of the range (cnt: fq_chunks(size) words) and their exclusive
+// offsets (choff: fq_chunks(size) + 1, the last one the range's newline total).
+// blk: fq_scan_blocks(fq_chunks(size)).
 hipError_t launch_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt, unsigned long long *choff,
-                           hipStream_t s);
-// Newline positions (nlpos: one per newline) and the block's framing into r.
-hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, int final, const unsigned long long *choff,
-                           uint64_t *nlpos, FqResult *r, hipStream_t s);
-// Per record: kept flag and length; rank = exclusive offsets of the kept records (records + 1).
-hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t records, FqResult *r,
-                              uint32_t *kept, uint32_t *len, unsigned long long *rank, hipStream_t s);
-hipError_t launch_fq_lencheck(const uint32_t *kept, const uint32_t *len, uint64_t records, uint32_t L, FqResult *r,
-                              hipStream_t s);
-hipError_t launch_fq_copy(const uint8_t *d, const uint64_t *nlpos, uint64_t records, const uint32_t *kept,
-                          const unsigned long long *rank, uint32_t L, uint8_t *out, hipStream_t s);
+                           unsigned long long *blk, hipStream_t s);
+// Every newline's position (nlpos: one per newline) and the block chain of the range into r: blocks
+// of B bytes (the last, shorter one only when range_final), or with one_block the whole range as
+// one block, final when one_final.
+hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, uint64_t B, int range_final, int one_block, int one_final,
+                           const unsigned long long *choff, uint64_t *nlpos, FqResult *r, hipStream_t s);
+// Per record of the parsed blocks: kept flag and length; rank = exclusive offsets of the kept
+// records (records + 1); the first kept record and the first kept record of another length than L
+// (L == 0: the first kept record's). blk: fq_scan_blocks(records).
+hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
+                              FqResult *r, uint32_t *kept, uint32_t *len, unsigned long long *rank,
+                              unsigned long long *blk, uint32_t L, hipStream_t s);
+// The kept reads, row-major, read i at out + i * L.
+hipError_t launch_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
+                          const uint32_t *kept, const unsigned long long *rank, uint32_t L, uint8_t *out,
+                          hipStream_t s);
 
 }  // namespace emp

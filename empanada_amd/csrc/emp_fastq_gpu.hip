@@ -1,7 +1,7 @@
-// FASTQ block parsing on the GPU with read_block's rules (preprocess.cu:195-237), the same rules
+// FASTQ parsing on the GPU with read_block's rules (preprocess.cu:195-237), the same rules
 // emp_fastq.cpp applies on the host:
 //
-//   - the block must start with '@';
+//   - each block must start with '@';
 //   - unless it is the last block, only complete records are used: n_lines = newlines before the
 //     last byte + 1; if n_lines % 4 != 0 or the block does not end in '\n', the first
 //     ((n_lines - 1) / 4) * 4 lines are used (none: a record longer than the block);
@@ -9,9 +9,12 @@
 //     (preprocess.cu:119-121,150-155), empty ones vanish, the others must be ACGTacgt;
 //   - kept reads have one length, at most 255.
 //
-// Passes: newline counts per 4 KiB chunk -> chunk offsets (one work-group scan) -> newline
-// positions -> framing (one thread) -> one wave per record classifies its sequence line ->
-// kept-record offsets -> one wave per kept record copies its bases (row-major, read_len each).
+// A range of FASTQ bytes in HBM is parsed as the CLI reads it (main.cu:42-58): blocks of B bytes,
+// each starting where the previous one's complete records ended. Passes over the range: newline
+// counts per 4 KiB chunk -> chunk offsets (a three-launch scan) -> every newline's position ->
+// the block chain (one wave: each block's framing from the newline counts) -> 16 lanes per record
+// classify its sequence line -> kept-record offsets -> 16 lanes per kept record copy its bases
+// (row-major, read_len each).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -25,16 +28,44 @@ namespace {
 constexpr int FQ_THREADS = 256;
 constexpr int FQ_PER = 16;
 constexpr uint32_t FQ_CHUNK = FQ_THREADS * FQ_PER;  // bytes per work-group of the newline passes
+constexpr int FQ_LANES = 16;                         // lanes per record in classify / copy
 
-__device__ __forceinline__ uint32_t newlines_of(const uint8_t *d, uint64_t p, uint64_t size, uint32_t &mask) {
-    mask = 0;
-    if (p + FQ_PER <= size) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(d + p);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+// Bit 7 of each byte of w set where that byte is '\n'.
+__device__ __forceinline__ uint32_t nl_bytes(uint32_t w) {
+    const uint32_t x = w ^ 0x0A0A0A0Au;
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+
+// Bit i of the result: byte i of the 16 bytes w[0..3] is '\n'.
+__device__ __forceinline__ uint32_t nl_mask16(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3) {
+    const uint32_t w[4] = {w0, w1, w2, w3};
+    uint32_t m = 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (((w[i >> 2] >> (8 * (i & 3))) & 0xFFu) == '\n') mask |= 1u << i;
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t b = nl_bytes(w[k]);
+        m |= ((b >> 7) & 1u) << (4 * k) | ((b >> 15) & 1u) << (4 * k + 1) | ((b >> 23) & 1u) << (4 * k + 2) |
+             ((b >> 31) & 1u) << (4 * k + 3);
+    }
+    return m;
+}
+
+// Newlines among bytes [p, p + 16) of the range at d (p a multiple of 16): mask bit i for byte
+// p + i. The range may start anywhere; its offset from 16-byte alignment, the same for every lane,
+// is s. Away from the range's ends two aligned 16-byte loads cover the 16 bytes (the 32 bytes
+// from the aligned address below d + p lie inside the range once p >= 16 and p + 32 <= size);
+// near the ends, byte loads.
+__device__ __forceinline__ uint32_t newlines_of(const uint8_t *d, uint64_t p, uint64_t size, uint32_t s, uint32_t &mask) {
+    if (p >= 16 && p + 32 <= size) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(d + p - s);
+        const uint4 a = q[0];
+        uint32_t m = nl_mask16(a.x, a.y, a.z, a.w);
+        if (s) {
+            const uint4 b = q[1];
+            m = (m | (nl_mask16(b.x, b.y, b.z, b.w) << 16)) >> s;
+        }
+        mask = m & 0xFFFFu;
     } else {
+        mask = 0;
         for (int i = 0; i < FQ_PER && p + i < size; ++i)
             if (d[p + i] == '\n') mask |= 1u << i;
     }
@@ -43,35 +74,85 @@ __device__ __forceinline__ uint32_t newlines_of(const uint8_t *d, uint64_t p, ui
 
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt) {
     __shared__ uint32_t sh[FQ_THREADS / 64];
+    const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(d) & 15u);
     const uint64_t p = (uint64_t)blockIdx.x * FQ_CHUNK + (uint64_t)threadIdx.x * FQ_PER;
     uint32_t mask;
-    const uint32_t c = p < size ? newlines_of(d, p, size, mask) : 0u;
+    const uint32_t c = p < size ? newlines_of(d, p, size, s, mask) : 0u;
     uint32_t tot;
     block_excl_scan<FQ_THREADS>(c, sh, tot);
     if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
-// Exclusive scan of n counts into off[0..n], off[n] = total (one work-group).
-__global__ void __launch_bounds__(1024) k_fq_scan(const uint32_t *cnt, uint32_t n, unsigned long long *off) {
+// Exclusive scan of n counts into off[0..n], off[n] = total, in three launches: offsets inside
+// blocks of SCAN_BLOCK counts (and the block totals), an exclusive scan of the block totals in
+// one work-group (which also writes off[n]), and the block offsets added.
+constexpr uint32_t SCAN_BLOCK = 4096;
+
+__global__ void __launch_bounds__(1024) k_fq_scan_local(const uint32_t *__restrict__ cnt, uint64_t n,
+                                                      unsigned long long *__restrict__ off,
+                                                      unsigned long long *__restrict__ blk) {
+    __shared__ unsigned long long sh[16];
+    const uint64_t i0 = (uint64_t)blockIdx.x * SCAN_BLOCK + 4ull * threadIdx.x;
+    uint32_t v[4];
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        v[u] = i0 + u < n ? cnt[i0 + u] : 0u;
+        sum += v[u];
+    }
+    unsigned long long tot;
+    unsigned long long o = block_excl_scan64<1024>(sum, sh, tot);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (i0 + u < n) off[i0 + u] = o;
+        o += v[u];
+    }
+    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_fq_scan_top(unsigned long long *__restrict__ blk, uint32_t nblk,
+                                                    unsigned long long *__restrict__ total) {
     __shared__ unsigned long long sh[16];
     unsigned long long carry = 0;
-    for (uint32_t base = 0; base < n; base += 1024) {
+    for (uint32_t base = 0; base < nblk; base += 1024) {
         const uint32_t i = base + threadIdx.x;
-        const unsigned long long v = i < n ? cnt[i] : 0ull;
+        const unsigned long long v = i < nblk ? blk[i] : 0ull;
         unsigned long long tot;
         const unsigned long long ex = block_excl_scan64<1024>(v, sh, tot);
-        if (i < n) off[i] = carry + ex;
+        if (i < nblk) blk[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) off[n] = carry;
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) k_fq_scan_add(unsigned long long *__restrict__ off, uint64_t n,
+                                                   const unsigned long long *__restrict__ blk) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) off[i] += blk[i / SCAN_BLOCK];
+}
+
+hipError_t launch_scan(const uint32_t *cnt, uint64_t n, unsigned long long *off, unsigned long long *blk,
+                       hipStream_t s) {
+    const uint64_t nblk = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nblk) {
+        hipLaunchKernelGGL(k_fq_scan_local, dim3((uint32_t)nblk), dim3(1024), 0, s, cnt, n, off, blk);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_fq_scan_top, dim3(1), dim3(1024), 0, s, blk, (uint32_t)nblk, off + n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nblk <= 1) return e;
+    hipLaunchKernelGGL(k_fq_scan_add, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, off, n, blk);
+    return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_lines(const uint8_t *d, uint64_t size,
                                                        const unsigned long long *choff, uint64_t *nlpos) {
     __shared__ uint32_t sh[FQ_THREADS / 64];
+    const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(d) & 15u);
     const uint64_t p = (uint64_t)blockIdx.x * FQ_CHUNK + (uint64_t)threadIdx.x * FQ_PER;
     uint32_t mask = 0;
-    const uint32_t c = p < size ? newlines_of(d, p, size, mask) : 0u;
+    const uint32_t c = p < size ? newlines_of(d, p, size, s, mask) : 0u;
     uint32_t tot;
     uint64_t at = choff[blockIdx.x] + block_excl_scan<FQ_THREADS>(c, sh, tot);
     while (mask) {
@@ -81,145 +162,231 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_lines(const uint8_t *d, uint6
     }
 }
 
-// Framing: which lines the block uses (read_block's resize, preprocess.cu:46-62,207-222).
-__global__ void k_fq_frame(const uint8_t *d, uint64_t size, int final, const unsigned long long *choff,
-                           uint32_t nchunks, const uint64_t *nlpos, FqResult *r) {
-    if (threadIdx.x != 0) return;
-    const uint64_t nl = choff[nchunks];
-    r->newlines = nl;
-    if (size == 0) return;
-    if (d[0] != '@') {
-        r->err = FQ_ERR_FORMAT;
-        return;
-    }
-    const bool ends_nl = d[size - 1] == '\n';
-    uint64_t lines;  // lines in use
-    uint64_t use = size;
-    if (!final) {
-        const uint64_t n_lines = nl - (ends_nl ? 1 : 0) + 1;
-        if (n_lines % 4 != 0 || !ends_nl) {
-            const uint64_t keep = ((n_lines - 1) / 4) * 4;
-            if (keep == 0) {
-                r->err = FQ_ERR_FORMAT;
-                return;
-            }
-            use = nlpos[keep - 1] + 1;
-            lines = keep;
-        } else {
-            lines = nl;
-        }
-    } else {
-        lines = nl + (ends_nl ? 0 : 1);
-    }
-    r->use = use;
-    r->lines = lines;
-    r->records = (lines + 2) / 4;  // sequence lines 1, 5, 9, ... below `lines`
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// One wave per record: its sequence line's class, length and kept flag.
-__global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
-                                                          FqResult *r, uint32_t *kept, uint32_t *len) {
-    const uint64_t rec = (uint64_t)blockIdx.x * (FQ_THREADS / 64) + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
-    if (rec >= r->records) return;
-    const uint64_t i = 4 * rec + 1;
-    const uint64_t s = nlpos[i - 1] + 1;
-    const uint64_t e = i < r->newlines ? nlpos[i] : size;
-    bool drop = false, bad = false;
-    for (uint64_t p = s + lane; p < e; p += 64) {
-        const uint8_t c = d[p];
-        const bool cls2 = c >= 64 && c < 128 && (c & 8);
-        const uint8_t lc = c | 0x20;
-        const bool ok = lc == 'a' || lc == 'c' || lc == 'g' || lc == 't';
-        drop |= cls2;
-        bad |= !ok && !cls2;
+// The block chain (read_block's framing, preprocess.cu:46-62,207-222), one wave. Block j starts at
+// pos_j, a record boundary (4 * records-before-it newlines precede it), and holds min(B, rest)
+// bytes; a block shorter than B is the last (when the range ends the file, `range_final`; else the
+// rest waits for more data). With `one_block` the range is one block whose finality is `one_final`.
+// Newlines before a position come from the chunk offsets plus the wave's count over that chunk.
+__global__ void __launch_bounds__(64) k_fq_chain(const uint8_t *d, uint64_t size, uint64_t B, int range_final,
+                                               int one_block, int one_final, const unsigned long long *choff,
+                                               uint32_t nchunks, const uint64_t *nlpos, FqResult *r) {
+    const uint32_t lane = threadIdx.x;
+    const uint64_t NL = choff[nchunks];
+    uint64_t pos = 0, recs = 0, blocks = 0;
+    uint32_t err = 0;
+    while (pos < size) {
+        const uint64_t got = one_block ? size : min(B, size - pos);
+        bool final;
+        if (one_block) {
+            final = one_final != 0;
+        } else {
+            if (got != B && !range_final) break;  // the rest of the range waits for more data
+            final = got != B;
+        }
+        if (d[pos] != '@') {
+            err = FQ_ERR_FORMAT;
+            break;
+        }
+        const uint64_t g0 = 4 * recs;  // newlines before pos
+        uint64_t lines, use;
+        if (final) {
+            lines = (NL - g0) + (d[size - 1] == '\n' ? 0 : 1);
+            use = got;
+        } else {
+            const uint64_t x = pos + got, c = x / FQ_CHUNK;
+            uint32_t k = 0;
+            for (uint64_t p = c * FQ_CHUNK + lane; p < x; p += 64) k += d[p] == '\n' ? 1u : 0u;
+            const uint64_t nlb = choff[c] + wave_sum(k) - g0;  // newlines in [pos, x)
+            const bool ends = d[x - 1] == '\n';
+            const uint64_t n_lines = nlb - (ends ? 1 : 0) + 1;
+            if (n_lines % 4 == 0 && ends) {
+                lines = nlb;
+                use = got;
+            } else {
+                const uint64_t keep = ((n_lines - 1) / 4) * 4;
+                if (keep == 0) {  // a record longer than the block
+                    err = FQ_ERR_FORMAT;
+                    break;
+                }
+                lines = keep;
+                use = nlpos[g0 + keep - 1] + 1 - pos;
+            }
+        }
+        recs += (lines + 2) / 4;  // sequence lines 1, 5, 9, ... below `lines`
+        pos += use;
+        ++blocks;
+        if (final || one_block) break;
     }
-    drop = __any(drop);
-    bad = __any(bad);
     if (lane == 0) {
+        r->newlines = NL;
+        r->use = pos;
+        r->records = recs;
+        r->blocks = blocks;
+        r->err = err;
+    }
+}
+
+// The aligned dword at absolute address a, with `valid` flagging (bit 7 of each byte) its bytes
+// inside [lo, hi). Loaded whole when it lies inside the range [d, d + size), byte by byte (the
+// valid bytes only) otherwise.
+__device__ __forceinline__ uint32_t window_dword(const uint8_t *d, uint64_t size, uintptr_t a, uintptr_t lo, uintptr_t hi,
+                                                 uint32_t &valid) {
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
+    valid = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (a + i >= lo && a + i < hi) valid |= 0x80u << (8 * i);
+    if (a >= db && a + 4 <= db + size) return *reinterpret_cast<const uint32_t *>(a);
+    uint32_t w = 0;
+    for (int i = 0; i < 4; ++i)
+        if ((valid >> (8 * i + 7)) & 1u) w |= (uint32_t)d[a + i - db] << (8 * i);
+    return w;
+}
+
+// Bit 7 of each byte of w set where that byte is not one of ACGTacgt.
+__device__ __forceinline__ uint32_t fq_nz(uint32_t v) { return (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u; }
+__device__ __forceinline__ uint32_t non_bases(uint32_t w) {
+    const uint32_t x = w | 0x20202020u;
+    return fq_nz(x ^ 0x61616161u) & fq_nz(x ^ 0x63636363u) & fq_nz(x ^ 0x67676767u) & fq_nz(x ^ 0x74747474u);
+}
+// Bit 7 of each byte set where the byte is in [64, 127] with bit 3 set (drops the read): bit 7
+// clear, bit 6 set (moved to bit 7 by the shift by 1), bit 3 set (by the shift by 4).
+__device__ __forceinline__ uint32_t drop_bytes(uint32_t w) { return ~w & (w << 1) & (w << 4) & 0x80808080u; }
+
+// Sequence line of record rec: [s, e).
+__device__ __forceinline__ void seq_line(const uint64_t *nlpos, uint64_t NL, uint64_t size, uint64_t rec, uint64_t &s,
+                                         uint64_t &e) {
+    s = nlpos[4 * rec] + 1;
+    e = 4 * rec + 1 < NL ? nlpos[4 * rec + 1] : size;
+}
+
+// 16 lanes per record: its sequence line's class, length and kept flag. The lanes take the
+// line's aligned dwords in turn; the group's flags meet in one ballot.
+__global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
+                                                          uint64_t NL, uint64_t records, uint32_t *kept, uint32_t *len,
+                                                          FqResult *r) {
+    const uint64_t rec = ((uint64_t)blockIdx.x * FQ_THREADS + threadIdx.x) / FQ_LANES;
+    const uint32_t sub = threadIdx.x & (FQ_LANES - 1);
+    bool drop = false, bad = false;
+    uint64_t s = 0, e = 0;
+    if (rec < records) {
+        seq_line(nlpos, NL, size, rec, s, e);
+        const uintptr_t lo = reinterpret_cast<uintptr_t>(d) + s, hi = reinterpret_cast<uintptr_t>(d) + e;
+        for (uintptr_t a = (lo & ~(uintptr_t)3) + 4 * sub; a < hi; a += 4 * FQ_LANES) {
+            uint32_t valid;
+            const uint32_t w = window_dword(d, size, a, lo, hi, valid);
+            const uint32_t dr = drop_bytes(w) & valid;
+            drop |= dr != 0;
+            bad |= (non_bases(w) & ~dr & valid) != 0;
+        }
+    }
+    const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(FQ_LANES - 1);
+    const bool gdrop = ((__ballot(drop) >> gshift) & 0xFFFFull) != 0;
+    const bool gbad = ((__ballot(bad) >> gshift) & 0xFFFFull) != 0;
+    if (rec < records && sub == 0) {
         const uint64_t n = e - s;
-        const uint32_t k = (n > 0 && !drop && !bad) ? 1u : 0u;
+        const uint32_t k = (n > 0 && !gdrop && !gbad) ? 1u : 0u;
         kept[rec] = k;
         len[rec] = (uint32_t)(n < 0xFFFFFFFFull ? n : 0xFFFFFFFFull);
-        if (n > 0 && !drop && bad) atomicMin(&r->first_bad, (unsigned long long)rec);
-        if (k) atomicMin(&r->first_kept, (unsigned long long)rec);
+        if (n > 0 && !gdrop && gbad) atomicMin(&r->first_bad, (unsigned long long)rec);
     }
 }
 
-// First kept record whose length is not L (or is over MAX_RD_LEN).
+// The first kept record (the only kept one of rank 0) and its length; the kept total.
+__global__ void __launch_bounds__(256) k_fq_first(const uint32_t *kept, const uint32_t *len,
+                                                const unsigned long long *rank, uint64_t records, FqResult *r) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < records && kept[i] && rank[i] == 0) {
+        r->first_kept = i;
+        r->kept_len = len[i];
+    }
+    if (i == 0) r->kept = rank[records];
+}
+
+// First kept record whose length is not L (L == 0: the first kept record's length), or over
+// MAX_RD_LEN.
 __global__ void __launch_bounds__(256) k_fq_lencheck(const uint32_t *kept, const uint32_t *len, uint64_t records,
                                                    uint32_t L, FqResult *r) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= records || !kept[i]) return;
-    if (len[i] != L || len[i] > (uint32_t)MAX_RD_LEN) atomicMin(&r->first_len, (unsigned long long)i);
+    const uint32_t want = L ? L : r->kept_len;
+    if (len[i] != want || len[i] > (uint32_t)MAX_RD_LEN) atomicMin(&r->first_len, (unsigned long long)i);
 }
 
-// One wave per record: kept ones copy their bases to out + (at + rank) * L.
-__global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, const uint64_t *nlpos, uint64_t records,
-                                                      const uint32_t *kept, const unsigned long long *rank,
-                                                      uint32_t L, uint8_t *out) {
-    const uint64_t rec = (uint64_t)blockIdx.x * (FQ_THREADS / 64) + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
+// 16 lanes per record: kept ones copy their bases to out + rank * L.
+__global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
+                                                      uint64_t NL, uint64_t records, const uint32_t *kept,
+                                                      const unsigned long long *rank, uint32_t L, uint8_t *out) {
+    const uint64_t rec = ((uint64_t)blockIdx.x * FQ_THREADS + threadIdx.x) / FQ_LANES;
+    const uint32_t sub = threadIdx.x & (FQ_LANES - 1);
     if (rec >= records || !kept[rec]) return;
-    const uint64_t s = nlpos[4 * rec] + 1;
+    uint64_t s, e;
+    seq_line(nlpos, NL, size, rec, s, e);
     uint8_t *o = out + rank[rec] * (uint64_t)L;
-    for (uint32_t p = lane; p < L; p += 64) o[p] = d[s + p];
+    const uint8_t *src = d + s;
+    for (uint32_t p = sub; p < L; p += FQ_LANES) o[p] = src[p];
 }
 
 }  // namespace
 
 uint64_t fq_chunks(uint64_t size) { return (size + FQ_CHUNK - 1) / FQ_CHUNK; }
 
+uint64_t fq_scan_blocks(uint64_t n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK + 1; }
+
 hipError_t launch_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt, unsigned long long *choff,
-                           hipStream_t s) {
+                           unsigned long long *blk, hipStream_t s) {
     const uint64_t nch = fq_chunks(size);
     if (nch) {
         hipLaunchKernelGGL(k_fq_count, dim3((uint32_t)nch), dim3(FQ_THREADS), 0, s, d, size, cnt);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_fq_scan, dim3(1), dim3(1024), 0, s, cnt, (uint32_t)nch, choff);
-    return hipGetLastError();
+    return launch_scan(cnt, nch, choff, blk, s);
 }
 
-hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, int final, const unsigned long long *choff,
-                           uint64_t *nlpos, FqResult *r, hipStream_t s) {
+hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, uint64_t B, int range_final, int one_block, int one_final,
+                           const unsigned long long *choff, uint64_t *nlpos, FqResult *r, hipStream_t s) {
     const uint64_t nch = fq_chunks(size);
     if (nch) {
         hipLaunchKernelGGL(k_fq_lines, dim3((uint32_t)nch), dim3(FQ_THREADS), 0, s, d, size, choff, nlpos);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_fq_frame, dim3(1), dim3(64), 0, s, d, size, final, choff, (uint32_t)nch, nlpos, r);
+    hipLaunchKernelGGL(k_fq_chain, dim3(1), dim3(64), 0, s, d, size, B, range_final, one_block, one_final, choff,
+                       (uint32_t)nch, nlpos, r);
     return hipGetLastError();
 }
 
-hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t records, FqResult *r,
-                              uint32_t *kept, uint32_t *len, unsigned long long *rank, hipStream_t s) {
+hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
+                              FqResult *r, uint32_t *kept, uint32_t *len, unsigned long long *rank,
+                              unsigned long long *blk, uint32_t L, hipStream_t s) {
     if (records == 0) return hipSuccess;
-    const uint32_t per = FQ_THREADS / 64;
-    hipLaunchKernelGGL(k_fq_classify, dim3((uint32_t)((records + per - 1) / per)), dim3(FQ_THREADS), 0, s, d, size,
-                       nlpos, r, kept, len);
+    const uint64_t groups = (records * FQ_LANES + FQ_THREADS - 1) / FQ_THREADS;
+    hipLaunchKernelGGL(k_fq_classify, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records,
+                       kept, len, r);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fq_scan, dim3(1), dim3(1024), 0, s, kept, (uint32_t)records, rank);
+    if ((e = launch_scan(kept, records, rank, blk, s)) != hipSuccess) return e;
+    const uint32_t g = (uint32_t)((records + 255) / 256);
+    hipLaunchKernelGGL(k_fq_first, dim3(g), dim3(256), 0, s, kept, len, rank, records, r);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fq_lencheck, dim3(g), dim3(256), 0, s, kept, len, records, L, r);
     return hipGetLastError();
 }
 
-hipError_t launch_fq_lencheck(const uint32_t *kept, const uint32_t *len, uint64_t records, uint32_t L, FqResult *r,
-                              hipStream_t s) {
+hipError_t launch_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
+                          const uint32_t *kept, const unsigned long long *rank, uint32_t L, uint8_t *out,
+                          hipStream_t s) {
     if (records == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fq_lencheck, dim3((uint32_t)((records + 255) / 256)), dim3(256), 0, s, kept, len, records, L,
-                       r);
-    return hipGetLastError();
-}
-
-hipError_t launch_fq_copy(const uint8_t *d, const uint64_t *nlpos, uint64_t records, const uint32_t *kept,
-                          const unsigned long long *rank, uint32_t L, uint8_t *out, hipStream_t s) {
-    if (records == 0) return hipSuccess;
-    const uint32_t per = FQ_THREADS / 64;
-    hipLaunchKernelGGL(k_fq_copy, dim3((uint32_t)((records + per - 1) / per)), dim3(FQ_THREADS), 0, s, d, nlpos,
-                       records, kept, rank, L, out);
+    const uint64_t groups = (records * FQ_LANES + FQ_THREADS - 1) / FQ_THREADS;
+    hipLaunchKernelGGL(k_fq_copy, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records, kept,
+                       rank, L, out);
     return hipGetLastError();
 }
 

@@ -257,11 +257,12 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 // One round's edges in output order: the n bucket slots (counts cnt, pool offsets src_off) scanned
 // into dst_off from *run (the output position after the earlier rounds; advanced by the round's
-// edges; rec = {edges, first position}), then copied from the pool to out_from / out_to (which may
-// be pinned host memory). Two launches on stream s.
+// edges; rec = {edges, first position, *ovf_cursor as read after the round's join}), then copied
+// from the pool to out_from / out_to (which may be pinned host memory). Two launches on stream s.
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
-                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint2 *pool,
-                              uint32_t *out_from, uint32_t *out_to, hipStream_t s);
+                              uint32_t n, unsigned long long *run, unsigned long long *rec,
+                              const unsigned long long *ovf_cursor, const uint2 *pool, uint32_t *out_from,
+                              uint32_t *out_to, hipStream_t s);
 // n work-groups: all buckets (a.ids null; buckets over a.cap are skipped) or the listed ones.
 hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,

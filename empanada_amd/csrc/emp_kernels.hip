@@ -1280,11 +1280,14 @@ __global__ void __launch_bounds__(1024) k_escan_top(unsigned long long *__restri
 
 // One round's edge offsets in the output, in bucket order: dst[i] = base + the edges of slots
 // before i, with base = *run (the edges of the rounds before) ; then *run += the round's edges and
-// rec = {edges, base}. One work-group; rounds of one run are scanned in order on one stream.
+// rec = {edges, base, the edge pool's overflow cursor as read now}. One work-group; rounds of one
+// run are scanned in order on one stream, after their join and while later joins may run (the
+// cursor read is then at least its value after this round's join).
 __global__ void __launch_bounds__(1024) k_round_scan(const uint32_t *__restrict__ cnt, uint32_t n,
                                                     unsigned long long *__restrict__ dst,
                                                     unsigned long long *__restrict__ run,
-                                                    unsigned long long *__restrict__ rec) {
+                                                    unsigned long long *__restrict__ rec,
+                                                    const unsigned long long *ovf_cursor) {
     __shared__ unsigned long long sh[16];
     const unsigned long long base = *run;
     unsigned long long carry = base;
@@ -1309,6 +1312,7 @@ __global__ void __launch_bounds__(1024) k_round_scan(const uint32_t *__restrict_
     if (threadIdx.x == 0) {
         rec[0] = carry - base;
         rec[1] = base;
+        rec[2] = __hip_atomic_load(ovf_cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *run = carry;
     }
 }
@@ -1470,10 +1474,11 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
 }
 
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
-                              uint32_t n, unsigned long long *run, unsigned long long *rec, const uint2 *pool,
-                              uint32_t *out_from, uint32_t *out_to, hipStream_t s) {
+                              uint32_t n, unsigned long long *run, unsigned long long *rec,
+                              const unsigned long long *ovf_cursor, const uint2 *pool, uint32_t *out_from,
+                              uint32_t *out_to, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec);
+    hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec, ovf_cursor);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     EdgeCopyArgs a{cnt, src_off, dst_off, nullptr, n, pool, out_from, out_to};

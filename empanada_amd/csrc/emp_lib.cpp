@@ -5,16 +5,24 @@
 // singletons (Config, Kmp/Vmp, M, d_buf: config.h:112-115, mempool.cuh:89-91, overlap.cu:41-43),
 // the host CQueues become HBM-resident bucketed lists, and every Thrust stage is a kernel of
 // emp_kernels.hip.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -54,7 +62,7 @@ struct DevCounters {
     // pipelined rounds stream their edges out: output position after the rounds copied so far,
     // and per round {edges, first position}
     unsigned long long obase;
-    unsigned long long orec[MAX_ROUNDS][2];
+    unsigned long long orec[MAX_ROUNDS][3];
 };
 
 double now_ms() {
@@ -139,9 +147,17 @@ struct emp_ctx {
     uint32_t *d_fq_kept = nullptr, *d_fq_len = nullptr;
     unsigned long long *d_fq_rank = nullptr;
     uint64_t fq_rec_cap = 0, fq_len_cap = 0, fq_rank_cap = 0;
+    unsigned long long *d_fq_blk = nullptr;  // block totals of the parser's scans
+    uint64_t fq_blk_cap = 0;
     FqResult *d_fq_res = nullptr, *h_fq_res = nullptr;
     uint8_t *d_acc = nullptr;
     uint64_t acc_bytes = 0, acc_n = 0;
+    // FASTQ files streamed into HBM (emp_add_fastq_file): the file's bytes, two pinned chunks
+    // and the stream that copies them
+    uint8_t *d_file = nullptr;
+    uint64_t file_cap = 0;
+    char *h_chunk[2] = {nullptr, nullptr};
+    hipStream_t cstream = nullptr;
 
     // overlap phase. Bucket set `set` holds round t's lists for t % 2 == set, per side (0 =
     // suffixes, 1 = prefixes): the dedup writes set 0, round t's join writes set (t + 1) % 2.
@@ -189,8 +205,14 @@ struct emp_ctx {
     // second stream, while later rounds run
     hipStream_t ostream = nullptr;
     bool out_streamed = false;
-    unsigned long long (*h_orec)[2] = nullptr;  // per round {edges, first position}, pinned
+    unsigned long long (*h_orec)[3] = nullptr;  // per round {edges, first position, overflow cursor}, pinned
     std::vector<hipEvent_t> orec_ev;
+
+    // emp_run_to_file: records written while the rounds run (a writer thread), and how many of
+    // the run's records are already in the file
+    FILE *out_file = nullptr;
+    uint64_t emitted = 0;
+    std::vector<hipEvent_t> dma_ev;  // per round: its edges are in the pinned host arena
 
     // statistics
     std::vector<emp_round_stats> rounds;
@@ -509,10 +531,16 @@ int read_result(emp_ctx *c) {
     return EMP_OK;
 }
 
-// read_block + the map phase on the device (preprocess.cu:195-237, fingerprint.cu:75-153): the
-// block is copied to HBM and parsed there; its kept reads join the device accumulation, which is
-// mapped in multiples of 16 reads (the rest waits for the next block or the run).
-int add_fastq_gpu(emp_ctx *c, const char *data, size_t size, bool final, size_t *consumed) {
+// read_block + the map phase on the device (preprocess.cu:195-237, fingerprint.cu:75-153) over a
+// range of FASTQ bytes: the caller's host block (copied to HBM first) or bytes already in HBM. The
+// range is parsed in blocks of B bytes, each resumed after the previous one's complete records, as
+// the CLI's loop reads a file (main.cu:42-58); with `one_block` it is a single block of any size,
+// final when `final` (emp_add_fastq's contract). Otherwise a block shorter than B is the last one
+// only when `final` (the range ends the file); else parsing stops before it and *consumed says
+// where. Kept reads join the device accumulation, which is mapped in multiples of 16 reads (the
+// rest waits for the next range or the run). Three host synchronisations per range.
+int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool one_block, bool final,
+                    size_t *consumed, bool on_device) {
     *consumed = 0;
     if (size == 0) return EMP_OK;
     int rc;
@@ -526,9 +554,11 @@ int add_fastq_gpu(emp_ctx *c, const char *data, size_t size, bool final, size_t 
         c->stage_n = 0;
     }
     const uint64_t nch = fq_chunks(size);
-    if ((rc = ensure_dev(c, &c->d_fq, c->fq_cap, size))) return rc;
+    if (!on_device && (rc = ensure_dev(c, &c->d_fq, c->fq_cap, size))) return rc;
+    const uint8_t *d = on_device ? reinterpret_cast<const uint8_t *>(data) : c->d_fq;
     if ((rc = ensure_dev(c, &c->d_fq_cnt, c->fq_chunk_cap, nch + 1))) return rc;
     if ((rc = ensure_dev(c, &c->d_fq_choff, c->fq_choff_cap, nch + 1))) return rc;
+    if ((rc = ensure_dev(c, &c->d_fq_blk, c->fq_blk_cap, fq_scan_blocks(nch)))) return rc;
     if (!c->d_fq_res) {
         void *q = nullptr;
         if ((rc = dalloc(c, &q, sizeof(FqResult)))) return rc;
@@ -539,58 +569,56 @@ int add_fastq_gpu(emp_ctx *c, const char *data, size_t size, bool final, size_t 
     R = FqResult{};
     R.first_bad = R.first_kept = R.first_len = ~0ull;
     HIPCHK(c, hipMemcpyAsync(c->d_fq_res, c->h_fq_res, sizeof(FqResult), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->d_fq, data, size, hipMemcpyHostToDevice, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(c->d_fq, data, size, hipMemcpyHostToDevice, c->stream));
     {
-        Timed t(c, "k_fastq", 2.0 * size);
-        HIPCHK(c, launch_fq_count(c->d_fq, size, c->d_fq_cnt, c->d_fq_choff, c->stream));
+        Timed t(c, "k_fastq", 1.0 * size);
+        HIPCHK(c, launch_fq_count(d, size, c->d_fq_cnt, c->d_fq_choff, c->d_fq_blk, c->stream));
     }
     unsigned long long nl = 0;
     HIPCHK(c, hipMemcpyAsync(&nl, c->d_fq_choff + nch, sizeof nl, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((rc = ensure_dev(c, &c->d_fq_nl, c->fq_nl_cap, nl + 1))) return rc;
     {
-        Timed t(c, "k_fastq", 1.0 * size);
-        HIPCHK(c, launch_fq_frame(c->d_fq, size, final ? 1 : 0, c->d_fq_choff, c->d_fq_nl, c->d_fq_res, c->stream));
+        Timed t(c, "k_fastq", 1.0 * size + 8.0 * nl);  // the range again, a position per newline
+        HIPCHK(c, launch_fq_frame(d, size, B, final ? 1 : 0, one_block ? 1 : 0, final ? 1 : 0, c->d_fq_choff,
+                                  c->d_fq_nl, c->d_fq_res, c->stream));
     }
     if ((rc = read_result(c))) return rc;
-    if (R.err == FQ_ERR_FORMAT)
-        return fail(c, EMP_ERR_FORMAT, "malformed FASTQ block: it must start with '@' and hold a complete record");
     const uint64_t recs = R.records;
+    const bool fresh = c->n == 0 && c->stage_n == 0 && c->acc_n == 0;  // the first kept read sets the length
     if (recs) {
         if ((rc = ensure_dev(c, &c->d_fq_kept, c->fq_rec_cap, recs))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_len, c->fq_len_cap, recs))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_rank, c->fq_rank_cap, recs + 1))) return rc;
+        if ((rc = ensure_dev(c, &c->d_fq_blk, c->fq_blk_cap, fq_scan_blocks(recs)))) return rc;
         {
-            Timed t(c, "k_fastq", 1.0 * size);
-            HIPCHK(c, launch_fq_classify(c->d_fq, size, c->d_fq_nl, recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len,
-                                         c->d_fq_rank, c->stream));
+            // two newline positions and the sequence line per record (about a quarter of a record),
+            // kept flag, length and offset written, each read twice more
+            Timed t(c, "k_fastq", 0.25 * R.use + 48.0 * recs);
+            HIPCHK(c, launch_fq_classify(d, size, c->d_fq_nl, nl, recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len,
+                                         c->d_fq_rank, c->d_fq_blk, fresh ? 0u : (uint32_t)c->L, c->stream));
         }
         if ((rc = read_result(c))) return rc;
     }
-    uint64_t kept = 0;
-    uint32_t L = (uint32_t)c->L;
-    if (R.first_kept != ~0ull) {
-        if (c->n == 0 && c->stage_n == 0 && c->acc_n == 0) {  // the first read sets the length
-            HIPCHK(c, hipMemcpyAsync(&L, c->d_fq_len + R.first_kept, sizeof L, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-        HIPCHK(c, launch_fq_lencheck(c->d_fq_kept, c->d_fq_len, recs, L, c->d_fq_res, c->stream));
-        HIPCHK(c, hipMemcpyAsync(&kept, c->d_fq_rank + recs, sizeof kept, hipMemcpyDeviceToHost, c->stream));
-        if ((rc = read_result(c))) return rc;
-    }
-    // the first offending record decides the error, as the host reader meets them in order
+    // The first offending record in file order decides the error, as the host reader meets them
+    // in order; a block that cannot be framed comes after every record of the blocks before it.
+    const uint32_t L = fresh ? R.kept_len : (uint32_t)c->L;
     if (R.first_bad != ~0ull && R.first_bad < R.first_len)
         return fail(c, EMP_ERR_BASE, "a kept read holds a byte other than ACGTacgt");
     if (R.first_len != ~0ull) {
         if (L > MAX_RD_LEN) return fail(c, EMP_ERR_READLEN, "read length %u outside [1, 255]", L);
         return fail(c, EMP_ERR_READLEN, "All reads must be of equal length (%d vs other)", (int)L);
     }
+    if (R.err == FQ_ERR_FORMAT)
+        return fail(c, EMP_ERR_FORMAT, "malformed FASTQ block %llu: it must start with '@' and hold a complete record",
+                    (unsigned long long)R.blocks);
+    const uint64_t kept = R.kept;
     if (kept) {
         if ((rc = set_len(c, (int)L))) return rc;
         if ((rc = ensure_dev_keep(c, &c->d_acc, c->acc_bytes, (c->acc_n + kept) * (uint64_t)L))) return rc;
         {
-            Timed t(c, "k_fastq", 2.0 * kept * L);
-            HIPCHK(c, launch_fq_copy(c->d_fq, c->d_fq_nl, recs, c->d_fq_kept, c->d_fq_rank, L,
+            Timed t(c, "k_fastq", 2.0 * kept * L + 24.0 * recs);
+            HIPCHK(c, launch_fq_copy(d, size, c->d_fq_nl, nl, recs, c->d_fq_kept, c->d_fq_rank, L,
                                      c->d_acc + c->acc_n * (uint64_t)L, c->stream));
         }
         c->acc_n += kept;
@@ -606,6 +634,134 @@ int add_fastq_gpu(emp_ctx *c, const char *data, size_t size, bool final, size_t 
     }
     *consumed = (size_t)R.use;
     return EMP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// A FASTQ file streamed into HBM: the CLI's block loop (main.cu:42-58 over read_block,
+// preprocess.cu:195-237) with the file read ahead. A reader thread fills two pinned chunks of
+// FILE_CHUNK bytes with parallel preads and copies each to the device image of the file on a copy
+// stream; block j (map_mem/16 bytes from the position block j-1 left off) is parsed on the
+// device as soon as the chunks it covers have arrived, while later chunks are read and copied.
+constexpr uint64_t FILE_CHUNK = 64ull << 20;
+constexpr int FILE_READERS = 4;
+
+int add_fastq_file(emp_ctx *c, const char *path) {
+    const uint64_t B = c->opts.map_mem / 16;  // main.cu:16
+    if (B == 0) return fail(c, EMP_ERR_ARG, "map_mem / 16 is zero");
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) return fail(c, EMP_ERR_IO, "Could not open %s", path);
+    struct stat stt;
+    if (fstat(fd, &stt) != 0) {
+        close(fd);
+        return fail(c, EMP_ERR_IO, "Could not stat %s", path);
+    }
+    const uint64_t size = (uint64_t)stt.st_size;
+    int rc = EMP_OK;
+    if (size == 0) {
+        close(fd);
+        return EMP_OK;
+    }
+    if ((rc = ensure_dev(c, &c->d_file, c->file_cap, size))) {
+        close(fd);
+        return rc;
+    }
+    if (!c->cstream && hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking) != hipSuccess) {
+        close(fd);
+        return fail(c, EMP_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (int i = 0; i < 2; ++i)
+        if (!c->h_chunk[i] && hipHostMalloc((void **)&c->h_chunk[i], FILE_CHUNK, hipHostMallocDefault) != hipSuccess) {
+            close(fd);
+            return fail(c, EMP_ERR_NOMEM, "hipHostMalloc of a %llu-byte chunk failed",
+                        (unsigned long long)FILE_CHUNK);
+        }
+    const uint64_t nchunks = (size + FILE_CHUNK - 1) / FILE_CHUNK;
+    std::vector<hipEvent_t> ev(nchunks, nullptr);
+    for (auto &e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            rc = fail(c, EMP_ERR_HIP, "hipEventCreate failed");
+            break;
+        }
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t enqueued = 0;        // chunks whose copy (and event) is on the copy stream
+    bool reader_failed = false;
+    std::atomic<bool> stop{false};
+    std::thread reader;
+    if (!rc) reader = std::thread([&]() {
+        const int dev = c->device;
+        (void)hipSetDevice(dev);
+        for (uint64_t k = 0; k < nchunks && !stop; ++k) {
+            const int slot = (int)(k & 1);
+            if (k >= 2 && hipEventSynchronize(ev[k - 2]) != hipSuccess) {  // the slot's previous copy is done
+                std::lock_guard<std::mutex> g(mu);
+                reader_failed = true;
+                break;
+            }
+            const uint64_t off = k * FILE_CHUNK, len = std::min(FILE_CHUNK, size - off);
+            std::atomic<bool> bad{false};
+            auto part = [&](int p) {
+                const uint64_t per = (len + FILE_READERS - 1) / FILE_READERS;
+                uint64_t a = std::min(len, per * (uint64_t)p), e = std::min(len, a + per);
+                while (a < e) {
+                    const ssize_t got = pread(fd, c->h_chunk[slot] + a, (size_t)(e - a), (off_t)(off + a));
+                    if (got <= 0) {
+                        bad = true;
+                        return;
+                    }
+                    a += (uint64_t)got;
+                }
+            };
+            std::thread helpers[FILE_READERS - 1];
+            for (int p = 1; p < FILE_READERS; ++p) helpers[p - 1] = std::thread(part, p);
+            part(0);
+            for (auto &h : helpers) h.join();
+            bool ok = !bad && hipMemcpyAsync(c->d_file + off, c->h_chunk[slot], len, hipMemcpyHostToDevice,
+                                             c->cstream) == hipSuccess &&
+                      hipEventRecord(ev[k], c->cstream) == hipSuccess;
+            std::lock_guard<std::mutex> g(mu);
+            if (!ok) {
+                reader_failed = true;
+                break;
+            }
+            enqueued = k + 1;
+            cv.notify_all();
+        }
+        cv.notify_all();
+    });
+    // The arrived prefix is parsed in ranges of at least PARSE_STEP bytes (whole blocks of B each,
+    // the short last one only with the file's end), while later chunks stream in.
+    const uint64_t PARSE_STEP = std::max<uint64_t>(B, 8 * FILE_CHUNK);
+    uint64_t pos = 0, waited = 0;  // waited: chunks the compute stream already waits for
+    while (!rc && pos < size) {
+        const uint64_t want = std::min(size, std::max(pos + PARSE_STEP, pos + B));
+        const uint64_t need = (want + FILE_CHUNK - 1) / FILE_CHUNK;  // chunks [0, need) hold the range
+        {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&]() { return enqueued >= need || reader_failed; });
+            if (enqueued < need) {
+                rc = fail(c, EMP_ERR_IO, "reading %s failed", path);
+                break;
+            }
+        }
+        for (; waited < need; ++waited)
+            if (hipStreamWaitEvent(c->stream, ev[waited], 0) != hipSuccess) rc = fail(c, EMP_ERR_HIP, "hipStreamWaitEvent");
+        if (rc) break;
+        const bool last = want == size;
+        size_t used = 0;
+        if ((rc = add_fastq_range(c, reinterpret_cast<const char *>(c->d_file + pos), (size_t)(want - pos), B, false,
+                                  last, &used, true)))
+            break;
+        if (last) break;
+        pos += used;
+    }
+    stop = true;
+    if (reader.joinable()) reader.join();
+    close(fd);
+    (void)hipStreamSynchronize(c->cstream);
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 // Maps what the device accumulation still holds (the run's last batch).
@@ -878,11 +1034,13 @@ int flush_records(emp_ctx *c, emp_record_sink sink, void *user) {
     }
     int rc = sync(c);
     if (rc) return rc;
-    for (const auto &r : c->records) {
+    for (size_t i = (size_t)std::min<uint64_t>(c->emitted, c->records.size()); i < c->records.size(); ++i) {
+        const auto &r = c->records[i];
         if (!sink) continue;
         int src = sink(user, r.n, r.ovlp, c->h_from + r.off, c->h_to + r.off);
         if (src) return fail(c, src, "record sink returned %d", src);
     }
+    c->emitted = 0;
     c->records.clear();
     c->out_used = 0;
     c->out_streamed = false;
@@ -1375,6 +1533,10 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
                                      c->dspill_cap, 1u << bbD, 0, L, false)))
         return rc;
     st.D = c->h_ctr->r[0].total[0];
+    if (c->opts.profile) {  // per kept entry: two 12-byte entries written (both sides), its 4-byte ends word
+        int id = kstat_id(c, "k_dedup");
+        c->kstats[id].bytes += 28.0 * (double)st.D;
+    }
     return EMP_OK;
 }
 
@@ -1517,6 +1679,61 @@ int rounds_sync(emp_ctx *c, RunState &st) {
     return EMP_OK;
 }
 
+int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to);
+
+// Writes records to the output file in order while the rounds run: each waits for the DMA of
+// its round's edges into the pinned host arena (its event), then goes out as write_edges_binary
+// lays it out (overlap.cu:148-161).
+struct RecordWriter {
+    FILE *f;
+    const uint32_t *from, *to;
+    struct Item {
+        uint64_t n, off;
+        int32_t ovlp;
+        hipEvent_t ev;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Item> q;
+    size_t head = 0;
+    bool done = false;
+    int err = 0;
+    std::thread th;
+    int device;
+    RecordWriter(FILE *f_, const uint32_t *fr, const uint32_t *t, int dev) : f(f_), from(fr), to(t), device(dev) {
+        th = std::thread([this]() {
+            (void)hipSetDevice(device);
+            for (;;) {
+                Item it;
+                {
+                    std::unique_lock<std::mutex> g(mu);
+                    cv.wait(g, [this]() { return head < q.size() || done; });
+                    if (head == q.size()) return;
+                    it = q[head++];
+                }
+                if (err) continue;
+                if (hipEventSynchronize(it.ev) != hipSuccess) err = EMP_ERR_HIP;
+                else err = file_sink(f, it.n, it.ovlp, from + it.off, to + it.off);
+            }
+        });
+    }
+    void push(uint64_t n, uint64_t off, int32_t ovlp, hipEvent_t ev) {
+        std::lock_guard<std::mutex> g(mu);
+        q.push_back(Item{n, off, ovlp, ev});
+        cv.notify_one();
+    }
+    int finish() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            done = true;
+            cv.notify_one();
+        }
+        if (th.joinable()) th.join();
+        return err;
+    }
+    ~RecordWriter() { finish(); }
+};
+
 // Rounds back to back: no host synchronisation between joins. Each join clears the bucket counts
 // it has read (the next writer of that set needs them zero) and fills round t+1's counter record;
 // the host reads each record back one round late, only to size the buckets of the round after
@@ -1536,13 +1753,40 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     if (!c->ostream) HIPCHK(c, hipStreamCreateWithFlags(&c->ostream, hipStreamNonBlocking));
     if (!c->h_orec) {
         void *p = nullptr;
-        HIPCHK(c, hipHostMalloc(&p, MAX_ROUNDS * 2 * sizeof(unsigned long long)));
-        c->h_orec = (unsigned long long(*)[2])p;
+        HIPCHK(c, hipHostMalloc(&p, MAX_ROUNDS * 3 * sizeof(unsigned long long)));
+        c->h_orec = (unsigned long long(*)[3])p;
     }
     if (c->orec_ev.empty()) {
         c->orec_ev.resize(MAX_ROUNDS);
         for (auto &e : c->orec_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    // With an output file, every round whose record is final goes to the file while later rounds
+    // run: its join handed no bucket to the large-bucket path and overflowed no spill area, the
+    // lists it read were complete (the same for every earlier round, or the run has already
+    // fallen back), and no edge of it or of an earlier round missed the pool (the overflow
+    // cursor, read after its join, is within the pool). A run that falls back later repeats those
+    // rounds with the same records; they are not written again.
+    std::unique_ptr<RecordWriter> writer;
+    if (c->out_file) {
+        if (c->dma_ev.empty()) {
+            c->dma_ev.resize(MAX_ROUNDS);
+            for (auto &e : c->dma_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        writer.reset(new RecordWriter(c->out_file, c->h_from, c->h_to, c->device));
+    }
+    bool emitting = writer != nullptr;
+    auto emit_round = [&](int r) -> int {  // the edges of rounds <= r are in the device arena
+        if (!emitting) return EMP_OK;
+        const unsigned long long *o = c->h_orec[r];
+        if ((uint64_t)r != c->emitted || o[2] > c->pool_ovf_cap || o[0] >= (1ull << 31)) {
+            emitting = false;  // not final here: this round and the ones after it wait for the end
+            return EMP_OK;
+        }
+        HIPCHK(c, hipEventRecord(c->dma_ev[r], c->ostream));
+        writer->push(o[0], o[1], L - 1 - r, c->dma_ev[r]);
+        ++c->emitted;
+        return EMP_OK;
+    };
     uint64_t copied = 0;  // arena edges already sent to the host
     auto send = [&](uint64_t upto) -> int {
         upto = std::min<uint64_t>(upto, c->out_cap);  // placed edges never exceed the arena (the pool's size)
@@ -1565,6 +1809,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     }
     auto fallback = [&]() {
         (void)hipStreamSynchronize(c->ostream);
+        if (writer && writer->finish()) return fail(c, EMP_ERR_IO, "writing the output file failed");
         return RUN_FALLBACK;
     };
     if (!c->h_rec) {
@@ -1595,6 +1840,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         if (t >= 3) {
             HIPCHK(c, hipEventSynchronize(c->orec_ev[t - 3]));
             if ((rc = send(c->h_orec[t - 3][1] + c->h_orec[t - 3][0]))) return rc;
+            if ((rc = emit_round(t - 3))) return rc;
         }
         const bool more = t + 1 < rounds;
         const int bb = bits[t];
@@ -1630,9 +1876,9 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         // the round's edges, in output order, into the device arena while later rounds run
         HIPCHK(c, hipStreamWaitEvent(c->ostream, c->rec_ev[t], 0));
         HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, 1u << bb,
-                                     &c->d_ctr->obase, c->d_ctr->orec[t], c->d_pool, c->d_out_from,
-                                     c->d_out_to, c->ostream));
-        HIPCHK(c, hipMemcpyAsync(c->h_orec[t], c->d_ctr->orec[t], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     &c->d_ctr->obase, c->d_ctr->orec[t], &c->d_ctr->pool_ovf, c->d_pool,
+                                     c->d_out_from, c->d_out_to, c->ostream));
+        HIPCHK(c, hipMemcpyAsync(c->h_orec[t], c->d_ctr->orec[t], 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                  c->ostream));
         HIPCHK(c, hipEventRecord(c->orec_ev[t], c->ostream));
         launched = t + 1;
@@ -1650,6 +1896,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         const ListCtr &r = k.r[t + 1];
         if (r.n_big || r.nx_ovf) return fallback();
     }
+    if (writer && writer->finish()) return fail(c, EMP_ERR_IO, "writing the output file failed");
     for (t = 0; t < launched; ++t) {
         const uint64_t St = k.r[t].total[0], Pt = k.r[t].total[1];
         if (St == 0 || Pt == 0) break;
@@ -1848,11 +2095,13 @@ void emp_ctx_destroy(emp_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ostream) (void)hipStreamSynchronize(c->ostream);
+    if (c->cstream) (void)hipStreamSynchronize(c->cstream);
     for (auto &a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     for (int i = 0; i < 2; ++i) {
         if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
         if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+        if (c->h_chunk[i]) (void)hipHostFree(c->h_chunk[i]);
     }
     if (c->h_from) (void)hipHostFree(c->h_from);
     if (c->h_to) (void)hipHostFree(c->h_to);
@@ -1862,6 +2111,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     for (auto e : c->rec_ev) (void)hipEventDestroy(e);
     if (c->h_orec) (void)hipHostFree(c->h_orec);
     for (auto e : c->orec_ev) (void)hipEventDestroy(e);
+    for (auto e : c->dma_ev) (void)hipEventDestroy(e);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -1869,6 +2119,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     if (c->ostream) (void)hipStreamDestroy(c->ostream);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     delete c;
 }
 
@@ -1877,7 +2128,21 @@ const char *emp_last_error(const emp_ctx *c) { return c ? c->err.c_str() : "no c
 int emp_add_fastq(emp_ctx *c, const char *data, size_t size, int final, size_t *consumed) {
     if (!c || (!data && size) || !consumed) return EMP_ERR_ARG;
     (void)hipSetDevice(c->device);
-    return add_fastq_gpu(c, data, size, final != 0, consumed);
+    return add_fastq_range(c, data, size, size, true, final != 0, consumed, false);
+}
+
+int emp_add_fastq_file(emp_ctx *c, const char *path) {
+    if (!c || !path) return EMP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    return add_fastq_file(c, path);
+}
+
+int emp_add_fastq_device(emp_ctx *c, const void *d_data, size_t size, int final, size_t *consumed) {
+    if (!c || (!d_data && size) || !consumed) return EMP_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    const uint64_t B = c->opts.map_mem / 16;  // main.cu:16
+    if (B == 0) return fail(c, EMP_ERR_ARG, "map_mem / 16 is zero");
+    return add_fastq_range(c, static_cast<const char *>(d_data), size, B, false, final != 0, consumed, true);
 }
 
 int emp_add_reads(emp_ctx *c, const char *reads, uint64_t n, int32_t read_len) {
@@ -1923,7 +2188,11 @@ int emp_run_to_file(emp_ctx *c, const char *path) {
     if (!c || !path) return EMP_ERR_ARG;
     FILE *f = fopen(path, "wb");
     if (!f) return fail(c, EMP_ERR_IO, "Could not open %s", path);
+    c->out_file = f;  // the back-to-back rounds write finished records while later rounds run
+    c->emitted = 0;
     int rc = emp_run(c, file_sink, f);
+    c->out_file = nullptr;
+    c->emitted = 0;
     if (fclose(f) != 0 && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "closing %s failed", path);
     return rc;
 }

@@ -20,7 +20,8 @@ ERRORS = {
 
 # Every symbol the headers in include/ declare.
 EXPORTS = [
-    "emp_opts_default", "emp_ctx_create", "emp_ctx_destroy", "emp_last_error", "emp_add_fastq", "emp_add_reads",
+    "emp_opts_default", "emp_ctx_create", "emp_ctx_destroy", "emp_last_error", "emp_add_fastq",
+    "emp_add_fastq_device", "emp_add_fastq_file", "emp_add_reads",
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
     "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
     "emp_set_audit", "emp_audit_count", "emp_get_audit",
@@ -91,6 +92,8 @@ def lib():
         L.emp_last_error.restype = cp
         L.emp_last_error.argtypes = [vp]
         L.emp_add_fastq.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+        L.emp_add_fastq_device.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+        L.emp_add_fastq_file.argtypes = [vp, cp]
         L.emp_add_reads.argtypes = [vp, vp, u64, i32]
         L.emp_add_reads_device.argtypes = [vp, vp, u64, i32]
         L.emp_run.argtypes = [vp, SINK, vp]
@@ -203,6 +206,23 @@ class Context:
             if final:
                 break
             off += used.value
+
+    def add_fastq_device(self, ptr: int, size: int, pieces: int = 1):
+        """FASTQ bytes already in device memory, parsed in blocks of map_mem / 16 bytes. With pieces > 1
+        the bytes are handed over in that many prefixes (each call resumes where the last stopped)."""
+        off = 0
+        for k in range(pieces):
+            end = size if k == pieces - 1 else min(size, max(off, size * (k + 1) // pieces))
+            used = ctypes.c_size_t()
+            last = 1 if end == size else 0
+            self._check(lib().emp_add_fastq_device(self._h, ptr + off, end - off, last, ctypes.byref(used)))
+            off += used.value
+            if last:
+                break
+
+    def add_fastq_file(self, path: str):
+        """A whole FASTQ file, streamed by the library (blocks of map_mem / 16 bytes)."""
+        self._check(lib().emp_add_fastq_file(self._h, path.encode()))
 
     def add_reads(self, reads: np.ndarray):
         r = np.ascontiguousarray(reads, dtype=np.uint8)

@@ -65,6 +65,24 @@ const char *emp_last_error(const emp_ctx *ctx);
  */
 int emp_add_fastq(emp_ctx *ctx, const char *data, size_t size, int final, size_t *consumed);
 
+/*
+ * The CLI's map phase over a whole FASTQ file (main.cu:42-58: read_block in blocks of map_mem/16
+ * bytes with its seek-back, preprocess.cu:195-237, then the map stages). The file is read ahead
+ * into pinned memory by parallel preads and streamed into HBM on a copy stream, each block parsed
+ * on the device once its bytes have arrived. Same reads, ids and errors as emp_add_fastq called
+ * block by block.
+ */
+int emp_add_fastq_file(emp_ctx *ctx, const char *path);
+
+/*
+ * FASTQ bytes already in device memory (borrowed for the call), parsed as the CLI reads a file:
+ * blocks of map_mem/16 bytes, each resumed after the previous one's complete records. `final`
+ * non-zero: the bytes end the file (the last, shorter block is parsed too); otherwise parsing stops
+ * before a block that would run past the end. *consumed receives the bytes used; the caller
+ * continues from there.
+ */
+int emp_add_fastq_device(emp_ctx *ctx, const void *d_data, size_t size, int final, size_t *consumed);
+
 /* Map phase over kept reads already parsed: host memory, row-major n x read_len, ACGTacgt. */
 int emp_add_reads(emp_ctx *ctx, const char *reads, uint64_t n, int32_t read_len);
 

@@ -69,6 +69,8 @@ def lib():
                                            ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_size_t)]
+        L.eo_run_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Opts), ctypes.POINTER(Summary),
+                                  ctypes.POINTER(Round)]
         L.eo_buf_free.argtypes = [ctypes.POINTER(Buf)]
         L.eo_strerror.restype = ctypes.c_char_p
         L.eo_strerror.argtypes = [ctypes.c_int]
@@ -136,6 +138,14 @@ def run_fastq(fastq: bytes, min_ovlp=31, map_mem=1 << 30, reduce_mem=1 << 30):
     summ = {f: getattr(s, f) for f, _ in Summary._fields_}
     rounds = [{f: getattr(rr[i], f) for f, _ in Round._fields_} for i in range(min(s.n_rounds, 256))]
     return rc, out, summ, rounds
+
+
+def run_file(in_path: str, out_path: str, min_ovlp=31, map_mem=1 << 30, reduce_mem=1 << 30):
+    """FASTQ file -> output.bin file (the reference's CLI path). Returns (rc, summary dict)."""
+    o = Opts(min_ovlp, map_mem, reduce_mem)
+    s = Summary()
+    rc = lib().eo_run_file(in_path.encode(), out_path.encode(), ctypes.byref(o), ctypes.byref(s), None)
+    return rc, {f: getattr(s, f) for f, _ in Summary._fields_}
 
 
 def run_reads(reads: bytes, n: int, L: int, min_ovlp=31, reduce_mem=1 << 30):

@@ -186,7 +186,7 @@ def test_match_overflow_bucket_with_a_wrapped_prefix_survivor():
     assert got == want
 
 
-def test_pool_overflow_in_pipelined_rounds(monkeypatch):
+def test_pool_overflow_in_pipelined_rounds(monkeypatch, tmp_path):
     """Edges of one round beyond the whole first edge pool (sub-ranges and overflow region): 6,000
     key groups of 30 x 30 reads sharing a 30-mer (10.8 M edges with their reverse complements at
     overlap 30) among 2 M reads, on the back-to-back rounds. Buckets whose edges find no room
@@ -213,6 +213,29 @@ def test_pool_overflow_in_pipelined_rounds(monkeypatch):
         got = c.run()
         assert c.summary()["n_edges"] == osum["n_edges"]
     assert got == want
+    _check_file_output(r, 29, want, tmp_path)
+
+
+def _check_file_output(r, o, want, tmp_path):
+    """emp_run_to_file writes the records of the back-to-back rounds while later rounds run; when
+    the run falls back to the synchronous rounds those records are not written twice."""
+    out = tmp_path / "output.bin"
+    with native.Context(min_ovlp=o) as c:
+        c.add_reads(r)
+        c.run_to_file(str(out))
+    assert out.read_bytes() == want
+
+
+@pytest.mark.parametrize("case", ["genome", "groups600", "groups3000"])
+def test_file_output_with_and_without_fallback(case, tmp_path):
+    if case == "genome":  # no fallback: every record written during the rounds
+        g = gen.genome(5 * 10**5, 71)
+        r, o = gen.reads(g, 100_000, 100, 0.5, 0.005, 72), 31
+    else:  # a large key group in round 10: the first rounds are written, then the run falls back
+        r, o = reads_of(_shared_overlap_reads(int(case[6:]), 5)), 29
+    rc, want, _, _ = ol.run_reads(r.tobytes(), r.shape[0], r.shape[1], min_ovlp=o)
+    assert rc == 0
+    _check_file_output(r, o, want, tmp_path)
 
 
 def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
@@ -457,7 +480,7 @@ EO2EMP = {0: 0, -2: -3, -3: -4, -4: -5}  # oracle status -> C ABI status
 
 
 @pytest.mark.parametrize("seed", range(8))
-def test_fastq_parser_rules_and_error_order(seed):
+def test_fastq_parser_rules_and_error_order(seed, tmp_path):
     """The device FASTQ parser against the oracle's reader: dropped reads (a byte in [64,127]
     with bit 3 set), empty sequence lines, lowercase, blocks cut mid-record, and which error a
     block reports when it holds a bad base and a length mismatch (the first in file order)."""
@@ -483,14 +506,27 @@ def test_fastq_parser_rules_and_error_order(seed):
         seqs.insert(k, "ACGR" + "A" * (L - 4))
         seqs.insert(k + rng.randrange(1, 5), "A" * (L - 1))
     data = FQ([s.encode() for s in seqs])
+    from empanada_amd import hipmem
+    dbuf = hipmem.DeviceBuffer.from_array(np.frombuffer(data, np.uint8))
+    path = tmp_path / "in.fastq"
+    path.write_bytes(data)
     for block in (1 << 20, 997, 4103):
         rc_o, want, _, _ = ol.run_fastq(data, min_ovlp=L // 2, map_mem=16 * block)
-        try:
-            with native.Context(min_ovlp=L // 2, map_mem=16 * block) as c:
-                c.add_fastq(data, block=block)
-                got, rc_g = c.run(), 0
-        except native.EmpError as e:
-            got, rc_g = None, e.code
-        assert rc_g == EO2EMP[rc_o], (block, rc_o, rc_g)
-        if rc_g == 0:
-            assert got == want
+        # host blocks one by one (emp_add_fastq), the whole buffer in HBM in one call or three
+        # (emp_add_fastq_device: the block chain on the device), the file (emp_add_fastq_file)
+        for how in ("host", "device", "device3", "file"):
+            try:
+                with native.Context(min_ovlp=L // 2, map_mem=16 * block) as c:
+                    if how == "host":
+                        c.add_fastq(data, block=block)
+                    elif how == "file":
+                        c.add_fastq_file(str(path))
+                    else:
+                        c.add_fastq_device(dbuf.address, len(data), pieces=3 if how == "device3" else 1)
+                    got, rc_g = c.run(), 0
+            except native.EmpError as e:
+                got, rc_g = None, e.code
+            assert rc_g == EO2EMP[rc_o], (block, how, rc_o, rc_g)
+            if rc_g == 0:
+                assert got == want, (block, how)
+    dbuf.free()

@@ -10,6 +10,7 @@
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import oracle_lib as ol
@@ -41,6 +42,37 @@ def test_incore_workload_sample_vs_oracle():
     with native.Context(min_ovlp=o) as c:
         c.add_fastq(fq, block=16 << 20)
         assert c.run() == want
+
+
+def test_fastq_file_and_device_entry_points(tmp_path):
+    """emp_add_fastq_file (the file streamed into HBM in 64 MiB chunks while blocks are parsed) and
+    emp_add_fastq_device (blocks already in HBM) give the oracle's output at the default block of
+    64 MiB, which straddles the chunks of a 3 % in-core sample (96 MB of FASTQ), and at blocks of
+    100 KiB; a block shorter than one record is the format error of the oracle's reader."""
+    from empanada_amd import hipmem
+    r, o = gen.workload_reads("incore", 0.03)
+    fq = tmp_path / "in.fastq"
+    gen.write_fastq(str(fq), r)
+    data = fq.read_bytes()
+    assert len(data) > 64 << 20
+    rc, want, _, _ = ol.run_fastq(data, min_ovlp=o)
+    assert rc == 0
+    for map_mem in (1 << 30, 1600 << 10):
+        assert ol.run_fastq(data, min_ovlp=o, map_mem=map_mem)[1] == want
+        with native.Context(min_ovlp=o, map_mem=map_mem) as c:
+            c.add_fastq_file(str(fq))
+            assert c.run() == want, map_mem
+    d = hipmem.DeviceBuffer.from_array(np.frombuffer(data, np.uint8))
+    for map_mem, pieces in ((1 << 30, 1), (1 << 30, 4), (16 * ((100 << 10) + 3), 3)):
+        with native.Context(min_ovlp=o, map_mem=map_mem) as c:
+            c.add_fastq_device(d.address, len(data), pieces=pieces)
+            assert c.run() == want, (map_mem, pieces)
+    d.free()
+    assert ol.run_fastq(data, min_ovlp=o, map_mem=16 * 150)[0] == -2
+    with native.Context(min_ovlp=o, map_mem=16 * 150) as c:
+        with pytest.raises(native.EmpError) as e:
+            c.add_fastq_file(str(fq))
+    assert e.value.code == -3
 
 
 def test_cli_block_loop_vs_oracle(tmp_path):
