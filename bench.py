@@ -107,6 +107,66 @@ def cpu_baseline_sample(workload, scale, min_ovlp, reduce_mem):
     }, same
 
 
+def ooc_bench(steps, cpu_scale, check_sync, no_cpu):
+    """The 3.1 Gbp workload (413 M x 150 bp reads, 10 % repeats, -o 75 -r 96g): the reads in host
+    memory are streamed into HBM inside the timed region (parallel copies into pinned stages, DMA
+    overlapped with the map kernel), then the overlap path with every record delivered to host
+    memory."""
+    from empanada_amd import gen, hipmem, native
+    n, L = gen.workload_size("ooc")
+    t = time.perf_counter()
+    reads, o = gen.workload_reads("ooc", 1.0)
+    t_gen = time.perf_counter() - t
+    res = {"workload": "3.1 Gbp genome with 10%% repeats (300 bp and 6 kbp families, 16 copies, 1%% divergence), "
+                       "%d x %d bp reads, 50%% RC, 0.5%% substitutions, -o %d -r 96g; reads in host memory streamed "
+                       "into HBM through pinned stages inside the timed region, records to host memory" % (n, L, o),
+           "reads": n, "read_len": L, "gen_s": round(t_gen, 2)}
+    with native.Context(min_ovlp=o, reduce_mem=96 << 30) as c:
+        def step():
+            c.reset()
+            c.add_reads(reads)
+            c.run(collect=False)
+        step()  # warm-up: allocations (about 110 GB of HBM)
+        hipmem.synchronize()
+        wall = 0.0
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            step()
+            hipmem.synchronize()
+            wall += time.perf_counter() - t0
+        ms = wall * 1e3 / steps
+        summ = c.summary()
+        c.set_profile(True)
+        c.kernel_clear()
+        step()
+        kern = c.kernels()
+        c.set_profile(False)
+    kj = kern.get("k_join", {"ms": 0, "bytes": 0, "launches": 1})
+    res.update({"steps": steps, "ms_per_step": round(ms, 1), "gbases_per_s": round(n * L / (ms * 1e-3) / 1e9, 3),
+                "h2d_gbytes_per_s_floor": round(n * L / (ms * 1e-3) / 1e9, 2),
+                "dedup": summ["n_dedup"], "rounds": summ["n_rounds"], "edges": summ["n_edges"],
+                "k_join_roofline_frac": round(kj["bytes"] / (kj["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if kj["ms"] else None,
+                "kernels_ms": {k: round(v["ms"], 2) for k, v in kern.items()}})
+    if check_sync:
+        outs = []
+        for sync in (False, True):
+            if sync:
+                os.environ["EMPANADA_SYNC_ROUNDS"] = "1"
+            with native.Context(min_ovlp=o, reduce_mem=96 << 30) as cc:
+                cc.add_reads(reads)
+                outs.append(cc.run())
+            os.environ.pop("EMPANADA_SYNC_ROUNDS", None)
+        res["parity_full_pipelined_vs_synchronous"] = outs[0] == outs[1]
+        del outs
+    del reads
+    if not no_cpu:
+        cpu, same = cpu_baseline_sample("ooc", cpu_scale, o, 96 << 30)
+        res["cpu_baseline"] = cpu
+        res["parity_sample_vs_oracle"] = same
+        res["speedup_vs_cpu"] = round(res["gbases_per_s"] / cpu["value"], 1)
+    return res
+
+
 def traffic_of(kernel, workload):
     """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
     1 GiB copy's counters) and recorded in profiles/roofline_traffic.json; None when not measured."""
@@ -195,6 +255,8 @@ def main():
     ap.add_argument("--check", action="store_true",
                     help="3.1 Gbp: also compare the pipelined rounds' output with the synchronous rounds' at full "
                          "size (in-core: the full-size output is always compared with the oracle's)")
+    ap.add_argument("--ooc-steps", type=int, default=1,
+                    help="timed steps of the 3.1 Gbp workload, streamed from pinned host memory (0: skip)")
     ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
     ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
     ap.add_argument("--served-loads", default="0.5,1,2,4,8",
@@ -387,6 +449,10 @@ def main():
             os.unlink(pth)
     del reads, gpu_out
 
+    ooc_line = None
+    if args.ooc_steps > 0 and not ooc:
+        ooc_line = ooc_bench(args.ooc_steps, args.cpu_scale or 0.01, args.check, args.no_cpu)
+
     batched = batched_bench(args.batched_sets) if args.batched_sets > 0 and not ooc else None
     served = None
     if args.served_jobs > 0 and batched:
@@ -440,6 +506,7 @@ def main():
         "parity_full_vs_oracle": same_full,
         "parity_full_pipelined_vs_synchronous": same_sync,
         "gen_s": round(t_gen, 2),
+        "ooc": ooc_line,
         "batched": batched,
         "served": served,
     }

@@ -152,6 +152,14 @@ struct emp_ctx {
     FqResult *d_fq_res = nullptr, *h_fq_res = nullptr;
     uint8_t *d_acc = nullptr;
     uint64_t acc_bytes = 0, acc_n = 0;
+    // kept reads in pinned host memory (emp_add_reads): two device batches they are copied into
+    // on the copy stream, while the map kernel runs on the one before
+    uint8_t *d_pin[2] = {nullptr, nullptr};
+    uint64_t pin_cap[2] = {0, 0};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr}, pin_done[2] = {nullptr, nullptr};
+    // large kept-read arrays in pageable memory: three pinned stages filled by parallel copies
+    char *h_big[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t big_done[3] = {nullptr, nullptr, nullptr};
     // FASTQ files streamed into HBM (emp_add_fastq_file): the file's bytes, two pinned chunks
     // and the stream that copies them
     uint8_t *d_file = nullptr;
@@ -762,6 +770,82 @@ int add_fastq_file(emp_ctx *c, const char *path) {
     for (auto e : ev)
         if (e) (void)hipEventDestroy(e);
     return rc;
+}
+
+// Kept reads in host memory (m of them, a multiple of 16, the context at a multiple of 16 with
+// nothing staged), in batches of about PIN_BATCH bytes: copied into two device batches on the copy
+// stream, each mapped on the compute stream while the next is copied. Pinned reads are copied
+// straight from the caller's buffer; pageable ones first go, by COPY_THREADS parallel copies, into
+// one of three pinned stages (the host fills batch b + 1 while batch b is on the link). The
+// caller's buffer is free again once the copies are done; the map kernels may still run.
+constexpr uint64_t PIN_BATCH = 256ull << 20;
+constexpr int COPY_THREADS = 4;
+
+void parallel_copy(char *dst, const char *src, uint64_t bytes) {
+    const uint64_t per = (bytes + COPY_THREADS - 1) / COPY_THREADS;
+    auto part = [&](int p) {
+        const uint64_t a = std::min(bytes, per * (uint64_t)p), e = std::min(bytes, a + per);
+        if (e > a) memcpy(dst + a, src + a, (size_t)(e - a));
+    };
+    std::thread th[COPY_THREADS - 1];
+    for (int p = 1; p < COPY_THREADS; ++p) th[p - 1] = std::thread(part, p);
+    part(0);
+    for (auto &t : th) t.join();
+}
+
+int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
+    if (m == 0) return EMP_OK;
+    const uint64_t L = (uint64_t)c->L;
+    if (2 * (c->n + m) >= (1ull << 32)) return fail(c, EMP_ERR_TOO_MANY, "more than 2^31 - 1 reads");
+    int rc;
+    if ((rc = ensure_cols(c, c->n + m)) || (rc = ensure_entries(c, 2 * (c->n + m)))) return rc;
+    const uint64_t per = std::max<uint64_t>(16, (PIN_BATCH / L) & ~15ull);
+    if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        if (c->pin_cap[k] < per * L) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dfree(c, c->d_pin[k]);
+            c->d_pin[k] = nullptr;
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, per * L))) return rc;
+            c->d_pin[k] = (uint8_t *)p;
+            c->pin_cap[k] = per * L;
+        }
+        if (!c->pin_ev[k]) {
+            HIPCHK(c, hipEventCreateWithFlags(&c->pin_ev[k], hipEventDisableTiming));
+            HIPCHK(c, hipEventCreateWithFlags(&c->pin_done[k], hipEventDisableTiming));
+            HIPCHK(c, hipEventRecord(c->pin_done[k], c->stream));
+        }
+    }
+    if (!pinned) {
+        for (int k = 0; k < 3; ++k) {
+            if (!c->h_big[k]) HIPCHK(c, hipHostMalloc((void **)&c->h_big[k], PIN_BATCH, hipHostMallocDefault));
+            if (!c->big_done[k]) {
+                HIPCHK(c, hipEventCreateWithFlags(&c->big_done[k], hipEventDisableTiming));
+                HIPCHK(c, hipEventRecord(c->big_done[k], c->cstream));
+            }
+        }
+    }
+    for (uint64_t b = 0, off = 0; off < m; ++b, off += per) {
+        const int slot = (int)(b & 1);
+        const uint64_t cnt = std::min(per, m - off);
+        const char *src = reads + off * L;
+        if (!pinned) {  // the stage's last copy to the device is done: refill it
+            const int hs = (int)(b % 3);
+            HIPCHK(c, hipEventSynchronize(c->big_done[hs]));
+            parallel_copy(c->h_big[hs], src, cnt * L);
+            src = c->h_big[hs];
+        }
+        HIPCHK(c, hipStreamWaitEvent(c->cstream, c->pin_done[slot], 0));  // the batch's last map has read the slot
+        HIPCHK(c, hipMemcpyAsync(c->d_pin[slot], src, cnt * L, hipMemcpyHostToDevice, c->cstream));
+        if (!pinned) HIPCHK(c, hipEventRecord(c->big_done[b % 3], c->cstream));
+        HIPCHK(c, hipEventRecord(c->pin_ev[slot], c->cstream));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->pin_ev[slot], 0));
+        if ((rc = map_batch(c, c->d_pin[slot], cnt))) return rc;
+        HIPCHK(c, hipEventRecord(c->pin_done[slot], c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->cstream));
+    return EMP_OK;
 }
 
 // Maps what the device accumulation still holds (the run's last batch).
@@ -2102,6 +2186,12 @@ void emp_ctx_destroy(emp_ctx *c) {
         if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
         if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
         if (c->h_chunk[i]) (void)hipHostFree(c->h_chunk[i]);
+        if (c->pin_ev[i]) (void)hipEventDestroy(c->pin_ev[i]);
+        if (c->pin_done[i]) (void)hipEventDestroy(c->pin_done[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (c->h_big[i]) (void)hipHostFree(c->h_big[i]);
+        if (c->big_done[i]) (void)hipEventDestroy(c->big_done[i]);
     }
     if (c->h_from) (void)hipHostFree(c->h_from);
     if (c->h_to) (void)hipHostFree(c->h_to);
@@ -2151,9 +2241,30 @@ int emp_add_reads(emp_ctx *c, const char *reads, uint64_t n, int32_t read_len) {
     (void)hipSetDevice(c->device);
     int rc = set_len(c, read_len);
     if (rc) return rc;
-    for (uint64_t i = 0; i < n; ++i) {
-        rc = stage_read(c, reads + i * (uint64_t)read_len, (size_t)read_len);
-        if (rc) return rc;
+    hipPointerAttribute_t attr;
+    const bool pinned = hipPointerGetAttributes(&attr, reads) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();  // pageable memory is an error of the query, not of the context
+    uint64_t i = 0;
+    const bool large = n * (uint64_t)read_len >= 2 * PIN_BATCH;
+    if ((pinned || large) && c->stage_n == 0 && c->acc_n == 0 && c->n % 16 == 0) {
+        const uint64_t m = n & ~15ull;  // the rest (fewer than 16) is staged, so later batches still start at a multiple of 16
+        if ((rc = add_reads_pinned(c, reads, m, pinned))) return rc;
+        i = m;
+    }
+    // staged: whole spans of reads into the pinned stage (stage_read sets the stage up and moves
+    // reads parsed on the device ahead of them)
+    while (i < n) {
+        if (c->acc_n || !c->h_stage[0]) {
+            if ((rc = stage_read(c, reads + i * (uint64_t)read_len, (size_t)read_len))) return rc;
+            ++i;
+            continue;
+        }
+        const uint64_t k = std::min<uint64_t>(n - i, STAGE_READS - c->stage_n);
+        memcpy(c->h_stage[c->stage_cur] + (size_t)c->stage_n * c->L, reads + i * (uint64_t)read_len,
+               (size_t)(k * (uint64_t)read_len));
+        c->stage_n += (uint32_t)k;
+        i += k;
+        if (c->stage_n == STAGE_READS && (rc = stage_submit(c))) return rc;
     }
     return EMP_OK;
 }

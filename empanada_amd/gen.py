@@ -55,9 +55,10 @@ def add_repeats(g: np.ndarray, seed: int, frac: float, len_f: int, copies: int, 
                                       used.size)
 
 
-def reads(g: np.ndarray, n: int, L: int, p_rc: float, p_sub: float, seed: int) -> np.ndarray:
-    """Row-major (n, L) uint8 array of ASCII reads."""
-    r = np.empty((n, L), dtype=np.uint8)
+def reads(g: np.ndarray, n: int, L: int, p_rc: float, p_sub: float, seed: int, out: np.ndarray = None) -> np.ndarray:
+    """Row-major (n, L) uint8 array of ASCII reads (into `out`, a C-contiguous (n, L) uint8 array, when given)."""
+    r = np.empty((n, L), dtype=np.uint8) if out is None else out
+    assert r.shape == (n, L) and r.dtype == np.uint8 and r.flags["C_CONTIGUOUS"]
     _load().empgen_reads(seed, g.ctypes.data, g.size, n, L, p_rc, p_sub, r.ctypes.data)
     return r
 
@@ -123,9 +124,16 @@ def batched_set(i: int, base: int = 1300, n_sets: int = 4096):
     return reads(g, p["n"], p["L"], 0.5, 0.005, p["seeds"][1]), p["o"]
 
 
-def workload_reads(name: str, scale: float = 1.0):
+def workload_size(name: str, scale: float = 1.0):
+    """(reads, read length) of a named workload at `scale`."""
+    w = WORKLOADS[name]
+    return max(int(w["n"] * scale), 1), w["L"]
+
+
+def workload_reads(name: str, scale: float = 1.0, out: np.ndarray = None):
     """Reads of a named workload; scale < 1 shrinks genome and read count together (same
-    coverage, same per-base work), for bounded samples. Returns (reads, min_ovlp)."""
+    coverage, same per-base work), for bounded samples. Returns (reads, min_ovlp); the reads go
+    into `out` when given (shape workload_size(name, scale))."""
     w = WORKLOADS[name]
     G = max(int(w["G"] * scale), 10 * w["L"])
     n = max(int(w["n"] * scale), 1)
@@ -134,4 +142,4 @@ def workload_reads(name: str, scale: float = 1.0):
         # each family size on its own grid of slots; copies of the two sizes may overlap
         used = np.zeros(G // len_f, dtype=np.uint8)
         add_repeats(g, seed, frac, len_f, copies, div, used)
-    return reads(g, n, w["L"], w["p_rc"], w["p_sub"], w["seeds"][1]), w["o"]
+    return reads(g, n, w["L"], w["p_rc"], w["p_sub"], w["seeds"][1], out=out), w["o"]

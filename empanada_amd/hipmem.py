@@ -35,6 +35,8 @@ def lib():
         L.hipEventSynchronize.argtypes = [vp]
         L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
         L.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.hipHostMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint]
+        L.hipHostFree.argtypes = [vp]
         L.hipGetErrorString.restype = ctypes.c_char_p
         L.hipGetErrorString.argtypes = [ctypes.c_int]
         _lib = L
@@ -80,6 +82,36 @@ class DeviceBuffer:
     def free(self):
         if self.ptr:
             lib().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostBuffer:
+    """Pinned host memory (hipHostMalloc), with a numpy view."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(lib().hipHostMalloc(ctypes.byref(self.ptr), max(nbytes, 1), 0), "hipHostMalloc")
+
+    def array(self, dtype=np.uint8, shape=None):
+        n = self.nbytes // np.dtype(dtype).itemsize
+        a = np.ctypeslib.as_array(ctypes.cast(self.ptr, ctypes.POINTER(ctypes.c_uint8)), shape=(self.nbytes,))
+        a = a.view(dtype)[:n]
+        return a.reshape(shape) if shape is not None else a
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            lib().hipHostFree(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):

@@ -83,7 +83,10 @@ int emp_add_fastq_file(emp_ctx *ctx, const char *path);
  */
 int emp_add_fastq_device(emp_ctx *ctx, const void *d_data, size_t size, int final, size_t *consumed);
 
-/* Map phase over kept reads already parsed: host memory, row-major n x read_len, ACGTacgt. */
+/* Map phase over kept reads already parsed: host memory, row-major n x read_len, ACGTacgt. Reads in
+ * pinned host memory (hipHostMalloc) are copied straight into HBM in double-buffered batches while
+ * earlier batches are mapped; large pageable arrays go through pinned stages filled by parallel
+ * copies, small ones through the context's staging buffers. */
 int emp_add_reads(emp_ctx *ctx, const char *reads, uint64_t n, int32_t read_len);
 
 /* Same, reads already in device memory (borrowed for the call). Unless this is the last batch,

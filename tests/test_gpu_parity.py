@@ -293,6 +293,18 @@ def test_many_batches_and_device_entry_point():
         c.add_reads_device(d.address, r.shape[0], 40)
         assert c.run() == want
     d.free()
+    # pinned host memory: copied straight into HBM batch by batch, in two calls whose first one
+    # ends off a multiple of 16 (its last reads are staged), then pageable memory again
+    h = hipmem.HostBuffer(r.nbytes)
+    hr = h.array(np.uint8, r.shape)
+    hr[:] = r
+    cut = 700_003
+    with native.Context(min_ovlp=25) as c:
+        c.add_reads(hr[:cut])
+        c.add_reads(hr[cut:cut + 200_000])
+        c.add_reads(r[cut + 200_000:])
+        assert c.run() == want
+    h.free()
 
 
 def test_reset_and_rerun_is_deterministic():

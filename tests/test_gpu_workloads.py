@@ -75,6 +75,19 @@ def test_fastq_file_and_device_entry_points(tmp_path):
     assert e.value.code == -3
 
 
+def test_large_pageable_reads_stream_through_pinned_stages():
+    """3,600,003 x 150 bp reads in pageable memory (540 MB, past the size where emp_add_reads
+    streams them through three pinned stages filled by parallel copies, seven batches) against the
+    oracle; the last 3 reads (off a multiple of 16) are staged."""
+    g = gen.genome(3 * 10**7, 81)
+    r = gen.reads(g, 3_600_003, 150, 0.5, 0.005, 82)
+    rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 150, min_ovlp=75)
+    assert rc == 0 and osum["n_edges"] > 0
+    with native.Context(min_ovlp=75) as c:
+        c.add_reads(r)
+        assert c.run() == want
+
+
 def test_cli_block_loop_vs_oracle(tmp_path):
     g = gen.genome(3 * 10**5, 51)
     r = gen.reads(g, 30000, 100, 0.5, 0.005, 52)
