@@ -178,15 +178,14 @@ def traffic_of(kernel, workload):
     return t["traffic_bytes_per_launch"] if t and t.get("workload") == workload else None
 
 
-def batched_bench(n_sets):
-    """The first n_sets sets of the batched workload: one batched engine vs the plain path per set."""
-    from empanada_amd import gen, native
-    t = time.perf_counter()
-    sets = [gen.batched_set(i) for i in range(n_sets)]
-    t_gen = time.perf_counter() - t
+def batched_bench(sets, plain_stride, hbm_budget):
+    """The sets of the batched workload through one batched engine (admission by HBM budget), and
+    the plain single-job path run once per set on every plain_stride-th set, on the same GPU."""
+    from empanada_amd import native
+    n_sets = len(sets)
     reads = sum(r.shape[0] for r, _ in sets)
     bases = sum(r.size for r, _ in sets)
-    with native.Batch(collect=False) as b:
+    with native.Batch(collect=False, hbm_budget=hbm_budget) as b:
         for r, o in sets[:8]:  # warm-up: allocations, code objects
             b.submit(r, o)
         b.run()
@@ -196,36 +195,40 @@ def batched_bench(n_sets):
             b.submit(r, o)
         b.run()
         t_b = time.perf_counter() - t
-        edges_b = sum(b.info(j)["n_edges"] for j in range(n_sets))
+        edges_b = [b.info(j)["n_edges"] for j in range(n_sets)]
+    sub = list(range(0, n_sets, plain_stride))
     with native.Context(min_ovlp=sets[0][1]) as c:
         c.add_reads(sets[0][0])
         c.run(collect=False)
-        edges_p = 0
+        edges_p = []
         t = time.perf_counter()
-        for r, o in sets:
+        for i in sub:
+            r, o = sets[i]
             c.reset()
             c.add_reads(r)
             c.run(collect=False)
-            edges_p += c.summary()["n_edges"]
+            edges_p.append(c.summary()["n_edges"])
         t_p = time.perf_counter() - t
+    jb, jp = n_sets / t_b, len(sub) / t_p
     return {
         "workload": "%d sets: genome log-uniform 5 kbp-5 Mbp, L in {100,150,250}, coverage 10-50x, 50%% RC, "
                     "0.5%% substitutions, -o 31 (seeds 1300+i, 1300+i+4096)" % n_sets,
-        "reads": reads, "bases": bases,
-        "batched_s": round(t_b, 4), "plain_s": round(t_p, 4),
-        "batched_jobs_per_s": round(n_sets / t_b, 2), "plain_jobs_per_s": round(n_sets / t_p, 2),
-        "speedup": round(t_p / t_b, 2),
+        "reads": reads, "bases": bases, "hbm_budget": hbm_budget,
+        "batched_s": round(t_b, 3), "batched_jobs_per_s": round(jb, 2),
         "batched_gbases_per_s": round(bases / t_b / 1e9, 3),
-        "edges_equal": edges_b == edges_p,
+        "plain_sets": len(sub), "plain_stride": plain_stride, "plain_s": round(t_p, 3),
+        "plain_jobs_per_s": round(jp, 2),
+        "speedup": round(jb / jp, 2),
+        "edges_equal_on_plain_sets": [edges_b[i] for i in sub] == edges_p,
         "reads_in": "host memory (both paths copy them to HBM inside the timed region)",
-        "gen_s": round(t_gen, 2),
+        "note": "plain path on every %d-th set (same distribution), batched on all of them" % plain_stride,
     }
 
 
-def served_bench(n_jobs, loads):
+def served_bench(jobs, loads, bound_ms):
     """Poisson arrivals of batched-workload sets at each offered load (jobs/s)."""
-    from empanada_amd import gen, serve
-    jobs = [gen.batched_set(i) for i in range(n_jobs)]
+    from empanada_amd import serve
+    n_jobs = len(jobs)
     out = []
     with serve.Server() as srv:
         srv.run(jobs[:4], [0.0] * 4)  # warm-up
@@ -237,7 +240,10 @@ def served_bench(n_jobs, loads):
             out.append(dict(offered_jobs_per_s=round(rate, 2), achieved_jobs_per_s=round(n_jobs / res["wall"], 2),
                             p50_ms=round(st["p50_ms"], 2), p99_ms=round(st["p99_ms"], 2),
                             all_ok=all(s == 0 for s in res["status"])))
-    return {"jobs_per_load": n_jobs, "arrivals": "Poisson, seed 1400 + load index", "loads": out}
+    within = [x["offered_jobs_per_s"] for x in out if x["p99_ms"] <= bound_ms]
+    return {"jobs_per_load": n_jobs, "arrivals": "Poisson, seed 1400 + load index", "loads": out,
+            "p99_bound_ms": bound_ms,
+            "max_offered_load_within_bound": max(within) if within else None}
 
 
 def main():
@@ -257,10 +263,14 @@ def main():
                          "size (in-core: the full-size output is always compared with the oracle's)")
     ap.add_argument("--ooc-steps", type=int, default=1,
                     help="timed steps of the 3.1 Gbp workload, streamed from pinned host memory (0: skip)")
-    ap.add_argument("--batched-sets", type=int, default=256, help="sets of the batched workload (0: skip)")
-    ap.add_argument("--served-jobs", type=int, default=96, help="jobs per offered load when serving (0: skip)")
-    ap.add_argument("--served-loads", default="0.5,1,2,4,8",
+    ap.add_argument("--batched-sets", type=int, default=4096, help="sets of the batched workload (0: skip)")
+    ap.add_argument("--plain-stride", type=int, default=8,
+                    help="the plain path's comparison runs every n-th set of the batched workload")
+    ap.add_argument("--batched-budget", type=int, default=160, help="the batched engine's HBM budget, GiB")
+    ap.add_argument("--served-jobs", type=int, default=1000, help="jobs per offered load when serving (0: skip)")
+    ap.add_argument("--served-loads", default="1,2,4,8",
                     help="offered loads as multiples of the plain path's jobs/s")
+    ap.add_argument("--p99-bound-ms", type=float, default=250.0, help="the declared p99 latency bound")
     ap.add_argument("--audit", action="store_true",
                     help="after the timed steps, one more step with the collision audit on (reported apart)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -453,11 +463,22 @@ def main():
     if args.ooc_steps > 0 and not ooc:
         ooc_line = ooc_bench(args.ooc_steps, args.cpu_scale or 0.01, args.check, args.no_cpu)
 
-    batched = batched_bench(args.batched_sets) if args.batched_sets > 0 and not ooc else None
-    served = None
-    if args.served_jobs > 0 and batched:
-        loads = [float(x) * batched["plain_jobs_per_s"] for x in args.served_loads.split(",")]
-        served = served_bench(args.served_jobs, loads)
+    batched = served = None
+    if (args.batched_sets > 0 or args.served_jobs > 0) and not ooc:
+        t = time.perf_counter()
+        sets = [gen.batched_set(i) for i in range(max(args.batched_sets, args.served_jobs))]
+        t_sets = time.perf_counter() - t
+        plain_rate = None
+        if args.batched_sets > 0:
+            batched = batched_bench(sets[:args.batched_sets], args.plain_stride, args.batched_budget << 30)
+            batched["gen_s"] = round(t_sets, 2)
+            plain_rate = batched["plain_jobs_per_s"]
+        if args.served_jobs > 0:
+            if plain_rate is None:
+                plain_rate = batched_bench(sets[:256], 4, args.batched_budget << 30)["plain_jobs_per_s"]
+            loads = [float(x) * plain_rate for x in args.served_loads.split(",")]
+            served = served_bench(sets[:args.served_jobs], loads, args.p99_bound_ms)
+        del sets
 
     line = {
         "metric": "Gbases/s end-to-end overlap detection on 1 GPU",
