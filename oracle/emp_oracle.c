@@ -169,7 +169,7 @@ const char *eo_strerror(int code) {
         case EO_ERR_FORMAT: return "malformed FASTQ (block does not start with '@', or a record is longer than a block)";
         case EO_ERR_READLEN: return "kept reads differ in length or exceed 255 bases";
         case EO_ERR_BASE: return "kept read holds a byte other than ACGTacgt";
-        case EO_ERR_OUT_OF_CORE: return "2n > reduce_mem/16: out-of-core merge order is not restated";
+        case EO_ERR_OUT_OF_CORE: return "out-of-core order: unused";
         case EO_ERR_PARTITION: return "more than M/2 equal keys in a partition";
         case EO_ERR_NOMEM: return "out of memory";
         case EO_ERR_ARG: return "bad argument";
@@ -355,31 +355,182 @@ typedef struct {
     size_t n;
 } list_t;
 
+/* ------------------------------------------------------------------------------------------ */
+/* The out-of-core sort (cqutils.cuh:36-84,139-247) and unique (cqutils.cuh:86-105,271-306).   */
+
+/* upper_bound over a[lo, hi) */
+static size_t upper_in(const uint64_t *a, size_t lo, size_t hi, uint64_t key) {
+    return lo + upper_bound_u64(a + lo, hi - lo, key);
+}
+
 /*
- * regenerate_fingerprints (overlap.cu:605-667) for the in-core case (count <= M: one chunk):
- * sort by read id (overlap.cu:613), partition even/odd and sort each half by id
- * (overlap.cu:636-642; an even/odd split of an id-sorted list keeps both halves id-sorted),
+ * cq_get_merge_partitions (cqutils.cuh:36-84): partitions of the key-sorted runs a and b, each
+ * side at most M/2 per partition, in merge order. Case 1 (a's window is not above b's first key):
+ * a's window alone; case 2 (b's window is not above a's first key): b's window alone, so b's
+ * entries equal to a's first key come first; case 3: both windows, the one with the larger last
+ * key cut by upper_bound at the other's last key; the rest once both windows are short.
+ */
+int eo_merge_partitions(const uint64_t *a, size_t nA, const uint64_t *b, size_t nB, size_t M, size_t *parts,
+                        size_t cap, size_t *n_parts) {
+    size_t np = 0, offA = 0, offB = 0;
+    const size_t H = M / 2;
+    *n_parts = 0;
+    if (H == 0) return EO_ERR_ARG;
+#define EO_PUSH(x, y) do { if (np >= cap) return EO_ERR_ARG; parts[2 * np] = (x); parts[2 * np + 1] = (y); ++np; } while (0)
+    while (offA < nA && offB < nB) {
+        size_t nPa = nA - offA < H ? nA - offA : H, nPb = nB - offB < H ? nB - offB : H;
+        if (nPa < H && nPb < H) {
+            EO_PUSH(nPa, nPb);
+            offA += nPa; offB += nPb;
+            break;
+        }
+        if (a[offA + nPa - 1] <= b[offB]) {                      /* case 1 */
+            EO_PUSH(nPa, 0);
+            offA += nPa;
+        } else if (b[offB + nPb - 1] <= a[offA]) {               /* case 2 */
+            EO_PUSH(0, nPb);
+            offB += nPb;
+        } else {                                                   /* case 3 */
+            const uint64_t la = a[offA + nPa - 1], lb = b[offB + nPb - 1];
+            if (la < lb) nPb = upper_in(b, offB, offB + nPb, la) - offB;
+            else if (lb < la) nPa = upper_in(a, offA, offA + nPa, lb) - offA;
+            EO_PUSH(nPa, nPb);
+            offA += nPa; offB += nPb;
+        }
+    }
+    if (offA == nA && offB < nB) EO_PUSH(0, nB - offB);
+    else if (offB == nB && offA < nA) EO_PUSH(nA - offA, 0);
+#undef EO_PUSH
+    *n_parts = np;
+    return EO_OK;
+}
+
+typedef struct {
+    uint64_t *k;
+    uint32_t *v;
+    size_t n;
+} run_t;
+
+/*
+ * cq_merge_by_key (cqutils.cuh:139-177): per partition, a mixed one is sorted as [a part; b part]
+ * (thrust::sort_by_key, stable: equal keys keep a before b), a one-sided one copied.
+ */
+static int merge_runs(const run_t *A, const run_t *B, size_t M, run_t *C) {
+    size_t cap = 4 + 4 * ((A->n + B->n) / (M / 2 ? M / 2 : 1));
+    size_t *parts = (size_t *)malloc(2 * cap * sizeof(size_t)), np = 0;
+    C->n = A->n + B->n;
+    C->k = (uint64_t *)malloc((C->n ? C->n : 1) * sizeof(uint64_t));
+    C->v = (uint32_t *)malloc((C->n ? C->n : 1) * sizeof(uint32_t));
+    if (!parts || !C->k || !C->v) { free(parts); return EO_ERR_NOMEM; }
+    int rc = eo_merge_partitions(A->k, A->n, B->k, B->n, M, parts, cap, &np);
+    if (rc) { free(parts); return rc; }
+    size_t oa = 0, ob = 0, oc = 0;
+    for (size_t q = 0; q < np; ++q) {
+        const size_t na = parts[2 * q], nb = parts[2 * q + 1];
+        memcpy(C->k + oc, A->k + oa, na * sizeof(uint64_t));
+        memcpy(C->v + oc, A->v + oa, na * sizeof(uint32_t));
+        memcpy(C->k + oc + na, B->k + ob, nb * sizeof(uint64_t));
+        memcpy(C->v + oc + na, B->v + ob, nb * sizeof(uint32_t));
+        if (na && nb) sort_u64_u32(C->k + oc, C->v + oc, na + nb);
+        oa += na; ob += nb; oc += na + nb;
+    }
+    free(parts);
+    return EO_OK;
+}
+
+/*
+ * cq_sort_by_key (cqutils.cuh:179-247) on (u64 key, u32 value) pairs: n <= M one stable sort;
+ * otherwise chunks of M in input order sorted each, then the FIFO of sorted runs merged pairwise
+ * (the front two, the result pushed to the back) until one run is left. Sorts keys/vals in place.
+ */
+int eo_cq_sort_by_key(uint64_t *keys, uint32_t *vals, size_t n, size_t M) {
+    if (n <= M) {
+        sort_u64_u32(keys, vals, n);
+        return EO_OK;
+    }
+    const size_t F = (n + M - 1) / M;
+    run_t *q = (run_t *)malloc((2 * F) * sizeof(run_t));
+    if (!q) return EO_ERR_NOMEM;
+    size_t head = 0, tail = 0;
+    for (size_t c = 0; c < F; ++c) {
+        const size_t lo = c * M, m = n - lo < M ? n - lo : M;
+        run_t r;
+        r.n = m;
+        r.k = (uint64_t *)malloc(m * sizeof(uint64_t));
+        r.v = (uint32_t *)malloc(m * sizeof(uint32_t));
+        memcpy(r.k, keys + lo, m * sizeof(uint64_t));
+        memcpy(r.v, vals + lo, m * sizeof(uint32_t));
+        sort_u64_u32(r.k, r.v, m);
+        q[tail++] = r;
+    }
+    int rc = EO_OK;
+    while (tail - head > 1 && !rc) {
+        run_t A = q[head++], B = q[head++], C;
+        rc = merge_runs(&A, &B, M, &C);
+        free(A.k); free(A.v); free(B.k); free(B.v);
+        if (!rc) q[tail++] = C;
+    }
+    if (!rc) {
+        memcpy(keys, q[head].k, n * sizeof(uint64_t));
+        memcpy(vals, q[head].v, n * sizeof(uint32_t));
+    }
+    for (size_t i = head; i < tail; ++i) { free(q[i].k); free(q[i].v); }
+    free(q);
+    return rc;
+}
+
+/*
+ * cq_get_partitions (cqutils.cuh:86-105) + cq_unique_by_key (cqutils.cuh:271-306) over a
+ * key-sorted list: windows of M cut at lower_bound of the window's last key (a window of one key
+ * never ends: the reference loops, EO_ERR_PARTITION here), the first entry of each key kept in
+ * each partition. Partitions never split a key, so this keeps the first of every key.
+ */
+static int unique_partitioned(list_t *l, size_t M) {
+    size_t off = 0, w = 0;
+    while (off < l->n) {
+        size_t np;
+        if (l->n - off <= M) {
+            np = l->n - off;
+        } else {
+            np = lower_bound_u64(l->fp + off, M, l->fp[off + M - 1]);
+            if (np == 0) return EO_ERR_PARTITION;
+        }
+        for (size_t i = off; i < off + np; ++i)
+            if (i == off || l->fp[i] != l->fp[i - 1]) { l->fp[w] = l->fp[i]; l->rid[w] = l->rid[i]; ++w; }
+        off += np;
+    }
+    l->n = w;
+    return EO_OK;
+}
+
+/*
+ * regenerate_fingerprints (overlap.cu:605-667): sort by read id (overlap.cu:613; the ids are
+ * distinct, so the chunked sort of cq_sort_by_key gives the fully sorted order too), then per
+ * chunk of M entries (overlap.cu:623-659): partition even/odd and sort each half by id
+ * (overlap.cu:636-642; an even/odd split of an id-sorted chunk keeps both halves id-sorted),
  * gather one base per entry from the column (filter_bases, overlap.cu:670-692: base of read
  * rid/2, complemented through d_revc for odd ids), then the rolling transform
- * (overlap.cu:647,651).
+ * (overlap.cu:647,651). The list ends as chunk 0's evens, chunk 0's odds, chunk 1's evens, ...
  */
-static void regenerate(list_t *l, const char *fwd, const char *rev, int is_sfx, int len) {
+static void regenerate(list_t *l, const char *fwd, const char *rev, int is_sfx, int len, size_t M) {
     size_t n = l->n;
     if (n == 0) return;
     sort_u32_u64(l->rid, l->fp, n);
     uint64_t *fp2 = (uint64_t *)malloc(n * sizeof(uint64_t));
     uint32_t *rid2 = (uint32_t *)malloc(n * sizeof(uint32_t));
-    size_t ne = 0;
-    for (size_t i = 0; i < n; ++i)
-        if (!(l->rid[i] & 1)) { fp2[ne] = l->fp[i]; rid2[ne] = l->rid[i]; ++ne; }
-    size_t no = ne;
-    for (size_t i = 0; i < n; ++i)
-        if (l->rid[i] & 1) { fp2[no] = l->fp[i]; rid2[no] = l->rid[i]; ++no; }
+    for (size_t lo = 0; lo < n; lo += M) {
+        const size_t hi = n - lo < M ? n : lo + M;
+        size_t w = lo;
+        for (size_t i = lo; i < hi; ++i)
+            if (!(l->rid[i] & 1)) { fp2[w] = l->fp[i]; rid2[w] = l->rid[i]; ++w; }
+        for (size_t i = lo; i < hi; ++i)
+            if (l->rid[i] & 1) { fp2[w] = l->fp[i]; rid2[w] = l->rid[i]; ++w; }
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         size_t i = (size_t)ii;
         uint32_t r = rid2[i];
-        char b = (i < ne) ? fwd[r / 2] : h_revc[rev[r / 2] & 7];
+        char b = !(r & 1) ? fwd[r / 2] : h_revc[rev[r / 2] & 7];
         fp2[i] = is_sfx ? eo_next_sfx(fp2[i], b, len) : eo_prev_pfx(fp2[i], b);
     }
     free(l->fp);
@@ -585,6 +736,38 @@ static void audit_list(const list_t *l, const char *reads, int L, int start, int
     r->strings = strings;
 }
 
+/*
+ * The dedup of build_graph alone (overlap.cu:212-225): the 2n map entries sorted by id, then by key
+ * (cq_sort_by_key with buffer M), the first of every key kept (cq_unique_by_key). ids receives
+ * the kept ids in key order (2n of room), *nd their count.
+ */
+int eo_dedup(const char *reads, uint64_t n, int L, size_t M, uint32_t *ids, uint64_t *nd) {
+    eo_init();
+    *nd = 0;
+    if (n == 0) return EO_OK;
+    if (L <= 0 || L > MAX_RD_LEN || M < 2) return EO_ERR_ARG;
+    list_t S;
+    S.n = 2 * n;
+    S.fp = (uint64_t *)malloc(S.n * sizeof(uint64_t));
+    S.rid = (uint32_t *)malloc(S.n * sizeof(uint32_t));
+    if (!S.fp || !S.rid) { free(S.fp); free(S.rid); return EO_ERR_NOMEM; }
+    for (uint64_t k = 0; k < n; ++k) {
+        S.fp[2 * k] = eo_key_of(reads + k * (uint64_t)L, L);
+        S.fp[2 * k + 1] = eo_key_of_rc(reads + k * (uint64_t)L, L);
+        S.rid[2 * k] = (uint32_t)(2 * k);
+        S.rid[2 * k + 1] = (uint32_t)(2 * k + 1);
+    }
+    int rc = eo_cq_sort_by_key(S.fp, S.rid, S.n, M);
+    if (!rc) rc = unique_partitioned(&S, M);
+    if (!rc) {
+        memcpy(ids, S.rid, S.n * sizeof(uint32_t));
+        *nd = S.n;
+    }
+    free(S.fp);
+    free(S.rid);
+    return rc;
+}
+
 int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,
                      eo_summary *sum, eo_round *rounds) {
     return eo_overlap_reads_audit(reads, n, L, min_ovlp, reduce_mem, out, sum, rounds, NULL, 0, NULL);
@@ -601,7 +784,7 @@ int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, u
     const size_t M = (size_t)(reduce_mem / 16);
     const uint64_t N0 = 2 * n;
     if (N0 >= (1ull << 32)) return EO_ERR_TOO_MANY;
-    if (N0 > M) return EO_ERR_OUT_OF_CORE;
+    if (M < 2) return EO_ERR_ARG;
 
     /* transpose_reads: col[j][k] = read_k[j] */
     char *cols = (char *)malloc((size_t)L * n);
@@ -623,17 +806,13 @@ int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, u
         S.rid[2 * kk + 1] = (uint32_t)(2 * kk + 1);
     }
 
-    /* overlap.cu:212-213: sort by id (already id-ordered), then stable sort by key */
+    /* overlap.cu:212-213: sort by id (already id-ordered), then sort by key (cq_sort_by_key) */
     sort_u32_u64(S.rid, S.fp, S.n);
-    sort_u64_u32(S.fp, S.rid, S.n);
+    int rc = eo_cq_sort_by_key(S.fp, S.rid, S.n, M);
+    if (rc) { free(cols); free(S.fp); free(S.rid); return rc; }
     audit_list(&S, reads, L, 0, L, 0, L, audit, audit_cap, n_audit); /* overlap.cu:215-219 */
-    /* cq_unique_by_key (cqutils.cuh:271-306), one partition since N0 <= M: keep the first. */
-    {
-        size_t w = 0;
-        for (size_t i = 0; i < S.n; ++i)
-            if (i == 0 || S.fp[i] != S.fp[i - 1]) { S.fp[w] = S.fp[i]; S.rid[w] = S.rid[i]; ++w; }
-        S.n = w;
-    }
+    /* cq_unique_by_key (cqutils.cuh:271-306): keep the first of each key */
+    if ((rc = unique_partitioned(&S, M))) { free(cols); free(S.fp); free(S.rid); return rc; }
     if (sum) sum->n_dedup = S.n;
 
     /* overlap.cu:235-242 */
@@ -644,7 +823,6 @@ int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, u
     memcpy(P.fp, S.fp, S.n * sizeof(uint64_t));
     memcpy(P.rid, S.rid, S.n * sizeof(uint32_t));
 
-    int rc = EO_OK;
     int round = 0;
     /* overlap.cu:258 */
     for (round = 0; (round < L - min_ovlp) && (S.n > 0); ++round) {
@@ -656,11 +834,10 @@ int eo_overlap_reads_audit(const char *reads, uint64_t n, int L, int min_ovlp, u
         const char *c_fwd = cols + (size_t)round * n;
         const char *c_bwd = cols + (size_t)(L - 1 - round) * n;
         /* overlap.cu:266-267 */
-        regenerate(&S, c_fwd, c_bwd, 1, L - round);
-        regenerate(&P, c_bwd, c_fwd, 0, 0);
+        regenerate(&S, c_fwd, c_bwd, 1, L - round, M);
+        regenerate(&P, c_bwd, c_fwd, 0, 0, M);
         /* overlap.cu:271-272 */
-        sort_u64_u32(S.fp, S.rid, S.n);
-        sort_u64_u32(P.fp, P.rid, P.n);
+        if ((rc = eo_cq_sort_by_key(S.fp, S.rid, S.n, M)) || (rc = eo_cq_sort_by_key(P.fp, P.rid, P.n, M))) break;
         /* overlap.cu:274-284 */
         audit_list(&S, reads, L, round + 1, L - 1 - round, 1, L - 1 - round, audit, audit_cap, n_audit);
         audit_list(&P, reads, L, 0, L - 1 - round, 2, L - 1 - round, audit, audit_cap, n_audit);

@@ -7,9 +7,10 @@
  * removal) so that its output.bin is the reference's. It is the parity checker for the HIP
  * library and the host-core baseline; the product library never links, loads or calls it.
  *
- * Semantics restated: in-core (2n <= M, M = reduce_mem / 16) including the key-aligned search
- * partitions that split a round into several records when a list reaches M/2. The reference's
- * out-of-core chunked sort/merge (N > M) is not restated: such runs return EO_ERR_OUT_OF_CORE.
+ * Semantics restated for every M = reduce_mem / 16: the key-aligned search partitions that split
+ * a round into several records when a list reaches M/2, and the out-of-core order of lists over M
+ * entries (the chunked sort and FIFO pairwise merge of cq_sort_by_key, the chunked regeneration,
+ * the partitioned unique).
  */
 #ifndef EMP_ORACLE_H
 #define EMP_ORACLE_H
@@ -27,8 +28,8 @@ enum {
     EO_ERR_FORMAT = -2,      /* first byte of a block is not '@', or a record is longer than a block */
     EO_ERR_READLEN = -3,     /* kept reads of unequal length, or length > 255 */
     EO_ERR_BASE = -4,        /* kept read holds a byte other than ACGTacgt */
-    EO_ERR_OUT_OF_CORE = -5, /* 2n > M: the reference's chunked sort/merge order is not restated */
-    EO_ERR_PARTITION = -6,   /* more than M/2 equal keys: the reference loops forever */
+    EO_ERR_OUT_OF_CORE = -5, /* unused: every M is restated */
+    EO_ERR_PARTITION = -6,   /* a partition with no cut (e.g. more than M/2 equal keys): the reference loops forever */
     EO_ERR_NOMEM = -7,
     EO_ERR_ARG = -8,
     EO_ERR_TOO_MANY = -9     /* 2n >= 2^32 read ids */
@@ -88,6 +89,18 @@ int eo_parse_fastq_mem(const char *data, uint64_t size, uint64_t block_bytes, ch
  */
 int eo_search_partitions(const uint64_t *a, size_t nA, const uint64_t *b, size_t nB, size_t M, size_t *parts,
                          size_t cap, size_t *n_parts);
+
+/*
+ * The out-of-core sort: cq_sort_by_key (cqutils.cuh:179-247) of n (key, value) pairs with device
+ * buffer M (in place), and its merge partitions (cq_get_merge_partitions, cqutils.cuh:36-84) of
+ * two key-sorted runs: up to cap (na, nb) pairs, *n_parts of them.
+ */
+int eo_cq_sort_by_key(uint64_t *keys, uint32_t *vals, size_t n, size_t M);
+int eo_merge_partitions(const uint64_t *a, size_t nA, const uint64_t *b, size_t nB, size_t M, size_t *parts,
+                        size_t cap, size_t *n_parts);
+
+/* The dedup alone: the kept ids of the 2n entries of n reads in key order (ids: 2n of room). */
+int eo_dedup(const char *reads, uint64_t n, int L, size_t M, uint32_t *ids, uint64_t *nd);
 
 /* Kept reads -> output records (appended to out). rounds may be NULL (else >= 256 entries). */
 int eo_overlap_reads(const char *reads, uint64_t n, int L, int min_ovlp, uint64_t reduce_mem, eo_buf *out,

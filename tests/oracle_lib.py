@@ -71,6 +71,14 @@ def lib():
                                            ctypes.POINTER(ctypes.c_size_t)]
         L.eo_run_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Opts), ctypes.POINTER(Summary),
                                   ctypes.POINTER(Round)]
+        L.eo_merge_partitions.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]
+        L.eo_cq_sort_by_key.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32),
+                                        ctypes.c_size_t, ctypes.c_size_t]
+        L.eo_dedup.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t,
+                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
         L.eo_buf_free.argtypes = [ctypes.POINTER(Buf)]
         L.eo_strerror.restype = ctypes.c_char_p
         L.eo_strerror.argtypes = [ctypes.c_int]
@@ -110,6 +118,34 @@ def search_partitions(a, b, M):
     n = ctypes.c_size_t()
     rc = lib().eo_search_partitions(A, len(a), B, len(b), M, parts, cap, ctypes.byref(n))
     return rc, [(parts[2 * i], parts[2 * i + 1]) for i in range(n.value)]
+
+
+def merge_partitions(a, b, M):
+    """cq_get_merge_partitions over key-sorted runs a, b: (rc, [(na, nb), ...])."""
+    A = (ctypes.c_uint64 * max(len(a), 1))(*a)
+    B = (ctypes.c_uint64 * max(len(b), 1))(*b)
+    cap = 4 + 4 * (len(a) + len(b))
+    parts = (ctypes.c_size_t * (2 * cap))()
+    n = ctypes.c_size_t()
+    rc = lib().eo_merge_partitions(A, len(a), B, len(b), M, parts, cap, ctypes.byref(n))
+    return rc, [(parts[2 * i], parts[2 * i + 1]) for i in range(n.value)]
+
+
+def cq_sort(keys, vals, M):
+    """cq_sort_by_key of (key, value) pairs with device buffer M: (rc, keys, values)."""
+    K = (ctypes.c_uint64 * max(len(keys), 1))(*keys)
+    V = (ctypes.c_uint32 * max(len(vals), 1))(*vals)
+    rc = lib().eo_cq_sort_by_key(K, V, len(keys), M)
+    return rc, list(K)[:len(keys)], list(V)[:len(vals)]
+
+
+def dedup(reads, M):
+    """The kept ids of the dedup (key order) over equal-length read strings with buffer M."""
+    n, L = len(reads), len(reads[0]) if reads else 0
+    ids = (ctypes.c_uint32 * max(2 * n, 1))()
+    nd = ctypes.c_uint64()
+    rc = lib().eo_dedup("".join(reads).encode(), n, L, M, ids, ctypes.byref(nd))
+    return rc, list(ids)[:nd.value]
 
 
 def parse(fastq: bytes, block_bytes: int):

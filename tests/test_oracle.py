@@ -203,8 +203,6 @@ def test_search_partitions_split_records():
     for ell, e in b:
         merged.setdefault(ell, []).extend(e)
     assert [(ell, merged[ell]) for ell, _ in a] == a
-    # out-of-core (2n > M) is not restated
-    assert ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 200)[0] == -5
 
 
 # Search partitions worked by hand from cqutils.cuh:113-135 (window w = min(rest, M/2) per side;
@@ -269,6 +267,138 @@ def test_search_partitions_of_a_run_match_the_records():
     prc, parts = ol.search_partitions(sfx, pfx, M)
     assert prc == 0
     assert rounds[0]["records"] == sum(1 for na, nb in parts if na and nb) > 1
+
+
+# Merge partitions worked by hand from cqutils.cuh:43-82 (windows of M/2; case 1: A's window last
+# <= B's first -> A's window alone; case 2: B's window last <= A's first -> B's window alone; case
+# 3: the side with the larger window last is cut by upper_bound at the other's; the rest once
+# both windows are short; a side left over alone at the end).
+MERGE_CASES = [
+    # (0,0) 1 2 | 3 4: case 1 (2,0). (2,0) 5 6 | 3 4: case 2 (0,2). (2,2) 5 6 | 7 8: case 1 (2,0).
+    # A done: (0,2).
+    ([1, 2, 5, 6], [3, 4, 7, 8], 4, [(2, 0), (0, 2), (2, 0), (0, 2)]),
+    # (0,0) 1 3 | 2 3: case 3 with equal window lasts, no cut (2,2). (2,2) 5 7 | 8 9: case 1 (2,0).
+    # A done: (0,2).
+    ([1, 3, 5, 7], [2, 3, 8, 9], 4, [(2, 2), (2, 0), (0, 2)]),
+    # (0,0) 1 4 | 2 3: case 3, 3 < 4: A cut at upper_bound(3) = 1 -> (1,2). (1,2) 4 6 | 5 8: case 3,
+    # 6 < 8: B cut at upper_bound(6) = 1 -> (2,1). (3,3) 9 | 8: both short -> (1,1).
+    ([1, 4, 6, 9], [2, 3, 5, 8], 4, [(1, 2), (2, 1), (1, 1)]),
+    # (0,0) 5 5 | 3 5: case 2 (B's last 5 <= A's first 5) -> (0,2): B's 5 goes before A's 5s.
+    # (0,2) 5 5 | 9 9: case 1 (2,0). (2,2) 6 7 | 9 9: case 1 (2,0). A done: (0,2).
+    ([5, 5, 6, 7], [3, 5, 9, 9], 4, [(0, 2), (2, 0), (2, 0), (0, 2)]),
+]
+
+
+@pytest.mark.parametrize("a,b,M,parts", MERGE_CASES)
+def test_merge_partitions_hand_worked(a, b, M, parts):
+    assert ol.merge_partitions(a, b, M) == (0, parts)
+
+
+def test_cq_sort_two_chunks_hand_worked():
+    """Keys 5 3 5 7 | 5 9 5 6 with values 0..7, M = 4. Chunk 0 sorts to 3(1) 5(0) 5(2) 7(3), chunk 1
+    to 5(4) 5(6) 6(7) 9(5). Merge with windows of 2: (0,0) 3 5 | 5 5 case 1 -> 3(1) 5(0); (2,0)
+    5 7 | 5 5 case 2 -> 5(4) 5(6); (2,2) 5 7 | 6 9 case 3 cuts B at upper_bound(7) -> [5(2) 7(3)
+    6(7)] sorted stably -> 5(2) 6(7) 7(3); then 9(5). Key 5 ends 0, 4, 6, 2: chunk 1's pair ahead of
+    chunk 0's second, where one stable sort would give 0, 2, 4, 6."""
+    rc, k, v = ol.cq_sort([5, 3, 5, 7, 5, 9, 5, 6], list(range(8)), 4)
+    assert rc == 0
+    assert k == [3, 5, 5, 5, 5, 6, 7, 9]
+    assert v == [1, 0, 4, 6, 2, 7, 3, 5]
+    assert ol.cq_sort([5, 3, 5, 7, 5, 9, 5, 6], list(range(8)), 8)[2] == [1, 0, 2, 4, 6, 7, 3, 5]
+
+
+def test_cq_sort_three_chunks_fifo_order_hand_worked():
+    """Keys 4 4 | 2 4 | 1 4 with values 0..5, M = 2: chunks 4(0) 4(1), 2(2) 4(3), 1(4) 4(5). The FIFO
+    merges chunks 0 and 1 first (windows of 1: case 2 -> 2(2); case 1 -> 4(0); case 1 -> 4(1); the
+    rest 4(3)), then chunk 2 with that run, chunk 2 as A: case 1 -> 1(4); case 2 -> 2(2); case 1 ->
+    4(5); the rest 4(0) 4(1) 4(3). Chunk 2's 4 comes first."""
+    rc, k, v = ol.cq_sort([4, 4, 2, 4, 1, 4], list(range(6)), 2)
+    assert rc == 0
+    assert k == [1, 2, 4, 4, 4, 4]
+    assert v == [4, 2, 5, 0, 1, 3]
+
+
+def test_cq_sort_distinct_keys_is_the_sorted_order():
+    rng = random.Random(4)
+    keys = rng.sample(range(1 << 40), 3000)
+    for M in (2, 7, 64, 999, 3000):
+        rc, k, v = ol.cq_sort(keys, list(range(3000)), M)
+        assert rc == 0 and k == sorted(keys) and [keys[i] for i in v] == k
+
+
+def _edge_sets(out):
+    """Per overlap length: the sorted edge list of all records of that length."""
+    d = {}
+    for ell, e in bf.decode(out):
+        d.setdefault(ell, []).extend(e)
+    return {ell: sorted(e) for ell, e in d.items()}
+
+
+def test_out_of_core_runs_keep_the_edge_sets():
+    """2n > M: the chunked sort and merges reorder equal keys, the chunked regeneration reorders the
+    lists, the records split by M/2; the edges of every overlap length are the in-core run's (the
+    same pairs), and with no equal keys in a list the concatenated records are the in-core ones."""
+    rng = random.Random(12)
+    # no two oriented reads equal (the dedup's choice between equal reads is what the order decides)
+    seen, reads = set(), []
+    for r in _random_reads(rng, 500, 40, 1500, 0.5, 0.01):
+        if r not in seen and bf.revcomp(r) not in seen and r != bf.revcomp(r):
+            seen.update((r, bf.revcomp(r)))
+            reads.append(r)
+    assert len(reads) > 300
+    data = FQ([r.encode() for r in reads])
+    rc, ref, rsum, _ = ol.run_fastq(data, min_ovlp=20)
+    assert rc == 0
+    for m in (700, 301, 128):
+        rc, out, summ, rounds = ol.run_fastq(data, min_ovlp=20, reduce_mem=16 * m)
+        assert rc == 0, m
+        assert summ["n_dedup"] == rsum["n_dedup"] and summ["n_edges"] == rsum["n_edges"]
+        assert _edge_sets(out) == _edge_sets(ref), m
+    # error-free reads of one strand, distinct start positions: no list holds a key twice
+    g = "".join(rng.choice("ACGT") for _ in range(20000))
+    starts = rng.sample(range(20000 - 40), 600)
+    reads = [g[p:p + 40] for p in starts]
+    data = FQ([r.encode() for r in reads])
+    rc, ref, _, _ = ol.run_fastq(data, min_ovlp=25)
+    rc2, out, _, _ = ol.run_fastq(data, min_ovlp=25, reduce_mem=16 * 500)
+    assert rc == rc2 == 0
+    merged = {}
+    for ell, e in bf.decode(out):
+        merged.setdefault(ell, []).extend(e)
+    assert [(ell, merged[ell]) for ell, _ in bf.decode(ref)] == bf.decode(ref)
+
+
+def test_out_of_core_dedup_keeps_the_first_of_the_merged_order():
+    """Read 0 and read 3 equal (X), M = 4 entries: chunk 0 holds ids 0-3 (X and read 1), chunk 1 ids
+    4-7 (read 2 and X again). Read 1's keys both exceed key(X) < key(rc X), and read 2 has one key
+    below key(X) and one above, so chunk 1's two smallest keys are key(read 2 or its rc) and key(X)
+    (id 6), and chunk 0's smallest is key(X) (id 0): the first merge window is case 2 (B's window
+    last, key(X), is not above A's first, key(X)), which puts id 6 ahead of id 0. The dedup keeps
+    id 6 where the in-core order (M = 8) keeps id 0."""
+    rng = random.Random(3)
+    KEY = ol.key_of
+    while True:  # X with key(X) < key(rc X), read 1 above key(X) on both strands, read 2 across it
+        x = "".join(rng.choice("ACGT") for _ in range(20))
+        if KEY(x) < KEY(bf.revcomp(x)):
+            break
+    while True:
+        r1 = "".join(rng.choice("ACGT") for _ in range(20))
+        if min(KEY(r1), KEY(bf.revcomp(r1))) > KEY(x):
+            break
+    while True:
+        r2 = "".join(rng.choice("ACGT") for _ in range(20))
+        k2 = sorted((KEY(r2), KEY(bf.revcomp(r2))))
+        if k2[0] < KEY(x) < k2[1] and k2[1] != KEY(bf.revcomp(x)):
+            break
+    reads = [x, r1, r2, x]
+    rc, incore = ol.dedup(reads, 8)
+    rc2, ooc = ol.dedup(reads, 4)
+    assert rc == rc2 == 0 and len(incore) == len(ooc) == 6
+    assert 0 in incore and 6 not in incore
+    assert 6 in ooc and 0 not in ooc
+    # key order: read 2's smaller key (id 4 or 5), then key(X): id 6 out of core, id 0 in core
+    assert ooc[0] in (4, 5) and ooc[1] == 6
+    assert incore[0] in (4, 5) and incore[1] == 0
 
 
 def test_search_partition_without_a_cut_is_an_error():
