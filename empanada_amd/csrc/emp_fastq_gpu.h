@@ -21,6 +21,10 @@ struct FqResult {
     uint32_t kept_len;              // length of the first kept record
 };
 
+// Exclusive scan of n counts into off[0..n] (off[n] = total); blk: fq_scan_blocks(n) scratch.
+hipError_t launch_scan(const uint32_t *cnt, uint64_t n, unsigned long long *off, unsigned long long *blk,
+                       hipStream_t s);
+
 uint64_t fq_chunks(uint64_t size);
 // Block totals a scan of n counts needs (the blk scratch of launch_fq_count / launch_fq_classify).
 uint64_t fq_scan_blocks(uint64_t n);

@@ -131,6 +131,8 @@ __global__ void __launch_bounds__(256) k_fq_scan_add(unsigned long long *__restr
     if (i < n) off[i] += blk[i / SCAN_BLOCK];
 }
 
+}  // namespace
+
 hipError_t launch_scan(const uint32_t *cnt, uint64_t n, unsigned long long *off, unsigned long long *blk,
                        hipStream_t s) {
     const uint64_t nblk = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
@@ -145,6 +147,8 @@ hipError_t launch_scan(const uint32_t *cnt, uint64_t n, unsigned long long *off,
     hipLaunchKernelGGL(k_fq_scan_add, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, off, n, blk);
     return hipGetLastError();
 }
+
+namespace {
 
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_lines(const uint8_t *d, uint64_t size,
                                                        const unsigned long long *choff, uint64_t *nlpos) {

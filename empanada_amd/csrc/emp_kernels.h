@@ -191,6 +191,9 @@ struct BigArgs {
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
     uint32_t *too_many;            // set when a bucket holds 2^31 edges or more (join)
+    // the out-of-core kernels (emp_ooc.h): tie-key scratch and the tie keys per id
+    unsigned long long *stie, *ptie;
+    const unsigned long long *tie_s, *tie_p;
 };
 
 // Up to 16 ranges of 32-bit words set to a value in one launch (per-round counter resets).

@@ -26,6 +26,7 @@
 //   k_audit    collision audit of one list (FPCHECK)
 #include "emp_common.h"
 #include "emp_kernels.h"
+#include "emp_ooc.h"
 
 #include <hip/hip_runtime.h>
 
@@ -1505,6 +1506,314 @@ hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, i
                        uint64_t *out, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_roll, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, keys, bases, n, kind, m1, m2, out);
+    return hipGetLastError();
+}
+
+}  // namespace emp
+
+// ---------------------------------------------------------------------------------------------
+// The out-of-core order (emp_ooc.h).
+namespace emp {
+
+// Key of entry e of bucket b of list l (a round's wide entry has h1's bits 21..28 in its bucket).
+__device__ __forceinline__ uint64_t ooc_key(const OocList &l, const Entry &e, uint32_t b) {
+    if (l.wshift < 0) return key_of(e);
+    return ((uint64_t)wide_key1(e.h1, b >> (WIDE_SHIFT - l.wshift)) << 32) | (e.h2 & KEY_MASK);
+}
+
+// Visits every entry of the list: fine slots by work-group per bucket (grid-stride), spilled
+// records by the whole grid.
+template <class Fn>
+__device__ __forceinline__ void ooc_each(const OocList &l, Fn fn) {
+    for (uint32_t b = blockIdx.x; b < l.nb; b += gridDim.x) {
+        const uint32_t n = min(l.cnt[b], l.cap);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) fn(l.fine[(size_t)b * l.cap + i], b);
+    }
+    if (l.spill) {
+        const uint32_t sn = min(*l.spill_n, l.spill_cap);
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < sn; j += gridDim.x * blockDim.x)
+            fn(l.spill[j].e, l.spill[j].bucket);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ooc_mark(OocList l, uint32_t *bitmap) {
+    ooc_each(l, [&](const Entry &e, uint32_t) { atomicOr(bitmap + (e.rid >> 5), 1u << (e.rid & 31)); });
+}
+
+__global__ void __launch_bounds__(256) k_ooc_popc(const uint32_t *bitmap, uint64_t n, uint32_t *cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) cnt[i] = (uint32_t)__popc(bitmap[i]);
+}
+
+// Chunk and tie key of every entry: tie = io(chunk) << 32 | order inside the chunk, the id for
+// the dedup (its input is id-ordered), (rid & 1) << 31 | rid >> 1 for a round (each chunk holds its
+// even ids, then its odd ids, each in id order).
+__global__ void __launch_bounds__(256) k_ooc_tie(OocTieArgs a) {
+    ooc_each(a.l, [&](const Entry &e, uint32_t b) {
+        const uint32_t rid = e.rid;
+        uint32_t c = 0;
+        if (a.F > 1) {
+            uint64_t pos = rid;
+            if (a.rounds) {
+                const uint32_t w = rid >> 5;
+                pos = a.base[w] + (uint64_t)__popc(a.bitmap[w] & ((1u << (rid & 31)) - 1u));
+            }
+            c = (uint32_t)(pos / a.M);
+            atomicAdd(a.bcnt + (size_t)(b >> a.gshift) * a.F + c, 1u);
+        }
+        const uint64_t inner = a.rounds ? (((uint64_t)(rid & 1) << 31) | (rid >> 1)) : rid;
+        a.tie[rid] = ((unsigned long long)a.io[c] << 32) | inner;
+        a.chunk[rid] = (uint16_t)c;
+    });
+}
+
+__global__ void __launch_bounds__(256) k_ooc_fetch(OocList l, const uint32_t *ids, const unsigned long long *off,
+                                                   const uint16_t *chunk, OocRec *out) {
+    __shared__ uint32_t s_pos;
+    const uint32_t b = ids[blockIdx.x];
+    if (threadIdx.x == 0) s_pos = 0;
+    __syncthreads();
+    OocRec *o = out + off[blockIdx.x];
+    const uint32_t n = min(l.cnt[b], l.cap);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const Entry e = l.fine[(size_t)b * l.cap + i];
+        o[i] = OocRec{ooc_key(l, e, b), e.rid, chunk[e.rid]};
+    }
+    if (l.cnt[b] > l.cap && l.spill) {
+        const uint32_t sn = min(*l.spill_n, l.spill_cap);
+        for (uint32_t j = threadIdx.x; j < sn; j += blockDim.x)
+            if (l.spill[j].bucket == b) {
+                const uint32_t k = n + atomicAdd(&s_pos, 1u);
+                if (k < l.cnt[b]) o[k] = OocRec{ooc_key(l, l.spill[j].e, b), l.spill[j].e.rid, chunk[l.spill[j].e.rid]};
+            }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ooc_override(const uint32_t *ids, const unsigned long long *vals, uint32_t n,
+                                                      unsigned long long *tie) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) tie[ids[i]] = vals[i];
+}
+
+// Bitonic sort of n2 (power of two) records by (key, tie), ascending, with the id along; global
+// memory private to the work-group.
+__device__ void bitonic_kt(uint64_t *key, unsigned long long *tie, uint32_t *rid, uint32_t n2) {
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t ka = key[i], kb = key[ixj];
+                    const unsigned long long ta = tie[i], tb = tie[ixj];
+                    const bool a_gt = ka > kb || (ka == kb && ta > tb);
+                    if (a_gt == ((i & k) == 0)) {
+                        key[i] = kb; key[ixj] = ka;
+                        tie[i] = tb; tie[ixj] = ta;
+                        const uint32_t r = rid[i]; rid[i] = rid[ixj]; rid[ixj] = r;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Bucket b's entries (slots + spilled records) as (key, tie, id) at K/T/R, padded to n2 with
+// maximal records.
+__device__ void ooc_gather(const Entry *slots, uint32_t cnt, uint32_t cap, const SpillRec *spill, const uint32_t *spill_n,
+                           uint32_t spill_cap, uint32_t b, int32_t wshift, const unsigned long long *ties, uint64_t *K,
+                           unsigned long long *T, uint32_t *R, uint32_t n2, uint32_t *s_pos) {
+    OocList l{slots, nullptr, cap, spill, spill_n, spill_cap, wshift, 0};
+    const uint32_t nm = min(cnt, cap);
+    for (uint32_t i = threadIdx.x; i < nm; i += blockDim.x) {
+        const Entry e = slots[(size_t)b * cap + i];
+        K[i] = ooc_key(l, e, b);
+        T[i] = ties[e.rid];
+        R[i] = e.rid;
+    }
+    if (threadIdx.x == 0) *s_pos = nm;
+    __syncthreads();
+    if (cnt > cap) {
+        const uint32_t sn = min(*spill_n, spill_cap);
+        for (uint32_t j = threadIdx.x; j < sn; j += blockDim.x)
+            if (spill[j].bucket == b) {
+                const uint32_t k = atomicAdd(s_pos, 1u);
+                if (k < cnt) {
+                    const Entry e = spill[j].e;
+                    K[k] = ooc_key(l, e, b);
+                    T[k] = ties[e.rid];
+                    R[k] = e.rid;
+                }
+            }
+    }
+    __syncthreads();
+    for (uint32_t i = cnt + threadIdx.x; i < n2; i += blockDim.x) { K[i] = ~0ull; T[i] = ~0ull; R[i] = ~0u; }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_dedup_ooc(BigArgs a) {
+    __shared__ uint32_t s_pos;
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = a.s_cnt[b];
+    if (n == 0) return;
+    const uint32_t n2 = pow2_at_least(n);
+    uint64_t *K = a.skey + a.soff[b];
+    unsigned long long *T = a.stie + a.soff[b];
+    uint32_t *R = a.srank + a.soff[b];
+    ooc_gather(a.s_in, n, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, -1, a.tie_s, K, T, R, n2, &s_pos);
+    bitonic_kt(K, T, R, n2);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (i == 0 || K[i] != K[i - 1]) {  // the first of its key in the merged order
+            if (a.nx.active) {
+                put_next(a.nx, (uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i], 0u, 0);
+                put_next(a.nx, (uint32_t)(K[i] >> 32), (uint32_t)K[i], R[i], 0u, 1);
+            } else {
+                atomicAdd(a.nx.total[0], 1u);
+                atomicAdd(a.nx.total[1], 1u);
+            }
+        }
+    }
+}
+
+// Join of every bucket in (key, tie) order. A suffix's partners are the prefix key group minus
+// the same read's members (counted over the group), a prefix is used when a suffix of another
+// read shares its key (overlap.cu:413-415,424-429,557-603,694-710).
+__global__ void __launch_bounds__(1024) k_join_ooc(BigArgs a) {
+    __shared__ unsigned long long sh64[16];
+    __shared__ uint32_t s_pos;
+    __shared__ unsigned long long s_base;
+    const uint32_t b = blockIdx.x;
+    const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
+    if (ns == 0 && np == 0) {
+        if (threadIdx.x == 0) a.edge_cnt[b] = 0;
+        return;
+    }
+    const uint32_t ns2 = pow2_at_least(ns), np2 = pow2_at_least(np);
+    uint64_t *SK = a.skey + a.soff[b], *PK = a.pkey + a.poff[b];
+    unsigned long long *ST = a.stie + a.soff[b], *PT = a.ptie + a.poff[b];
+    uint32_t *SR = a.srank + a.soff[b], *PR = a.prank + a.poff[b];
+    uint32_t *SA = a.saux + a.soff[b], *PA = a.paux + a.poff[b];
+    const int32_t ws = a.nx.src_shift;
+    ooc_gather(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, ws, a.tie_s, SK, ST, SR, ns2, &s_pos);
+    ooc_gather(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, ws, a.tie_p, PK, PT, PR, np2, &s_pos);
+    bitonic_kt(SK, ST, SR, ns2);
+    bitonic_kt(PK, PT, PR, np2);
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint64_t k = SK[i];
+        const uint32_t lo = lower_u64(PK, 0, np, k), hi = upper_u64(PK, lo, np, k), rd = SR[i] >> 1;
+        uint32_t c = 0;
+        for (uint32_t j = lo; j < hi; ++j) c += (PR[j] >> 1) != rd ? 1u : 0u;
+        SA[i] = c;
+    }
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+        const uint64_t k = PK[j];
+        const uint32_t lo = lower_u64(SK, 0, ns, k), hi = upper_u64(SK, lo, ns, k), rd = PR[j] >> 1;
+        uint32_t used = 0;
+        for (uint32_t i = lo; i < hi && !used; ++i) used = (SR[i] >> 1) != rd ? 1u : 0u;
+        PA[j] = used;
+    }
+    __syncthreads();
+    const uint32_t per = (ns + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = min(ns, threadIdx.x * per), hi = min(ns, lo + per);
+    unsigned long long run = 0;
+    for (uint32_t i = lo; i < hi; ++i) run += SA[i];
+    unsigned long long tot64;
+    const unsigned long long off64 = block_excl_scan64<1024>(run, sh64, tot64);
+    const bool too_many = tot64 >= (1ull << 31);
+    const uint32_t tot = too_many ? 0u : (uint32_t)tot64;
+    if (!too_many) {
+        uint32_t off = (uint32_t)off64;
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint32_t c = SA[i];
+            SA[i] = off;
+            off += c;
+        }
+    }
+    const uint32_t sub = b & (SUBS - 1);
+    if (threadIdx.x == 0) {
+        unsigned long long base = ~0ull;
+        if (too_many) atomicOr(a.too_many, 1u);
+        if (tot) {
+            const unsigned long long c = atomicAdd(a.pool_cursor + sub, (unsigned long long)tot);
+            base = place_edges(c, tot, sub, a.pool_cursor + sub, a.pool_cap, a.ovf_cursor, a.ovf_cap);
+        }
+        s_base = base;
+        a.edge_off[b] = base;
+        a.edge_cnt[b] = base == ~0ull ? 0u : tot;
+    }
+    __syncthreads();
+    if (too_many) return;
+    const unsigned long long base = s_base;
+    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+        const uint32_t o = SA[i], e = (i + 1 < ns) ? SA[i + 1] : tot;
+        const uint32_t srid = SR[i];
+        if (e > o && base != ~0ull) {
+            const uint64_t k = SK[i];
+            unsigned long long pos = base + o;
+            for (uint32_t j = lower_u64(PK, 0, np, k); j < np && PK[j] == k && pos < base + e; ++j)
+                if ((PR[j] >> 1) != (srid >> 1)) {
+                    a.pool[pos] = make_uint2(srid, PR[j]);
+                    ++pos;
+                }
+        }
+        if (e == o) {  // a survivor
+            if (a.nx.active)
+                put_next(a.nx, (uint32_t)(SK[i] >> 32), (uint32_t)SK[i], srid, carry_after(a.cols, a.rg, srid, true), 0);
+            else
+                atomicAdd(a.nx.total[0], 1u);
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x) {
+        if (!PA[j]) {
+            if (a.nx.active)
+                put_next(a.nx, (uint32_t)(PK[j] >> 32), (uint32_t)PK[j], PR[j], carry_after(a.cols, a.rg, PR[j], false), 1);
+            else
+                atomicAdd(a.nx.total[1], 1u);
+        }
+    }
+}
+
+hipError_t launch_ooc_mark(const OocList &l, uint32_t *bitmap, hipStream_t s) {
+    const uint32_t g = l.nb < 65536 ? (l.nb ? l.nb : 1) : 65536;
+    hipLaunchKernelGGL(k_ooc_mark, dim3(g), dim3(256), 0, s, l, bitmap);
+    return hipGetLastError();
+}
+
+hipError_t launch_ooc_popc(const uint32_t *bitmap, uint64_t n, uint32_t *cnt, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ooc_popc, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, bitmap, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_ooc_tie(const OocTieArgs &a, hipStream_t s) {
+    const uint32_t g = a.l.nb < 65536 ? (a.l.nb ? a.l.nb : 1) : 65536;
+    hipLaunchKernelGGL(k_ooc_tie, dim3(g), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ooc_fetch(const OocList &l, const uint32_t *ids, const unsigned long long *off, uint32_t n,
+                            const uint16_t *chunk, OocRec *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ooc_fetch, dim3(n), dim3(256), 0, s, l, ids, off, chunk, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ooc_override(const uint32_t *ids, const unsigned long long *vals, uint32_t n,
+                               unsigned long long *tie, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ooc_override, dim3((n + 255) / 256), dim3(256), 0, s, ids, vals, n, tie);
+    return hipGetLastError();
+}
+
+hipError_t launch_dedup_ooc(const BigArgs &a, uint32_t nb, hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dedup_ooc, dim3(nb), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_join_ooc(const BigArgs &a, uint32_t nb, hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_join_ooc, dim3(nb), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 

@@ -31,6 +31,8 @@
 #include "emp_fastq.h"
 #include "emp_fastq_gpu.h"
 #include "emp_kernels.h"
+#include "emp_ooc.h"
+#include "emp_ooc_plan.h"
 
 using namespace emp;
 
@@ -205,6 +207,28 @@ struct emp_ctx {
     uint32_t *d_brank[2] = {nullptr, nullptr}, *d_baux[2] = {nullptr, nullptr};
     uint64_t bscratch_cap = 0;
     DevCounters *d_ctr = nullptr, *h_ctr = nullptr;
+    // the out-of-core order (2n > M): tie keys per id for each side, chunk per id, the id bitmap
+    // of a round's list with its word counts and offsets, the planner's count table, the merge
+    // tree's in-order indices, group reads, explicit ties, per-bucket scratch offsets and the
+    // scratch ties of the global-memory sorts
+    unsigned long long *d_tie[2] = {nullptr, nullptr};
+    uint16_t *d_chunk = nullptr;
+    uint64_t ooc_ids = 0;
+    uint32_t *d_obm = nullptr, *d_opc = nullptr;
+    unsigned long long *d_obase = nullptr, *d_oblk = nullptr;
+    uint32_t *d_obcnt = nullptr, *d_oio = nullptr;
+    uint32_t *d_ofids = nullptr;
+    unsigned long long *d_ofoff = nullptr;
+    uint64_t ofid_cap = 0, ofoff_cap = 0;
+    OocRec *d_orec = nullptr;
+    uint64_t orec_cap = 0;
+    uint32_t *d_ovid = nullptr;
+    unsigned long long *d_ovval = nullptr;
+    uint64_t ovid_cap = 0, ovval_cap = 0;
+    unsigned long long *d_ooff[2] = {nullptr, nullptr};
+    uint32_t ooff_nb = 0;
+    unsigned long long *d_btie[2] = {nullptr, nullptr};
+    uint64_t btie_cap = 0;
     // pipelined rounds: each round's counter record read back (pinned) with an event per round
     bool pipeline = true;
     ListCtr *h_rec = nullptr;
@@ -1533,6 +1557,283 @@ int big_path(emp_ctx *c, uint32_t nbig, int bb, bool dedup, const RoundArgs &rg,
 }
 
 // ---------------------------------------------------------------------------------------------
+// The out-of-core order (2n > M). cq_sort_by_key (cqutils.cuh:179-247) orders equal keys of a
+// list of more than M entries by its FIFO merge tree; every entry gets a tie key that reproduces
+// that order (the default: in-order index of its chunk, order inside the chunk; explicit ranks
+// for the keys the merge partitions reorder, found by the host planner of emp_ooc_plan.h). The
+// dedup and the joins of such lists then sort each bucket by (key, tie) in global scratch.
+
+int ensure_ooc(emp_ctx *c, uint64_t ids) {
+    int rc;
+    if (ids > c->ooc_ids) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        void *p = nullptr;
+        for (int s = 0; s < 2; ++s) {
+            dfree(c, c->d_tie[s]);
+            c->d_tie[s] = nullptr;
+            if ((rc = dalloc(c, &p, ids * sizeof(unsigned long long)))) return rc;
+            c->d_tie[s] = (unsigned long long *)p;
+        }
+        dfree(c, c->d_chunk);
+        c->d_chunk = nullptr;
+        if ((rc = dalloc(c, &p, ids * sizeof(uint16_t)))) return rc;
+        c->d_chunk = (uint16_t *)p;
+        const uint64_t words = (ids + 31) / 32;
+        dfree(c, c->d_obm);
+        dfree(c, c->d_opc);
+        dfree(c, c->d_obase);
+        dfree(c, c->d_oblk);
+        c->d_obm = c->d_opc = nullptr;
+        c->d_obase = c->d_oblk = nullptr;
+        if ((rc = dalloc(c, &p, words * sizeof(uint32_t)))) return rc;
+        c->d_obm = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, words * sizeof(uint32_t)))) return rc;
+        c->d_opc = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, (words + 1) * sizeof(unsigned long long)))) return rc;
+        c->d_obase = (unsigned long long *)p;
+        if ((rc = dalloc(c, &p, (fq_scan_blocks(words) + 1) * sizeof(unsigned long long)))) return rc;
+        c->d_oblk = (unsigned long long *)p;
+        c->ooc_ids = ids;
+    }
+    if (!c->d_obcnt) {
+        void *p = nullptr;
+        if ((rc = dalloc(c, &p, (PLAN_MAX_CELLS + PLAN_MAX_CHUNKS) * sizeof(uint32_t)))) return rc;
+        c->d_obcnt = (uint32_t *)p;
+        if ((rc = dalloc(c, &p, (PLAN_MAX_CHUNKS + 1) * sizeof(uint32_t)))) return rc;
+        c->d_oio = (uint32_t *)p;
+    }
+    return EMP_OK;
+}
+
+template <class T>
+int grow_dev(emp_ctx *c, T **p, uint64_t &cap, uint64_t need) {
+    if (need <= cap) return EMP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c, *p);
+    *p = nullptr;
+    const uint64_t n = std::max<uint64_t>(need, cap * 2);
+    void *q = nullptr;
+    int rc = dalloc(c, &q, n * sizeof(T));
+    if (rc) return rc;
+    *p = (T *)q;
+    cap = n;
+    return EMP_OK;
+}
+
+// The entries of one bucket group of a list, read from the device for the planner.
+class DevSource : public PlanSource {
+public:
+    DevSource(emp_ctx *c, const OocList &l, int gshift, const std::vector<uint32_t> &cnt)
+        : c_(c), l_(l), gshift_(gshift), cnt_(cnt) {}
+    int fetch(uint32_t g, std::vector<PlanRec> &out) override {
+        const uint32_t b0 = g << gshift_, b1 = std::min<uint32_t>(l_.nb, (g + 1) << gshift_);
+        std::vector<uint32_t> ids;
+        std::vector<unsigned long long> off;
+        uint64_t tot = 0;
+        for (uint32_t b = b0; b < b1; ++b)
+            if (cnt_[b]) {
+                ids.push_back(b);
+                off.push_back(tot);
+                tot += cnt_[b];
+            }
+        out.clear();
+        if (ids.empty()) return EMP_OK;
+        int rc;
+        if ((rc = grow_dev(c_, &c_->d_ofids, c_->ofid_cap, ids.size())) ||
+            (rc = grow_dev(c_, &c_->d_ofoff, c_->ofoff_cap, ids.size())) ||
+            (rc = grow_dev(c_, &c_->d_orec, c_->orec_cap, tot)))
+            return rc;
+        std::vector<OocRec> h(tot);
+        HIPCHK(c_, hipMemcpyAsync(c_->d_ofids, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                  c_->stream));
+        HIPCHK(c_, hipMemcpyAsync(c_->d_ofoff, off.data(), off.size() * sizeof(unsigned long long),
+                                  hipMemcpyHostToDevice, c_->stream));
+        HIPCHK(c_, launch_ooc_fetch(l_, c_->d_ofids, c_->d_ofoff, (uint32_t)ids.size(), c_->d_chunk, c_->d_orec,
+                                    c_->stream));
+        HIPCHK(c_, hipMemcpyAsync(h.data(), c_->d_orec, tot * sizeof(OocRec), hipMemcpyDeviceToHost, c_->stream));
+        if ((rc = sync(c_))) return rc;
+        out.resize(tot);
+        for (uint64_t i = 0; i < tot; ++i) out[i] = PlanRec{h[i].key, h[i].rid, h[i].chunk};
+        return EMP_OK;
+    }
+
+private:
+    emp_ctx *c_;
+    OocList l_;
+    int gshift_;
+    const std::vector<uint32_t> &cnt_;
+};
+
+// Tie keys of one list of n entries (d_tie[side]); with `unique`, also the check that the
+// dedup's cq_get_partitions (cqutils.cuh:86-105) ends.
+int ooc_ties(emp_ctx *c, const OocList &l, uint64_t n, bool rounds, int side, uint64_t M, bool unique) {
+    const uint64_t F64 = n <= M ? 1 : (n + M - 1) / M;
+    if (F64 > PLAN_MAX_CHUNKS)
+        return fail(c, EMP_ERR_OUT_OF_CORE, "%llu entries need %llu sort chunks of M = %llu (at most %u): raise -r",
+                    (unsigned long long)n, (unsigned long long)F64, (unsigned long long)M, PLAN_MAX_CHUNKS);
+    const uint32_t F = (uint32_t)F64;
+    const std::vector<uint32_t> io = plan_tree_io(F);
+    int bb = 0;
+    while ((1u << bb) < l.nb) ++bb;
+    const int gshift = plan_group_shift(l.nb, F);
+    const uint32_t G = std::max<uint32_t>(1u, l.nb >> gshift);
+    OocTieArgs a{};
+    a.l = l;
+    a.rounds = rounds ? 1 : 0;
+    a.M = M;
+    a.F = F;
+    a.io = c->d_oio;
+    a.tie = c->d_tie[side];
+    a.chunk = c->d_chunk;
+    a.gshift = gshift;
+    HIPCHK(c, hipMemcpyAsync(c->d_oio, io.data(), F * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    if (F > 1) {
+        HIPCHK(c, hipMemsetAsync(c->d_obcnt, 0, (size_t)G * F * sizeof(uint32_t), c->stream));
+        a.bcnt = c->d_obcnt;
+        if (rounds) {  // rank of every id among the list's ids: bitmap, set bits per word, offsets
+            const uint64_t words = (2 * c->n + 31) / 32;
+            HIPCHK(c, hipMemsetAsync(c->d_obm, 0, words * sizeof(uint32_t), c->stream));
+            HIPCHK(c, launch_ooc_mark(l, c->d_obm, c->stream));
+            HIPCHK(c, launch_ooc_popc(c->d_obm, words, c->d_opc, c->stream));
+            HIPCHK(c, launch_scan(c->d_opc, words, c->d_obase, c->d_oblk, c->stream));
+            a.bitmap = c->d_obm;
+            a.base = c->d_obase;
+        }
+    }
+    {
+        Timed t(c, "k_ooc_tie", 24.0 * (double)n);
+        HIPCHK(c, launch_ooc_tie(a, c->stream));
+    }
+    if (F == 1) return sync(c);  // one chunk: the input order alone
+    PlanList pl;
+    pl.n = n;
+    pl.M = M;
+    pl.F = F;
+    pl.G = G;
+    pl.key_shift = 29 - bb + gshift;
+    pl.rounds = a.rounds;
+    pl.bcnt.resize((size_t)G * F);
+    std::vector<uint32_t> cnt(l.nb);
+    HIPCHK(c, hipMemcpyAsync(pl.bcnt.data(), c->d_obcnt, pl.bcnt.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), l.cnt, l.nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    int rc = sync(c);
+    if (rc) return rc;
+    DevSource src(c, l, gshift, cnt);
+    Planner p(pl, src);
+    std::vector<uint32_t> ids;
+    std::vector<unsigned long long> vals;
+    if ((rc = p.overrides(ids, vals)))
+        return rc == EMP_ERR_CAPACITY ? fail(c, rc, "out-of-core order: inconsistent counts of a list of %llu entries",
+                                             (unsigned long long)n)
+                                      : rc;
+    if (unique && (rc = p.unique_check()))
+        return rc == EMP_ERR_PARTITION
+                   ? fail(c, rc, "dedup: a window of M = %llu entries holds one key (cq_get_partitions never ends)",
+                          (unsigned long long)M)
+                   : rc;
+    if (!ids.empty()) {
+        if ((rc = grow_dev(c, &c->d_ovid, c->ovid_cap, ids.size())) ||
+            (rc = grow_dev(c, &c->d_ovval, c->ovval_cap, ids.size())))
+            return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_ovid, ids.data(), ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_ovval, vals.data(), vals.size() * sizeof(unsigned long long),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_ooc_override(c->d_ovid, c->d_ovval, (uint32_t)ids.size(), c->d_tie[side], c->stream));
+    }
+    return sync(c);
+}
+
+// Scratch offsets (powers of two of the counts) of every bucket of one or two lists; sizes the
+// global-memory sort scratch.
+int ooc_offsets(emp_ctx *c, uint32_t nb, const uint32_t *d_cnt_s, const uint32_t *d_cnt_p) {
+    int rc;
+    if (nb > c->ooff_nb) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int s = 0; s < 2; ++s) {
+            dfree(c, c->d_ooff[s]);
+            c->d_ooff[s] = nullptr;
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, (size_t)nb * sizeof(unsigned long long)))) return rc;
+            c->d_ooff[s] = (unsigned long long *)p;
+        }
+        c->ooff_nb = nb;
+    }
+    std::vector<uint32_t> cs(nb), cp(d_cnt_p ? nb : 0);
+    HIPCHK(c, hipMemcpyAsync(cs.data(), d_cnt_s, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if (d_cnt_p) HIPCHK(c, hipMemcpyAsync(cp.data(), d_cnt_p, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = sync(c))) return rc;
+    std::vector<unsigned long long> so(nb), po(nb);
+    uint64_t as = 0, ap = 0;
+    // as the kernels pad: an empty side of a bucket still takes one record
+    auto p2 = [](uint64_t n) { uint64_t p = 1; while (p < n) p <<= 1; return p; };
+    for (uint32_t b = 0; b < nb; ++b) {
+        so[b] = as;
+        as += p2(cs[b]);
+        po[b] = ap;
+        ap += d_cnt_p ? p2(cp[b]) : 0;
+    }
+    const uint64_t need = std::max<uint64_t>(std::max(as, ap), 1);
+    if ((rc = ensure_big_scratch(c, need))) return rc;
+    if (need > c->btie_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int s = 0; s < 2; ++s) {
+            dfree(c, c->d_btie[s]);
+            c->d_btie[s] = nullptr;
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, need * sizeof(unsigned long long)))) return rc;
+            c->d_btie[s] = (unsigned long long *)p;
+        }
+        c->btie_cap = need;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_ooff[0], so.data(), nb * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ooff[1], po.data(), nb * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+BigArgs ooc_args(emp_ctx *c, const SideIn &sin, const SideIn *pin, const RoundArgs &rg, const NextOut &nx,
+                 uint64_t slot0) {
+    BigArgs a{};
+    a.soff = c->d_ooff[0];
+    a.poff = c->d_ooff[1];
+    a.skey = c->d_bkey[0];
+    a.pkey = c->d_bkey[1];
+    a.srank = c->d_brank[0];
+    a.prank = c->d_brank[1];
+    a.saux = c->d_baux[0];
+    a.paux = c->d_baux[1];
+    a.stie = c->d_btie[0];
+    a.ptie = c->d_btie[1];
+    a.tie_s = c->d_tie[0];
+    a.tie_p = c->d_tie[1];
+    a.s_in = sin.fine;
+    a.s_cnt = sin.cnt;
+    a.s_spill = sin.spill;
+    a.s_spill_n = sin.spill_n;
+    a.spill_cap = sin.spill_cap;
+    if (pin) {
+        a.p_in = pin->fine;
+        a.p_cnt = pin->cnt;
+        a.p_spill = pin->spill;
+        a.p_spill_n = pin->spill_n;
+        a.p_spill_cap = pin->spill_cap;
+    }
+    a.cap = BUCKET_CAP;
+    a.rg = rg;
+    a.cols = c->d_cols;
+    a.nx = nx;
+    a.pool = c->d_pool;
+    a.pool_cursor = c->d_ctr->pool;
+    a.pool_cap = c->pool_cap;
+    a.ovf_cursor = &c->d_ctr->pool_ovf;
+    a.ovf_cap = c->pool_ovf_cap;
+    a.edge_off = c->d_edge_off + slot0;
+    a.edge_cnt = c->d_edge_cnt + slot0;
+    a.too_many = &c->d_ctr->too_many;
+    return a;
+}
+
+// ---------------------------------------------------------------------------------------------
 // The overlap phase: dedup, then the rounds (synchronous or pipelined), then the output.
 
 constexpr int RUN_FALLBACK = 1;  // pipelined rounds met an exceptional round: rerun synchronously
@@ -1569,7 +1870,8 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
     if (st.rounds > 0 && (rc = ensure_edge_slots(c, (uint64_t)st.rounds * st.stride))) return rc;
     const RoundArgs none{};
     const RoundArgs rg0 = st.rounds > 0 ? round_args(c, 0, true) : none;
-    bool need_part = true;
+    const bool ooc = N0 > st.M;  // the out-of-core order decides which id of a key is kept
+    bool need_part = true, ties = false;
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "dedup does not settle");
         {
@@ -1580,6 +1882,38 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
             if ((rc = run_zero(c, z))) return rc;
         }
         if (need_part && (rc = partition(c, N0, bbD))) return rc;
+        if (ooc) {
+            // cq_sort_by_key + cq_unique_by_key (overlap.cu:212-225, cqutils.cuh:179-247,271-306):
+            // the first entry of every key in the merged order
+            if ((rc = read_counters(c))) return rc;
+            if (c->h_ctr->abort) {
+                if ((rc = grow_after_abort(c, N0, bbD))) return rc;
+                need_part = true;
+                ties = false;
+                continue;
+            }
+            need_part = false;
+            const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
+            if (!ties) {
+                const OocList l{din.fine, din.cnt, BUCKET_CAP, din.spill, din.spill_n, din.spill_cap, -1, 1u << bbD};
+                if ((rc = ensure_ooc(c, N0)) || (rc = ooc_ties(c, l, N0, false, 0, st.M, true)) ||
+                    (rc = ooc_offsets(c, 1u << bbD, din.cnt, nullptr)))
+                    return rc;
+                ties = true;
+            }
+            BigArgs a = ooc_args(c, din, nullptr, none, next_out(c, 0, bbD, bbD, rg0), 0);
+            {
+                Timed t(c, "k_dedup_ooc", 0.0);
+                HIPCHK(c, launch_dedup_ooc(a, 1u << bbD, c->stream));
+            }
+            if ((rc = read_counters(c))) return rc;
+            if (c->h_ctr->r[0].nx_ovf) {
+                for (int side = 0; side < 2; ++side)
+                    if ((rc = grow_bspill(c, 0, side, c->bspill_cap[0][side] * 4))) return rc;
+                continue;
+            }
+            break;
+        }
         DedupArgs a{};
         a.in = c->d_dfine;
         a.cnt = c->d_dcnt;
@@ -1654,6 +1988,15 @@ int rounds_sync(emp_ctx *c, RunState &st) {
         rs.bucket_bits = bb;
         const uint64_t slot0 = (uint64_t)t * stride;
         const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
+        // a list over M: cq_sort_by_key's merge order (overlap.cu:271-272) as tie keys per side
+        const bool ooc = St > M || Pt > M;
+        if (ooc) {
+            const OocList ls{sin.fine, sin.cnt, BUCKET_CAP, sin.spill, sin.spill_n, sin.spill_cap, 29 - bb, nbt};
+            const OocList lp{pin.fine, pin.cnt, BUCKET_CAP, pin.spill, pin.spill_n, pin.spill_cap, 29 - bb, nbt};
+            if ((rc = ensure_ooc(c, 2 * c->n)) || (rc = ooc_ties(c, ls, St, true, 0, M, false)) ||
+                (rc = ooc_ties(c, lp, Pt, true, 1, M, false)) || (rc = ooc_offsets(c, nbt, sin.cnt, pin.cnt)))
+                return rc;
+        }
         for (int attempt = 0;; ++attempt) {
             if (attempt > 8) return fail(c, EMP_ERR_CAPACITY, "round %d does not settle", t);
             if (attempt > 0)  // a rerun: the pool cursors go back to the end of the previous round
@@ -1664,34 +2007,45 @@ int rounds_sync(emp_ctx *c, RunState &st) {
                 zero_next(c, z, t + 1, bbn);
                 if ((rc = run_zero(c, z))) return rc;
             }
-            JoinArgs a{};
-            a.s_in = sin.fine;
-            a.p_in = pin.fine;
-            a.s_cnt = sin.cnt;
-            a.p_cnt = pin.cnt;
-            a.zero_in = 0;
-            a.nx = next_out(c, t + 1, bbn, bb, rgn);
-            a.pool = c->d_pool;
-            a.pool_cursor = c->d_ctr->pool;
-            a.pool_cap = c->pool_cap;
-            a.ovf_cursor = &c->d_ctr->pool_ovf;
-            a.ovf_cap = c->pool_ovf_cap;
-            a.edge_off = c->d_edge_off + slot0;
-            a.edge_cnt = c->d_edge_cnt + slot0;
-            a.big = c->d_big;
-            a.n_big = &c->d_ctr->r[t + 1].n_big;
-            a.big_cap = BIG_CAP;
-            a.abort = &c->d_ctr->abort;
-            {
-                Timed tm(c, "k_join", 0.0);
-                HIPCHK(c, launch_join(a, nbt, c->stream));
+            if (ooc) {
+                BigArgs a = ooc_args(c, sin, &pin, rg, next_out(c, t + 1, bbn, bb, rgn), slot0);
+                {
+                    Timed tm(c, "k_join_ooc", 0.0);
+                    HIPCHK(c, launch_join_ooc(a, nbt, c->stream));
+                }
+                if ((rc = read_counters(c))) return rc;
+                if (c->h_ctr->too_many)  // the reference's int edge offsets (overlap.cu:561,567)
+                    return fail(c, EMP_ERR_PARTITION, "a key group holds 2^31 edges or more");
+            } else {
+                JoinArgs a{};
+                a.s_in = sin.fine;
+                a.p_in = pin.fine;
+                a.s_cnt = sin.cnt;
+                a.p_cnt = pin.cnt;
+                a.zero_in = 0;
+                a.nx = next_out(c, t + 1, bbn, bb, rgn);
+                a.pool = c->d_pool;
+                a.pool_cursor = c->d_ctr->pool;
+                a.pool_cap = c->pool_cap;
+                a.ovf_cursor = &c->d_ctr->pool_ovf;
+                a.ovf_cap = c->pool_ovf_cap;
+                a.edge_off = c->d_edge_off + slot0;
+                a.edge_cnt = c->d_edge_cnt + slot0;
+                a.big = c->d_big;
+                a.n_big = &c->d_ctr->r[t + 1].n_big;
+                a.big_cap = BIG_CAP;
+                a.abort = &c->d_ctr->abort;
+                {
+                    Timed tm(c, "k_join", 0.0);
+                    HIPCHK(c, launch_join(a, nbt, c->stream));
+                }
+                if ((rc = read_counters(c))) return rc;
+                const ListCtr &rn = c->h_ctr->r[t + 1];
+                rs.big_buckets = (int32_t)rn.n_big;
+                if (!rn.nx_ovf && rn.n_big &&
+                    (rc = big_path(c, rn.n_big, bb, false, rg, slot0, sin, pin, next_out(c, t + 1, bbn, bb, rgn))))
+                    return rc;
             }
-            if ((rc = read_counters(c))) return rc;
-            const ListCtr &rn = c->h_ctr->r[t + 1];
-            rs.big_buckets = (int32_t)rn.n_big;
-            if (!rn.nx_ovf && rn.n_big &&
-                (rc = big_path(c, rn.n_big, bb, false, rg, slot0, sin, pin, next_out(c, t + 1, bbn, bb, rgn))))
-                return rc;
             if (c->h_ctr->r[t + 1].nx_ovf) {
                 for (int side = 0; side < 2; ++side)
                     if ((rc = grow_bspill(c, nset, side, c->bspill_cap[nset][side] * 4))) return rc;
@@ -2038,11 +2392,7 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     const int rounds = L - c->opts.min_ovlp;
     const uint64_t N0 = 2 * n;
     const uint64_t M = c->opts.reduce_mem / 16;
-    if (N0 > M)
-        return fail(c, EMP_ERR_OUT_OF_CORE,
-                    "2n = %llu entries exceed M = reduce_mem/16 = %llu: the reference's out-of-core merge order "
-                    "is not reproduced; raise -r",
-                    (unsigned long long)N0, (unsigned long long)M);
+    if (M < 2) return fail(c, EMP_ERR_ARG, "reduce_mem / 16 must be at least 2");
 
     const int bbD = round_bits(N0);  // the dedup's buckets and round 0's
     if ((rc = ensure_run_buffers(c, N0, bbD))) return rc;

@@ -17,6 +17,7 @@ ERRORS = {
     -6: "EMP_ERR_OUT_OF_CORE", -7: "EMP_ERR_NOMEM", -8: "EMP_ERR_TOO_MANY", -9: "EMP_ERR_PARTITION",
     -10: "EMP_ERR_IO", -11: "EMP_ERR_CAPACITY",
 }
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 
 # Every symbol the headers in include/ declare.
 EXPORTS = [
@@ -25,7 +26,8 @@ EXPORTS = [
     "emp_add_reads_device", "emp_run", "emp_run_to_file", "emp_reset", "emp_get_summary", "emp_get_round",
     "emp_kernel_count", "emp_kernel_info", "emp_kernel_clear", "emp_set_profile", "emp_stream", "emp_version",
     "emp_set_audit", "emp_audit_count", "emp_get_audit",
-    "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll", "emp_stage_fingerprints_noalloc",
+    "emp_stage_fingerprints", "emp_stage_columns", "emp_stage_roll", "emp_stage_ooc_ties",
+    "emp_stage_fingerprints_noalloc",
     "emp_batch_opts_default", "emp_batch_create", "emp_batch_destroy", "emp_batch_last_error", "emp_batch_submit",
     "emp_batch_submit_fastq", "emp_batch_job_bytes", "emp_batch_step", "emp_batch_run", "emp_batch_job_info",
     "emp_batch_reset", "emp_batch_clock_ms", "emp_batch_kernel_count", "emp_batch_kernel_info",
@@ -117,6 +119,7 @@ def lib():
         L.emp_stage_fingerprints.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_columns.argtypes = [i32, vp, u64, i32, vp]
         L.emp_stage_roll.argtypes = [i32, vp, vp, u64, i32, i32, vp]
+        L.emp_stage_ooc_ties.argtypes = [vp, vp, u64, u64, i32, i32, vp]
         L.emp_stage_fingerprints_noalloc.argtypes = [vp, u64, i32, vp, vp, vp]
         L.emp_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
         L.emp_batch_create.argtypes = [ctypes.POINTER(BatchOpts), ctypes.POINTER(vp)]
@@ -459,6 +462,21 @@ def stage_columns(reads: np.ndarray, device=0) -> np.ndarray:
     if rc:
         raise EmpError(rc, "emp_stage_columns")
     return out
+
+
+def stage_ooc_ties(keys: np.ndarray, rids: np.ndarray, M: int, rounds: bool, bucket_bits: int):
+    """Tie keys of the out-of-core order (host planner, no device work): (ties, status), status
+    EMP_OK or EMP_ERR_PARTITION (the dedup's unique would not end; ties are still written)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    r = np.ascontiguousarray(rids, dtype=np.uint32)
+    if k.shape != r.shape:
+        raise ValueError("keys and rids differ in shape")
+    out = np.empty(k.size, dtype=np.uint64)
+    rc = lib().emp_stage_ooc_ties(k.ctypes.data, r.ctypes.data, k.size, M, 1 if rounds else 0, bucket_bits,
+                                  out.ctypes.data)
+    if rc not in (EMP_OK, -9):
+        raise EmpError(rc, "emp_stage_ooc_ties")
+    return out, rc
 
 
 def stage_roll(keys: np.ndarray, bases: np.ndarray, kind: int, length: int, device=0) -> np.ndarray:

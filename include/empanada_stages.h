@@ -37,6 +37,25 @@ int emp_stage_roll(int32_t device, const uint64_t *keys, const char *bases, uint
                    uint64_t *out);
 
 /*
+ * The out-of-core order of cq_sort_by_key (cqutils.cuh:179-247: chunks of M sorted in input
+ * order, the FIFO of sorted runs merged pairwise through cq_get_merge_partitions, 36-84, and
+ * cq_merge_by_key, 139-177) as the overlap phase reproduces it once a list exceeds
+ * M = reduce_mem / 16: writes a tie key per entry such that the entries sorted by (key, tie) are
+ * in cq_sort_by_key's output order. The list is n entries (keys[i], rids[i]), keys of two 29-bit
+ * halves (HIGH32 << 32 | LOW32), ids distinct; its input order is
+ *   rounds = 0, the dedup (overlap.cu:212-213): id order, ids a permutation of 0..n-1;
+ *   rounds = 1, a round (overlap.cu:266-272): the list sorted by id, cut into chunks of M, each
+ *     chunk's even ids then its odd ids (regenerate_fingerprints, overlap.cu:613-659).
+ * bucket_bits (1..29) is the bucketing the library would use for the list: bucket of a key =
+ * HIGH32 >> (29 - bucket_bits). This is the host planner of that path on host arrays (no device
+ * work). Returns EMP_ERR_OUT_OF_CORE when n needs more than 65535 chunks, and for rounds = 0
+ * EMP_ERR_PARTITION (ties written) when cq_get_partitions (cqutils.cuh:86-105) would not end:
+ * a window of M entries, not the last one, holding a single key.
+ */
+int emp_stage_ooc_ties(const uint64_t *keys, const uint32_t *rids, uint64_t n, uint64_t M, int32_t rounds,
+                       int32_t bucket_bits, uint64_t *ties);
+
+/*
  * Allocation-free form of emp_stage_fingerprints. It has no counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). Input and output are the caller's device memory: d_reads (n x

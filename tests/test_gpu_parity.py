@@ -345,11 +345,16 @@ def test_error_paths():
     with pytest.raises(native.EmpError) as e:
         run_gpu_fastq(b"xx\nACGT\n+\nIIII\n")
     assert e.value.code == -3
+    with pytest.raises(native.EmpError) as e:  # reduce_mem / 16 below 2
+        run_gpu_fastq(FQ([b"ACGT"]), reduce_mem=16)
+    assert e.value.code == -1
+    # 2n = 300 > M = 200: the out-of-core merge order, as the oracle restates it
     rng = random.Random(5)
     reads = _random_reads(rng, 150, 30, 600, 0.5, 0.0)
-    with pytest.raises(native.EmpError) as e:
-        run_gpu_fastq(FQ([x.encode() for x in reads]), min_ovlp=12, reduce_mem=16 * 200)
-    assert e.value.code == -6
+    data = FQ([x.encode() for x in reads])
+    rc, want, _, _ = ol.run_fastq(data, min_ovlp=12, reduce_mem=16 * 200)
+    assert rc == 0
+    assert run_gpu_fastq(data, min_ovlp=12, reduce_mem=16 * 200)[0] == want
 
 
 @pytest.mark.parametrize("m", [300, 320, 450, 540])
