@@ -25,7 +25,7 @@ extern "C" {
 #define EMP_ERR_FORMAT -3       /* FASTQ block does not start with '@', or a record exceeds a block */
 #define EMP_ERR_READLEN -4      /* kept reads of unequal length, or longer than 255 */
 #define EMP_ERR_BASE -5         /* kept read holds a byte other than ACGTacgt */
-#define EMP_ERR_OUT_OF_CORE -6  /* 2n > reduce_mem/16: the reference's chunked merge order */
+#define EMP_ERR_OUT_OF_CORE -6  /* a list needs more than 65535 sort chunks of reduce_mem/16 */
 #define EMP_ERR_NOMEM -7        /* device memory or the context's HBM budget exhausted */
 #define EMP_ERR_TOO_MANY -8     /* 2n >= 2^32 oriented read ids */
 #define EMP_ERR_PARTITION -9    /* a key-aligned search partition cannot be formed */
