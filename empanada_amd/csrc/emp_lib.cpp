@@ -2756,6 +2756,7 @@ int emp_stage_fingerprints(int32_t device, const char *reads, uint64_t n, int32_
 
 int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len, uint32_t *cols) {
     if ((!reads || !cols) && n) return EMP_ERR_ARG;
+    if (len < 1 || len > MAX_RD_LEN) return EMP_ERR_ARG;
     if (n == 0) return EMP_OK;
     emp_opts o;
     emp_opts_default(&o);
@@ -2767,25 +2768,9 @@ int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len
     rc = dalloc(c, &d, n * (uint64_t)len);
     if (!rc && hipMemcpy(d, reads, n * (uint64_t)len, hipMemcpyHostToDevice) != hipSuccess) rc = EMP_ERR_HIP;
     if (!rc) rc = emp_add_reads_device(c, d, n, len);
-    // device layout (wide store: 2 reads x 8 columns per word) -> columns of 16 reads per word
-    const uint64_t W = (n + 15) / 16;
-    const uint32_t G8 = col_groups8((uint32_t)len);
-    std::vector<uint32_t> dev;
-    if (!rc) {
-        dev.resize(c->cols_chunks * G8 * (size_t)CHUNK_WWORDS);
-        if (hipMemcpy(dev.data(), c->d_cols, dev.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
-    }
-    if (!rc) {
-        memset(cols, 0, (size_t)len * W * 4);
-        for (uint64_t k = 0; k < n; ++k) {
-            const size_t base = ((k >> CHUNK_SHIFT) * G8) * (size_t)CHUNK_WWORDS + ((k >> 1) & (CHUNK_WWORDS - 1));
-            for (int j = 0; j < len; ++j) {
-                const uint32_t w = dev[base + (size_t)(j >> 3) * CHUNK_WWORDS];
-                const uint32_t code = (w >> (2 * (8 * (k & 1) + (j & 7)))) & 3u;
-                cols[(uint64_t)j * W + (k >> 4)] |= code << (2 * (k & 15));
-            }
-        }
-    }
+    // the store the rounds gather from, as k_map wrote it
+    const uint64_t words = ((n + CHUNK_READS - 1) >> CHUNK_SHIFT) * col_groups8((uint32_t)len) * (uint64_t)CHUNK_WWORDS;
+    if (!rc && hipMemcpy(cols, c->d_cols, words * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = EMP_ERR_HIP;
     emp_ctx_destroy(c);
     return rc;
 }

@@ -89,26 +89,38 @@ void empgen_genome(uint64_t seed, uint64_t G, char *out) {
 
 /*
  * Overwrite a fraction `frac` of the genome with repeat copies. Families of consensus length
- * len_f, `copies` copies each, per-copy substitution rate div_f. Copies occupy disjoint slots
- * of a grid of stride len_f (slot chosen uniformly without replacement by rejection), so they
- * never overlap. Returns the number of families written.
+ * len_f, `copies` copies each, per-copy substitution rate div_f. A copy takes a slot of a grid of
+ * stride len_f, chosen uniformly by rejection among the slots whose `unit`-base cells (len_f a
+ * multiple of unit) are all free in `used` (one byte per cell, n_cells of them, shared by every
+ * family size), so copies of any sizes never overlap. Returns the number of families written.
  */
 uint64_t empgen_add_repeats(uint64_t seed, char *genome, uint64_t G, double frac, uint64_t len_f,
-                            uint64_t copies, double div_f, unsigned char *slot_used, uint64_t n_slots) {
-    if (len_f == 0 || copies == 0 || frac <= 0.0) return 0;
+                            uint64_t copies, double div_f, unsigned char *used, uint64_t n_cells, uint64_t unit) {
+    if (len_f == 0 || copies == 0 || frac <= 0.0 || unit == 0 || len_f % unit) return 0;
+    const uint64_t per = len_f / unit;
     uint64_t slots = G / len_f;
-    if (slots > n_slots) slots = n_slots;
+    if (slots * per > n_cells) slots = n_cells / per;
+    uint64_t free_slots = 0;
+    for (uint64_t s = 0; s < slots; ++s) {
+        uint64_t k = 0;
+        while (k < per && !used[s * per + k]) ++k;
+        free_slots += k == per;
+    }
     uint64_t fams = (uint64_t)(frac * (double)G / (double)(len_f * copies));
-    if (fams * copies > slots / 2) fams = slots / (2 * copies);
+    if (fams * copies > free_slots / 2) fams = free_slots / (2 * copies);
     xo_t r;
     xo_seed(&r, seed);
     char *cons = (char *)malloc(len_f);
     for (uint64_t f = 0; f < fams; ++f) {
         for (uint64_t j = 0; j < len_f; ++j) cons[j] = ACGT[xo_next(&r) >> 62];
         for (uint64_t c = 0; c < copies; ++c) {
-            uint64_t s;
-            do { s = xo_below(&r, slots); } while (slot_used[s]);
-            slot_used[s] = 1;
+            uint64_t s, k;
+            do {
+                s = xo_below(&r, slots);
+                k = 0;
+                while (k < per && !used[s * per + k]) ++k;
+            } while (k < per);
+            memset(used + s * per, 1, per);
             int rc = (xo_next(&r) >> 63) != 0;
             char *dst = genome + s * len_f;
             for (uint64_t j = 0; j < len_f; ++j) {

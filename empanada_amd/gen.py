@@ -12,6 +12,7 @@ The workloads, with their seeds (genome, reads):
                 (1300 + i, 1300 + i + 4096)
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -33,7 +34,7 @@ def _load():
         L.empgen_add_repeats.restype = ctypes.c_uint64
         L.empgen_add_repeats.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
                                          ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p,
-                                         ctypes.c_uint64]
+                                         ctypes.c_uint64, ctypes.c_uint64]
         L.empgen_fastq_size.restype = ctypes.c_uint64
         L.empgen_fastq_size.argtypes = [ctypes.c_uint64, ctypes.c_int]
         L.empgen_fastq.restype = ctypes.c_uint64
@@ -50,9 +51,11 @@ def genome(G: int, seed: int) -> np.ndarray:
 
 
 def add_repeats(g: np.ndarray, seed: int, frac: float, len_f: int, copies: int, div: float,
-                used: np.ndarray) -> int:
+                used: np.ndarray, unit: int) -> int:
+    """Repeat copies of length len_f on free slots of `used` (one byte per `unit` bases, shared by
+    every family size: copies never overlap)."""
     return _load().empgen_add_repeats(seed, g.ctypes.data, g.size, frac, len_f, copies, div, used.ctypes.data,
-                                      used.size)
+                                      used.size, unit)
 
 
 def reads(g: np.ndarray, n: int, L: int, p_rc: float, p_sub: float, seed: int, out: np.ndarray = None) -> np.ndarray:
@@ -138,8 +141,12 @@ def workload_reads(name: str, scale: float = 1.0, out: np.ndarray = None):
     G = max(int(w["G"] * scale), 10 * w["L"])
     n = max(int(w["n"] * scale), 1)
     g = genome(G, w["seeds"][0])
-    for frac, len_f, copies, div, seed in w.get("repeats", ()):
-        # each family size on its own grid of slots; copies of the two sizes may overlap
-        used = np.zeros(G // len_f, dtype=np.uint8)
-        add_repeats(g, seed, frac, len_f, copies, div, used)
+    fams = w.get("repeats", ())
+    if fams:
+        # one occupancy map for every family size (cells of their common divisor), the longest
+        # families placed first: copies of different sizes never overlap
+        unit = math.gcd(*[f[1] for f in fams])
+        used = np.zeros(G // unit, dtype=np.uint8)
+        for frac, len_f, copies, div, seed in sorted(fams, key=lambda f: -f[1]):
+            add_repeats(g, seed, frac, len_f, copies, div, used, unit)
     return reads(g, n, w["L"], w["p_rc"], w["p_sub"], w["seeds"][1], out=out), w["o"]

@@ -454,10 +454,11 @@ def stage_fingerprints(reads: np.ndarray, device=0) -> np.ndarray:
 
 
 def stage_columns(reads: np.ndarray, device=0) -> np.ndarray:
+    """The column store (chunks of 2^20 reads, groups of 8 columns, 2 reads x 8 columns per word) as
+    a (chunks, ceil(L/8), 2^19) uint32 array."""
     r = np.ascontiguousarray(reads, dtype=np.uint8)
     n, L = r.shape
-    W = (n + 15) // 16
-    out = np.empty((L, W), dtype=np.uint32)
+    out = np.empty(((n + (1 << 20) - 1) >> 20, (L + 7) // 8, 1 << 19), dtype=np.uint32)
     rc = lib().emp_stage_columns(device, r.ctypes.data, n, L, out.ctypes.data)
     if rc:
         raise EmpError(rc, "emp_stage_columns")

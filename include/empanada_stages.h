@@ -22,9 +22,12 @@ extern "C" {
 int emp_stage_fingerprints(int32_t device, const char *reads, uint64_t n, int32_t len, uint64_t *keys);
 
 /*
- * transpose_reads + TBlockList::pack (fingerprint.cu:125-153, blocks.cuh:30-47), in this
- * library's layout: column j of n reads as ceil(n/16) 32-bit words, base k in bits
- * 2*(k%16)+1 : 2*(k%16) of word k/16, code (c>>1)&3 (A0 C1 T2 G3). cols holds len * ceil(n/16) words.
+ * transpose_reads + TBlockList::pack (fingerprint.cu:125-153, blocks.cuh:30-47): the 2-bit column
+ * store as the map kernel writes it for the rounds, code (c>>1)&3 (A0 C1 T2 G3). Reads are grouped
+ * in chunks of 2^20, columns in groups of 8; one 32-bit word holds 2 reads x 8 columns: base j of
+ * read k is at bits 2*(8*(k%2) + j%8) of word (ch * G8 + j/8) * 2^19 + (k % 2^20) / 2, with
+ * ch = k / 2^20 and G8 = ceil(len/8). cols holds ceil(n / 2^20) * G8 * 2^19 words; bits of reads
+ * past n are unspecified, columns past len are zero.
  */
 int emp_stage_columns(int32_t device, const char *reads, uint64_t n, int32_t len, uint32_t *cols);
 

@@ -44,19 +44,27 @@ def test_stage_fingerprints_rejects_non_acgt():
         native.stage_fingerprints(reads_of(["ACGN"]))
 
 
-@pytest.mark.parametrize("n,L", [(1, 7), (16, 100), (1000, 33), (70000, 20)])
+@pytest.mark.parametrize("n,L", [(1, 7), (16, 100), (1000, 33), (70001, 20), (1048577, 9)])
 def test_stage_columns(n, L):
+    """The map kernel's column store, the one the rounds gather from: base j of read k at bits
+    2(8(k%2) + j%8) of word (k / 2^20, j / 8, (k % 2^20) / 2); columns past L zero."""
     rng = np.random.default_rng(n)
-    r = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=(n, L))
-    cols = native.stage_columns(r)
-    codes = ((r >> 1) & 3).astype(np.uint32)  # A0 C1 T2 G3 (blocks.cuh:33-40)
-    W = (n + 15) // 16
-    pad = np.zeros((W * 16, L), dtype=np.uint32)
-    pad[:n] = codes
-    want = np.zeros((L, W), dtype=np.uint32)
-    for i in range(16):
-        want |= pad[i::16].T << (2 * i)
-    assert np.array_equal(cols, want)
+    r = rng.choice(np.frombuffer(b"ACGTacgt", dtype=np.uint8), size=(n, L))
+    store = native.stage_columns(r)
+    codes = ((r >> 1) & 3).astype(np.uint32)  # A0 C1 T2 G3 (blocks.cuh:33-40), lowercase alike
+    G8 = (L + 7) // 8
+    pad = np.zeros((store.shape[0] << 20, G8 * 8), dtype=np.uint32)
+    pad[:n, :L] = codes
+    want = np.zeros(store.shape, dtype=np.uint32)
+    for half in (0, 1):
+        rows = pad[half::2].reshape(store.shape[0], 1 << 19, G8, 8)  # (chunk, word, group, column)
+        for c in range(8):
+            want |= rows[:, :, :, c].transpose(0, 2, 1) << np.uint32(2 * (8 * half + c))
+    valid = np.zeros((store.shape[0] << 20,), dtype=bool)
+    valid[:n] = True
+    vmask = np.where(valid[0::2], 0xFFFF, 0).astype(np.uint32) | np.where(valid[1::2], 0xFFFF0000, 0).astype(np.uint32)
+    vmask = vmask.reshape(store.shape[0], 1, 1 << 19)
+    assert np.array_equal(store & vmask, want & vmask)
 
 
 def test_stage_roll_matches_oracle_and_wrap():

@@ -67,3 +67,20 @@ def test_cli_rejects_bad_option_and_unit():
     assert r.returncode == 2 and "Invalid option" in r.stdout
     r = subprocess.run([native.CLI_PATH, "a.fq", "b.bin", "-m", "3q"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_repeat_copies_never_overlap():
+    """The 3.1 Gbp workload's repeat families: copies of both sizes (6 kbp first, then 300 bp) take
+    disjoint slots of one occupancy map, and every copy's slot is marked."""
+    import numpy as np
+
+    from empanada_amd import gen
+    g = gen.genome(3_000_000, 1200)
+    used = np.zeros(g.size // 300, dtype=np.uint8)
+    f6 = gen.add_repeats(g, 1208, 0.05, 6000, 16, 0.01, used, 300)
+    after6 = used.copy()
+    f3 = gen.add_repeats(g, 1207, 0.05, 300, 16, 0.01, used, 300)
+    assert f6 >= 1 and f3 >= 1
+    assert after6.sum() == 20 * 16 * f6                       # whole 6 kbp slots, 20 cells each
+    assert used.sum() == after6.sum() + 16 * f3               # 300 bp copies only on free cells
+    assert (after6.reshape(-1, 20).sum(axis=1) % 20 == 0).all()  # 6 kbp copies on their own grid
