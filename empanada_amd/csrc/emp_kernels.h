@@ -258,6 +258,15 @@ hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
+// Copies of a round's records in host memory (device views of pinned buffers), written by the
+// round's edge scan so that no copy runs between joins: rec gets {edges, first position, overflow
+// cursor}; list gets list_words words of list_src (the next round's counter record). Null: none.
+struct RecMirror {
+    unsigned long long *rec;
+    uint32_t *list;
+    const uint32_t *list_src;
+    uint32_t list_words;
+};
 // One round's edges in output order: the n bucket slots (counts cnt, pool offsets src_off) scanned
 // into dst_off from *run (the output position after the earlier rounds; advanced by the round's
 // edges; rec = {edges, first position, *ovf_cursor as read after the round's join}), then copied
@@ -265,7 +274,7 @@ hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
                               uint32_t n, unsigned long long *run, unsigned long long *rec,
                               const unsigned long long *ovf_cursor, const uint2 *pool, uint32_t *out_from,
-                              uint32_t *out_to, hipStream_t s);
+                              uint32_t *out_to, const RecMirror &m, hipStream_t s);
 // n work-groups: all buckets (a.ids null; buckets over a.cap are skipped) or the listed ones.
 hipError_t launch_audit(const AuditArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_roll(const uint64_t *keys, const uint8_t *bases, uint64_t n, int kind, uint32_t m1, uint32_t m2,

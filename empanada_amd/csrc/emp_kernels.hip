@@ -1288,7 +1288,7 @@ __global__ void __launch_bounds__(1024) k_round_scan(const uint32_t *__restrict_
                                                     unsigned long long *__restrict__ dst,
                                                     unsigned long long *__restrict__ run,
                                                     unsigned long long *__restrict__ rec,
-                                                    const unsigned long long *ovf_cursor) {
+                                                    const unsigned long long *ovf_cursor, RecMirror m) {
     __shared__ unsigned long long sh[16];
     const unsigned long long base = *run;
     unsigned long long carry = base;
@@ -1311,11 +1311,18 @@ __global__ void __launch_bounds__(1024) k_round_scan(const uint32_t *__restrict_
         carry += tot;
     }
     if (threadIdx.x == 0) {
+        const unsigned long long o = __hip_atomic_load(ovf_cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         rec[0] = carry - base;
         rec[1] = base;
-        rec[2] = __hip_atomic_load(ovf_cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        rec[2] = o;
         *run = carry;
+        if (m.rec) {
+            m.rec[0] = carry - base;
+            m.rec[1] = base;
+            m.rec[2] = o;
+        }
     }
+    if (m.list && threadIdx.x < m.list_words) m.list[threadIdx.x] = m.list_src[threadIdx.x];
 }
 
 __global__ void __launch_bounds__(256) k_edge_copy(EdgeCopyArgs a) {
@@ -1477,9 +1484,9 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
 hipError_t launch_round_edges(const uint32_t *cnt, const unsigned long long *src_off, unsigned long long *dst_off,
                               uint32_t n, unsigned long long *run, unsigned long long *rec,
                               const unsigned long long *ovf_cursor, const uint2 *pool, uint32_t *out_from,
-                              uint32_t *out_to, hipStream_t s) {
+                              uint32_t *out_to, const RecMirror &m, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec, ovf_cursor);
+    hipLaunchKernelGGL(k_round_scan, dim3(1), dim3(1024), 0, s, cnt, n, dst_off, run, rec, ovf_cursor, m);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     EdgeCopyArgs a{cnt, src_off, dst_off, nullptr, n, pool, out_from, out_to};

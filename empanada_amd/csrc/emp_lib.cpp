@@ -231,13 +231,15 @@ struct emp_ctx {
     uint64_t btie_cap = 0;
     // pipelined rounds: each round's counter record read back (pinned) with an event per round
     bool pipeline = true;
-    ListCtr *h_rec = nullptr;
+    ListCtr *h_rec = nullptr, *d_rec_view = nullptr;  // pinned, and its device view
     std::vector<hipEvent_t> rec_ev;
     // their edges go straight to the pinned output arena (device view of h_from / h_to) on a
     // second stream, while later rounds run
     hipStream_t ostream = nullptr;
+    hipStream_t dstream = nullptr;  // the arena's DMA to the pinned host arena (apart from ostream's scans)
     bool out_streamed = false;
     unsigned long long (*h_orec)[3] = nullptr;  // per round {edges, first position, overflow cursor}, pinned
+    unsigned long long (*d_orec_view)[3] = nullptr;  // its device view
     std::vector<hipEvent_t> orec_ev;
 
     // emp_run_to_file: records written while the rounds run (a writer thread), and how many of
@@ -2189,10 +2191,13 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     // nothing is pending in it at the start of a run.
     if ((rc = ensure_arena(c, c->pool_cap * SUBS + c->pool_ovf_cap, nullptr, nullptr))) return rc;
     if (!c->ostream) HIPCHK(c, hipStreamCreateWithFlags(&c->ostream, hipStreamNonBlocking));
+    if (!c->dstream) HIPCHK(c, hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking));
     if (!c->h_orec) {
-        void *p = nullptr;
-        HIPCHK(c, hipHostMalloc(&p, MAX_ROUNDS * 3 * sizeof(unsigned long long)));
+        void *p = nullptr, *dp = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, MAX_ROUNDS * 3 * sizeof(unsigned long long), hipHostMallocMapped));
         c->h_orec = (unsigned long long(*)[3])p;
+        HIPCHK(c, hipHostGetDevicePointer(&dp, p, 0));
+        c->d_orec_view = (unsigned long long(*)[3])dp;
     }
     if (c->orec_ev.empty()) {
         c->orec_ev.resize(MAX_ROUNDS);
@@ -2220,7 +2225,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
             emitting = false;  // not final here: this round and the ones after it wait for the end
             return EMP_OK;
         }
-        HIPCHK(c, hipEventRecord(c->dma_ev[r], c->ostream));
+        HIPCHK(c, hipEventRecord(c->dma_ev[r], c->dstream));
         writer->push(o[0], o[1], L - 1 - r, c->dma_ev[r]);
         ++c->emitted;
         return EMP_OK;
@@ -2230,8 +2235,10 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         upto = std::min<uint64_t>(upto, c->out_cap);  // placed edges never exceed the arena (the pool's size)
         if (upto <= copied) return EMP_OK;
         const size_t bytes = (size_t)(upto - copied) * 4;
-        HIPCHK(c, hipMemcpyAsync(c->h_from + copied, c->d_out_from + copied, bytes, hipMemcpyDeviceToHost, c->ostream));
-        HIPCHK(c, hipMemcpyAsync(c->h_to + copied, c->d_out_to + copied, bytes, hipMemcpyDeviceToHost, c->ostream));
+        // on a stream of its own: a copy of several MB queued on ostream would hold the scans of
+        // later rounds, whose events the host waits on before it launches the next joins
+        HIPCHK(c, hipMemcpyAsync(c->h_from + copied, c->d_out_from + copied, bytes, hipMemcpyDeviceToHost, c->dstream));
+        HIPCHK(c, hipMemcpyAsync(c->h_to + copied, c->d_out_to + copied, bytes, hipMemcpyDeviceToHost, c->dstream));
         copied = upto;
         return EMP_OK;
     };
@@ -2247,13 +2254,16 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     }
     auto fallback = [&]() {
         (void)hipStreamSynchronize(c->ostream);
+        (void)hipStreamSynchronize(c->dstream);
         if (writer && writer->finish()) return fail(c, EMP_ERR_IO, "writing the output file failed");
         return RUN_FALLBACK;
     };
     if (!c->h_rec) {
-        void *p = nullptr;
-        HIPCHK(c, hipHostMalloc(&p, (MAX_ROUNDS + 1) * sizeof(ListCtr)));
+        void *p = nullptr, *dp = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, (MAX_ROUNDS + 1) * sizeof(ListCtr), hipHostMallocMapped));
         c->h_rec = (ListCtr *)p;
+        HIPCHK(c, hipHostGetDevicePointer(&dp, p, 0));
+        c->d_rec_view = (ListCtr *)dp;
     }
     if (c->rec_ev.empty()) {
         c->rec_ev.resize(MAX_ROUNDS);
@@ -2264,19 +2274,18 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     uint64_t Sk = c->h_rec[0].total[0], Pk = c->h_rec[0].total[1];  // newest sizes known
     int launched = 0;
     for (int t = 0; t < rounds; ++t) {
-        // the record of round t-1 (written by join t-2) is complete: check it, size round t+1
-        if (t >= 2) {
-            HIPCHK(c, hipEventSynchronize(c->rec_ev[t - 2]));
-            const ListCtr &r = c->h_rec[t - 1];
+        // Round t-3's edge scan has ended: the record of round t-2 (written by join t-3) is in
+        // host memory, and so are round t-3's edges in the device arena. Check that record and
+        // size round t+1 from it (lists only shrink). The scan of a round runs beside the next
+        // join and often ends with it (the joins fill every CU), so the host looks three rounds
+        // back: it then never holds the next join's launch.
+        if (t >= 3) {
+            HIPCHK(c, hipEventSynchronize(c->orec_ev[t - 3]));
+            const ListCtr &r = c->h_rec[t - 2];
             if (r.n_big || r.nx_ovf) return fallback();
             Sk = r.total[0];
             Pk = r.total[1];
-            if (Sk == 0 || Pk == 0) break;  // round t-1 writes no record, nor any later round
-        }
-        // round t-3's edges are in the device arena: its compaction started when join t-3 ended,
-        // two joins ago (join t-2 has ended too: its record is in)
-        if (t >= 3) {
-            HIPCHK(c, hipEventSynchronize(c->orec_ev[t - 3]));
+            if (Sk == 0 || Pk == 0) break;  // round t-2 writes no record, nor any later round
             if ((rc = send(c->h_orec[t - 3][1] + c->h_orec[t - 3][0]))) return rc;
             if ((rc = emit_round(t - 3))) return rc;
         }
@@ -2308,16 +2317,16 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
             Timed tm(c, "k_join", 0.0);
             HIPCHK(c, launch_join(a, 1u << bb, c->stream));
         }
-        HIPCHK(c, hipMemcpyAsync(&c->h_rec[t + 1], &c->d_ctr->r[t + 1], sizeof(ListCtr), hipMemcpyDeviceToHost,
-                                 c->stream));
         HIPCHK(c, hipEventRecord(c->rec_ev[t], c->stream));
-        // the round's edges, in output order, into the device arena while later rounds run
+        // the round's edges, in output order, into the device arena while later rounds run; the
+        // scan also writes the round's output record and the next round's counter record into
+        // pinned host memory, so the joins run back to back with no copy between them
         HIPCHK(c, hipStreamWaitEvent(c->ostream, c->rec_ev[t], 0));
+        const RecMirror mirror{c->d_orec_view[t], reinterpret_cast<uint32_t *>(&c->d_rec_view[t + 1]),
+                               reinterpret_cast<const uint32_t *>(&c->d_ctr->r[t + 1]), sizeof(ListCtr) / 4};
         HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, 1u << bb,
                                      &c->d_ctr->obase, c->d_ctr->orec[t], &c->d_ctr->pool_ovf, c->d_pool,
-                                     c->d_out_from, c->d_out_to, c->ostream));
-        HIPCHK(c, hipMemcpyAsync(c->h_orec[t], c->d_ctr->orec[t], 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 c->ostream));
+                                     c->d_out_from, c->d_out_to, mirror, c->ostream));
         HIPCHK(c, hipEventRecord(c->orec_ev[t], c->ostream));
         launched = t + 1;
     }
@@ -2326,7 +2335,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
     HIPCHK(c, hipStreamSynchronize(c->ostream));
     if ((rc = read_counters(c))) return rc;
     if (launched && (rc = send(c->h_ctr->obase))) return rc;
-    HIPCHK(c, hipStreamSynchronize(c->ostream));
+    HIPCHK(c, hipStreamSynchronize(c->dstream));
     const DevCounters &k = *c->h_ctr;
     if (k.pool_ovf > c->pool_ovf_cap) return fallback();
     int t = 0;
@@ -2529,6 +2538,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ostream) (void)hipStreamSynchronize(c->ostream);
+    if (c->dstream) (void)hipStreamSynchronize(c->dstream);
     if (c->cstream) (void)hipStreamSynchronize(c->cstream);
     for (auto &a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
@@ -2559,6 +2569,7 @@ void emp_ctx_destroy(emp_ctx *c) {
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     if (c->ostream) (void)hipStreamDestroy(c->ostream);
+    if (c->dstream) (void)hipStreamDestroy(c->dstream);
     if (c->cstream) (void)hipStreamDestroy(c->cstream);
     delete c;
 }
