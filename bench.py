@@ -178,59 +178,108 @@ def traffic_of(kernel, workload):
     return t["traffic_bytes_per_launch"] if t and t.get("workload") == workload else None
 
 
-def batched_bench(sets, plain_stride, hbm_budget):
-    """The sets of the batched workload through one batched engine (admission by HBM budget), and
-    the plain single-job path run once per set on every plain_stride-th set, on the same GPU."""
-    from empanada_amd import native
+def batched_bench(sets, plain_stride, hbm_budget, tuning_cache):
+    """The sets of the batched workload through one batched engine (admission by HBM budget),
+    against the plain single-job path run once per set on every plain_stride-th set, on the same
+    GPU, with the reads (a) resident in HBM and (b) in host memory (copied to HBM inside the timed
+    region). With resident reads the engine also runs untuned (its built-in bucket loads) and with
+    every kernel launched one by one instead of replaying each round step's captured HIP graph; the
+    tuning is the autotuner's choice, from its JSON cache or measured now on calibration sets of the
+    same distribution that the run does not measure."""
+    from empanada_amd import hipmem, native, tune
     n_sets = len(sets)
     reads = sum(r.shape[0] for r, _ in sets)
     bases = sum(r.size for r, _ in sets)
-    with native.Batch(collect=False, hbm_budget=hbm_budget) as b:
-        for r, o in sets[:8]:  # warm-up: allocations, code objects
-            b.submit(r, o)
-        b.run()
-        b.reset()
-        t = time.perf_counter()
-        for r, o in sets:
-            b.submit(r, o)
-        b.run()
-        t_b = time.perf_counter() - t
-        edges_b = [b.info(j)["n_edges"] for j in range(n_sets)]
+    t = time.perf_counter()
+    tuning, reused = tune.load_or_tune(tuning_cache, tune.calibration_sample, budget=hbm_budget)
+    t_tune = time.perf_counter() - t
+    t = time.perf_counter()
+    dev = hipmem.DeviceJobs(sets)
+    t_up = time.perf_counter() - t
+    djobs = dev.jobs()
+
+    def run_batch(tuning, graph, device):
+        with native.Batch(collect=False, hbm_budget=hbm_budget, tuning=tuning, graph=graph) as b:
+            def submit_all(k):
+                for i in range(k):
+                    if device:
+                        b.submit_device(*djobs[i])
+                    else:
+                        b.submit(*sets[i])
+            submit_all(n_sets)  # warm-up pass: buffers sized for the whole workload, code objects
+            b.run()
+            b.reset()
+            t = time.perf_counter()
+            submit_all(n_sets)
+            b.run()
+            dt = time.perf_counter() - t
+            return dt, [b.info(j)["n_edges"] for j in range(n_sets)], b.graph_stats()
+
     sub = list(range(0, n_sets, plain_stride))
-    with native.Context(min_ovlp=sets[0][1]) as c:
-        c.add_reads(sets[0][0])
-        c.run(collect=False)
-        edges_p = []
-        t = time.perf_counter()
-        for i in sub:
-            r, o = sets[i]
-            c.reset()
-            c.add_reads(r)
+
+    big = max(sub, key=lambda i: sets[i][0].size)
+
+    def run_plain(device):
+        with native.Context(min_ovlp=sets[0][1]) as c:
+            c.add_reads(sets[big][0])  # warm-up: buffers sized for the largest set, code objects
             c.run(collect=False)
-            edges_p.append(c.summary()["n_edges"])
-        t_p = time.perf_counter() - t
+            edges = []
+            t = time.perf_counter()
+            for i in sub:
+                c.reset()
+                if device:
+                    addr, n, L, _ = djobs[i]
+                    c.add_reads_device(addr, n, L)
+                else:
+                    c.add_reads(sets[i][0])
+                c.run(collect=False)
+                edges.append(c.summary()["n_edges"])
+            return time.perf_counter() - t, edges
+
+    t_b, edges_b, (captures, replays) = run_batch(tuning, True, True)
+    t_u, edges_u, _ = run_batch(None, True, True)
+    t_n, edges_n, _ = run_batch(None, False, True)
+    t_p, edges_p = run_plain(True)
+    dev.free()
+    t_bh, edges_bh, _ = run_batch(tuning, True, False)
+    t_ph, edges_ph = run_plain(False)
     jb, jp = n_sets / t_b, len(sub) / t_p
     return {
         "workload": "%d sets: genome log-uniform 5 kbp-5 Mbp, L in {100,150,250}, coverage 10-50x, 50%% RC, "
                     "0.5%% substitutions, -o 31 (seeds 1300+i, 1300+i+4096)" % n_sets,
         "reads": reads, "bases": bases, "hbm_budget": hbm_budget,
+        "reads_in": "HBM (uploaded before the timed region, %.1f s); host_reads: host memory, copied to HBM "
+                    "inside the timed region by both paths" % t_up,
         "batched_s": round(t_b, 3), "batched_jobs_per_s": round(jb, 2),
         "batched_gbases_per_s": round(bases / t_b / 1e9, 3),
         "plain_sets": len(sub), "plain_stride": plain_stride, "plain_s": round(t_p, 3),
         "plain_jobs_per_s": round(jp, 2),
         "speedup": round(jb / jp, 2),
-        "edges_equal_on_plain_sets": [edges_b[i] for i in sub] == edges_p,
-        "reads_in": "host memory (both paths copy them to HBM inside the timed region)",
+        "untuned_jobs_per_s": round(n_sets / t_u, 2),
+        "tuned_vs_untuned": round(t_u / t_b, 3),
+        "untuned_no_graph_jobs_per_s": round(n_sets / t_n, 2),
+        "graph_vs_no_graph": round(t_n / t_u, 3),
+        "graph_captures": captures, "graph_replays": replays,
+        "host_reads": {"batched_jobs_per_s": round(n_sets / t_bh, 2), "plain_jobs_per_s": round(len(sub) / t_ph, 2),
+                       "speedup": round((n_sets / t_bh) / (len(sub) / t_ph), 2),
+                       "batched_gbases_per_s": round(bases / t_bh / 1e9, 3)},
+        "tuning": {"cache": os.path.relpath(tuning_cache, ROOT), "reused_from_cache": reused,
+                   "seconds": round(t_tune, 1), "bucket_cap": tuning["bucket_cap"],
+                   "classes": tuning["classes"],
+                   "calibration": "%d sets of the same distribution not measured here (i = 4096..), reads in HBM" %
+                                  tuning.get("sample_jobs", 0)},
+        "edges_equal_tuned_untuned_no_graph_host": edges_b == edges_u == edges_n == edges_bh,
+        "edges_equal_on_plain_sets": [edges_b[i] for i in sub] == edges_p == edges_ph,
         "note": "plain path on every %d-th set (same distribution), batched on all of them" % plain_stride,
     }
 
 
-def served_bench(jobs, loads, bound_ms):
+def served_bench(jobs, loads, bound_ms, tuning):
     """Poisson arrivals of batched-workload sets at each offered load (jobs/s)."""
     from empanada_amd import serve
     n_jobs = len(jobs)
     out = []
-    with serve.Server() as srv:
+    with serve.Server(tuning=tuning) as srv:
         srv.run(jobs[:4], [0.0] * 4)  # warm-up
         for k, rate in enumerate(loads):
             srv.engine.reset()
@@ -267,6 +316,8 @@ def main():
     ap.add_argument("--plain-stride", type=int, default=8,
                     help="the plain path's comparison runs every n-th set of the batched workload")
     ap.add_argument("--batched-budget", type=int, default=160, help="the batched engine's HBM budget, GiB")
+    ap.add_argument("--tuning-cache", default=os.path.join(ROOT, "empanada_amd", "tuning_gfx950.json"),
+                    help="the batched engine's autotuner cache (JSON; tuned on the GPU when absent or stale)")
     ap.add_argument("--served-jobs", type=int, default=1000, help="jobs per offered load when serving (0: skip)")
     ap.add_argument("--served-loads", default="1,2,4,8",
                     help="offered loads as multiples of the plain path's jobs/s")
@@ -469,15 +520,19 @@ def main():
         sets = [gen.batched_set(i) for i in range(max(args.batched_sets, args.served_jobs))]
         t_sets = time.perf_counter() - t
         plain_rate = None
+        tuning = None
         if args.batched_sets > 0:
-            batched = batched_bench(sets[:args.batched_sets], args.plain_stride, args.batched_budget << 30)
+            batched = batched_bench(sets[:args.batched_sets], args.plain_stride, args.batched_budget << 30,
+                                    args.tuning_cache)
             batched["gen_s"] = round(t_sets, 2)
             plain_rate = batched["plain_jobs_per_s"]
+            tuning = batched["tuning"]
         if args.served_jobs > 0:
             if plain_rate is None:
-                plain_rate = batched_bench(sets[:256], 4, args.batched_budget << 30)["plain_jobs_per_s"]
+                plain_rate = batched_bench(sets[:256], 4, args.batched_budget << 30,
+                                           args.tuning_cache)["plain_jobs_per_s"]
             loads = [float(x) * plain_rate for x in args.served_loads.split(",")]
-            served = served_bench(sets[:args.served_jobs], loads, args.p99_bound_ms)
+            served = served_bench(sets[:args.served_jobs], loads, args.p99_bound_ms, tuning)
         del sets
 
     line = {

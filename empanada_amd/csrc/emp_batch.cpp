@@ -20,6 +20,7 @@
 #include "../../include/empanada_batch.h"
 #include "emp_batch.h"
 #include "emp_fastq.h"
+#include "emp_hostcopy.h"
 
 using namespace emp;
 
@@ -117,6 +118,17 @@ struct HJob {
     int t = 0, rounds = 0;
     uint64_t charge = 0, n_dedup = 0, n_edges = 0;
     double submit_ms = 0, admit_ms = 0, done_ms = 0;
+    uint32_t fine_load = BTARGET, coarse_load = BCOARSE;  // from the job's tuning class
+};
+
+// The launches of a round step captured in a graph, replayed while the step fits: grids (tiles
+// of each pass and side, fine buckets) and reset ranges are upper bounds; key holds every buffer
+// address and capacity the captured arguments hold.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key;
+    uint32_t g1[2] = {0, 0}, g2[2] = {0, 0}, gf = 0;
+    uint64_t gc = 0;
 };
 
 struct BCounters {
@@ -174,7 +186,7 @@ struct emp_batch {
     // step tables (pinned host mirrors)
     BJob *d_jobs = nullptr, *h_jobs = nullptr;
     uint32_t *d_pre = nullptr, *h_pre = nullptr;  // t1[2], t2[2], fine prefix: 5 x (cap + 1)
-    uint32_t *d_lcnt = nullptr, *h_lcnt = nullptr;  // per step of n jobs: [2][n] survivors, [n] edges
+    uint32_t *d_lcnt = nullptr, *h_lcnt = nullptr;  // per job, at a stride of tab_cap: [2] survivors, edges
     uint64_t tab_cap = 0;
 
     // step scratch
@@ -205,6 +217,13 @@ struct emp_batch {
     std::vector<KStat> kstats;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
+
+    // tuning (emp_batch_set_tuning): the join/dedup variant and the per-class bucket loads
+    uint32_t fcap = BCAP;
+    std::vector<emp_batch_class> classes;
+    // the round step's graph
+    StepGraph graph;
+    uint64_t graph_captures = 0, graph_replays = 0;
 };
 
 namespace {
@@ -405,15 +424,22 @@ struct StepLayout {
     uint64_t live = 0;
 };
 
-int plan_step(emp_batch *b, uint32_t nj, bool dedup, StepLayout &sl) {
+int plan_step(emp_batch *b, uint32_t nj, bool dedup, const std::vector<int> &step_ids, StepLayout &sl) {
     uint32_t *t1 = b->h_pre, *t2 = b->h_pre + 2 * (b->tab_cap + 1), *fp = b->h_pre + 4 * (b->tab_cap + 1);
     const uint64_t stride = b->tab_cap + 1;
     sl = StepLayout{};
     for (uint32_t j = 0; j < nj; ++j) {
         BJob &J = b->h_jobs[j];
         const uint64_t x = std::max(J.cnt[0], J.cnt[1]);
-        const uint32_t nb0 = (uint32_t)std::max<uint64_t>(1, ceil_div(x, BTARGET));
-        J.nc = (uint32_t)std::min<uint64_t>(BSPLIT, std::max<uint64_t>(1, ceil_div(x, BCOARSE)));
+        // the job's class loads; the fine load at least x / BSPLIT^2 (at most BSPLIT fine buckets
+        // per coarse one)
+        const HJob &h = b->jobs[(size_t)step_ids[j]];
+        const uint64_t fl = std::max<uint64_t>(h.fine_load, ceil_div(x, (uint64_t)BSPLIT * BSPLIT));
+        const uint32_t nb0 = (uint32_t)std::max<uint64_t>(1, ceil_div(x, fl));
+        // coarse buckets from the class's load, at least enough that each fans out to at most
+        // BSPLIT fine buckets
+        J.nc = (uint32_t)std::min<uint64_t>(
+            BSPLIT, std::max<uint64_t>(std::max<uint64_t>(1, ceil_div(x, h.coarse_load)), ceil_div(nb0, BSPLIT)));
         J.m = (uint32_t)ceil_div(nb0, J.nc);
         if (J.m > BSPLIT) return fail(b, EMP_ERR_CAPACITY, "a job list of %llu entries exceeds the batch limit",
                                       (unsigned long long)x);
@@ -441,12 +467,14 @@ int plan_step(emp_batch *b, uint32_t nj, bool dedup, StepLayout &sl) {
         sl.ncoarse += J.nc;
         sl.nfine += J.nb;
     }
-    for (int s = 0; s < 2; ++s) {
-        t1[s * stride + nj] = sl.tiles1[s];
-        t2[s * stride + nj] = sl.tiles2[s];
+    for (uint64_t j = nj; j <= b->tab_cap; ++j) {  // the totals from the last job to the capacity
+        for (int s = 0; s < 2; ++s) {
+            t1[s * stride + j] = sl.tiles1[s];
+            t2[s * stride + j] = sl.tiles2[s];
+        }
+        fp[j] = (uint32_t)sl.nfine;
     }
-    fp[nj] = (uint32_t)sl.nfine;
-    if (sl.nfine >= (1ull << 32) / BCAP) return fail(b, EMP_ERR_CAPACITY, "too many fine buckets in one step");
+    if (sl.nfine >= (1ull << 32) / b->fcap) return fail(b, EMP_ERR_CAPACITY, "too many fine buckets in one step");
     return EMP_OK;
 }
 
@@ -478,7 +506,7 @@ int ensure_scratch(emp_batch *b, const StepLayout &sl) {
         BCHK(b, hipStreamSynchronize(b->stream));
         const uint64_t cap = std::max<uint64_t>(sl.nfine + sl.nfine / 4, 4096);
         for (int s = 0; s < 2; ++s) {
-            if ((rc = realloc_dev(b, &b->d_fine[s], cap * BCAP))) return rc;
+            if ((rc = realloc_dev(b, &b->d_fine[s], cap * b->fcap))) return rc;
             if ((rc = realloc_dev(b, &b->d_fcnt[s], cap))) return rc;
         }
         if ((rc = realloc_dev(b, &b->d_edge_off, cap))) return rc;
@@ -549,13 +577,15 @@ void after_abort(emp_batch *b) {
 
 int read_counters(emp_batch *b, uint32_t nj) {
     BCHK(b, hipMemcpyAsync(b->h_ctr, b->d_ctr, sizeof(BCounters), hipMemcpyDeviceToHost, b->stream));
-    BCHK(b, hipMemcpyAsync(b->h_lcnt, b->d_lcnt, 3ull * nj * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+    (void)nj;
+    BCHK(b, hipMemcpyAsync(b->h_lcnt, b->d_lcnt, 3ull * b->tab_cap * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           b->stream));
     return sync(b);
 }
 
 // Counter resets of a step: coarse and fine counts (with `part`), survivor and edge counts,
 // flags and pool cursors.
-int zero_step(emp_batch *b, const StepLayout &sl, bool part, bool dedup) {
+int zero_step(emp_batch *b, uint64_t ncoarse, uint64_t nfine, bool part, bool dedup) {
     ZeroArgs z{};
     auto add = [&](uint32_t *p, uint64_t n) {
         z.p[z.count] = p;
@@ -565,8 +595,8 @@ int zero_step(emp_batch *b, const StepLayout &sl, bool part, bool dedup) {
     };
     if (part) {
         for (int s = 0; s < (dedup ? 1 : 2); ++s) {
-            add(b->d_ccnt[s], sl.ncoarse);
-            add(b->d_fcnt[s], sl.nfine);
+            add(b->d_ccnt[s], ncoarse);
+            add(b->d_fcnt[s], nfine);
         }
         add(&b->d_ctr->abort, 1);
         add(b->d_ctr->spill, 2);
@@ -580,11 +610,11 @@ int zero_step(emp_batch *b, const StepLayout &sl, bool part, bool dedup) {
 }
 
 // One partition (two passes) of side s of the step.
-int partition_side(emp_batch *b, uint32_t nj, int s, bool regen, const StepLayout &sl, double bytes) {
+int partition_side(emp_batch *b, int s, bool regen, uint32_t groups1, uint32_t groups2, double bytes) {
     const uint64_t stride = b->tab_cap + 1;
     BPartArgs a{};
     a.jobs = b->d_jobs;
-    a.njobs = nj;
+    a.njobs = (uint32_t)b->tab_cap;
     a.side = s;
     a.lists[0] = b->d_list[0];
     a.lists[1] = b->d_list[1];
@@ -597,15 +627,16 @@ int partition_side(emp_batch *b, uint32_t nj, int s, bool regen, const StepLayou
     a.spill_n = &b->d_ctr->spill[s];
     a.spill_cap = b->spill_cap;
     a.abort = &b->d_ctr->abort;
+    a.fcap = b->fcap;
     a.tpre = b->d_pre + s * stride;
     {
         Timed t(b, "k_bpart1", bytes);
-        BCHK(b, launch_bpart(a, 1, regen, sl.tiles1[s], b->stream));
+        BCHK(b, launch_bpart(a, 1, regen, groups1, b->stream));
     }
     a.tpre = b->d_pre + (2 + s) * stride;
     {
         Timed t(b, "k_bpart2", bytes);
-        BCHK(b, launch_bpart(a, 2, false, sl.tiles2[s], b->stream));
+        BCHK(b, launch_bpart(a, 2, false, groups2, b->stream));
     }
     return EMP_OK;
 }
@@ -613,7 +644,7 @@ int partition_side(emp_batch *b, uint32_t nj, int s, bool regen, const StepLayou
 BJoinArgs join_args(emp_batch *b, uint32_t nj) {
     BJoinArgs a{};
     a.jobs = b->d_jobs;
-    a.njobs = nj;
+    a.njobs = (uint32_t)b->tab_cap;
     a.fpre = b->d_pre + 4 * (b->tab_cap + 1);
     a.s_in = b->d_fine[0];
     a.p_in = b->d_fine[1];
@@ -622,7 +653,7 @@ BJoinArgs join_args(emp_batch *b, uint32_t nj) {
     a.lists[0] = b->d_list[0];
     a.lists[1] = b->d_list[1];
     a.lcnt = b->d_lcnt;
-    a.ejob = b->d_lcnt + 2 * nj;
+    a.ejob = b->d_lcnt + 2 * b->tab_cap;
     a.pool = b->d_pool;
     a.pool_cursor = b->d_ctr->pool;
     a.pool_cap = b->pool_cap;
@@ -663,7 +694,7 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
     BCHK(b, hipMemcpyAsync(b->d_boff[1], po.data(), nbig * sizeof(unsigned long long), hipMemcpyHostToDevice, b->stream));
     BBigArgs a{};
     a.jobs = b->d_jobs;
-    a.njobs = nj;
+    a.njobs = (uint32_t)b->tab_cap;
     a.fpre = b->d_pre + 4 * (b->tab_cap + 1);
     a.ids = b->d_big;
     a.soff = b->d_boff[0];
@@ -687,7 +718,7 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
     a.lists[0] = b->d_list[0];
     a.lists[1] = b->d_list[1];
     a.lcnt = b->d_lcnt;
-    a.ejob = b->d_lcnt + 2 * nj;
+    a.ejob = b->d_lcnt + 2 * b->tab_cap;
     a.pool = b->d_pool;
     a.pool_cursor = b->d_ctr->pool;
     a.pool_cap = b->pool_cap;
@@ -696,6 +727,7 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
     a.edge_off = b->d_edge_off;
     a.edge_cnt = b->d_edge_cnt;
     a.too_many = &b->d_ctr->too_many;
+    a.fcap = b->fcap;
     {
         Timed t(b, dedup ? "k_bdedup_big" : "k_bjoin_big", 0.0);
         BCHK(b, dedup ? launch_bdedup_big(a, nbig, b->stream) : launch_bjoin_big(a, nbig, b->stream));
@@ -737,7 +769,7 @@ int map_job(emp_batch *b, HJob &h, uint32_t ji) {
             BCHK(b, hipEventSynchronize(b->stage_ev[cur]));
             b->stage_busy[cur] = false;
         }
-        memcpy(b->h_stage[cur], h.reads + done * (uint64_t)h.L, m * (uint64_t)h.L);
+        parallel_copy(b->h_stage[cur], h.reads + done * (uint64_t)h.L, m * (uint64_t)h.L);
         BCHK(b, hipMemcpyAsync(b->d_stage[cur], b->h_stage[cur], m * (uint64_t)h.L, hipMemcpyHostToDevice, b->stream));
         a.reads = b->d_stage[cur];
         a.n = (uint32_t)m;
@@ -837,18 +869,18 @@ int admit(emp_batch *b, emp_job_done done, void *user) {
     StepLayout sl;
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(b, EMP_ERR_CAPACITY, "dedup does not settle");
-        if ((rc = plan_step(b, nj, true, sl))) return rc;
+        if ((rc = plan_step(b, nj, true, nw, sl))) return rc;
         if ((rc = ensure_scratch(b, sl))) return rc;
         if ((rc = upload_tables(b, nj))) return rc;
-        if ((rc = zero_step(b, sl, true, true))) return rc;
-        if ((rc = partition_side(b, nj, 0, false, sl, 24.0 * sl.live))) return rc;
+        if ((rc = zero_step(b, sl.ncoarse, sl.nfine, true, true))) return rc;
+        if ((rc = partition_side(b, 0, false, sl.tiles1[0], sl.tiles2[0], 24.0 * sl.live))) return rc;
         BJoinArgs a = join_args(b, nj);
         a.p_in = nullptr;
         a.p_cnt = nullptr;
         a.edge_cnt = nullptr;
         {
             Timed t(b, "k_bdedup", 12.0 * sl.live);
-            BCHK(b, launch_bdedup(a, (uint32_t)sl.nfine, b->stream));
+            BCHK(b, launch_bdedup(a, (uint32_t)sl.nfine, b->fcap, b->stream));
         }
         if ((rc = read_counters(b, nj))) return rc;
         if (b->h_ctr->abort) {
@@ -877,6 +909,71 @@ int admit(emp_batch *b, emp_job_done done, void *user) {
             b->active.push_back(id);
         }
     }
+    return EMP_OK;
+}
+
+std::vector<uint64_t> graph_key(const emp_batch *b) {
+    return {(uint64_t)(uintptr_t)b->d_jobs,     (uint64_t)(uintptr_t)b->d_pre,      (uint64_t)(uintptr_t)b->d_lcnt,
+            b->tab_cap,                         (uint64_t)(uintptr_t)b->d_list[0],  (uint64_t)(uintptr_t)b->d_list[1],
+            (uint64_t)(uintptr_t)b->d_cols,     (uint64_t)(uintptr_t)b->d_coarse[0], (uint64_t)(uintptr_t)b->d_coarse[1],
+            (uint64_t)(uintptr_t)b->d_ccnt[0],  (uint64_t)(uintptr_t)b->d_ccnt[1],  b->ccnt_cap,
+            (uint64_t)(uintptr_t)b->d_fine[0],  (uint64_t)(uintptr_t)b->d_fine[1],  (uint64_t)(uintptr_t)b->d_fcnt[0],
+            (uint64_t)(uintptr_t)b->d_fcnt[1],  b->fine_cap,                        (uint64_t)(uintptr_t)b->d_spill[0],
+            (uint64_t)(uintptr_t)b->d_spill[1], b->spill_cap,                       (uint64_t)(uintptr_t)b->d_pool,
+            b->pool_cap,                        b->pool_ovf_cap,                    (uint64_t)(uintptr_t)b->d_edge_off,
+            (uint64_t)(uintptr_t)b->d_edge_cnt, (uint64_t)(uintptr_t)b->d_big,      (uint64_t)(uintptr_t)b->d_ctr,
+            b->fcap};
+}
+
+// The launches of a round step (counter resets, the two partition passes of both sides, the join)
+// as one graph. The graph is captured with grids a quarter above the step's needs and replayed
+// for later steps while they fit (lists only shrink; admitted jobs may need more) and no buffer
+// it names has moved; far smaller steps capture anew, so that few empty work-groups launch.
+int graph_step(emp_batch *b, uint32_t nj, const StepLayout &sl) {
+    StepGraph &g = b->graph;
+    const std::vector<uint64_t> key = graph_key(b);
+    auto fits = [&](uint64_t need, uint64_t have) { return need <= have && have <= 2 * need + 1024; };
+    const bool ok = g.exec && g.key == key && fits(sl.tiles1[0], g.g1[0]) && fits(sl.tiles1[1], g.g1[1]) &&
+                    fits(sl.tiles2[0], g.g2[0]) && fits(sl.tiles2[1], g.g2[1]) && fits(sl.nfine, g.gf) &&
+                    sl.ncoarse <= g.gc;
+    if (!ok) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        auto grow = [](uint64_t x) { return (uint32_t)std::min<uint64_t>(0x7FFFFFFF, x + x / 4 + 64); };
+        for (int s = 0; s < 2; ++s) {
+            g.g1[s] = grow(sl.tiles1[s]);
+            g.g2[s] = grow(sl.tiles2[s]);
+        }
+        g.gf = (uint32_t)std::min<uint64_t>(grow(sl.nfine), b->fine_cap);
+        g.gc = std::min<uint64_t>(grow(sl.ncoarse), b->ccnt_cap);
+        g.key = key;
+        hipGraph_t graph = nullptr;
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = zero_step(b, g.gc, g.gf, true, false);
+        if (!rc) rc = partition_side(b, 0, true, g.g1[0], g.g2[0], 0.0);
+        if (!rc) rc = partition_side(b, 1, true, g.g1[1], g.g2[1], 0.0);
+        hipError_t e = hipSuccess;
+        if (!rc) e = launch_bjoin(join_args(b, nj), g.gf, b->fcap, b->stream);
+        const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(b, EMP_ERR_HIP, "capturing the round step failed: %s",
+                        hipGetErrorString(e != hipSuccess ? e : ee));
+        }
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.exec = nullptr;
+            return fail(b, EMP_ERR_HIP, "instantiating the round step failed: %s", hipGetErrorString(e));
+        }
+        ++b->graph_captures;
+    }
+    BCHK(b, hipGraphLaunch(g.exec, b->stream));
+    ++b->graph_replays;
     return EMP_OK;
 }
 
@@ -910,26 +1007,28 @@ int round_step(emp_batch *b, emp_job_sink sink, emp_job_done done, void *user) {
     for (int attempt = 0;; ++attempt) {
         if (attempt > 8) return fail(b, EMP_ERR_CAPACITY, "step does not settle");
         if (need_part) {
-            if ((rc = plan_step(b, nj, false, sl))) return rc;
+            if ((rc = plan_step(b, nj, false, b->active, sl))) return rc;
             if ((rc = ensure_scratch(b, sl))) return rc;
             if ((rc = ensure_pool(b, std::max<uint64_t>(4096, sl.live / SUBS / 2),
                                   std::max<uint64_t>(1u << 20, sl.live / 8))))
                 return rc;
             if ((rc = upload_tables(b, nj))) return rc;
         }
-        if ((rc = zero_step(b, sl, need_part, false))) return rc;
-        if (need_part) {
-            uint64_t live[2] = {0, 0};
-            for (uint32_t i = 0; i < nj; ++i) {
-                live[0] += b->h_jobs[i].cnt[0];
-                live[1] += b->h_jobs[i].cnt[1];
+        if (need_part && !b->opts.profile && !b->opts.no_graph) {
+            if ((rc = graph_step(b, nj, sl))) return rc;
+        } else {
+            if ((rc = zero_step(b, sl.ncoarse, sl.nfine, need_part, false))) return rc;
+            if (need_part) {
+                uint64_t live[2] = {0, 0};
+                for (uint32_t i = 0; i < nj; ++i) {
+                    live[0] += b->h_jobs[i].cnt[0];
+                    live[1] += b->h_jobs[i].cnt[1];
+                }
+                if ((rc = partition_side(b, 0, true, sl.tiles1[0], sl.tiles2[0], 24.0 * live[0]))) return rc;
+                if ((rc = partition_side(b, 1, true, sl.tiles1[1], sl.tiles2[1], 24.0 * live[1]))) return rc;
             }
-            if ((rc = partition_side(b, nj, 0, true, sl, 24.0 * live[0]))) return rc;
-            if ((rc = partition_side(b, nj, 1, true, sl, 24.0 * live[1]))) return rc;
-        }
-        {
             Timed t(b, "k_bjoin", 12.0 * sl.live);
-            BCHK(b, launch_bjoin(join_args(b, nj), (uint32_t)sl.nfine, b->stream));
+            BCHK(b, launch_bjoin(join_args(b, nj), (uint32_t)sl.nfine, b->fcap, b->stream));
         }
         if ((rc = read_counters(b, nj))) return rc;
         if (b->h_ctr->abort) {
@@ -947,7 +1046,7 @@ int round_step(emp_batch *b, emp_job_sink sink, emp_job_done done, void *user) {
         }
         break;
     }
-    const uint32_t *ejob = b->h_lcnt + 2 * nj;
+    const uint32_t *ejob = b->h_lcnt + 2 * b->tab_cap;
     uint64_t total = 0;
     for (uint32_t i = 0; i < nj; ++i) total += ejob[i];
     if ((rc = ensure_out(b, std::max<uint64_t>(total, 1)))) return rc;
@@ -977,7 +1076,7 @@ int round_step(emp_batch *b, emp_job_sink sink, emp_job_done done, void *user) {
         off += n;
         h.n_edges += n;
         h.S = b->h_lcnt[i];
-        h.P = b->h_lcnt[nj + i];
+        h.P = b->h_lcnt[b->tab_cap + i];
         ++h.t;
         if (h.t >= h.rounds || h.S == 0 || h.P == 0)
             finish_job(b, id, EMP_OK, done, user);
@@ -988,6 +1087,18 @@ int round_step(emp_batch *b, emp_job_sink sink, emp_job_done done, void *user) {
     return EMP_OK;
 }
 
+// The job's bucket loads: the first tuning class it belongs to.
+void classify(const emp_batch *b, HJob &h) {
+    h.fine_load = BTARGET;
+    h.coarse_load = BCOARSE;
+    for (const emp_batch_class &c : b->classes)
+        if (h.L <= c.read_len_max && h.n <= c.n_max) {
+            h.fine_load = c.fine_load;
+            h.coarse_load = c.coarse_load;
+            return;
+        }
+}
+
 int submit_job(emp_batch *b, HJob &&h, int32_t *job) {
     if (h.n > BJOB_MAX_READS)
         return fail(b, EMP_ERR_TOO_MANY, "a batched job holds at most %llu reads (got %llu)",
@@ -995,6 +1106,7 @@ int submit_job(emp_batch *b, HJob &&h, int32_t *job) {
     if (h.n && (h.L <= 0 || h.L > MAX_RD_LEN)) return fail(b, EMP_ERR_READLEN, "read length %d outside [1, 255]", h.L);
     if (h.o < 0) return fail(b, EMP_ERR_ARG, "negative minimum overlap");
     h.submit_ms = now_ms() - b->t0;
+    classify(b, h);
     b->jobs.push_back(std::move(h));
     const int id = (int)b->jobs.size() - 1;
     b->queued.push_back(id);
@@ -1066,6 +1178,7 @@ void emp_batch_destroy(emp_batch *b) {
         if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
         if (b->stage_ev[i]) (void)hipEventDestroy(b->stage_ev[i]);
     }
+    if (b->graph.exec) (void)hipGraphExecDestroy(b->graph.exec);
     void *host[] = {b->h_jobs, b->h_pre, b->h_lcnt, b->h_from, b->h_to, b->h_ctr};
     for (void *h : host)
         if (h) (void)hipHostFree(h);
@@ -1079,6 +1192,42 @@ void emp_batch_destroy(emp_batch *b) {
 }
 
 const char *emp_batch_last_error(const emp_batch *b) { return b ? b->err.c_str() : "no engine"; }
+
+int emp_batch_set_tuning(emp_batch *b, const emp_batch_tuning *t) {
+    if (!b) return EMP_ERR_ARG;
+    if (!b->queued.empty() || !b->active.empty())
+        return fail(b, EMP_ERR_ARG, "the tuning changes only while no job is queued or resident");
+    const uint32_t fcap = t ? t->bucket_cap : (uint32_t)BCAP;
+    if (fcap != (uint32_t)BCAP && fcap != (uint32_t)BCAP_L)
+        return fail(b, EMP_ERR_ARG, "bucket_cap must be %d or %d", BCAP, BCAP_L);
+    if (t && t->n_classes > EMP_BATCH_MAX_CLASSES) return fail(b, EMP_ERR_ARG, "more than %d classes", EMP_BATCH_MAX_CLASSES);
+    std::vector<emp_batch_class> cls;
+    for (uint32_t i = 0; t && i < t->n_classes; ++i) {
+        const emp_batch_class &c = t->classes[i];
+        if (c.fine_load < 32 || c.fine_load > fcap || c.coarse_load < c.fine_load || c.coarse_load > (1u << 20))
+            return fail(b, EMP_ERR_ARG, "class %u: loads %u / %u outside 32..%u / fine..2^20", i, c.fine_load,
+                        c.coarse_load, fcap);
+        cls.push_back(c);
+    }
+    if (fcap != b->fcap) {  // the fine slots take the new stride at the next step
+        BCHK(b, hipStreamSynchronize(b->stream));
+        for (int s = 0; s < 2; ++s) {
+            dfree(b, b->d_fine[s]);
+            b->d_fine[s] = nullptr;
+        }
+        b->fine_cap = 0;
+        b->fcap = fcap;
+    }
+    b->classes.swap(cls);
+    return EMP_OK;
+}
+
+int emp_batch_graph_stats(const emp_batch *b, uint64_t *captures, uint64_t *replays) {
+    if (!b) return EMP_ERR_ARG;
+    if (captures) *captures = b->graph_captures;
+    if (replays) *replays = b->graph_replays;
+    return EMP_OK;
+}
 
 uint64_t emp_batch_job_bytes(uint64_t n, int32_t read_len) {
     return read_len > 0 ? job_charge(n, read_len) : 0;

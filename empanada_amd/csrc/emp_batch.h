@@ -19,11 +19,14 @@ namespace emp {
 // A job's fine buckets hold BTARGET entries per side on average and at most BCAP (the rest
 // spills to the large-bucket path); at that load the matched records of a bucket stay below BMCAP
 // up to a match rate of about 100 %.
-constexpr int BCAP = 512;             // fine-bucket capacity per side
+constexpr int BCAP = 512;             // fine-bucket capacity per side (the default kernel variant)
 constexpr int BMCAP = 256;            // matched records per side a bucket may hold in LDS
 constexpr int BJOIN_THREADS = 256;    // work-group size of the batched join and dedup
-constexpr uint32_t BTARGET = 256;     // mean fine-bucket load of a job
-constexpr uint32_t BCOARSE = 16384;   // mean coarse-bucket load of a job
+// The other variant of the join and dedup: buckets of 1,024 per side, 512 matched records,
+// 512 threads (emp_batch_set_tuning chooses).
+constexpr int BCAP_L = 1024, BMCAP_L = 512, BJOIN_THREADS_L = 512;
+constexpr uint32_t BTARGET = 256;     // mean fine-bucket load of a job (default tuning)
+constexpr uint32_t BCOARSE = 16384;   // mean coarse-bucket load of a job (default tuning)
 constexpr uint32_t BSPLIT = 256;      // coarse buckets per job, and fine per coarse, at most
 constexpr uint64_t BJOB_MAX_READS = 1ull << 23;  // 2^24 entries per side: nb_j <= BSPLIT^2
 
@@ -43,6 +46,11 @@ struct BJob {
     RoundArgs rg;       // the job's round (rg.active = 0 in a dedup step)
 };
 
+// Every launch of a step takes the job tables' capacity as its job count: the prefix tables hold
+// the step's totals from its last job on, job j's counters sit at a stride of that capacity, and a
+// work-group past the step's total exits, so launches captured in a graph with larger grids can be
+// replayed for later steps.
+//
 // One partition pass of one side over every job of a step.
 //   pass 1: job segments -> coarse buckets (regenerating the entries for the job's round)
 //   pass 2: coarse buckets -> fine buckets (overflow to the spill area)
@@ -55,8 +63,9 @@ struct BPartArgs {
     const uint32_t *cols;
     Entry *coarse;           // this side's coarse slots
     uint32_t *ccnt;          // this side's coarse counts
-    Entry *fine;             // this side's fine slots, BCAP per bucket
+    Entry *fine;             // this side's fine slots, fcap per bucket
     uint32_t *fcnt;          // this side's fine counts
+    uint32_t fcap;           // fine-bucket capacity (BCAP or BCAP_L)
     SpillRec *spill;
     uint32_t *spill_n;
     uint32_t spill_cap;
@@ -110,11 +119,13 @@ struct BBigArgs {
     unsigned long long *edge_off;
     uint32_t *edge_cnt;
     uint32_t *too_many;  // set when a bucket holds 2^31 edges or more
+    uint32_t fcap;       // fine-bucket capacity
 };
 
 hipError_t launch_bpart(const BPartArgs &a, int pass, bool regen, uint32_t groups, hipStream_t s);
-hipError_t launch_bdedup(const BJoinArgs &a, uint32_t nb, hipStream_t s);
-hipError_t launch_bjoin(const BJoinArgs &a, uint32_t nb, hipStream_t s);
+// fcap selects the kernel variant (BCAP or BCAP_L); the fine slots have that stride.
+hipError_t launch_bdedup(const BJoinArgs &a, uint32_t nb, uint32_t fcap, hipStream_t s);
+hipError_t launch_bjoin(const BJoinArgs &a, uint32_t nb, uint32_t fcap, hipStream_t s);
 hipError_t launch_bdedup_big(const BBigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_bjoin_big(const BBigArgs &a, uint32_t n, hipStream_t s);
 

@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart1(BPartArgs a) {
     static_assert(NT >= NB && T % NT == 0, "one bin per thread");
     __shared__ Entry buf[T];
     __shared__ uint32_t hist[NB], loc[NB], base[NB], sh[NT / 64];
+    if (blockIdx.x >= a.tpre[a.njobs]) return;  // a grid sized for a larger step (a replayed graph)
     const uint32_t j = job_of(a.tpre, a.njobs, blockIdx.x);
     const BJob &J = a.jobs[j];
     const int side = a.side;
@@ -120,6 +121,7 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
     constexpr int NT = PART_THREADS, T = PART_TILE, PER = T / NT, NB = BSPLIT;
     __shared__ Entry buf[T];
     __shared__ uint32_t hist[NB], loc[NB], base[NB], sh[NT / 64];
+    if (blockIdx.x >= a.tpre[a.njobs]) return;  // a grid sized for a larger step (a replayed graph)
     const uint32_t j = job_of(a.tpre, a.njobs, blockIdx.x);
     const BJob &J = a.jobs[j];
     const int side = a.side;
@@ -161,8 +163,8 @@ __global__ void __launch_bounds__(PART_THREADS) k_bpart2(BPartArgs a) {
         const uint32_t dd = range_of(v.h1, nb) - f0;
         const uint32_t slot = base[dd] + (p - loc[dd]);
         const uint32_t bucket = fb + dd;
-        if (slot < (uint32_t)BCAP) {
-            a.fine[(size_t)bucket * BCAP + slot] = v;
+        if (slot < a.fcap) {
+            a.fine[(size_t)bucket * a.fcap + slot] = v;
         } else {
             const uint32_t s = atomicAdd(a.spill_n, 1u);
             if (s < a.spill_cap) a.spill[s] = SpillRec{v, bucket};
@@ -190,6 +192,7 @@ __global__ void __launch_bounds__(NT) k_bdedup(BJoinArgs a) {
     __shared__ unsigned long long kbase;
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
+    if (b >= a.fpre[a.njobs]) return;  // a grid sized for a larger step (a replayed graph)
     const uint32_t n = a.s_cnt[b];
     if (n > (uint32_t)CAP) {
         if (threadIdx.x == 0) to_big(a, b);
@@ -254,6 +257,7 @@ __global__ void __launch_bounds__(NT) k_bjoin(BJoinArgs a) {
     __shared__ unsigned long long sbase[2];
     if (*a.abort) return;
     const uint32_t b = blockIdx.x;
+    if (b >= a.fpre[a.njobs]) return;  // a grid sized for a larger step (a replayed graph)
     const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
     if (ns > (uint32_t)CAP || np > (uint32_t)CAP) {
         if (threadIdx.x == 0) to_big(a, b);
@@ -446,8 +450,8 @@ __global__ void __launch_bounds__(1024) k_bjoin_big(BBigArgs a) {
     uint64_t *SK = a.skey + a.soff[blockIdx.x], *PK = a.pkey + a.poff[blockIdx.x];
     uint32_t *SR = a.srank + a.soff[blockIdx.x], *PR = a.prank + a.poff[blockIdx.x];
     uint32_t *SA = a.saux + a.soff[blockIdx.x], *PA = a.paux + a.poff[blockIdx.x];
-    gather_bucket(a.s_in, ns, BCAP, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos);
-    gather_bucket(a.p_in, np, BCAP, a.p_spill, a.p_spill_n, a.spill_cap, b, PK, PR, np2, true, &s_pos);
+    gather_bucket(a.s_in, ns, a.fcap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos);
+    gather_bucket(a.p_in, np, a.fcap, a.p_spill, a.p_spill_n, a.spill_cap, b, PK, PR, np2, true, &s_pos);
     bitonic(SK, SR, nullptr, ns2);
     bitonic(PK, PR, nullptr, np2);
     for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
@@ -542,7 +546,7 @@ __global__ void __launch_bounds__(1024) k_bdedup_big(BBigArgs a) {
     const uint32_t n = a.s_cnt[b], n2 = pow2_at_least(n);
     uint64_t *K = a.skey + a.soff[blockIdx.x];
     uint32_t *R = a.srank + a.soff[blockIdx.x];
-    gather_bucket(a.s_in, n, BCAP, a.s_spill, a.s_spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
+    gather_bucket(a.s_in, n, a.fcap, a.s_spill, a.s_spill_n, a.spill_cap, b, K, R, n2, false, &s_pos);
     bitonic(K, R, nullptr, n2);
     const uint64_t seg = a.jobs[j].seg;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -565,15 +569,25 @@ hipError_t launch_bpart(const BPartArgs &a, int pass, bool regen, uint32_t group
     return hipGetLastError();
 }
 
-hipError_t launch_bdedup(const BJoinArgs &a, uint32_t nb, hipStream_t s) {
+hipError_t launch_bdedup(const BJoinArgs &a, uint32_t nb, uint32_t fcap, hipStream_t s) {
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_bdedup<BCAP, BJOIN_THREADS>), dim3(nb), dim3(BJOIN_THREADS), 0, s, a);
+    if (fcap == (uint32_t)BCAP_L)
+        hipLaunchKernelGGL((k_bdedup<BCAP_L, BJOIN_THREADS_L>), dim3(nb), dim3(BJOIN_THREADS_L), 0, s, a);
+    else if (fcap == (uint32_t)BCAP)
+        hipLaunchKernelGGL((k_bdedup<BCAP, BJOIN_THREADS>), dim3(nb), dim3(BJOIN_THREADS), 0, s, a);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_bjoin(const BJoinArgs &a, uint32_t nb, hipStream_t s) {
+hipError_t launch_bjoin(const BJoinArgs &a, uint32_t nb, uint32_t fcap, hipStream_t s) {
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_bjoin<BCAP, BMCAP, BJOIN_THREADS>), dim3(nb), dim3(BJOIN_THREADS), 0, s, a);
+    if (fcap == (uint32_t)BCAP_L)
+        hipLaunchKernelGGL((k_bjoin<BCAP_L, BMCAP_L, BJOIN_THREADS_L>), dim3(nb), dim3(BJOIN_THREADS_L), 0, s, a);
+    else if (fcap == (uint32_t)BCAP)
+        hipLaunchKernelGGL((k_bjoin<BCAP, BMCAP, BJOIN_THREADS>), dim3(nb), dim3(BJOIN_THREADS), 0, s, a);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

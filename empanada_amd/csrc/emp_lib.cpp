@@ -30,6 +30,7 @@
 #include "../../include/empanada_stages.h"
 #include "emp_fastq.h"
 #include "emp_fastq_gpu.h"
+#include "emp_hostcopy.h"
 #include "emp_kernels.h"
 #include "emp_ooc.h"
 #include "emp_ooc_plan.h"
@@ -805,19 +806,6 @@ int add_fastq_file(emp_ctx *c, const char *path) {
 // one of three pinned stages (the host fills batch b + 1 while batch b is on the link). The
 // caller's buffer is free again once the copies are done; the map kernels may still run.
 constexpr uint64_t PIN_BATCH = 256ull << 20;
-constexpr int COPY_THREADS = 4;
-
-void parallel_copy(char *dst, const char *src, uint64_t bytes) {
-    const uint64_t per = (bytes + COPY_THREADS - 1) / COPY_THREADS;
-    auto part = [&](int p) {
-        const uint64_t a = std::min(bytes, per * (uint64_t)p), e = std::min(bytes, a + per);
-        if (e > a) memcpy(dst + a, src + a, (size_t)(e - a));
-    };
-    std::thread th[COPY_THREADS - 1];
-    for (int p = 1; p < COPY_THREADS; ++p) th[p - 1] = std::thread(part, p);
-    part(0);
-    for (auto &t : th) t.join();
-}
 
 int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
     if (m == 0) return EMP_OK;

@@ -79,6 +79,14 @@ class DeviceBuffer:
     def address(self):
         return self.ptr.value
 
+    def copy_from(self, a: np.ndarray, offset: int = 0):
+        """Copies host array a to byte `offset` of the buffer."""
+        a = np.ascontiguousarray(a)
+        if offset < 0 or offset + a.nbytes > self.nbytes:
+            raise ValueError("copy outside the buffer")
+        _check(lib().hipMemcpy(ctypes.c_void_p(self.ptr.value + offset), a.ctypes.data, a.nbytes, H2D),
+               "hipMemcpy H2D")
+
     def free(self):
         if self.ptr:
             lib().hipFree(self.ptr)
@@ -144,3 +152,26 @@ def copy_d2d(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
 
 def synchronize():
     _check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceJobs:
+    """Read sets resident in HBM: one buffer, each set at a 256-byte aligned offset. jobs() gives
+    (address, n, read length, min overlap) per set."""
+
+    def __init__(self, sets):
+        offs, total = [], 0
+        for r, _ in sets:
+            offs.append(total)
+            total += (r.nbytes + 255) & ~255
+        self.buf = DeviceBuffer(total)
+        for (r, _), off in zip(sets, offs):
+            if r.nbytes:
+                self.buf.copy_from(r, off)
+        self._jobs = [(self.buf.address + off, r.shape[0], r.shape[1] if r.ndim == 2 else 0, o)
+                      for (r, o), off in zip(sets, offs)]
+
+    def jobs(self):
+        return self._jobs
+
+    def free(self):
+        self.buf.free()

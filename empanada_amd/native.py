@@ -31,7 +31,8 @@ EXPORTS = [
     "emp_batch_opts_default", "emp_batch_create", "emp_batch_destroy", "emp_batch_last_error", "emp_batch_submit",
     "emp_batch_submit_fastq", "emp_batch_job_bytes", "emp_batch_step", "emp_batch_run", "emp_batch_job_info",
     "emp_batch_reset", "emp_batch_clock_ms", "emp_batch_kernel_count", "emp_batch_kernel_info",
-    "emp_batch_kernel_clear", "emp_batch_set_profile", "emp_batch_stream",
+    "emp_batch_kernel_clear", "emp_batch_set_profile", "emp_batch_stream", "emp_batch_set_tuning",
+    "emp_batch_graph_stats",
 ]
 
 
@@ -60,7 +61,20 @@ class Summary(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("device", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("stream", ctypes.c_void_p), ("max_active", ctypes.c_uint32)]
+                ("stream", ctypes.c_void_p), ("max_active", ctypes.c_uint32), ("no_graph", ctypes.c_int32)]
+
+
+class BatchClass(ctypes.Structure):
+    _fields_ = [("read_len_max", ctypes.c_int32), ("n_max", ctypes.c_uint32), ("fine_load", ctypes.c_uint32),
+                ("coarse_load", ctypes.c_uint32)]
+
+
+MAX_CLASSES = 16
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("bucket_cap", ctypes.c_uint32), ("n_classes", ctypes.c_uint32),
+                ("classes", BatchClass * MAX_CLASSES)]
 
 
 class JobInfo(ctypes.Structure):
@@ -140,6 +154,8 @@ def lib():
         L.emp_batch_kernel_count.argtypes = [vp]
         L.emp_batch_kernel_info.argtypes = [vp, i32, ctypes.POINTER(cp), ctypes.POINTER(u64),
                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.emp_batch_set_tuning.argtypes = [vp, ctypes.POINTER(BatchTuning)]
+        L.emp_batch_graph_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.emp_batch_kernel_clear.argtypes = [vp]
         L.emp_batch_set_profile.argtypes = [vp, i32]
         L.emp_batch_stream.restype = vp
@@ -249,7 +265,8 @@ class Context:
                     chunks.append(np.ctypeslib.as_array(to, shape=(n,)).tobytes())
             return 0
 
-        self._check(lib().emp_run(self._h, sink, None))
+        # nothing collected: no sink (the library still copies the records to host memory)
+        self._check(lib().emp_run(self._h, sink if collect else SINK(), None))
         return b"".join(chunks) if collect else None
 
     def run_to_file(self, path: str):
@@ -321,11 +338,13 @@ class Batch:
     """Many overlap jobs sharing one engine (wraps emp_batch). Each job's output is collected as
     its output.bin bytes."""
 
-    def __init__(self, hbm_budget=0, max_active=0, device=0, profile=False, stream=None, collect=True):
+    def __init__(self, hbm_budget=0, max_active=0, device=0, profile=False, stream=None, collect=True, graph=True,
+                 tuning=None):
         o = BatchOpts()
         lib().emp_batch_opts_default(ctypes.byref(o))
         o.hbm_budget, o.max_active, o.device, o.profile, o.stream = hbm_budget, max_active, device, \
             1 if profile else 0, stream
+        o.no_graph = 0 if graph else 1
         self._h = ctypes.c_void_p()
         rc = lib().emp_batch_create(ctypes.byref(o), ctypes.byref(self._h))
         if rc:
@@ -351,6 +370,8 @@ class Batch:
             self._keep.pop(job, None)
 
         self._sink, self._done = sink, done
+        if tuning is not None:
+            self.set_tuning(tuning)
 
     def _check(self, rc):
         if rc:
@@ -396,14 +417,19 @@ class Batch:
         self._check(lib().emp_batch_submit_fastq(self._h, data, len(data), min_ovlp, ctypes.byref(job)))
         return job.value
 
+    def _sink_arg(self):
+        # nothing collected: no sink (the engine still copies each step's records to host memory)
+        return self._sink if self.collect else JOB_SINK()
+
     def step(self):
         """One step; returns (active, queued)."""
         a, q = ctypes.c_int32(), ctypes.c_int32()
-        self._check(lib().emp_batch_step(self._h, self._sink, self._done, None, ctypes.byref(a), ctypes.byref(q)))
+        self._check(lib().emp_batch_step(self._h, self._sink_arg(), self._done, None, ctypes.byref(a),
+                                         ctypes.byref(q)))
         return a.value, q.value
 
     def run(self):
-        self._check(lib().emp_batch_run(self._h, self._sink, self._done, None))
+        self._check(lib().emp_batch_run(self._h, self._sink_arg(), self._done, None))
 
     def output(self, job: int) -> bytes:
         return records_bytes(self.records.get(job, []))
@@ -435,12 +461,39 @@ class Batch:
     def kernel_clear(self):
         lib().emp_batch_kernel_clear(self._h)
 
+    def set_tuning(self, tuning):
+        """tuning: None (defaults) or dict(bucket_cap=512|1024, classes=[dict(read_len_max, n_max,
+        fine_load, coarse_load), ...]), as empanada_amd/tune.py writes it."""
+        if tuning is None:
+            self._check(lib().emp_batch_set_tuning(self._h, None))
+            return
+        t = BatchTuning()
+        t.bucket_cap = int(tuning.get("bucket_cap", 512))
+        cls = tuning.get("classes", [])
+        if len(cls) > MAX_CLASSES:
+            raise ValueError("at most %d tuning classes" % MAX_CLASSES)
+        t.n_classes = len(cls)
+        for i, c in enumerate(cls):
+            t.classes[i] = BatchClass(int(c["read_len_max"]), int(c["n_max"]), int(c["fine_load"]),
+                                      int(c["coarse_load"]))
+        self._check(lib().emp_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def graph_stats(self):
+        """(graphs captured, round steps replayed from a graph)."""
+        c, r = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().emp_batch_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     def set_profile(self, on: bool):
         lib().emp_batch_set_profile(self._h, 1 if on else 0)
 
 
 def job_bytes(n: int, L: int) -> int:
     return lib().emp_batch_job_bytes(n, L)
+
+
+def version() -> str:
+    return lib().emp_version().decode()
 
 
 def stage_fingerprints(reads: np.ndarray, device=0) -> np.ndarray:

@@ -22,8 +22,9 @@ class Server:
     """One engine serving a stream of jobs. collect=True keeps every job's records (output.bin
     bytes via output(job))."""
 
-    def __init__(self, hbm_budget=0, max_active=0, device=0, collect=False):
-        self.engine = native.Batch(hbm_budget=hbm_budget, max_active=max_active, device=device, collect=collect)
+    def __init__(self, hbm_budget=0, max_active=0, device=0, collect=False, tuning=None):
+        self.engine = native.Batch(hbm_budget=hbm_budget, max_active=max_active, device=device, collect=collect,
+                                   tuning=tuning)
 
     def close(self):
         self.engine.close()

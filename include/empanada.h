@@ -97,7 +97,8 @@ int emp_add_reads_device(emp_ctx *ctx, const void *d_reads, uint64_t n, int32_t 
  * Overlap phase: build_graph (overlap.cu:206-293) — dedup, then rounds of regenerate, sort,
  * suffix/prefix join, edge output and used-read removal. Each record of output.bin
  * (overlap.cu:148-161) is passed to `sink` in file order; the arrays are valid during the call.
- * A non-zero return from the sink aborts the run with that value.
+ * A non-zero return from the sink aborts the run with that value. sink may be NULL: the records
+ * are still produced and copied to host memory, then dropped.
  */
 typedef int (*emp_record_sink)(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to);
 int emp_run(emp_ctx *ctx, emp_record_sink sink, void *user);

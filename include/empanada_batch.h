@@ -35,7 +35,25 @@ typedef struct {
     int32_t profile;       /* non-zero: time every kernel launch with HIP events */
     void *stream;          /* hipStream_t to run on; NULL = the engine creates one */
     uint32_t max_active;   /* resident jobs at most; 0 = no limit */
+    int32_t no_graph;      /* non-zero: launch a round step's kernels one by one instead of replaying the
+                              step's captured HIP graph (profile also launches them one by one) */
 } emp_batch_opts;
+
+/* A class of jobs and its bucket loads (emp_batch_set_tuning). */
+typedef struct {
+    int32_t read_len_max;  /* a job belongs to the first class with read_len <= read_len_max */
+    uint32_t n_max;        /*   and n_reads <= n_max */
+    uint32_t fine_load;    /* mean entries per fine bucket and side: 32 .. bucket_cap */
+    uint32_t coarse_load;  /* mean entries per coarse bucket and side: fine_load .. 2^20 */
+} emp_batch_class;
+
+#define EMP_BATCH_MAX_CLASSES 16
+typedef struct {
+    uint32_t bucket_cap;   /* fine-bucket capacity, which selects the join and dedup kernels:
+                              512 (256 threads) or 1024 (512 threads) */
+    uint32_t n_classes;
+    emp_batch_class classes[EMP_BATCH_MAX_CLASSES];  /* a job in no class: loads 256 / 16384 */
+} emp_batch_tuning;
 
 /* Job states reported by emp_batch_job_info. */
 #define EMP_JOB_QUEUED 0
@@ -53,7 +71,8 @@ typedef struct {
 } emp_job_info;
 
 /* Receives job `job`'s records in output.bin order (arrays valid during the call). A non-zero
- * return aborts the step with that value. */
+ * return aborts the step with that value. A NULL sink drops the records (they are still copied
+ * to host memory). */
 typedef int (*emp_job_sink)(void *user, int32_t job, uint64_t n, int32_t ovlp, const uint32_t *from,
                             const uint32_t *to);
 /* Called once per job when it ends (status EMP_OK or the job's error). */
@@ -97,6 +116,17 @@ int emp_batch_job_info(const emp_batch *b, int32_t job, emp_job_info *out);
 /* Forgets every finished job (ids restart at 0); allocations are kept. Fails while jobs are
  * queued or resident. */
 int emp_batch_reset(emp_batch *b);
+
+/*
+ * Sets the engine's tuning, as the autotuner (empanada_amd/tune.py) picks it per class of jobs
+ * from measurements and keeps it in a JSON cache; NULL restores the defaults (bucket_cap 512,
+ * loads 256 / 16384 for every job). Outputs do not depend on it. Fails while jobs are queued or
+ * resident.
+ */
+int emp_batch_set_tuning(emp_batch *b, const emp_batch_tuning *t);
+
+/* Round steps launched from a captured graph so far, and graphs captured for them. */
+int emp_batch_graph_stats(const emp_batch *b, uint64_t *captures, uint64_t *replays);
 
 /* Milliseconds on the engine clock. */
 double emp_batch_clock_ms(const emp_batch *b);

@@ -173,3 +173,62 @@ def test_served_jobs_are_the_same_alone_and_in_the_mix():
             j = b.submit(r, o)
             b.run()
             assert b.output(j) == got
+
+
+def test_graph_replays_equal_launches_and_the_oracle():
+    """Round steps replayed from captured graphs (grids above the step's needs, tables updated
+    between replays, admission steps launched eagerly in between) against the same jobs with every
+    kernel launched one by one, and against the oracle."""
+    jobs = make_jobs(JOBS)
+    wants = [oracle_out(r, o)[0] for r, o in jobs]
+    outs = {}
+    for graph in (True, False):
+        with native.Batch(graph=graph) as b:
+            ids, k = [], 0
+            while k < len(jobs) or len(b.finished) < len(jobs):
+                if k < len(jobs):  # one job admitted per step: tables change under the graph
+                    ids.append(b.submit(*jobs[k]))
+                    k += 1
+                b.step()
+            outs[graph] = [b.output(j) for j in ids]
+            captures, replays = b.graph_stats()
+        if graph:
+            assert captures >= 1 and replays > 2 * captures
+        else:
+            assert (captures, replays) == (0, 0)
+    assert outs[True] == outs[False] == wants
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(bucket_cap=1024, classes=[dict(read_len_max=255, n_max=1 << 23, fine_load=768, coarse_load=65536)]),
+    dict(bucket_cap=1024, classes=[dict(read_len_max=100, n_max=5000, fine_load=256, coarse_load=8192),
+                                   dict(read_len_max=255, n_max=1 << 23, fine_load=512, coarse_load=16384)]),
+    dict(bucket_cap=512, classes=[dict(read_len_max=150, n_max=1 << 23, fine_load=128, coarse_load=65536),
+                                  dict(read_len_max=255, n_max=1 << 23, fine_load=384, coarse_load=8192)]),
+])
+def test_tunings_give_the_oracles_records(tuning):
+    """Every kernel variant and bucket load the autotuner may pick leaves the records unchanged."""
+    jobs = make_jobs(JOBS)
+    with native.Batch(tuning=tuning) as b:
+        ids = [b.submit(r, o) for r, o in jobs]
+        b.run()
+        for (r, o), j in zip(jobs, ids):
+            assert b.status[j] == 0
+            assert b.output(j) == oracle_out(r, o)[0], "job %d" % j
+
+
+def test_tuning_is_checked():
+    with native.Batch() as b:
+        for bad in (dict(bucket_cap=768), dict(bucket_cap=512, classes=[dict(read_len_max=100, n_max=10,
+                                                                             fine_load=16, coarse_load=64)]),
+                    dict(bucket_cap=512, classes=[dict(read_len_max=100, n_max=10, fine_load=600,
+                                                       coarse_load=1000)])):
+            with pytest.raises(native.EmpError) as e:
+                b.set_tuning(bad)
+            assert e.value.code == -1
+        r, o = make_jobs(JOBS[:1])[0]
+        b.submit(r, o)
+        with pytest.raises(native.EmpError):  # not while a job is queued
+            b.set_tuning(None)
+        b.run()
+        b.set_tuning(None)
