@@ -135,12 +135,18 @@ struct PartArgs {
     uint32_t *spill_n;
     uint32_t spill_cap;
     uint32_t *abort;         // set when an intermediate bucket or the spill area overflows
+    // a 4-byte payload that moves with each entry (null: none): a linear input's entry j has
+    // pay_in[j / 2] (the map's per-read word, shared by a read's two entries), a grouped input's
+    // the word at its own slot; pay_out is written at the output slot (spilled entries drop it)
+    const uint32_t *pay_in;
+    uint32_t *pay_out;
 };
 
 struct DedupArgs {
     const Entry *in;
     const uint32_t *cnt;
-    const uint32_t *ends;    // per read: first and last column groups (MapArgs::ends)
+    const uint32_t *pay;     // per slot (b * CAP + i): the entry's read's ends word (MapArgs::ends),
+                             // carried through the partition; null: gathered from the columns
     NextOut nx;              // the kept entries, regenerated: round 0's suffix and prefix lists
     uint32_t *big, *n_big;
     uint32_t big_cap;

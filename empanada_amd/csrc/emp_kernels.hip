@@ -466,28 +466,65 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
             base[threadIdx.x] = h ? atomicAdd(&a.out_cnt[gb + threadIdx.x], h) : 0u;
         }
         lds_barrier();
+        uint32_t at[PER];  // each entry's position in the tile's bin order (the payload's too)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             uint32_t i = threadIdx.x + NT * q;
-            if (i < nt) buf[loc[d[q]] + r[q]] = e[q];
+            at[q] = i < nt ? loc[d[q]] + r[q] : 0u;
+            if (i < nt) buf[at[q]] = e[q];
         }
         lds_barrier();
-        for (uint32_t p = threadIdx.x; p < nt; p += NT) {
-            const Entry v = buf[p];
-            const uint32_t dd = (v.h1 >> a.out_shift) & (F - 1);
-            const uint32_t slot = base[dd] + (p - loc[dd]);
-            const uint32_t bucket = gb + dd;
-            if (slot < a.out_cap) {
-                a.out[(size_t)bucket * a.out_cap + slot] = v;
-            } else if (a.spill) {
-                uint32_t s = atomicAdd(a.spill_n, 1u);
-                if (s < a.spill_cap) a.spill[s] = SpillRec{v, bucket};
-                else atomicOr(a.abort, 1u);
-            } else {
-                atomicOr(a.abort, 1u);
+        static_assert(PART_MAX_BITS <= 8 && PER % 4 == 0, "bin digits packed four to a word");
+        uint32_t dig[PER / 4];  // the bin of each position this thread writes (read again for the payload)
+#pragma unroll
+        for (int k = 0; k < PER / 4; ++k) dig[k] = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t p = threadIdx.x + NT * k;
+            if (p < nt) {
+                const Entry v = buf[p];
+                const uint32_t dd = (v.h1 >> a.out_shift) & (F - 1);
+                dig[k / 4] |= dd << (8 * (k % 4));
+                const uint32_t slot = base[dd] + (p - loc[dd]);
+                const uint32_t bucket = gb + dd;
+                if (slot < a.out_cap) {
+                    a.out[(size_t)bucket * a.out_cap + slot] = v;
+                } else if (a.spill) {
+                    uint32_t s = atomicAdd(a.spill_n, 1u);
+                    if (s < a.spill_cap) a.spill[s] = SpillRec{v, bucket};
+                    else atomicOr(a.abort, 1u);
+                } else {
+                    atomicOr(a.abort, 1u);
+                }
             }
         }
         lds_barrier();
+        if (a.pay_in) {
+            // the payloads take the same path through the tile's LDS, loaded now that the entries'
+            // registers are free; each thread writes the payloads of the positions it wrote
+            uint32_t pv[PER];
+            const uint32_t *pin = a.in_cnt ? a.pay_in + (size_t)g * a.in_cap : a.pay_in;
+            const uint32_t psh = a.in_cnt ? 0u : 1u;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) pv[q] = pin[(t0 + min(threadIdx.x + NT * q, last)) >> psh];
+            uint32_t *pb = reinterpret_cast<uint32_t *>(buf);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t i = threadIdx.x + NT * q;
+                if (i < nt) pb[at[q]] = pv[q];
+            }
+            lds_barrier();
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const uint32_t p = threadIdx.x + NT * k;
+                if (p < nt) {
+                    const uint32_t dd = (dig[k / 4] >> (8 * (k % 4))) & 0xFFu;
+                    const uint32_t slot = base[dd] + (p - loc[dd]);
+                    if (slot < a.out_cap) a.pay_out[(size_t)(gb + dd) * a.out_cap + slot] = pb[p];
+                }
+            }
+            lds_barrier();
+        }
     }
 }
 
@@ -551,12 +588,12 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         uint32_t *row = wcnt[threadIdx.x >> 6];
         if ((threadIdx.x & 63) < NSLOT) row[threadIdx.x & 63] = 0;
         uint32_t ts[PER], tp[PER];
-        if (a.ends && a.nx.rg.col_t == 0) {
+        if (a.pay && a.nx.rg.col_t == 0) {
             // round 0 gathers column group 0 or the last one (suffix and prefix in turn): both
-            // are in the read's ends word, one 4-byte load per kept entry
+            // are in the read's ends word, which the partition carried to the entry's slot
             uint32_t ew[PER];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) ew[q] = a.ends[((keepmask >> q) & 1u) ? kept[q].rid >> 1 : 0u];
+            for (int q = 0; q < PER; ++q) ew[q] = a.pay[(size_t)b * CAP + min(threadIdx.x + NT * q, n - 1)];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 ts[q] = tp[q] = 0x10000u;

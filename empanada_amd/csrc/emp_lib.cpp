@@ -184,6 +184,10 @@ struct emp_ctx {
     uint64_t inter_cap = 0;  // entries per intermediate array
     Entry *d_dfine = nullptr;
     uint32_t *d_dcnt = nullptr;
+    // the partition's payload (each entry's read's ends word) beside the intermediate and final slots
+    uint32_t *d_inter_pay[2] = {nullptr, nullptr};
+    uint32_t *d_dpay = nullptr;
+    uint64_t dpay_cap = 0;
     SpillRec *d_dspill = nullptr;
     uint32_t dspill_cap = 0;
     double slack = 1.125;
@@ -956,9 +960,23 @@ int ensure_run_buffers(emp_ctx *c, uint64_t x, int bb) {
     c->d_dfine = c->d_bfine[1][0];
     const uint64_t need = inter_need(x, bb, c->slack);
     if (need > c->inter_cap) {
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < 2; ++k) {
             if ((rc = realloc_entries(c, &c->d_inter[k], need))) return rc;
+            dfree(c, c->d_inter_pay[k]);
+            c->d_inter_pay[k] = nullptr;
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, need * sizeof(uint32_t)))) return rc;
+            c->d_inter_pay[k] = (uint32_t *)p;
+        }
         c->inter_cap = need;
+    }
+    if ((uint64_t)nb * BUCKET_CAP > c->dpay_cap) {
+        dfree(c, c->d_dpay);
+        c->d_dpay = nullptr;
+        void *p = nullptr;
+        if ((rc = dalloc(c, &p, (size_t)nb * BUCKET_CAP * sizeof(uint32_t)))) return rc;
+        c->d_dpay = (uint32_t *)p;
+        c->dpay_cap = (uint64_t)nb * BUCKET_CAP;
     }
     if (!c->d_inter_cnt[0]) {
         for (int k = 0; k < 2; ++k) {
@@ -1023,8 +1041,12 @@ int partition(emp_ctx *c, uint64_t x, int bb) {
         a.spill_n = &c->d_ctr->spill;
         a.spill_cap = c->dspill_cap;
         a.abort = &c->d_ctr->abort;
+        // each entry's read's ends word goes along (round 0's regeneration in the dedup reads it)
+        a.pay_in = i == 0 ? c->d_ends : c->d_inter_pay[(i - 1) & 1];
+        a.pay_out = fin ? c->d_dpay : c->d_inter_pay[i & 1];
+        if (!c->d_ends) a.pay_in = nullptr;
         {
-            Timed t(c, "k_part", 24.0 * (double)x);
+            Timed t(c, "k_part", (a.pay_in ? 32.0 : 24.0) * (double)x);
             HIPCHK(c, launch_part(a, groups, c->stream));
         }
         in = out;
@@ -1907,14 +1929,14 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         DedupArgs a{};
         a.in = c->d_dfine;
         a.cnt = c->d_dcnt;
-        a.ends = c->d_ends;
+        a.pay = c->d_ends ? c->d_dpay : nullptr;
         a.nx = next_out(c, 0, bbD, bbD, rg0);
         a.big = c->d_big;
         a.n_big = &c->d_ctr->r[0].n_big;
         a.big_cap = BIG_CAP;
         a.abort = &c->d_ctr->abort;
         {
-            Timed t(c, "k_dedup", 12.0 * N0);
+            Timed t(c, "k_dedup", (c->d_ends ? 16.0 : 12.0) * N0);  // entries and their payload
             HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
         }
         if ((rc = read_counters(c))) return rc;
@@ -1941,9 +1963,9 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
                                      c->dspill_cap, 1u << bbD, 0, L, false)))
         return rc;
     st.D = c->h_ctr->r[0].total[0];
-    if (c->opts.profile) {  // per kept entry: two 12-byte entries written (both sides), its 4-byte ends word
+    if (c->opts.profile && !ooc) {  // per kept entry: two 12-byte entries written (both sides)
         int id = kstat_id(c, "k_dedup");
-        c->kstats[id].bytes += 28.0 * (double)st.D;
+        c->kstats[id].bytes += 24.0 * (double)st.D;
     }
     return EMP_OK;
 }
