@@ -12,9 +12,9 @@
 // A range of FASTQ bytes in HBM is parsed as the CLI reads it (main.cu:42-58): blocks of B bytes,
 // each starting where the previous one's complete records ended. Passes over the range: newline
 // counts per 4 KiB chunk -> chunk offsets (a three-launch scan) -> every newline's position ->
-// the block chain (one wave: each block's framing from the newline counts) -> 16 lanes per record
-// classify its sequence line -> kept-record offsets -> 16 lanes per kept record copy its bases
-// (row-major, read_len each).
+// the block chain (one wave: each block's framing from the newline counts) -> classify each
+// record's sequence line -> kept-record offsets -> copy the kept bases (row-major, read_len each);
+// classify and copy give each wave 64 records, 16 lanes per record.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,7 +28,6 @@ namespace {
 constexpr int FQ_THREADS = 256;
 constexpr int FQ_PER = 16;
 constexpr uint32_t FQ_CHUNK = FQ_THREADS * FQ_PER;  // bytes per work-group of the newline passes
-constexpr int FQ_LANES = 16;                         // lanes per record in classify / copy
 
 // Bit 7 of each byte of w set where that byte is '\n'.
 __device__ __forceinline__ uint32_t nl_bytes(uint32_t w) {
@@ -236,23 +235,6 @@ __global__ void __launch_bounds__(64) k_fq_chain(const uint8_t *d, uint64_t size
     }
 }
 
-// The aligned dword at absolute address a, with `valid` flagging (bit 7 of each byte) its bytes
-// inside [lo, hi). Loaded whole when it lies inside the range [d, d + size), byte by byte (the
-// valid bytes only) otherwise.
-__device__ __forceinline__ uint32_t window_dword(const uint8_t *d, uint64_t size, uintptr_t a, uintptr_t lo, uintptr_t hi,
-                                                 uint32_t &valid) {
-    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
-    valid = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (a + i >= lo && a + i < hi) valid |= 0x80u << (8 * i);
-    if (a >= db && a + 4 <= db + size) return *reinterpret_cast<const uint32_t *>(a);
-    uint32_t w = 0;
-    for (int i = 0; i < 4; ++i)
-        if ((valid >> (8 * i + 7)) & 1u) w |= (uint32_t)d[a + i - db] << (8 * i);
-    return w;
-}
-
 // Bit 7 of each byte of w set where that byte is not one of ACGTacgt.
 __device__ __forceinline__ uint32_t fq_nz(uint32_t v) { return (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u; }
 __device__ __forceinline__ uint32_t non_bases(uint32_t w) {
@@ -270,35 +252,116 @@ __device__ __forceinline__ void seq_line(const uint64_t *nlpos, uint64_t NL, uin
     e = 4 * rec + 1 < NL ? nlpos[4 * rec + 1] : size;
 }
 
-// 16 lanes per record: its sequence line's class, length and kept flag. The lanes take the
-// line's aligned dwords in turn; the group's flags meet in one ballot.
+// Classify and copy take 64 records per wave: lane l first loads record l's line bounds (one round
+// trip for the wave), then 16 lanes per record work on four records at a time, a batch of 16
+// records whose loads are all issued before any is used.
+constexpr uint32_t FQ_WAVE_RECS = 64;
+constexpr int FQ_BATCH = 4;  // records per 16-lane group in one batch
+constexpr int FQ_SLOTS = 4;  // aligned dwords per lane and record in classify: lines up to 253 bytes
+
+// Bit 7 of the bytes of the aligned dword at a that lie in [lo, hi) (a < hi, a + 4 > lo).
+__device__ __forceinline__ uint32_t valid_bytes(uintptr_t a, uintptr_t lo, uintptr_t hi) {
+    const uint32_t st = lo > a ? (uint32_t)(lo - a) : 0u;                 // 0..3
+    const uint32_t en = hi - a < 4 ? (uint32_t)(hi - a) : 4u;             // 1..4
+    return (0xFFFFFFFFu << (8 * st)) & (0xFFFFFFFFu >> (8 * (4 - en))) & 0x80808080u;
+}
+
+// The dword of the line at aligned address a, the bytes outside [d, d + size) zero (byte loads
+// near the range's ends, where the aligned dword would leave it).
+__device__ __forceinline__ uint32_t edge_dword(const uint8_t *d, uint64_t size, uintptr_t a) {
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
+    uint32_t w = 0;
+    for (int i = 0; i < 4; ++i)
+        if (a + i >= db && a + i < db + size) w |= (uint32_t)d[a + i - db] << (8 * i);
+    return w;
+}
+
+// Drop and bad flags of the bytes of dword w (valid: the bytes of the line).
+__device__ __forceinline__ void flag_dword(uint32_t w, uint32_t valid, bool &drop, bool &bad) {
+    const uint32_t dr = drop_bytes(w) & valid;
+    drop |= dr != 0;
+    bad |= (non_bases(w) & ~dr & valid) != 0;
+}
+
+// Per record of the range: its sequence line's length, kept flag (non-empty, no dropping byte, all
+// ACGTacgt) and, for a kept-class line with another byte, the first such record.
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
                                                           uint64_t NL, uint64_t records, uint32_t *kept, uint32_t *len,
                                                           FqResult *r) {
-    const uint64_t rec = ((uint64_t)blockIdx.x * FQ_THREADS + threadIdx.x) / FQ_LANES;
-    const uint32_t sub = threadIdx.x & (FQ_LANES - 1);
-    bool drop = false, bad = false;
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 4, sub = lane & 15;
+    const uint64_t rec0 = (uint64_t)blockIdx.x * FQ_THREADS + (threadIdx.x & ~63u);
+    if (rec0 >= records) return;  // the whole wave
+    const uint64_t myrec = rec0 + lane;
     uint64_t s = 0, e = 0;
-    if (rec < records) {
-        seq_line(nlpos, NL, size, rec, s, e);
-        const uintptr_t lo = reinterpret_cast<uintptr_t>(d) + s, hi = reinterpret_cast<uintptr_t>(d) + e;
-        for (uintptr_t a = (lo & ~(uintptr_t)3) + 4 * sub; a < hi; a += 4 * FQ_LANES) {
-            uint32_t valid;
-            const uint32_t w = window_dword(d, size, a, lo, hi, valid);
-            const uint32_t dr = drop_bytes(w) & valid;
-            drop |= dr != 0;
-            bad |= (non_bases(w) & ~dr & valid) != 0;
+    if (myrec < records) seq_line(nlpos, NL, size, myrec, s, e);
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
+    bool mydrop = false, mybad = false;
+    for (uint32_t base = 0; base < FQ_WAVE_RECS; base += 4 * FQ_BATCH) {
+        uint32_t w[FQ_BATCH][FQ_SLOTS];
+        uintptr_t lo[FQ_BATCH], hi[FQ_BATCH];
+#pragma unroll
+        for (int k = 0; k < FQ_BATCH; ++k) {
+            const uint32_t rl = base + 4 * k + g;
+            lo[k] = db + (uintptr_t)__shfl((unsigned long long)s, (int)rl, 64);
+            hi[k] = db + (uintptr_t)__shfl((unsigned long long)e, (int)rl, 64);
+#pragma unroll
+            for (int j = 0; j < FQ_SLOTS; ++j) {
+                const uintptr_t a = (lo[k] & ~(uintptr_t)3) + 4 * (sub + 16 * j);
+                const bool in = a < hi[k] && a >= db && a + 4 <= db + size;
+                w[k][j] = in ? *reinterpret_cast<const uint32_t *>(a) : 0u;
+            }
+        }
+        uint64_t bd[FQ_BATCH], bb[FQ_BATCH];
+#pragma unroll
+        for (int k = 0; k < FQ_BATCH; ++k) {
+            bool drop = false, bad = false;
+#pragma unroll
+            for (int j = 0; j < FQ_SLOTS; ++j) {
+                const uintptr_t a = (lo[k] & ~(uintptr_t)3) + 4 * (sub + 16 * j);
+                if (a < hi[k]) {
+                    const uint32_t x = (a >= db && a + 4 <= db + size) ? w[k][j] : edge_dword(d, size, a);
+                    flag_dword(x, valid_bytes(a, lo[k], hi[k]), drop, bad);
+                }
+            }
+            bd[k] = __ballot(drop);
+            bb[k] = __ballot(bad);
+        }
+        // lane base + 4k + g' takes its record's flags from group g' of batch k
+        if (lane >= base && lane < base + 4 * FQ_BATCH) {
+            const uint32_t k = (lane - base) >> 2, gg = (lane - base) & 3;
+            uint64_t xd = bd[0], xb = bb[0];
+#pragma unroll
+            for (int q = 1; q < FQ_BATCH; ++q)
+                if ((int)k == q) { xd = bd[q]; xb = bb[q]; }
+            mydrop = ((xd >> (16 * gg)) & 0xFFFFull) != 0;
+            mybad = ((xb >> (16 * gg)) & 0xFFFFull) != 0;
         }
     }
-    const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(FQ_LANES - 1);
-    const bool gdrop = ((__ballot(drop) >> gshift) & 0xFFFFull) != 0;
-    const bool gbad = ((__ballot(bad) >> gshift) & 0xFFFFull) != 0;
-    if (rec < records && sub == 0) {
+    // lines past the fast path's 253 bytes (no kept read is that long): the wave walks each
+    // rest, 64 dwords at a time
+    uint64_t lng = __ballot(myrec < records && e - s > (uint64_t)(64 * FQ_SLOTS - 3));
+    while (lng) {
+        const int rl = __ffsll((long long)lng) - 1;
+        lng &= lng - 1;
+        const uintptr_t lo = db + (uintptr_t)__shfl((unsigned long long)s, rl, 64);
+        const uintptr_t hi = db + (uintptr_t)__shfl((unsigned long long)e, rl, 64);
+        bool drop = false, bad = false;
+        for (uintptr_t a = (lo & ~(uintptr_t)3) + 4 * (16 * FQ_SLOTS + lane); a < hi; a += 4 * 64) {
+            const uint32_t x = (a >= db && a + 4 <= db + size) ? *reinterpret_cast<const uint32_t *>(a) : edge_dword(d, size, a);
+            flag_dword(x, valid_bytes(a, lo, hi), drop, bad);
+        }
+        const bool gd = __ballot(drop) != 0, gb = __ballot(bad) != 0;
+        if ((int)lane == rl) {
+            mydrop |= gd;
+            mybad |= gb;
+        }
+    }
+    if (myrec < records) {
         const uint64_t n = e - s;
-        const uint32_t k = (n > 0 && !gdrop && !gbad) ? 1u : 0u;
-        kept[rec] = k;
-        len[rec] = (uint32_t)(n < 0xFFFFFFFFull ? n : 0xFFFFFFFFull);
-        if (n > 0 && !gdrop && gbad) atomicMin(&r->first_bad, (unsigned long long)rec);
+        const bool k = n > 0 && !mydrop && !mybad;
+        kept[myrec] = k ? 1u : 0u;
+        len[myrec] = (uint32_t)(n < 0xFFFFFFFFull ? n : 0xFFFFFFFFull);
+        if (n > 0 && !mydrop && mybad) atomicMin(&r->first_bad, (unsigned long long)myrec);
     }
 }
 
@@ -323,18 +386,81 @@ __global__ void __launch_bounds__(256) k_fq_lencheck(const uint32_t *kept, const
     if (len[i] != want || len[i] > (uint32_t)MAX_RD_LEN) atomicMin(&r->first_len, (unsigned long long)i);
 }
 
-// 16 lanes per record: kept ones copy their bases to out + rank * L.
+// Kept records copy their bases to out + rank * L (row-major). A row's aligned output dwords come
+// from two aligned source dwords each (a funnel shift); the bytes before and after them are copied
+// singly. 64 records per wave, in batches of 16 whose loads are in flight together.
+__device__ __forceinline__ uint32_t unaligned_dword(const uint8_t *d, uint64_t size, uintptr_t sa, uint32_t lo,
+                                                    uint32_t hi, bool fast) {
+    if (fast) return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(sa & 3u));
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
+    uint32_t w = 0;
+    for (int i = 0; i < 4; ++i) w |= (uint32_t)d[sa + i - db] << (8 * i);  // inside the line: in range
+    (void)size;
+    return w;
+}
+
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
                                                       uint64_t NL, uint64_t records, const uint32_t *kept,
                                                       const unsigned long long *rank, uint32_t L, uint8_t *out) {
-    const uint64_t rec = ((uint64_t)blockIdx.x * FQ_THREADS + threadIdx.x) / FQ_LANES;
-    const uint32_t sub = threadIdx.x & (FQ_LANES - 1);
-    if (rec >= records || !kept[rec]) return;
-    uint64_t s, e;
-    seq_line(nlpos, NL, size, rec, s, e);
-    uint8_t *o = out + rank[rec] * (uint64_t)L;
-    const uint8_t *src = d + s;
-    for (uint32_t p = sub; p < L; p += FQ_LANES) o[p] = src[p];
+    (void)NL;
+    const uint32_t lane = threadIdx.x & 63, g = lane >> 4, sub = lane & 15;
+    const uint64_t rec0 = (uint64_t)blockIdx.x * FQ_THREADS + (threadIdx.x & ~63u);
+    if (rec0 >= records) return;  // the whole wave
+    const uint64_t myrec = rec0 + lane;
+    const bool mine = myrec < records && kept[myrec] != 0;
+    uint64_t src = 0, dst = 0;
+    if (mine) {
+        src = nlpos[4 * myrec] + 1;
+        dst = rank[myrec] * (uint64_t)L;
+    }
+    const uint64_t km = __ballot(mine);
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d), ob = reinterpret_cast<uintptr_t>(out);
+    for (uint32_t base = 0; base < FQ_WAVE_RECS; base += 4 * FQ_BATCH) {
+        if (((km >> base) & 0xFFFFull) == 0) continue;
+        bool act[FQ_BATCH];
+        uintptr_t sk[FQ_BATCH], dk[FQ_BATCH];
+        uint32_t hb[FQ_BATCH], nd[FQ_BATCH];
+#pragma unroll
+        for (int k = 0; k < FQ_BATCH; ++k) {
+            const uint32_t rl = base + 4 * k + g;
+            act[k] = ((km >> rl) & 1ull) != 0;
+            sk[k] = db + (uintptr_t)__shfl((unsigned long long)src, (int)rl, 64);
+            dk[k] = ob + (uintptr_t)__shfl((unsigned long long)dst, (int)rl, 64);
+            const uint32_t h = (uint32_t)((4u - (dk[k] & 3u)) & 3u);
+            hb[k] = h < L ? h : L;              // bytes before the row's first aligned dword
+            nd[k] = (L - hb[k]) >> 2;           // aligned dwords
+        }
+        // the bytes before and after the aligned dwords: lanes 0..2 and 8..10 of the group
+#pragma unroll
+        for (int k = 0; k < FQ_BATCH; ++k) {
+            if (!act[k]) continue;
+            const uint32_t tl = L - hb[k] - 4 * nd[k], t0 = hb[k] + 4 * nd[k];
+            if (sub < hb[k]) reinterpret_cast<uint8_t *>(dk[k])[sub] = reinterpret_cast<const uint8_t *>(sk[k])[sub];
+            if (sub >= 8 && sub - 8 < tl)
+                reinterpret_cast<uint8_t *>(dk[k])[t0 + sub - 8] = reinterpret_cast<const uint8_t *>(sk[k])[t0 + sub - 8];
+        }
+        for (uint32_t j0 = 0; 4 * j0 < L; j0 += 16) {
+            uint32_t lo[FQ_BATCH], hi[FQ_BATCH];
+            bool fast[FQ_BATCH];
+#pragma unroll
+            for (int k = 0; k < FQ_BATCH; ++k) {
+                const uint32_t kd = j0 + sub;
+                const uintptr_t sa = sk[k] + hb[k] + 4 * kd, a0 = sa & ~(uintptr_t)3;
+                fast[k] = a0 >= db && a0 + 8 <= db + size;
+                const bool ld = act[k] && kd < nd[k] && fast[k];
+                lo[k] = ld ? *reinterpret_cast<const uint32_t *>(a0) : 0u;
+                hi[k] = ld ? *reinterpret_cast<const uint32_t *>(a0 + 4) : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < FQ_BATCH; ++k) {
+                const uint32_t kd = j0 + sub;
+                if (act[k] && kd < nd[k]) {
+                    const uintptr_t sa = sk[k] + hb[k] + 4 * kd;
+                    *reinterpret_cast<uint32_t *>(dk[k] + hb[k] + 4 * kd) = unaligned_dword(d, size, sa, lo[k], hi[k], fast[k]);
+                }
+            }
+        }
+    }
 }
 
 }  // namespace
@@ -371,7 +497,7 @@ hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *n
                               FqResult *r, uint32_t *kept, uint32_t *len, unsigned long long *rank,
                               unsigned long long *blk, uint32_t L, hipStream_t s) {
     if (records == 0) return hipSuccess;
-    const uint64_t groups = (records * FQ_LANES + FQ_THREADS - 1) / FQ_THREADS;
+    const uint64_t groups = (records + FQ_THREADS - 1) / FQ_THREADS;  // a wave per 64 records
     hipLaunchKernelGGL(k_fq_classify, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records,
                        kept, len, r);
     hipError_t e = hipGetLastError();
@@ -388,7 +514,7 @@ hipError_t launch_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos
                           const uint32_t *kept, const unsigned long long *rank, uint32_t L, uint8_t *out,
                           hipStream_t s) {
     if (records == 0) return hipSuccess;
-    const uint64_t groups = (records * FQ_LANES + FQ_THREADS - 1) / FQ_THREADS;
+    const uint64_t groups = (records + FQ_THREADS - 1) / FQ_THREADS;  // a wave per 64 records
     hipLaunchKernelGGL(k_fq_copy, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records, kept,
                        rank, L, out);
     return hipGetLastError();
