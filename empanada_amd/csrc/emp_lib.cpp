@@ -5,6 +5,7 @@
 // singletons (Config, Kmp/Vmp, M, d_buf: config.h:112-115, mempool.cuh:89-91, overlap.cu:41-43),
 // the host CQueues become HBM-resident bucketed lists, and every Thrust stage is a kernel of
 // emp_kernels.hip.
+#include <errno.h>
 #include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
@@ -12,6 +13,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <sys/uio.h>
+#include <sys/vfs.h>
 #include <time.h>
 #include <unistd.h>
 
@@ -108,6 +111,18 @@ struct Pending {
     double bytes;
 };
 
+// output.bin through its descriptor. On ext4 and xfs the blocks under the next writes are
+// reserved ahead of them (fallocate, OUT_RESERVE at a time; the file is cut back to the bytes
+// written at the end): buffered writes into reserved extents skip the per-page delayed-allocation
+// work, about a sixth of the time of writing 143 MB here.
+constexpr uint64_t OUT_RESERVE = 64ull << 20;
+struct OutFile {
+    int fd = -1;
+    uint64_t off = 0;       // bytes written
+    uint64_t reserved = 0;  // file size set by the reservations
+    bool prealloc = false;
+    bool tried = false;
+};
 }  // namespace
 
 struct emp_ctx {
@@ -249,7 +264,7 @@ struct emp_ctx {
 
     // emp_run_to_file: records written while the rounds run (a writer thread), and how many of
     // the run's records are already in the file
-    FILE *out_file = nullptr;
+    OutFile *out_file = nullptr;
     uint64_t emitted = 0;
     std::vector<hipEvent_t> dma_ev;  // per round: its edges are in the pinned host arena
 
@@ -2135,7 +2150,7 @@ int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const 
 // its round's edges into the pinned host arena (its event), then goes out as write_edges_binary
 // lays it out (overlap.cu:148-161).
 struct RecordWriter {
-    FILE *f;
+    OutFile *f;
     const uint32_t *from, *to;
     struct Item {
         uint64_t n, off;
@@ -2150,7 +2165,7 @@ struct RecordWriter {
     int err = 0;
     std::thread th;
     int device;
-    RecordWriter(FILE *f_, const uint32_t *fr, const uint32_t *t, int dev) : f(f_), from(fr), to(t), device(dev) {
+    RecordWriter(OutFile *f_, const uint32_t *fr, const uint32_t *t, int dev) : f(f_), from(fr), to(t), device(dev) {
         th = std::thread([this]() {
             (void)hipSetDevice(device);
             for (;;) {
@@ -2468,13 +2483,47 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     return flush_records(c, sink, user);
 }
 
-int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to) {
-    FILE *f = (FILE *)user;
-    if (fwrite(&n, sizeof n, 1, f) != 1) return EMP_ERR_IO;
-    if (fwrite(&ovlp, sizeof ovlp, 1, f) != 1) return EMP_ERR_IO;
-    if (n && fwrite(from, sizeof(uint32_t), n, f) != n) return EMP_ERR_IO;
-    if (n && fwrite(to, sizeof(uint32_t), n, f) != n) return EMP_ERR_IO;
+// Writes the iovecs whole (writev may take part of them: at most about 2 GB per call).
+int write_all(int fd, struct iovec *iov, int cnt) {
+    while (cnt > 0) {
+        const ssize_t w = writev(fd, iov, cnt);
+        if (w < 0) {
+            if (errno == EINTR) continue;
+            return EMP_ERR_IO;
+        }
+        size_t left = (size_t)w;
+        while (cnt > 0 && left >= iov->iov_len) {
+            left -= iov->iov_len;
+            ++iov;
+            --cnt;
+        }
+        if (cnt > 0) {
+            iov->iov_base = (char *)iov->iov_base + left;
+            iov->iov_len -= left;
+        }
+    }
     return EMP_OK;
+}
+
+// One record as write_edges_binary lays it out (overlap.cu:148-161): n (8 bytes), the overlap
+// (4 bytes), from[n], to[n], in one writev. Ahead of the writes, the file's blocks are reserved
+// in steps of OUT_RESERVE (see OutFile).
+int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to) {
+    OutFile &of = *(OutFile *)user;
+    const uint64_t bytes = 12 + 8 * n;
+    if (of.prealloc && of.off + bytes > of.reserved) {
+        const uint64_t end = std::max(of.off + bytes, of.reserved + OUT_RESERVE);
+        of.tried = true;
+        if (fallocate(of.fd, 0, (off_t)of.reserved, (off_t)(end - of.reserved)) == 0) of.reserved = end;
+        else of.prealloc = false;  // the writes go on without
+    }
+    uint8_t head[12];
+    memcpy(head, &n, 8);
+    memcpy(head + 8, &ovlp, 4);
+    struct iovec iov[3] = {{head, 12}, {(void *)from, 4 * n}, {(void *)to, 4 * n}};
+    const int rc = write_all(of.fd, iov, n ? 3 : 1);
+    if (rc == EMP_OK) of.off += bytes;
+    return rc;
 }
 
 }  // namespace
@@ -2668,14 +2717,20 @@ int emp_run(emp_ctx *c, emp_record_sink sink, void *user) {
 
 int emp_run_to_file(emp_ctx *c, const char *path) {
     if (!c || !path) return EMP_ERR_ARG;
-    FILE *f = fopen(path, "wb");
-    if (!f) return fail(c, EMP_ERR_IO, "Could not open %s", path);
-    c->out_file = f;  // the back-to-back rounds write finished records while later rounds run
+    OutFile of;
+    of.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0666);  // fopen "wb"
+    if (of.fd < 0) return fail(c, EMP_ERR_IO, "Could not open %s", path);
+    struct statfs sf;
+    constexpr long EXT4_MAGIC = 0xEF53, XFS_MAGIC = 0x58465342;
+    of.prealloc = fstatfs(of.fd, &sf) == 0 && ((long)sf.f_type == EXT4_MAGIC || (long)sf.f_type == XFS_MAGIC);
+    c->out_file = &of;  // the back-to-back rounds write finished records while later rounds run
     c->emitted = 0;
-    int rc = emp_run(c, file_sink, f);
+    int rc = emp_run(c, file_sink, &of);
     c->out_file = nullptr;
     c->emitted = 0;
-    if (fclose(f) != 0 && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "closing %s failed", path);
+    if (of.tried && ftruncate(of.fd, (off_t)of.off) != 0 && rc == EMP_OK)
+        rc = fail(c, EMP_ERR_IO, "truncating %s failed", path);
+    if (close(of.fd) != 0 && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "closing %s failed", path);
     return rc;
 }
 

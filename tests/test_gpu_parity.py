@@ -246,6 +246,25 @@ def test_file_output_with_and_without_fallback(case, tmp_path):
     _check_file_output(r, o, want, tmp_path)
 
 
+def test_file_output_over_a_longer_file_spanning_reservations(tmp_path):
+    """An output.bin of more than 64 MB (the file's blocks are reserved 64 MB at a time on ext4 and
+    xfs, the last reservation cut back to the bytes written), written where a longer file was: the
+    bytes of emp_run's records and nothing after them."""
+    g = gen.genome(27 * 10**6, 91)
+    r = gen.reads(g, 8_000_000, 100, 0.5, 0.005, 92)
+    out = tmp_path / "output.bin"
+    with native.Context(min_ovlp=63) as c:
+        c.add_reads(r)
+        want = c.run()
+        assert len(want) > (64 << 20)
+        out.write_bytes(b"\xff" * (len(want) + 12345))
+        c.reset()
+        c.add_reads(r)
+        c.run_to_file(str(out))
+    assert out.stat().st_size == len(want)
+    assert out.read_bytes() == want
+
+
 def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
     """2,000,000 x 100 bp reads (4 M entries, 12-bit buckets): the back-to-back rounds (each join
     clears the counts it read) and the rounds with the host between them give the oracle's bytes
