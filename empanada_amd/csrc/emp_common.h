@@ -77,19 +77,28 @@ __device__ __forceinline__ void set_carry(Entry &e, uint32_t f) {
 
 // Wide carry (the single-job engine's round buckets, 14 bits: see WIDE_SHIFT). An entry gathers its
 // read's 16-bit unit of 8 columns when it enters a group: every eighth round per direction.
+// Forward: the unit's columns c+1 .. 7; backward: columns c-1 .. 0 in that order, which is the
+// unit with its eight 2-bit fields reversed (a bit reversal, then the two bits of each field
+// swapped back), shifted down to its last c fields. Both are computed and one is selected: the
+// direction differs between lanes, and a branch per lane costs more than the other form.
 __device__ __forceinline__ uint32_t wide_carry_of(uint32_t u, uint32_t col, bool fwd) {
-    const uint32_t c = col & 7u;
-    if (fwd) return (u & 0xFFFFu) >> (2 * (c + 1));
-    uint32_t f = 0;
-    for (uint32_t i = 0; i < c; ++i) f |= ((u >> (2 * (c - 1 - i))) & 3u) << (2 * i);
-    return f;
+    const uint32_t c = col & 7u, v = u & 0xFFFFu;
+    const uint32_t fw = v >> (2 * (c + 1));
+    uint32_t y = __builtin_bitreverse32(v) >> 16;
+    y = ((y >> 1) & 0x5555u) | ((y & 0x5555u) << 1);
+    const uint32_t bw = y >> (16 - 2 * c);
+    return fwd ? fw : bw;
 }
 
+// Whether entry rid (side sfx) enters a column group this round (forward entries at a group's
+// first column, backward ones at its last, every entry at round 0), and the group (g). The
+// round's facts are uniform; per lane only the direction is picked, as integer selects.
 __device__ __forceinline__ bool wide_needs_gather(const RoundArgs &r, uint32_t rid, bool sfx, uint32_t &g) {
-    const bool fwd = sfx != (bool)(rid & 1);
-    const uint32_t col = round_col(r, fwd);
-    g = col >> 3;
-    return fwd ? (col & 7u) == 0 : ((col & 7u) == 7 || r.col_t == 0);
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(r.col_t), cb = __builtin_amdgcn_readfirstlane(r.col_back);
+    const uint32_t nf = (ct & 7u) == 0u ? 1u : 0u, nb = ((cb & 7u) == 7u || ct == 0u) ? 1u : 0u;
+    const uint32_t fwd = (sfx ? 1u : 0u) ^ (rid & 1u);
+    g = (fwd ? ct : cb) >> 3;
+    return ((fwd & nf) | ((fwd ^ 1u) & nb)) != 0u;
 }
 
 // The 14-bit carry of a stored entry, and its h1 key with bits 21..28 restored from top8 (the top
@@ -107,15 +116,12 @@ __device__ __forceinline__ uint32_t regen_wide(const RoundArgs &r, uint32_t &h1,
     const bool odd = rid & 1;
     const bool fwd = sfx != odd;
     const uint32_t col = round_col(r, fwd);
-    uint32_t code;
-    if (gathered != 0xFFFFFFFFu) {
-        code = (gathered >> (2 * (col & 7u))) & 3u;
-        f = wide_carry_of(gathered, col, fwd);
-    } else {
-        code = f & 3u;
-        f >>= 2;
-    }
-    const uint32_t ev = enc_of_code(odd ? (code ^ 2u) : code);
+    // the gathered unit's code and carry, or the carried ones, computed both and selected
+    const bool has = gathered != 0xFFFFFFFFu;
+    const uint32_t gcode = (gathered >> (2 * (col & 7u))) & 3u, gcarry = wide_carry_of(gathered, col, fwd);
+    const uint32_t code = has ? gcode : (f & 3u);
+    f = has ? gcarry : (f >> 2);
+    const uint32_t ev = enc_of_code(code ^ (odd ? 2u : 0u));
     if (sfx)
         next_sfx(h1, h2, ev, r.m1, r.m2);
     else

@@ -81,12 +81,12 @@ __device__ __forceinline__ uint32_t sfx_slot(const NextOut &nx, uint32_t b, uint
     return d < 3u ? 3u * (ev - 1u) + d : SLOT_SFX_OTHER;
 }
 
-// h1o: the prefix's key before PrevPfx (the u64 wrap for h1o < e lands anywhere).
+// h1o: the prefix's key before PrevPfx (the u64 wrap for h1o < e lands anywhere: the fallback
+// slot; k and d are formed for it too and not used).
 __device__ __forceinline__ uint32_t pfx_slot(const NextOut &nx, uint32_t b, uint32_t h1o, uint32_t ev, uint32_t h1n) {
-    if (h1o < ev) return SLOT_PFX_OTHER;
     const uint32_t k = (h1o - ev) % 5u;
     const uint32_t d = (h1n >> nx.shift) - pfx_first(nx, b, k);
-    return d < 2u ? 12u + 2u * k + d : SLOT_PFX_OTHER;
+    return (h1o >= ev && d < 2u) ? 12u + 2u * k + d : SLOT_PFX_OTHER;
 }
 
 // Target bucket of direct slot s (< 22) of source bucket b.
@@ -128,23 +128,27 @@ __device__ __forceinline__ uint32_t next_gather(const NextOut &nx, const Entry (
     uint32_t nmask = 0, grp[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        grp[q] = 0;
-        if (((mask >> q) & 1u) && wide_needs_gather(nx.rg, e[q].rid, sfx, grp[q])) nmask |= 1u << q;
+        const uint32_t act = (mask >> q) & 1u;  // slots past the bucket hold no loaded entry
+        const bool need = wide_needs_gather(nx.rg, act ? e[q].rid : 0u, sfx, grp[q]);
+        nmask |= (act & (need ? 1u : 0u)) << q;
         tgt[q] = 0u;
     }
     if (__any(nmask != 0)) {
+        const uint32_t G8 = __builtin_amdgcn_readfirstlane(nx.rg.G);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const bool need = (nmask >> q) & 1u;
-            tgt[q] = nx.cols[need ? wide_word(nx.rg.G, grp[q], e[q].rid >> 1) : 0];
+            const size_t w = wide_word(G8, grp[q], e[q].rid >> 1);  // formed for every slot, selected
+            tgt[q] = nx.cols[(nmask >> q) & 1u ? w : 0];
         }
     }
     return nmask;
 }
 
-// The read's 16 bits of a loaded word (need), else the word with bit 16 set.
+// The read's 16 bits of a loaded word (need), else the word with bit 16 set (both formed, one
+// selected).
 __device__ __forceinline__ uint32_t gathered(uint32_t w, uint32_t rid, bool need) {
-    return need ? (w >> (16 * ((rid >> 1) & 1u))) & 0xFFFFu : w | 0x10000u;
+    const uint32_t u = (w >> (16 * ((rid >> 1) & 1u))) & 0xFFFFu, none = w | 0x10000u;
+    return need ? u : none;
 }
 
 template <int PER>
@@ -661,15 +665,16 @@ __device__ __forceinline__ uint32_t pfx_gather(const NextOut &nx, const JoinShar
     uint32_t nmask = 0, grp[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        grp[q] = 0;
-        if (((pmask >> q) & 1u) && wide_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q])) nmask |= 1u << q;
+        const bool need = wide_needs_gather(nx.rg, S.prid[threadIdx.x + NT * q], false, grp[q]);
+        nmask |= (((pmask >> q) & 1u) & (need ? 1u : 0u)) << q;
         tgt[q] = 0u;
     }
     if (__any(nmask != 0)) {
+        const uint32_t G8 = __builtin_amdgcn_readfirstlane(nx.rg.G);
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const bool need = (nmask >> q) & 1u;
-            tgt[q] = nx.cols[need ? wide_word(nx.rg.G, grp[q], S.prid[threadIdx.x + NT * q] >> 1) : 0];
+            const size_t w = wide_word(G8, grp[q], S.prid[threadIdx.x + NT * q] >> 1);
+            tgt[q] = nx.cols[(nmask >> q) & 1u ? w : 0];
         }
     }
     return nmask;
