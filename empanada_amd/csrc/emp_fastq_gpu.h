@@ -19,6 +19,8 @@ struct FqResult {
     uint64_t kept;                  // kept records
     uint32_t err;                   // FQ_ERR_FORMAT
     uint32_t kept_len;              // length of the first kept record
+    unsigned long long first_odd;   // classify with copy: first record not kept or of another length
+    uint64_t len0;                  // length of record 0's sequence line (the block chain)
 };
 
 // Exclusive scan of n counts into off[0..n] (off[n] = total); blk: fq_scan_blocks(n) scratch.
@@ -45,6 +47,16 @@ hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, uint64_t B, int rang
 hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
                               FqResult *r, uint32_t *kept, uint32_t *len, unsigned long long *rank,
                               unsigned long long *blk, uint32_t L, hipStream_t s);
+// The classify pass (kept flags, lengths, first_bad) with the rows of the records that are kept and
+// L long copied to out + record * L; first_odd: the first record that is not (none: every record
+// is a kept read of length L and the rows are final). Otherwise launch_fq_rank and launch_fq_copy
+// finish as after launch_fq_classify.
+hipError_t launch_fq_classify_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL,
+                                   uint64_t records, FqResult *r, uint32_t *kept, uint32_t *len, uint32_t L,
+                                   uint8_t *out, hipStream_t s);
+// The rank pass of launch_fq_classify (after its flags and lengths).
+hipError_t launch_fq_rank(uint64_t records, FqResult *r, const uint32_t *kept, const uint32_t *len,
+                          unsigned long long *rank, unsigned long long *blk, uint32_t L, hipStream_t s);
 // The kept reads, row-major, read i at out + i * L.
 hipError_t launch_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL, uint64_t records,
                           const uint32_t *kept, const unsigned long long *rank, uint32_t L, uint8_t *out,

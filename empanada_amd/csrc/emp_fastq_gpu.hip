@@ -171,15 +171,39 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Newlines among bytes [c0, x) of the range, c0 the start of x's chunk (x - c0 < FQ_CHUNK): the
+// wave's lanes take 16 bytes each in four steps, every step's loads issued before any is counted.
+__device__ __forceinline__ uint32_t chunk_head_newlines(const uint8_t *d, uint64_t size, uint32_t s, uint64_t c0,
+                                                        uint64_t x) {
+    uint32_t mask[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const uint64_t p = c0 + (uint64_t)FQ_PER * (threadIdx.x + 64 * st);
+        mask[st] = 0;
+        if (p < x) newlines_of(d, p, size, s, mask[st]);
+    }
+    uint32_t k = 0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const uint64_t p = c0 + (uint64_t)FQ_PER * (threadIdx.x + 64 * st);
+        const uint64_t n = x > p ? x - p : 0;
+        k += __popc(n >= 16 ? mask[st] : mask[st] & ((1u << n) - 1u));
+    }
+    return wave_sum(k);
+}
+
 // The block chain (read_block's framing, preprocess.cu:46-62,207-222), one wave. Block j starts at
 // pos_j, a record boundary (4 * records-before-it newlines precede it), and holds min(B, rest)
 // bytes; a block shorter than B is the last (when the range ends the file, `range_final`; else the
 // rest waits for more data). With `one_block` the range is one block whose finality is `one_final`.
 // Newlines before a position come from the chunk offsets plus the wave's count over that chunk.
+// The blocks follow one another (each starts where the last one's records end), so the wave's
+// time is a few dependent round trips per block: the leading '@', the chunk offset, the block's
+// last byte and the partial chunk's bytes are loaded together, then the last kept newline.
 __global__ void __launch_bounds__(64) k_fq_chain(const uint8_t *d, uint64_t size, uint64_t B, int range_final,
                                                int one_block, int one_final, const unsigned long long *choff,
                                                uint32_t nchunks, const uint64_t *nlpos, FqResult *r) {
-    const uint32_t lane = threadIdx.x;
+    const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(d) & 15u);
     const uint64_t NL = choff[nchunks];
     uint64_t pos = 0, recs = 0, blocks = 0;
     uint32_t err = 0;
@@ -192,21 +216,26 @@ __global__ void __launch_bounds__(64) k_fq_chain(const uint8_t *d, uint64_t size
             if (got != B && !range_final) break;  // the rest of the range waits for more data
             final = got != B;
         }
-        if (d[pos] != '@') {
-            err = FQ_ERR_FORMAT;
-            break;
-        }
+        const bool at = d[pos] == '@';  // checked once this block's other loads are out
         const uint64_t g0 = 4 * recs;  // newlines before pos
         uint64_t lines, use;
         if (final) {
+            if (!at) {
+                err = FQ_ERR_FORMAT;
+                break;
+            }
             lines = (NL - g0) + (d[size - 1] == '\n' ? 0 : 1);
             use = got;
         } else {
             const uint64_t x = pos + got, c = x / FQ_CHUNK;
-            uint32_t k = 0;
-            for (uint64_t p = c * FQ_CHUNK + lane; p < x; p += 64) k += d[p] == '\n' ? 1u : 0u;
-            const uint64_t nlb = choff[c] + wave_sum(k) - g0;  // newlines in [pos, x)
+            const unsigned long long cb = choff[c];
             const bool ends = d[x - 1] == '\n';
+            const uint32_t head = chunk_head_newlines(d, size, s, c * FQ_CHUNK, x);
+            if (!at) {
+                err = FQ_ERR_FORMAT;
+                break;
+            }
+            const uint64_t nlb = cb + head - g0;  // newlines in [pos, x)
             const uint64_t n_lines = nlb - (ends ? 1 : 0) + 1;
             if (n_lines % 4 == 0 && ends) {
                 lines = nlb;
@@ -226,12 +255,13 @@ __global__ void __launch_bounds__(64) k_fq_chain(const uint8_t *d, uint64_t size
         ++blocks;
         if (final || one_block) break;
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         r->newlines = NL;
         r->use = pos;
         r->records = recs;
         r->blocks = blocks;
         r->err = err;
+        r->len0 = recs ? (1 < NL ? nlpos[1] : size) - (nlpos[0] + 1) : 0;  // record 0's sequence line
     }
 }
 
@@ -283,11 +313,19 @@ __device__ __forceinline__ void flag_dword(uint32_t w, uint32_t valid, bool &dro
     bad |= (non_bases(w) & ~dr & valid) != 0;
 }
 
+__device__ __forceinline__ void copy_rows(const uint8_t *d, uint64_t size, uint64_t km, uint64_t src, uint64_t dst,
+                                          uint32_t L, uint8_t *out, uint32_t g, uint32_t sub);
+
 // Per record of the range: its sequence line's length, kept flag (non-empty, no dropping byte, all
-// ACGTacgt) and, for a kept-class line with another byte, the first such record.
+// ACGTacgt) and, for a kept-class line with another byte, the first such record. With COPY, the
+// records that are kept and L long also copy their bases to out + record * L (the lines are in
+// cache from the flag pass), and every other record is reported in first_odd: when there is none,
+// the kept reads are the records and the rows are the copy's output; otherwise the rank pass and
+// the copy run as without COPY.
+template <bool COPY>
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
                                                           uint64_t NL, uint64_t records, uint32_t *kept, uint32_t *len,
-                                                          FqResult *r) {
+                                                          FqResult *r, uint32_t L, uint8_t *out) {
     const uint32_t lane = threadIdx.x & 63, g = lane >> 4, sub = lane & 15;
     const uint64_t rec0 = (uint64_t)blockIdx.x * FQ_THREADS + (threadIdx.x & ~63u);
     if (rec0 >= records) return;  // the whole wave
@@ -356,13 +394,17 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
             mybad |= gb;
         }
     }
+    bool row = false;
     if (myrec < records) {
         const uint64_t n = e - s;
         const bool k = n > 0 && !mydrop && !mybad;
         kept[myrec] = k ? 1u : 0u;
         len[myrec] = (uint32_t)(n < 0xFFFFFFFFull ? n : 0xFFFFFFFFull);
         if (n > 0 && !mydrop && mybad) atomicMin(&r->first_bad, (unsigned long long)myrec);
+        row = k && n == L;
+        if (COPY && !row) atomicMin(&r->first_odd, (unsigned long long)myrec);
     }
+    if (COPY) copy_rows(d, size, __ballot(row), s, myrec * (uint64_t)L, L, out, g, sub);
 }
 
 // The first kept record (the only kept one of rank 0) and its length; the kept total.
@@ -399,21 +441,11 @@ __device__ __forceinline__ uint32_t unaligned_dword(const uint8_t *d, uint64_t s
     return w;
 }
 
-__global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
-                                                      uint64_t NL, uint64_t records, const uint32_t *kept,
-                                                      const unsigned long long *rank, uint32_t L, uint8_t *out) {
-    (void)NL;
-    const uint32_t lane = threadIdx.x & 63, g = lane >> 4, sub = lane & 15;
-    const uint64_t rec0 = (uint64_t)blockIdx.x * FQ_THREADS + (threadIdx.x & ~63u);
-    if (rec0 >= records) return;  // the whole wave
-    const uint64_t myrec = rec0 + lane;
-    const bool mine = myrec < records && kept[myrec] != 0;
-    uint64_t src = 0, dst = 0;
-    if (mine) {
-        src = nlpos[4 * myrec] + 1;
-        dst = rank[myrec] * (uint64_t)L;
-    }
-    const uint64_t km = __ballot(mine);
+// The rows of the wave's records with their bit set in km (lane l: record l of the wave, its bases
+// at d + src, its row at out + dst), L bytes each; g and sub: the lane's 16-lane group and place
+// in it.
+__device__ __forceinline__ void copy_rows(const uint8_t *d, uint64_t size, uint64_t km, uint64_t src, uint64_t dst,
+                                          uint32_t L, uint8_t *out, uint32_t g, uint32_t sub) {
     const uintptr_t db = reinterpret_cast<uintptr_t>(d), ob = reinterpret_cast<uintptr_t>(out);
     for (uint32_t base = 0; base < FQ_WAVE_RECS; base += 4 * FQ_BATCH) {
         if (((km >> base) & 0xFFFFull) == 0) continue;
@@ -463,6 +495,22 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, uint64
     }
 }
 
+__global__ void __launch_bounds__(FQ_THREADS) k_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
+                                                      uint64_t records, const uint32_t *kept,
+                                                      const unsigned long long *rank, uint32_t L, uint8_t *out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t rec0 = (uint64_t)blockIdx.x * FQ_THREADS + (threadIdx.x & ~63u);
+    if (rec0 >= records) return;  // the whole wave
+    const uint64_t myrec = rec0 + lane;
+    const bool mine = myrec < records && kept[myrec] != 0;
+    uint64_t src = 0, dst = 0;
+    if (mine) {
+        src = nlpos[4 * myrec] + 1;
+        dst = rank[myrec] * (uint64_t)L;
+    }
+    copy_rows(d, size, __ballot(mine), src, dst, L, out, lane >> 4, lane & 15);
+}
+
 }  // namespace
 
 uint64_t fq_chunks(uint64_t size) { return (size + FQ_CHUNK - 1) / FQ_CHUNK; }
@@ -498,10 +546,27 @@ hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *n
                               unsigned long long *blk, uint32_t L, hipStream_t s) {
     if (records == 0) return hipSuccess;
     const uint64_t groups = (records + FQ_THREADS - 1) / FQ_THREADS;  // a wave per 64 records
-    hipLaunchKernelGGL(k_fq_classify, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records,
-                       kept, len, r);
+    hipLaunchKernelGGL(k_fq_classify<false>, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL,
+                       records, kept, len, r, 0u, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    return launch_fq_rank(records, r, kept, len, rank, blk, L, s);
+}
+
+hipError_t launch_fq_classify_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL,
+                                   uint64_t records, FqResult *r, uint32_t *kept, uint32_t *len, uint32_t L,
+                                   uint8_t *out, hipStream_t s) {
+    if (records == 0) return hipSuccess;
+    const uint64_t groups = (records + FQ_THREADS - 1) / FQ_THREADS;
+    hipLaunchKernelGGL(k_fq_classify<true>, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL,
+                       records, kept, len, r, L, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fq_rank(uint64_t records, FqResult *r, const uint32_t *kept, const uint32_t *len,
+                          unsigned long long *rank, unsigned long long *blk, uint32_t L, hipStream_t s) {
+    if (records == 0) return hipSuccess;
+    hipError_t e;
     if ((e = launch_scan(kept, records, rank, blk, s)) != hipSuccess) return e;
     const uint32_t g = (uint32_t)((records + 255) / 256);
     hipLaunchKernelGGL(k_fq_first, dim3(g), dim3(256), 0, s, kept, len, rank, records, r);
@@ -514,9 +579,10 @@ hipError_t launch_fq_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos
                           const uint32_t *kept, const unsigned long long *rank, uint32_t L, uint8_t *out,
                           hipStream_t s) {
     if (records == 0) return hipSuccess;
+    (void)NL;
     const uint64_t groups = (records + FQ_THREADS - 1) / FQ_THREADS;  // a wave per 64 records
-    hipLaunchKernelGGL(k_fq_copy, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, NL, records, kept,
-                       rank, L, out);
+    hipLaunchKernelGGL(k_fq_copy, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, nlpos, records, kept, rank,
+                       L, out);
     return hipGetLastError();
 }
 

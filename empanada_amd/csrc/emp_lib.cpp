@@ -621,7 +621,7 @@ int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool 
     }
     FqResult &R = *c->h_fq_res;
     R = FqResult{};
-    R.first_bad = R.first_kept = R.first_len = ~0ull;
+    R.first_bad = R.first_kept = R.first_len = R.first_odd = ~0ull;
     HIPCHK(c, hipMemcpyAsync(c->d_fq_res, c->h_fq_res, sizeof(FqResult), hipMemcpyHostToDevice, c->stream));
     if (!on_device) HIPCHK(c, hipMemcpyAsync(c->d_fq, data, size, hipMemcpyHostToDevice, c->stream));
     {
@@ -640,19 +640,48 @@ int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool 
     if ((rc = read_result(c))) return rc;
     const uint64_t recs = R.records;
     const bool fresh = c->n == 0 && c->stage_n == 0 && c->acc_n == 0;  // the first kept read sets the length
+    // Rows written by the classify pass itself: every record a kept read of the expected length
+    // (the context's, or record 0's when the first kept read sets it), known once it has run.
+    bool rows_done = false;
     if (recs) {
         if ((rc = ensure_dev(c, &c->d_fq_kept, c->fq_rec_cap, recs))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_len, c->fq_len_cap, recs))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_rank, c->fq_rank_cap, recs + 1))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_blk, c->fq_blk_cap, fq_scan_blocks(recs)))) return rc;
-        {
-            // two newline positions and the sequence line per record (about a quarter of a record),
-            // kept flag, length and offset written, each read twice more
-            Timed t(c, "k_fastq", 0.25 * R.use + 48.0 * recs);
-            HIPCHK(c, launch_fq_classify(d, size, c->d_fq_nl, nl, recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len,
-                                         c->d_fq_rank, c->d_fq_blk, fresh ? 0u : (uint32_t)c->L, c->stream));
+        const uint64_t Lt = fresh ? R.len0 : (uint64_t)c->L;
+        if (Lt >= 1 && Lt <= (uint64_t)MAX_RD_LEN) {
+            if ((rc = ensure_dev_keep(c, &c->d_acc, c->acc_bytes, (c->acc_n + recs) * Lt))) return rc;
+            {
+                // two newline positions and the sequence line per record (about a quarter of a
+                // record), kept flag and length written, the row written
+                Timed t(c, "k_fastq", 0.25 * R.use + 24.0 * recs + (double)Lt * recs);
+                HIPCHK(c, launch_fq_classify_copy(d, size, c->d_fq_nl, nl, recs, c->d_fq_res, c->d_fq_kept,
+                                                  c->d_fq_len, (uint32_t)Lt, c->d_acc + c->acc_n * Lt, c->stream));
+            }
+            if ((rc = read_result(c))) return rc;
+            if (R.first_odd == ~0ull && R.first_bad == ~0ull) {
+                rows_done = true;
+                R.kept = recs;
+                R.first_kept = 0;
+                R.kept_len = (uint32_t)Lt;
+            } else {
+                {
+                    Timed t(c, "k_fastq", 24.0 * recs);
+                    HIPCHK(c, launch_fq_rank(recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len, c->d_fq_rank, c->d_fq_blk,
+                                             fresh ? 0u : (uint32_t)c->L, c->stream));
+                }
+                if ((rc = read_result(c))) return rc;
+            }
+        } else {
+            {
+                // two newline positions and the sequence line per record (about a quarter of a
+                // record), kept flag, length and offset written, each read twice more
+                Timed t(c, "k_fastq", 0.25 * R.use + 48.0 * recs);
+                HIPCHK(c, launch_fq_classify(d, size, c->d_fq_nl, nl, recs, c->d_fq_res, c->d_fq_kept, c->d_fq_len,
+                                             c->d_fq_rank, c->d_fq_blk, fresh ? 0u : (uint32_t)c->L, c->stream));
+            }
+            if ((rc = read_result(c))) return rc;
         }
-        if ((rc = read_result(c))) return rc;
     }
     // The first offending record in file order decides the error, as the host reader meets them
     // in order; a block that cannot be framed comes after every record of the blocks before it.
@@ -670,7 +699,7 @@ int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool 
     if (kept) {
         if ((rc = set_len(c, (int)L))) return rc;
         if ((rc = ensure_dev_keep(c, &c->d_acc, c->acc_bytes, (c->acc_n + kept) * (uint64_t)L))) return rc;
-        {
+        if (!rows_done) {
             Timed t(c, "k_fastq", 2.0 * kept * L + 24.0 * recs);
             HIPCHK(c, launch_fq_copy(d, size, c->d_fq_nl, nl, recs, c->d_fq_kept, c->d_fq_rank, L,
                                      c->d_acc + c->acc_n * (uint64_t)L, c->stream));

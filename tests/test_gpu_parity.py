@@ -520,6 +520,59 @@ def test_repeat_genome_sample_vs_oracle():
     assert got == want
 
 
+@pytest.mark.parametrize("L", [2, 37, 100, 150, 255])
+@pytest.mark.parametrize("first", ["clean", "dropped", "longer"])
+def test_fastq_clean_records_copied_by_the_classify_pass(L, first, tmp_path):
+    """FASTQ whose records are all kept reads of one length: the classify pass writes the rows
+    itself (rows of L bytes, unaligned to 4 bytes for most L), over one range or three (the later
+    ones with the context's length set). With record 0 dropped ('N') or longer than the rest, the
+    length taken from record 0 is wrong or record 0 is no read: the rank pass and the copy run
+    instead. Against the oracle, through the device, host-block and file entry points."""
+    rng = random.Random(1000 + L)
+    G = 3000 + L
+    g = "".join(rng.choice("ACGT") for _ in range(G))
+    seqs = []
+    for _ in range(2500):
+        p = rng.randrange(0, G - L + 1)
+        s = g[p:p + L]
+        seqs.append(s if rng.random() < 0.5 else bf.revcomp(s))
+    if first == "dropped":
+        seqs[0] = "N" + seqs[0][1:]
+    elif first == "longer":
+        seqs.insert(0, seqs[0] + "A")
+    data = FQ([s.encode() for s in seqs])
+    o = max(1, (2 * L) // 3)
+    rc, want, osum, _ = ol.run_fastq(data, min_ovlp=o)
+    if first == "longer":
+        assert rc == -3  # READLEN: the other lengths come after record 0's
+    else:
+        assert rc == 0
+    from empanada_amd import hipmem
+    dbuf = hipmem.DeviceBuffer.from_array(np.frombuffer(data, np.uint8))
+    path = tmp_path / "in.fastq"
+    path.write_bytes(data)
+    try:
+        for how in ("device", "device3", "host", "file"):
+            try:
+                with native.Context(min_ovlp=o, map_mem=16 * 8192) as c:
+                    if how == "host":
+                        c.add_fastq(data, block=8192)
+                    elif how == "file":
+                        c.add_fastq_file(str(path))
+                    else:
+                        c.add_fastq_device(dbuf.address, len(data), pieces=3 if how == "device3" else 1)
+                    got, code = c.run(), 0
+                    if code == 0:
+                        assert c.summary()["n_reads"] == osum["n_reads"], how
+            except native.EmpError as e:
+                got, code = None, e.code
+            assert code == EO2EMP[rc], (how, code)
+            if code == 0:
+                assert got == want, how
+    finally:
+        dbuf.free()
+
+
 EO2EMP = {0: 0, -2: -3, -3: -4, -4: -5}  # oracle status -> C ABI status
 
 
