@@ -316,6 +316,50 @@ __device__ __forceinline__ void flag_dword(uint32_t w, uint32_t valid, bool &dro
 __device__ __forceinline__ void copy_rows(const uint8_t *d, uint64_t size, uint64_t km, uint64_t src, uint64_t dst,
                                           uint32_t L, uint8_t *out, uint32_t g, uint32_t sub);
 
+// Lane i of every 16-lane row gets the value of lane (i + 1) % 16 of its row (DPP row_ror:15), or
+// of lane (i + 2) % 16 (row_ror:14). Every lane of the wave takes part.
+__device__ __forceinline__ uint32_t row_next1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x12F, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t row_next2(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x12E, 0xF, 0xF, false);
+}
+
+// Whether every aligned dword under the n bytes at d + s lies in the range (the classify loads
+// then hold the whole line).
+__device__ __forceinline__ bool line_in_range(uintptr_t db, uint64_t size, uintptr_t lo, uint64_t n) {
+    const uintptr_t a = lo & ~(uintptr_t)3;
+    return a >= db && a + ((lo - a + n + 3) & ~(uint64_t)3) <= db + size;
+}
+
+// The row of record rec (16-lane group: lane sub holds the line's aligned dwords sub + 16 j in
+// w[j], the line starting at lo) written to out + rec * L from those registers: output dword m of
+// the row's aligned part is the funnel shift of source dwords m + t and m + t + 1, which lanes
+// fetch from their row neighbours; the few bytes before and after the aligned part are copied
+// singly. row: the group writes (uniform in the group); every lane of the wave calls.
+__device__ __forceinline__ void row_from_regs(const uint8_t *d, const uint32_t (&w)[FQ_SLOTS], uintptr_t lo, uint64_t rec,
+                                              uint32_t L, uint8_t *out, uint32_t sub, bool row) {
+    const uintptr_t dk = reinterpret_cast<uintptr_t>(out) + rec * L;
+    const uint32_t h = (uint32_t)((4u - (dk & 3u)) & 3u);
+    const uint32_t hb = h < L ? h : L, nd = (L - hb) >> 2, tl = L - hb - 4 * nd;
+    const uint32_t o = (uint32_t)(lo & 3u), t = (o + hb) >> 2, rr = (o + hb) & 3u;
+#pragma unroll
+    for (int j = 0; j < FQ_SLOTS; ++j) {
+        const uint32_t wn = j + 1 < FQ_SLOTS ? w[j + 1 < FQ_SLOTS ? j + 1 : j] : 0u;
+        const uint32_t a1 = row_next1(w[j]), b1 = row_next1(wn), a2 = row_next2(w[j]), b2 = row_next2(wn);
+        const uint32_t x1 = sub == 15 ? b1 : a1, x2 = sub >= 14 ? b2 : a2;
+        const uint32_t q0 = t ? x1 : w[j], q1 = t ? x2 : x1;
+        const uint32_t m = sub + 16 * j;
+        if (row && m < nd) *reinterpret_cast<uint32_t *>(dk + hb + 4 * m) = __builtin_amdgcn_alignbyte(q1, q0, rr);
+    }
+    const uintptr_t db = reinterpret_cast<uintptr_t>(d);
+    if (row && sub < hb) reinterpret_cast<uint8_t *>(dk)[sub] = d[lo - db + sub];
+    if (row && sub >= 8 && sub - 8 < tl) {
+        const uint32_t i = hb + 4 * nd + sub - 8;
+        reinterpret_cast<uint8_t *>(dk)[i] = d[lo - db + i];
+    }
+}
+
 // Per record of the range: its sequence line's length, kept flag (non-empty, no dropping byte, all
 // ACGTacgt) and, for a kept-class line with another byte, the first such record. With COPY, the
 // records that are kept and L long also copy their bases to out + record * L (the lines are in
@@ -364,6 +408,16 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
             bd[k] = __ballot(drop);
             bb[k] = __ballot(bad);
         }
+        // the batch's clean rows straight from the loaded dwords (lines the fast path covers)
+        if (COPY && L <= 64 * FQ_SLOTS - 3) {
+#pragma unroll
+            for (int k = 0; k < FQ_BATCH; ++k) {
+                const uint64_t rec = rec0 + base + 4 * k + g, n = hi[k] - lo[k];
+                const bool flagged = ((bd[k] >> (16 * g)) & 0xFFFFull) != 0 || ((bb[k] >> (16 * g)) & 0xFFFFull) != 0;
+                const bool row = rec < records && n == L && !flagged && line_in_range(db, size, lo[k], n);
+                row_from_regs(d, w[k], lo[k], rec, L, out, sub, row);
+            }
+        }
         // lane base + 4k + g' takes its record's flags from group g' of batch k
         if (lane >= base && lane < base + 4 * FQ_BATCH) {
             const uint32_t k = (lane - base) >> 2, gg = (lane - base) & 3;
@@ -404,7 +458,11 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
         row = k && n == L;
         if (COPY && !row) atomicMin(&r->first_odd, (unsigned long long)myrec);
     }
-    if (COPY) copy_rows(d, size, __ballot(row), s, myrec * (uint64_t)L, L, out, g, sub);
+    if (COPY) {
+        // rows not written from registers: lines longer than the fast path's, or at the range's ends
+        const bool done = L <= 64 * FQ_SLOTS - 3 && line_in_range(db, size, db + s, e - s);
+        copy_rows(d, size, __ballot(row && !done), s, myrec * (uint64_t)L, L, out, g, sub);
+    }
 }
 
 // The first kept record (the only kept one of rank 0) and its length; the kept total.
