@@ -27,7 +27,7 @@ namespace {
 
 constexpr int FQ_THREADS = 256;
 constexpr int FQ_PER = 16;
-constexpr uint32_t FQ_CHUNK = FQ_THREADS * FQ_PER;  // bytes per work-group of the newline passes
+constexpr uint32_t FQ_CHUNK = FQ_THREADS * FQ_PER;  // bytes per chunk: a wave of the count and line passes
 
 // Bit 7 of each byte of w set where that byte is '\n'.
 __device__ __forceinline__ uint32_t nl_bytes(uint32_t w) {
@@ -71,15 +71,52 @@ __device__ __forceinline__ uint32_t newlines_of(const uint8_t *d, uint64_t p, ui
     return __popc(mask);
 }
 
-__global__ void __launch_bounds__(FQ_THREADS) k_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt) {
-    __shared__ uint32_t sh[FQ_THREADS / 64];
+// The count and line passes give each wave one 4 KiB chunk, each lane 64 contiguous bytes of it
+// (four 16-byte loads, five when the range is not 16-byte aligned, all in flight together): one
+// wave scan orders the lanes' newlines, with no LDS and no barrier.
+constexpr uint32_t FQ_LANE_BYTES = FQ_CHUNK / 64;
+constexpr uint32_t FQ_CHUNKS_PER_GROUP = FQ_THREADS / 64;
+
+// Newlines among bytes [p, p + 64) of the range (p a multiple of 64; s: the range's offset from
+// 16-byte alignment), bit i for byte p + i.
+__device__ __forceinline__ uint64_t newlines64(const uint8_t *d, uint64_t p, uint64_t size, uint32_t s) {
+    if (p >= 16 && p + 80 <= size) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(d + p - s);
+        const uint4 a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
+        const uint4 a4 = s ? q[4] : make_uint4(0, 0, 0, 0);
+        const uint64_t lo = (uint64_t)nl_mask16(a0.x, a0.y, a0.z, a0.w) | (uint64_t)nl_mask16(a1.x, a1.y, a1.z, a1.w) << 16 |
+                            (uint64_t)nl_mask16(a2.x, a2.y, a2.z, a2.w) << 32 | (uint64_t)nl_mask16(a3.x, a3.y, a3.z, a3.w) << 48;
+        if (!s) return lo;
+        const uint64_t hi = nl_mask16(a4.x, a4.y, a4.z, a4.w);
+        return (lo >> s) | (hi << (64 - s));
+    }
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < FQ_LANE_BYTES && p + i < size; ++i)
+        if (d[p + i] == '\n') m |= 1ull << i;
+    return m;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__global__ void __launch_bounds__(FQ_THREADS) k_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt, uint64_t nch) {
     const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(d) & 15u);
-    const uint64_t p = (uint64_t)blockIdx.x * FQ_CHUNK + (uint64_t)threadIdx.x * FQ_PER;
-    uint32_t mask;
-    const uint32_t c = p < size ? newlines_of(d, p, size, s, mask) : 0u;
+    const uint64_t ch = (uint64_t)blockIdx.x * FQ_CHUNKS_PER_GROUP + (threadIdx.x >> 6);
+    if (ch >= nch) return;  // the whole wave
+    const uint64_t p = ch * FQ_CHUNK + (uint64_t)(threadIdx.x & 63) * FQ_LANE_BYTES;
+    const uint32_t c = p < size ? (uint32_t)__popcll(newlines64(d, p, size, s)) : 0u;
     uint32_t tot;
-    block_excl_scan<FQ_THREADS>(c, sh, tot);
-    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+    wave_excl_sum(c, tot);
+    if ((threadIdx.x & 63) == 0) cnt[ch] = tot;
 }
 
 // Exclusive scan of n counts into off[0..n], off[n] = total, in three launches: offsets inside
@@ -150,16 +187,16 @@ hipError_t launch_scan(const uint32_t *cnt, uint64_t n, unsigned long long *off,
 namespace {
 
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_lines(const uint8_t *d, uint64_t size,
-                                                       const unsigned long long *choff, uint64_t *nlpos) {
-    __shared__ uint32_t sh[FQ_THREADS / 64];
+                                                       const unsigned long long *choff, uint64_t *nlpos, uint64_t nch) {
     const uint32_t s = (uint32_t)(reinterpret_cast<uintptr_t>(d) & 15u);
-    const uint64_t p = (uint64_t)blockIdx.x * FQ_CHUNK + (uint64_t)threadIdx.x * FQ_PER;
-    uint32_t mask = 0;
-    const uint32_t c = p < size ? newlines_of(d, p, size, s, mask) : 0u;
+    const uint64_t ch = (uint64_t)blockIdx.x * FQ_CHUNKS_PER_GROUP + (threadIdx.x >> 6);
+    if (ch >= nch) return;  // the whole wave
+    const uint64_t p = ch * FQ_CHUNK + (uint64_t)(threadIdx.x & 63) * FQ_LANE_BYTES;
+    uint64_t mask = p < size ? newlines64(d, p, size, s) : 0ull;
     uint32_t tot;
-    uint64_t at = choff[blockIdx.x] + block_excl_scan<FQ_THREADS>(c, sh, tot);
+    uint64_t at = choff[ch] + wave_excl_sum((uint32_t)__popcll(mask), tot);
     while (mask) {
-        const int i = __ffs(mask) - 1;
+        const int i = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
         nlpos[at++] = p + (uint64_t)i;
     }
@@ -579,7 +616,8 @@ hipError_t launch_fq_count(const uint8_t *d, uint64_t size, uint32_t *cnt, unsig
                            unsigned long long *blk, hipStream_t s) {
     const uint64_t nch = fq_chunks(size);
     if (nch) {
-        hipLaunchKernelGGL(k_fq_count, dim3((uint32_t)nch), dim3(FQ_THREADS), 0, s, d, size, cnt);
+        const uint64_t groups = (nch + FQ_CHUNKS_PER_GROUP - 1) / FQ_CHUNKS_PER_GROUP;
+        hipLaunchKernelGGL(k_fq_count, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, cnt, nch);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -590,7 +628,8 @@ hipError_t launch_fq_frame(const uint8_t *d, uint64_t size, uint64_t B, int rang
                            const unsigned long long *choff, uint64_t *nlpos, FqResult *r, hipStream_t s) {
     const uint64_t nch = fq_chunks(size);
     if (nch) {
-        hipLaunchKernelGGL(k_fq_lines, dim3((uint32_t)nch), dim3(FQ_THREADS), 0, s, d, size, choff, nlpos);
+        const uint64_t groups = (nch + FQ_CHUNKS_PER_GROUP - 1) / FQ_CHUNKS_PER_GROUP;
+        hipLaunchKernelGGL(k_fq_lines, dim3((uint32_t)groups), dim3(FQ_THREADS), 0, s, d, size, choff, nlpos, nch);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
