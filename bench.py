@@ -42,6 +42,12 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr per finished phase (the JSON line alone goes to stdout)."""
+    print("[bench %7.1f s] %s" % (time.perf_counter() - T0, what), file=sys.stderr, flush=True)
 
 
 def host_cpu():
@@ -240,12 +246,18 @@ def batched_bench(sets, plain_stride, hbm_budget, tuning_cache):
             return time.perf_counter() - t, edges
 
     t_b, edges_b, (captures, replays) = run_batch(tuning, True, True)
+    progress("batched, tuned: %.3f s" % t_b)
     t_u, edges_u, _ = run_batch(None, True, True)
+    progress("batched, untuned: %.3f s" % t_u)
     t_n, edges_n, _ = run_batch(None, False, True)
+    progress("batched, launches without graphs: %.3f s" % t_n)
     t_p, edges_p = run_plain(True)
+    progress("plain path on %d sets: %.3f s" % (len(sub), t_p))
     dev.free()
     t_bh, edges_bh, _ = run_batch(tuning, True, False)
+    progress("batched, reads from host memory: %.3f s" % t_bh)
     t_ph, edges_ph = run_plain(False)
+    progress("plain path, reads from host memory: %.3f s" % t_ph)
     jb, jp = n_sets / t_b, len(sub) / t_p
     return {
         "workload": "%d sets: genome log-uniform 5 kbp-5 Mbp, L in {100,150,250}, coverage 10-50x, 50%% RC, "
@@ -292,6 +304,7 @@ def served_bench(jobs, loads, bound_ms, tuning):
             out.append(dict(offered_jobs_per_s=round(rate, 2), achieved_jobs_per_s=round(n_jobs / res["wall"], 2),
                             p50_ms=round(st["p50_ms"], 2), p99_ms=round(st["p99_ms"], 2),
                             all_ok=all(s == 0 for s in res["status"])))
+            progress("served load %.1f jobs/s: p99 %.1f ms" % (rate, st["p99_ms"]))
     within = [x["offered_jobs_per_s"] for x in out if x["p99_ms"] <= bound_ms]
     return {"jobs_per_load": n_jobs, "arrivals": "Poisson, seed 1400 + load index", "loads": out,
             "p99_bound_ms": bound_ms,
@@ -351,6 +364,7 @@ def main():
     reads, min_ovlp = gen.workload_reads(args.workload, args.scale)
     n, L = reads.shape
     t_gen = time.perf_counter() - t_gen
+    progress("workload generated: %d x %d bp" % (n, L))
     # 2n entries must stay below M = reduce_mem / 16 (in-core order): -r 96g for the 3.1 Gbp case
     reduce_mem = (96 << 30) if ooc else (1 << 30)
     map_mem = 1 << 30
@@ -434,7 +448,9 @@ def main():
     ms_per_step, gpu_ms_region = timed(main_step, args.steps, False, before)
     summ = ctx.summary()
     # the roofline: the same steps again with HIP events around every launch on the library stream
+    progress("%d timed steps: %.3f ms per step" % (args.steps, ms_per_step))
     ms_per_step_prof, _ = timed(main_step, args.steps, True, before)
+    progress("%d profiled steps: %.3f ms per step" % (args.steps, ms_per_step_prof))
     kern = ctx.kernels()
     rounds = ctx.rounds()
     value = n * L / (ms_per_step * 1e-3) / 1e9
@@ -503,6 +519,7 @@ def main():
     cpu, same_full = (None, None)
     if not args.no_cpu and not ooc:
         cpu, same_full = cpu_baseline(fq_path, n, L, min_ovlp, reduce_mem, gpu_out)
+        progress("CPU baseline: %.4f Gbases/s, output equal: %s" % (cpu["value"], same_full))
     same_sample = same_sync = None
     if ooc and not args.no_cpu:
         cpu, same_sample = cpu_baseline_sample(args.workload, args.cpu_scale or 0.01, min_ovlp, reduce_mem)
@@ -531,6 +548,7 @@ def main():
     ooc_line = None
     if args.ooc_steps > 0 and not ooc:
         ooc_line = ooc_bench(args.ooc_steps, args.cpu_scale or 0.01, args.check, args.no_cpu)
+        progress("3.1 Gbp workload: %.1f ms per step" % ooc_line["ms_per_step"])
 
     batched = served = None
     if (args.batched_sets > 0 or args.served_jobs > 0) and not ooc:
@@ -540,6 +558,7 @@ def main():
         plain_rate = None
         tuning = None
         if args.batched_sets > 0:
+            progress("batched workload generated: %d sets" % len(sets))
             batched = batched_bench(sets[:args.batched_sets], args.plain_stride, args.batched_budget << 30,
                                     args.tuning_cache)
             batched["gen_s"] = round(t_sets, 2)
@@ -551,6 +570,7 @@ def main():
                                            args.tuning_cache)["plain_jobs_per_s"]
             loads = [float(x) * plain_rate for x in args.served_loads.split(",")]
             served = served_bench(sets[:args.served_jobs], loads, args.p99_bound_ms, tuning)
+            progress("served: %d loads" % len(loads))
         del sets
 
     line = {

@@ -7,6 +7,7 @@
 namespace emp {
 
 constexpr uint32_t FQ_ERR_FORMAT = 1;  // a block without a leading '@', or holding no complete record
+constexpr uint32_t FQ_ROW_MAX = 253;    // longest read the classify pass writes rows for (launch_fq_classify_copy)
 
 struct FqResult {
     uint64_t newlines;              // '\n' bytes in the range
@@ -48,9 +49,9 @@ hipError_t launch_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *n
                               FqResult *r, uint32_t *kept, uint32_t *len, unsigned long long *rank,
                               unsigned long long *blk, uint32_t L, hipStream_t s);
 // The classify pass (kept flags, lengths, first_bad) with the rows of the records that are kept and
-// L long copied to out + record * L; first_odd: the first record that is not (none: every record
-// is a kept read of length L and the rows are final). Otherwise launch_fq_rank and launch_fq_copy
-// finish as after launch_fq_classify.
+// L long (L <= FQ_ROW_MAX) copied to out + record * L; first_odd: the first record that is not
+// (none: every record is a kept read of length L and the rows are final). Otherwise launch_fq_rank
+// and launch_fq_copy finish as after launch_fq_classify.
 hipError_t launch_fq_classify_copy(const uint8_t *d, uint64_t size, const uint64_t *nlpos, uint64_t NL,
                                    uint64_t records, FqResult *r, uint32_t *kept, uint32_t *len, uint32_t L,
                                    uint8_t *out, hipStream_t s);

@@ -325,6 +325,7 @@ __device__ __forceinline__ void seq_line(const uint64_t *nlpos, uint64_t NL, uin
 constexpr uint32_t FQ_WAVE_RECS = 64;
 constexpr int FQ_BATCH = 4;  // records per 16-lane group in one batch
 constexpr int FQ_SLOTS = 4;  // aligned dwords per lane and record in classify: lines up to 253 bytes
+static_assert(FQ_ROW_MAX == 64 * FQ_SLOTS - 3, "the classify pass holds lines of FQ_ROW_MAX bytes");
 
 // Bit 7 of the bytes of the aligned dword at a that lie in [lo, hi) (a < hi, a + 4 > lo).
 __device__ __forceinline__ uint32_t valid_bytes(uintptr_t a, uintptr_t lo, uintptr_t hi) {
@@ -398,11 +399,11 @@ __device__ __forceinline__ void row_from_regs(const uint8_t *d, const uint32_t (
 }
 
 // Per record of the range: its sequence line's length, kept flag (non-empty, no dropping byte, all
-// ACGTacgt) and, for a kept-class line with another byte, the first such record. With COPY, the
-// records that are kept and L long also copy their bases to out + record * L (the lines are in
-// cache from the flag pass), and every other record is reported in first_odd: when there is none,
-// the kept reads are the records and the rows are the copy's output; otherwise the rank pass and
-// the copy run as without COPY.
+// ACGTacgt) and, for a kept-class line with another byte, the first such record. With COPY (L at
+// most FQ_ROW_MAX), the records that are kept and L long also write their rows to out + record * L
+// from the dwords the batch holds, and every other record is reported in first_odd: when there is
+// none, the kept reads are the records and the rows are final; otherwise the rank pass and the
+// copy run as without COPY.
 template <bool COPY>
 __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, uint64_t size, const uint64_t *nlpos,
                                                           uint64_t NL, uint64_t records, uint32_t *kept, uint32_t *len,
@@ -415,11 +416,12 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
     if (myrec < records) seq_line(nlpos, NL, size, myrec, s, e);
     const uintptr_t db = reinterpret_cast<uintptr_t>(d);
     bool mydrop = false, mybad = false;
-    for (uint32_t base = 0; base < FQ_WAVE_RECS; base += 4 * FQ_BATCH) {
-        uint32_t w[FQ_BATCH][FQ_SLOTS];
-        uintptr_t lo[FQ_BATCH], hi[FQ_BATCH];
+    constexpr int BATCH = COPY ? 2 : FQ_BATCH;  // the rows' registers leave room for two records per group
+    for (uint32_t base = 0; base < FQ_WAVE_RECS; base += 4 * BATCH) {
+        uint32_t w[BATCH][FQ_SLOTS];
+        uintptr_t lo[BATCH], hi[BATCH];
 #pragma unroll
-        for (int k = 0; k < FQ_BATCH; ++k) {
+        for (int k = 0; k < BATCH; ++k) {
             const uint32_t rl = base + 4 * k + g;
             lo[k] = db + (uintptr_t)__shfl((unsigned long long)s, (int)rl, 64);
             hi[k] = db + (uintptr_t)__shfl((unsigned long long)e, (int)rl, 64);
@@ -430,9 +432,9 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
                 w[k][j] = in ? *reinterpret_cast<const uint32_t *>(a) : 0u;
             }
         }
-        uint64_t bd[FQ_BATCH], bb[FQ_BATCH];
+        uint64_t bd[BATCH], bb[BATCH];
 #pragma unroll
-        for (int k = 0; k < FQ_BATCH; ++k) {
+        for (int k = 0; k < BATCH; ++k) {
             bool drop = false, bad = false;
 #pragma unroll
             for (int j = 0; j < FQ_SLOTS; ++j) {
@@ -446,9 +448,9 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
             bb[k] = __ballot(bad);
         }
         // the batch's clean rows straight from the loaded dwords (lines the fast path covers)
-        if (COPY && L <= 64 * FQ_SLOTS - 3) {
+        if (COPY) {
 #pragma unroll
-            for (int k = 0; k < FQ_BATCH; ++k) {
+            for (int k = 0; k < BATCH; ++k) {
                 const uint64_t rec = rec0 + base + 4 * k + g, n = hi[k] - lo[k];
                 const bool flagged = ((bd[k] >> (16 * g)) & 0xFFFFull) != 0 || ((bb[k] >> (16 * g)) & 0xFFFFull) != 0;
                 const bool row = rec < records && n == L && !flagged && line_in_range(db, size, lo[k], n);
@@ -456,11 +458,11 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
             }
         }
         // lane base + 4k + g' takes its record's flags from group g' of batch k
-        if (lane >= base && lane < base + 4 * FQ_BATCH) {
+        if (lane >= base && lane < base + 4 * BATCH) {
             const uint32_t k = (lane - base) >> 2, gg = (lane - base) & 3;
             uint64_t xd = bd[0], xb = bb[0];
 #pragma unroll
-            for (int q = 1; q < FQ_BATCH; ++q)
+            for (int q = 1; q < BATCH; ++q)
                 if ((int)k == q) { xd = bd[q]; xb = bb[q]; }
             mydrop = ((xd >> (16 * gg)) & 0xFFFFull) != 0;
             mybad = ((xb >> (16 * gg)) & 0xFFFFull) != 0;
@@ -492,13 +494,10 @@ __global__ void __launch_bounds__(FQ_THREADS) k_fq_classify(const uint8_t *d, ui
         kept[myrec] = k ? 1u : 0u;
         len[myrec] = (uint32_t)(n < 0xFFFFFFFFull ? n : 0xFFFFFFFFull);
         if (n > 0 && !mydrop && mybad) atomicMin(&r->first_bad, (unsigned long long)myrec);
-        row = k && n == L;
+        // a record that is not a kept read of length L, or whose row the batch did not write
+        // (its line at the range's very end), sends the range to the rank pass and the copy
+        row = k && n == L && line_in_range(db, size, db + s, n);
         if (COPY && !row) atomicMin(&r->first_odd, (unsigned long long)myrec);
-    }
-    if (COPY) {
-        // rows not written from registers: lines longer than the fast path's, or at the range's ends
-        const bool done = L <= 64 * FQ_SLOTS - 3 && line_in_range(db, size, db + s, e - s);
-        copy_rows(d, size, __ballot(row && !done), s, myrec * (uint64_t)L, L, out, g, sub);
     }
 }
 

@@ -649,7 +649,7 @@ int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool 
         if ((rc = ensure_dev(c, &c->d_fq_rank, c->fq_rank_cap, recs + 1))) return rc;
         if ((rc = ensure_dev(c, &c->d_fq_blk, c->fq_blk_cap, fq_scan_blocks(recs)))) return rc;
         const uint64_t Lt = fresh ? R.len0 : (uint64_t)c->L;
-        if (Lt >= 1 && Lt <= (uint64_t)MAX_RD_LEN) {
+        if (Lt >= 1 && Lt <= (uint64_t)FQ_ROW_MAX) {
             if ((rc = ensure_dev_keep(c, &c->d_acc, c->acc_bytes, (c->acc_n + recs) * Lt))) return rc;
             {
                 // two newline positions and the sequence line per record (about a quarter of a
