@@ -37,6 +37,7 @@
 #include "emp_kernels.h"
 #include "emp_ooc.h"
 #include "emp_ooc_plan.h"
+#include "emp_outfile.h"
 
 using namespace emp;
 
@@ -111,18 +112,6 @@ struct Pending {
     double bytes;
 };
 
-// output.bin through its descriptor. On ext4 and xfs the blocks under the next writes are
-// reserved ahead of them (fallocate, OUT_RESERVE at a time; the file is cut back to the bytes
-// written at the end): buffered writes into reserved extents skip the per-page delayed-allocation
-// work, about a sixth of the time of writing 143 MB here.
-constexpr uint64_t OUT_RESERVE = 64ull << 20;
-struct OutFile {
-    int fd = -1;
-    uint64_t off = 0;       // bytes written
-    uint64_t reserved = 0;  // file size set by the reservations
-    bool prealloc = false;
-    bool tried = false;
-};
 }  // namespace
 
 struct emp_ctx {
@@ -266,6 +255,7 @@ struct emp_ctx {
     // the run's records are already in the file
     OutFile *out_file = nullptr;
     uint64_t emitted = 0;
+    std::unique_ptr<DirectIO> dio;   // its direct writes' AIO context and bounce buffers
     std::vector<hipEvent_t> dma_ev;  // per round: its edges are in the pinned host arena
 
     // statistics
@@ -1216,6 +1206,8 @@ int ensure_arena(emp_ctx *c, uint64_t cap, emp_record_sink sink, void *user) {
     if (cap <= c->out_cap) return EMP_OK;
     int rc = flush_records(c, sink, user);
     if (rc) return rc;
+    // the output file may still read records from the arena (chunks not written yet)
+    if (c->out_file && c->out_file->settle() != EMP_OK) return fail(c, EMP_ERR_IO, "writing the output file failed");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     dfree(c, c->d_out_from);
     dfree(c, c->d_out_to);
@@ -2206,8 +2198,22 @@ struct RecordWriter {
                     it = q[head++];
                 }
                 if (err) continue;
-                if (hipEventSynchronize(it.ev) != hipSuccess) err = EMP_ERR_HIP;
-                else err = file_sink(f, it.n, it.ovlp, from + it.off, to + it.off);
+                if (hipEventSynchronize(it.ev) != hipSuccess) {
+                    err = EMP_ERR_HIP;
+                    continue;
+                }
+                err = f->append(it.n, it.ovlp, from + it.off, to + it.off);
+                // every later record already in host memory joins this write (the direct side
+                // takes a share of the file while the buffered writes lag behind)
+                for (;;) {
+                    {
+                        std::lock_guard<std::mutex> g(mu);
+                        if (err || head == q.size() || hipEventQuery(q[head].ev) != hipSuccess) break;
+                        it = q[head++];
+                    }
+                    err = f->append(it.n, it.ovlp, from + it.off, to + it.off);
+                }
+                if (!err) err = f->flush();
             }
         });
     }
@@ -2310,6 +2316,8 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         (void)hipStreamSynchronize(c->ostream);
         (void)hipStreamSynchronize(c->dstream);
         if (writer && writer->finish()) return fail(c, EMP_ERR_IO, "writing the output file failed");
+        // the synchronous rerun refills the arena: nothing may read it for the file any more
+        if (c->out_file && c->out_file->settle() != EMP_OK) return fail(c, EMP_ERR_IO, "writing the output file failed");
         return RUN_FALLBACK;
     };
     if (!c->h_rec) {
@@ -2512,47 +2520,12 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     return flush_records(c, sink, user);
 }
 
-// Writes the iovecs whole (writev may take part of them: at most about 2 GB per call).
-int write_all(int fd, struct iovec *iov, int cnt) {
-    while (cnt > 0) {
-        const ssize_t w = writev(fd, iov, cnt);
-        if (w < 0) {
-            if (errno == EINTR) continue;
-            return EMP_ERR_IO;
-        }
-        size_t left = (size_t)w;
-        while (cnt > 0 && left >= iov->iov_len) {
-            left -= iov->iov_len;
-            ++iov;
-            --cnt;
-        }
-        if (cnt > 0) {
-            iov->iov_base = (char *)iov->iov_base + left;
-            iov->iov_len -= left;
-        }
-    }
-    return EMP_OK;
-}
-
 // One record as write_edges_binary lays it out (overlap.cu:148-161): n (8 bytes), the overlap
-// (4 bytes), from[n], to[n], in one writev. Ahead of the writes, the file's blocks are reserved
-// in steps of OUT_RESERVE (see OutFile).
+// (4 bytes), from[n], to[n], appended to the output file (emp_outfile.h).
 int file_sink(void *user, uint64_t n, int32_t ovlp, const uint32_t *from, const uint32_t *to) {
-    OutFile &of = *(OutFile *)user;
-    const uint64_t bytes = 12 + 8 * n;
-    if (of.prealloc && of.off + bytes > of.reserved) {
-        const uint64_t end = std::max(of.off + bytes, of.reserved + OUT_RESERVE);
-        of.tried = true;
-        if (fallocate(of.fd, 0, (off_t)of.reserved, (off_t)(end - of.reserved)) == 0) of.reserved = end;
-        else of.prealloc = false;  // the writes go on without
-    }
-    uint8_t head[12];
-    memcpy(head, &n, 8);
-    memcpy(head + 8, &ovlp, 4);
-    struct iovec iov[3] = {{head, 12}, {(void *)from, 4 * n}, {(void *)to, 4 * n}};
-    const int rc = write_all(of.fd, iov, n ? 3 : 1);
-    if (rc == EMP_OK) of.off += bytes;
-    return rc;
+    OutFile *f = static_cast<OutFile *>(user);
+    const int rc = f->append(n, ovlp, from, to);
+    return rc ? rc : f->flush();
 }
 
 }  // namespace
@@ -2747,19 +2720,14 @@ int emp_run(emp_ctx *c, emp_record_sink sink, void *user) {
 int emp_run_to_file(emp_ctx *c, const char *path) {
     if (!c || !path) return EMP_ERR_ARG;
     OutFile of;
-    of.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0666);  // fopen "wb"
-    if (of.fd < 0) return fail(c, EMP_ERR_IO, "Could not open %s", path);
-    struct statfs sf;
-    constexpr long EXT4_MAGIC = 0xEF53, XFS_MAGIC = 0x58465342;
-    of.prealloc = fstatfs(of.fd, &sf) == 0 && ((long)sf.f_type == EXT4_MAGIC || (long)sf.f_type == XFS_MAGIC);
+    if (!c->dio) c->dio.reset(new DirectIO());
+    if (of.open(path, c->dio.get()) != EMP_OK) return fail(c, EMP_ERR_IO, "Could not open %s", path);
     c->out_file = &of;  // the back-to-back rounds write finished records while later rounds run
     c->emitted = 0;
     int rc = emp_run(c, file_sink, &of);
     c->out_file = nullptr;
     c->emitted = 0;
-    if (of.tried && ftruncate(of.fd, (off_t)of.off) != 0 && rc == EMP_OK)
-        rc = fail(c, EMP_ERR_IO, "truncating %s failed", path);
-    if (close(of.fd) != 0 && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "closing %s failed", path);
+    if (of.finish() != EMP_OK && rc == EMP_OK) rc = fail(c, EMP_ERR_IO, "writing %s failed", path);
     return rc;
 }
 
