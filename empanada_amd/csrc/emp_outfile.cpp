@@ -32,10 +32,12 @@ DirectIO::~DirectIO() {
 
 bool DirectIO::init() {
     if (ready) return true;
-    aio_context_t c = 0;
-    if (syscall(SYS_io_setup, NBUF, &c) != 0) return false;
-    ctx = (unsigned long)c;
-    for (int i = 0; i < NBUF; ++i) {
+    if (!ctx) {
+        aio_context_t c = 0;
+        if (syscall(SYS_io_setup, NBUF, &c) != 0) return false;
+        ctx = (unsigned long)c;
+    }
+    while (bufs.size() < (size_t)NBUF) {
         void *p = nullptr;
         if (posix_memalign(&p, 4096, OutFile::CHUNK) != 0) return false;
         bufs.push_back((uint8_t *)p);
