@@ -31,6 +31,7 @@ DirectIO::~DirectIO() {
 }
 
 bool DirectIO::init() {
+    if (broken) return false;
     if (ready) return true;
     if (!ctx) {
         aio_context_t c = 0;
@@ -160,10 +161,14 @@ void OutFile::pieces_of(uint64_t lo, uint64_t hi, std::vector<Piece> &out) {
 int OutFile::gather_write(uint64_t lo, uint64_t hi) {
     std::vector<Piece> ps;
     pieces_of(lo, hi, ps);  // the writer thread is the only one that changes pieces_
+    return pwrite_pieces(ps, lo);
+}
+
+// The pieces' bytes, back to back, at file offset off (buffered).
+int OutFile::pwrite_pieces(const std::vector<Piece> &ps, uint64_t off) {
     constexpr size_t MAXV = 1024;
     struct iovec iov[MAXV];
     size_t i = 0;
-    uint64_t off = lo;
     while (i < ps.size()) {
         int cnt = 0;
         for (; i < ps.size() && cnt < (int)MAXV; ++i, ++cnt) iov[cnt] = iovec{(void *)ps[i].p, (size_t)ps[i].len};
@@ -191,37 +196,50 @@ int OutFile::gather_write(uint64_t lo, uint64_t hi) {
 }
 
 // Direct side: copies each queued chunk into a free bounce buffer and submits it; reaps
-// completions while it waits for buffers or for more chunks.
+// completions while it waits for buffers or for more chunks. A chunk whose direct write fails (at
+// submission or at completion) is written again, buffered, from its bounce buffer (still intact);
+// if the completions cannot be read at all, every chunk in flight is, and from then on every chunk
+// the helper takes is written buffered from the records. Only a failed buffered write fails the
+// file.
 void OutFile::helper() {
     const aio_context_t ctx = (aio_context_t)dio_->ctx;
     struct io_event ev[NBUF];
     std::vector<int> freeb;
     for (int i = 0; i < NBUF; ++i) freeb.push_back(i);
+    uint64_t chunk_of[NBUF] = {};
+    bool busy[NBUF] = {};
     int inflight = 0, err = 0;
+    bool broken = false;
     std::vector<Piece> ps;
+    auto finished = [&](uint64_t n) {
+        std::lock_guard<std::mutex> g(mu_);
+        written_ += n;
+        cv_.notify_all();
+    };
+    auto rewrite = [&](int bi) {  // the chunk in bounce buffer bi, buffered
+        const std::vector<Piece> one{Piece{chunk_of[bi] * CHUNK, dio_->bufs[bi], CHUNK}};
+        if (pwrite_pieces(one, chunk_of[bi] * CHUNK) != EMP_OK) err = EMP_ERR_IO;
+    };
     auto reap = [&](long min_nr, struct timespec *to) {
         const long r = syscall(SYS_io_getevents, ctx, min_nr, (long)NBUF, ev, to);
         if (r < 0) {
             if (errno == EINTR) return;
-            // the completions cannot be read: the writes in flight count as finished, failed,
-            // and their buffers are not used again
-            err = EMP_ERR_IO;
-            std::lock_guard<std::mutex> g(mu_);
-            written_ += (uint64_t)inflight;
+            for (int i = 0; i < NBUF; ++i)
+                if (busy[i]) rewrite(i);  // the buffers stay out of use: a write may still read them
+            finished((uint64_t)inflight);
             inflight = 0;
-            cv_.notify_all();
+            broken = true;
+            dio_->broken = true;  // it may still hold events of this file's writes
             return;
         }
         for (long i = 0; i < r; ++i) {
-            if ((long long)ev[i].res != (long long)CHUNK) err = EMP_ERR_IO;
-            freeb.push_back((int)ev[i].data);
+            const int bi = (int)ev[i].data;
+            if ((long long)ev[i].res != (long long)CHUNK) rewrite(bi);
+            busy[bi] = false;
+            freeb.push_back(bi);
         }
         inflight -= (int)r;
-        if (r > 0) {
-            std::lock_guard<std::mutex> g(mu_);
-            written_ += (uint64_t)r;
-            cv_.notify_all();
-        }
+        if (r > 0) finished((uint64_t)r);
     };
     for (;;) {
         uint64_t c = 0;
@@ -233,6 +251,7 @@ void OutFile::helper() {
                 c = queue_.front();
                 queue_.pop_front();
                 have = true;
+                pieces_of(c * CHUNK, (c + 1) * CHUNK, ps);
             } else if (done_ && inflight == 0) {
                 stop = true;
             }
@@ -243,38 +262,33 @@ void OutFile::helper() {
             reap(1, &to);
             continue;
         }
-        while (freeb.empty() && inflight > 0) reap(1, nullptr);
-        bool ok = !err && !freeb.empty();
-        if (ok) {
-            const int bi = freeb.back();
-            freeb.pop_back();
-            uint8_t *buf = dio_->bufs[bi];
-            {
-                std::lock_guard<std::mutex> g(mu_);
-                pieces_of(c * CHUNK, (c + 1) * CHUNK, ps);
-            }
-            for (const Piece &p : ps) memcpy(buf + (p.off - c * CHUNK), p.p, p.len);
-            struct iocb cb;
-            memset(&cb, 0, sizeof cb);
-            cb.aio_data = (uint64_t)bi;
-            cb.aio_lio_opcode = IOCB_CMD_PWRITE;
-            cb.aio_fildes = (uint32_t)dfd_;
-            cb.aio_buf = (uint64_t)(uintptr_t)buf;
-            cb.aio_nbytes = CHUNK;
-            cb.aio_offset = (int64_t)(c * CHUNK);
-            struct iocb *pcb = &cb;
-            if (syscall(SYS_io_submit, ctx, 1L, &pcb) == 1) {
-                ++inflight;
-            } else {
-                freeb.push_back(bi);
-                ok = false;
-            }
+        while (!broken && freeb.empty() && inflight > 0) reap(1, nullptr);
+        if (broken || freeb.empty()) {
+            if (pwrite_pieces(ps, c * CHUNK) != EMP_OK) err = EMP_ERR_IO;
+            finished(1);
+            continue;
         }
-        if (!ok) {  // counted as finished so that nothing waits for it; the file write fails
-            err = EMP_ERR_IO;
-            std::lock_guard<std::mutex> g(mu_);
-            ++written_;
-            cv_.notify_all();
+        const int bi = freeb.back();
+        freeb.pop_back();
+        uint8_t *buf = dio_->bufs[bi];
+        for (const Piece &p : ps) memcpy(buf + (p.off - c * CHUNK), p.p, p.len);
+        chunk_of[bi] = c;
+        struct iocb cb;
+        memset(&cb, 0, sizeof cb);
+        cb.aio_data = (uint64_t)bi;
+        cb.aio_lio_opcode = IOCB_CMD_PWRITE;
+        cb.aio_fildes = (uint32_t)dfd_;
+        cb.aio_buf = (uint64_t)(uintptr_t)buf;
+        cb.aio_nbytes = CHUNK;
+        cb.aio_offset = (int64_t)(c * CHUNK);
+        struct iocb *pcb = &cb;
+        if (syscall(SYS_io_submit, ctx, 1L, &pcb) == 1) {
+            busy[bi] = true;
+            ++inflight;
+        } else {
+            rewrite(bi);
+            freeb.push_back(bi);
+            finished(1);
         }
     }
     std::lock_guard<std::mutex> g(mu_);

@@ -19,6 +19,7 @@ namespace emp {
 struct DirectIO {
     unsigned long ctx = 0;  // aio_context_t
     bool ready = false;
+    bool broken = false;  // its completions could not be read: not used again
     std::vector<uint8_t *> bufs;
     ~DirectIO();
     bool init();
@@ -31,7 +32,8 @@ struct DirectIO {
 // O_DIRECT through Linux AIO instead: a helper thread copies each such chunk into an aligned
 // bounce buffer and submits it; the submission returns before the device has the data, so the
 // inode lock is not held while it moves. Chunks are page-aligned and each is written by one side
-// only, so the page cache never holds a page that a direct write also covers.
+// only, so the page cache never holds a page that a direct write also covers (a chunk whose direct
+// write fails is written again, buffered, with the same bytes).
 //
 // Direct writes need every byte below the chunk's end inside the file's size (no extending
 // writes): the blocks ahead of the writes are reserved (fallocate) in OUT_RESERVE steps, as for
@@ -75,6 +77,7 @@ private:
     int write_buffered(uint64_t limit);
     int drain();
     int gather_write(uint64_t lo, uint64_t hi);
+    int pwrite_pieces(const std::vector<Piece> &ps, uint64_t off);
     void pieces_of(uint64_t lo, uint64_t hi, std::vector<Piece> &out);
     void helper();
     int stop_helper();
