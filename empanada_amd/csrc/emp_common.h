@@ -242,8 +242,20 @@ __device__ __forceinline__ uint64_t key_of(const Entry &e) {
     return ((uint64_t)(e.h1 & KEY_MASK) << 32) | (e.h2 & KEY_MASK);
 }
 
-// Next-round target slots (NextOut). The entries of one bucket of width w = 2^src_shift land in a
-// few buckets of the next round (width 2^shift >= w), found from the entry's removed base e
+// Bucket sets: a set of nb buckets puts key h (< 2^29) into bucket floor(h nb / 2^29), the product
+// taken as the high word of (8 h) nb; for nb = 2^k that is h >> (29 - k). Buckets are in key order,
+// and when nb is a multiple of 256 every key of a bucket has the same top 8 bits (h >> 21 =
+// floor(b / (nb / 256)) by floor(floor(x) / m) = floor(x / m)), the bits a round's entries carry
+// codes in.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t h, uint32_t nb) { return __umulhi(h << 3, nb); }
+
+// Smallest key of bucket b: ceil(b 2^29 / nb).
+__device__ __forceinline__ uint32_t bucket_lo(uint32_t b, uint32_t nb) {
+    return (uint32_t)((((uint64_t)b << 29) + nb - 1u) / nb);
+}
+
+// Next-round target slots (NextOut). The entries of one bucket of width w = 2^29 / src_nb land in
+// a few buckets of the next round (width 2^29 / nb >= w), found from the entry's removed base e
 // (encoding 1..4) and, for prefixes, k = (h1 - e) mod 5:
 //   slots 0..11   suffixes: 3 (e - 1) + d, d = target - first target of base e (mod the bucket
 //                 count; d <= 2, the third only when the shifted range wraps past P1);
@@ -342,15 +354,14 @@ __device__ __forceinline__ bool has_u32(const uint32_t *r, uint32_t lo, uint32_t
 
 // Collects the entries of bucket b (slots + spilled records) into key/value scratch, padded with
 // sentinels to n2. as_rank: store rid_rank(rid) (join) or rid (dedup).
-// wshift >= 0: the entries are wide (round buckets of the single-job engine, bucket = h1 >> wshift)
-// and their keys are restored from the bucket index.
+// top8 >= 0: the entries are wide (round buckets of the single-job engine) and their keys' bits
+// 21..28 are top8, the bucket's.
 __device__ inline void gather_bucket(const Entry *slots, uint32_t cnt, uint32_t cap, const SpillRec *spill,
                               const uint32_t *spill_n, uint32_t spill_cap, uint32_t b, uint64_t *key, uint32_t *val,
-                              uint32_t n2, bool as_rank, uint32_t *s_pos, int32_t wshift = -1) {
+                              uint32_t n2, bool as_rank, uint32_t *s_pos, int32_t top8 = -1) {
     const uint32_t nm = min(cnt, cap);
-    const uint32_t top8 = wshift >= 0 ? b >> (WIDE_SHIFT - wshift) : 0u;
     auto kof = [&](const Entry &e) {
-        return wshift >= 0 ? ((uint64_t)wide_key1(e.h1, top8) << 32) | (e.h2 & KEY_MASK) : key_of(e);
+        return top8 >= 0 ? ((uint64_t)wide_key1(e.h1, (uint32_t)top8) << 32) | (e.h2 & KEY_MASK) : key_of(e);
     };
     for (uint32_t i = threadIdx.x; i < nm; i += blockDim.x) {
         Entry e = slots[(size_t)b * cap + i];

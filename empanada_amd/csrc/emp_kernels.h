@@ -84,8 +84,10 @@ struct RoundArgs {
 // atomic of its own.
 struct NextOut {
     int32_t active;          // 0: no next round (survivors are only counted)
-    int32_t shift;           // target bucket = (h1 & KEY_MASK) >> shift
-    int32_t src_shift;       // source buckets of the writing launch: h1 >> src_shift
+    uint32_t nb;             // its bucket count: target bucket of key h = bucket_of(h, nb)
+    uint32_t src_nb;         // bucket count of the writing launch's source buckets
+    int32_t shift;           // 29 - log2(nb) when nb is a power of two (the out-of-core kernels)
+    int32_t src_shift;       // the same for src_nb
     RoundArgs rg;            // the next round
     Entry *fine[2];          // its buckets per side, BUCKET_CAP slots each
     uint32_t *cnt[2];        // their counts

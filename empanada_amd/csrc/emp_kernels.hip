@@ -61,38 +61,47 @@ __device__ __forceinline__ uint32_t carry_after(const uint32_t *__restrict__ col
 // NSLOT): count (every thread: gathers, regenerates, finds its entries' slots and ranks them per
 // wave), reserve (one thread per slot: one run per target bucket), write.
 
-// First target bucket of the suffixes of source bucket b whose removed base has encoding ev:
-// the bucket of its smallest key minus NextSfx's subtrahend e * R1^(l-1) mod P1.
-__device__ __forceinline__ uint32_t sfx_first(const NextOut &nx, uint32_t b, uint32_t ev) {
-    const uint32_t lo = b << nx.src_shift, c = red1(ev * nx.rg.m1);
-    return (lo >= c ? lo - c : lo + P1 - c) >> nx.shift;
+// The functions below take the source bucket as its smallest key lo (bucket_lo, once per
+// work-group).
+
+// First target bucket of the suffixes of the source bucket whose removed base has encoding ev: the
+// bucket of its smallest key minus NextSfx's subtrahend e * R1^(l-1) mod P1.
+__device__ __forceinline__ uint32_t sfx_first(const NextOut &nx, uint32_t lo, uint32_t ev) {
+    const uint32_t c = red1(ev * nx.rg.m1);
+    return bucket_of(lo >= c ? lo - c : lo + P1 - c, nx.nb);
 }
 
-// First target bucket of the prefixes of source bucket b with (h1 - e) mod 5 == k: PrevPfx gives
+// First target bucket of the prefixes of the source bucket with (h1 - e) mod 5 == k: PrevPfx gives
 // (h1 - e + k P1) / 5, smallest for the smallest h1 - e (h1 >= lo, e <= 4).
-__device__ __forceinline__ uint32_t pfx_first(const NextOut &nx, uint32_t b, uint32_t k) {
-    const uint32_t lo = b << nx.src_shift, u = lo >= 4u ? lo - 4u : 0u;
-    return ((u + k * P1) / 5u) >> nx.shift;
+__device__ __forceinline__ uint32_t pfx_first(const NextOut &nx, uint32_t lo, uint32_t k) {
+    const uint32_t u = lo >= 4u ? lo - 4u : 0u;
+    return bucket_of((u + k * P1) / 5u, nx.nb);
 }
 
-__device__ __forceinline__ uint32_t sfx_slot(const NextOut &nx, uint32_t b, uint32_t ev, uint32_t h1n) {
-    const uint32_t nbmask = (1u << (29 - nx.shift)) - 1u;
-    const uint32_t d = ((h1n >> nx.shift) - sfx_first(nx, b, ev)) & nbmask;
+// d: the target's distance from the first target of base ev, modulo the bucket count (the shifted
+// range can wrap past P1 to bucket 0).
+__device__ __forceinline__ uint32_t sfx_slot(const NextOut &nx, uint32_t lo, uint32_t ev, uint32_t h1n) {
+    const uint32_t t = bucket_of(h1n, nx.nb), f = sfx_first(nx, lo, ev);
+    const uint32_t d = t >= f ? t - f : t + nx.nb - f;
     return d < 3u ? 3u * (ev - 1u) + d : SLOT_SFX_OTHER;
 }
 
 // h1o: the prefix's key before PrevPfx (the u64 wrap for h1o < e lands anywhere: the fallback
 // slot; k and d are formed for it too and not used).
-__device__ __forceinline__ uint32_t pfx_slot(const NextOut &nx, uint32_t b, uint32_t h1o, uint32_t ev, uint32_t h1n) {
+__device__ __forceinline__ uint32_t pfx_slot(const NextOut &nx, uint32_t lo, uint32_t h1o, uint32_t ev, uint32_t h1n) {
     const uint32_t k = (h1o - ev) % 5u;
-    const uint32_t d = (h1n >> nx.shift) - pfx_first(nx, b, k);
+    const uint32_t d = bucket_of(h1n, nx.nb) - pfx_first(nx, lo, k);
     return (h1o >= ev && d < 2u) ? 12u + 2u * k + d : SLOT_PFX_OTHER;
 }
 
-// Target bucket of direct slot s (< 22) of source bucket b.
-__device__ __forceinline__ uint32_t slot_bucket(const NextOut &nx, uint32_t b, uint32_t s) {
-    if (s < 12u) return (sfx_first(nx, b, s / 3u + 1u) + s % 3u) & ((1u << (29 - nx.shift)) - 1u);
-    return pfx_first(nx, b, (s - 12u) / 2u) + (s - 12u) % 2u;
+// Target bucket of direct slot s (< 22) of the source bucket (called only for a slot that holds
+// entries, so the bucket exists).
+__device__ __forceinline__ uint32_t slot_bucket(const NextOut &nx, uint32_t lo, uint32_t s) {
+    if (s < 12u) {
+        const uint32_t t = sfx_first(nx, lo, s / 3u + 1u) + s % 3u;
+        return t >= nx.nb ? t - nx.nb : t;
+    }
+    return pfx_first(nx, lo, (s - 12u) / 2u) + (s - 12u) % 2u;
 }
 
 // Stores entry e of side `side` at position pos of bucket nb (past the capacity: the spill area).
@@ -163,7 +172,7 @@ __device__ __forceinline__ void next_gather_finish(const Entry (&e)[PER], uint32
 // out of the write step, whose stores would otherwise all wait for them.
 __device__ __forceinline__ uint32_t slot_rank(const NextOut &nx, uint32_t *row, uint32_t slot, uint32_t h1n, uint32_t side) {
     if (slot < 22u) return atomicAdd(row + slot, 1u);
-    const uint32_t pos = atomicAdd((side ? nx.cnt[1] : nx.cnt[0]) + (h1n >> nx.shift), 1u);
+    const uint32_t pos = atomicAdd((side ? nx.cnt[1] : nx.cnt[0]) + bucket_of(h1n, nx.nb), 1u);
     return pos < (uint32_t)BUCKET_CAP ? pos : (uint32_t)BUCKET_CAP;
 }
 
@@ -184,12 +193,13 @@ __device__ __forceinline__ uint32_t regen_entry(const RoundArgs &r, Entry &e, bo
 }
 
 // Rank step (after next_gather): regenerates each entry in place (e[q] becomes its next-round
-// entry, regen_entry), finds its slot for source bucket b and takes a rank among the wave's entries
+// entry, regen_entry), finds its slot for the source bucket (smallest key lo) and takes a rank
+// among the wave's entries
 // of that slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches
 // them). ntot: the work-group's per-side totals. tgt[q] becomes the carry's top bits, the slot and
 // the rank (tgt_word).
 template <int PER>
-__device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
+__device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t lo, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
                                           uint32_t mask, uint32_t (&tgt)[PER], uint32_t side, bool wide, uint32_t top8) {
     const bool sfx = side == 0;
     uint32_t wn = 0;
@@ -200,7 +210,7 @@ __device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t b, uint32_
             uint32_t x, h1o;
             const uint32_t ev = regen_entry(nx.rg, e[q], wide, top8, sfx, tgt_word(tgt[q]), x, h1o);
             const uint32_t h1n = e[q].h1 & KEY_MASK;
-            const uint32_t slot = sfx ? sfx_slot(nx, b, ev, h1n) : pfx_slot(nx, b, h1o, ev, h1n);
+            const uint32_t slot = sfx ? sfx_slot(nx, lo, ev, h1n) : pfx_slot(nx, lo, h1o, ev, h1n);
             tgt[q] = x | (slot << 9) | (slot_rank(nx, row, slot, h1n, side) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
@@ -221,9 +231,9 @@ __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *ro
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         if ((mask >> q) & 1u) {
-            const uint32_t nb = (e[q].h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
+            const uint32_t tb = bucket_of(e[q].h1 & KEY_MASK, nx.nb), sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
-            next_put(nx, side, nb, pos, stored(e[q], tgt[q]));
+            next_put(nx, side, tb, pos, stored(e[q], tgt[q]));
         }
     }
 }
@@ -235,13 +245,13 @@ __device__ __forceinline__ void next_write(const NextOut &nx, const uint32_t *ro
 // the argument (which the compiler loads per lane); the select holds more scalar registers, which
 // pays in the dedup and not in the join.
 template <int NW, bool SEL = false>
-__device__ __forceinline__ uint32_t slot_reserve(const NextOut &nx, uint32_t b, const uint32_t (*wcnt)[NSLOT], uint32_t s) {
+__device__ __forceinline__ uint32_t slot_reserve(const NextOut &nx, uint32_t lo, const uint32_t (*wcnt)[NSLOT], uint32_t s) {
     uint32_t tot = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) tot += wcnt[w][s];
     if (s >= 22u || tot == 0) return 0u;
     uint32_t *const cnt = SEL ? (s < 12u ? nx.cnt[0] : nx.cnt[1]) : nx.cnt[s < 12u ? 0 : 1];
-    return atomicAdd(cnt + slot_bucket(nx, b, s), tot);
+    return atomicAdd(cnt + slot_bucket(nx, lo, s), tot);
 }
 
 template <int NW>
@@ -262,10 +272,10 @@ __device__ __forceinline__ void put_next(const NextOut &nx, uint32_t h1, uint32_
     uint32_t g, unit = 0xFFFFFFFFu;
     if (wide_needs_gather(nx.rg, rid, sfx, g)) unit = wide_codes(nx.cols, nx.rg.G, g, rid >> 1);
     regen_wide(nx.rg, h1, h2, rid, f, sfx, unit);
-    const uint32_t nb = h1 >> nx.shift;
-    const uint32_t pos = atomicAdd(nx.cnt[side] + nb, 1u);
+    const uint32_t tb = bucket_of(h1, nx.nb);
+    const uint32_t pos = atomicAdd(nx.cnt[side] + tb, 1u);
     atomicAdd(nx.total[side], 1u);
-    next_put(nx, side, nb, pos, wide_entry(h1, h2, rid, f));
+    next_put(nx, side, tb, pos, wide_entry(h1, h2, rid, f));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -616,11 +626,12 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
         Entry kp[PER];  // the prefix side's copy (each side is regenerated in place)
 #pragma unroll
         for (int q = 0; q < PER; ++q) kp[q] = kept[q];
-        next_rank<PER>(a.nx, b, row, ntot, kept, keepmask, ts, 0, false, 0u);
-        next_rank<PER>(a.nx, b, row, ntot, kp, keepmask, tp, 1, false, 0u);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(bucket_lo(b, a.nx.src_nb));
+        next_rank<PER>(a.nx, lo, row, ntot, kept, keepmask, ts, 0, false, 0u);
+        next_rank<PER>(a.nx, lo, row, ntot, kp, keepmask, tp, 1, false, 0u);
         lds_barrier();
         if (threadIdx.x < NSLOT) {
-            slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64, true>(a.nx, b, wcnt, threadIdx.x));
+            slot_starts<NT / 64>(wcnt, threadIdx.x, slot_reserve<NT / 64, true>(a.nx, lo, wcnt, threadIdx.x));
         } else if (threadIdx.x < NSLOT + 2) {
             const uint32_t sd = threadIdx.x - NSLOT;
             if (ntot[sd]) atomicAdd(sd ? a.nx.total[1] : a.nx.total[0], ntot[sd]);
@@ -687,7 +698,7 @@ __device__ __forceinline__ void pfx_gather_finish(const JoinShared<CAP, MCAP, NT
 }
 
 template <int CAP, int MCAP, int NT>
-__device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t b, uint32_t *row,
+__device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP, NT> &S, uint32_t lo, uint32_t *row,
                                          uint32_t pmask, uint32_t (&tgt)[CAP / NT], uint32_t top8) {
     constexpr int PER = CAP / NT;
     uint32_t wn = 0;
@@ -703,7 +714,7 @@ __device__ __forceinline__ void pfx_rank(const NextOut &nx, JoinShared<CAP, MCAP
             const uint32_t ev = regen_entry(nx.rg, e, true, top8, false, tgt_word(tgt[q]), x, h1o);
             S.pk1[i] = e.h1;
             S.pk2[i] = e.h2;
-            const uint32_t slot = pfx_slot(nx, b, h1o, ev, e.h1 & KEY_MASK);
+            const uint32_t slot = pfx_slot(nx, lo, h1o, ev, e.h1 & KEY_MASK);
             tgt[q] = x | (slot << 9) | (slot_rank(nx, row, slot, e.h1 & KEY_MASK, 1) << 14);
         }
         wn += (uint32_t)__popcll(__ballot(act));
@@ -720,9 +731,9 @@ __device__ __forceinline__ void pfx_write(const NextOut &nx, const JoinShared<CA
         if ((pmask >> q) & 1u) {
             const uint32_t i = threadIdx.x + NT * q;
             const Entry e{S.pk1[i], S.pk2[i], S.prid[i]};
-            const uint32_t nb = (e.h1 & KEY_MASK) >> nx.shift, sl = (tgt[q] >> 9) & 31u;
+            const uint32_t tb = bucket_of(e.h1 & KEY_MASK, nx.nb), sl = (tgt[q] >> 9) & 31u;
             const uint32_t pos = sl < 22u ? row[sl] + (tgt[q] >> 14) : tgt[q] >> 14;
-            next_put(nx, 1, nb, pos, stored(e, tgt[q]));
+            next_put(nx, 1, tb, pos, stored(e, tgt[q]));
         }
     }
 }
@@ -745,6 +756,8 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     }
     if (threadIdx.x == 0) { S.ms = 0; S.mp = 0; S.ovf = 0; S.etot = 0; S.ntot[0] = 0; S.ntot[1] = 0; }
     for (uint32_t i = threadIdx.x; i < CAP; i += NT) S.u.h.head[i] = EMPTY;
+    // the bucket's smallest key and the top 8 key bits all its keys share
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(bucket_lo(b, a.nx.src_nb)), top8 = lo >> WIDE_SHIFT;
     uint32_t sc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) sc[q] = 0;
@@ -849,9 +862,8 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         const uint32_t gp = pfx_gather<CAP, MCAP, NT>(a.nx, S, pmask, ptg);
         next_gather_finish<PER>(se, gs, tgt);
         pfx_gather_finish<CAP, MCAP, NT>(S, gp, ptg);
-        const uint32_t top8 = b >> (WIDE_SHIFT - a.nx.src_shift);
-        next_rank<PER>(a.nx, b, row, S.ntot, se, smask, tgt, 0, true, top8);
-        pfx_rank<CAP, MCAP, NT>(a.nx, S, b, row, pmask, ptg, top8);
+        next_rank<PER>(a.nx, lo, row, S.ntot, se, smask, tgt, 0, true, top8);
+        pfx_rank<CAP, MCAP, NT>(a.nx, S, lo, row, pmask, ptg, top8);
     } else {  // last round: only counted
         if (smask) atomicAdd(&S.ntot[0], (uint32_t)__popc(smask));
         if (pmask) atomicAdd(&S.ntot[1], (uint32_t)__popc(pmask));
@@ -871,7 +883,7 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
     if (threadIdx.x == 16 && tot) {
         resv = atomicAdd(a.pool_cursor + (b & (SUBS - 1)) + (lane_id() - 16u), (unsigned long long)tot);
     } else if (threadIdx.x >= NXT0 && threadIdx.x < NXT0 + NSLOT && a.nx.active) {
-        resv = slot_reserve<NT / 64>(a.nx, b, S.u.m.wcnt, threadIdx.x - NXT0);
+        resv = slot_reserve<NT / 64>(a.nx, lo, S.u.m.wcnt, threadIdx.x - NXT0);
     }
     if (threadIdx.x >= TOT0 && threadIdx.x < TOT0 + 2) {
         const uint32_t sd = threadIdx.x - TOT0;
@@ -890,15 +902,21 @@ __device__ __forceinline__ void join_bucket(const JoinArgs &a, JoinShared<CAP, M
         if (has1) { k1 = S.u.m.msk[t]; r1 = S.u.m.msr[t]; c1 = S.u.m.msc[t]; }
         if (has2) { k2 = S.u.m.mpk[j2]; r2 = S.u.m.mpr[j2]; }
         if (nms + nmp > 48u) {
-            // Binned: 64 bins per side on the highest key bits that vary inside the bucket (h1's
-            // bits below its bucket bits; the bins are in key order), counted, scanned by wave 0,
-            // record indices scattered into bin order (in the dead used flags), then each record
-            // ranked only against its own bin.
+            // Binned: 64 bins per side, the bucket's key range cut in 64 (the bin of h1 in the set
+            // of 64 times as many buckets, less 64 b: the bins are in key order), counted, scanned
+            // by wave 0, record indices scattered into bin order (in the dead used flags), then
+            // each record ranked only against its own bin.
             static_assert(4 * MCAP <= CAP, "record indices in the used flags");
-            const uint32_t bsh = 32u + (a.nx.src_shift >= 6u ? a.nx.src_shift - 6u : 0u);
+            const uint32_t nb64 = a.nx.src_nb << 6, hi8 = top8 << WIDE_SHIFT, b64 = b << 6;
             uint32_t b1 = 0, b2 = 0, l1 = 0, l2 = 0;
-            if (has1) { b1 = (uint32_t)(k1 >> bsh) & 63u; l1 = atomicAdd(&S.u.m.hist[0][b1], 1u); }
-            if (has2) { b2 = (uint32_t)(k2 >> bsh) & 63u; l2 = atomicAdd(&S.u.m.hist[1][b2], 1u); }
+            if (has1) {
+                b1 = bucket_of((uint32_t)(k1 >> 32) | hi8, nb64) - b64;
+                l1 = atomicAdd(&S.u.m.hist[0][b1], 1u);
+            }
+            if (has2) {
+                b2 = bucket_of((uint32_t)(k2 >> 32) | hi8, nb64) - b64;
+                l2 = atomicAdd(&S.u.m.hist[1][b2], 1u);
+            }
             lds_barrier();
             if (t < 64) {
                 const uint32_t x0 = S.u.m.hist[0][t], x1 = S.u.m.hist[1][t];
@@ -1088,8 +1106,9 @@ __global__ void __launch_bounds__(1024) k_join_big(BigArgs a) {
     uint32_t *SR = a.srank + a.soff[blockIdx.x], *PR = a.prank + a.poff[blockIdx.x];
     uint32_t *SA = a.saux + a.soff[blockIdx.x], *PA = a.paux + a.poff[blockIdx.x];
     // round buckets hold wide entries: their keys are restored from the bucket index
-    gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos, a.nx.src_shift);
-    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, PK, PR, np2, true, &s_pos, a.nx.src_shift);
+    const int32_t top8 = (int32_t)(bucket_lo(b, a.nx.src_nb) >> WIDE_SHIFT);
+    gather_bucket(a.s_in, ns, a.cap, a.s_spill, a.s_spill_n, a.spill_cap, b, SK, SR, ns2, true, &s_pos, top8);
+    gather_bucket(a.p_in, np, a.cap, a.p_spill, a.p_spill_n, a.p_spill_cap, b, PK, PR, np2, true, &s_pos, top8);
     bitonic(SK, SR, nullptr, ns2);
     bitonic(PK, PR, nullptr, np2);
     // valid partner counts: the key group on the other side minus the same-read members
@@ -1243,7 +1262,8 @@ __global__ void __launch_bounds__(AUDIT_THREADS) k_audit(AuditArgs a) {
     uint32_t *R = big ? a.srid + a.off[blockIdx.x] : lr;
     const uint32_t n2 = pow2_at_least(n);
     if (threadIdx.x == 0) { red[0] = 0; red[1] = 0; }
-    gather_bucket(a.fine, n, a.cap, a.spill, a.spill_n, a.spill_cap, b, K, R, n2, false, &s_pos, a.wshift);
+    gather_bucket(a.fine, n, a.cap, a.spill, a.spill_n, a.spill_cap, b, K, R, n2, false, &s_pos,
+                  a.wshift >= 0 ? (int32_t)(b >> (WIDE_SHIFT - a.wshift)) : -1);
     bitonic(K, R, nullptr, n2);
     unsigned long long keys = 0, strs = 0;
     for (uint32_t i = threadIdx.x; i < n; i += AUDIT_THREADS) {

@@ -100,6 +100,22 @@ int bucket_bits(uint64_t x) {
 // entry are implied by its bucket and free to carry codes.
 int round_bits(uint64_t x) { return std::max(WIDE_MIN_BITS, bucket_bits(x)); }
 
+// Bucket count of a pipelined round's set: the smallest multiple of 2^WIDE_MIN_BITS with
+// ceil(X / count) <= BUCKET_TARGET (key h in bucket floor(h count / 2^29); a multiple of 256
+// keeps h1's bits 21..28 implied by the bucket). Powers of two alone leave a list just above one
+// at half the target per bucket.
+uint32_t round_buckets(uint64_t x) {
+    const uint64_t unit = 1ull << WIDE_MIN_BITS;
+    const uint64_t need = (x + BUCKET_TARGET - 1) / BUCKET_TARGET;
+    return (uint32_t)std::max<uint64_t>(unit, (need + unit - 1) / unit * unit);
+}
+
+int ceil_log2(uint32_t x) {
+    int b = 0;
+    while ((1ull << b) < x) ++b;
+    return b;
+}
+
 struct KStat {
     std::string name;
     uint64_t launches = 0;
@@ -1281,12 +1297,16 @@ SideIn side_in(emp_ctx *c, int t, int side) {
 
 // Round t's bucket set as the output of a launch over 2^src_bb source buckets: 2^bb buckets per
 // side, regenerated for round rg (inactive: none, survivors only counted).
-NextOut next_out(emp_ctx *c, int t, int bb, int src_bb, const RoundArgs &rg) {
+// Round t's lists in bucket set t % 2: nb buckets, written by a launch over src_nb source buckets
+// (nb <= src_nb). The shift fields serve the out-of-core kernels, whose sets are powers of two.
+NextOut next_out_nb(emp_ctx *c, int t, uint32_t nb, uint32_t src_nb, const RoundArgs &rg) {
     const int set = t & 1;
     NextOut o{};
     o.active = rg.active;
-    o.shift = 29 - bb;
-    o.src_shift = 29 - src_bb;
+    o.nb = nb;
+    o.src_nb = src_nb;
+    o.shift = 29 - ceil_log2(nb);
+    o.src_shift = 29 - ceil_log2(src_nb);
     o.rg = rg;
     for (int side = 0; side < 2; ++side) {
         o.fine[side] = c->d_bfine[set][side];
@@ -1299,6 +1319,10 @@ NextOut next_out(emp_ctx *c, int t, int bb, int src_bb, const RoundArgs &rg) {
     o.ovf = &c->d_ctr->r[t].nx_ovf;
     o.cols = c->d_cols;
     return o;
+}
+
+NextOut next_out(emp_ctx *c, int t, int bb, int src_bb, const RoundArgs &rg) {
+    return next_out_nb(c, t, 1u << bb, 1u << src_bb, rg);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1888,6 +1912,7 @@ constexpr int RUN_FALLBACK = 1;  // pipelined rounds met an exceptional round: r
 
 struct RunState {
     int rounds = 0, L = 0, bbD = 0;
+    uint32_t nb0 = 0;     // buckets of round 0's lists (2^bbD unless the rounds are pipelined)
     uint64_t M = 0, stride = 0, D = 0;
     int bbS[2] = {0, 0};  // bucket bits of the bucket set written for rounds of each parity
     int t = 0;            // rounds done
@@ -1949,7 +1974,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
                     return rc;
                 ties = true;
             }
-            BigArgs a = ooc_args(c, din, nullptr, none, next_out(c, 0, bbD, bbD, rg0), 0);
+            BigArgs a = ooc_args(c, din, nullptr, none, next_out_nb(c, 0, st.nb0, 1u << bbD, rg0), 0);
             {
                 Timed t(c, "k_dedup_ooc", 0.0);
                 HIPCHK(c, launch_dedup_ooc(a, 1u << bbD, c->stream));
@@ -1966,7 +1991,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         a.in = c->d_dfine;
         a.cnt = c->d_dcnt;
         a.pay = c->d_ends ? c->d_dpay : nullptr;
-        a.nx = next_out(c, 0, bbD, bbD, rg0);
+        a.nx = next_out_nb(c, 0, st.nb0, 1u << bbD, rg0);
         a.big = c->d_big;
         a.n_big = &c->d_ctr->r[0].n_big;
         a.big_cap = BIG_CAP;
@@ -1985,7 +2010,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         const ListCtr &r0 = c->h_ctr->r[0];
         const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
         if (!r0.nx_ovf && r0.n_big &&
-            (rc = big_path(c, r0.n_big, bbD, true, none, 0, din, din, next_out(c, 0, bbD, bbD, rg0))))
+            (rc = big_path(c, r0.n_big, bbD, true, none, 0, din, din, next_out_nb(c, 0, st.nb0, 1u << bbD, rg0))))
             return rc;
         if (c->h_ctr->r[0].nx_ovf) {
             for (int side = 0; side < 2; ++side)
@@ -2034,6 +2059,7 @@ int rounds_sync(emp_ctx *c, RunState &st) {
         rs.p = Pt;
         rs.ovlp = L - 1 - t;
         rs.bucket_bits = bb;
+        rs.buckets = 1u << bb;
         const uint64_t slot0 = (uint64_t)t * stride;
         const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
         // a list over M: cq_sort_by_key's merge order (overlap.cu:271-272) as tie keys per side
@@ -2331,7 +2357,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         c->rec_ev.resize(MAX_ROUNDS);
         for (auto &e : c->rec_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    std::vector<int> bits(rounds + 1, st.bbD);  // bucket bits of round t's lists
+    std::vector<uint32_t> nbs(rounds + 1, st.nb0);  // buckets of round t's lists
     c->h_rec[0] = c->h_ctr->r[0];
     uint64_t Sk = c->h_rec[0].total[0], Pk = c->h_rec[0].total[1];  // newest sizes known
     int launched = 0;
@@ -2352,8 +2378,8 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
             if ((rc = emit_round(t - 3))) return rc;
         }
         const bool more = t + 1 < rounds;
-        const int bb = bits[t];
-        bits[t + 1] = more ? std::min(bb, round_bits(std::max(Sk, Pk))) : bb;
+        const uint32_t nb = nbs[t];
+        nbs[t + 1] = more ? std::min(nb, round_buckets(std::max(Sk, Pk))) : nb;
         const RoundArgs rgn = more ? round_args(c, t + 1, true) : none;
         const uint64_t slot0 = (uint64_t)t * stride;
         const SideIn sin = side_in(c, t, 0), pin = side_in(c, t, 1);
@@ -2363,7 +2389,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         a.s_cnt = sin.cnt;
         a.p_cnt = pin.cnt;
         a.zero_in = 1;
-        a.nx = next_out(c, t + 1, bits[t + 1], bb, rgn);
+        a.nx = next_out_nb(c, t + 1, nbs[t + 1], nb, rgn);
         a.pool = c->d_pool;
         a.pool_cursor = c->d_ctr->pool;
         a.pool_cap = c->pool_cap;
@@ -2377,7 +2403,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         a.abort = &c->d_ctr->abort;
         {
             Timed tm(c, "k_join", 0.0);
-            HIPCHK(c, launch_join(a, 1u << bb, c->stream));
+            HIPCHK(c, launch_join(a, nb, c->stream));
         }
         HIPCHK(c, hipEventRecord(c->rec_ev[t], c->stream));
         // the round's edges, in output order, into the device arena while later rounds run; the
@@ -2386,7 +2412,7 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         HIPCHK(c, hipStreamWaitEvent(c->ostream, c->rec_ev[t], 0));
         const RecMirror mirror{c->d_orec_view[t], reinterpret_cast<uint32_t *>(&c->d_rec_view[t + 1]),
                                reinterpret_cast<const uint32_t *>(&c->d_ctr->r[t + 1]), sizeof(ListCtr) / 4};
-        HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, 1u << bb,
+        HIPCHK(c, launch_round_edges(c->d_edge_cnt + slot0, c->d_edge_off + slot0, c->d_dst_off + slot0, nb,
                                      &c->d_ctr->obase, c->d_ctr->orec[t], &c->d_ctr->pool_ovf, c->d_pool,
                                      c->d_out_from, c->d_out_to, mirror, c->ostream));
         HIPCHK(c, hipEventRecord(c->orec_ev[t], c->ostream));
@@ -2417,7 +2443,8 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         rs.edges = k.orec[t][0];
         rs.ovlp = L - 1 - t;
         rs.records = 1;
-        rs.bucket_bits = bits[t];
+        rs.bucket_bits = ceil_log2(nbs[t]);
+        rs.buckets = nbs[t];
         if (c->opts.profile) {
             int id = kstat_id(c, "k_join");
             c->kstats[id].bytes += 12.0 * (St + Pt) + (t + 1 < rounds ? 12.0 * (rs.s_after + rs.p_after) : 0.0) +
@@ -2479,6 +2506,11 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
     // a spill or pool overflow) sends the run to the synchronous rounds, from the dedup on.
     bool pipelined = !c->audit && c->pipeline;
     for (int pass = 0;; ++pass) {
+        // Round 0's lists get a bucket count sized to the list when the rounds will run pipelined
+        // (the dedup keeps at most N0 entries, so N0 < M/2 settles it now); the synchronous rounds
+        // (framing, the audit, the fallback) keep powers of two.
+        const bool sized = pipelined && N0 < M / 2 && rounds > 0;
+        st.nb0 = sized ? std::min<uint32_t>(1u << bbD, round_buckets(N0)) : 1u << bbD;
         if ((rc = dedup_phase(c, st, N0))) return rc;
         S.n_dedup = st.D;
         if (pipelined && st.D < M / 2 && st.D > 0 && rounds > 0) {

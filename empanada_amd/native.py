@@ -45,7 +45,8 @@ class Opts(ctypes.Structure):
 class RoundStats(ctypes.Structure):
     _fields_ = [("s", ctypes.c_uint64), ("p", ctypes.c_uint64), ("s_after", ctypes.c_uint64),
                 ("p_after", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("ovlp", ctypes.c_int32),
-                ("records", ctypes.c_int32), ("bucket_bits", ctypes.c_int32), ("big_buckets", ctypes.c_int32)]
+                ("records", ctypes.c_int32), ("bucket_bits", ctypes.c_int32), ("big_buckets", ctypes.c_int32),
+                ("buckets", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class Audit(ctypes.Structure):

@@ -115,8 +115,10 @@ typedef struct {
     uint64_t edges;           /* edges written (after self-edge removal) */
     int32_t ovlp;             /* overlap length L-1-t */
     int32_t records;          /* records written this round */
-    int32_t bucket_bits;      /* partition fan-out used for the round */
+    int32_t bucket_bits;      /* ceil(log2(buckets)) */
     int32_t big_buckets;      /* buckets handled by the large-bucket path */
+    uint32_t buckets;         /* buckets of the round's lists (key h in bucket floor(h buckets / 2^29)) */
+    uint32_t pad;
 } emp_round_stats;
 
 typedef struct {
