@@ -142,6 +142,10 @@ struct PartArgs {
     // the word at its own slot; pay_out is written at the output slot (spilled entries drop it)
     const uint32_t *pay_in;
     uint32_t *pay_out;
+    // fin_nb != 0: the last pass of a partition into fin_nb buckets (a multiple of 256; key h in
+    // bucket floor(h fin_nb / 2^29)) over groups keyed by the top 8 key bits: group g holds output
+    // buckets g F .. g F + F - 1, F = fin_nb / 256. 0: output bucket h1 >> out_shift.
+    uint32_t fin_nb;
 };
 
 struct DedupArgs {

@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
     __shared__ Entry buf[T];
     __shared__ uint32_t hist[NB], loc[NB], base[NB], sh[NT / 64];
     const uint32_t g = blockIdx.x / a.tiles_x, tile0 = blockIdx.x % a.tiles_x;
-    const uint32_t F = 1u << a.dbits;
+    const uint32_t F = a.fin_nb ? a.fin_nb >> 8 : 1u << a.dbits;
     uint32_t n_g;
     const Entry *in;
     if (a.in_cnt) {
@@ -451,7 +451,9 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
         n_g = *a.in_n;
         in = a.in;
     }
-    const uint32_t gb = a.keyed ? (g << a.dbits) : 0u;
+    const uint32_t gb = a.keyed ? g * F : 0u;
+    // an entry's bin: its output bucket less the group's first
+    auto digit = [&](uint32_t h1) { return a.fin_nb ? bucket_of(h1 & KEY_MASK, a.fin_nb) - gb : (h1 >> a.out_shift) & (F - 1); };
     for (uint32_t tile = tile0; (size_t)tile * T < n_g; tile += a.tiles_x) {
         const uint32_t t0 = tile * T, nt = min((uint32_t)T, n_g - t0);
         if (threadIdx.x < NB) hist[threadIdx.x] = 0;
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
         for (int q = 0; q < PER; ++q) {
             uint32_t i = threadIdx.x + NT * q;
             if (i < nt) {
-                d[q] = (e[q].h1 >> a.out_shift) & (F - 1);
+                d[q] = digit(e[q].h1);
                 r[q] = atomicAdd(&hist[d[q]], 1u);
             }
         }
@@ -497,7 +499,7 @@ __global__ void __launch_bounds__(PART_THREADS) __attribute__((amdgpu_waves_per_
             const uint32_t p = threadIdx.x + NT * k;
             if (p < nt) {
                 const Entry v = buf[p];
-                const uint32_t dd = (v.h1 >> a.out_shift) & (F - 1);
+                const uint32_t dd = digit(v.h1);
                 dig[k / 4] |= dd << (8 * (k % 4));
                 const uint32_t slot = base[dd] + (p - loc[dd]);
                 const uint32_t bucket = gb + dd;
@@ -652,18 +654,27 @@ __global__ void __launch_bounds__(NT) k_dedup(DedupArgs a) {
 // table (chain heads + links) while probing and the matched records afterwards. The suffix
 // entries stay in registers.
 
-// Loads the suffix and prefix entries of bucket b (nothing for a bucket over capacity).
+// Loads the suffix and prefix entries of bucket b (nothing past the first PRE * NT slots for a
+// bucket over capacity). The first PRE * NT slots of each side are loaded whatever the counts, so
+// that these loads do not wait for the counts' (buckets are sized for about 1,800 entries per
+// side; slots past a count hold stale entries that nothing reads).
 template <int CAP, int NT>
 __device__ __forceinline__ void load_bucket(const JoinArgs &a, uint32_t b, uint32_t ns, uint32_t np, Entry (&se)[CAP / NT],
                                             Entry (&pe)[CAP / NT]) {
+    constexpr int PRE = CAP / NT >= 2 ? 2 : 0;
     const Entry *sin = a.s_in + (size_t)b * CAP;
     const Entry *pin = a.p_in + (size_t)b * CAP;
     const bool fit = ns <= (uint32_t)CAP && np <= (uint32_t)CAP;
 #pragma unroll
     for (int q = 0; q < CAP / NT; ++q) {
         const uint32_t i = threadIdx.x + NT * q;
-        if (fit && i < ns) se[q] = sin[i];
-        if (fit && i < np) pe[q] = pin[i];
+        if (q < PRE) {
+            se[q] = sin[i];
+            pe[q] = pin[i];
+        } else {
+            if (fit && i < ns) se[q] = sin[i];
+            if (fit && i < np) pe[q] = pin[i];
+        }
     }
 }
 
@@ -1056,18 +1067,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     static_assert(sizeof(((SM *)nullptr)->u.m) <= sizeof(((SM *)nullptr)->u.h), "matched records fit the hash table");
     __shared__ SM S;
 
-    if (*a.abort) return;
     const uint32_t G = gridDim.x, nb = a.nb;
     uint32_t b = blockIdx.x;
     if (b >= nb) return;
     if (!PERSIST) {
+        // the abort flag is read beside the counts and the first loads, not before them
+        const uint32_t abort_set = *a.abort;
         const uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
         Entry se[PER], pe[PER];
         load_bucket<CAP, NT>(a, b, ns, np, se, pe);
+        if (abort_set) return;
         stage_prefixes<CAP, MCAP, NT>(S, np, pe);
         join_bucket<CAP, MCAP, NT>(a, S, b, ns, np, se);
         return;
     }
+    if (*a.abort) return;
     uint32_t ns = a.s_cnt[b], np = a.p_cnt[b];
     uint32_t b1 = b + G, ns1 = 0, np1 = 0;
     if (b1 < nb) { ns1 = a.s_cnt[b1]; np1 = a.p_cnt[b1]; }

@@ -1062,8 +1062,12 @@ int grow_after_abort(emp_ctx *c, uint64_t x, int bb) {
 
 // The dedup's sort (overlap.cu:211-212): the x map entries (linear, count in d_ctr->in) into
 // 2^bb buckets by passes of at most PART_MAX_BITS bits.
-int partition(emp_ctx *c, uint64_t x, int bb) {
+// fin_nb != 0: the last pass puts the entries into fin_nb buckets (a multiple of 256, at most
+// 2^bb) by bucket_of; needs a two-pass plan whose first pass takes the top 8 bits.
+int partition(emp_ctx *c, uint64_t x, int bb, uint32_t fin_nb = 0) {
     const std::vector<Level> pl = plan_levels(x, bb, c->slack);
+    if (fin_nb && (pl.size() != 2 || pl[0].dbits != 8 || fin_nb % 256 || fin_nb > (1u << bb)))
+        return fail(c, EMP_ERR_ARG, "dedup partition into %u buckets", fin_nb);
     const Entry *in = c->d_entries;
     const uint32_t *in_cnt = nullptr;
     uint32_t in_cap = 0, groups = 1;
@@ -1095,6 +1099,7 @@ int partition(emp_ctx *c, uint64_t x, int bb) {
         a.pay_in = i == 0 ? c->d_ends : c->d_inter_pay[(i - 1) & 1];
         a.pay_out = fin ? c->d_dpay : c->d_inter_pay[i & 1];
         if (!c->d_ends) a.pay_in = nullptr;
+        a.fin_nb = fin ? fin_nb : 0u;
         {
             Timed t(c, "k_part", (a.pay_in ? 32.0 : 24.0) * (double)x);
             HIPCHK(c, launch_part(a, groups, c->stream));
@@ -1913,6 +1918,7 @@ constexpr int RUN_FALLBACK = 1;  // pipelined rounds met an exceptional round: r
 struct RunState {
     int rounds = 0, L = 0, bbD = 0;
     uint32_t nb0 = 0;     // buckets of round 0's lists (2^bbD unless the rounds are pipelined)
+    uint32_t nbD = 0;     // buckets of the dedup partition (2^bbD, or nb0 for pipelined rounds)
     uint64_t M = 0, stride = 0, D = 0;
     int bbS[2] = {0, 0};  // bucket bits of the bucket set written for rounds of each parity
     int t = 0;            // rounds done
@@ -1954,7 +1960,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
             z.add((uint32_t *)c->d_ctr->pool, 2 * (SUBS + 2));  // edge-pool cursors
             if ((rc = run_zero(c, z))) return rc;
         }
-        if (need_part && (rc = partition(c, N0, bbD))) return rc;
+        if (need_part && (rc = partition(c, N0, bbD, st.nbD == (1u << bbD) ? 0u : st.nbD))) return rc;
         if (ooc) {
             // cq_sort_by_key + cq_unique_by_key (overlap.cu:212-225, cqutils.cuh:179-247,271-306):
             // the first entry of every key in the merged order
@@ -1991,14 +1997,14 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         a.in = c->d_dfine;
         a.cnt = c->d_dcnt;
         a.pay = c->d_ends ? c->d_dpay : nullptr;
-        a.nx = next_out_nb(c, 0, st.nb0, 1u << bbD, rg0);
+        a.nx = next_out_nb(c, 0, st.nb0, st.nbD, rg0);
         a.big = c->d_big;
         a.n_big = &c->d_ctr->r[0].n_big;
         a.big_cap = BIG_CAP;
         a.abort = &c->d_ctr->abort;
         {
             Timed t(c, "k_dedup", (c->d_ends ? 16.0 : 12.0) * N0);  // entries and their payload
-            HIPCHK(c, launch_dedup(a, 1u << bbD, c->stream));
+            HIPCHK(c, launch_dedup(a, st.nbD, c->stream));
         }
         if ((rc = read_counters(c))) return rc;
         if (c->h_ctr->abort) {
@@ -2010,7 +2016,7 @@ int dedup_phase(emp_ctx *c, RunState &st, uint64_t N0) {
         const ListCtr &r0 = c->h_ctr->r[0];
         const SideIn din{c->d_dfine, c->d_dcnt, c->d_dspill, &c->d_ctr->spill, c->dspill_cap};
         if (!r0.nx_ovf && r0.n_big &&
-            (rc = big_path(c, r0.n_big, bbD, true, none, 0, din, din, next_out_nb(c, 0, st.nb0, 1u << bbD, rg0))))
+            (rc = big_path(c, r0.n_big, bbD, true, none, 0, din, din, next_out_nb(c, 0, st.nb0, st.nbD, rg0))))
             return rc;
         if (c->h_ctr->r[0].nx_ovf) {
             for (int side = 0; side < 2; ++side)
@@ -2511,6 +2517,9 @@ int run_impl(emp_ctx *c, emp_record_sink sink, void *user) {
         // (framing, the audit, the fallback) keep powers of two.
         const bool sized = pipelined && N0 < M / 2 && rounds > 0;
         st.nb0 = sized ? std::min<uint32_t>(1u << bbD, round_buckets(N0)) : 1u << bbD;
+        // the dedup partition's buckets sized the same way where its plan allows (two passes, the
+        // first on the top 8 key bits): otherwise half full for a list just above a power of two
+        st.nbD = sized && bbD > 8 && bbD <= 8 + PART_MAX_BITS ? st.nb0 : 1u << bbD;
         if ((rc = dedup_phase(c, st, N0))) return rc;
         S.n_dedup = st.D;
         if (pipelined && st.D < M / 2 && st.D > 0 && rounds > 0) {
