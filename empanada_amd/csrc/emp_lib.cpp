@@ -48,12 +48,14 @@ constexpr uint32_t BIG_CAP = 1u << 16;      // bucket ids recorded for the large
 
 // Counters of one round's lists, filled by the launch that writes the round's bucket set (the
 // dedup for round 0, round t-1's join for round t).
-struct ListCtr {
+// One cache line per round: every work-group of a join adds to the next round's totals while it
+// reads the joined round's (the bucket sizes), and a line shared by the two slows both down.
+struct alignas(128) ListCtr {
     uint32_t total[2];  // entries per side
     uint32_t spill[2];  // records past a bucket's capacity, per side
     uint32_t n_big;     // buckets the writing launch handed to the large-bucket path
     uint32_t nx_ovf;    // a spill area overflowed
-    uint32_t pad[2];
+    uint32_t pad[26];
 };
 constexpr int MAX_ROUNDS = 256;  // L - o <= 255
 
@@ -61,14 +63,14 @@ struct DevCounters {
     unsigned long long pool[SUBS];  // edge-pool cursor per sub-range
     unsigned long long pool_ovf;    // cursor of the pool's overflow region
     unsigned long long pool_lost;   // edges of sub-range reservations abandoned (past the end)
-    uint32_t abort;                 // the dedup partition overflowed an intermediate bucket or its spill
+    alignas(128) uint32_t abort;    // the dedup partition overflowed an intermediate bucket or its spill
     uint32_t spill;                 // records spilled by the dedup partition's final pass
     uint32_t in;                    // dedup partition input size, set by the host
     uint32_t too_many;              // a large bucket held 2^31 edges or more
     ListCtr r[MAX_ROUNDS + 1];      // r[t]: round t's lists
     // pipelined rounds stream their edges out: output position after the rounds copied so far,
     // and per round {edges, first position}
-    unsigned long long obase;
+    alignas(128) unsigned long long obase;
     unsigned long long orec[MAX_ROUNDS][3];
 };
 
@@ -2449,8 +2451,8 @@ int rounds_pipelined(emp_ctx *c, RunState &st) {
         rs.edges = k.orec[t][0];
         rs.ovlp = L - 1 - t;
         rs.records = 1;
-        rs.bucket_bits = ceil_log2(nbs[t]);
         rs.buckets = nbs[t];
+        rs.bucket_bits = ceil_log2(nbs[t]);
         if (c->opts.profile) {
             int id = kstat_id(c, "k_join");
             c->kstats[id].bytes += 12.0 * (St + Pt) + (t + 1 < rounds ? 12.0 * (rs.s_after + rs.p_after) : 0.0) +
