@@ -266,9 +266,9 @@ def test_file_output_over_a_longer_file_spanning_reservations(tmp_path):
 
 
 def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
-    """2,000,000 x 100 bp reads (4 M entries, 12-bit buckets): the back-to-back rounds (each join
-    clears the counts it read) and the rounds with the host between them give the oracle's bytes
-    and the same per-round counts."""
+    """2,000,000 x 100 bp reads (4 M entries): the back-to-back rounds (each join clears the counts
+    it read; bucket sets in multiples of 256 buckets) and the rounds with the host between them
+    (powers of two) give the oracle's bytes and the same per-round counts."""
     g = gen.genome(5 * 10**6, 21)
     r = gen.reads(g, 2_000_000, 100, 0.5, 0.005, 22)
     rc, want, osum, orounds = ol.run_reads(r.tobytes(), r.shape[0], 100, min_ovlp=40)
@@ -286,6 +286,16 @@ def test_pipelined_and_synchronous_rounds_agree(monkeypatch):
         assert out == want, mode
         assert [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in rounds] == \
                [(x["s"], x["p"], x["edges"], x["s_after"], x["p_after"]) for x in orounds], mode
+    # bucket counts: powers of two on the synchronous rounds; on the pipelined ones multiples of 256
+    # holding at most 1,792 entries per bucket on average, never growing from one round to the next,
+    # and round 0's sized from the 2n entries before the dedup (4,000,000 / 1,792 -> 2,304 buckets)
+    sync_nb = [x["buckets"] for x in got["sync"][1]]
+    pipe_nb = [x["buckets"] for x in got["pipelined"][1]]
+    assert all(nb & (nb - 1) == 0 for nb in sync_nb)
+    assert pipe_nb[0] == 2304 and any(nb & (nb - 1) for nb in pipe_nb)
+    assert all(nb % 256 == 0 for nb in pipe_nb)
+    assert all(b <= a for a, b in zip(pipe_nb, pipe_nb[1:]))
+    assert all(max(x["s"], x["p"]) <= 1792 * x["buckets"] for x in got["pipelined"][1])
 
 
 def test_fastq_and_read_entry_points_agree_and_block_size_is_irrelevant():
