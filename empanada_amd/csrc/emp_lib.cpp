@@ -185,11 +185,11 @@ struct emp_ctx {
     // large kept-read arrays in pageable memory: three pinned stages filled by parallel copies
     char *h_big[3] = {nullptr, nullptr, nullptr};
     hipEvent_t big_done[3] = {nullptr, nullptr, nullptr};
-    // FASTQ files streamed into HBM (emp_add_fastq_file): the file's bytes, two pinned chunks
+    // FASTQ files streamed into HBM (emp_add_fastq_file): the file's bytes, three pinned chunks
     // and the stream that copies them
     uint8_t *d_file = nullptr;
     uint64_t file_cap = 0;
-    char *h_chunk[2] = {nullptr, nullptr};
+    char *h_chunk[3] = {nullptr, nullptr, nullptr};
     hipStream_t cstream = nullptr;
 
     // overlap phase. Bucket set `set` holds round t's lists for t % 2 == set, per side (0 =
@@ -729,12 +729,16 @@ int add_fastq_range(emp_ctx *c, const char *data, size_t size, uint64_t B, bool 
 
 // ---------------------------------------------------------------------------------------------
 // A FASTQ file streamed into HBM: the CLI's block loop (main.cu:42-58 over read_block,
-// preprocess.cu:195-237) with the file read ahead. A reader thread fills two pinned chunks of
-// FILE_CHUNK bytes with parallel preads and copies each to the device image of the file on a copy
-// stream; block j (map_mem/16 bytes from the position block j-1 left off) is parsed on the
-// device as soon as the chunks it covers have arrived, while later chunks are read and copied.
+// preprocess.cu:195-237) with the file read ahead. FILE_READERS reader threads, alive for the whole
+// file, each read their share of every chunk (pread) into one of FILE_SLOTS pinned chunks of
+// FILE_CHUNK bytes; a copier thread sends each complete chunk to the device image of the file on a
+// copy stream and frees its slot once that copy is done. Block j (map_mem/16 bytes from the
+// position block j-1 left off) is parsed on the device as soon as the chunks it covers have
+// arrived, while later chunks are read and copied. (Page-cache reads reach about 50 GB/s with 4
+// threads and 80 with 8, the link about 50: with 8 readers and 3 slots the copies never wait.)
 constexpr uint64_t FILE_CHUNK = 64ull << 20;
-constexpr int FILE_READERS = 4;
+constexpr int FILE_READERS = 8;
+constexpr int FILE_SLOTS = 3;
 
 int add_fastq_file(emp_ctx *c, const char *path) {
     const uint64_t B = c->opts.map_mem / 16;  // main.cu:16
@@ -760,7 +764,7 @@ int add_fastq_file(emp_ctx *c, const char *path) {
         close(fd);
         return fail(c, EMP_ERR_HIP, "hipStreamCreate failed");
     }
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FILE_SLOTS; ++i)
         if (!c->h_chunk[i] && hipHostMalloc((void **)&c->h_chunk[i], FILE_CHUNK, hipHostMallocDefault) != hipSuccess) {
             close(fd);
             return fail(c, EMP_ERR_NOMEM, "hipHostMalloc of a %llu-byte chunk failed",
@@ -775,51 +779,79 @@ int add_fastq_file(emp_ctx *c, const char *path) {
         }
     std::mutex mu;
     std::condition_variable cv;
-    uint64_t enqueued = 0;        // chunks whose copy (and event) is on the copy stream
+    std::vector<int> parts(nchunks, 0);  // readers done with each chunk
+    uint64_t freed = 0;                  // chunk k's slot is free once k < freed + FILE_SLOTS
+    uint64_t enqueued = 0;               // chunks whose copy (and event) is on the copy stream
     bool reader_failed = false;
-    std::atomic<bool> stop{false};
-    std::thread reader;
-    if (!rc) reader = std::thread([&]() {
-        const int dev = c->device;
-        (void)hipSetDevice(dev);
-        for (uint64_t k = 0; k < nchunks && !stop; ++k) {
-            const int slot = (int)(k & 1);
-            if (k >= 2 && hipEventSynchronize(ev[k - 2]) != hipSuccess) {  // the slot's previous copy is done
-                std::lock_guard<std::mutex> g(mu);
-                reader_failed = true;
-                break;
-            }
-            const uint64_t off = k * FILE_CHUNK, len = std::min(FILE_CHUNK, size - off);
-            std::atomic<bool> bad{false};
-            auto part = [&](int p) {
+    bool stop = false;
+    std::vector<std::thread> readers;
+    std::thread copier;
+    if (!rc) {
+        for (int p = 0; p < FILE_READERS; ++p) readers.emplace_back([&, p]() {
+            for (uint64_t k = 0; k < nchunks; ++k) {
+                {
+                    std::unique_lock<std::mutex> g(mu);
+                    cv.wait(g, [&]() { return stop || reader_failed || k < freed + FILE_SLOTS; });
+                    if (stop || reader_failed) return;
+                }
+                const uint64_t off = k * FILE_CHUNK, len = std::min(FILE_CHUNK, size - off);
                 const uint64_t per = (len + FILE_READERS - 1) / FILE_READERS;
-                uint64_t a = std::min(len, per * (uint64_t)p), e = std::min(len, a + per);
+                uint64_t a = std::min(len, per * (uint64_t)p);
+                const uint64_t e = std::min(len, a + per);
+                char *const buf = c->h_chunk[k % FILE_SLOTS];
+                bool ok = true;
                 while (a < e) {
-                    const ssize_t got = pread(fd, c->h_chunk[slot] + a, (size_t)(e - a), (off_t)(off + a));
+                    const ssize_t got = pread(fd, buf + a, (size_t)(e - a), (off_t)(off + a));
                     if (got <= 0) {
-                        bad = true;
-                        return;
+                        ok = false;
+                        break;
                     }
                     a += (uint64_t)got;
                 }
-            };
-            std::thread helpers[FILE_READERS - 1];
-            for (int p = 1; p < FILE_READERS; ++p) helpers[p - 1] = std::thread(part, p);
-            part(0);
-            for (auto &h : helpers) h.join();
-            bool ok = !bad && hipMemcpyAsync(c->d_file + off, c->h_chunk[slot], len, hipMemcpyHostToDevice,
-                                             c->cstream) == hipSuccess &&
-                      hipEventRecord(ev[k], c->cstream) == hipSuccess;
-            std::lock_guard<std::mutex> g(mu);
-            if (!ok) {
-                reader_failed = true;
-                break;
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (ok) ++parts[k];
+                    else reader_failed = true;
+                }
+                cv.notify_all();
+                if (!ok) return;
             }
-            enqueued = k + 1;
+        });
+        copier = std::thread([&]() {
+            (void)hipSetDevice(c->device);
+            for (uint64_t k = 0; k < nchunks; ++k) {
+                {
+                    std::unique_lock<std::mutex> g(mu);
+                    cv.wait(g, [&]() { return stop || reader_failed || parts[k] == FILE_READERS; });
+                    if (stop || reader_failed) break;
+                }
+                const uint64_t off = k * FILE_CHUNK, len = std::min(FILE_CHUNK, size - off);
+                bool ok = hipMemcpyAsync(c->d_file + off, c->h_chunk[k % FILE_SLOTS], len, hipMemcpyHostToDevice,
+                                         c->cstream) == hipSuccess &&
+                          hipEventRecord(ev[k], c->cstream) == hipSuccess;
+                // slots whose copies are done are free; when the readers would otherwise wait for
+                // one, wait for the oldest copy here
+                uint64_t f = freed;
+                while (ok && f < k + 1 && hipEventQuery(ev[f]) == hipSuccess) ++f;
+                if (ok && k + 1 < nchunks && k + 1 >= f + FILE_SLOTS) {
+                    ok = hipEventSynchronize(ev[f]) == hipSuccess;
+                    ++f;
+                }
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    freed = f;
+                    if (!ok) {
+                        reader_failed = true;
+                    } else {
+                        enqueued = k + 1;
+                    }
+                }
+                cv.notify_all();
+                if (!ok) break;
+            }
             cv.notify_all();
-        }
-        cv.notify_all();
-    });
+        });
+    }
     // The arrived prefix is parsed in ranges of at least PARSE_STEP bytes (whole blocks of B each,
     // the short last one only with the file's end), while later chunks stream in.
     const uint64_t PARSE_STEP = std::max<uint64_t>(B, 8 * FILE_CHUNK);
@@ -846,8 +878,13 @@ int add_fastq_file(emp_ctx *c, const char *path) {
         if (last) break;
         pos += used;
     }
-    stop = true;
-    if (reader.joinable()) reader.join();
+    {
+        std::lock_guard<std::mutex> g(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    for (auto &t : readers) t.join();
+    if (copier.joinable()) copier.join();
     close(fd);
     (void)hipStreamSynchronize(c->cstream);
     for (auto e : ev)
@@ -2646,10 +2683,11 @@ void emp_ctx_destroy(emp_ctx *c) {
     if (c->cstream) (void)hipStreamSynchronize(c->cstream);
     for (auto &a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
+    for (char *p : c->h_chunk)
+        if (p) (void)hipHostFree(p);
     for (int i = 0; i < 2; ++i) {
         if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
         if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
-        if (c->h_chunk[i]) (void)hipHostFree(c->h_chunk[i]);
         if (c->pin_ev[i]) (void)hipEventDestroy(c->pin_ev[i]);
         if (c->pin_done[i]) (void)hipEventDestroy(c->pin_done[i]);
     }
