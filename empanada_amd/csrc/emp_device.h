@@ -44,8 +44,9 @@ __host__ __device__ __forceinline__ uint32_t col_groups8(uint32_t L) { return (L
 constexpr uint32_t KEY_MASK = MASK29;
 
 // Entries of the single-job engine's round buckets carry 8 more codes' bits in bits 21..28 of h1:
-// a bucket set has at least 2^WIDE_MIN_BITS buckets (bucket = h1 >> (29 - bits)), so those key
-// bits are the bucket index's top 8 and are restored from it (wide_key1). The 14-bit carry is
+// a bucket set's count is a multiple of 2^WIDE_MIN_BITS (key h in bucket floor(h nb / 2^29)), so
+// those key bits are the same for every key of a bucket and are restored from it (wide_key1, the
+// top bits of the bucket's smallest key). The 14-bit carry is
 // h1 >> 29 | (h2 >> 29) << 3 | ((h1 >> 21) & 0xFF) << 6, the next codes two bits each, first at
 // bits 0-1. Keys of one bucket compare on h1 & LOW_MASK.
 constexpr int WIDE_MIN_BITS = 8;

@@ -73,8 +73,8 @@ struct RoundArgs {
 };
 
 // The next round's two lists (side 0 suffixes, side 1 prefixes), written by the kernel that finds
-// their entries, already regenerated and in their final buckets (bucket = top bits of h1). A
-// bucket of width w feeds few buckets of the next round (as many buckets or fewer):
+// their entries, already regenerated and in their final buckets (bucket_of(h1, nb)). A bucket of
+// width w feeds few buckets of the next round (as many buckets or fewer):
 //   - NextSfx subtracts e * R1^(l-1) mod P1 from h1 (e in 1..4): at most 2 buckets per base;
 //   - PrevPfx multiplies h1 - e by R1^-1 = (P1 + 1) / 5, which is y = (h1 - e + k P1) / 5 with
 //     k = (h1 - e) mod 5: 5 ranges of width w / 5, at most 2 buckets each (6 in practice; the

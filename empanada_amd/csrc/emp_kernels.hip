@@ -3,12 +3,13 @@
 // Data in HBM:
 //   reads    row-major ASCII kept reads (input of the map kernel only)
 //   columns  2-bit codes per read position, chunked by 2^20 reads (emp_device.h)
-//   entries  12-byte (h1, h2, rid) records, in buckets by the top bits of h1: bucket b of a list
-//            with `bits` bucket bits (at least 8 for a round's lists) holds the entries with
-//            h1 >> (29 - bits) == b, in a fixed-capacity slot array [bucket][cap] plus a count
-//            (spilled records past it). A round's entries keep carried codes in h1's bits 21..28
-//            (implied by the bucket: wide_key1) and 29..31 and h2's bits 29..31. Order inside a
-//            bucket is arbitrary: every order the output depends on is re-established in LDS.
+//   entries  12-byte (h1, h2, rid) records, in buckets in key order: bucket b of a list of nb
+//            buckets (a multiple of 256 for a round's lists) holds the entries with
+//            floor(h1 nb / 2^29) == b (bucket_of; h1 >> (29 - bits) for nb = 2^bits), in a
+//            fixed-capacity slot array [bucket][cap] plus a count (spilled records past it). A
+//            round's entries keep carried codes in h1's bits 21..28 (implied by the bucket:
+//            wide_key1) and 29..31 and h2's bits 29..31. Order inside a bucket is arbitrary: every
+//            order the output depends on is re-established in LDS.
 //
 // Kernels:
 //   k_map      transposition + forward/reverse-complement Rabin-Karp as byte dot products
@@ -194,9 +195,8 @@ __device__ __forceinline__ uint32_t regen_entry(const RoundArgs &r, Entry &e, bo
 
 // Rank step (after next_gather): regenerates each entry in place (e[q] becomes its next-round
 // entry, regen_entry), finds its slot for the source bucket (smallest key lo) and takes a rank
-// among the wave's entries
-// of that slot from the wave's own counters (row: NSLOT of them, zeroed; no other wave touches
-// them). ntot: the work-group's per-side totals. tgt[q] becomes the carry's top bits, the slot and
+// among the wave's entries of that slot from the wave's own counters (row: NSLOT of them, zeroed;
+// no other wave touches them). ntot: the work-group's per-side totals. tgt[q] becomes the carry's top bits, the slot and
 // the rank (tgt_word).
 template <int PER>
 __device__ __forceinline__ void next_rank(const NextOut &nx, uint32_t lo, uint32_t *row, uint32_t *ntot, Entry (&e)[PER],
