@@ -103,14 +103,19 @@ int OutFile::append(uint64_t n, int32_t ovlp, const uint32_t *from, const uint32
     // complete ones they have not reached go to the direct side. Otherwise every byte is written
     // buffered: a direct write takes about a millisecond to complete, which would only add to the
     // end of the file once the buffered writes keep up.
+    // The direct side never holds more than DIRECT_LAG chunks not yet written: when the device
+    // slows down (a disk's cache full after gigabytes of direct writes), later designated chunks
+    // go buffered instead of waiting in its queue, and the end of the file never waits for more
+    // than DIRECT_LAG direct chunks.
     next_chunk_ = std::max(next_chunk_, (wb_ + CHUNK - 1) / CHUNK);
     const bool behind = avail_ - wb_ > BACKLOG;
     for (; (next_chunk_ + 1) * CHUNK <= avail_; ++next_chunk_) {
         if (!behind || !designated(next_chunk_)) continue;
+        std::lock_guard<std::mutex> g(mu_);
+        if (taken_ - written_ >= DIRECT_LAG) continue;
         if (queued_.size() <= next_chunk_) queued_.resize(next_chunk_ + 1, 0);
         queued_[next_chunk_] = 1;
         direct_bytes_ += CHUNK;
-        std::lock_guard<std::mutex> g(mu_);
         queue_.push_back(next_chunk_);
         ++taken_;
         cv_.notify_all();

@@ -28,8 +28,8 @@ struct DirectIO {
 // The file is written from two sides at once. Buffered writes into the page cache take the inode's
 // lock, so one thread does them all; they reach about 12-14 GB/s on one core, less while the
 // rounds run. While they lag behind the records by more than BACKLOG bytes, a fixed share of the
-// file's 2 MB chunks among those they have not reached (DIRECT_SET of every 16) is written with
-// O_DIRECT through Linux AIO instead: a helper thread copies each such chunk into an aligned
+// file's 2 MB chunks among those they have not reached (DIRECT_SET of every 16, while the direct
+// side holds fewer than DIRECT_LAG of them) is written with O_DIRECT through Linux AIO instead: a helper thread copies each such chunk into an aligned
 // bounce buffer and submits it; the submission returns before the device has the data, so the
 // inode lock is not held while it moves. Chunks are page-aligned and each is written by one side
 // only, so the page cache never holds a page that a direct write also covers (a chunk whose direct
@@ -47,6 +47,7 @@ public:
     static constexpr uint64_t CHUNK = 2ull << 20;
     static constexpr uint32_t DIRECT_SET = 0x2492u;  // chunks c with bit (c % 16) set: 1, 4, 7, 10, 13
     static constexpr uint64_t BACKLOG = 16ull << 20;
+    static constexpr uint64_t DIRECT_LAG = 6;  // direct chunks taken and not yet written, at most
 
     // Opens path as fopen "wb" does. direct: the context's resources (null: buffered only).
     int open(const char *path, DirectIO *direct);
