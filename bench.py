@@ -13,8 +13,8 @@ Beside it, on the same line:
 - ``e2e_from_page_cache``: the reference's t_e2e, FASTQ file open (page-cached) to output.bin
   closed, through emp_add_fastq_file (the file streamed over PCIe while blocks are parsed) and
   emp_run_to_file: the PCIe-inclusive rate;
-- ``overwrite_existing``: the headline step onto the previous step's output.bin (fopen "wb"
-  truncates it, and ext4 then writes the rewritten file back when it is closed);
+- ``overwrite_existing``: the headline step onto the previous step's output.bin (written over in
+  place and cut to the new size: the file fopen "wb" would leave);
 - ``reads_in_hbm``: the overlap path alone over kept reads already in HBM, records to host memory;
 - ``roofline`` of the dominant kernel (HIP events recorded by the library on its stream around
   every launch, algorithmic bytes per launch from the live per-round counts);
@@ -433,9 +433,8 @@ def main():
 
     def fresh_output():
         """Removes the previous step's output.bin (outside the timed region): every step writes a
-        new file, as one run of the tool does. Rewriting an existing file through fopen "wb"
-        truncates it to zero, and ext4 then flushes the whole file when it is closed
-        (auto_da_alloc); that case is measured apart (overwrite_existing)."""
+        new file, as a first run of the tool does. Writing onto an existing file is measured apart
+        (overwrite_existing)."""
         if os.path.exists(out_path):
             os.unlink(out_path)
 
@@ -461,12 +460,12 @@ def main():
         with open(out_path, "rb") as f:
             gpu_out = f.read()
         file_steps = max(1, min(args.steps, 5))
-        # the headline step onto an output.bin left by the step before (truncated by the open)
+        # the headline step onto an output.bin left by the step before
         ms_over, _ = timed(step_fastq, file_steps, False)
         overwrite = {"ms_per_step": round(ms_over, 3),
                      "gbases_per_s": round(n * L / (ms_over * 1e-3) / 1e9, 4), "steps": file_steps,
-                     "path": "as the metric, with output.bin already there (fopen \"wb\" truncates it; "
-                             "ext4 writes the rewritten file back at close)"}
+                     "path": "as the metric, with output.bin already there (written over in place and "
+                             "cut to the new size: the file fopen \"wb\" leaves)"}
         step_file()  # warm-up of the streaming buffers
         ms_file, _ = timed(step_file, file_steps, False, fresh_output)
         with open(out_path, "rb") as f:

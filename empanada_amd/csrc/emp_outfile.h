@@ -39,8 +39,7 @@ struct DirectIO {
 // writes): the blocks ahead of the writes are reserved (fallocate) in OUT_RESERVE steps, as for
 // the buffered writes, and cut back to the bytes written when the file is closed. Without
 // reservations (file systems other than ext4 and xfs), without O_DIRECT, or on a file that
-// existed before (its truncation makes ext4 flush the rewritten pages when the file is closed,
-// and direct writes to it cost more than they save), every byte is written buffered.
+// existed before (written over in place, then cut to size), every byte is written buffered.
 class OutFile {
 public:
     static constexpr uint64_t OUT_RESERVE = 64ull << 20;
@@ -84,7 +83,7 @@ private:
     int stop_helper();
 
     int fd_ = -1, dfd_ = -1;
-    bool prealloc_ = false, tried_ = false, direct_on_ = false;
+    bool prealloc_ = false, tried_ = false, direct_on_ = false, inplace_ = false;
     uint64_t reserved_ = 0, avail_ = 0, wb_ = 0, next_chunk_ = 0, direct_bytes_ = 0;
     std::deque<std::array<uint8_t, 12>> heads_;
     std::vector<uint8_t> queued_;  // per chunk: 1 when handed to the direct side

@@ -49,6 +49,13 @@ int main(int argc, char **argv) {
         std::string want;
         OutFile of;
         unlink(path);
+        if (trial % 4 == 3) {  // a file already there, longer or shorter than the new one
+            FILE *old = fopen(path, "wb");
+            const size_t len = (size_t)(rng() % (32 * tot + 4)) + 1;
+            std::vector<char> junk(len, (char)0xA5);
+            if (!old || fwrite(junk.data(), 1, len, old) != len) return 1;
+            fclose(old);
+        }
         if (of.open(path, trial % 3 == 2 ? nullptr : &dio)) {
             printf("open failed\n");
             return 1;
