@@ -269,6 +269,8 @@ hipError_t launch_dedup_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_join_big(const BigArgs &a, uint32_t n, hipStream_t s);
 hipError_t launch_edge_compact(const EdgeCopyArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+// n bases packed four to a byte (ceil(n / 16) dwords read) -> n bytes at out (16-byte aligned)
+hipError_t launch_unpack(const uint32_t *packed, uint8_t *out, uint64_t n, hipStream_t s);
 hipError_t launch_zero(const ZeroArgs &a, hipStream_t s);
 // Copies of a round's records in host memory (device views of pinned buffers), written by the
 // round's edge scan so that no copy runs between joins: rec gets {edges, first position, overflow

@@ -1306,6 +1306,37 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t v, uint6
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
 
+// Kept reads packed on the host (2-bit codes, four to a byte, base i at bits 2 (i % 4) of byte
+// i / 4) back to one byte per base, the code's base in capitals (A C T G for codes 0-3: the map
+// kernel reads (c >> 1) & 3 from them). A thread expands one packed dword into 16 bases.
+__global__ void __launch_bounds__(256) k_unpack(const uint32_t *__restrict__ packed, uint8_t *__restrict__ out, uint64_t n) {
+    const uint64_t i = blockIdx.x * 256ull + threadIdx.x;
+    if (16 * i >= n) return;
+    const uint32_t w = packed[i];
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t b = w >> (8 * k);  // four codes of this byte
+        // 'A' 0x41, 'C' 0x43, 'T' 0x54, 'G' 0x47 per code, one byte each
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v |= ((0x47544341u >> (8 * ((b >> (2 * j)) & 3u))) & 0xFFu) << (8 * j);
+        o[k] = v;
+    }
+    if (16 * i + 16 <= n) {
+        *reinterpret_cast<uint4 *>(out + 16 * i) = make_uint4(o[0], o[1], o[2], o[3]);
+    } else {
+        for (uint64_t j = 16 * i; j < n; ++j) out[j] = (uint8_t)(o[(j - 16 * i) / 4] >> (8 * ((j - 16 * i) % 4)));
+    }
+}
+
+hipError_t launch_unpack(const uint32_t *packed, uint8_t *out, uint64_t n, hipStream_t s) {
+    const uint64_t threads = (n + 15) / 16;
+    if (threads == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s, packed, out, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------------------
 // Edge compaction at the end of a run: the per-bucket edge counts of all rounds, in (round,
 // bucket) order (= output.bin order), get exclusive offsets in two levels (blocks of

@@ -181,6 +181,10 @@ struct emp_ctx {
     // on the copy stream, while the map kernel runs on the one before
     uint8_t *d_pin[2] = {nullptr, nullptr};
     uint64_t pin_cap[2] = {0, 0};
+    // the same batches packed on the host (2-bit codes, a quarter of the bytes over PCIe), expanded
+    // into d_pin on the device
+    uint8_t *d_pack[2] = {nullptr, nullptr};
+    uint64_t pack_cap[2] = {0, 0};
     hipEvent_t pin_ev[2] = {nullptr, nullptr}, pin_done[2] = {nullptr, nullptr};
     // large kept-read arrays in pageable memory: three pinned stages filled by parallel copies
     char *h_big[3] = {nullptr, nullptr, nullptr};
@@ -895,9 +899,12 @@ int add_fastq_file(emp_ctx *c, const char *path) {
 // Kept reads in host memory (m of them, a multiple of 16, the context at a multiple of 16 with
 // nothing staged), in batches of about PIN_BATCH bytes: copied into two device batches on the copy
 // stream, each mapped on the compute stream while the next is copied. Pinned reads are copied
-// straight from the caller's buffer; pageable ones first go, by COPY_THREADS parallel copies, into
-// one of three pinned stages (the host fills batch b + 1 while batch b is on the link). The
-// caller's buffer is free again once the copies are done; the map kernels may still run.
+// straight from the caller's buffer. Pageable ones go first into one of three pinned stages (the
+// host fills batch b + 1 while batch b is on the link), packed by PACK_THREADS threads to 2-bit
+// codes, a quarter of the bytes, which the device expands (k_unpack) before the map kernel; a batch
+// holding a byte other than ACGTacgt is copied as it is (COPY_THREADS parallel copies), and the map
+// kernel's check reports it. The caller's buffer is free again once the copies are done; the map
+// kernels may still run.
 constexpr uint64_t PIN_BATCH = 256ull << 20;
 
 int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
@@ -918,6 +925,15 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
             c->d_pin[k] = (uint8_t *)p;
             c->pin_cap[k] = per * L;
         }
+        if (!pinned && c->pack_cap[k] < per * L / 4 + 16) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dfree(c, c->d_pack[k]);
+            c->d_pack[k] = nullptr;
+            void *p = nullptr;
+            if ((rc = dalloc(c, &p, per * L / 4 + 16))) return rc;
+            c->d_pack[k] = (uint8_t *)p;
+            c->pack_cap[k] = per * L / 4 + 16;
+        }
         if (!c->pin_ev[k]) {
             HIPCHK(c, hipEventCreateWithFlags(&c->pin_ev[k], hipEventDisableTiming));
             HIPCHK(c, hipEventCreateWithFlags(&c->pin_done[k], hipEventDisableTiming));
@@ -937,17 +953,24 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
         const int slot = (int)(b & 1);
         const uint64_t cnt = std::min(per, m - off);
         const char *src = reads + off * L;
+        bool packed = false;
         if (!pinned) {  // the stage's last copy to the device is done: refill it
             const int hs = (int)(b % 3);
             HIPCHK(c, hipEventSynchronize(c->big_done[hs]));
-            parallel_copy(c->h_big[hs], src, cnt * L);
+            packed = parallel_pack(reinterpret_cast<uint8_t *>(c->h_big[hs]), src, cnt * L);
+            if (!packed) parallel_copy(c->h_big[hs], src, cnt * L);
             src = c->h_big[hs];
         }
         HIPCHK(c, hipStreamWaitEvent(c->cstream, c->pin_done[slot], 0));  // the batch's last map has read the slot
-        HIPCHK(c, hipMemcpyAsync(c->d_pin[slot], src, cnt * L, hipMemcpyHostToDevice, c->cstream));
+        if (packed)
+            HIPCHK(c, hipMemcpyAsync(c->d_pack[slot], src, (cnt * L + 3) / 4, hipMemcpyHostToDevice, c->cstream));
+        else
+            HIPCHK(c, hipMemcpyAsync(c->d_pin[slot], src, cnt * L, hipMemcpyHostToDevice, c->cstream));
         if (!pinned) HIPCHK(c, hipEventRecord(c->big_done[b % 3], c->cstream));
         HIPCHK(c, hipEventRecord(c->pin_ev[slot], c->cstream));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->pin_ev[slot], 0));
+        if (packed)
+            HIPCHK(c, launch_unpack(reinterpret_cast<const uint32_t *>(c->d_pack[slot]), c->d_pin[slot], cnt * L, c->stream));
         if ((rc = map_batch(c, c->d_pin[slot], cnt))) return rc;
         HIPCHK(c, hipEventRecord(c->pin_done[slot], c->stream));
     }

@@ -1,0 +1,99 @@
+// Kept reads packed for the link (emp_hostcopy.h): ASCII bases to 2-bit codes on the host cores,
+// four to a byte, so that a quarter of the bytes cross PCIe; the device expands them again
+// (k_unpack) before the map kernel, which reads the same codes from the expanded rows.
+#include "emp_hostcopy.h"
+
+#include <immintrin.h>
+
+#include <atomic>
+
+namespace emp {
+
+namespace {
+
+inline bool valid_base(uint8_t c) {
+    const uint8_t l = c | 0x20;
+    return l == 'a' || l == 'c' || l == 'g' || l == 't';
+}
+
+// Bytes [0, n) of src into dst (n / 4 bytes, n a multiple of 4).
+bool pack_scalar(uint8_t *dst, const uint8_t *src, uint64_t n) {
+    bool ok = true;
+    for (uint64_t i = 0; i < n; i += 4) {
+        uint8_t b = 0;
+        for (int k = 0; k < 4; ++k) {
+            const uint8_t c = src[i + k];
+            ok &= valid_base(c);
+            b |= (uint8_t)(((c >> 1) & 3u) << (2 * k));
+        }
+        dst[i / 4] = b;
+    }
+    return ok;
+}
+
+// 32 bases per step: codes (c >> 1) & 3 per byte, then code0 + 4 code1 + 16 code2 + 64 code3 per
+// four bytes as two multiply-adds (byte pairs, then 16-bit pairs), the eight sums narrowed to bytes.
+// Validity: (c | 0x20) is one of a, c, g, t.
+__attribute__((target("avx2"))) bool pack_avx2(uint8_t *dst, const uint8_t *src, uint64_t n) {
+    const __m256i m3 = _mm256_set1_epi8(3), lower = _mm256_set1_epi8(0x20);
+    const __m256i w_pairs = _mm256_set1_epi16(0x0401), w_quads = _mm256_set1_epi32(0x00100001);
+    const __m256i ca = _mm256_set1_epi8('a'), cc = _mm256_set1_epi8('c'), cg = _mm256_set1_epi8('g'),
+                  ct = _mm256_set1_epi8('t');
+    __m256i good = _mm256_set1_epi8(-1);
+    uint64_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        const __m256i v = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(src + i));
+        const __m256i l = _mm256_or_si256(v, lower);
+        const __m256i ok = _mm256_or_si256(_mm256_or_si256(_mm256_cmpeq_epi8(l, ca), _mm256_cmpeq_epi8(l, cc)),
+                                           _mm256_or_si256(_mm256_cmpeq_epi8(l, cg), _mm256_cmpeq_epi8(l, ct)));
+        good = _mm256_and_si256(good, ok);
+        const __m256i code = _mm256_and_si256(_mm256_srli_epi16(v, 1), m3);
+        const __m256i p16 = _mm256_maddubs_epi16(code, w_pairs);  // c0 + 4 c1 per byte pair
+        const __m256i p32 = _mm256_madd_epi16(p16, w_quads);      // (c0 + 4 c1) + 16 (c2 + 4 c3)
+        __m256i q = _mm256_packus_epi32(p32, p32);                // per 128-bit half: 4 words, twice
+        q = _mm256_packus_epi16(q, q);                            // per half: 4 bytes, four times
+        const uint32_t lo = (uint32_t)_mm256_extract_epi32(q, 0), hi = (uint32_t)_mm256_extract_epi32(q, 4);
+        memcpy(dst + i / 4, &lo, 4);
+        memcpy(dst + i / 4 + 4, &hi, 4);
+    }
+    bool ok = _mm256_movemask_epi8(good) == -1;
+    if (i < n) ok &= pack_scalar(dst + i / 4, src + i, n - i);
+    return ok;
+}
+
+}  // namespace
+
+bool parallel_pack(uint8_t *dst, const char *src, uint64_t n) {
+    const uint8_t *s = reinterpret_cast<const uint8_t *>(src);
+    const uint64_t body = n & ~3ull;
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    auto run = [&](uint8_t *d, const uint8_t *x, uint64_t m) { return avx2 ? pack_avx2(d, x, m) : pack_scalar(d, x, m); };
+    bool ok = true;
+    if (body < COPY_SPLIT_MIN) {
+        ok = run(dst, s, body);
+    } else {
+        // parts of a multiple of 32 bases, so that every part starts on a whole packed byte
+        const uint64_t per = ((body + PACK_THREADS - 1) / PACK_THREADS + 31) & ~31ull;
+        std::atomic<bool> all{true};
+        auto part = [&](int p) {
+            const uint64_t a = std::min(body, per * (uint64_t)p), e = std::min(body, a + per);
+            if (e > a && !run(dst + a / 4, s + a, e - a)) all = false;
+        };
+        std::thread th[PACK_THREADS - 1];
+        for (int p = 1; p < PACK_THREADS; ++p) th[p - 1] = std::thread(part, p);
+        part(0);
+        for (auto &t : th) t.join();
+        ok = all;
+    }
+    if (body < n) {  // the last one to three bases
+        uint8_t b = 0;
+        for (uint64_t i = body; i < n; ++i) {
+            ok &= valid_base(s[i]);
+            b |= (uint8_t)(((s[i] >> 1) & 3u) << (2 * (i - body)));
+        }
+        dst[body / 4] = b;
+    }
+    return ok;
+}
+
+}  // namespace emp

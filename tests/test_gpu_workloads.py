@@ -77,8 +77,11 @@ def test_fastq_file_and_device_entry_points(tmp_path):
 
 def test_large_pageable_reads_stream_through_pinned_stages():
     """3,600,003 x 150 bp reads in pageable memory (540 MB, past the size where emp_add_reads
-    streams them through three pinned stages filled by parallel copies, seven batches) against the
-    oracle; the last 3 reads (off a multiple of 16) are staged."""
+    streams them through three pinned stages, packed to 2-bit codes on the host and expanded on
+    the device: batches of 1,789,568 reads, the third one short) against the oracle; the last 3
+    reads (off a multiple of 16) are staged. Every 7th read in lower case gives the same output
+    (lower case codes as upper case). One byte other than ACGTacgt in the second batch: that batch
+    goes over unpacked and the map kernel's check reports EMP_ERR_BASE."""
     g = gen.genome(3 * 10**7, 81)
     r = gen.reads(g, 3_600_003, 150, 0.5, 0.005, 82)
     rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 150, min_ovlp=75)
@@ -86,6 +89,16 @@ def test_large_pageable_reads_stream_through_pinned_stages():
     with native.Context(min_ovlp=75) as c:
         c.add_reads(r)
         assert c.run() == want
+    r[::7] |= 0x20
+    with native.Context(min_ovlp=75) as c:
+        c.add_reads(r)
+        assert c.run() == want
+    r[3_000_001, 77] = ord("N")
+    with native.Context(min_ovlp=75) as c:
+        with pytest.raises(native.EmpError) as e:
+            c.add_reads(r)
+            c.run()
+    assert e.value.code == -5
 
 
 def test_cli_block_loop_vs_oracle(tmp_path):
