@@ -33,7 +33,7 @@ inline void parallel_copy(char *dst, const char *src, uint64_t bytes) {
 // (c >> 1) & 3, four to a byte (base i at bits 2 (i % 4) of byte i / 4; ceil(n / 4) bytes), split over
 // PACK_THREADS threads. Returns false when a byte is not one of ACGTacgt: the caller then sends the
 // bytes as they are and the device's check reports the error as it does for any batch.
-constexpr int PACK_THREADS = 8;
+constexpr int PACK_THREADS = 12;
 bool parallel_pack(uint8_t *dst, const char *src, uint64_t n);
 
 }  // namespace emp

@@ -969,8 +969,10 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
         if (!pinned) HIPCHK(c, hipEventRecord(c->big_done[b % 3], c->cstream));
         HIPCHK(c, hipEventRecord(c->pin_ev[slot], c->cstream));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->pin_ev[slot], 0));
-        if (packed)
+        if (packed) {
+            Timed t(c, "k_unpack", 1.25 * (double)(cnt * L));
             HIPCHK(c, launch_unpack(reinterpret_cast<const uint32_t *>(c->d_pack[slot]), c->d_pin[slot], cnt * L, c->stream));
+        }
         if ((rc = map_batch(c, c->d_pin[slot], cnt))) return rc;
         HIPCHK(c, hipEventRecord(c->pin_done[slot], c->stream));
     }
