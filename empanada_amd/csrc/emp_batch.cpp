@@ -178,6 +178,7 @@ struct emp_batch {
     // host reads staging
     char *h_stage[2] = {nullptr, nullptr};
     uint8_t *d_stage[2] = {nullptr, nullptr};
+    uint8_t *d_spack[2] = {nullptr, nullptr};  // a stage's reads as 2-bit codes, expanded into d_stage
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_busy[2] = {false, false};
     int stage_cur = 0;
@@ -740,7 +741,9 @@ int big_path(emp_batch *b, uint32_t nj, const StepLayout &sl, bool dedup) {
 
 // Map phase of job h: its reads (host, through the pinned double buffer, or device) become the
 // job's column region and 2n entries in its segment of list array 1. `ji` indexes the job's
-// error flag.
+// error flag. Host reads cross PCIe as 2-bit codes (packed by host threads into the stage,
+// expanded by k_unpack), or as they are when a stage holds a byte other than ACGTacgt, so that
+// the map kernel's check reports it.
 int map_job(emp_batch *b, HJob &h, uint32_t ji) {
     int Lpad = (h.L + 3) & ~3;
     if (((Lpad / 4) & 1) == 0) Lpad += 4;
@@ -769,8 +772,16 @@ int map_job(emp_batch *b, HJob &h, uint32_t ji) {
             BCHK(b, hipEventSynchronize(b->stage_ev[cur]));
             b->stage_busy[cur] = false;
         }
-        parallel_copy(b->h_stage[cur], h.reads + done * (uint64_t)h.L, m * (uint64_t)h.L);
-        BCHK(b, hipMemcpyAsync(b->d_stage[cur], b->h_stage[cur], m * (uint64_t)h.L, hipMemcpyHostToDevice, b->stream));
+        const char *src = h.reads + done * (uint64_t)h.L;
+        const uint64_t bytes = m * (uint64_t)h.L;
+        if (parallel_pack(reinterpret_cast<uint8_t *>(b->h_stage[cur]), src, bytes)) {
+            BCHK(b, hipMemcpyAsync(b->d_spack[cur], b->h_stage[cur], (bytes + 3) / 4, hipMemcpyHostToDevice, b->stream));
+            Timed t(b, "k_unpack", 1.25 * (double)bytes);
+            BCHK(b, launch_unpack(reinterpret_cast<const uint32_t *>(b->d_spack[cur]), b->d_stage[cur], bytes, b->stream));
+        } else {
+            parallel_copy(b->h_stage[cur], src, bytes);
+            BCHK(b, hipMemcpyAsync(b->d_stage[cur], b->h_stage[cur], bytes, hipMemcpyHostToDevice, b->stream));
+        }
         a.reads = b->d_stage[cur];
         a.n = (uint32_t)m;
         a.k0 = done;
@@ -794,6 +805,8 @@ int ensure_staging(emp_batch *b) {
         int rc = dalloc(b, &p, STAGE_BYTES);
         if (rc) return rc;
         b->d_stage[i] = (uint8_t *)p;
+        if ((rc = dalloc(b, &p, STAGE_BYTES / 4 + 16))) return rc;
+        b->d_spack[i] = (uint8_t *)p;
         BCHK(b, hipEventCreateWithFlags(&b->stage_ev[i], hipEventDisableTiming));
     }
     return EMP_OK;

@@ -137,6 +137,30 @@ def test_a_failing_job_does_not_disturb_the_others():
         assert b.output(j1) == oracle_out(*jobs[1])[0]
 
 
+def test_host_reads_cross_as_codes():
+    """Jobs from host memory cross PCIe as 2-bit codes, expanded on the device: a job of 700,001 x
+    101 bp reads (two 64 MB stages, the second ending inside a 4-byte word and inside a packed
+    byte) with every 5th read in lower case gives the oracle's records for the upper-case reads; the
+    same job with one stray byte in its second stage crosses that stage as it is and fails with
+    EMP_ERR_BASE; a small clean job beside them is undisturbed."""
+    g = gen.genome(2_000_000, 121)
+    r = gen.reads(g, 700_001, 101, 0.5, 0.005, 122)
+    want, _ = oracle_out(r, 51)
+    low = r.copy()
+    low[::5] |= 0x20
+    bad = r.copy()
+    bad[690_000, 50] = ord("N")
+    small = make_jobs(JOBS[:1])[0]
+    with native.Batch() as b:
+        j0 = b.submit(low, 51)
+        j1 = b.submit(bad, 51)
+        j2 = b.submit(*small)
+        b.run()
+        assert b.status[j0] == 0 and b.output(j0) == want
+        assert b.status[j1] == -5
+        assert b.status[j2] == 0 and b.output(j2) == oracle_out(*small)[0]
+
+
 def test_fastq_submission_and_reuse():
     g = gen.genome(30_000, 201)
     r = gen.reads(g, 4000, 100, 0.5, 0.005, 202)
