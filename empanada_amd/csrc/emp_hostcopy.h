@@ -31,7 +31,8 @@ inline void parallel_copy(char *dst, const char *src, uint64_t bytes) {
 
 // Kept reads packed for the link: the n bases of src (reads back to back, ASCII) as 2-bit codes
 // (c >> 1) & 3, four to a byte (base i at bits 2 (i % 4) of byte i / 4; ceil(n / 4) bytes), split over
-// PACK_THREADS threads. Returns false when a byte is not one of ACGTacgt: the caller then sends the
+// up to PACK_THREADS threads of a pool kept for the process (a part per MB at least); callers on
+// several threads take turns. Returns false when a byte is not one of ACGTacgt: the caller then sends the
 // bytes as they are and the device's check reports the error as it does for any batch.
 constexpr int PACK_THREADS = 12;
 bool parallel_pack(uint8_t *dst, const char *src, uint64_t n);

@@ -5,7 +5,13 @@
 
 #include <immintrin.h>
 
+#include <unistd.h>
+
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <vector>
 
 namespace emp {
 
@@ -61,6 +67,61 @@ __attribute__((target("avx2"))) bool pack_avx2(uint8_t *dst, const uint8_t *src,
     return ok;
 }
 
+// PACK_THREADS - 1 threads kept for the life of the process (spawned per call they cost about 0.3 ms
+// a call on an 8-core host, as much as packing 4 MB takes). run() gives parts 1 .. parts - 1 to them,
+// part 0 to the caller, and returns when all are done; calls from several threads take turns. The
+// pool is never destroyed: at exit its threads are waiting on the condition variable. A child
+// process after fork() has none of its threads, so there the caller packs every part itself.
+class PackPool {
+public:
+    PackPool() : pid_(getpid()) {
+        for (int i = 1; i < PACK_THREADS; ++i) std::thread([this, i] { work(i); }).detach();
+    }
+    void run(int parts, const std::function<void(int)> &fn) {
+        if (parts <= 1 || getpid() != pid_) {
+            for (int p = 0; p < parts; ++p) fn(p);
+            return;
+        }
+        std::lock_guard<std::mutex> turn(call_mu_);
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            fn_ = &fn;
+            parts_ = parts;
+            left_ = parts - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> l(mu_);
+        done_.wait(l, [this] { return left_ == 0; });
+        fn_ = nullptr;
+    }
+
+private:
+    void work(int id) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> l(mu_);
+        for (;;) {
+            cv_.wait(l, [&] { return gen_ != seen; });
+            seen = gen_;
+            if (id >= parts_) continue;  // not needed for this call
+            const std::function<void(int)> *fn = fn_;
+            l.unlock();
+            (*fn)(id);
+            l.lock();
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    const pid_t pid_;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int parts_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+};
+
+constexpr uint64_t PACK_PART_MIN = 1ull << 20;  // bases per part, at least (but for the last)
+
 }  // namespace
 
 bool parallel_pack(uint8_t *dst, const char *src, uint64_t n) {
@@ -69,20 +130,18 @@ bool parallel_pack(uint8_t *dst, const char *src, uint64_t n) {
     static const bool avx2 = __builtin_cpu_supports("avx2");
     auto run = [&](uint8_t *d, const uint8_t *x, uint64_t m) { return avx2 ? pack_avx2(d, x, m) : pack_scalar(d, x, m); };
     bool ok = true;
-    if (body < COPY_SPLIT_MIN) {
+    const int parts = (int)std::min<uint64_t>(PACK_THREADS, body / PACK_PART_MIN);
+    if (parts <= 1) {
         ok = run(dst, s, body);
     } else {
+        static PackPool *pool = new PackPool();
         // parts of a multiple of 32 bases, so that every part starts on a whole packed byte
-        const uint64_t per = ((body + PACK_THREADS - 1) / PACK_THREADS + 31) & ~31ull;
+        const uint64_t per = ((body + parts - 1) / parts + 31) & ~31ull;
         std::atomic<bool> all{true};
-        auto part = [&](int p) {
+        pool->run(parts, [&](int p) {
             const uint64_t a = std::min(body, per * (uint64_t)p), e = std::min(body, a + per);
             if (e > a && !run(dst + a / 4, s + a, e - a)) all = false;
-        };
-        std::thread th[PACK_THREADS - 1];
-        for (int p = 1; p < PACK_THREADS; ++p) th[p - 1] = std::thread(part, p);
-        part(0);
-        for (auto &t : th) t.join();
+        });
         ok = all;
     }
     if (body < n) {  // the last one to three bases

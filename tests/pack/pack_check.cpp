@@ -2,10 +2,13 @@
 // (A 0, C 1, T 2, G 3, lower case alike) at bits 2 (i % 4) of byte i / 4, nothing written past
 // byte (n + 3) / 4, and false exactly when a byte other than ACGTacgt is present, wherever it sits
 // (the first or last byte of a thread's part, the 1-3 base tail, the scalar steps after the last
-// 32-base step). Prints "all equal" when every case holds.
+// 32-base step); the same from four threads calling at once (the calls share one thread pool).
+// Prints "all equal" when every case holds.
 #include <stdio.h>
 
+#include <atomic>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "emp_hostcopy.h"
@@ -53,6 +56,31 @@ int main() {
                        same ? "equal" : "differ");
             }
         }
+    }
+    std::atomic<int> cfails{0};
+    auto caller = [&](int t) {
+        std::mt19937_64 r(1000 + t);
+        for (int k = 0; k < 12; ++k) {
+            const uint64_t n = (3ull << 20) + r() % (6ull << 20);
+            std::vector<char> s(n);
+            for (auto &c : s) c = bases[r() % 8];
+            const int64_t w = (k % 3 == 2) ? (int64_t)(r() % n) : -1;
+            if (w >= 0) s[w] = 'N';
+            std::vector<uint8_t> d((n + 3) / 4 + 8, 0xEE);
+            bool same = emp::parallel_pack(d.data(), s.data(), n) == (w < 0);
+            for (uint64_t i = 0; i < n && same && w < 0; ++i)
+                same = ((d[i / 4] >> (2 * (i % 4))) & 3u) == ((uint8_t(s[i]) >> 1) & 3u);
+            for (uint64_t i = (n + 3) / 4; i < d.size() && same; ++i) same = d[i] == 0xEE;
+            if (!same) ++cfails;
+        }
+    };
+    std::thread th[4];
+    for (int t = 0; t < 4; ++t) th[t] = std::thread(caller, t);
+    for (auto &t : th) t.join();
+    cases += 48;
+    if (cfails) {
+        fails += cfails;
+        printf("%d concurrent calls differ\n", cfails.load());
     }
     printf("%d cases, %s\n", cases, fails ? "FAILED" : "all equal");
     return fails != 0;
