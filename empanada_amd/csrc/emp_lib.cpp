@@ -898,13 +898,14 @@ int add_fastq_file(emp_ctx *c, const char *path) {
 
 // Kept reads in host memory (m of them, a multiple of 16, the context at a multiple of 16 with
 // nothing staged), in batches of about PIN_BATCH bytes: copied into two device batches on the copy
-// stream, each mapped on the compute stream while the next is copied. Pinned reads are copied
-// straight from the caller's buffer. Pageable ones go first into one of three pinned stages (the
-// host fills batch b + 1 while batch b is on the link), packed by PACK_THREADS threads to 2-bit
-// codes, a quarter of the bytes, which the device expands (k_unpack) before the map kernel; a batch
-// holding a byte other than ACGTacgt is copied as it is (COPY_THREADS parallel copies), and the map
-// kernel's check reports it. The caller's buffer is free again once the copies are done; the map
-// kernels may still run.
+// stream, each mapped on the compute stream while the next is copied. Pageable reads, and pinned
+// ones of 2 PIN_BATCH bytes or more, go first into one of three pinned stages (the host fills batch
+// b + 1 while batch b is on the link), packed by PACK_THREADS threads to 2-bit codes, a quarter of
+// the bytes, which the device expands (k_unpack) before the map kernel: packing runs at the host's
+// memory bandwidth, several times the link's rate. A batch holding a byte other than ACGTacgt
+// crosses as it is (pageable: COPY_THREADS parallel copies into the stage first), and the map
+// kernel's check reports it. Smaller pinned arrays are copied straight from the caller's buffer.
+// The caller's buffer is free again once the copies are done; the map kernels may still run.
 constexpr uint64_t PIN_BATCH = 256ull << 20;
 
 int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
@@ -914,6 +915,7 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
     int rc;
     if ((rc = ensure_cols(c, c->n + m)) || (rc = ensure_entries(c, 2 * (c->n + m)))) return rc;
     const uint64_t per = std::max<uint64_t>(16, (PIN_BATCH / L) & ~15ull);
+    const bool staged = !pinned || m * L >= 2 * PIN_BATCH;
     if (!c->cstream) HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
     for (int k = 0; k < 2; ++k) {
         if (c->pin_cap[k] < per * L) {
@@ -925,7 +927,7 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
             c->d_pin[k] = (uint8_t *)p;
             c->pin_cap[k] = per * L;
         }
-        if (!pinned && c->pack_cap[k] < per * L / 4 + 16) {
+        if (staged && c->pack_cap[k] < per * L / 4 + 16) {
             HIPCHK(c, hipStreamSynchronize(c->stream));
             dfree(c, c->d_pack[k]);
             c->d_pack[k] = nullptr;
@@ -940,7 +942,7 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
             HIPCHK(c, hipEventRecord(c->pin_done[k], c->stream));
         }
     }
-    if (!pinned) {
+    if (staged) {
         for (int k = 0; k < 3; ++k) {
             if (!c->h_big[k]) HIPCHK(c, hipHostMalloc((void **)&c->h_big[k], PIN_BATCH, hipHostMallocDefault));
             if (!c->big_done[k]) {
@@ -954,19 +956,19 @@ int add_reads_pinned(emp_ctx *c, const char *reads, uint64_t m, bool pinned) {
         const uint64_t cnt = std::min(per, m - off);
         const char *src = reads + off * L;
         bool packed = false;
-        if (!pinned) {  // the stage's last copy to the device is done: refill it
+        if (staged) {  // the stage's last copy to the device is done: refill it
             const int hs = (int)(b % 3);
             HIPCHK(c, hipEventSynchronize(c->big_done[hs]));
             packed = parallel_pack(reinterpret_cast<uint8_t *>(c->h_big[hs]), src, cnt * L);
-            if (!packed) parallel_copy(c->h_big[hs], src, cnt * L);
-            src = c->h_big[hs];
+            if (!packed && !pinned) parallel_copy(c->h_big[hs], src, cnt * L);
+            if (packed || !pinned) src = c->h_big[hs];
         }
         HIPCHK(c, hipStreamWaitEvent(c->cstream, c->pin_done[slot], 0));  // the batch's last map has read the slot
         if (packed)
             HIPCHK(c, hipMemcpyAsync(c->d_pack[slot], src, (cnt * L + 3) / 4, hipMemcpyHostToDevice, c->cstream));
         else
             HIPCHK(c, hipMemcpyAsync(c->d_pin[slot], src, cnt * L, hipMemcpyHostToDevice, c->cstream));
-        if (!pinned) HIPCHK(c, hipEventRecord(c->big_done[b % 3], c->cstream));
+        if (staged) HIPCHK(c, hipEventRecord(c->big_done[b % 3], c->cstream));
         HIPCHK(c, hipEventRecord(c->pin_ev[slot], c->cstream));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->pin_ev[slot], 0));
         if (packed) {

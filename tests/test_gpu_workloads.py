@@ -80,8 +80,11 @@ def test_large_pageable_reads_stream_through_pinned_stages():
     streams them through three pinned stages, packed to 2-bit codes on the host and expanded on
     the device: batches of 1,789,568 reads, the third one short) against the oracle; the last 3
     reads (off a multiple of 16) are staged. Every 7th read in lower case gives the same output
-    (lower case codes as upper case). One byte other than ACGTacgt in the second batch: that batch
-    goes over unpacked and the map kernel's check reports EMP_ERR_BASE."""
+    (lower case codes as upper case), also from pinned host memory (an array this large is packed
+    through the stages as well). One byte other than ACGTacgt in the second batch: that batch goes
+    over unpacked (pinned: straight from the caller's buffer) and the map kernel's check reports
+    EMP_ERR_BASE."""
+    from empanada_amd import hipmem
     g = gen.genome(3 * 10**7, 81)
     r = gen.reads(g, 3_600_003, 150, 0.5, 0.005, 82)
     rc, want, osum, _ = ol.run_reads(r.tobytes(), r.shape[0], 150, min_ovlp=75)
@@ -90,15 +93,22 @@ def test_large_pageable_reads_stream_through_pinned_stages():
         c.add_reads(r)
         assert c.run() == want
     r[::7] |= 0x20
-    with native.Context(min_ovlp=75) as c:
-        c.add_reads(r)
-        assert c.run() == want
+    h = hipmem.HostBuffer(r.nbytes)
+    hr = h.array(np.uint8, r.shape)
+    hr[:] = r
+    for src in (r, hr):
+        with native.Context(min_ovlp=75) as c:
+            c.add_reads(src)
+            assert c.run() == want
     r[3_000_001, 77] = ord("N")
-    with native.Context(min_ovlp=75) as c:
-        with pytest.raises(native.EmpError) as e:
-            c.add_reads(r)
-            c.run()
-    assert e.value.code == -5
+    hr[3_000_001, 77] = ord("N")
+    for src in (r, hr):
+        with native.Context(min_ovlp=75) as c:
+            with pytest.raises(native.EmpError) as e:
+                c.add_reads(src)
+                c.run()
+        assert e.value.code == -5
+    h.free()
 
 
 def test_cli_block_loop_vs_oracle(tmp_path):
