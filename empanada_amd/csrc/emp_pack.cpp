@@ -39,29 +39,49 @@ bool pack_scalar(uint8_t *dst, const uint8_t *src, uint64_t n) {
 
 // 32 bases per step: codes (c >> 1) & 3 per byte, then code0 + 4 code1 + 16 code2 + 64 code3 per
 // four bytes as two multiply-adds (byte pairs, then 16-bit pairs), the eight sums narrowed to bytes.
-// Validity: (c | 0x20) is one of a, c, g, t.
-__attribute__((target("avx2"))) bool pack_avx2(uint8_t *dst, const uint8_t *src, uint64_t n) {
+// Validity: (c | 0x20) is one of a, c, g, t. Returns the step's 8 packed bytes; `good` gathers the
+// validity of every byte.
+__attribute__((target("avx2"))) inline uint64_t pack32(const uint8_t *src, __m256i &good) {
     const __m256i m3 = _mm256_set1_epi8(3), lower = _mm256_set1_epi8(0x20);
     const __m256i w_pairs = _mm256_set1_epi16(0x0401), w_quads = _mm256_set1_epi32(0x00100001);
-    const __m256i ca = _mm256_set1_epi8('a'), cc = _mm256_set1_epi8('c'), cg = _mm256_set1_epi8('g'),
-                  ct = _mm256_set1_epi8('t');
+    const __m256i v = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(src));
+    const __m256i l = _mm256_or_si256(v, lower);
+    const __m256i ok = _mm256_or_si256(
+        _mm256_or_si256(_mm256_cmpeq_epi8(l, _mm256_set1_epi8('a')), _mm256_cmpeq_epi8(l, _mm256_set1_epi8('c'))),
+        _mm256_or_si256(_mm256_cmpeq_epi8(l, _mm256_set1_epi8('g')), _mm256_cmpeq_epi8(l, _mm256_set1_epi8('t'))));
+    good = _mm256_and_si256(good, ok);
+    const __m256i code = _mm256_and_si256(_mm256_srli_epi16(v, 1), m3);
+    const __m256i p16 = _mm256_maddubs_epi16(code, w_pairs);  // c0 + 4 c1 per byte pair
+    const __m256i p32 = _mm256_madd_epi16(p16, w_quads);      // (c0 + 4 c1) + 16 (c2 + 4 c3)
+    __m256i q = _mm256_packus_epi32(p32, p32);                // per 128-bit half: 4 words, twice
+    q = _mm256_packus_epi16(q, q);                            // per half: 4 bytes, four times
+    return (uint64_t)(uint32_t)_mm256_extract_epi32(q, 0) | ((uint64_t)(uint32_t)_mm256_extract_epi32(q, 4) << 32);
+}
+
+// Whole 32-base steps, then the scalar rest. Where the output is 32-byte aligned, four steps go
+// out as one non-temporal 32-byte store: the packed bytes are read by the DMA engine, not by this
+// core, and a normal store would first read each output line from memory. The fence at the end
+// orders the streamed stores before the caller's next step (the copy to the device).
+__attribute__((target("avx2"))) bool pack_avx2(uint8_t *dst, const uint8_t *src, uint64_t n) {
     __m256i good = _mm256_set1_epi8(-1);
     uint64_t i = 0;
-    for (; i + 32 <= n; i += 32) {
-        const __m256i v = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(src + i));
-        const __m256i l = _mm256_or_si256(v, lower);
-        const __m256i ok = _mm256_or_si256(_mm256_or_si256(_mm256_cmpeq_epi8(l, ca), _mm256_cmpeq_epi8(l, cc)),
-                                           _mm256_or_si256(_mm256_cmpeq_epi8(l, cg), _mm256_cmpeq_epi8(l, ct)));
-        good = _mm256_and_si256(good, ok);
-        const __m256i code = _mm256_and_si256(_mm256_srli_epi16(v, 1), m3);
-        const __m256i p16 = _mm256_maddubs_epi16(code, w_pairs);  // c0 + 4 c1 per byte pair
-        const __m256i p32 = _mm256_madd_epi16(p16, w_quads);      // (c0 + 4 c1) + 16 (c2 + 4 c3)
-        __m256i q = _mm256_packus_epi32(p32, p32);                // per 128-bit half: 4 words, twice
-        q = _mm256_packus_epi16(q, q);                            // per half: 4 bytes, four times
-        const uint32_t lo = (uint32_t)_mm256_extract_epi32(q, 0), hi = (uint32_t)_mm256_extract_epi32(q, 4);
-        memcpy(dst + i / 4, &lo, 4);
-        memcpy(dst + i / 4 + 4, &hi, 4);
+    for (; i + 32 <= n && ((uintptr_t)(dst + i / 4) & 31u); i += 32) {
+        const uint64_t w = pack32(src + i, good);
+        memcpy(dst + i / 4, &w, 8);
     }
+    bool streamed = false;
+    for (; i + 128 <= n; i += 128) {
+        const uint64_t w0 = pack32(src + i, good), w1 = pack32(src + i + 32, good);
+        const uint64_t w2 = pack32(src + i + 64, good), w3 = pack32(src + i + 96, good);
+        _mm256_stream_si256(reinterpret_cast<__m256i *>(dst + i / 4),
+                            _mm256_set_epi64x((long long)w3, (long long)w2, (long long)w1, (long long)w0));
+        streamed = true;
+    }
+    for (; i + 32 <= n; i += 32) {
+        const uint64_t w = pack32(src + i, good);
+        memcpy(dst + i / 4, &w, 8);
+    }
+    if (streamed) _mm_sfence();
     bool ok = _mm256_movemask_epi8(good) == -1;
     if (i < n) ok &= pack_scalar(dst + i / 4, src + i, n - i);
     return ok;
