@@ -437,18 +437,13 @@ int ensure_entries(emp_ctx *c, uint64_t n_entries) {
     return EMP_OK;
 }
 
+// The column store's layout depends on the read length; the host stages and their device
+// batches (stage_bytes each) are kept, and grown by stage_read when a longer length needs more.
 void free_reads_storage(emp_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     dfree(c, c->d_cols);
     c->d_cols = nullptr;
     c->cols_chunks = 0;
-    for (int i = 0; i < 2; ++i) {
-        dfree(c, c->d_batch[i]);
-        c->d_batch[i] = nullptr;
-        if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
-        c->h_stage[i] = nullptr;
-    }
-    c->stage_bytes = 0;
 }
 
 int set_len(emp_ctx *c, int L) {
@@ -531,9 +526,17 @@ int stage_read(emp_ctx *c, const char *seq, size_t len) {
     if (rc) return rc;
     rc = set_len(c, (int)std::min<size_t>(len, 1u << 20));
     if (rc) return rc;
-    if (!c->h_stage[0]) {
+    if (!c->h_stage[0] || c->stage_bytes < (size_t)STAGE_READS * c->L) {
+        // first use, or reads longer than the stages were sized for (only at a run's first read:
+        // the previous run's copies are waited for before its stages are freed)
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         c->stage_bytes = (size_t)STAGE_READS * c->L;
         for (int i = 0; i < 2; ++i) {
+            if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+            c->h_stage[i] = nullptr;
+            dfree(c, c->d_batch[i]);  // stage_submit allocates it again at stage_bytes
+            c->d_batch[i] = nullptr;
+            c->stage_busy[i] = false;
             HIPCHK(c, hipHostMalloc((void **)&c->h_stage[i], c->stage_bytes, hipHostMallocDefault));
             if (!c->stage_ev[i]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
         }
@@ -2784,7 +2787,7 @@ int emp_add_reads(emp_ctx *c, const char *reads, uint64_t n, int32_t read_len) {
     // staged: whole spans of reads into the pinned stage (stage_read sets the stage up and moves
     // reads parsed on the device ahead of them)
     while (i < n) {
-        if (c->acc_n || !c->h_stage[0]) {
+        if (c->acc_n || !c->h_stage[0] || c->stage_bytes < (size_t)STAGE_READS * c->L) {
             if ((rc = stage_read(c, reads + i * (uint64_t)read_len, (size_t)read_len))) return rc;
             ++i;
             continue;

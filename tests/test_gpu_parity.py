@@ -637,3 +637,18 @@ def test_fastq_parser_rules_and_error_order(seed, tmp_path):
             if rc_g == 0:
                 assert got == want, (block, how)
     dbuf.free()
+
+
+def test_context_reused_across_read_lengths():
+    """One context, reset between runs whose read lengths go 100, 250, 100, 150 bp (pageable arrays
+    through the pinned stages, which grow for 250 bp and are kept for the shorter lengths after
+    it): every run gives the oracle's output for its own reads."""
+    with native.Context(min_ovlp=31) as c:
+        for k, L in enumerate((100, 250, 100, 150)):
+            g = gen.genome(50_000, 600 + k)
+            r = gen.reads(g, 8000 + 16 * k + 3, L, 0.5, 0.005, 700 + k)
+            rc, want, summ, _ = ol.run_reads(r.tobytes(), r.shape[0], L, min_ovlp=31)
+            assert rc == 0 and summ["n_edges"] > 0
+            c.reset()
+            c.add_reads(r)
+            assert c.run() == want, L
