@@ -145,7 +145,8 @@ struct emp_ctx {
     int L = 0;
     uint64_t n = 0;  // reads mapped (submitted to the device)
     uint32_t *d_cols = nullptr;  // column store (grouped layout of emp_device.h)
-    size_t cols_chunks = 0;      // chunks allocated
+    size_t cols_chunks = 0;      // chunks allocated at the current read length
+    size_t cols_bytes = 0;       // bytes allocated (kept when the read length changes)
     Entry *d_entries = nullptr;
     uint64_t entries_cap = 0;
     uint32_t *d_ends = nullptr;  // per read: first and last column groups (entries_cap / 2 of them)
@@ -401,18 +402,24 @@ int sync(emp_ctx *c) {
 int ensure_cols(emp_ctx *c, uint64_t n_total) {
     const size_t need = (size_t)((n_total + CHUNK_READS - 1) >> CHUNK_SHIFT);
     if (need <= c->cols_chunks) return EMP_OK;
-    const size_t cap = std::max(need, c->cols_chunks * 2);
     const size_t chunk_bytes = (size_t)col_groups8((uint32_t)c->L) * CHUNK_WWORDS * sizeof(uint32_t);
+    if (need * chunk_bytes <= c->cols_bytes) {  // a run's first reads: the store kept from a run at another length fits
+        c->cols_chunks = c->cols_bytes / chunk_bytes;
+        return EMP_OK;
+    }
+    const size_t cap = std::max(need, c->cols_chunks * 2);
     void *p = nullptr;
     int rc = dalloc(c, &p, cap * chunk_bytes);
     if (rc) return rc;
     if (c->d_cols) {
-        HIPCHK(c, hipMemcpyAsync(p, c->d_cols, c->cols_chunks * chunk_bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (c->cols_chunks)
+            HIPCHK(c, hipMemcpyAsync(p, c->d_cols, c->cols_chunks * chunk_bytes, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         dfree(c, c->d_cols);
     }
     c->d_cols = (uint32_t *)p;
     c->cols_chunks = cap;
+    c->cols_bytes = cap * chunk_bytes;
     return EMP_OK;
 }
 
@@ -437,12 +444,11 @@ int ensure_entries(emp_ctx *c, uint64_t n_entries) {
     return EMP_OK;
 }
 
-// The column store's layout depends on the read length; the host stages and their device
-// batches (stage_bytes each) are kept, and grown by stage_read when a longer length needs more.
+// A new read length: the column store's layout depends on it, so its chunks are counted again
+// (ensure_cols) in the bytes kept; the host stages and their device batches (stage_bytes each) are
+// kept too, and grown by stage_read when a longer length needs more.
 void free_reads_storage(emp_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
-    dfree(c, c->d_cols);
-    c->d_cols = nullptr;
     c->cols_chunks = 0;
 }
 
